@@ -1,0 +1,539 @@
+"""
+Float64 NumPy restatement of jax-power's mesh power-spectrum path (the oracle).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the shipped package imports this module:
+it is used by ``tests/``, by ``__graft_entry__.smoke()`` as the checker, and by
+``bench.py``'s ``cpu_baseline`` leg as the timed CPU port.  Every function cites
+the reference file:line it restates (paths relative to the jax-power repo,
+``jaxpower/`` omitted for package files).
+
+Conventions followed (all from the reference):
+  * grid units   x~ = (x + L/2 - c) / cellsize                     mesh.py:2684
+  * resamplers   id0 = round (odd order) / floor (even order),
+                 stencil offsets arange(p) - (p-1)//2, periodic wrap  resamplers.py:73-103
+  * interlacing  shifts j/n_i on positions, phase exp(+i s sum k_circ)/n_i,
+                 compensation after the interlacing sum             mesh.py:2676-2752
+  * spectra      hermitian mode weights 1 + (kz > 0), digitize(right=False),
+                 DC mode kept in the first l=0 bin, removed for l>0  mesh.py:2812-2910,
+                                                                     mesh2.py:25-119, 253-374
+Pinning of every function is in tests/test_oracle.py; a function listed as
+"parity unpinned" there has no independent check.
+"""
+from itertools import product
+import math
+
+import numpy as np
+from scipy import special
+
+
+# ----------------------------------------------------------------------------
+# Geometry: MeshAttrs / get_mesh_attrs / next_fft_size
+# ----------------------------------------------------------------------------
+
+def fill3(value, dtype=float):
+    """Broadcast a scalar or 3-sequence to a length-3 array (mesh.py _np_array_fill)."""
+    value = np.asarray(value, dtype=dtype)
+    return np.broadcast_to(value, (3,)).copy()
+
+
+class Attrs:
+    """Host geometry of a mesh: MeshAttrs (mesh.py:1453-1571), float64 only."""
+
+    def __init__(self, meshsize, boxsize=None, boxcenter=0.):
+        self.meshsize = fill3(meshsize, dtype='i8')
+        if boxsize is None:
+            boxsize = 1. * self.meshsize
+        self.boxsize = fill3(boxsize)
+        self.boxcenter = fill3(boxcenter)
+
+    @property
+    def cellsize(self):  # mesh.py:1558-1561
+        return self.boxsize / self.meshsize
+
+    @property
+    def kfun(self):  # mesh.py:1563-1566
+        return 2. * np.pi / self.boxsize
+
+    @property
+    def knyq(self):  # mesh.py:1568-1571
+        return np.pi * self.meshsize / self.boxsize
+
+    def rcoords(self):
+        """Node positions, kind='position' (mesh.py:1573-1603, 360-403)."""
+        offset = self.boxcenter - self.boxsize / 2.
+        return tuple(np.arange(n) * d + o for n, d, o in zip(self.meshsize, self.cellsize, offset))
+
+    def kcoords(self, kind='wavenumber'):
+        """Half-spectrum Fourier coordinates (mesh.py:1605-1636, fftfreq 360-403).
+        kind='wavenumber': fftfreq(n) * kfun * n ; kind='circular': fftfreq(n) * 2pi/n * n.
+        Last axis uses rfftfreq (hermitian)."""
+        toret = []
+        for axis, n in enumerate(self.meshsize):
+            spacing = self.kfun[axis] if kind == 'wavenumber' else 2. * np.pi / n
+            f = np.fft.rfftfreq(n) if axis == 2 else np.fft.fftfreq(n)
+            toret.append(f * spacing * n)
+        return toret
+
+
+def next_fft_size(n, primes=(2, 3, 5, 7), divisors=()):
+    """mesh.py:2226-2262: smallest m >= n with prime factors in primes and divisible by divisors."""
+    n = int(n)
+    primes = tuple(p for p in primes if p > 1)
+    divisors = tuple(int(d) for d in divisors if d > 1)
+    D = 1
+    for d in divisors:
+        D = D * d // math.gcd(D, d)
+    m = ((n + D - 1) // D) * D
+    if not primes:
+        return m
+    while True:
+        x = m
+        for p in primes:
+            while x % p == 0:
+                x //= p
+        if x == 1:
+            return m
+        m += D
+
+
+def get_mesh_attrs(*positions, meshsize=None, boxsize=None, boxcenter=None, cellsize=None,
+                   boxpad=2., approximate=False, primes=None, divisors=None):
+    """mesh.py:2270-2381 (single process, 3 dimensions)."""
+    if cellsize is not None and meshsize is not None:
+        if boxsize is not None:
+            raise ValueError('cannot specify boxsize, cellsize and meshsize simultaneously')
+        meshsize = fill3(meshsize, 'i8')
+        boxsize = meshsize * fill3(cellsize)
+    if boxsize is None or boxcenter is None:
+        if not positions:
+            raise ValueError('positions must be provided if boxsize and boxcenter are not specified')
+        pos_min = np.min([np.min(p, axis=0) for p in positions if p.size], axis=0)
+        pos_max = np.max([np.max(p, axis=0) for p in positions if p.size], axis=0)
+        delta = np.abs(pos_max - pos_min)
+        if boxcenter is None:
+            boxcenter = 0.5 * (pos_min + pos_max)
+        if boxsize is None:
+            boxsize = delta.max() * boxpad
+    boxsize = fill3(boxsize)
+    if meshsize is None:
+        if cellsize is None:
+            raise ValueError('meshsize (or cellsize) must be specified')
+        cellsize = fill3(cellsize)
+        meshsize = np.ceil(boxsize / cellsize).astype('i8')
+        primes = tuple(primes or ())
+        divisors = tuple(divisors or ())
+        meshsize = np.array([next_fft_size(m, primes=primes, divisors=divisors) for m in meshsize])
+        newboxsize = meshsize * cellsize
+        if not positions and not approximate and not np.allclose(newboxsize, boxsize):
+            raise ValueError('cannot enforce cellsize')
+        boxsize = newboxsize
+    return Attrs(meshsize, boxsize, fill3(boxcenter))
+
+
+# ----------------------------------------------------------------------------
+# Resamplers (resamplers.py)
+# ----------------------------------------------------------------------------
+
+def resampler_kernel(order, s):
+    """1-D assignment kernels, resamplers.py:64-70."""
+    s = np.asarray(s, dtype='f8')
+    if order == 1:
+        return np.ones_like(s)
+    if order == 2:
+        return 1. - s
+    if order == 3:
+        return (s <= 0.5) * (0.75 - s**2) + (0.5 < s) / 2. * (1.5 - s)**2
+    if order == 4:
+        return (s <= 1.) / 6. * (4. - 6. * s**2 + 3. * s**3) + (1. < s) / 6. * (2. - s)**3
+    raise ValueError(order)
+
+
+ORDERS = {'ngp': 1, 'cic': 2, 'tsc': 3, 'pcs': 4}
+
+
+def _stencil(order):
+    ishifts = np.arange(order) - (order - 1) // 2
+    return np.array(list(product(ishifts, repeat=3)), dtype='i8')
+
+
+def paint_grid(meshsize, xg, weights, order):
+    """Scatter-add particles given in grid units, resamplers.py:73-103.
+    One np.bincount per stencil offset (same sum as the reference's scatter)."""
+    shape = tuple(int(n) for n in meshsize)
+    mesh = np.zeros(int(np.prod(shape)), dtype='f8')
+    xg = np.asarray(xg, dtype='f8')
+    w = np.broadcast_to(np.asarray(weights, dtype='f8'), xg.shape[:1])
+    id0 = (np.round if order % 2 else np.floor)(xg).astype('i8')
+    for ishift in _stencil(order):
+        idx = id0 + ishift
+        ker = resampler_kernel(order, np.abs(idx - xg)).prod(axis=-1)
+        idx = idx % np.array(shape)
+        flat = np.ravel_multi_index(tuple(idx.T), shape)
+        mesh += np.bincount(flat, weights=w * ker, minlength=mesh.size)
+    return mesh.reshape(shape)
+
+
+def paint_grid_bruteforce(meshsize, xg, weights, order):
+    """Pure-Python loop version of paint_grid (tiny inputs only), resamplers.py:73-103."""
+    shape = tuple(int(n) for n in meshsize)
+    mesh = np.zeros(shape)
+    for x, w in zip(np.asarray(xg, dtype='f8'), np.broadcast_to(weights, (len(xg),))):
+        i0 = [int(np.round(v)) if order % 2 else int(np.floor(v)) for v in x]
+        for off in _stencil(order):
+            idx = [i0[a] + off[a] for a in range(3)]
+            k = 1.
+            for a in range(3):
+                k *= float(resampler_kernel(order, abs(idx[a] - x[a])))
+            mesh[idx[0] % shape[0], idx[1] % shape[1], idx[2] % shape[2]] += w * k
+    return mesh
+
+
+def read_grid(mesh, xg, order):
+    """Gather, resamplers.py:106-131."""
+    shape = np.array(mesh.shape)
+    xg = np.asarray(xg, dtype='f8')
+    out = np.zeros(len(xg))
+    id0 = (np.round if order % 2 else np.floor)(xg).astype('i8')
+    for ishift in _stencil(order):
+        idx = id0 + ishift
+        ker = resampler_kernel(order, np.abs(idx - xg)).prod(axis=-1)
+        idx = idx % shape
+        out += mesh[tuple(idx.T)] * ker
+    return out
+
+
+def compensation(attrs, order):
+    """prod_i sinc(k_i/2pi)^(-order) on the half spectrum, k circular: resamplers.py:16-26."""
+    kx, ky, kz = attrs.kcoords('circular')
+    cx, cy, cz = (np.sinc(k / (2. * np.pi)) for k in (kx, ky, kz))
+    return (cx[:, None, None] * cy[None, :, None] * cz[None, None, :])**(-order)
+
+
+def to_grid(attrs, positions):
+    """mesh.py:2684."""
+    return (np.asarray(positions, dtype='f8') + attrs.boxsize / 2. - attrs.boxcenter) / attrs.cellsize
+
+
+def r2c(value):
+    """MeshAttrs.r2c with fft_backend='jax': rfftn (mesh.py:1679-1697)."""
+    return np.fft.rfftn(value)
+
+
+def c2r(value, meshsize):
+    """MeshAttrs.c2r: irfftn(s=meshsize) (mesh.py:1699-1719)."""
+    return np.fft.irfftn(value, s=tuple(int(n) for n in meshsize), axes=(0, 1, 2))
+
+
+def paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compensate=False, out='real'):
+    """_paint, mesh.py:2673-2754 (single device)."""
+    order = ORDERS[resampler]
+    if weights is None:
+        weights = np.ones(len(positions))
+    xg = to_grid(attrs, positions)
+    ni = max(interlacing, 1)
+    if ni <= 1:
+        mesh = paint_grid(attrs.meshsize, xg, weights, order)
+        if compensate:
+            cmesh = r2c(mesh) * compensation(attrs, order)
+            return c2r(cmesh, attrs.meshsize) if out == 'real' else cmesh
+        return mesh if out == 'real' else r2c(mesh)
+    kx, ky, kz = attrs.kcoords('circular')
+    ksum = kx[:, None, None] + ky[None, :, None] + kz[None, None, :]
+    acc = 0.
+    for shift in np.arange(ni) * 1. / ni:
+        acc = acc + r2c(paint_grid(attrs.meshsize, xg + shift, weights, order)) * np.exp(shift * 1j * ksum) / ni
+    if compensate:
+        acc = acc * compensation(attrs, order)
+    return c2r(acc, attrs.meshsize) if out == 'real' else acc
+
+
+def read(attrs, mesh, positions, resampler='cic', compensate=False):
+    """RealMeshField.read / _read, mesh.py:2030-2115 (single device)."""
+    order = ORDERS[resampler]
+    if compensate:
+        mesh = c2r(r2c(mesh) * compensation(attrs, order), attrs.meshsize)
+    return read_grid(mesh, to_grid(attrs, positions), order)
+
+
+# ----------------------------------------------------------------------------
+# Binning (mesh.py:2788-2910, mesh2.py:25-119)
+# ----------------------------------------------------------------------------
+
+def knorm(attrs):
+    """|k| on the half spectrum, summed x, y, z in that order as in mesh2.py:54."""
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    k2 = kx[:, None, None]**2 + ky[None, :, None]**2
+    k2 = k2 + kz[None, None, :]**2
+    return np.sqrt(k2)
+
+
+def hermitian_weights(attrs):
+    """1 + (kz > 0), mesh.py:2812-2828 (the Nyquist plane kz=+knyq also gets 2)."""
+    kz = attrs.kcoords('wavenumber')[2]
+    return np.broadcast_to((1 + (kz > 0)).astype('i8')[None, None, :],
+                           (attrs.meshsize[0], attrs.meshsize[1], len(kz)))
+
+
+def make_edges(attrs, edges=None):
+    """Edge array from dict {'min','max','step'} (mesh2.py:39-47); arrays pass through."""
+    if edges is None:
+        edges = {}
+    if isinstance(edges, dict):
+        step = edges.get('step', None)
+        vecmax = attrs.kfun.min() * np.min(attrs.meshsize) / 2.
+        if step is None:
+            return find_unique_edges(attrs, edges.get('min', 0.), edges.get('max', np.inf))
+        return np.arange(edges.get('min', 0.), edges.get('max', vecmax), step)
+    edges = np.asarray(edges, dtype='f8')
+    if edges.ndim == 2:
+        edges = np.append(edges[:, 0], edges[-1, 1])
+    return edges
+
+
+def find_unique_edges(attrs, xmin=0., xmax=np.inf):
+    """mesh.py:2788-2809 (single device)."""
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    x2 = (kx[:, None, None]**2 + ky[None, :, None]**2 + kz[None, None, :]**2).ravel()
+    x0 = attrs.kfun.min()
+    x2 = x2[(x2 >= xmin**2) & (x2 <= xmax**2)]
+    _, index = np.unique(np.int64(x2 / (0.5 * x0)**2 + 0.5), return_index=True)
+    x = np.sqrt(x2[index])
+    tmp = (x[:-1] + x[1:]) / 2.
+    return np.concatenate([[tmp[0] - (x[1] - x[0])], tmp, [tmp[-1] + (x[-1] - x[-2])]])
+
+
+class BinPoles:
+    """BinMesh2SpectrumPoles (mesh2.py:65-119), mode_oversampling=0."""
+
+    def __init__(self, attrs, edges=None, ells=0):
+        self.attrs = attrs
+        edges = make_edges(attrs, edges)
+        k = knorm(attrs)
+        self.wmodes = hermitian_weights(attrs)
+        self.ibin = np.digitize(k, edges, right=False)  # mesh.py:2840
+        length = len(edges) + 1
+        self.nmodes = np.bincount(self.ibin.ravel(), weights=self.wmodes.ravel(), minlength=length)[1:-1].round().astype('i8')
+        xsum = np.bincount(self.ibin.ravel(), weights=(k * self.wmodes).ravel(), minlength=length)[1:-1]
+        with np.errstate(invalid='ignore', divide='ignore'):
+            self.xavg = xsum / self.nmodes  # mesh2.py:62 (NaN for empty bins)
+        self.edges = np.column_stack([edges[:-1], edges[1:]])
+        self.ells = tuple(sorted(ells if np.ndim(ells) else (ells,)))
+
+    def __call__(self, value, remove_zero=False):
+        """mesh2.py:100-119 and mesh.py:2874-2910."""
+        value = np.array(value)
+        if remove_zero:
+            value[0, 0, 0] = 0.
+        value = value * self.wmodes
+        n = len(self.xavg) + 2
+        ib = self.ibin.ravel()
+        if np.iscomplexobj(value):
+            s = np.bincount(ib, weights=value.real.ravel(), minlength=n) + 1j * np.bincount(ib, weights=value.imag.ravel(), minlength=n)
+        else:
+            s = np.bincount(ib, weights=value.ravel(), minlength=n)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            return s[1:-1] / self.nmodes
+
+
+# ----------------------------------------------------------------------------
+# Legendre and real spherical harmonics (utils.py:190-360)
+# ----------------------------------------------------------------------------
+
+_LEGENDRE = {
+    0: lambda x: np.ones_like(x),
+    1: lambda x: x,
+    2: lambda x: 3 * x**2 / 2 - 1 / 2,
+    3: lambda x: 5 * x**3 / 2 - 3 * x / 2,
+    4: lambda x: 35 * x**4 / 8 - 15 * x**2 / 4 + 3 / 8,
+    5: lambda x: 63 * x**5 / 8 - 35 * x**3 / 4 + 15 * x / 8,
+    6: lambda x: 231 * x**6 / 16 - 315 * x**4 / 16 + 105 * x**2 / 16 - 5 / 16,
+}
+
+
+def legendre(ell, x):
+    """Closed-form Legendre polynomials, utils.py:341-360."""
+    return _LEGENDRE[ell](np.asarray(x, dtype='f8'))
+
+
+def real_ylm(ell, m, x, y, z):
+    """Real spherical harmonic of (x, y, z)/|.| with the reference normalisation
+    sqrt((2l+1)/4pi) sqrt((l-|m|)!/(l+|m|)!) sqrt(2) (utils.py:227-238), zero at the origin
+    (utils.py:252-254).  Built from scipy's associated Legendre function; the per-m sign
+    convention may differ from the reference's sympy expression, which cancels in
+    sum_m Y_lm(a) Y_lm(b) (the only combination the estimator uses, mesh2.py:319-329)."""
+    x, y, z = np.broadcast_arrays(*(np.asarray(v, dtype='f8') for v in (x, y, z)))
+    r = np.sqrt(x**2 + y**2 + z**2)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        zh = np.where(r == 0, 0., z / np.where(r == 0, 1., r))
+    phi = np.arctan2(y, x)
+    amp = np.sqrt((2 * ell + 1) / (4 * np.pi))
+    if m != 0:
+        amp *= np.sqrt(math.factorial(ell - abs(m)) / math.factorial(ell + abs(m))) * np.sqrt(2.)
+    toret = amp * special.lpmv(abs(m), ell, zh)
+    if m < 0:
+        toret = toret * np.sin(abs(m) * phi)
+    elif m > 0:
+        toret = toret * np.cos(m * phi)
+    return np.where(r == 0, 0., toret)
+
+
+# ----------------------------------------------------------------------------
+# Estimator (mesh2.py:253-374) and normalisation / shot noise (mesh2.py:505-621)
+# ----------------------------------------------------------------------------
+
+def los_vector(los):
+    """mesh2.py:195-203."""
+    if isinstance(los, str):
+        v = [0., 0., 0.]
+        v['xyz'.index(los)] = 1.
+        return np.array(v)
+    return np.asarray(los, dtype='f8')
+
+
+def spectrum_poles(attrs, bin, mesh1, mesh2=None, los='z'):
+    """compute_mesh2_spectrum, mesh2.py:253-374.  mesh1/mesh2 are real (N,N,N) or complex
+    half spectra (N,N,N//2+1).  Returns dict with k, nmodes, num_raw (per ell), norm."""
+    ells = bin.ells
+    same = mesh2 is None
+    norm = np.prod(attrs.meshsize) / np.prod(attrs.cellsize) * np.ones_like(bin.xavg)  # mesh2.py:277
+
+    def c(m):
+        return m if np.iscomplexobj(m) else r2c(m)
+
+    def r(m):
+        return c2r(m, attrs.meshsize) if np.iscomplexobj(m) else m
+
+    num = []
+    if isinstance(los, str) and los in ('firstpoint', 'local', 'endpoint'):  # mesh2.py:293-344
+        meshes = [mesh1, mesh1 if same else mesh2]
+        swap = los == 'endpoint'
+        if swap:
+            meshes = meshes[::-1]
+        rmesh1 = meshes[0]
+        A0 = c(rmesh1 if same else meshes[1])
+        if 0 in ells:
+            Aell = (A0.real**2 + A0.imag**2) if same else np.conj(c(rmesh1)) * A0
+            num.append(bin(Aell))
+        rmesh1 = r(rmesh1)
+        xs = attrs.rcoords()
+        xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+        kx, ky, kz = attrs.kcoords('wavenumber')
+        kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+        for ell in ells:
+            if ell == 0:
+                continue
+            C = 0.
+            for m in range(-ell, ell + 1):
+                C = C + r2c(rmesh1 * real_ylm(ell, m, xx, xy, xz)) * real_ylm(ell, m, kx, ky, kz)
+            num.append(4. * np.pi * bin(np.conj(C) * A0, remove_zero=True))
+        if swap:
+            num = [np.conj(n) for n in num]
+        los_attr = 'firstpoint' if los == 'local' else los
+    else:  # mesh2.py:346-374
+        vlos = los_vector(los)
+        A1 = c(mesh1)
+        Aell = (A1.real**2 + A1.imag**2) if same else np.conj(A1) * c(mesh2)
+        if 0 in ells:
+            num.append(bin(Aell))
+        kx, ky, kz = attrs.kcoords('wavenumber')
+        kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+        kn = np.sqrt(kx**2 + ky**2 + kz**2)
+        kn[0, 0, 0] = 1.
+        mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / kn
+        for ell in ells:
+            if ell == 0:
+                continue
+            num.append((2 * ell + 1) * bin(Aell * legendre(ell, mu), remove_zero=True))
+        los_attr = vlos
+    num = [np.real(n) if ell % 2 == 0 else np.imag(n) for ell, n in zip(ells, num)]
+    return dict(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num, norm=norm,
+                volume=np.prod(attrs.kfun) * bin.nmodes, ells=ells, los=los_attr)
+
+
+def box_normalization(attrs, weights1, weights2=None):
+    """compute_box2_normalization -> compute_box_normalization, mesh.py:3121-3129 / mesh2.py:505-513."""
+    if weights2 is None:
+        weights2 = weights1
+    size = float(np.prod(attrs.meshsize))
+    return (np.sum(weights1) / size) * (np.sum(weights2) / size) * size * np.prod(attrs.cellsize)**(-1)
+
+
+def regrid_attrs(attrs, cellsize=10.):
+    """get_halo_for_new_mattrs with cellsize (mesh.py:3002-3039), single device."""
+    new = get_mesh_attrs(boxsize=attrs.boxsize, boxcenter=attrs.boxcenter, cellsize=cellsize, approximate=True)
+    if np.any((new.meshsize - attrs.meshsize) % 2):
+        meshsize = new.meshsize + 1
+        new = Attrs(meshsize, new.cellsize * meshsize, new.boxcenter)
+    return new
+
+
+def fkp_normalization(attrs, data_positions, data_weights, randoms_positions, randoms_weights, cellsize=10.):
+    """compute_fkp2_normalization (mesh2.py:516-574), auto spectrum, split=None:
+    alpha * sum(D R) / cellvolume with D, R CIC-painted on a regridded mesh (mesh.py:3092-3118)."""
+    alpha = np.sum(data_weights) / np.sum(randoms_weights)
+    new = attrs if cellsize is None else regrid_attrs(attrs, cellsize)
+    d = paint(new, data_positions, data_weights, resampler='cic', interlacing=0, compensate=False)
+    r = paint(new, randoms_positions, randoms_weights, resampler='cic', interlacing=0, compensate=False)
+    return alpha * np.sum(d * r) * np.prod(new.cellsize)**(-1)
+
+
+def fkp_shotnoise(data_weights, randoms_weights=None):
+    """compute_fkp2_shotnoise (mesh2.py:577-621): sum of squared FKP weights
+    [w_d, -alpha w_r] (mesh.py:2965-2971)."""
+    sn = np.sum(np.asarray(data_weights, dtype='f8')**2)
+    if randoms_weights is not None:
+        alpha = np.sum(data_weights) / np.sum(randoms_weights)
+        sn += alpha**2 * np.sum(np.asarray(randoms_weights, dtype='f8')**2)
+    return sn
+
+
+def pole_value(num_raw, norm, num_shotnoise=0.):
+    """Mesh2SpectrumPole.value(): (num_raw - num_shotnoise) / norm (lsstypes; semantics inferred
+    from tests/test_mesh2.py:60, 954-958)."""
+    return (np.asarray(num_raw) - num_shotnoise) / norm
+
+
+# ----------------------------------------------------------------------------
+# Brute-force checkers (tiny N): direct DFT and direct local-LOS estimator
+# ----------------------------------------------------------------------------
+
+def dft3_half(value):
+    """Direct O(N^6) r2c DFT, unnormalised e^{-ikx}: rfftn by definition."""
+    nx, ny, nz = value.shape
+    out = np.zeros((nx, ny, nz // 2 + 1), dtype='c16')
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij')
+    for a in range(nx):
+        for b in range(ny):
+            for c in range(nz // 2 + 1):
+                ph = np.exp(-2j * np.pi * (a * ix / nx + b * iy / ny + c * iz / nz))
+                out[a, b, c] = np.sum(value * ph)
+    return out
+
+
+def local_los_bruteforce(attrs, rmesh, ell):
+    """Direct form of the firstpoint estimator before binning: for every half-spectrum mode,
+    conj(sum_x rho(x) e^{-ikx} (2l+1) P_l(khat.xhat)) * A0(k) with A0 = rfftn(rho)
+    (addition theorem applied to mesh2.py:319-333, 4pi * sum_m Y_lm Y_lm = (2l+1) P_l)."""
+    nx, ny, nz = rmesh.shape
+    xs = attrs.rcoords()
+    X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1).reshape(-1, 3)
+    rx = np.linalg.norm(X, axis=-1)
+    xhat = np.where(rx[:, None] == 0, 0., X / np.where(rx == 0, 1., rx)[:, None])
+    ix = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij'), axis=-1).reshape(-1, 3)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    A0 = r2c(rmesh)
+    rho = rmesh.ravel()
+    out = np.zeros(A0.shape, dtype='c16')
+    for a in range(nx):
+        for b in range(ny):
+            for c in range(nz // 2 + 1):
+                kv = np.array([kx[a], ky[b], kz[c]])
+                kn = np.linalg.norm(kv)
+                if kn == 0:
+                    continue
+                mu = xhat @ (kv / kn)
+                pl = legendre(ell, mu) * (rx != 0)
+                ph = np.exp(-2j * np.pi * (a * ix[:, 0] / nx + b * ix[:, 1] / ny + c * ix[:, 2] / nz))
+                C = (2 * ell + 1) * np.sum(rho * pl * ph)
+                out[a, b, c] = np.conj(C) * A0[a, b, c]
+    return out

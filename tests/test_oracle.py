@@ -1,0 +1,334 @@
+"""Pins the float64 oracle against the reference's own known-answer tests, closed forms,
+invariants and brute force.  Each check is chosen so that a plausible slip in the oracle
+(dropped term, wrong sign or index, transposed operand) fails at least one of them.
+
+Reference tests mirrored: tests/test_mesh.py:24-48 (get_mesh_attrs KATs), 110-121 and
+216-232 (mass conservation), 89-107 (round trip); tests/test_mesh2.py:23-61 (closed-form P0
+of a unitary-amplitude Gaussian field), 75-91 (shot noise), 459-467 (Ylm);
+tests/test_particle2.py:58-63 (Legendre).
+"""
+import itertools
+
+import numpy as np
+import pytest
+from scipy import special
+
+from oracle import power_oracle as po
+
+
+# ---------------------------------------------------------------- resamplers
+
+@pytest.mark.parametrize('order,values', [
+    (1, {0.: 1., 0.3: 1.}),
+    (2, {0.: 1., 0.25: 0.75, 1.: 0.}),
+    (3, {0.: 0.75, 0.5: 0.5, 1.: 0.125, 1.5: 0.}),
+    (4, {0.: 2. / 3., 1.: 1. / 6., 2.: 0.}),
+])
+def test_kernel_closed_forms(order, values):
+    for s, expected in values.items():
+        assert np.isclose(po.resampler_kernel(order, s), expected, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize('order', [1, 2, 3, 4])
+def test_partition_of_unity(order):
+    rng = np.random.default_rng(order)
+    x = rng.uniform(-3, 3, 1000)
+    id0 = (np.round if order % 2 else np.floor)(x)
+    offs = np.arange(order) - (order - 1) // 2
+    tot = sum(po.resampler_kernel(order, np.abs(id0 + o - x)) for o in offs)
+    assert np.allclose(tot, 1., atol=1e-14)
+
+
+@pytest.mark.parametrize('order', [1, 2, 3, 4])
+def test_paint_vs_bruteforce_and_invariants(order):
+    rng = np.random.default_rng(10 + order)
+    shape = (4, 3, 5)  # tests/test_mesh.py:112
+    xg = rng.uniform(-1, 6, (50, 3))
+    w = 1. + rng.uniform(size=50)
+    mesh = po.paint_grid(shape, xg, w, order)
+    assert np.allclose(mesh, po.paint_grid_bruteforce(shape, xg, w, order), atol=1e-13)
+    assert np.isclose(mesh.sum(), w.sum())  # tests/test_mesh.py:119
+    # shifting every particle by one cell along y rolls the mesh by one along y
+    shifted = po.paint_grid(shape, xg + np.array([0., 1., 0.]), w, order)
+    assert np.allclose(shifted, np.roll(mesh, 1, axis=1), atol=1e-13)
+
+
+def test_paint_integer_position_kats():
+    # a CIC particle exactly on node (1,2,3) lands only there; TSC spreads 3/4 and 1/8 per axis
+    mesh = po.paint_grid((6, 6, 6), np.array([[1., 2., 3.]]), np.array([2.]), 2)
+    assert mesh[1, 2, 3] == 2. and np.count_nonzero(mesh) == 1
+    mesh = po.paint_grid((6, 6, 6), np.array([[1., 2., 3.]]), np.array([1.]), 3)
+    assert np.isclose(mesh[1, 2, 3], 0.75**3) and np.isclose(mesh[0, 2, 3], 0.125 * 0.75**2)
+    # periodic wrap: x~ = -0.25 with CIC paints 0.25 into cell N-1 and 0.75 into cell 0
+    mesh = po.paint_grid((4, 4, 4), np.array([[-0.25, 0., 0.]]), np.array([1.]), 2)
+    assert np.isclose(mesh[3, 0, 0], 0.25) and np.isclose(mesh[0, 0, 0], 0.75)
+
+
+@pytest.mark.parametrize('resampler', ['ngp', 'cic', 'tsc', 'pcs'])
+@pytest.mark.parametrize('interlacing', [0, 2, 3])
+@pytest.mark.parametrize('compensate', [False, True])
+def test_paint_conserves_sum(resampler, interlacing, compensate):
+    # tests/test_mesh.py:216-232: paint().sum() == sum(weights) for every option
+    rng = np.random.default_rng(3)
+    attrs = po.Attrs(meshsize=(8, 6, 10), boxsize=(80., 60., 100.), boxcenter=(1., 2., 3.))
+    pos = rng.uniform(-40, 40, (200, 3)) + attrs.boxcenter
+    w = 1. + rng.uniform(size=200)
+    mesh = po.paint(attrs, pos, w, resampler=resampler, interlacing=interlacing, compensate=compensate)
+    assert mesh.shape == (8, 6, 10)
+    assert np.isclose(mesh.sum(), w.sum(), rtol=1e-12)
+
+
+def test_interlacing_phase_reproduces_shifted_paint():
+    # with interlacing, each shifted mesh times exp(+i s sum k) is the unshifted-centred field:
+    # for NGP-free check use a single particle on a node and s = 1/2 with CIC -> compare to direct
+    attrs = po.Attrs(meshsize=8, boxsize=8.)
+    pos = np.array([[0.5, 0.5, 0.5]]) - 4. + 0.  # grid units 4.5
+    w = np.array([1.])
+    acc = po.paint(attrs, pos + 4., w, resampler='cic', interlacing=2, out='complex')
+    # direct: mean of the two phased transforms
+    kx, ky, kz = attrs.kcoords('circular')
+    ks = kx[:, None, None] + ky[None, :, None] + kz[None, None, :]
+    xg = po.to_grid(attrs, pos + 4.)
+    direct = sum(np.fft.rfftn(po.paint_grid(attrs.meshsize, xg + s, w, 2)) * np.exp(1j * s * ks) / 2 for s in (0., 0.5))
+    assert np.allclose(acc, direct)
+    # Nyquist sign convention: last axis uses +pi (rfftfreq), first axes -pi (fftfreq)
+    assert np.isclose(kz[-1], np.pi) and np.isclose(kx[4], -np.pi)
+
+
+# ---------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize('shape', [(2, 2, 2), (4, 2, 6), (3, 5, 4), (4, 4, 4)])
+def test_dft_matches_definition(shape):
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=shape)
+    assert np.allclose(po.r2c(x), po.dft3_half(x), atol=1e-10)
+    assert np.allclose(po.c2r(po.r2c(x), shape), x, atol=1e-12)  # tests/test_mesh.py:94
+
+
+def test_compensation_closed_forms():
+    attrs = po.Attrs(meshsize=8, boxsize=100.)
+    for order in (1, 2, 3, 4):
+        comp = po.compensation(attrs, order)
+        assert comp[0, 0, 0] == 1.
+        # Nyquist along z: k = pi -> sinc(1/2) = 2/pi -> (pi/2)^order
+        assert np.isclose(comp[0, 0, 4], (np.pi / 2)**order)
+        # k = 2pi/8 along x: sin(pi/8)/(pi/8)
+        assert np.isclose(comp[1, 0, 0], (np.sin(np.pi / 8) / (np.pi / 8))**(-order))
+
+
+# ---------------------------------------------------------------- binning
+
+def _lattice_counts(attrs, edges):
+    """Full-grid (not half) integer enumeration of modes per bin."""
+    n = attrs.meshsize
+    m = [np.where(np.arange(k) < (k + 1) // 2, np.arange(k), np.arange(k) - k) for k in n]
+    kf = attrs.kfun
+    k = np.sqrt((m[0][:, None, None] * kf[0])**2 + (m[1][None, :, None] * kf[1])**2 + (m[2][None, None, :] * kf[2])**2)
+    return np.histogram(k.ravel(), bins=edges)[0]
+
+
+@pytest.mark.parametrize('meshsize,boxsize,step', [(16, 1000., 0.01), (32, 2000., 0.005), ((16, 8, 12), (100., 80., 90.), 0.02)])
+def test_nmodes_full_grid_enumeration(meshsize, boxsize, step):
+    attrs = po.Attrs(meshsize=meshsize, boxsize=boxsize)
+    bin = po.BinPoles(attrs, edges={'step': step}, ells=(0, 2, 4))
+    edges = np.append(bin.edges[:, 0], bin.edges[-1, 1])
+    assert edges[-1] < attrs.knyq.min()  # edges stop below Nyquist (mesh2.py:39-44)
+    full = _lattice_counts(attrs, edges)
+    # hermitian weights 1 + (kz > 0) reproduce the full-grid count below Nyquist
+    assert np.array_equal(bin.nmodes, full)
+    assert bin.nmodes.dtype.kind == 'i'
+    # first bin holds the DC mode (count 1 + its neighbours)
+    assert bin.ibin[0, 0, 0] == 1
+
+
+def test_nmodes_integer_thresholds_cubic():
+    # cubic box: bin membership is a function of the integer n^2 = i^2 + j^2 + l^2
+    attrs = po.Attrs(meshsize=64, boxsize=1000.)
+    bin = po.BinPoles(attrs, edges={'step': 0.01})
+    edges = np.append(bin.edges[:, 0], bin.edges[-1, 1])
+    kf = attrs.kfun[0]
+    nmax = 32
+    counts = np.zeros(len(edges) - 1, dtype='i8')
+    r = np.arange(-nmax, nmax)
+    n2 = (r[:, None, None]**2 + r[None, :, None]**2 + r[None, None, :]**2).ravel()
+    thr = (edges / kf)**2
+    ib = np.searchsorted(thr, n2, side='right')
+    valid = (ib >= 1) & (ib < len(edges))
+    np.add.at(counts, ib[valid] - 1, 1)
+    assert np.array_equal(counts, bin.nmodes)
+
+
+def test_xavg_and_bincount_kats():
+    attrs = po.Attrs(meshsize=8, boxsize=2 * np.pi * 8)  # kfun = 1/8
+    bin = po.BinPoles(attrs, edges=np.array([0., 0.1, 0.13, 0.2]))
+    # bin 0 [0,0.1): DC only; bin 1 [0.1,0.13): |k|=1/8 (6 modes); bin 2 [0.13, 0.2): sqrt2/8 (12 modes)
+    assert list(bin.nmodes) == [1, 6, 12]
+    assert np.allclose(bin.xavg, [0., 0.125, np.sqrt(2) / 8])
+    ones = np.ones((8, 8, 5))
+    assert np.allclose(bin(ones), 1.)
+    assert np.allclose(bin(ones, remove_zero=True), [0., 1., 1.])  # nmodes keeps the DC mode
+
+
+def test_find_unique_edges():
+    attrs = po.Attrs(meshsize=8, boxsize=2 * np.pi * 8)
+    edges = po.find_unique_edges(attrs, xmax=0.3)
+    x = np.sqrt(np.array([0, 1, 2, 3, 4, 5]) / 64.)
+    mid = (x[:-1] + x[1:]) / 2
+    assert np.allclose(edges[1:-1], mid)
+    assert np.isclose(edges[0], mid[0] - (x[1] - x[0]))
+
+
+# ---------------------------------------------------------------- special functions
+
+def test_legendre_vs_scipy():
+    x = np.linspace(-1, 1, 101)
+    for ell in range(7):
+        assert np.allclose(po.legendre(ell, x), special.eval_legendre(ell, x), rtol=1e-9, atol=1e-12)
+
+
+def test_ylm_normalisation_and_addition_theorem():
+    rng = np.random.default_rng(42)
+    a = rng.normal(size=(200, 3))
+    b = rng.normal(size=(200, 3))
+    ah = a / np.linalg.norm(a, axis=1)[:, None]
+    bh = b / np.linalg.norm(b, axis=1)[:, None]
+    theta = np.arccos(ah[:, 2])
+    phi = np.arctan2(ah[:, 1], ah[:, 0])
+    for ell in (0, 1, 2, 3, 4):
+        s = sum(po.real_ylm(ell, m, *a.T) * po.real_ylm(ell, m, *(3. * b).T) for m in range(-ell, ell + 1))
+        assert np.allclose(s, (2 * ell + 1) / (4 * np.pi) * special.eval_legendre(ell, np.sum(ah * bh, axis=1)), atol=1e-12)
+        for m in range(-ell, ell + 1):
+            # |real Y| matches the complex spherical harmonic of scipy (tests/test_mesh2.py:459-467)
+            Y = special.sph_harm_y(ell, abs(m), theta, phi)
+            ref = Y.real if m >= 0 else Y.imag
+            if m != 0:
+                ref = np.sqrt(2.) * ref
+            assert np.allclose(np.abs(po.real_ylm(ell, m, *a.T)), np.abs(ref), atol=1e-12)
+            assert po.real_ylm(ell, m, 0., 0., 0.) == 0.  # utils.py:252-254
+
+
+# ---------------------------------------------------------------- estimator
+
+def _unitary_gaussian(attrs, pk, seed=0):
+    """Oracle version of generate_gaussian_mesh(unitary_amplitude=True) (mock.py:15-47)."""
+    rng = np.random.default_rng(seed)
+    c = po.r2c(rng.normal(size=tuple(attrs.meshsize)))
+    k = po.knorm(attrs)
+    ker = np.sqrt(pk(k) / np.prod(attrs.cellsize)) * np.sqrt(float(np.prod(attrs.meshsize))) / np.abs(c)
+    return po.c2r(c * ker, attrs.meshsize)
+
+
+def test_p0_closed_form_gaussian_field():
+    # tests/test_mesh2.py:23-61: a unitary-amplitude field gives P0(k) = mode-average of P(k)
+    attrs = po.Attrs(meshsize=32, boxsize=1000.)
+    pk = lambda k: np.where(k > 0, 1e4 * (k / 0.03)**3 * np.exp(-k / 0.03), 0.)
+    mesh = _unitary_gaussian(attrs, pk)
+    bin = po.BinPoles(attrs, edges={'step': 0.01}, ells=(0, 2, 4))
+    res = po.spectrum_poles(attrs, bin, mesh, los='x')
+    p0 = po.pole_value(res['num_raw'][0], res['norm'])
+    # independent lattice sum over the full grid
+    n = 32
+    m = np.where(np.arange(n) < n // 2, np.arange(n), np.arange(n) - n)
+    kf = attrs.kfun[0]
+    kk = kf * np.sqrt(m[:, None, None]**2 + m[None, :, None]**2 + m[None, None, :]**2)
+    edges = np.append(bin.edges[:, 0], bin.edges[-1, 1])
+    num = np.histogram(kk.ravel(), bins=edges, weights=pk(kk).ravel())[0]
+    cnt = np.histogram(kk.ravel(), bins=edges)[0]
+    assert np.allclose(p0, num / cnt, rtol=1e-9)
+    # isotropic power: higher multipoles vanish only on average; P2 is the lattice average of 5 P L2
+    mu = np.where(kk > 0, m[:, None, None] * kf / np.where(kk > 0, kk, 1.), 0.)
+    num2 = np.histogram(kk.ravel(), bins=edges, weights=(5 * pk(kk) * special.eval_legendre(2, mu)).ravel())[0]
+    p2 = po.pole_value(res['num_raw'][1], res['norm'])
+    assert np.allclose(p2, num2 / cnt, rtol=1e-8, atol=1e-8 * np.abs(num / cnt).max())
+
+
+def test_los_axis_permutation():
+    # the multipoles along x of a field equal those along z of the axis-swapped field
+    rng = np.random.default_rng(5)
+    attrs = po.Attrs(meshsize=16, boxsize=500.)
+    mesh = rng.normal(size=(16, 16, 16))
+    bin = po.BinPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    rx = po.spectrum_poles(attrs, bin, mesh, los='x')
+    rz = po.spectrum_poles(attrs, bin, np.ascontiguousarray(np.swapaxes(mesh, 0, 2)), los='z')
+    for a, b in zip(rx['num_raw'], rz['num_raw']):
+        assert np.allclose(a, b, rtol=1e-10, atol=1e-12 * np.abs(rx['num_raw'][0]).max())
+
+
+def test_cross_equals_auto_for_same_mesh():
+    rng = np.random.default_rng(8)
+    attrs = po.Attrs(meshsize=8, boxsize=100.)
+    mesh = rng.normal(size=(8, 8, 8))
+    bin = po.BinPoles(attrs, edges={'step': 0.1}, ells=(0, 1, 2))
+    auto = po.spectrum_poles(attrs, bin, mesh, los='z')
+    cross = po.spectrum_poles(attrs, bin, mesh, mesh.copy(), los='z')
+    for a, b in zip(auto['num_raw'], cross['num_raw']):
+        assert np.allclose(a, b, equal_nan=True, atol=1e-9)
+
+
+@pytest.mark.parametrize('ell', [2, 4])
+def test_local_los_vs_bruteforce(ell):
+    # firstpoint estimator (Ylm sums + FFTs) equals the direct double sum with P_l(khat.xhat)
+    rng = np.random.default_rng(ell)
+    attrs = po.Attrs(meshsize=6, boxsize=60., boxcenter=(100., -20., 45.))
+    mesh = rng.normal(size=(6, 6, 6))
+    bin = po.BinPoles(attrs, edges={'step': 0.05}, ells=(0, ell))
+    res = po.spectrum_poles(attrs, bin, mesh, los='firstpoint')
+    direct = bin(po.local_los_bruteforce(attrs, mesh, ell), remove_zero=True)
+    assert np.allclose(res['num_raw'][1], direct.real, rtol=1e-9, atol=1e-9 * np.nanmax(np.abs(direct)), equal_nan=True)
+    # endpoint swaps the meshes and conjugates (mesh2.py:296, 337): for an auto spectrum
+    # of even ell it equals firstpoint
+    res_end = po.spectrum_poles(attrs, bin, mesh, los='endpoint')
+    assert np.allclose(res_end['num_raw'][1], res['num_raw'][1], equal_nan=True)
+
+
+# ---------------------------------------------------------------- mesh attrs, norm, shot noise
+
+def test_get_mesh_attrs_kats():
+    # tests/test_mesh.py:24-48
+    attrs = po.get_mesh_attrs(boxsize=1000., boxcenter=0., cellsize=10.)
+    assert np.all(attrs.meshsize == 100) and np.allclose(attrs.cellsize, 10.)
+    attrs = po.get_mesh_attrs(boxsize=1000., boxcenter=0., meshsize=128)
+    assert np.all(attrs.meshsize == 128) and np.allclose(attrs.cellsize, 1000. / 128)
+    rng = np.random.default_rng(84)
+    pos = rng.uniform(0., 100., (100, 3))
+    attrs = po.get_mesh_attrs(pos, meshsize=128)
+    assert np.all(attrs.meshsize == 128)
+    attrs = po.get_mesh_attrs(pos, cellsize=12.)
+    assert np.allclose(attrs.cellsize, 12.)
+    attrs = po.get_mesh_attrs(pos, cellsize=12., primes=(2, 3, 5), divisors=(2, 3))
+    assert np.all(attrs.meshsize == 18) and np.allclose(attrs.cellsize, 12.)
+    assert po.next_fft_size(17, primes=(2, 3, 5, 7)) == 18
+    assert po.next_fft_size(97) == 98 and po.next_fft_size(121, primes=(2,)) == 128
+
+
+def test_regrid_parity_rule():
+    # mesh.py:3024-3033: new meshsize ceil(L/10); if any axis changes parity add 1 to every axis
+    attrs = po.Attrs(meshsize=64, boxsize=1000., boxcenter=500.)
+    new = po.regrid_attrs(attrs, 10.)
+    assert np.all(new.meshsize == 100) and np.allclose(new.boxsize, 1000.)
+    attrs = po.Attrs(meshsize=64, boxsize=995., boxcenter=500.)  # ceil(99.5) = 100: parity kept
+    new = po.regrid_attrs(attrs, 10.)
+    assert np.all(new.meshsize == 100) and np.allclose(new.boxsize, 1000.)
+    attrs = po.Attrs(meshsize=64, boxsize=1005., boxcenter=500.)  # ceil(100.5) = 101: odd change
+    new = po.regrid_attrs(attrs, 10.)
+    assert np.all(new.meshsize == 102) and np.allclose(new.boxsize, 1020.) and np.allclose(new.cellsize, 10.)
+
+
+def test_normalization_closed_forms():
+    rng = np.random.default_rng(1)
+    attrs = po.Attrs(meshsize=10, boxsize=100., boxcenter=0.)
+    # particles on nodes with CIC: each lands in exactly one cell, so sum(D R) is a lattice sum
+    nodes = rng.integers(0, 10, (300, 3))
+    pos_d = nodes[:150] * 10. - 50.
+    pos_r = nodes[150:] * 10. - 50.
+    wd, wr = rng.uniform(1, 2, 150), rng.uniform(1, 2, 150)
+    D = np.zeros((10, 10, 10)); np.add.at(D, tuple(nodes[:150].T), wd)
+    R = np.zeros((10, 10, 10)); np.add.at(R, tuple(nodes[150:].T), wr)
+    norm = po.fkp_normalization(attrs, pos_d, wd, pos_r, wr, cellsize=None)
+    assert np.isclose(norm, wd.sum() / wr.sum() * np.sum(D * R) / 1000.)
+    # box normalisation (sum w)^2 / (N^3 Vcell)
+    assert np.isclose(po.box_normalization(attrs, wd), wd.sum()**2 / (1000 * 1000.))
+    # shot noise: sum of squared FKP weights [w_d, -alpha w_r] (tests/test_mesh2.py:75-91)
+    alpha = wd.sum() / wr.sum()
+    assert np.isclose(po.fkp_shotnoise(wd, wr), np.sum(np.concatenate([wd, -alpha * wr])**2))
+    assert np.isclose(po.fkp_shotnoise(wd), np.sum(wd**2))
