@@ -1,0 +1,115 @@
+/*
+ * C ABI of libjaxpower_amd.so: the MI355X (gfx950) mesh power-spectrum path.
+ *
+ * Conventions
+ *   - every pointer to array data is a device pointer unless stated otherwise; geometry
+ *     arrays (meshsize, boxsize, boxcenter, los) and product descriptors are host pointers;
+ *   - `stream` is a hipStream_t (NULL = default stream); calls are asynchronous on it;
+ *   - real meshes are C-order float32 (N0, N1, N2); half spectra are complex64
+ *     (N0, N1, pitch) with N2/2+1 valid entries per row and pitch >= N2/2+1;
+ *   - every function returns 0 on success or a negative code (-1 invalid argument,
+ *     -2 unsupported, -3 HIP error, -4 workspace too small); mp_last_error() explains it;
+ *   - workspace arguments are caller-owned device buffers of at least the size returned by
+ *     the matching *_workspace_bytes query; the library itself never allocates device memory
+ *     except a 32 KiB twiddle table per device, set up on the first FFT call.
+ *
+ * Each entry point names the jax-power function it replaces (jaxpower/<file>:<line>).
+ */
+#ifndef JAXPOWER_AMD_H
+#define JAXPOWER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors, version, timing ------------------------------------------------------ */
+const char* mp_last_error(void);
+int mp_version(void);
+/* Per-kernel HIP-event timing of every launch made by the library (for roofline figures). */
+int mp_timing_enable(int enable);
+int mp_timing_collect(void);
+int mp_timing_reset(void);
+/* Writes "kernel total_ms launches\n" lines into buf; returns the number of kernels. */
+int mp_timing_report(char* buf, int64_t buflen);
+
+/* ---- painting --------------------------------------------------------------------- */
+/* resamplers.paint driven by _paint with out='real' and no compensation
+ * (jaxpower/resamplers.py:73-103, jaxpower/mesh.py:2673-2736): paints np particles into
+ * max(interlacing,1) meshes, mesh j shifted by +j/interlacing grid units, into
+ * meshes[j * N0*N1*N2 ...].  positions: (np, 3) float32 (positions_f64 = 0) or float64;
+ * weights: (np,) float32 or NULL for unit weights; order 1..4 = ngp, cic, tsc, pcs. */
+int64_t mp_paint_workspace_bytes(const int32_t meshsize[3], int64_t np, int order, int interlacing);
+int mp_paint(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+             const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
+             int interlacing, float* meshes, void* workspace, size_t workspace_bytes);
+/* Same result with per-particle float atomics (any mesh shape; no workspace). */
+int mp_paint_atomic(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+                    const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
+                    int interlacing, float* meshes);
+
+/* _paint(..., out='complex') (jaxpower/mesh.py:2673-2754): interlaced paint, r2c of every
+ * shifted mesh, phase exp(+i s_j (kx+ky+kz)) / n_i and sum, then (compensate != 0) the
+ * window compensation prod sinc(k/2pi)^-order (jaxpower/resamplers.py:16-26). */
+int64_t mp_paint_spectrum_workspace_bytes(const int32_t meshsize[3], int64_t np, int order, int interlacing);
+int mp_paint_spectrum(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+                      const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
+                      int interlacing, int compensate, void* out, int64_t out_pitch, void* workspace,
+                      size_t workspace_bytes);
+
+/* resamplers.read via RealMeshField.read (jaxpower/resamplers.py:106-131,
+ * jaxpower/mesh.py:2065-2115): out[i] = sum_stencil mesh * K. */
+int mp_read(void* stream, const float* mesh, const int32_t meshsize[3], const double boxsize[3],
+            const double boxcenter[3], const void* positions, int positions_f64, int64_t np, int order, float* out);
+
+/* ---- FFT -------------------------------------------------------------------------- */
+/* Row pitch (complex elements) the library uses for its own half spectra. */
+int64_t mp_spectrum_pitch(int32_t nz);
+/* MeshAttrs.r2c, fft_backend='jax' (jaxpower/mesh.py:1679-1697): rfftn, unnormalised. */
+int mp_rfft3d(void* stream, const float* in, void* out, const int32_t meshsize[3], int64_t out_pitch);
+/* MeshAttrs.c2r (jaxpower/mesh.py:1699-1719): irfftn(s=meshsize), 1/N^3 normalisation. */
+int64_t mp_irfft3d_workspace_bytes(const int32_t meshsize[3]);
+int mp_irfft3d(void* stream, const void* in, int64_t in_pitch, float* out, const int32_t meshsize[3],
+               void* workspace, size_t workspace_bytes);
+
+/* ---- binning ---------------------------------------------------------------------- */
+/* _get_bin_attrs with hermitian weights (jaxpower/mesh.py:2812-2863): nmodes[b] (exact
+ * integers) and ksum[b] = sum w |k| for bins b = 0..nedges-2.  kx, ky, kz: device float64
+ * wavenumbers per index; k2_thresholds[e] = min{t : fl(sqrt(t)) >= edge[e]}. */
+int mp_bin_modes(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                 const double* k2_thresholds, int nedges, int64_t* nmodes, double* ksum);
+/* BinMesh2SpectrumPoles.__call__ over the products of compute_mesh2_spectrum
+ * (jaxpower/mesh2.py:100-119, 304-333, 351-369): for product p,
+ *   out[(p*nbins + b)*2 + {0,1}] = {Re, Im} sum_{modes in b} w conj(M_i) M_j P_ell(mu)
+ * with mu = k.los/|k|, P_0 = 1, the (0,0,0) mode skipped when prod_remove_zero[p];
+ * prod_i[p] = -1 stands for the constant 1 (plain binning of M_j). */
+int64_t mp_bin_workspace_bytes(int nprod, int nedges);
+int mp_bin_poles(void* stream, int nmesh, const void* const* meshes, const int64_t* pitches,
+                 const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                 const double* k2_thresholds, int nedges, const double los[3], int nprod, const int32_t* prod_i,
+                 const int32_t* prod_j, const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out,
+                 void* workspace, size_t workspace_bytes);
+
+/* ---- local line of sight ---------------------------------------------------------- */
+/* One term of the firstpoint estimator (jaxpower/mesh2.py:319-329):
+ * out = sum_m r2c(rmesh * Y_lm(xhat)) * Y_lm(khat), ell in 1..4, x measured from the origin. */
+int64_t mp_ylm_workspace_bytes(const int32_t meshsize[3]);
+int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t meshsize[3], const double boxsize[3],
+                      const double boxcenter[3], int ell, void* out, int64_t out_pitch, void* workspace,
+                      size_t workspace_bytes);
+
+/* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
+ *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */
+int64_t mp_reduce_workspace_bytes(void);
+/* out[0] = sum a*b (b != NULL), sum a^2 (square != 0) or sum a, accumulated in float64. */
+int mp_sum_f64(void* stream, const float* a, const float* b, int64_t n, int square, double* out, void* workspace,
+               size_t workspace_bytes);
+int mp_fill_f32(void* stream, float* p, int64_t n, float value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAXPOWER_AMD_H */

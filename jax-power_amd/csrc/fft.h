@@ -1,0 +1,64 @@
+// Internal interfaces of the FFT passes and their fused prologues/epilogues.
+#pragma once
+
+#include "common.h"
+#include "sph.h"
+
+namespace mp {
+
+constexpr int TW_N = 4096;  // twiddle table length (largest supported transform length is 2048)
+
+enum { PRO_NONE = 0, PRO_YLM = 1 };
+
+// Real-space prologue of the z pass: multiplies each real input by Y_lm(xhat) of the node
+// position (local line of sight, jaxpower/mesh2.py:319-323), or by 1.
+struct RealPrologue {
+  int kind;  // PRO_NONE or PRO_YLM
+  int ell, m;
+  int ny, nz;
+  float x0, y0, z0;  // position of node (0,0,0): boxcenter - boxsize/2
+  float dx, dy, dz;  // cellsize
+  __device__ __forceinline__ float eval(int64_t row, int z) const {
+    int ix = (int)(row / ny), iy = (int)(row % ny);
+    float px = x0 + dx * ix, py = y0 + dy * iy, pz = z0 + dz * z;
+    return ylm_of_vector(ell, m, px, py, pz);
+  }
+};
+
+// Fourier-space factor of the x pass: Y_lm(khat) with k from the half-spectrum indices.
+struct KPrologue {
+  int ell, m;
+  int nx, ny;
+  float kfx, kfy, kfz;  // fundamental wavenumbers
+  __device__ __forceinline__ float eval(int ix, int iy, int iz) const {
+    int mx = ix < (nx + 1) / 2 ? ix : ix - nx;
+    int my = iy < (ny + 1) / 2 ? iy : iy - ny;
+    return ylm_of_vector(ell, m, kfx * mx, kfy * my, kfz * iz);
+  }
+};
+
+// Tables for the interlacing combine: per-axis phases exp(i s_j k_circ) for each shift j
+// (row-major [j][index]) and per-axis window compensation; null pointers disable each part.
+struct CombineTables {
+  int stride;  // distance between the phase rows of consecutive shifts
+  const float2* phx;
+  const float2* phy;
+  const float2* phz;
+  const float* cx;
+  const float* cy;
+  const float* cz;
+};
+
+int ensure_twiddles();
+int64_t spectrum_pitch(int nz);
+int rfft3d(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro);
+// z and y passes only (the x pass is fused into a consumer).
+int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro);
+int combine_x(hipStream_t s, const float2* partial, int64_t mesh_stride, int ni, float2* out, const int32_t n[3],
+              int64_t in_pitch, int64_t out_pitch, const CombineTables& tab, float scale);
+int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n[3], int64_t in_pitch, int64_t acc_pitch,
+              const KPrologue& kp, int first);
+int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const int32_t n[3], float2* scratch,
+            int64_t scratch_pitch);
+
+}  // namespace mp

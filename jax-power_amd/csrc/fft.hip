@@ -1,0 +1,614 @@
+// Real-to-complex and complex-to-real 3-D FFTs for power-of-two meshes on gfx950.
+//
+// Replaces MeshAttrs.r2c / c2r with fft_backend='jax' (jaxpower/mesh.py:1679-1719):
+// forward unnormalised e^{-ikx} onto the half spectrum (Nx, Ny, Nz/2+1), inverse with 1/N^3.
+//
+// Layout in HBM: real meshes are C-order (x slowest, z fastest); half spectra are
+// (Nx, Ny, pitch) complex64 rows with Nz/2+1 valid entries and pitch >= Nz/2+1 (our own
+// buffers pad the pitch to a multiple of 16 so every row starts on a 128-byte boundary).
+//
+// Passes (each one HBM round trip):
+//   z: rows of Nz reals viewed as Nz/2 complex -> Nz/2-point FFT in LDS -> even/odd split
+//      -> Nz/2+1 outputs per row.
+//   y, x: tiles of (L along the FFT axis) x (C contiguous columns) are staged in LDS with the
+//      contiguous axis fastest (coalesced 128-byte row segments, conflict-free LDS access),
+//      transformed by a radix-8/4/2 Stockham FFT, and written back.
+//   The interlaced paint fuses its phase shifts, 1/n_i, and the window compensation into the
+//   x pass, which reads all n_i partial spectra and writes one combined spectrum.
+#include "common.h"
+#include "fft.h"
+
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+namespace mp {
+
+// exp(-2 pi i e / TW_N), e in [0, TW_N): one table serves every power-of-two length <= TW_N.
+__device__ float2 g_twiddle[TW_N];
+
+static std::mutex g_tw_mutex;
+static std::vector<int> g_tw_ready;  // per device
+
+int ensure_twiddles() {
+  int dev = 0;
+  MP_CHECK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tw_mutex);
+  if ((int)g_tw_ready.size() <= dev) g_tw_ready.resize(dev + 1, 0);
+  if (g_tw_ready[dev]) return MP_OK;
+  std::vector<float2> host(TW_N);
+  for (int e = 0; e < TW_N; ++e) {
+    // exact reduction of the angle to the first octant keeps the table symmetric
+    double a = -2.0 * M_PI * (double)e / (double)TW_N;
+    host[e] = make_float2((float)cos(a), (float)sin(a));
+  }
+  for (int e = 0; e < TW_N; e += TW_N / 4) {  // exact values on the axes
+    int q = e / (TW_N / 4);
+    const float re[4] = {1.f, 0.f, -1.f, 0.f}, im[4] = {0.f, -1.f, 0.f, 1.f};
+    host[e] = make_float2(re[q], im[q]);
+  }
+  MP_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_twiddle), host.data(), sizeof(float2) * TW_N));
+  g_tw_ready[dev] = 1;
+  return MP_OK;
+}
+
+// ------------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot90(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+  float2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
+  float2 t2 = cadd(v1, v3), t3 = rot90<INV>(csub(v1, v3));
+  v0 = cadd(t0, t2);
+  v2 = csub(t0, t2);
+  v1 = cadd(t1, t3);
+  v3 = csub(t1, t3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v) {
+  if constexpr (R == 2) {
+    dft2<INV>(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+  } else {  // R == 8: two radix-4 on even/odd, then W8 twiddles
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    const float h = 0.70710678118654752f;
+    // W8^1 = h(1 -+ i), W8^2 = -+i, W8^3 = h(-1 -+ i)  (upper sign forward)
+    float2 w1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y)) : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+    float2 w2 = rot90<INV>(o2);
+    float2 w3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y)) : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1);
+    v[5] = csub(e1, w1);
+    v[2] = cadd(e2, w2);
+    v[6] = csub(e2, w2);
+    v[3] = cadd(e3, w3);
+    v[7] = csub(e3, w3);
+  }
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int first_radix(int L) { return ilog2(L) % 3 == 0 ? 8 : (ilog2(L) % 3 == 1 ? 2 : 4); }
+
+// One Stockham stage over C independent length-L transforms held in LDS; element (l, c)
+// lives at buf[l * LS + c * CS].  CFAST: consecutive threads take consecutive c.
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int R, int NS>
+__device__ __forceinline__ void fft_stage(float2* buf) {
+  constexpr int NB = C * (L / R);
+  constexpr int BPT = (NB + NT - 1) / NT;
+  constexpr int TWSTEP = TW_N / (NS * R);
+  float2 v[BPT][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      int c = CFAST ? b % C : b / (L / R);
+      int jj = CFAST ? b / C : b % (L / R);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[(jj + r * (L / R)) * LS + c * CS];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      int c = CFAST ? b % C : b / (L / R);
+      int jj = CFAST ? b / C : b % (L / R);
+      int k = jj % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = g_twiddle[r * k * TWSTEP];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft<R, INV>(v[q]);
+      int out = (jj / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(out + r * NS) * LS + c * CS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int NS>
+__device__ __forceinline__ void fft_stages(float2* buf) {
+  if constexpr (NS < L) {
+    constexpr int R = NS == 1 ? first_radix(L) : 8;
+    fft_stage<L, C, LS, CS, CFAST, NT, INV, R, NS>(buf);
+    fft_stages<L, C, LS, CS, CFAST, NT, INV, NS * R>(buf);
+  }
+}
+
+// Full in-place FFT of C columns in LDS (callers sync before; this syncs after).
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf) {
+  fft_stages<L, C, LS, CS, CFAST, NT, INV, 1>(buf);
+}
+
+// ------------------------------------------------------------------ launch geometry
+
+// Column passes: C columns of length L per block (C*L*8 bytes of LDS = 64 KiB at L >= 512).
+template <int L>
+struct ColCfg {
+  static constexpr int C = L <= 512 ? 16 : 8192 / L;
+  static constexpr int NT0 = C * L / 8;
+  static constexpr int NT = NT0 < 64 ? 64 : NT0;
+  static constexpr size_t LDS = (size_t)C * L * sizeof(float2);
+};
+
+// z passes: C rows of M = Nz/2 complex per block, rows padded by one element in LDS.
+template <int M>
+struct RowCfg {
+  static constexpr int C = M <= 256 ? 16 : 4096 / M;
+  static constexpr int LP = M + 1;
+  static constexpr int NT0 = C * M / 8;
+  static constexpr int NT = NT0 < 64 ? 64 : NT0;
+  static constexpr size_t LDS = (size_t)C * LP * sizeof(float2);
+};
+
+// ------------------------------------------------------------------ z pass kernels
+
+// Real rows (nrows x 2M floats) -> half-complex rows (nrows x pitch, M+1 valid).
+// Optional real-space prologue: multiply each input by a real function of the node (used for
+// the local line of sight), selected by `pro`.
+template <int M>
+__global__ void __launch_bounds__(RowCfg<M>::NT)
+zpass_r2c_kernel(const float* __restrict__ in, float2* __restrict__ out, int64_t nrows, int64_t out_pitch,
+                 RealPrologue pro) {
+  using Cfg = RowCfg<M>;
+  constexpr int C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int64_t row0 = (int64_t)blockIdx.x * C;
+  const float2* in2 = reinterpret_cast<const float2*>(in);
+  for (int idx = tid; idx < C * M; idx += NT) {
+    int c = idx / M, l = idx % M;
+    int64_t row = row0 + c;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < nrows) {
+      v = in2[row * M + l];
+      if (pro.kind != PRO_NONE) {
+        v.x *= pro.eval(row, 2 * l);
+        v.y *= pro.eval(row, 2 * l + 1);
+      }
+    }
+    lds[c * LP + l] = v;
+  }
+  __syncthreads();
+  fft_lds<M, C, 1, LP, false, NT, false>(lds);
+  constexpr int NK = M / 2 + 1;
+  for (int idx = tid; idx < C * NK; idx += NT) {
+    int c = idx / NK, k = idx % NK;
+    int64_t row = row0 + c;
+    if (row >= nrows) continue;
+    float2* o = out + row * out_pitch;
+    float2 zk = lds[c * LP + k];
+    if (k == 0) {
+      o[0] = make_float2(zk.x + zk.y, 0.f);
+      o[M] = make_float2(zk.x - zk.y, 0.f);
+      continue;
+    }
+    float2 zmk = cconj(lds[c * LP + (M - k)]);
+    float2 E = cscale(cadd(zk, zmk), 0.5f);
+    float2 D = cscale(csub(zk, zmk), 0.5f);
+    float2 O = make_float2(D.y, -D.x);                  // -i * D
+    float2 t = cmul(g_twiddle[k * (TW_N / (2 * M))], O);  // exp(-2 pi i k / Nz) * O
+    o[k] = cadd(E, t);
+    if (2 * k != M) o[M - k] = cconj(csub(E, t));
+  }
+}
+
+// Half-complex rows -> real rows, times `scale`.  Imaginary parts of the k=0 and k=M entries
+// are ignored, as numpy's irfftn does on its last axis.
+template <int M>
+__global__ void __launch_bounds__(RowCfg<M>::NT)
+zpass_c2r_kernel(const float2* __restrict__ in, int64_t in_pitch, float* __restrict__ out, int64_t nrows, float scale) {
+  using Cfg = RowCfg<M>;
+  constexpr int C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int64_t row0 = (int64_t)blockIdx.x * C;
+  constexpr int NK = M / 2 + 1;
+  for (int idx = tid; idx < C * NK; idx += NT) {
+    int c = idx / NK, k = idx % NK;
+    int64_t row = row0 + c;
+    float2 zk = make_float2(0.f, 0.f), zmk = make_float2(0.f, 0.f);
+    if (row < nrows) {
+      const float2* x = in + row * in_pitch;
+      if (k == 0) {
+        float x0 = x[0].x, xm = x[M].x;
+        zk = make_float2(0.5f * (x0 + xm), 0.5f * (x0 - xm));
+      } else {
+        float2 xk = x[k], xmk = cconj(x[M - k]);
+        float2 E = cscale(cadd(xk, xmk), 0.5f);
+        float2 O = cmul(cscale(csub(xk, xmk), 0.5f), cconj(g_twiddle[k * (TW_N / (2 * M))]));
+        zk = make_float2(E.x - O.y, E.y + O.x);        // E + i O
+        zmk = make_float2(E.x + O.y, -E.y + O.x);      // conj(E) + i conj(O)
+      }
+    }
+    lds[c * LP + k] = zk;
+    if (k != 0 && 2 * k != M) lds[c * LP + (M - k)] = zmk;
+  }
+  __syncthreads();
+  fft_lds<M, C, 1, LP, false, NT, true>(lds);
+  float2* out2 = reinterpret_cast<float2*>(out);
+  for (int idx = tid; idx < C * M; idx += NT) {
+    int c = idx / M, l = idx % M;
+    int64_t row = row0 + c;
+    if (row < nrows) out2[row * M + l] = cscale(lds[c * LP + l], scale);
+  }
+}
+
+// ------------------------------------------------------------------ column passes
+
+// Generic column FFT: block (chunk, outer) transforms columns c0..c0+C-1 (c < ncols) of
+// in[outer * in_ostride + l * in_fstride + c] into out[...] (may alias in).
+template <int L, bool INV>
+__global__ void __launch_bounds__(ColCfg<L>::NT)
+col_fft_kernel(const float2* in, float2* out, int ncols, int64_t in_fstride, int64_t in_ostride,
+               int64_t out_fstride, int64_t out_ostride, float scale) {
+  using Cfg = ColCfg<L>;
+  constexpr int C = Cfg::C, NT = Cfg::NT;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C;
+  const float2* src = in + (int64_t)blockIdx.y * in_ostride + c0;
+  for (int idx = tid; idx < C * L; idx += NT) {
+    int c = idx % C, l = idx / C;
+    lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<L, C, C, 1, true, NT, INV>(lds);
+  float2* dst = out + (int64_t)blockIdx.y * out_ostride + c0;
+  for (int idx = tid; idx < C * L; idx += NT) {
+    int c = idx % C, l = idx / C;
+    if (c0 + c < ncols) dst[l * out_fstride + c] = cscale(lds[l * C + c], scale);
+  }
+}
+
+// Interlacing combine along x (the last forward pass of the painted meshes):
+//   out(kx, ky, kz) = scale * comp_x comp_y comp_z * sum_j FFT_x(in_j)(kx) ph_x[j,kx] ph_y[j,ky] ph_z[j,kz]
+// Block (chunk, y) handles columns kz in [c0, c0+C) of the y-th row for all kx.
+template <int L>
+__global__ void __launch_bounds__(ColCfg<L>::NT)
+xpass_combine_kernel(const float2* __restrict__ in, int64_t in_mesh_stride, int ni, float2* __restrict__ out,
+                     int ncols, int ny, int64_t in_fstride, int64_t in_ostride, int64_t out_fstride,
+                     int64_t out_ostride, CombineTables tab, float scale) {
+  using Cfg = ColCfg<L>;
+  constexpr int C = Cfg::C, NT = Cfg::NT;
+  constexpr int EPT = (C * L + NT - 1) / NT;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C;
+  const int y = blockIdx.y;
+  float2 acc[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
+  for (int j = 0; j < ni; ++j) {
+    const float2* src = in + j * in_mesh_stride + (int64_t)y * in_ostride + c0;
+    for (int idx = tid; idx < C * L; idx += NT) {
+      int c = idx % C, l = idx / C;
+      lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lds<L, C, C, 1, true, NT, false>(lds);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      int idx = tid + e * NT;
+      if (idx < C * L) {
+        int c = idx % C, l = idx / C;
+        float2 v = lds[l * C + c];
+        if (tab.phx) {
+          int kz = min(c0 + c, ncols - 1);
+          float2 ph = cmul(cmul(tab.phx[j * tab.stride + l], tab.phy[j * tab.stride + y]), tab.phz[j * tab.stride + kz]);
+          v = cmul(v, ph);
+        }
+        acc[e] = cadd(acc[e], v);
+      }
+    }
+    __syncthreads();
+  }
+  float2* dst = out + (int64_t)y * out_ostride + c0;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    int idx = tid + e * NT;
+    if (idx < C * L) {
+      int c = idx % C, l = idx / C;
+      if (c0 + c < ncols) {
+        float s = scale;
+        if (tab.cx) s *= tab.cx[l] * tab.cy[y] * tab.cz[c0 + c];
+        dst[l * out_fstride + c] = cscale(acc[e], s);
+      }
+    }
+  }
+}
+
+// Local line of sight: acc(kx, ky, kz) += FFT_x(in)(kx) * Y_lm(k) for one (l, m).
+template <int L>
+__global__ void __launch_bounds__(ColCfg<L>::NT)
+xpass_ylm_acc_kernel(const float2* __restrict__ in, float2* __restrict__ acc, int ncols, int64_t in_fstride,
+                     int64_t in_ostride, int64_t acc_fstride, int64_t acc_ostride, KPrologue kp, int first) {
+  using Cfg = ColCfg<L>;
+  constexpr int C = Cfg::C, NT = Cfg::NT;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C;
+  const int y = blockIdx.y;
+  const float2* src = in + (int64_t)y * in_ostride + c0;
+  for (int idx = tid; idx < C * L; idx += NT) {
+    int c = idx % C, l = idx / C;
+    lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<L, C, C, 1, true, NT, false>(lds);
+  float2* dst = acc + (int64_t)y * acc_ostride + c0;
+  for (int idx = tid; idx < C * L; idx += NT) {
+    int c = idx % C, l = idx / C;
+    if (c0 + c < ncols) {
+      float yk = kp.eval(l, y, c0 + c);
+      float2 v = cscale(lds[l * C + c], yk);
+      float2* d = dst + l * acc_fstride + c;
+      *d = first ? v : cadd(*d, v);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host dispatch
+
+template <int M>
+static int launch_zpass_r2c(const float* in, float2* out, int64_t nrows, int64_t pitch, const RealPrologue& pro,
+                            hipStream_t stream) {
+  using Cfg = RowCfg<M>;
+  dim3 grid((unsigned)ceil_div(nrows, Cfg::C));
+  MP_LAUNCH("fft_zpass_r2c", zpass_r2c_kernel<M>, grid, dim3(Cfg::NT), Cfg::LDS, stream, in, out, nrows, pitch, pro);
+  return MP_OK;
+}
+
+template <int M>
+static int launch_zpass_c2r(const float2* in, int64_t pitch, float* out, int64_t nrows, float scale, hipStream_t stream) {
+  using Cfg = RowCfg<M>;
+  dim3 grid((unsigned)ceil_div(nrows, Cfg::C));
+  MP_LAUNCH("fft_zpass_c2r", zpass_c2r_kernel<M>, grid, dim3(Cfg::NT), Cfg::LDS, stream, in, pitch, out, nrows, scale);
+  return MP_OK;
+}
+
+template <int L, bool INV>
+static int launch_col(const char* name, const float2* in, float2* out, int ncols, int64_t nouter, int64_t ifs, int64_t ios,
+                      int64_t ofs, int64_t oos, float scale, hipStream_t stream) {
+  using Cfg = ColCfg<L>;
+  dim3 grid((unsigned)ceil_div(ncols, Cfg::C), (unsigned)nouter);
+  MP_LAUNCH(name, (col_fft_kernel<L, INV>), grid, dim3(Cfg::NT), Cfg::LDS, stream, in, out, ncols, ifs, ios, ofs, oos, scale);
+  return MP_OK;
+}
+
+template <int L>
+static int launch_combine(const float2* in, int64_t mstride, int ni, float2* out, int ncols, int ny, int64_t ifs, int64_t ios,
+                          int64_t ofs, int64_t oos, const CombineTables& tab, float scale, hipStream_t stream) {
+  using Cfg = ColCfg<L>;
+  dim3 grid((unsigned)ceil_div(ncols, Cfg::C), (unsigned)ny);
+  MP_LAUNCH("fft_xpass_combine", xpass_combine_kernel<L>, grid, dim3(Cfg::NT), Cfg::LDS, stream, in, mstride, ni, out, ncols, ny,
+            ifs, ios, ofs, oos, tab, scale);
+  return MP_OK;
+}
+
+template <int L>
+static int launch_ylm_acc(const float2* in, float2* acc, int ncols, int ny, int64_t ifs, int64_t ios, int64_t afs, int64_t aos,
+                          const KPrologue& kp, int first, hipStream_t stream) {
+  using Cfg = ColCfg<L>;
+  dim3 grid((unsigned)ceil_div(ncols, Cfg::C), (unsigned)ny);
+  MP_LAUNCH("fft_xpass_ylm_acc", xpass_ylm_acc_kernel<L>, grid, dim3(Cfg::NT), Cfg::LDS, stream, in, acc, ncols, ifs, ios, afs, aos,
+            kp, first);
+  return MP_OK;
+}
+
+#define MP_POW2_SWITCH(n, F, ...)         \
+  switch (n) {                            \
+    case 1: return F<1>(__VA_ARGS__);     \
+    case 2: return F<2>(__VA_ARGS__);     \
+    case 4: return F<4>(__VA_ARGS__);     \
+    case 8: return F<8>(__VA_ARGS__);     \
+    case 16: return F<16>(__VA_ARGS__);   \
+    case 32: return F<32>(__VA_ARGS__);   \
+    case 64: return F<64>(__VA_ARGS__);   \
+    case 128: return F<128>(__VA_ARGS__); \
+    case 256: return F<256>(__VA_ARGS__); \
+    case 512: return F<512>(__VA_ARGS__); \
+    case 1024: return F<1024>(__VA_ARGS__); \
+    case 2048: return F<2048>(__VA_ARGS__); \
+    default: return set_error(MP_ERR_UNSUPPORTED, "FFT length %d is not a power of two in [1, 2048]", (int)(n)); \
+  }
+
+static int zpass_r2c(int M, const float* in, float2* out, int64_t nrows, int64_t pitch, const RealPrologue& pro, hipStream_t s) {
+  if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  MP_POW2_SWITCH(M, launch_zpass_r2c, in, out, nrows, pitch, pro, s);
+}
+
+static int zpass_c2r(int M, const float2* in, int64_t pitch, float* out, int64_t nrows, float scale, hipStream_t s) {
+  if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  MP_POW2_SWITCH(M, launch_zpass_c2r, in, pitch, out, nrows, scale, s);
+}
+
+template <int L>
+static int launch_col_fwd(const char* name, const float2* in, float2* out, int ncols, int64_t nouter, int64_t ifs, int64_t ios,
+                          int64_t ofs, int64_t oos, float scale, hipStream_t s) {
+  return launch_col<L, false>(name, in, out, ncols, nouter, ifs, ios, ofs, oos, scale, s);
+}
+template <int L>
+static int launch_col_inv(const char* name, const float2* in, float2* out, int ncols, int64_t nouter, int64_t ifs, int64_t ios,
+                          int64_t ofs, int64_t oos, float scale, hipStream_t s) {
+  return launch_col<L, true>(name, in, out, ncols, nouter, ifs, ios, ofs, oos, scale, s);
+}
+
+static int col_fft(int L, bool inv, const char* name, const float2* in, float2* out, int ncols, int64_t nouter, int64_t ifs,
+                   int64_t ios, int64_t ofs, int64_t oos, float scale, hipStream_t s) {
+  if (L < 2) return set_error(MP_ERR_UNSUPPORTED, "column FFT length %d < 2", L);
+  if (inv) {
+    MP_POW2_SWITCH(L, launch_col_inv, name, in, out, ncols, nouter, ifs, ios, ofs, oos, scale, s);
+  }
+  MP_POW2_SWITCH(L, launch_col_fwd, name, in, out, ncols, nouter, ifs, ios, ofs, oos, scale, s);
+}
+
+static int xpass_combine(int L, const float2* in, int64_t mstride, int ni, float2* out, int ncols, int ny, int64_t ifs,
+                         int64_t ios, int64_t ofs, int64_t oos, const CombineTables& tab, float scale, hipStream_t s) {
+  if (L < 2) return set_error(MP_ERR_UNSUPPORTED, "x length %d < 2", L);
+  MP_POW2_SWITCH(L, launch_combine, in, mstride, ni, out, ncols, ny, ifs, ios, ofs, oos, tab, scale, s);
+}
+
+static int xpass_ylm_acc(int L, const float2* in, float2* acc, int ncols, int ny, int64_t ifs, int64_t ios, int64_t afs,
+                         int64_t aos, const KPrologue& kp, int first, hipStream_t s) {
+  if (L < 2) return set_error(MP_ERR_UNSUPPORTED, "x length %d < 2", L);
+  MP_POW2_SWITCH(L, launch_ylm_acc, in, acc, ncols, ny, ifs, ios, afs, aos, kp, first, s);
+}
+
+// Check the LDS attribute once per kernel instantiation that needs more than 64 KiB (none
+// with the current tile sizes: every configuration is <= 64 KiB).
+static int check_shape(const int32_t n[3]) {
+  for (int a = 0; a < 3; ++a) {
+    if (!is_pow2(n[a])) return set_error(MP_ERR_UNSUPPORTED, "meshsize[%d] = %d is not a power of two", a, n[a]);
+    if (n[a] > 2048) return set_error(MP_ERR_UNSUPPORTED, "meshsize[%d] = %d > 2048", a, n[a]);
+  }
+  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return set_error(MP_ERR_UNSUPPORTED, "meshsize must be >= 2 per axis");
+  return MP_OK;
+}
+
+// ------------------------------------------------------------------ internal API
+
+int rfft3d(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2, ncols = M + 1;
+  if (pitch < ncols) return set_error(MP_ERR_INVALID, "pitch %lld < Nz/2+1", (long long)pitch);
+  const int64_t nx = n[0], ny = n[1];
+  rc = zpass_r2c(M, in, out, nx * ny, pitch, pro, s);
+  if (rc) return rc;
+  rc = col_fft(n[1], false, "fft_ypass", out, out, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
+  if (rc) return rc;
+  return col_fft(n[0], false, "fft_xpass", out, out, ncols, ny, ny * pitch, pitch, ny * pitch, pitch, 1.f, s);
+}
+
+int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2, ncols = M + 1;
+  const int64_t nx = n[0], ny = n[1];
+  rc = zpass_r2c(M, in, out, nx * ny, pitch, pro, s);
+  if (rc) return rc;
+  return col_fft(n[1], false, "fft_ypass", out, out, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
+}
+
+int combine_x(hipStream_t s, const float2* partial, int64_t mesh_stride, int ni, float2* out, const int32_t n[3],
+              int64_t in_pitch, int64_t out_pitch, const CombineTables& tab, float scale) {
+  const int ncols = n[2] / 2 + 1;
+  const int64_t ny = n[1];
+  return xpass_combine(n[0], partial, mesh_stride, ni, out, ncols, n[1], ny * in_pitch, in_pitch, ny * out_pitch, out_pitch,
+                       tab, scale, s);
+}
+
+int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n[3], int64_t in_pitch, int64_t acc_pitch,
+              const KPrologue& kp, int first) {
+  const int ncols = n[2] / 2 + 1;
+  const int64_t ny = n[1];
+  return xpass_ylm_acc(n[0], partial, acc, ncols, n[1], ny * in_pitch, in_pitch, ny * acc_pitch, acc_pitch, kp, first, s);
+}
+
+int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const int32_t n[3], float2* scratch,
+            int64_t scratch_pitch) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2, ncols = M + 1;
+  const int64_t nx = n[0], ny = n[1];
+  rc = col_fft(n[0], true, "fft_xpass_inv", in, scratch, ncols, ny, ny * in_pitch, in_pitch, ny * scratch_pitch, scratch_pitch,
+               1.f, s);
+  if (rc) return rc;
+  rc = col_fft(n[1], true, "fft_ypass_inv", scratch, scratch, ncols, nx, scratch_pitch, ny * scratch_pitch, scratch_pitch,
+               ny * scratch_pitch, 1.f, s);
+  if (rc) return rc;
+  // the z pass returns M * (x_even + i x_odd) for M = Nz/2 (unnormalised M-point inverse)
+  const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)M));
+  return zpass_c2r(M, scratch, scratch_pitch, out, nx * ny, scale, s);
+}
+
+int64_t spectrum_pitch(int nz) { return (int64_t)align_up((size_t)(nz / 2 + 1), 16); }
+
+}  // namespace mp
+
+using namespace mp;
+
+// ------------------------------------------------------------------ C ABI
+
+MP_EXPORT int64_t mp_spectrum_pitch(int32_t nz) { return spectrum_pitch(nz); }
+
+MP_EXPORT int mp_rfft3d(void* stream, const float* in, void* out, const int32_t meshsize[3], int64_t out_pitch) {
+  MP_REQUIRE(in && out && meshsize, "mp_rfft3d: null pointer");
+  RealPrologue pro{};
+  return rfft3d((hipStream_t)stream, in, (float2*)out, meshsize, out_pitch, pro);
+}
+
+MP_EXPORT int64_t mp_irfft3d_workspace_bytes(const int32_t meshsize[3]) {
+  return (int64_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * (int64_t)sizeof(float2) + 256;
+}
+
+MP_EXPORT int mp_irfft3d(void* stream, const void* in, int64_t in_pitch, float* out, const int32_t meshsize[3],
+                         void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(in && out && meshsize, "mp_irfft3d: null pointer");
+  MP_REQUIRE(in_pitch >= meshsize[2] / 2 + 1, "mp_irfft3d: in_pitch too small");
+  Workspace ws(workspace, workspace_bytes);
+  int64_t sp = spectrum_pitch(meshsize[2]);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_irfft3d: workspace too small");
+  return irfft3d((hipStream_t)stream, (const float2*)in, in_pitch, out, meshsize, scratch, sp);
+}

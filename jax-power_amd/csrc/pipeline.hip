@@ -1,0 +1,230 @@
+// Composite entry points: interlaced paint -> spectrum, paint -> real mesh, the local
+// line-of-sight Y_lm accumulation, and FP64 reductions.
+//
+// mp_paint_spectrum replaces _paint(..., out='complex') (jaxpower/mesh.py:2673-2754): the
+// n_i shifted meshes are painted in one pass, each is transformed along z and y, and one x
+// pass transforms all n_i partial spectra, applies exp(+i s_j (kx+ky+kz)) / n_i (mesh.py:
+// 2740-2744), sums them, and applies the compensation prod_a sinc(k_a/2pi)^-p
+// (resamplers.py:16-26) before the single write of the result.
+#include "common.h"
+#include "fft.h"
+#include "paint.h"
+
+#include <cmath>
+
+namespace mp {
+
+int fill_f32(hipStream_t s, float* p, int64_t n, float v);
+int read_mesh(hipStream_t s, const float* mesh, const GridMap& g, const void* pos, int pos_f64, int64_t np, int order, float* out);
+
+// Per-axis interlacing phases exp(i s_j k_circ) and compensation sinc(k/2pi)^-order, in FP64
+// then rounded to FP32.  k_circ = 2 pi fftfreq(n) (rfftfreq on the last axis), mesh.py:1605-1636.
+__global__ void combine_tables_kernel(int n0, int n1, int n2, int ni, int order, float2* __restrict__ ph, float* __restrict__ comp) {
+  const int ncols = n2 / 2 + 1;
+  const int len = n0 + n1 + ncols;
+  const int total = (ni + 1) * len;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int j = t / len, r = t % len;
+  int axis = r < n0 ? 0 : (r < n0 + n1 ? 1 : 2);
+  int m = axis == 0 ? r : (axis == 1 ? r - n0 : r - n0 - n1);
+  int n = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+  int f = (axis == 2 || m < (n + 1) / 2) ? m : m - n;
+  double x = (double)f / (double)n;  // fftfreq value
+  if (j < ni) {
+    double s = (double)j / (double)ni;
+    double a = 2.0 * M_PI * x * s;
+    ph[t] = make_float2((float)cos(a), (float)sin(a));
+  } else {
+    double c = 1.0;
+    if (f != 0 && order > 0) {
+      double px = M_PI * x;
+      c = pow(sin(px) / px, -(double)order);
+    }
+    comp[r] = (float)c;
+  }
+}
+
+size_t paint_spectrum_workspace_bytes(const int32_t n[3], int64_t np, int order, int ni) {
+  const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+  const int64_t nspec = (int64_t)n[0] * n[1] * spectrum_pitch(n[2]);
+  const int64_t len = n[0] + n[1] + n[2] / 2 + 1;
+  size_t bytes = 0;
+  bytes += align_up((size_t)ni * ncell * 4, 256);
+  bytes += align_up((size_t)ni * nspec * 8, 256);
+  bytes += align_up((size_t)(ni + 1) * len * 8, 256);
+  bytes += align_up((size_t)len * 4, 256);
+  bytes += paint_workspace_bytes(n, np, order, ni);
+  return bytes + 1024;
+}
+
+int paint_spectrum(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
+                   int compensate, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes) {
+  const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
+  MP_REQUIRE(out_pitch >= n[2] / 2 + 1, "paint_spectrum: out_pitch too small");
+  const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+  const int64_t P = spectrum_pitch(n[2]);
+  const int64_t nspec = (int64_t)n[0] * n[1] * P;
+  const int len = n[0] + n[1] + n[2] / 2 + 1;
+  Workspace wsp(ws, ws_bytes);
+  float* meshes = wsp.take<float>((size_t)ni * ncell);
+  float2* partial = wsp.take<float2>((size_t)ni * nspec);
+  float2* ph = wsp.take<float2>((size_t)(ni + 1) * len);
+  float* comp = wsp.take<float>((size_t)len);
+  size_t pbytes = paint_workspace_bytes(n, np, order, ni);
+  void* pws = wsp.take<char>(pbytes);
+  if (!meshes || !partial || !ph || !comp || !pws)
+    return set_error(MP_ERR_WORKSPACE, "paint_spectrum: workspace of %zu bytes < %zu needed", ws_bytes,
+                     paint_spectrum_workspace_bytes(n, np, order, ni));
+  int rc = paint_meshes(s, pos, pos_f64, w, np, g, order, ni, meshes, ncell, pws, pbytes, 0);
+  if (rc) return rc;
+  RealPrologue pro{};
+  for (int j = 0; j < ni; ++j) {
+    rc = rfft3d_zy(s, meshes + j * ncell, partial + j * nspec, n, P, pro);
+    if (rc) return rc;
+  }
+  const int total = (ni + 1) * len;
+  MP_LAUNCH("combine_tables", combine_tables_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, n[0], n[1], n[2], ni,
+            compensate ? order : 0, ph, comp);
+  CombineTables tab{};
+  tab.stride = len;
+  if (ni > 1) {
+    tab.phx = ph;
+    tab.phy = ph + n[0];
+    tab.phz = ph + n[0] + n[1];
+  }
+  if (compensate) {
+    tab.cx = comp;
+    tab.cy = comp + n[0];
+    tab.cz = comp + n[0] + n[1];
+  }
+  return combine_x(s, partial, nspec, ni, out, n, P, out_pitch, tab, 1.f / (float)ni);
+}
+
+// ------------------------------------------------------------------ FP64 reductions
+
+constexpr int RED_NT = 256;
+constexpr int RED_BLOCKS = 1024;
+
+__global__ void __launch_bounds__(RED_NT)
+dot_partial_kernel(const float* __restrict__ a, const float* __restrict__ b, int64_t n, int square, double* __restrict__ partial) {
+  __shared__ double sh[RED_NT];
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RED_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_NT) {
+    double x = (double)a[i];
+    s += b ? x * (double)b[i] : (square ? x * x : x);
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = RED_NT / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(RED_NT) final_sum_kernel(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+  __shared__ double sh[RED_NT];
+  double s = 0.;
+  for (int i = threadIdx.x; i < nblocks; i += RED_NT) s += partial[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = RED_NT / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+static GridMap make_map(const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3]) {
+  GridMap g;
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = meshsize[a];
+    g.cell[a] = boxsize[a] / (double)meshsize[a];
+    g.shift[a] = boxsize[a] / 2. - boxcenter[a];
+  }
+  return g;
+}
+
+MP_EXPORT int64_t mp_paint_spectrum_workspace_bytes(const int32_t meshsize[3], int64_t np, int order, int interlacing) {
+  return (int64_t)paint_spectrum_workspace_bytes(meshsize, np, order, interlacing < 1 ? 1 : interlacing);
+}
+
+MP_EXPORT int mp_paint_spectrum(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+                                const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
+                                int interlacing, int compensate, void* out, int64_t out_pitch, void* workspace,
+                                size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && boxsize && boxcenter && out, "mp_paint_spectrum: null pointer");
+  MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
+  GridMap g = make_map(meshsize, boxsize, boxcenter);
+  return paint_spectrum((hipStream_t)stream, positions, positions_f64, weights, np, g, order, interlacing < 1 ? 1 : interlacing,
+                        compensate, (float2*)out, out_pitch, workspace, workspace_bytes);
+}
+
+MP_EXPORT int64_t mp_ylm_workspace_bytes(const int32_t meshsize[3]) {
+  return (int64_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * 8 + 1024;
+}
+
+// C_l(k) = sum_m r2c(rho * Y_lm(xhat))(k) * Y_lm(khat), jaxpower/mesh2.py:319-329.
+MP_EXPORT int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t meshsize[3], const double boxsize[3],
+                                const double boxcenter[3], int ell, void* out, int64_t out_pitch, void* workspace,
+                                size_t workspace_bytes) {
+  MP_REQUIRE(rmesh && meshsize && boxsize && boxcenter && out, "mp_ylm_accumulate: null pointer");
+  MP_REQUIRE(ell >= 1 && ell <= MAX_ELL, "mp_ylm_accumulate: ell %d not in 1..%d", ell, MAX_ELL);
+  MP_REQUIRE(out_pitch >= meshsize[2] / 2 + 1, "mp_ylm_accumulate: out_pitch too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t P = spectrum_pitch(meshsize[2]);
+  Workspace wsp(workspace, workspace_bytes);
+  float2* tmp = wsp.take<float2>((size_t)meshsize[0] * meshsize[1] * P);
+  if (!tmp) return set_error(MP_ERR_WORKSPACE, "mp_ylm_accumulate: workspace too small");
+  for (int m = -ell; m <= ell; ++m) {
+    RealPrologue pro{};
+    pro.kind = PRO_YLM;
+    pro.ell = ell;
+    pro.m = m;
+    pro.ny = meshsize[1];
+    pro.nz = meshsize[2];
+    pro.x0 = (float)(boxcenter[0] - boxsize[0] / 2.);
+    pro.y0 = (float)(boxcenter[1] - boxsize[1] / 2.);
+    pro.z0 = (float)(boxcenter[2] - boxsize[2] / 2.);
+    pro.dx = (float)(boxsize[0] / meshsize[0]);
+    pro.dy = (float)(boxsize[1] / meshsize[1]);
+    pro.dz = (float)(boxsize[2] / meshsize[2]);
+    int rc = rfft3d_zy(s, rmesh, tmp, meshsize, P, pro);
+    if (rc) return rc;
+    KPrologue kp{};
+    kp.ell = ell;
+    kp.m = m;
+    kp.nx = meshsize[0];
+    kp.ny = meshsize[1];
+    kp.kfx = (float)(2. * M_PI / boxsize[0]);
+    kp.kfy = (float)(2. * M_PI / boxsize[1]);
+    kp.kfz = (float)(2. * M_PI / boxsize[2]);
+    rc = ylm_acc_x(s, tmp, (float2*)out, meshsize, P, out_pitch, kp, m == -ell);
+    if (rc) return rc;
+  }
+  return MP_OK;
+}
+
+MP_EXPORT int64_t mp_reduce_workspace_bytes(void) { return RED_BLOCKS * sizeof(double) + 256; }
+
+// out[0] = sum_i a_i * b_i (b given), sum_i a_i^2 (square), or sum_i a_i, accumulated in FP64.
+MP_EXPORT int mp_sum_f64(void* stream, const float* a, const float* b, int64_t n, int square, double* out, void* workspace,
+                         size_t workspace_bytes) {
+  MP_REQUIRE(out && (a || n == 0), "mp_sum_f64: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  Workspace wsp(workspace, workspace_bytes);
+  double* partial = wsp.take<double>(RED_BLOCKS);
+  if (!partial) return set_error(MP_ERR_WORKSPACE, "mp_sum_f64: workspace too small");
+  int blocks = (int)(ceil_div(n, RED_NT) < RED_BLOCKS ? ceil_div(n, RED_NT) : RED_BLOCKS);
+  if (blocks < 1) blocks = 1;
+  MP_LAUNCH("sum_partial", dot_partial_kernel, dim3(blocks), dim3(RED_NT), 0, s, a, b, n, square, partial);
+  MP_LAUNCH("sum_final", final_sum_kernel, dim3(1), dim3(RED_NT), 0, s, (const double*)partial, blocks, out);
+  return MP_OK;
+}
+
+MP_EXPORT int mp_fill_f32(void* stream, float* p, int64_t n, float value) { return fill_f32((hipStream_t)stream, p, n, value); }

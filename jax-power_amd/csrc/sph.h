@@ -1,0 +1,97 @@
+// Legendre polynomials and real spherical harmonics on the device.
+//
+// Real Y_lm follow the normalisation of jaxpower/utils.py:227-238:
+//   sqrt((2l+1)/4pi) sqrt((l-|m|)!/(l+|m|)!) sqrt(2) P_l^|m|(cos theta) {cos m phi | sin |m| phi},
+// written as homogeneous polynomials of the unit vector so that Y_lm(0) = 0 (utils.py:252-254).
+// Only sum_m Y_lm(a) Y_lm(b) = (2l+1)/(4pi) P_l(a.b) enters the estimator, so the sign of
+// each m term is a convention that cancels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mp {
+
+constexpr int MAX_ELL = 4;
+
+// P_l(mu) by the three-term recurrence (closed forms of utils.py:341-360 up to rounding).
+__host__ __device__ __forceinline__ float legendre(int ell, float mu) {
+  if (ell == 0) return 1.f;
+  float p0 = 1.f, p1 = mu;
+  for (int n = 1; n < ell; ++n) {
+    float p2 = ((2 * n + 1) * mu * p1 - n * p0) / (n + 1);
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+__host__ __device__ __forceinline__ float ylm_unit(int ell, int m, float x, float y, float z) {
+  switch (ell) {
+    case 0:
+      return 0.28209479177387814f;  // 1/(2 sqrt(pi))
+    case 1: {
+      const float a = 0.4886025119029199f;  // sqrt(3/(4pi))
+      return m < 0 ? a * y : (m == 0 ? a * z : a * x);
+    }
+    case 2: {
+      const float c1 = 1.0925484305920792f;  // (1/2) sqrt(15/pi)
+      const float c0 = 0.31539156525252005f; // (1/4) sqrt(5/pi)
+      const float c2 = 0.5462742152960396f;  // (1/4) sqrt(15/pi)
+      switch (m) {
+        case -2: return c1 * x * y;
+        case -1: return c1 * y * z;
+        case 0: return c0 * (2.f * z * z - x * x - y * y);
+        case 1: return c1 * x * z;
+        default: return c2 * (x * x - y * y);
+      }
+    }
+    case 3: {
+      const float c3 = 0.5900435899266435f;   // (1/4) sqrt(35/(2pi))
+      const float c2 = 2.890611442640554f;    // (1/2) sqrt(105/pi)
+      const float c1 = 0.4570457994644658f;   // (1/4) sqrt(21/(2pi))
+      const float c0 = 0.3731763325901154f;   // (1/4) sqrt(7/pi)
+      const float c2b = 1.445305721320277f;   // (1/4) sqrt(105/pi)
+      float x2 = x * x, y2 = y * y, z2 = z * z;
+      switch (m) {
+        case -3: return c3 * y * (3.f * x2 - y2);
+        case -2: return c2 * x * y * z;
+        case -1: return c1 * y * (4.f * z2 - x2 - y2);
+        case 0: return c0 * z * (2.f * z2 - 3.f * x2 - 3.f * y2);
+        case 1: return c1 * x * (4.f * z2 - x2 - y2);
+        case 2: return c2b * z * (x2 - y2);
+        default: return c3 * x * (x2 - 3.f * y2);
+      }
+    }
+    default: {  // ell == 4
+      const float c4 = 2.5033429417967046f;   // (3/4) sqrt(35/pi)
+      const float c3 = 1.7701307697799304f;   // (3/4) sqrt(35/(2pi))
+      const float c2 = 0.9461746957575601f;   // (3/4) sqrt(5/pi)
+      const float c1 = 0.6690465435572892f;   // (3/4) sqrt(5/(2pi))
+      const float c0 = 0.10578554691520431f;  // (3/16) sqrt(1/pi)
+      const float c2b = 0.47308734787878004f; // (3/8) sqrt(5/pi)
+      const float c4b = 0.6258357354491761f;  // (3/16) sqrt(35/pi)
+      float x2 = x * x, y2 = y * y, z2 = z * z, r2 = x2 + y2 + z2;
+      switch (m) {
+        case -4: return c4 * x * y * (x2 - y2);
+        case -3: return c3 * (3.f * x2 - y2) * y * z;
+        case -2: return c2 * x * y * (7.f * z2 - r2);
+        case -1: return c1 * y * z * (7.f * z2 - 3.f * r2);
+        case 0: return c0 * (35.f * z2 * z2 - 30.f * z2 * r2 + 3.f * r2 * r2);
+        case 1: return c1 * x * z * (7.f * z2 - 3.f * r2);
+        case 2: return c2b * (x2 - y2) * (7.f * z2 - r2);
+        case 3: return c3 * (x2 - 3.f * y2) * x * z;
+        default: return c4b * (x2 * (x2 - 3.f * y2) - y2 * (3.f * x2 - y2));
+      }
+    }
+  }
+}
+
+// Y_lm of the direction of (x, y, z); zero at the origin.
+__host__ __device__ __forceinline__ float ylm_of_vector(int ell, int m, float x, float y, float z) {
+  float r2 = x * x + y * y + z * z;
+  if (r2 == 0.f) return 0.f;
+  float inv = 1.f / sqrtf(r2);
+  return ylm_unit(ell, m, x * inv, y * inv, z * inv);
+}
+
+}  // namespace mp

@@ -1,0 +1,113 @@
+"""ctypes binding of libjaxpower_amd.so (declared in include/jaxpower_amd.h).
+
+torch is imported first so that the library binds to the HIP runtime torch already loaded
+(both carry the soname libamdhip64.so.7): device pointers and streams are then shared.
+The product path has no fallback: if the library or a device is missing, calls raise.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime the library must share)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, 'lib', 'libjaxpower_amd.so')
+
+c_int, c_int64, c_size_t, c_double, c_float, c_void_p, c_char_p = (ctypes.c_int, ctypes.c_int64, ctypes.c_size_t,
+                                                                   ctypes.c_double, ctypes.c_float, ctypes.c_void_p,
+                                                                   ctypes.c_char_p)
+P_I32 = ctypes.POINTER(ctypes.c_int32)
+P_I64 = ctypes.POINTER(ctypes.c_int64)
+P_F64 = ctypes.POINTER(ctypes.c_double)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    'mp_last_error': (c_char_p, []),
+    'mp_version': (c_int, []),
+    'mp_timing_enable': (c_int, [c_int]),
+    'mp_timing_collect': (c_int, []),
+    'mp_timing_reset': (c_int, []),
+    'mp_timing_report': (c_int, [c_char_p, c_int64]),
+    'mp_paint_workspace_bytes': (c_int64, [P_I32, c_int64, c_int, c_int]),
+    'mp_paint': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, P_I32, P_F64, P_F64, c_int, c_int, c_void_p,
+                         c_void_p, c_size_t]),
+    'mp_paint_atomic': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, P_I32, P_F64, P_F64, c_int, c_int, c_void_p]),
+    'mp_paint_spectrum_workspace_bytes': (c_int64, [P_I32, c_int64, c_int, c_int]),
+    'mp_paint_spectrum': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, P_I32, P_F64, P_F64, c_int, c_int, c_int,
+                                  c_void_p, c_int64, c_void_p, c_size_t]),
+    'mp_read': (c_int, [c_void_p, c_void_p, P_I32, P_F64, P_F64, c_void_p, c_int, c_int64, c_int, c_void_p]),
+    'mp_spectrum_pitch': (c_int64, [ctypes.c_int32]),
+    'mp_rfft3d': (c_int, [c_void_p, c_void_p, c_void_p, P_I32, c_int64]),
+    'mp_irfft3d_workspace_bytes': (c_int64, [P_I32]),
+    'mp_irfft3d': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, P_I32, c_void_p, c_size_t]),
+    'mp_bin_modes': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    'mp_bin_workspace_bytes': (c_int64, [c_int, c_int]),
+    'mp_bin_poles': (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), P_I64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_int, P_F64, c_int, P_I32, P_I32, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
+    'mp_ylm_workspace_bytes': (c_int64, [P_I32]),
+    'mp_ylm_accumulate': (c_int, [c_void_p, c_void_p, P_I32, P_F64, P_F64, c_int, c_void_p, c_int64, c_void_p, c_size_t]),
+    'mp_reduce_workspace_bytes': (c_int64, []),
+    'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
+    'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),
+}
+
+
+class LibraryError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIBPATH):
+    """Load and type the shared library (once)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LibraryError(f'{path} not found: build it with `python jax-power_amd/build_lib.py`')
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def check(rc, name=''):
+    if rc != 0:
+        msg = load().mp_last_error().decode(errors='replace')
+        exc = NotImplementedError if rc == -2 else LibraryError
+        raise exc(f'{name} failed ({rc}): {msg}')
+
+
+def call(name, *args):
+    rc = getattr(load(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def i32x3(values):
+    return (ctypes.c_int32 * 3)(*[int(v) for v in values])
+
+
+def f64x3(values):
+    return (ctypes.c_double * 3)(*[float(v) for v in values])
+
+
+def ptr(tensor):
+    return ctypes.c_void_p(tensor.data_ptr()) if tensor is not None else ctypes.c_void_p(None)
+
+
+def stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_device(tensor, name='array'):
+    if not (isinstance(tensor, torch.Tensor) and tensor.is_cuda):
+        raise ValueError(f'{name} must be a torch tensor on the GPU, got {type(tensor)} on '
+                         f'{getattr(tensor, "device", None)}')
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)

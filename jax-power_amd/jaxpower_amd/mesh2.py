@@ -1,0 +1,291 @@
+"""Power-spectrum multipoles from meshes (jaxpower/mesh2.py:25-119, 195-374, 505-621).
+
+``BinMesh2SpectrumPoles`` keeps the bin geometry: edges, exact integer mode counts, mode-averaged
+|k|, and the device tables the binning kernel reads (per-axis wavenumbers and per-edge k^2
+thresholds).  It never stores the per-mode bin index: the kernel recomputes it from k^2.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .mesh import (MeshAttrs, BaseMeshField, RealMeshField, ComplexMeshField, ParticleField, FKPField, format_meshes,
+                   compute_normalization, compute_box_normalization, _as_spectrum, _sum_f64, _as_device_tensor,
+                   _default_device)
+from .types import Mesh2SpectrumPole, Mesh2SpectrumPoles
+
+
+def _format_ells(ells):
+    if np.ndim(ells) == 0:
+        ells = (ells,)
+    return tuple(sorted(int(ell) for ell in ells))
+
+
+def _k2_thresholds(edges):
+    """t_e = min{t : fl(sqrt(t)) >= e}: digitize(|k|, edges) == number of t_e <= k^2."""
+    thr = np.empty(len(edges), dtype='f8')
+    for i, e in enumerate(np.asarray(edges, dtype='f8')):
+        if not e > 0.:
+            thr[i] = -1. if e <= 0. else np.inf
+            continue
+        t = e * e
+        while np.sqrt(t) >= e:
+            t = np.nextafter(t, 0.)
+        while np.sqrt(t) < e:
+            t = np.nextafter(t, np.inf)
+        thr[i] = t
+    return thr
+
+
+def _kvectors(mattrs):
+    """Per-axis wavenumbers fftfreq(n) * kfun * n (rfftfreq on the last axis), float64
+    (mesh.py:360-403, 1605-1636)."""
+    toret = []
+    for axis, n in enumerate(mattrs.meshsize):
+        n = int(n)
+        f = np.fft.rfftfreq(n) if axis == 2 else np.fft.fftfreq(n)
+        toret.append(f * mattrs.kfun[axis] * n)
+    return toret
+
+
+def _find_unique_edges(mattrs, xmin=0., xmax=np.inf, device=None):
+    """mesh.py:2788-2809: edges at mid-points between the distinct |k| of the lattice."""
+    kx, ky, kz = (torch.from_numpy(k).to(device) for k in _kvectors(mattrs))
+    x2 = (kx[:, None, None]**2 + ky[None, :, None]**2 + kz[None, None, :]**2).reshape(-1)
+    x2 = x2[(x2 >= xmin**2) & (x2 <= xmax**2)]
+    x0 = float(mattrs.kfun.min())
+    keys = (x2 / (0.5 * x0)**2 + 0.5).to(torch.int64)
+    keys, inverse = torch.unique(keys, return_inverse=True)
+    first = torch.full((len(keys),), x2.numel(), dtype=torch.int64, device=x2.device)
+    first = first.scatter_reduce(0, inverse, torch.arange(x2.numel(), device=x2.device), reduce='amin')
+    x = torch.sqrt(x2[first]).cpu().numpy()
+    tmp = (x[:-1] + x[1:]) / 2.
+    return np.concatenate([[tmp[0] - (x[1] - x[0])], tmp, [tmp[-1] + (x[-1] - x[-2])]])
+
+
+class BinMesh2SpectrumPoles:
+    """Binning operator for mesh to power spectrum (mesh2.py:65-119)."""
+
+    def __init__(self, mattrs, edges=None, ells=0, mode_oversampling=0, device=None):
+        if not isinstance(mattrs, MeshAttrs):
+            mattrs = mattrs.attrs
+        if mode_oversampling:
+            raise NotImplementedError('mode_oversampling > 0 is not implemented on this path')
+        device = device or _default_device()
+        if edges is None:
+            edges = {}
+        if isinstance(edges, dict):
+            step = edges.get('step', None)
+            vecmax = mattrs.kfun.min() * np.min(mattrs.meshsize) / 2.
+            if step is None:
+                edges = _find_unique_edges(mattrs, edges.get('min', 0.), edges.get('max', np.inf), device=device)
+            else:
+                edges = np.arange(edges.get('min', 0.), edges.get('max', vecmax), step)
+        edges = np.asarray(edges, dtype='f8')
+        if edges.ndim == 2:
+            assert np.allclose(edges[1:, 0], edges[:-1, 1])
+            edges = np.append(edges[:, 0], edges[-1, 1])
+        if len(edges) < 2:
+            raise ValueError('need at least two edges')
+        self.mattrs = mattrs
+        self.ells = _format_ells(ells)
+        self._edges1d = edges
+        self.edges = np.column_stack([edges[:-1], edges[1:]])
+        self._device = device
+        kx, ky, kz = _kvectors(mattrs)
+        self._tables = dict(kx=torch.from_numpy(kx).to(device), ky=torch.from_numpy(ky).to(device),
+                            kz=torch.from_numpy(kz).to(device),
+                            thr=torch.from_numpy(_k2_thresholds(edges)).to(device))
+        nbins = len(edges) - 1
+        nmodes = torch.empty(nbins, dtype=torch.int64, device=device)
+        ksum = torch.empty(nbins, dtype=torch.float64, device=device)
+        t = self._tables
+        _lib.call('mp_bin_modes', _lib.stream_handle(device), _lib.i32x3(mattrs.meshsize), _lib.ptr(t['kx']), _lib.ptr(t['ky']),
+                  _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(edges), _lib.ptr(nmodes), _lib.ptr(ksum))
+        self.nmodes = nmodes.cpu().numpy()
+        with np.errstate(invalid='ignore', divide='ignore'):
+            self.xavg = ksum.cpu().numpy() / self.nmodes
+        self.wmodes = None  # hermitian weights 1 + (kz > 0) are applied inside the kernel
+
+    @property
+    def nbins(self):
+        return len(self.xavg)
+
+    def products(self, meshes, products, los=(0., 0., 0.)):
+        """Raw FP64 sums for products (i, j, ell, remove_zero) of conj(mesh_i) * mesh_j
+        (i = -1: the constant 1) weighted by P_ell(k.los/|k|); returns complex (nprod, nbins)."""
+        device = meshes[0].device
+        arrays, pitches = [], []
+        for m in meshes:
+            m, pitch = _as_spectrum(m)
+            arrays.append(m)
+            pitches.append(pitch)
+        nprod = len(products)
+        out = torch.empty((nprod, self.nbins, 2), dtype=torch.float64, device=device)
+        ws = _lib.workspace(_lib.load().mp_bin_workspace_bytes(nprod, len(self._edges1d)), device)
+        ptrs = (_lib.c_void_p * len(arrays))(*[a.data_ptr() for a in arrays])
+        pitch_arr = (_lib.ctypes.c_int64 * len(arrays))(*pitches)
+        col = lambda k: (_lib.ctypes.c_int32 * nprod)(*[int(p[k]) for p in products])
+        t = self._tables
+        _lib.call('mp_bin_poles', _lib.stream_handle(device), len(arrays), ptrs, pitch_arr, _lib.i32x3(self.mattrs.meshsize),
+                  _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(self._edges1d),
+                  _lib.f64x3(los), nprod, col(0), col(1), col(2), col(3), _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        out = out.cpu().numpy()
+        return out[..., 0] + 1j * out[..., 1]
+
+    def __call__(self, mesh, remove_zero=False):
+        """Sum_{modes in bin} w * value / nmodes (mesh2.py:100-119)."""
+        value = mesh.value if isinstance(mesh, BaseMeshField) else _as_device_tensor(mesh, device=self._device)
+        if not value.is_complex():
+            value = value.to(torch.complex64)
+        s = self.products([value], [(-1, 0, 0, bool(remove_zero))])[0]
+        with np.errstate(invalid='ignore', divide='ignore'):
+            s = s / self.nmodes
+        return s if np.iscomplexobj(s) and np.any(s.imag) else s.real
+
+
+def _get_los_vector(los):
+    if isinstance(los, str):
+        vlos = [0., 0., 0.]
+        vlos['xyz'.index(los)] = 1.
+        return np.array(vlos)
+    return np.asarray(los, dtype='f8')
+
+
+def _format_los(los):
+    """mesh2.py:215-222."""
+    vlos, swap = None, False
+    if isinstance(los, str) and los in ['firstpoint', 'local', 'endpoint']:
+        if los == 'local':
+            los = 'firstpoint'
+        swap = los == 'endpoint'
+    else:
+        vlos = _get_los_vector(los)
+    return los, vlos, swap
+
+
+def _2c(mesh):
+    return mesh if isinstance(mesh, ComplexMeshField) else mesh.r2c()
+
+
+def _2r(mesh):
+    return mesh if isinstance(mesh, RealMeshField) else mesh.c2r()
+
+
+def compute_mesh2_spectrum(*meshes, bin=None, los='z'):
+    """Power-spectrum multipoles from one (auto) or two (cross) meshes (mesh2.py:253-374)."""
+    meshes, fields = format_meshes(*meshes, nmeshes=2)
+    mattrs = meshes[0].attrs
+    norm = np.prod(mattrs.meshsize, dtype='f8') / np.prod(mattrs.cellsize) * np.ones_like(bin.xavg)
+    los, vlos, swap = _format_los(los)
+    attrs = dict(los=vlos.tolist() if vlos is not None else los)
+    ells = bin.ells
+    same = fields[1] == fields[0]
+    nonzeroells = [ell for ell in ells if ell != 0]
+
+    if vlos is None:  # local (varying) line of sight, mesh2.py:293-344
+        if swap:
+            meshes = meshes[::-1]
+        rmesh1 = meshes[0]
+        A0 = _2c(rmesh1 if same else meshes[1])
+        bmeshes, products, scales = [A0.value], [], []
+        if 0 in ells:
+            if same:
+                products.append((0, 0, 0, False))
+            else:
+                bmeshes.append(_2c(rmesh1).value)
+                products.append((len(bmeshes) - 1, 0, 0, False))
+            scales.append(1.)
+        if nonzeroells:
+            rreal = _2r(rmesh1)
+            for ell in nonzeroells:
+                C = mattrs.create('complex', device=A0.value.device).value
+                _ylm_accumulate(rreal, ell, C)
+                bmeshes.append(C)
+                products.append((len(bmeshes) - 1, 0, 0, True))
+                scales.append(4. * np.pi)
+        sums = bin.products(bmeshes, products)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            num = [s * scale / bin.nmodes for s, scale in zip(sums, scales)]
+        if swap:
+            num = [np.conj(n) for n in num]
+    else:  # fixed line of sight, mesh2.py:346-374
+        A1 = _2c(meshes[0])
+        bmeshes = [A1.value]
+        j = 0
+        if not same:
+            bmeshes.append(_2c(meshes[1]).value)
+            j = 1
+        products = [(0, j, ell, ell != 0) for ell in ells]
+        sums = bin.products(bmeshes, products, los=vlos)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            num = [s * (2 * ell + 1 if ell else 1) / bin.nmodes for s, ell in zip(sums, ells)]
+    num = [n.real if ell % 2 == 0 else n.imag for ell, n in zip(ells, num)]
+    poles = [Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num[i], num_shotnoise=0.,
+                               norm=norm, volume=np.prod(mattrs.kfun) * bin.nmodes, ell=ell, attrs=attrs)
+             for i, ell in enumerate(ells)]
+    return Mesh2SpectrumPoles(poles, attrs=attrs)
+
+
+def _ylm_accumulate(rmesh, ell, out):
+    a = rmesh.attrs
+    value = rmesh.value.contiguous()
+    n = _lib.i32x3(a.meshsize)
+    ws = _lib.workspace(_lib.load().mp_ylm_workspace_bytes(n), value.device)
+    out_, pitch = _as_spectrum(out)
+    assert out_.data_ptr() == out.data_ptr()
+    _lib.call('mp_ylm_accumulate', _lib.stream_handle(value.device), _lib.ptr(value), n, _lib.f64x3(a.boxsize),
+              _lib.f64x3(a.boxcenter), ell, _lib.ptr(out), pitch, _lib.ptr(ws), ws.numel())
+
+
+def compute_mesh2(*meshes, bin=None, los='z'):
+    """Dispatch on the binning type (mesh2.py:225-250)."""
+    if isinstance(bin, BinMesh2SpectrumPoles):
+        return compute_mesh2_spectrum(*meshes, bin=bin, los=los)
+    raise ValueError(f'bin must be BinMesh2SpectrumPoles on this path, not {type(bin)}')
+
+
+def compute_box2_normalization(*inputs, bin=None):
+    """(sum w)^2 / (N^3 Vcell) for a constant density (mesh2.py:505-513)."""
+    norm = float(compute_box_normalization(*format_meshes(*inputs, nmeshes=2)[0]))
+    if bin is not None:
+        return [norm] * len(bin.ells)
+    return norm
+
+
+def compute_fkp2_normalization(*fkps, bin=None, cellsize=10., split=None, fields=None, **kwargs):
+    """FKP normalisation alpha * sum(D R) / cellvolume on a regridded CIC mesh (mesh2.py:516-574)."""
+    if split is not None:
+        raise NotImplementedError('split normalisation needs the reference random streams; use split=None')
+    kwargs.update(cellsize=cellsize)
+    fkps, fields = format_meshes(*fkps, nmeshes=2)
+    if fields[1] == fields[0]:
+        fkp = fkps[0]
+        alpha = fkp.data.sum() / fkp.randoms.sum()
+        norm = alpha * compute_normalization(fkp.data, fkp.randoms, **kwargs)
+    else:
+        alpha2 = fkps[1].data.sum() / fkps[1].randoms.sum()
+        norm = alpha2 * compute_normalization(fkps[0].data, fkps[1].randoms, **kwargs)
+        alpha2 = fkps[0].data.sum() / fkps[0].randoms.sum()
+        norm = norm + alpha2 * compute_normalization(fkps[1].data, fkps[0].randoms, **kwargs)
+        norm = norm / 2
+    norm = float(norm)
+    if bin is not None:
+        return [norm] * len(bin.ells)
+    return norm
+
+
+def compute_fkp2_shotnoise(*fkps, bin=None, fields=None):
+    """Sum of products of FKP weights (mesh2.py:577-621); non-zero for ell = 0 only."""
+    fkps, fields = format_meshes(*fkps, fields=fields, nmeshes=2)
+
+    def get_particles(fkp):
+        return fkp.particles if isinstance(fkp, FKPField) else fkp
+
+    if fields[1] == fields[0]:
+        p = [get_particles(fkp) for fkp in fkps]
+        shotnoise = float(_sum_f64(p[0].weights, p[1].weights))
+    else:
+        shotnoise = 0.
+    if bin is not None:
+        return [shotnoise * (ell == 0) * np.ones_like(bin.xavg) for ell in bin.ells]
+    return shotnoise

@@ -1,0 +1,135 @@
+"""CPU tests: the C-ABI library loads and exports every declared symbol, and the host-side
+logic of the package (geometry, bin thresholds, containers, argument checks) matches the
+oracle.  No GPU compute is called here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, 'include', 'jaxpower_amd.h')).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(mp_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from jaxpower_amd import _lib
+    lib = _lib.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f'{name} has no ctypes signature'
+    assert lib.mp_version() == 1
+
+
+def test_host_queries_without_device():
+    from jaxpower_amd import _lib
+    lib = _lib.load()
+    assert lib.mp_spectrum_pitch(512) == 272 and lib.mp_spectrum_pitch(64) == 48 and lib.mp_spectrum_pitch(8) == 16
+    n = _lib.i32x3([512, 512, 512])
+    tiled = lib.mp_paint_workspace_bytes(n, 16_000_000, 3, 3)
+    assert tiled >= 16_000_000 * 16 + 4 * (128**3)
+    assert lib.mp_paint_workspace_bytes(_lib.i32x3([20, 20, 20]), 1000, 3, 3) == 256  # atomic path needs none
+    assert lib.mp_bin_workspace_bytes(3, 161) >= 1024 * 3 * 160 * 16
+    # argument errors are reported without touching a device
+    rc = lib.mp_bin_poles(None, 0, None, None, n, None, None, None, None, 1, None, 0, None, None, None, None, None, None, 0)
+    assert rc == -1 and b'null' in lib.mp_last_error() or b'edges' in lib.mp_last_error()
+
+
+def test_mesh_attrs_matches_oracle():
+    from jaxpower_amd import MeshAttrs, get_mesh_attrs, next_fft_size
+    from oracle import power_oracle as po
+    a = MeshAttrs(meshsize=64, boxsize=1000., boxcenter=500.)
+    b = po.Attrs(64, 1000., 500.)
+    for name in ['meshsize', 'boxsize', 'boxcenter', 'cellsize', 'kfun', 'knyq']:
+        assert np.allclose(getattr(a, name), getattr(b, name), rtol=0, atol=0)
+    assert dict(**a)['meshsize'].tolist() == [64] * 3
+    assert a.clone(meshsize=32).cellsize[0] == 1000. / 32
+    with pytest.raises(NotImplementedError):
+        MeshAttrs(meshsize=8, dtype='f8')
+    with pytest.raises(NotImplementedError):
+        MeshAttrs(meshsize=8, fft_backend='jaxdecomp')
+    # tests/test_mesh.py:24-48 KATs
+    assert np.all(get_mesh_attrs(boxsize=1000., boxcenter=0., cellsize=10.).meshsize == 100)
+    rng = np.random.default_rng(84)
+    pos = rng.uniform(0., 100., (100, 3))
+    attrs = get_mesh_attrs(pos, cellsize=12., primes=(2, 3, 5), divisors=(2, 3))
+    assert np.all(attrs.meshsize == 18) and np.allclose(attrs.cellsize, 12.)
+    ref = po.get_mesh_attrs(pos, meshsize=128)
+    attrs = get_mesh_attrs(pos, meshsize=128)
+    assert np.allclose(attrs.boxsize, ref.boxsize) and np.allclose(attrs.boxcenter, ref.boxcenter)
+    for n in range(1, 300):
+        assert next_fft_size(n) == po.next_fft_size(n)
+
+
+def test_regrid_matches_oracle():
+    from jaxpower_amd import MeshAttrs
+    from jaxpower_amd.mesh import get_halo_for_new_mattrs
+    from oracle import power_oracle as po
+    for L in (995., 1000., 1005., 1234.5):
+        a = MeshAttrs(meshsize=64, boxsize=L, boxcenter=500.)
+        new = get_halo_for_new_mattrs(a, cellsize=10.)[1]
+        ref = po.regrid_attrs(po.Attrs(64, L, 500.), 10.)
+        assert np.array_equal(new.meshsize, ref.meshsize) and np.allclose(new.boxsize, ref.boxsize)
+
+
+def test_k2_thresholds_reproduce_float64_digitize():
+    from jaxpower_amd.mesh2 import _k2_thresholds, _kvectors
+    from jaxpower_amd import MeshAttrs
+    from oracle import power_oracle as po
+    for meshsize, boxsize, step in [(64, 1000., 0.01), (128, 2000., 0.005), ((32, 16, 64), (300., 150., 500.), 0.013)]:
+        a = MeshAttrs(meshsize=meshsize, boxsize=boxsize)
+        o = po.Attrs(meshsize, boxsize)
+        edges = np.arange(0., a.kfun.min() * np.min(a.meshsize) / 2., step)
+        thr = _k2_thresholds(edges)
+        kx, ky, kz = _kvectors(a)
+        for got, want in zip((kx, ky, kz), o.kcoords('wavenumber')):
+            assert np.array_equal(got, want)
+        k2 = (kx[:, None, None]**2 + ky[None, :, None]**2) + kz[None, None, :]**2
+        ib = np.searchsorted(thr, k2.ravel(), side='right')
+        assert np.array_equal(ib, np.digitize(np.sqrt(k2).ravel(), edges, right=False))
+    # thresholds sit exactly at the float64 boundary
+    e = np.array([0., 0.355, 0.71, 1e-3])
+    t = _k2_thresholds(e)
+    for ee, tt in zip(e[1:], t[1:]):
+        assert np.sqrt(tt) >= ee and np.sqrt(np.nextafter(tt, 0.)) < ee
+
+
+def test_spectrum_container():
+    from jaxpower_amd import Mesh2SpectrumPole, Mesh2SpectrumPoles, read_observable
+    k = np.array([0.1, 0.2, 0.3])
+    poles = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=k, k_edges=np.column_stack([k - 0.05, k + 0.05]), nmodes=np.array([1, 2, 3]),
+                                                  num_raw=np.array([10., 20., 30.]) * (ell + 1), norm=2., ell=ell,
+                                                  attrs={'los': 'z'}) for ell in (0, 2)])
+    assert poles.ells == (0, 2)
+    poles = poles.clone(norm=[4., 4.], num_shotnoise=[np.full(3, 2.), np.zeros(3)])
+    assert np.allclose(poles.get(0).value(), (np.array([10., 20., 30.]) - 2.) / 4.)
+    assert np.allclose(poles.get(ells=2).value(), np.array([30., 60., 90.]) / 4.)
+    assert np.allclose(poles.get(0).values('shotnoise'), 0.5)
+    assert poles.get(0).values('nmodes').dtype.kind == 'i'
+    sel = poles.select(k=(0.15, 0.35))
+    assert np.allclose(sel.get(0).coords('k'), [0.2, 0.3])
+
+
+def test_spectrum_container_io(tmp_path):
+    from jaxpower_amd import Mesh2SpectrumPole, Mesh2SpectrumPoles, read_observable
+    k = np.array([0.1, 0.2])
+    poles = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=k, k_edges=np.column_stack([k - 0.05, k + 0.05]), nmodes=np.array([3, 4]),
+                                                  num_raw=np.array([1., 2.]), norm=1., ell=0, attrs={'los': 'x'})])
+    fn = tmp_path / 'p.npz'
+    poles.write(fn)
+    back = read_observable(fn)
+    assert back.ells == (0,) and np.allclose(back.get(0).value(), [1., 2.]) and back.attrs['los'] == 'x'
+
+
+def test_format_meshes():
+    from jaxpower_amd.mesh import format_meshes
+    m, f = format_meshes('a', nmeshes=2)
+    assert m == ['a', 'a'] and f == (0, 0)
+    m, f = format_meshes('a', 'b', nmeshes=2)
+    assert m == ['a', 'b'] and f == (0, 1)
