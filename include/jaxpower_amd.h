@@ -58,6 +58,10 @@ int mp_paint_spectrum(void* stream, const void* positions, int positions_f64, co
                       const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
                       int interlacing, int compensate, void* out, int64_t out_pitch, void* workspace,
                       size_t workspace_bytes);
+/* Paint path of mp_paint_spectrum: 0 automatic (fused paint + z-FFT in LDS when the shape
+ * allows), 1 tiled paint then z pass, 2 fused only, 3 per-particle atomics then z pass.
+ * Returns the previous setting.  No counterpart in the reference (an implementation choice). */
+int mp_set_paint_path(int path);
 
 /* resamplers.read via RealMeshField.read (jaxpower/resamplers.py:106-131,
  * jaxpower/mesh.py:2065-2115): out[i] = sum_stencil mesh * K. */

@@ -50,10 +50,12 @@ struct CombineTables {
 };
 
 int ensure_twiddles();
+const float2* twiddle_device_ptr();
 int64_t spectrum_pitch(int nz);
 int rfft3d(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro);
 // z and y passes only (the x pass is fused into a consumer).
 int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro);
+int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch);
 int combine_x(hipStream_t s, const float2* partial, int64_t mesh_stride, int ni, float2* out, const int32_t n[3],
               int64_t in_pitch, int64_t out_pitch, const CombineTables& tab, float scale);
 int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n[3], int64_t in_pitch, int64_t acc_pitch,

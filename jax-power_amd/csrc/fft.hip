@@ -17,6 +17,7 @@
 //   x pass, which reads all n_i partial spectra and writes one combined spectrum.
 #include "common.h"
 #include "fft.h"
+#include "fft_core.h"
 
 #include <cmath>
 #include <mutex>
@@ -50,127 +51,6 @@ int ensure_twiddles() {
   MP_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_twiddle), host.data(), sizeof(float2) * TW_N));
   g_tw_ready[dev] = 1;
   return MP_OK;
-}
-
-// ------------------------------------------------------------------ device helpers
-
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-// multiply by -i (forward) or +i (inverse)
-template <bool INV>
-__device__ __forceinline__ float2 rot90(float2 a) {
-  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
-}
-
-template <bool INV>
-__device__ __forceinline__ void dft2(float2& a, float2& b) {
-  float2 t = a;
-  a = cadd(t, b);
-  b = csub(t, b);
-}
-
-template <bool INV>
-__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
-  float2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
-  float2 t2 = cadd(v1, v3), t3 = rot90<INV>(csub(v1, v3));
-  v0 = cadd(t0, t2);
-  v2 = csub(t0, t2);
-  v1 = cadd(t1, t3);
-  v3 = csub(t1, t3);
-}
-
-template <int R, bool INV>
-__device__ __forceinline__ void dft(float2* v) {
-  if constexpr (R == 2) {
-    dft2<INV>(v[0], v[1]);
-  } else if constexpr (R == 4) {
-    dft4<INV>(v[0], v[1], v[2], v[3]);
-  } else {  // R == 8: two radix-4 on even/odd, then W8 twiddles
-    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-    dft4<INV>(e0, e1, e2, e3);
-    dft4<INV>(o0, o1, o2, o3);
-    const float h = 0.70710678118654752f;
-    // W8^1 = h(1 -+ i), W8^2 = -+i, W8^3 = h(-1 -+ i)  (upper sign forward)
-    float2 w1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y)) : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
-    float2 w2 = rot90<INV>(o2);
-    float2 w3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y)) : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
-    v[0] = cadd(e0, o0);
-    v[4] = csub(e0, o0);
-    v[1] = cadd(e1, w1);
-    v[5] = csub(e1, w1);
-    v[2] = cadd(e2, w2);
-    v[6] = csub(e2, w2);
-    v[3] = cadd(e3, w3);
-    v[7] = csub(e3, w3);
-  }
-}
-
-constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-constexpr int first_radix(int L) { return ilog2(L) % 3 == 0 ? 8 : (ilog2(L) % 3 == 1 ? 2 : 4); }
-
-// One Stockham stage over C independent length-L transforms held in LDS; element (l, c)
-// lives at buf[l * LS + c * CS].  CFAST: consecutive threads take consecutive c.
-template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int R, int NS>
-__device__ __forceinline__ void fft_stage(float2* buf) {
-  constexpr int NB = C * (L / R);
-  constexpr int BPT = (NB + NT - 1) / NT;
-  constexpr int TWSTEP = TW_N / (NS * R);
-  float2 v[BPT][R];
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int q = 0; q < BPT; ++q) {
-    int b = tid + q * NT;
-    if (NB % NT == 0 || b < NB) {
-      int c = CFAST ? b % C : b / (L / R);
-      int jj = CFAST ? b / C : b % (L / R);
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = buf[(jj + r * (L / R)) * LS + c * CS];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < BPT; ++q) {
-    int b = tid + q * NT;
-    if (NB % NT == 0 || b < NB) {
-      int c = CFAST ? b % C : b / (L / R);
-      int jj = CFAST ? b / C : b % (L / R);
-      int k = jj % NS;
-      if constexpr (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          float2 w = g_twiddle[r * k * TWSTEP];
-          if (INV) w.y = -w.y;
-          v[q][r] = cmul(v[q][r], w);
-        }
-      }
-      dft<R, INV>(v[q]);
-      int out = (jj / NS) * NS * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) buf[(out + r * NS) * LS + c * CS] = v[q][r];
-    }
-  }
-  __syncthreads();
-}
-
-template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int NS>
-__device__ __forceinline__ void fft_stages(float2* buf) {
-  if constexpr (NS < L) {
-    constexpr int R = NS == 1 ? first_radix(L) : 8;
-    fft_stage<L, C, LS, CS, CFAST, NT, INV, R, NS>(buf);
-    fft_stages<L, C, LS, CS, CFAST, NT, INV, NS * R>(buf);
-  }
-}
-
-// Full in-place FFT of C columns in LDS (callers sync before; this syncs after).
-template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV>
-__device__ __forceinline__ void fft_lds(float2* buf) {
-  fft_stages<L, C, LS, CS, CFAST, NT, INV, 1>(buf);
 }
 
 // ------------------------------------------------------------------ launch geometry
@@ -223,26 +103,13 @@ zpass_r2c_kernel(const float* __restrict__ in, float2* __restrict__ out, int64_t
     lds[c * LP + l] = v;
   }
   __syncthreads();
-  fft_lds<M, C, 1, LP, false, NT, false>(lds);
+  fft_lds<M, C, 1, LP, false, NT, false>(lds, g_twiddle);
   constexpr int NK = M / 2 + 1;
   for (int idx = tid; idx < C * NK; idx += NT) {
     int c = idx / NK, k = idx % NK;
     int64_t row = row0 + c;
     if (row >= nrows) continue;
-    float2* o = out + row * out_pitch;
-    float2 zk = lds[c * LP + k];
-    if (k == 0) {
-      o[0] = make_float2(zk.x + zk.y, 0.f);
-      o[M] = make_float2(zk.x - zk.y, 0.f);
-      continue;
-    }
-    float2 zmk = cconj(lds[c * LP + (M - k)]);
-    float2 E = cscale(cadd(zk, zmk), 0.5f);
-    float2 D = cscale(csub(zk, zmk), 0.5f);
-    float2 O = make_float2(D.y, -D.x);                  // -i * D
-    float2 t = cmul(g_twiddle[k * (TW_N / (2 * M))], O);  // exp(-2 pi i k / Nz) * O
-    o[k] = cadd(E, t);
-    if (2 * k != M) o[M - k] = cconj(csub(E, t));
+    rfft_post(lds + c * LP, out + row * out_pitch, M, k, g_twiddle);
   }
 }
 
@@ -278,7 +145,7 @@ zpass_c2r_kernel(const float2* __restrict__ in, int64_t in_pitch, float* __restr
     if (k != 0 && 2 * k != M) lds[c * LP + (M - k)] = zmk;
   }
   __syncthreads();
-  fft_lds<M, C, 1, LP, false, NT, true>(lds);
+  fft_lds<M, C, 1, LP, false, NT, true>(lds, g_twiddle);
   float2* out2 = reinterpret_cast<float2*>(out);
   for (int idx = tid; idx < C * M; idx += NT) {
     int c = idx / M, l = idx % M;
@@ -306,7 +173,7 @@ col_fft_kernel(const float2* in, float2* out, int ncols, int64_t in_fstride, int
     lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<L, C, C, 1, true, NT, INV>(lds);
+  fft_lds<L, C, C, 1, true, NT, INV>(lds, g_twiddle);
   float2* dst = out + (int64_t)blockIdx.y * out_ostride + c0;
   for (int idx = tid; idx < C * L; idx += NT) {
     int c = idx % C, l = idx / C;
@@ -339,7 +206,7 @@ xpass_combine_kernel(const float2* __restrict__ in, int64_t in_mesh_stride, int 
       lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds<L, C, C, 1, true, NT, false>(lds);
+    fft_lds<L, C, C, 1, true, NT, false>(lds, g_twiddle);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       int idx = tid + e * NT;
@@ -388,7 +255,7 @@ xpass_ylm_acc_kernel(const float2* __restrict__ in, float2* __restrict__ acc, in
     lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<L, C, C, 1, true, NT, false>(lds);
+  fft_lds<L, C, C, 1, true, NT, false>(lds, g_twiddle);
   float2* dst = acc + (int64_t)y * acc_ostride + c0;
   for (int idx = tid; idx < C * L; idx += NT) {
     int c = idx % C, l = idx / C;
@@ -548,6 +415,16 @@ int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], i
   return col_fft(n[1], false, "fft_ypass", out, out, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
 }
 
+int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int ncols = n[2] / 2 + 1;
+  const int64_t nx = n[0], ny = n[1];
+  return col_fft(n[1], false, "fft_ypass", data, data, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
+}
+
 int combine_x(hipStream_t s, const float2* partial, int64_t mesh_stride, int ni, float2* out, const int32_t n[3],
               int64_t in_pitch, int64_t out_pitch, const CombineTables& tab, float scale) {
   const int ncols = n[2] / 2 + 1;
@@ -583,6 +460,12 @@ int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const
 }
 
 int64_t spectrum_pitch(int nz) { return (int64_t)align_up((size_t)(nz / 2 + 1), 16); }
+
+const float2* twiddle_device_ptr() {
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_twiddle)) != hipSuccess) return nullptr;
+  return (const float2*)p;
+}
 
 }  // namespace mp
 

@@ -1,0 +1,152 @@
+// Device building blocks of the FFT passes: complex helpers, radix-2/4/8 butterflies, an
+// in-LDS Stockham FFT over C independent columns, and the r2c post-processing step.
+// Twiddles come from a table tw[e] = exp(-2 pi i e / TW_N).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft.h"
+
+namespace mp {
+
+// ------------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot90(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+  float2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
+  float2 t2 = cadd(v1, v3), t3 = rot90<INV>(csub(v1, v3));
+  v0 = cadd(t0, t2);
+  v2 = csub(t0, t2);
+  v1 = cadd(t1, t3);
+  v3 = csub(t1, t3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v) {
+  if constexpr (R == 2) {
+    dft2<INV>(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+  } else {  // R == 8: two radix-4 on even/odd, then W8 twiddles
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    const float h = 0.70710678118654752f;
+    // W8^1 = h(1 -+ i), W8^2 = -+i, W8^3 = h(-1 -+ i)  (upper sign forward)
+    float2 w1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y)) : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+    float2 w2 = rot90<INV>(o2);
+    float2 w3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y)) : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1);
+    v[5] = csub(e1, w1);
+    v[2] = cadd(e2, w2);
+    v[6] = csub(e2, w2);
+    v[3] = cadd(e3, w3);
+    v[7] = csub(e3, w3);
+  }
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int first_radix(int L) { return ilog2(L) % 3 == 0 ? 8 : (ilog2(L) % 3 == 1 ? 2 : 4); }
+
+// One Stockham stage over C independent length-L transforms held in LDS; element (l, c)
+// lives at buf[l * LS + c * CS].  CFAST: consecutive threads take consecutive c.
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int R, int NS>
+__device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict__ tw) {
+  constexpr int NB = C * (L / R);
+  constexpr int BPT = (NB + NT - 1) / NT;
+  constexpr int TWSTEP = TW_N / (NS * R);
+  float2 v[BPT][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      int c = CFAST ? b % C : b / (L / R);
+      int jj = CFAST ? b / C : b % (L / R);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[(jj + r * (L / R)) * LS + c * CS];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      int c = CFAST ? b % C : b / (L / R);
+      int jj = CFAST ? b / C : b % (L / R);
+      int k = jj % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * TWSTEP];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft<R, INV>(v[q]);
+      int out = (jj / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(out + r * NS) * LS + c * CS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int NS>
+__device__ __forceinline__ void fft_stages(float2* buf, const float2* __restrict__ tw) {
+  if constexpr (NS < L) {
+    constexpr int R = NS == 1 ? first_radix(L) : 8;
+    fft_stage<L, C, LS, CS, CFAST, NT, INV, R, NS>(buf, tw);
+    fft_stages<L, C, LS, CS, CFAST, NT, INV, NS * R>(buf, tw);
+  }
+}
+
+// Full in-place FFT of C columns in LDS (callers sync before; this syncs after).
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
+  fft_stages<L, C, LS, CS, CFAST, NT, INV, 1>(buf, tw);
+}
+
+// Post-processing of the r2c z pass for output index k in [0, M/2] of one row: the row holds
+// Z = FFT_M(x[2m] + i x[2m+1]); writes X[k] and X[M-k] of the N = 2M point real transform.
+__device__ __forceinline__ void rfft_post(const float2* row, float2* o, int M, int k, const float2* __restrict__ tw) {
+  float2 zk = row[k];
+  if (k == 0) {
+    o[0] = make_float2(zk.x + zk.y, 0.f);
+    o[M] = make_float2(zk.x - zk.y, 0.f);
+    return;
+  }
+  float2 zmk = cconj(row[M - k]);
+  float2 E = cscale(cadd(zk, zmk), 0.5f);
+  float2 D = cscale(csub(zk, zmk), 0.5f);
+  float2 O = make_float2(D.y, -D.x);                // -i * D
+  float2 t = cmul(tw[k * (TW_N / (2 * M))], O);    // exp(-2 pi i k / (2M)) * O
+  o[k] = cadd(E, t);
+  if (2 * k != M) o[M - k] = cconj(csub(E, t));
+}
+
+
+}  // namespace mp

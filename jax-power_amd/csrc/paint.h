@@ -18,4 +18,14 @@ size_t paint_workspace_bytes(const int32_t n[3], int64_t np, int order, int ni);
 int paint_meshes(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order,
                  int ni, float* meshes, int64_t mesh_stride, void* ws, size_t ws_bytes, int force_atomic);
 
+// Fused paint + z-axis r2c (paint_zfft.hip): ni half-spectrum meshes after the z pass.
+bool paint_zfft_supported(const int32_t n[3], int64_t np, int order, int ni);
+size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np);
+int paint_zfft(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
+               float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes);
+
+// Path used by the interlaced spectrum paint: 0 automatic (fused when supported), 1 separate
+// paint + z pass, 2 fused only (error when unsupported), 3 per-particle atomics + z pass.
+int paint_path_option();
+
 }  // namespace mp

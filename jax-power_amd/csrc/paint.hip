@@ -76,7 +76,7 @@ int fill_f32(hipStream_t s, float* p, int64_t n, float v) {
   return MP_OK;
 }
 
-static int fill_i32(hipStream_t s, int* p, int64_t n, int v) {
+int fill_i32_dev(hipStream_t s, int* p, int64_t n, int v) {
   if (n <= 0) return MP_OK;
   int64_t blocks = ceil_div(n, 256);
   if (blocks > 65536) blocks = 65536;
@@ -344,6 +344,8 @@ __global__ void scan_add_kernel(int* __restrict__ data, int64_t n, const int* __
   if (copy) copy[i] = v;
 }
 
+size_t scan_workspace_bytes(int64_t n);
+
 static size_t scan_workspace_ints(int64_t n) {
   size_t tot = 0;
   while (n > SCAN_BLOCK) {
@@ -353,8 +355,10 @@ static size_t scan_workspace_ints(int64_t n) {
   return tot + 64;
 }
 
+size_t scan_workspace_bytes(int64_t n) { return align_up(scan_workspace_ints(n) * 4, 256); }
+
 // In-place exclusive scan of data[0..n); copy (optional) receives the result too.
-static int exclusive_scan(hipStream_t s, int* data, int64_t n, int* tmp, int* copy) {
+int exclusive_scan_i32(hipStream_t s, int* data, int64_t n, int* tmp, int* copy) {
   int64_t nblocks = ceil_div(n, SCAN_BLOCK);
   if (nblocks <= 1) {
     MP_LAUNCH("scan_block", scan_block_kernel, dim3(1), dim3(SCAN_NT), 0, s, data, n, (int*)nullptr);
@@ -363,7 +367,7 @@ static int exclusive_scan(hipStream_t s, int* data, int64_t n, int* tmp, int* co
   }
   int* sums = tmp;
   MP_LAUNCH("scan_block", scan_block_kernel, dim3((unsigned)nblocks), dim3(SCAN_NT), 0, s, data, n, sums);
-  int rc = exclusive_scan(s, sums, nblocks, tmp + align_up((size_t)nblocks, 64), nullptr);
+  int rc = exclusive_scan_i32(s, sums, nblocks, tmp + align_up((size_t)nblocks, 64), nullptr);
   if (rc) return rc;
   MP_LAUNCH("scan_add", scan_add_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, data, n, (const int*)sums, copy);
   return MP_OK;
@@ -477,12 +481,12 @@ static int paint_typed(hipStream_t s, const T* pos, const float* w, int64_t np, 
   float4* sorted = wsp.take<float4>(np > 0 ? np : 1);
   if (!offsets || !cursor || !scan_tmp || !sorted)
     return set_error(MP_ERR_WORKSPACE, "paint: workspace of %zu bytes < %zu needed", ws_bytes, paint_workspace_bytes(g.n, np, order, ni));
-  int rc = fill_i32(s, offsets, nbt + 1, 0);
+  int rc = fill_i32_dev(s, offsets, nbt + 1, 0);
   if (rc) return rc;
   if (np > 0) {
     MP_LAUNCH("paint_brick_count", brick_count_kernel<T>, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, s, pos, np, g, tg, offsets);
   }
-  rc = exclusive_scan(s, offsets, nbt + 1, scan_tmp, cursor);
+  rc = exclusive_scan_i32(s, offsets, nbt + 1, scan_tmp, cursor);
   if (rc) return rc;
   if (np > 0) {
     MP_LAUNCH("paint_brick_scatter", brick_scatter_kernel<T>, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, s, pos, w, np, g, tg,

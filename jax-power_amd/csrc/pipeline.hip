@@ -45,7 +45,22 @@ __global__ void combine_tables_kernel(int n0, int n1, int n2, int ni, int order,
   }
 }
 
+static int g_paint_path = 1;  // fused path opt-in until validated on the device
+int paint_path_option() { return g_paint_path; }
+
+static bool use_fused(const int32_t n[3], int64_t np, int order, int ni) {
+  int opt = paint_path_option();
+  if (opt == 1 || opt == 3) return false;
+  return paint_zfft_supported(n, np, order, ni);
+}
+
 size_t paint_spectrum_workspace_bytes(const int32_t n[3], int64_t np, int order, int ni) {
+  const int64_t nspec_ = (int64_t)n[0] * n[1] * spectrum_pitch(n[2]);
+  if (use_fused(n, np, order, ni)) {
+    const int64_t len = n[0] + n[1] + n[2] / 2 + 1;
+    size_t bytes = align_up((size_t)ni * nspec_ * 8, 256) + align_up((size_t)(ni + 1) * len * 8, 256) + align_up((size_t)len * 4, 256);
+    return bytes + paint_zfft_workspace_bytes(n, np) + 1024;
+  }
   const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
   const int64_t nspec = (int64_t)n[0] * n[1] * spectrum_pitch(n[2]);
   const int64_t len = n[0] + n[1] + n[2] / 2 + 1;
@@ -66,22 +81,39 @@ int paint_spectrum(hipStream_t s, const void* pos, int pos_f64, const float* w, 
   const int64_t P = spectrum_pitch(n[2]);
   const int64_t nspec = (int64_t)n[0] * n[1] * P;
   const int len = n[0] + n[1] + n[2] / 2 + 1;
+  const bool fused = use_fused(n, np, order, ni);
+  if (paint_path_option() == 2 && !fused) return set_error(MP_ERR_UNSUPPORTED, "paint_spectrum: fused path not available");
   Workspace wsp(ws, ws_bytes);
-  float* meshes = wsp.take<float>((size_t)ni * ncell);
   float2* partial = wsp.take<float2>((size_t)ni * nspec);
   float2* ph = wsp.take<float2>((size_t)(ni + 1) * len);
   float* comp = wsp.take<float>((size_t)len);
-  size_t pbytes = paint_workspace_bytes(n, np, order, ni);
-  void* pws = wsp.take<char>(pbytes);
-  if (!meshes || !partial || !ph || !comp || !pws)
-    return set_error(MP_ERR_WORKSPACE, "paint_spectrum: workspace of %zu bytes < %zu needed", ws_bytes,
-                     paint_spectrum_workspace_bytes(n, np, order, ni));
-  int rc = paint_meshes(s, pos, pos_f64, w, np, g, order, ni, meshes, ncell, pws, pbytes, 0);
-  if (rc) return rc;
-  RealPrologue pro{};
-  for (int j = 0; j < ni; ++j) {
-    rc = rfft3d_zy(s, meshes + j * ncell, partial + j * nspec, n, P, pro);
+  int rc = MP_OK;
+  if (fused) {
+    size_t pbytes = paint_zfft_workspace_bytes(n, np);
+    void* pws = wsp.take<char>(pbytes);
+    if (!partial || !ph || !comp || !pws)
+      return set_error(MP_ERR_WORKSPACE, "paint_spectrum: workspace of %zu bytes < %zu needed", ws_bytes,
+                       paint_spectrum_workspace_bytes(n, np, order, ni));
+    rc = paint_zfft(s, pos, pos_f64, w, np, g, order, ni, partial, P, nspec, pws, pbytes);
     if (rc) return rc;
+    for (int j = 0; j < ni; ++j) {
+      rc = ypass(s, partial + j * nspec, n, P);
+      if (rc) return rc;
+    }
+  } else {
+    float* meshes = wsp.take<float>((size_t)ni * ncell);
+    size_t pbytes = paint_workspace_bytes(n, np, order, ni);
+    void* pws = wsp.take<char>(pbytes);
+    if (!meshes || !partial || !ph || !comp || !pws)
+      return set_error(MP_ERR_WORKSPACE, "paint_spectrum: workspace of %zu bytes < %zu needed", ws_bytes,
+                       paint_spectrum_workspace_bytes(n, np, order, ni));
+    rc = paint_meshes(s, pos, pos_f64, w, np, g, order, ni, meshes, ncell, pws, pbytes, paint_path_option() == 3);
+    if (rc) return rc;
+    RealPrologue pro{};
+    for (int j = 0; j < ni; ++j) {
+      rc = rfft3d_zy(s, meshes + j * ncell, partial + j * nspec, n, P, pro);
+      if (rc) return rc;
+    }
   }
   const int total = (ni + 1) * len;
   MP_LAUNCH("combine_tables", combine_tables_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, n[0], n[1], n[2], ni,
@@ -225,6 +257,14 @@ MP_EXPORT int mp_sum_f64(void* stream, const float* a, const float* b, int64_t n
   MP_LAUNCH("sum_partial", dot_partial_kernel, dim3(blocks), dim3(RED_NT), 0, s, a, b, n, square, partial);
   MP_LAUNCH("sum_final", final_sum_kernel, dim3(1), dim3(RED_NT), 0, s, (const double*)partial, blocks, out);
   return MP_OK;
+}
+
+// Selects the paint path of mp_paint_spectrum (0 automatic, 1 separate paint + z pass,
+// 2 fused paint + z pass only, 3 per-particle atomics + z pass); returns the previous value.
+MP_EXPORT int mp_set_paint_path(int path) {
+  int old = g_paint_path;
+  g_paint_path = (path >= 0 && path <= 3) ? path : 0;
+  return old;
 }
 
 MP_EXPORT int mp_fill_f32(void* stream, float* p, int64_t n, float value) { return fill_f32((hipStream_t)stream, p, n, value); }

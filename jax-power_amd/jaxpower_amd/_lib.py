@@ -48,6 +48,7 @@ SIGNATURES = {
     'mp_reduce_workspace_bytes': (c_int64, []),
     'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),
+    'mp_set_paint_path': (c_int, [c_int]),
 }
 
 
