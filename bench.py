@@ -33,6 +33,8 @@ def parse():
     p.add_argument('--warmup', type=int, default=3)
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--seed', type=int, default=42)
+    p.add_argument('--paint-path', type=int, default=None,
+                   help='0 automatic, 1 tiled paint + z pass, 2 fused paint + z-FFT, 3 atomics + z pass')
     return p.parse_args()
 
 
@@ -44,6 +46,10 @@ def algorithmic_bytes(n=MESHSIZE, np_=NPART, ni=NI, nprod=len(ELLS)):
         'paint_brick_count': np_ * 12,                     # float32 positions
         'paint_brick_scatter': np_ * 16 + np_ * 16,        # positions + weights in, sorted records out
         'paint_tile': np_ * 16 + ni * ncell * 4,           # sorted records in, n_i meshes out
+        'paint_rowsort_hist': np_ * 12,                    # float32 positions
+        'paint_rowsort_scatter': np_ * 16 + np_ * 18,      # positions + weights in, records + y keys out
+        'paint_rowsort_fine': np_ * 18 + np_ * 16,         # records + keys in, sorted records out
+        'paint_zfft': np_ * 16 + ni * H * 8,               # sorted records in, n_i z-transformed spectra out
         'fft_zpass_r2c': ncell * 4 + H * 8,
         'fft_ypass': 2 * H * 8,
         'fft_xpass_combine': ni * H * 8 + H * 8,
@@ -91,6 +97,8 @@ def main():
                               compute_fkp2_shotnoise, generate_lognormal_particles, _lib)
 
     torch.cuda.set_device(0)
+    if args.paint_path is not None:
+        _lib.load().mp_set_paint_path(args.paint_path)
     attrs = MeshAttrs(meshsize=MESHSIZE, boxsize=BOXSIZE, boxcenter=BOXSIZE / 2.)
     particles = generate_lognormal_particles(attrs, size=NPART, seed=args.seed)
     bin = BinMesh2SpectrumPoles(attrs, edges={'step': STEP}, ells=ELLS)
@@ -153,7 +161,8 @@ def main():
         'data': 'synthetic: seeded lognormal Poisson mock, P_G(k)=2e4 (k/0.1)^-1.5 exp(-(k/0.5)^2)',
         'config': {'workload': 'plain periodic box, one mock per step', 'meshsize': MESHSIZE, 'boxsize': BOXSIZE,
                    'particles': particles.size, 'resampler': 'tsc', 'interlacing': NI, 'compensate': True, 'los': 'z',
-                   'ells': list(ELLS), 'kstep': STEP, 'nbins': bin.nbins, 'path': 'paint(out=complex) -> compute_mesh2_spectrum'},
+                   'ells': list(ELLS), 'kstep': STEP, 'nbins': bin.nbins, 'path': 'paint(out=complex) -> compute_mesh2_spectrum',
+                   'paint_path': _lib.load().mp_set_paint_path(-1)},
         'roofline': {'kernel': dominant, 'bound': 'hbm', 'achieved': d['achieved_GBs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(d['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': None,
                      'algorithmic_bytes_per_launch': alg[dominant], 'avg_ms': d['avg_ms']},

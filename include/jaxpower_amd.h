@@ -60,7 +60,8 @@ int mp_paint_spectrum(void* stream, const void* positions, int positions_f64, co
                       size_t workspace_bytes);
 /* Paint path of mp_paint_spectrum: 0 automatic (fused paint + z-FFT in LDS when the shape
  * allows), 1 tiled paint then z pass, 2 fused only, 3 per-particle atomics then z pass.
- * Returns the previous setting.  No counterpart in the reference (an implementation choice). */
+ * Returns the previous setting; a negative path only queries.  No counterpart in the
+ * reference (an implementation choice). */
 int mp_set_paint_path(int path);
 
 /* resamplers.read via RealMeshField.read (jaxpower/resamplers.py:106-131,

@@ -65,17 +65,18 @@ __device__ __forceinline__ int first_bin(const double* thr, int nedges, double k
 }
 
 // Per-block histogram of Sigma w * product * L_ell, two doubles (re, im) per (product, bin).
-template <int NP>
+template <int NP, bool USE_LDS>
 __global__ void __launch_bounds__(BIN_NT)
-bin_poles_kernel(BinGeom g, BinProducts P, double* __restrict__ partial, int use_lds, double* __restrict__ out_atomic) {
+bin_poles_kernel(BinGeom g, BinProducts P, double* __restrict__ partial, double* __restrict__ out_atomic) {
   extern __shared__ double hist[];
   const int nbins = g.nedges - 1;
   const int hsize = NP * nbins * 2;
-  if (use_lds) {
+  if constexpr (USE_LDS) {
     for (int i = threadIdx.x; i < hsize; i += BIN_NT) hist[i] = 0.;
     __syncthreads();
   }
-  double* H = use_lds ? hist : out_atomic;
+  // the histogram lives in LDS (ds_add_f64) or, for very many bins, in global memory
+  double* H = USE_LDS ? hist : out_atomic;
   const int nseg = (g.ncols + BIN_SEG - 1) / BIN_SEG;
   const int64_t nwork = (int64_t)g.n0 * g.n1 * nseg;
   for (int64_t q = (int64_t)blockIdx.x * BIN_NT + threadIdx.x; q < nwork; q += (int64_t)gridDim.x * BIN_NT) {
@@ -141,20 +142,28 @@ bin_poles_kernel(BinGeom g, BinProducts P, double* __restrict__ partial, int use
       }
     }
   }
-  if (use_lds) {
+  if constexpr (USE_LDS) {
     __syncthreads();
     double* dst = partial + (int64_t)blockIdx.x * hsize;
     for (int i = threadIdx.x; i < hsize; i += BIN_NT) dst[i] = hist[i];
   }
 }
 
-// out[e] = sum_b partial[b][e] in block order (deterministic).
-__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int size, double* __restrict__ out) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= size) return;
+// out[e] = sum_b partial[b][e]: one block per entry e, fixed summation order (deterministic).
+constexpr int RED_NT = 256;
+__global__ void __launch_bounds__(RED_NT)
+reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int size, double* __restrict__ out) {
+  __shared__ double sh[RED_NT];
+  const int e = blockIdx.x;
   double s = 0.;
-  for (int b = 0; b < nblocks; ++b) s += partial[(int64_t)b * size + e];
-  out[e] = s;
+  for (int b = threadIdx.x; b < nblocks; b += RED_NT) s += partial[(int64_t)b * size + e];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = RED_NT / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[e] = sh[0];
 }
 
 __global__ void fill_f64_kernel(double* p, int64_t n) {
@@ -223,13 +232,13 @@ static int launch_poles(hipStream_t s, const BinGeom& g, const BinProducts& P, d
     Workspace w(ws, ws_bytes);
     double* partial = w.take<double>((size_t)blocks * hsize);
     if (!partial) return set_error(MP_ERR_WORKSPACE, "bin: workspace too small");
-    MP_LAUNCH("bin_poles", bin_poles_kernel<NP>, dim3(blocks), dim3(BIN_NT), hsize * sizeof(double), s, g, P, partial, 1,
+    MP_LAUNCH("bin_poles", (bin_poles_kernel<NP, true>), dim3(blocks), dim3(BIN_NT), hsize * sizeof(double), s, g, P, partial,
               (double*)nullptr);
-    MP_LAUNCH("bin_reduce", reduce_partials_kernel, dim3((unsigned)ceil_div(hsize, 256)), dim3(256), 0, s, (const double*)partial,
-              blocks, hsize, out);
+    MP_LAUNCH("bin_reduce", reduce_partials_kernel, dim3((unsigned)hsize), dim3(RED_NT), 0, s, (const double*)partial, blocks, hsize,
+              out);
   } else {
     MP_LAUNCH("bin_zero", fill_f64_kernel, dim3((unsigned)ceil_div(hsize, 256)), dim3(256), 0, s, out, (int64_t)hsize);
-    MP_LAUNCH("bin_poles", bin_poles_kernel<NP>, dim3(blocks), dim3(BIN_NT), 0, s, g, P, (double*)nullptr, 0, out);
+    MP_LAUNCH("bin_poles", (bin_poles_kernel<NP, false>), dim3(blocks), dim3(BIN_NT), 0, s, g, P, (double*)nullptr, out);
   }
   return MP_OK;
 }

@@ -42,6 +42,23 @@ __host__ __device__ __forceinline__ float kern1d(float s) {
   }
 }
 
+__device__ __forceinline__ int gcd_u32(unsigned a, unsigned b) {
+  while (b) {
+    unsigned t = a % b;
+    a = b;
+    b = t;
+  }
+  return (int)a;
+}
+
+// An odd stride near 0.618 * total that is coprime with total (1 for total <= 2).
+__device__ __forceinline__ int coprime_stride(int total) {
+  if (total <= 2) return 1;
+  unsigned s = ((unsigned)((double)total * 0.6180339887498949)) | 1u;
+  while (gcd_u32(s, (unsigned)total) != 1) s += 2;
+  return (int)(s % (unsigned)total);
+}
+
 __host__ __device__ __forceinline__ int pos_mod(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
@@ -173,7 +190,7 @@ paint_tile_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   constexpr int TCELLS = TILE * TILE * TILE;
   __shared__ float acc[NI][TCELLS];
   __shared__ int bstart[MAX_TILE_BRICKS];
-  __shared__ int bpre[MAX_TILE_BRICKS + 1];
+  __shared__ int bpre[MAX_TILE_BRICKS + 2];
   const int tid = threadIdx.x;
 
   int tile = blockIdx.x;
@@ -234,7 +251,13 @@ paint_tile_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __syncthreads();
 
   const int total = bpre[nbr];
-  for (int p = tid; p < total; p += TILE_NT) {
+  // strided visiting order (q * S mod total, S coprime with total): consecutive lanes take
+  // particles from different bricks, which keeps LDS atomic conflicts low for clustered data
+  if (tid == 0) bpre[MAX_TILE_BRICKS + 1] = coprime_stride(total);
+  __syncthreads();
+  const uint64_t S = (uint64_t)bpre[MAX_TILE_BRICKS + 1];
+  for (int q = tid; q < total; q += TILE_NT) {
+    const int p = (int)(((uint64_t)q * S) % (uint64_t)total);
     int lo_i = 0, hi_i = nbr - 1;  // largest i with bpre[i] <= p
     while (lo_i < hi_i) {
       int mid = (lo_i + hi_i + 1) >> 1;

@@ -48,6 +48,23 @@ __host__ __device__ __forceinline__ float kern1(float s) {
   }
 }
 
+__device__ __forceinline__ int gcd_u32(unsigned a, unsigned b) {
+  while (b) {
+    unsigned t = a % b;
+    a = b;
+    b = t;
+  }
+  return (int)a;
+}
+
+// An odd stride near 0.618 * total that is coprime with total (1 for total <= 2).
+__device__ __forceinline__ int coprime_stride(int total) {
+  if (total <= 2) return 1;
+  unsigned s = ((unsigned)((double)total * 0.6180339887498949)) | 1u;
+  while (gcd_u32(s, (unsigned)total) != 1) s += 2;
+  return (int)(s % (unsigned)total);
+}
+
 __device__ __forceinline__ int pmod(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
@@ -59,25 +76,24 @@ __device__ __forceinline__ void grid_coords(const T* __restrict__ pos, int64_t i
   for (int a = 0; a < 3; ++a) xg[a] = (pos[3 * i + a] + (T)g.shift[a]) / (T)g.cell[a];
 }
 
-template <typename T>
-__global__ void row_count_kernel(const T* __restrict__ pos, int64_t np, GridMap g, int* __restrict__ counts) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  T xg[3];
-  grid_coords(pos, i, g, xg);
-  int hx = pmod((int)floor(xg[0]), g.n[0]), hy = pmod((int)floor(xg[1]), g.n[1]);
-  atomicAdd(&counts[(int64_t)hx * g.n[1] + hy], 1);
-}
+// ------------------------------------------------------------------ two-level row sort
+//
+// Level A buckets particles by home x (atomics on an LDS histogram per block, one global
+// atomic per (block, bucket) to reserve space); level B sorts each x bucket by home y in one
+// workgroup and writes the (x, y) row offsets directly.  No per-particle global atomics.
+
+constexpr int RS_NT = 1024;
+constexpr int RS_CHUNK = 16384;
+constexpr int RS_MAX_KEYS = 2048;
 
 template <typename T>
-__global__ void row_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g,
-                                   int* __restrict__ cursor, float4* __restrict__ sorted) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
+__device__ __forceinline__ void row_record(const T* __restrict__ pos, const float* __restrict__ w, int64_t i, const GridMap& g,
+                                           int& hx, int& hy, float4& rec) {
   T xg[3];
   grid_coords(pos, i, g, xg);
   T fx = floor(xg[0]), fy = floor(xg[1]);
-  int hx = pmod((int)fx, g.n[0]), hy = pmod((int)fy, g.n[1]);
+  hx = pmod((int)fx, g.n[0]);
+  hy = pmod((int)fy, g.n[1]);
   float ox = fminf(fmaxf((float)(xg[0] - fx), 0.f), 1.f);
   float oy = fminf(fmaxf((float)(xg[1] - fy), 0.f), 1.f);
   // z wrapped into [0, Nz) as fixed point
@@ -85,8 +101,121 @@ __global__ void row_scatter_kernel(const T* __restrict__ pos, const float* __res
   double zs = (double)z * (double)(1u << ZFRAC);
   double zmax = (double)g.n[2] * (double)(1u << ZFRAC) - 1.;
   unsigned zbits = (unsigned)fmin(fmax(zs, 0.), zmax);
-  int slot = atomicAdd(&cursor[(int64_t)hx * g.n[1] + hy], 1);
-  sorted[slot] = make_float4(ox, oy, __uint_as_float(zbits), w ? w[i] : 1.f);
+  rec = make_float4(ox, oy, __uint_as_float(zbits), w ? w[i] : 1.f);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RS_NT)
+rowsort_hist_kernel(const T* __restrict__ pos, int64_t np, GridMap g, int* __restrict__ bucket_total) {
+  __shared__ int hist[RS_MAX_KEYS];
+  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * RS_CHUNK;
+  const int64_t i1 = i0 + RS_CHUNK < np ? i0 + RS_CHUNK : np;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += RS_NT) {
+    T xg[3];
+    grid_coords(pos, i, g, xg);
+    atomicAdd(&hist[pmod((int)floor(xg[0]), g.n[0])], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT)
+    if (hist[b]) atomicAdd(&bucket_total[b], hist[b]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RS_NT)
+rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int* __restrict__ bucket_cursor,
+                       float4* __restrict__ tmp_rec, short* __restrict__ tmp_key) {
+  __shared__ int hist[RS_MAX_KEYS];
+  __shared__ int base[RS_MAX_KEYS];
+  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * RS_CHUNK;
+  const int64_t i1 = i0 + RS_CHUNK < np ? i0 + RS_CHUNK : np;
+  constexpr int PER = RS_CHUNK / RS_NT;
+  int rank[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    int64_t i = i0 + threadIdx.x + (int64_t)q * RS_NT;
+    if (i < i1) {
+      T xg[3];
+      grid_coords(pos, i, g, xg);
+      rank[q] = atomicAdd(&hist[pmod((int)floor(xg[0]), g.n[0])], 1);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT)
+    if (hist[b]) base[b] = atomicAdd(&bucket_cursor[b], hist[b]);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    int64_t i = i0 + threadIdx.x + (int64_t)q * RS_NT;
+    if (i < i1) {
+      int hx, hy;
+      float4 rec;
+      row_record(pos, w, i, g, hx, hy, rec);
+      int64_t slot = (int64_t)base[hx] + rank[q];
+      tmp_rec[slot] = rec;
+      tmp_key[slot] = (short)hy;
+    }
+  }
+}
+
+__device__ __forceinline__ int wave_scan_incl(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d *= 2) {
+    int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of data[0..n) in LDS, n <= 2 * RS_NT; returns the total.
+__device__ int block_excl_scan(int* data, int n, int* wsum) {
+  const int tid = threadIdx.x;
+  int a = 2 * tid < n ? data[2 * tid] : 0;
+  int b = 2 * tid + 1 < n ? data[2 * tid + 1] : 0;
+  int local = a + b;
+  int incl = wave_scan_incl(local);
+  __syncthreads();
+  if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+  __syncthreads();
+  if (tid < 64) {
+    int x = tid < RS_NT / 64 ? wsum[tid] : 0;
+    int xi = wave_scan_incl(x);
+    if (tid < RS_NT / 64) wsum[tid] = xi - x;
+    if (tid == RS_NT / 64 - 1) wsum[RS_NT / 64] = xi;
+  }
+  __syncthreads();
+  int run = wsum[tid >> 6] + incl - local;
+  if (2 * tid < n) data[2 * tid] = run;
+  if (2 * tid + 1 < n) data[2 * tid + 1] = run + a;
+  __syncthreads();
+  return wsum[RS_NT / 64];
+}
+
+// One workgroup per x bucket: counting sort by y; writes row offsets and the final records.
+__global__ void __launch_bounds__(RS_NT)
+rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key,
+                    int ny, int64_t nkeys, float4* __restrict__ sorted, int* __restrict__ offsets) {
+  __shared__ int cnt[RS_MAX_KEYS];
+  __shared__ int wsum[RS_NT / 64 + 1];
+  const int hx = blockIdx.x;
+  const int start = bucket_start[hx], end = bucket_start[hx + 1];
+  for (int y = threadIdx.x; y < ny; y += RS_NT) cnt[y] = 0;
+  __syncthreads();
+  for (int i = start + threadIdx.x; i < end; i += RS_NT) atomicAdd(&cnt[tmp_key[i]], 1);
+  __syncthreads();
+  block_excl_scan(cnt, ny, wsum);
+  for (int y = threadIdx.x; y < ny; y += RS_NT) offsets[(int64_t)hx * ny + y] = start + cnt[y];
+  if (hx == 0 && threadIdx.x == 0) offsets[nkeys] = bucket_start[gridDim.x];
+  __syncthreads();
+  for (int i = start + threadIdx.x; i < end; i += RS_NT) {
+    int y = tmp_key[i];
+    int r = atomicAdd(&cnt[y], 1);
+    sorted[start + r] = tmp_rec[i];
+  }
 }
 
 // One workgroup: RX x RY rows of shifted mesh j, painted in LDS, then z-FFT'd (r2c) in LDS.
@@ -101,7 +230,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   extern __shared__ float2 lds2[];
   float* rows = reinterpret_cast<float*>(lds2);
   __shared__ int sstart[PZ_MAX_SEG];
-  __shared__ int spre[PZ_MAX_SEG + 1];
+  __shared__ int spre[PZ_MAX_SEG + 2];
   const int tid = threadIdx.x;
 
   // XCD-aware work index: blocks sharing an x strip (hence most of their particles) and the
@@ -159,7 +288,14 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __syncthreads();
 
   const int total = spre[nseg];
-  for (int p = tid; p < total; p += PZ_NT) {
+  // visit particles in a strided order (q * S mod total, S coprime with total) so that the
+  // lanes of a wavefront touch distant rows and z cells: fewer LDS atomic conflicts when
+  // particles cluster
+  if (tid == 0) spre[PZ_MAX_SEG] = coprime_stride(total);
+  __syncthreads();
+  const uint64_t S = (uint64_t)spre[PZ_MAX_SEG];
+  for (int q = tid; q < total; q += PZ_NT) {
+    const int p = (int)(((uint64_t)q * S) % (uint64_t)total);
     int lo_i = 0, hi_i = nseg - 1;
     while (lo_i < hi_i) {
       int mid = (lo_i + hi_i + 1) >> 1;
@@ -274,8 +410,11 @@ bool paint_zfft_supported(const int32_t n[3], int64_t np, int order, int ni) {
 
 size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np) {
   int64_t nkeys = (int64_t)n[0] * n[1];
-  size_t bytes = align_up((size_t)(nkeys + 1) * 4, 256) * 2 + scan_workspace_bytes(nkeys + 1);
-  bytes += align_up((size_t)(np > 0 ? np : 1) * 16, 256);
+  size_t npp = (size_t)(np > 0 ? np : 1);
+  size_t bytes = align_up((size_t)(nkeys + 1) * 4, 256);          // row offsets
+  bytes += 2 * align_up((size_t)(n[0] + 1) * 4, 256);             // bucket start / cursor
+  bytes += scan_workspace_bytes(n[0] + 1);
+  bytes += align_up(npp * 16, 256) * 2 + align_up(npp * 2, 256);  // tmp records, sorted records, tmp keys
   return bytes + 256;
 }
 
@@ -317,21 +456,28 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
                             float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes) {
   const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
   const int64_t nkeys = (int64_t)n[0] * n[1];
+  MP_REQUIRE(n[0] <= RS_MAX_KEYS && n[1] <= RS_MAX_KEYS, "paint_zfft: meshsize beyond %d", RS_MAX_KEYS);
   Workspace wsp(ws, ws_bytes);
   int* offsets = wsp.take<int>(nkeys + 1);
-  int* cursor = wsp.take<int>(nkeys + 1);
-  int* scan_tmp = (int*)wsp.take<char>(scan_workspace_bytes(nkeys + 1));
+  int* bstart = wsp.take<int>(n[0] + 1);
+  int* bcursor = wsp.take<int>(n[0] + 1);
+  int* scan_tmp = (int*)wsp.take<char>(scan_workspace_bytes(n[0] + 1));
+  float4* tmp_rec = wsp.take<float4>(np > 0 ? np : 1);
   float4* sorted = wsp.take<float4>(np > 0 ? np : 1);
-  if (!offsets || !cursor || !scan_tmp || !sorted)
+  short* tmp_key = wsp.take<short>(np > 0 ? np : 1);
+  if (!offsets || !bstart || !bcursor || !scan_tmp || !tmp_rec || !sorted || !tmp_key)
     return set_error(MP_ERR_WORKSPACE, "paint_zfft: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(n, np));
-  int rc = fill_i32_dev(s, offsets, nkeys + 1, 0);
+  int rc = fill_i32_dev(s, bstart, n[0] + 1, 0);
   if (rc) return rc;
-  if (np > 0) MP_LAUNCH("paint_row_count", row_count_kernel<T>, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, s, pos, np, g, offsets);
-  rc = exclusive_scan_i32(s, offsets, nkeys + 1, scan_tmp, cursor);
+  const unsigned nchunk = (unsigned)ceil_div(np, RS_CHUNK);
+  if (np > 0) MP_LAUNCH("paint_rowsort_hist", rowsort_hist_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, np, g, bstart);
+  rc = exclusive_scan_i32(s, bstart, n[0] + 1, scan_tmp, bcursor);
   if (rc) return rc;
   if (np > 0)
-    MP_LAUNCH("paint_row_scatter", row_scatter_kernel<T>, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, s, pos, w, np, g, cursor,
-              sorted);
+    MP_LAUNCH("paint_rowsort_scatter", rowsort_scatter_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, bcursor, tmp_rec,
+              tmp_key);
+  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), 0, s, (const int*)bstart,
+            (const float4*)tmp_rec, (const short*)tmp_key, n[1], nkeys, sorted, offsets);
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
   pz_rows(n[2] / 2, rg.rx, rg.ry);

@@ -260,10 +260,11 @@ MP_EXPORT int mp_sum_f64(void* stream, const float* a, const float* b, int64_t n
 }
 
 // Selects the paint path of mp_paint_spectrum (0 automatic, 1 separate paint + z pass,
-// 2 fused paint + z pass only, 3 per-particle atomics + z pass); returns the previous value.
+// 2 fused paint + z pass only, 3 per-particle atomics + z pass); returns the previous value,
+// and a negative `path` only returns the current one.
 MP_EXPORT int mp_set_paint_path(int path) {
   int old = g_paint_path;
-  g_paint_path = (path >= 0 && path <= 3) ? path : 0;
+  if (path >= 0) g_paint_path = path <= 3 ? path : 0;  // a negative value only queries
   return old;
 }
 
