@@ -94,8 +94,9 @@ __device__ __forceinline__ void row_record(const T* __restrict__ pos, const floa
   T fx = floor(xg[0]), fy = floor(xg[1]);
   hx = pmod((int)fx, g.n[0]);
   hy = pmod((int)fy, g.n[1]);
-  float ox = fminf(fmaxf((float)(xg[0] - fx), 0.f), 1.f);
-  float oy = fminf(fmaxf((float)(xg[1] - fy), 0.f), 1.f);
+  // offsets strictly below 1, so that round/floor(offset + shift) takes two values at most
+  float ox = fminf(fmaxf((float)(xg[0] - fx), 0.f), 0.99999994f);
+  float oy = fminf(fmaxf((float)(xg[1] - fy), 0.f), 0.99999994f);
   // z wrapped into [0, Nz) as fixed point
   T z = xg[2] - floor(xg[2] / (T)g.n[2]) * (T)g.n[2];
   double zs = (double)z * (double)(1u << ZFRAC);
@@ -195,11 +196,14 @@ __device__ int block_excl_scan(int* data, int n, int* wsum) {
   return wsum[RS_NT / 64];
 }
 
-// One workgroup per x bucket: counting sort by y; writes row offsets and the final records.
+// One workgroup per x bucket: counting sort by y (row offsets are written here), then each
+// wavefront sorts whole rows by z cell, so that particles of one cell end up contiguous.
+// tmp_rec[start, end) is free once the y scatter has read it, and receives the final order.
 __global__ void __launch_bounds__(RS_NT)
-rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key,
-                    int ny, int64_t nkeys, float4* __restrict__ sorted, int* __restrict__ offsets) {
-  __shared__ int cnt[RS_MAX_KEYS];
+rowsort_fine_kernel(const int* __restrict__ bucket_start, float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
+                    int nz, int64_t nkeys, float4* __restrict__ ysorted, int* __restrict__ offsets) {
+  extern __shared__ int zbins[];  // RS_NT / 64 waves x nz
+  __shared__ int cnt[RS_MAX_KEYS + 1];
   __shared__ int wsum[RS_NT / 64 + 1];
   const int hx = blockIdx.x;
   const int start = bucket_start[hx], end = bucket_start[hx + 1];
@@ -214,7 +218,80 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restri
   for (int i = start + threadIdx.x; i < end; i += RS_NT) {
     int y = tmp_key[i];
     int r = atomicAdd(&cnt[y], 1);
-    sorted[start + r] = tmp_rec[i];
+    ysorted[start + r] = tmp_rec[i];
+  }
+  __syncthreads();  // cnt[y] now holds the end of row y; ysorted is visible to the workgroup
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int* zb = zbins + wave * nz;
+  for (int y = wave; y < ny; y += RS_NT / 64) {
+    const int rs = start + (y ? cnt[y - 1] : 0), re = start + cnt[y];
+    const int n = re - rs;
+    if (n <= 1) {
+      if (n == 1 && lane == 0) tmp_rec[rs] = ysorted[rs];
+      continue;
+    }
+    for (int z = lane; z < nz; z += 64) zb[z] = 0;
+    for (int i = rs + lane; i < re; i += 64) atomicAdd(&zb[__float_as_uint(ysorted[i].z) >> ZFRAC], 1);
+    // exclusive scan of the nz bins: each lane owns nz/64 consecutive bins
+    const int per = (nz + 63) / 64;
+    int local = 0;
+    for (int q = 0; q < per; ++q)
+      if (lane * per + q < nz) local += zb[lane * per + q];
+    int incl = wave_scan_incl(local);
+    int run = incl - local;
+    for (int q = 0; q < per; ++q) {
+      if (lane * per + q >= nz) break;
+      int c = zb[lane * per + q];
+      zb[lane * per + q] = run;
+      run += c;
+    }
+    for (int i = rs + lane; i < re; i += 64) {
+      float4 rec = ysorted[i];
+      int r = atomicAdd(&zb[__float_as_uint(rec.z) >> ZFRAC], 1);
+      tmp_rec[rs + r] = rec;
+    }
+  }
+}
+
+// Weights of the W = ORDER + 1 window cells b + OMIN + t (t in [0, W)) for coordinate u
+// (offset + shift) whose round/floor is b or b + 1.
+template <int ORDER, int W>
+__device__ __forceinline__ void window_weights(float u, int b, float* wt) {
+  constexpr int OMIN = -((ORDER - 1) / 2);
+  const float r = (ORDER % 2) ? rintf(u) : floorf(u);
+  const bool up = (int)r != b;
+  float k[ORDER];
+#pragma unroll
+  for (int o = 0; o < ORDER; ++o) k[o] = kern1<ORDER>(fabsf(r + (float)(OMIN + o) - u));
+#pragma unroll
+  for (int t = 0; t < W; ++t) {
+    float lo = t < ORDER ? k[t] : 0.f;        // window starts at the stencil (no shift)
+    float hi = t >= 1 ? k[t - 1] : 0.f;       // stencil shifted up by one cell
+    wt[t] = up ? hi : lo;
+  }
+}
+
+// Adds a W^3 window of accumulated weights into the LDS rows (cells outside the block's rows
+// are skipped; z wraps periodically) and clears it.
+template <int W, int RX, int RY, int NZ>
+__device__ __forceinline__ void flush_window(float (&acc)[W][W][W], float* rows, int wx0, int wy0, int wz0) {
+  constexpr int RP = NZ + 2;  // row pitch in floats
+#pragma unroll
+  for (int a = 0; a < W; ++a) {
+    const int cx = wx0 + a;
+    const bool okx = cx >= 0 && cx < RX;
+#pragma unroll
+    for (int bb = 0; bb < W; ++bb) {
+      const int cy = wy0 + bb;
+      const bool ok = okx && cy >= 0 && cy < RY;
+      float* row = rows + (cx * RY + cy) * RP;
+#pragma unroll
+      for (int cc = 0; cc < W; ++cc) {
+        const float v = acc[a][bb][cc];
+        if (ok && v != 0.f) atomicAdd(row + ((wz0 + cc) & (NZ - 1)), v);
+        acc[a][bb][cc] = 0.f;
+      }
+    }
   }
 }
 
@@ -227,9 +304,12 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   constexpr int LPC = M + 1;       // complex row pitch in LDS
   constexpr int NROW = RX * RY;
   constexpr int OMIN = -((ORDER - 1) / 2);
+  constexpr int W = ORDER + 1;     // per-axis window covering both roundings of one home cell
+  constexpr int CHUNK = 16;
   extern __shared__ float2 lds2[];
   float* rows = reinterpret_cast<float*>(lds2);
   __shared__ int sstart[PZ_MAX_SEG];
+  __shared__ int scount[PZ_MAX_SEG];
   __shared__ int spre[PZ_MAX_SEG + 2];
   const int tid = threadIdx.x;
 
@@ -248,7 +328,9 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
 
   for (int i = tid; i < NROW * LPC; i += PZ_NT) lds2[i] = make_float2(0.f, 0.f);
 
-  // neighbourhood rows: homes in [x0 - hi, x0 + RX - 1 - lo]
+  // neighbourhood rows: homes in [x0 - hi, x0 + RX - 1 - lo]; each row's particles are
+  // contiguous and sorted by z cell.  Work items are chunks of up to CHUNK consecutive
+  // particles of one row.
   const int sx = RX + rg.hi - rg.lo, sy = RY + rg.hi - rg.lo;
   const int nseg = sx * sy;
   for (int i = tid; i < nseg; i += PZ_NT) {
@@ -256,8 +338,10 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     int hx = pmod(x0 - rg.hi + ix, rg.n[0]), hy = pmod(y0 - rg.hi + iy, rg.n[1]);
     int64_t key = (int64_t)hx * rg.n[1] + hy;
     int st = offsets[key];
+    int c = offsets[key + 1] - st;
     sstart[i] = st;
-    spre[i + 1] = offsets[key + 1] - st;
+    scount[i] = c;
+    spre[i + 1] = (c + CHUNK - 1) / CHUNK;
   }
   __syncthreads();
   if (tid < 64) {
@@ -286,66 +370,61 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     }
   }
   __syncthreads();
-
-  const int total = spre[nseg];
-  // visit particles in a strided order (q * S mod total, S coprime with total) so that the
-  // lanes of a wavefront touch distant rows and z cells: fewer LDS atomic conflicts when
-  // particles cluster
-  if (tid == 0) spre[PZ_MAX_SEG] = coprime_stride(total);
+  const int total = spre[nseg];  // number of chunks
+  // visit chunks in a strided order (q * S mod total, S coprime with total) so that the lanes
+  // of a wavefront work on distant rows: fewer LDS atomic conflicts
+  if (tid == 0) spre[PZ_MAX_SEG + 1] = coprime_stride(total);
   __syncthreads();
-  const uint64_t S = (uint64_t)spre[PZ_MAX_SEG];
+  const uint64_t S = (uint64_t)spre[PZ_MAX_SEG + 1];
+  // round/floor(offset + s) takes the values b and b + 1 only, offsets being in [0, 1)
+  const int b = (ORDER % 2) ? (int)rintf(s) : (int)floorf(s);
   for (int q = tid; q < total; q += PZ_NT) {
-    const int p = (int)(((uint64_t)q * S) % (uint64_t)total);
+    const int c = (int)(((uint64_t)q * S) % (uint64_t)total);
     int lo_i = 0, hi_i = nseg - 1;
     while (lo_i < hi_i) {
       int mid = (lo_i + hi_i + 1) >> 1;
-      if (spre[mid] <= p) lo_i = mid;
+      if (spre[mid] <= c) lo_i = mid;
       else hi_i = mid - 1;
     }
     const int seg = lo_i;
-    const float4 rec = sorted[sstart[seg] + (p - spre[seg])];
-    const int hxr = seg / sy - rg.hi;   // home row relative to x0 (unwrapped)
-    const int hyr = seg % sy - rg.hi;
-    const unsigned zb = __float_as_uint(rec.z);
-    const int zc = (int)(zb >> ZFRAC);
-    const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
-    int cx[ORDER], cy[ORDER], cz[ORDER];
-    float kx[ORDER], ky[ORDER], kz[ORDER];
-    {
-      float u = rec.x + s;
-      float r = (ORDER % 2) ? rintf(u) : floorf(u);
+    const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
+    const int p1 = min(p0 + CHUNK, sstart[seg] + scount[seg]);
+    // window origin of this row relative to the block: cells hxr + b + OMIN + [0, W)
+    const int wx0 = seg / sy - rg.hi + b + OMIN;
+    const int wy0 = seg % sy - rg.hi + b + OMIN;
+    float acc[W][W][W];
 #pragma unroll
-      for (int o = 0; o < ORDER; ++o) {
-        cx[o] = hxr + (int)r + OMIN + o;
-        kx[o] = (cx[o] >= 0 && cx[o] < RX) ? kern1<ORDER>(fabsf(r + (float)(OMIN + o) - u)) : 0.f;
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+      for (int bb = 0; bb < W; ++bb)
+#pragma unroll
+        for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = 0.f;
+    int zcur = -1;
+    for (int p = p0; p < p1; ++p) {
+      const float4 rec = sorted[p];
+      const unsigned zb = __float_as_uint(rec.z);
+      const int zc = (int)(zb >> ZFRAC);
+      if (zc != zcur) {
+        if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
+        zcur = zc;
       }
-      u = rec.y + s;
-      r = (ORDER % 2) ? rintf(u) : floorf(u);
+      const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
+      float wx[W], wy[W], wz[W];
+      window_weights<ORDER, W>(rec.x + s, b, wx);
+      window_weights<ORDER, W>(rec.y + s, b, wy);
+      window_weights<ORDER, W>(fz + s, b, wz);
 #pragma unroll
-      for (int o = 0; o < ORDER; ++o) {
-        cy[o] = hyr + (int)r + OMIN + o;
-        ky[o] = (cy[o] >= 0 && cy[o] < RY) ? kern1<ORDER>(fabsf(r + (float)(OMIN + o) - u)) : 0.f;
-      }
-      u = fz + s;
-      r = (ORDER % 2) ? rintf(u) : floorf(u);
+      for (int a = 0; a < W; ++a) {
+        const float wa = rec.w * wx[a];
 #pragma unroll
-      for (int o = 0; o < ORDER; ++o) {
-        cz[o] = (zc + (int)r + OMIN + o) & (NZ - 1);
-        kz[o] = kern1<ORDER>(fabsf(r + (float)(OMIN + o) - u));
-      }
-    }
+        for (int bb = 0; bb < W; ++bb) {
+          const float wab = wa * wy[bb];
 #pragma unroll
-    for (int ox = 0; ox < ORDER; ++ox) {
-      if (!(cx[ox] >= 0 && cx[ox] < RX)) continue;
-#pragma unroll
-      for (int oy = 0; oy < ORDER; ++oy) {
-        if (!(cy[oy] >= 0 && cy[oy] < RY)) continue;
-        float* row = rows + (cx[ox] * RY + cy[oy]) * (2 * LPC);
-        const float kxy = rec.w * kx[ox] * ky[oy];
-#pragma unroll
-        for (int oz = 0; oz < ORDER; ++oz) atomicAdd(row + cz[oz], kxy * kz[oz]);
+          for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = fmaf(wab, wz[cc], acc[a][bb][cc]);
+        }
       }
     }
+    if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
   }
   __syncthreads();
 
@@ -476,8 +555,12 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
   if (np > 0)
     MP_LAUNCH("paint_rowsort_scatter", rowsort_scatter_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, bcursor, tmp_rec,
               tmp_key);
-  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), 0, s, (const int*)bstart,
-            (const float4*)tmp_rec, (const short*)tmp_key, n[1], nkeys, sorted, offsets);
+  const size_t zlds = (size_t)(RS_NT / 64) * n[2] * sizeof(int);
+  if (zlds > 65536)
+    MP_CHECK_HIP(hipFuncSetAttribute((const void*)rowsort_fine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)zlds));
+  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), zlds, s, (const int*)bstart, tmp_rec,
+            (const short*)tmp_key, n[1], n[2], nkeys, sorted, offsets);
+  sorted = tmp_rec;  // the z-sorted records end up in tmp_rec
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
   pz_rows(n[2] / 2, rg.rx, rg.ry);
