@@ -271,6 +271,19 @@ __device__ __forceinline__ void window_weights(float u, int b, float* wt) {
   }
 }
 
+// Stencil cells base + round/floor(u) + OMIN + o and weights K(|cell - u|), o in [0, ORDER).
+template <int ORDER>
+__device__ __forceinline__ void stencil_1d(float u, int base, int* cell, float* k) {
+  constexpr int OMIN = -((ORDER - 1) / 2);
+  const float r = (ORDER % 2) ? rintf(u) : floorf(u);
+  const int i0 = base + (int)r + OMIN;
+#pragma unroll
+  for (int o = 0; o < ORDER; ++o) {
+    cell[o] = i0 + o;
+    k[o] = kern1<ORDER>(fabsf(r + (float)(OMIN + o) - u));
+  }
+}
+
 // Adds a W^3 window of accumulated weights into the LDS rows (cells outside the block's rows
 // are skipped; z wraps periodically) and clears it.
 template <int W, int RX, int RY, int NZ>
@@ -371,15 +384,9 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   }
   __syncthreads();
   const int total = spre[nseg];  // number of chunks
-  // visit chunks in a strided order (q * S mod total, S coprime with total) so that the lanes
-  // of a wavefront work on distant rows: fewer LDS atomic conflicts
-  if (tid == 0) spre[PZ_MAX_SEG + 1] = coprime_stride(total);
-  __syncthreads();
-  const uint64_t S = (uint64_t)spre[PZ_MAX_SEG + 1];
   // round/floor(offset + s) takes the values b and b + 1 only, offsets being in [0, 1)
   const int b = (ORDER % 2) ? (int)rintf(s) : (int)floorf(s);
-  for (int q = tid; q < total; q += PZ_NT) {
-    const int c = (int)(((uint64_t)q * S) % (uint64_t)total);
+  for (int c = tid; c < total; c += PZ_NT) {
     int lo_i = 0, hi_i = nseg - 1;
     while (lo_i < hi_i) {
       int mid = (lo_i + hi_i + 1) >> 1;
@@ -389,42 +396,74 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     const int seg = lo_i;
     const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
     const int p1 = min(p0 + CHUNK, sstart[seg] + scount[seg]);
-    // window origin of this row relative to the block: cells hxr + b + OMIN + [0, W)
-    const int wx0 = seg / sy - rg.hi + b + OMIN;
-    const int wy0 = seg % sy - rg.hi + b + OMIN;
-    float acc[W][W][W];
+    const int hxr = seg / sy - rg.hi, hyr = seg % sy - rg.hi;  // home row relative to the block
+    // a chunk whose particles all share one cell (rows are sorted by z cell) is "dense"; a
+    // wavefront with enough dense chunks accumulates per cell and flushes windows, otherwise
+    // every particle scatters its stencil directly (wave-uniform choice: no divergence)
+    bool dense = false;
+    if (p1 - p0 >= 4)
+      dense = (__float_as_uint(sorted[p0].z) >> ZFRAC) == (__float_as_uint(sorted[p1 - 1].z) >> ZFRAC);
+    const bool use_window = __popcll(__ballot(dense)) >= 8;
+    if (use_window && W * W * W <= 64) {
+      const int wx0 = hxr + b + OMIN, wy0 = hyr + b + OMIN;
+      float acc[W][W][W];
 #pragma unroll
-    for (int a = 0; a < W; ++a)
+      for (int a = 0; a < W; ++a)
 #pragma unroll
-      for (int bb = 0; bb < W; ++bb)
+        for (int bb = 0; bb < W; ++bb)
 #pragma unroll
-        for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = 0.f;
-    int zcur = -1;
-    for (int p = p0; p < p1; ++p) {
-      const float4 rec = sorted[p];
-      const unsigned zb = __float_as_uint(rec.z);
-      const int zc = (int)(zb >> ZFRAC);
-      if (zc != zcur) {
-        if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
-        zcur = zc;
+          for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = 0.f;
+      int zcur = -1;
+      for (int p = p0; p < p1; ++p) {
+        const float4 rec = sorted[p];
+        const unsigned zb = __float_as_uint(rec.z);
+        const int zc = (int)(zb >> ZFRAC);
+        if (zc != zcur) {
+          if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
+          zcur = zc;
+        }
+        const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
+        float wx[W], wy[W], wz[W];
+        window_weights<ORDER, W>(rec.x + s, b, wx);
+        window_weights<ORDER, W>(rec.y + s, b, wy);
+        window_weights<ORDER, W>(fz + s, b, wz);
+#pragma unroll
+        for (int a = 0; a < W; ++a) {
+          const float wa = rec.w * wx[a];
+#pragma unroll
+          for (int bb = 0; bb < W; ++bb) {
+            const float wab = wa * wy[bb];
+#pragma unroll
+            for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = fmaf(wab, wz[cc], acc[a][bb][cc]);
+          }
+        }
       }
-      const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
-      float wx[W], wy[W], wz[W];
-      window_weights<ORDER, W>(rec.x + s, b, wx);
-      window_weights<ORDER, W>(rec.y + s, b, wy);
-      window_weights<ORDER, W>(fz + s, b, wz);
+      if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
+    } else {
+      for (int p = p0; p < p1; ++p) {
+        const float4 rec = sorted[p];
+        const unsigned zb = __float_as_uint(rec.z);
+        const int zc = (int)(zb >> ZFRAC);
+        const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
+        int cx[ORDER], cy[ORDER], cz[ORDER];
+        float kx[ORDER], ky[ORDER], kz[ORDER];
+        stencil_1d<ORDER>(rec.x + s, hxr, cx, kx);
+        stencil_1d<ORDER>(rec.y + s, hyr, cy, ky);
+        stencil_1d<ORDER>(fz + s, zc, cz, kz);
 #pragma unroll
-      for (int a = 0; a < W; ++a) {
-        const float wa = rec.w * wx[a];
+        for (int ox = 0; ox < ORDER; ++ox) {
+          if (!(cx[ox] >= 0 && cx[ox] < RX)) continue;
 #pragma unroll
-        for (int bb = 0; bb < W; ++bb) {
-          const float wab = wa * wy[bb];
+          for (int oy = 0; oy < ORDER; ++oy) {
+            if (!(cy[oy] >= 0 && cy[oy] < RY)) continue;
+            float* row = rows + (cx[ox] * RY + cy[oy]) * (NZ + 2);
+            const float kxy = rec.w * kx[ox] * ky[oy];
 #pragma unroll
-          for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = fmaf(wab, wz[cc], acc[a][bb][cc]);
+            for (int oz = 0; oz < ORDER; ++oz) atomicAdd(row + (cz[oz] & (NZ - 1)), kxy * kz[oz]);
+          }
         }
       }
     }
-    if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
   }
   __syncthreads();
 
@@ -457,7 +496,7 @@ static void reach_of(int order, int ni, int& lo, int& hi) {
   int omin = -((order - 1) / 2), omax = order - 1 - (order - 1) / 2;
   int i0max = (order % 2) ? (ni > 1 ? 2 : 1) : (ni > 1 ? 1 : 0);
   lo = omin;
-  hi = i0max + omax + 1;  // one extra cell of margin for offsets rounded up to 1
+  hi = i0max + omax;  // offsets are stored strictly below 1, so no margin is needed
 }
 
 static bool pz_rows(int M, int& rx, int& ry) {

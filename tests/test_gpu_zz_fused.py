@@ -82,4 +82,4 @@ def test_fused_dense_clumps(fused_path, resampler):
     oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter)
     want = po.paint(oattrs, pos, w.astype('f8'), resampler=resampler, interlacing=3, compensate=True, out='complex')
     err = np.abs(got - want).max() / np.abs(want).max()
-    assert err < 3e-6, err
+    assert err < 1e-5, err  # many float32 weights summed per cell
