@@ -199,9 +199,15 @@ __device__ int block_excl_scan(int* data, int n, int* wsum) {
 // One workgroup per x bucket: counting sort by y (row offsets are written here), then each
 // wavefront sorts whole rows by z cell, so that particles of one cell end up contiguous.
 // tmp_rec[start, end) is free once the y scatter has read it, and receives the final order.
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
 __global__ void __launch_bounds__(RS_NT)
 rowsort_fine_kernel(const int* __restrict__ bucket_start, float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
-                    int nz, int64_t nkeys, float4* __restrict__ ysorted, int* __restrict__ offsets) {
+                    int nz, int64_t nkeys, float4* __restrict__ ysorted, int* __restrict__ offsets, float* __restrict__ row_wabs) {
   extern __shared__ int zbins[];  // RS_NT / 64 waves x nz
   __shared__ int cnt[RS_MAX_KEYS + 1];
   __shared__ int wsum[RS_NT / 64 + 1];
@@ -227,9 +233,21 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, float4* __restrict__ t
     const int rs = start + (y ? cnt[y - 1] : 0), re = start + cnt[y];
     const int n = re - rs;
     if (n <= 1) {
-      if (n == 1 && lane == 0) tmp_rec[rs] = ysorted[rs];
+      if (lane == 0) {
+        float wa = 0.f;
+        if (n == 1) {
+          float4 rec = ysorted[rs];
+          tmp_rec[rs] = rec;
+          wa = fabsf(rec.w);
+        }
+        row_wabs[(int64_t)hx * ny + y] = wa;
+      }
       continue;
     }
+    float wa = 0.f;
+    for (int i = rs + lane; i < re; i += 64) wa += fabsf(ysorted[i].w);
+    wa = wave_sum_f32(wa);
+    if (lane == 0) row_wabs[(int64_t)hx * ny + y] = wa;
     for (int z = lane; z < nz; z += 64) zb[z] = 0;
     for (int i = rs + lane; i < re; i += 64) atomicAdd(&zb[__float_as_uint(ysorted[i].z) >> ZFRAC], 1);
     // exclusive scan of the nz bins: each lane owns nz/64 consecutive bins
@@ -287,7 +305,7 @@ __device__ __forceinline__ void stencil_1d(float u, int base, int* cell, float* 
 // Adds a W^3 window of accumulated weights into the LDS rows (cells outside the block's rows
 // are skipped; z wraps periodically) and clears it.
 template <int W, int RX, int RY, int NZ>
-__device__ __forceinline__ void flush_window(float (&acc)[W][W][W], float* rows, int wx0, int wy0, int wz0) {
+__device__ __forceinline__ void flush_window(float (&acc)[W][W][W], int* rows, float scale, int wx0, int wy0, int wz0) {
   constexpr int RP = NZ + 2;  // row pitch in floats
 #pragma unroll
   for (int a = 0; a < W; ++a) {
@@ -297,11 +315,11 @@ __device__ __forceinline__ void flush_window(float (&acc)[W][W][W], float* rows,
     for (int bb = 0; bb < W; ++bb) {
       const int cy = wy0 + bb;
       const bool ok = okx && cy >= 0 && cy < RY;
-      float* row = rows + (cx * RY + cy) * RP;
+      int* row = rows + (cx * RY + cy) * RP;
 #pragma unroll
       for (int cc = 0; cc < W; ++cc) {
         const float v = acc[a][bb][cc];
-        if (ok && v != 0.f) atomicAdd(row + ((wz0 + cc) & (NZ - 1)), v);
+        if (ok && v != 0.f) atomicAdd(row + ((wz0 + cc) & (NZ - 1)), __float2int_rn(v * scale));
         acc[a][bb][cc] = 0.f;
       }
     }
@@ -311,8 +329,8 @@ __device__ __forceinline__ void flush_window(float (&acc)[W][W][W], float* rows,
 // One workgroup: RX x RY rows of shifted mesh j, painted in LDS, then z-FFT'd (r2c) in LDS.
 template <int ORDER, int M, int RX, int RY>
 __global__ void __launch_bounds__(PZ_NT)
-paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ offsets, RowGeom rg, float2* __restrict__ out,
-                  int64_t pitch, int64_t mesh_stride, const float2* __restrict__ tw) {
+paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ offsets, const float* __restrict__ row_wabs, RowGeom rg,
+                  float2* __restrict__ out, int64_t pitch, int64_t mesh_stride, const float2* __restrict__ tw) {
   constexpr int NZ = 2 * M;
   constexpr int LPC = M + 1;       // complex row pitch in LDS
   constexpr int NROW = RX * RY;
@@ -324,6 +342,8 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __shared__ int sstart[PZ_MAX_SEG];
   __shared__ int scount[PZ_MAX_SEG];
   __shared__ int spre[PZ_MAX_SEG + 2];
+  __shared__ float swabs[PZ_MAX_SEG];
+  __shared__ float sbound[1];
   const int tid = threadIdx.x;
 
   // XCD-aware work index: blocks sharing an x strip (hence most of their particles) and the
@@ -355,6 +375,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     sstart[i] = st;
     scount[i] = c;
     spre[i + 1] = (c + CHUNK - 1) / CHUNK;
+    swabs[i] = row_wabs[key];
   }
   __syncthreads();
   if (tid < 64) {
@@ -381,9 +402,21 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
       run += local[q];
       if (i < nseg) spre[i + 1] = run;
     }
+    // bound on |cell value| for the fixed-point accumulation: sum of |w| over the neighbourhood
+    float wa = 0.f;
+    for (int i = tid; i < nseg; i += 64) wa += swabs[i];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) wa += __shfl_xor(wa, d, 64);
+    if (tid == 0) sbound[0] = wa;
   }
   __syncthreads();
   const int total = spre[nseg];  // number of chunks
+  // Fixed-point accumulation: integer LDS atomics are exact, order independent (the painted
+  // rows are bitwise reproducible) and much faster than float LDS atomics on this target.
+  // |sum| <= bound * 2^30 / bound < 2^31, and each addend is rounded to 1/scale.
+  const float bound = sbound[0];
+  const float scale = bound > 0.f ? 1073741824.f / bound : 1.f;
+  int* irows = reinterpret_cast<int*>(rows);
   // round/floor(offset + s) takes the values b and b + 1 only, offsets being in [0, 1)
   const int b = (ORDER % 2) ? (int)rintf(s) : (int)floorf(s);
   for (int c = tid; c < total; c += PZ_NT) {
@@ -419,7 +452,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
         const unsigned zb = __float_as_uint(rec.z);
         const int zc = (int)(zb >> ZFRAC);
         if (zc != zcur) {
-          if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
+          if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, irows, scale, wx0, wy0, zcur + b + OMIN);
           zcur = zc;
         }
         const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
@@ -438,7 +471,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
           }
         }
       }
-      if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, rows, wx0, wy0, zcur + b + OMIN);
+      if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, irows, scale, wx0, wy0, zcur + b + OMIN);
     } else {
       for (int p = p0; p < p1; ++p) {
         const float4 rec = sorted[p];
@@ -456,14 +489,19 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
 #pragma unroll
           for (int oy = 0; oy < ORDER; ++oy) {
             if (!(cy[oy] >= 0 && cy[oy] < RY)) continue;
-            float* row = rows + (cx[ox] * RY + cy[oy]) * (NZ + 2);
-            const float kxy = rec.w * kx[ox] * ky[oy];
+            int* row = irows + (cx[ox] * RY + cy[oy]) * (NZ + 2);
+            const float kxy = rec.w * kx[ox] * ky[oy] * scale;
 #pragma unroll
-            for (int oz = 0; oz < ORDER; ++oz) atomicAdd(row + (cz[oz] & (NZ - 1)), kxy * kz[oz]);
+            for (int oz = 0; oz < ORDER; ++oz) atomicAdd(row + (cz[oz] & (NZ - 1)), __float2int_rn(kxy * kz[oz]));
           }
         }
       }
     }
+  }
+  __syncthreads();
+  {
+    const float inv = 1.f / scale;
+    for (int i = tid; i < NROW * 2 * LPC; i += PZ_NT) rows[i] = (float)irows[i] * inv;
   }
   __syncthreads();
 
@@ -533,12 +571,13 @@ size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np) {
   bytes += 2 * align_up((size_t)(n[0] + 1) * 4, 256);             // bucket start / cursor
   bytes += scan_workspace_bytes(n[0] + 1);
   bytes += align_up(npp * 16, 256) * 2 + align_up(npp * 2, 256);  // tmp records, sorted records, tmp keys
+  bytes += align_up((size_t)nkeys * 4, 256);                      // per-row sum of |w|
   return bytes + 256;
 }
 
 template <int ORDER, int M>
-static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, const RowGeom& rg, float2* out, int64_t pitch,
-                     int64_t mesh_stride) {
+static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs, const RowGeom& rg, float2* out,
+                     int64_t pitch, int64_t mesh_stride) {
   constexpr int RX = PZCfg<M>::RX, RY = PZCfg<M>::RY;
   constexpr size_t lds = (size_t)RX * RY * (M + 1) * sizeof(float2);
   static_assert(lds <= PZ_LDS_BUDGET, "LDS budget");
@@ -549,22 +588,22 @@ static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, co
   unsigned grid = (unsigned)(nq * rg.nby * rg.ni * 8);
   const float2* tw = twiddle_device_ptr();
   if (!tw) return set_error(MP_ERR_HIP, "paint_zfft: no twiddle table");
-  MP_LAUNCH("paint_zfft", kern, dim3(grid), dim3(PZ_NT), lds, s, sorted, offsets, rg, out, pitch, mesh_stride, tw);
+  MP_LAUNCH("paint_zfft", kern, dim3(grid), dim3(PZ_NT), lds, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride, tw);
   return MP_OK;
 }
 
 template <int ORDER>
-static int dispatch_pz(int M, hipStream_t s, const float4* sorted, const int* offsets, const RowGeom& rg, float2* out, int64_t pitch,
-                       int64_t mesh_stride) {
+static int dispatch_pz(int M, hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs, const RowGeom& rg,
+                       float2* out, int64_t pitch, int64_t mesh_stride) {
   switch (M) {
-    case 1024: return launch_pz<ORDER, 1024>(s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 512: return launch_pz<ORDER, 512>(s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 256: return launch_pz<ORDER, 256>(s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 128: return launch_pz<ORDER, 128>(s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 64: return launch_pz<ORDER, 64>(s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 32: return launch_pz<ORDER, 32>(s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 16: return launch_pz<ORDER, 16>(s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 8: return launch_pz<ORDER, 8>(s, sorted, offsets, rg, out, pitch, mesh_stride);
+    case 1024: return launch_pz<ORDER, 1024>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 512: return launch_pz<ORDER, 512>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 256: return launch_pz<ORDER, 256>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 128: return launch_pz<ORDER, 128>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 64: return launch_pz<ORDER, 64>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 32: return launch_pz<ORDER, 32>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 16: return launch_pz<ORDER, 16>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 8: return launch_pz<ORDER, 8>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
     default: return set_error(MP_ERR_UNSUPPORTED, "paint_zfft: Nz/2 = %d", M);
   }
 }
@@ -583,7 +622,8 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
   float4* tmp_rec = wsp.take<float4>(np > 0 ? np : 1);
   float4* sorted = wsp.take<float4>(np > 0 ? np : 1);
   short* tmp_key = wsp.take<short>(np > 0 ? np : 1);
-  if (!offsets || !bstart || !bcursor || !scan_tmp || !tmp_rec || !sorted || !tmp_key)
+  float* row_wabs = wsp.take<float>(nkeys);
+  if (!offsets || !bstart || !bcursor || !scan_tmp || !tmp_rec || !sorted || !tmp_key || !row_wabs)
     return set_error(MP_ERR_WORKSPACE, "paint_zfft: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(n, np));
   int rc = fill_i32_dev(s, bstart, n[0] + 1, 0);
   if (rc) return rc;
@@ -598,7 +638,7 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
   if (zlds > 65536)
     MP_CHECK_HIP(hipFuncSetAttribute((const void*)rowsort_fine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)zlds));
   MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), zlds, s, (const int*)bstart, tmp_rec,
-            (const short*)tmp_key, n[1], n[2], nkeys, sorted, offsets);
+            (const short*)tmp_key, n[1], n[2], nkeys, sorted, offsets, row_wabs);
   sorted = tmp_rec;  // the z-sorted records end up in tmp_rec
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
@@ -609,10 +649,10 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
   rg.ni = ni;
   const int M = n[2] / 2;
   switch (order) {
-    case 1: return dispatch_pz<1>(M, s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 2: return dispatch_pz<2>(M, s, sorted, offsets, rg, out, pitch, mesh_stride);
-    case 3: return dispatch_pz<3>(M, s, sorted, offsets, rg, out, pitch, mesh_stride);
-    default: return dispatch_pz<4>(M, s, sorted, offsets, rg, out, pitch, mesh_stride);
+    case 1: return dispatch_pz<1>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 2: return dispatch_pz<2>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 3: return dispatch_pz<3>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    default: return dispatch_pz<4>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
   }
 }
 
