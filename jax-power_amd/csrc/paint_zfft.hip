@@ -196,9 +196,9 @@ __device__ int block_excl_scan(int* data, int n, int* wsum) {
   return wsum[RS_NT / 64];
 }
 
-// One workgroup per x bucket: counting sort by y (row offsets are written here), then each
-// wavefront sorts whole rows by z cell, so that particles of one cell end up contiguous.
-// tmp_rec[start, end) is free once the y scatter has read it, and receives the final order.
+// One workgroup per x bucket: counting sort by y; writes the row offsets and the sum of |w|
+// of every row.  tmp_rec[start, end) is free once the y scatter has read it, and receives
+// the final order.
 __device__ __forceinline__ float wave_sum_f32(float v) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -208,7 +208,6 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
 __global__ void __launch_bounds__(RS_NT)
 rowsort_fine_kernel(const int* __restrict__ bucket_start, float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
                     int nz, int64_t nkeys, float4* __restrict__ ysorted, int* __restrict__ offsets, float* __restrict__ row_wabs) {
-  extern __shared__ int zbins[];  // RS_NT / 64 waves x nz
   __shared__ int cnt[RS_MAX_KEYS + 1];
   __shared__ int wsum[RS_NT / 64 + 1];
   const int hx = blockIdx.x;
@@ -227,65 +226,19 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, float4* __restrict__ t
     ysorted[start + r] = tmp_rec[i];
   }
   __syncthreads();  // cnt[y] now holds the end of row y; ysorted is visible to the workgroup
+  // per-row sum of |w| (bounds the fixed-point accumulation of the paint), and the y-sorted
+  // records copied back to tmp_rec, where the paint reads them
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int* zb = zbins + wave * nz;
   for (int y = wave; y < ny; y += RS_NT / 64) {
     const int rs = start + (y ? cnt[y - 1] : 0), re = start + cnt[y];
-    const int n = re - rs;
-    if (n <= 1) {
-      if (lane == 0) {
-        float wa = 0.f;
-        if (n == 1) {
-          float4 rec = ysorted[rs];
-          tmp_rec[rs] = rec;
-          wa = fabsf(rec.w);
-        }
-        row_wabs[(int64_t)hx * ny + y] = wa;
-      }
-      continue;
-    }
     float wa = 0.f;
-    for (int i = rs + lane; i < re; i += 64) wa += fabsf(ysorted[i].w);
-    wa = wave_sum_f32(wa);
-    if (lane == 0) row_wabs[(int64_t)hx * ny + y] = wa;
-    for (int z = lane; z < nz; z += 64) zb[z] = 0;
-    for (int i = rs + lane; i < re; i += 64) atomicAdd(&zb[__float_as_uint(ysorted[i].z) >> ZFRAC], 1);
-    // exclusive scan of the nz bins: each lane owns nz/64 consecutive bins
-    const int per = (nz + 63) / 64;
-    int local = 0;
-    for (int q = 0; q < per; ++q)
-      if (lane * per + q < nz) local += zb[lane * per + q];
-    int incl = wave_scan_incl(local);
-    int run = incl - local;
-    for (int q = 0; q < per; ++q) {
-      if (lane * per + q >= nz) break;
-      int c = zb[lane * per + q];
-      zb[lane * per + q] = run;
-      run += c;
-    }
     for (int i = rs + lane; i < re; i += 64) {
       float4 rec = ysorted[i];
-      int r = atomicAdd(&zb[__float_as_uint(rec.z) >> ZFRAC], 1);
-      tmp_rec[rs + r] = rec;
+      tmp_rec[i] = rec;
+      wa += fabsf(rec.w);
     }
-  }
-}
-
-// Weights of the W = ORDER + 1 window cells b + OMIN + t (t in [0, W)) for coordinate u
-// (offset + shift) whose round/floor is b or b + 1.
-template <int ORDER, int W>
-__device__ __forceinline__ void window_weights(float u, int b, float* wt) {
-  constexpr int OMIN = -((ORDER - 1) / 2);
-  const float r = (ORDER % 2) ? rintf(u) : floorf(u);
-  const bool up = (int)r != b;
-  float k[ORDER];
-#pragma unroll
-  for (int o = 0; o < ORDER; ++o) k[o] = kern1<ORDER>(fabsf(r + (float)(OMIN + o) - u));
-#pragma unroll
-  for (int t = 0; t < W; ++t) {
-    float lo = t < ORDER ? k[t] : 0.f;        // window starts at the stencil (no shift)
-    float hi = t >= 1 ? k[t - 1] : 0.f;       // stencil shifted up by one cell
-    wt[t] = up ? hi : lo;
+    wa = wave_sum_f32(wa);
+    if (lane == 0) row_wabs[(int64_t)hx * ny + y] = wa;
   }
 }
 
@@ -302,30 +255,6 @@ __device__ __forceinline__ void stencil_1d(float u, int base, int* cell, float* 
   }
 }
 
-// Adds a W^3 window of accumulated weights into the LDS rows (cells outside the block's rows
-// are skipped; z wraps periodically) and clears it.
-template <int W, int RX, int RY, int NZ>
-__device__ __forceinline__ void flush_window(float (&acc)[W][W][W], int* rows, float scale, int wx0, int wy0, int wz0) {
-  constexpr int RP = NZ + 2;  // row pitch in floats
-#pragma unroll
-  for (int a = 0; a < W; ++a) {
-    const int cx = wx0 + a;
-    const bool okx = cx >= 0 && cx < RX;
-#pragma unroll
-    for (int bb = 0; bb < W; ++bb) {
-      const int cy = wy0 + bb;
-      const bool ok = okx && cy >= 0 && cy < RY;
-      int* row = rows + (cx * RY + cy) * RP;
-#pragma unroll
-      for (int cc = 0; cc < W; ++cc) {
-        const float v = acc[a][bb][cc];
-        if (ok && v != 0.f) atomicAdd(row + ((wz0 + cc) & (NZ - 1)), __float2int_rn(v * scale));
-        acc[a][bb][cc] = 0.f;
-      }
-    }
-  }
-}
-
 // One workgroup: RX x RY rows of shifted mesh j, painted in LDS, then z-FFT'd (r2c) in LDS.
 template <int ORDER, int M, int RX, int RY>
 __global__ void __launch_bounds__(PZ_NT)
@@ -335,7 +264,6 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   constexpr int LPC = M + 1;       // complex row pitch in LDS
   constexpr int NROW = RX * RY;
   constexpr int OMIN = -((ORDER - 1) / 2);
-  constexpr int W = ORDER + 1;     // per-axis window covering both roundings of one home cell
   constexpr int CHUNK = 16;
   extern __shared__ float2 lds2[];
   float* rows = reinterpret_cast<float*>(lds2);
@@ -344,6 +272,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __shared__ int spre[PZ_MAX_SEG + 2];
   __shared__ float swabs[PZ_MAX_SEG];
   __shared__ float sbound[1];
+  __shared__ float tbound[NROW];
   const int tid = threadIdx.x;
 
   // XCD-aware work index: blocks sharing an x strip (hence most of their particles) and the
@@ -402,23 +331,32 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
       run += local[q];
       if (i < nseg) spre[i + 1] = run;
     }
-    // bound on |cell value| for the fixed-point accumulation: sum of |w| over the neighbourhood
-    float wa = 0.f;
-    for (int i = tid; i < nseg; i += 64) wa += swabs[i];
+  }
+  // bound on |cell value| for the fixed-point accumulation: for each row of the block, the sum
+  // of |w| over the home rows that can reach it; the block uses the largest
+  if (tid < NROW) {
+    const int rx = tid / RY, ry = tid % RY, span = rg.hi - rg.lo;
+    float sum = 0.f;
+    for (int a = 0; a <= span; ++a)
+      for (int bb = 0; bb <= span; ++bb) sum += swabs[(rx + a) * sy + (ry + bb)];
+    tbound[tid] = sum;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    float m = 0.f;
+    for (int i = tid; i < NROW; i += 64) m = fmaxf(m, tbound[i]);
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) wa += __shfl_xor(wa, d, 64);
-    if (tid == 0) sbound[0] = wa;
+    for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    if (tid == 0) sbound[0] = m;
   }
   __syncthreads();
   const int total = spre[nseg];  // number of chunks
   // Fixed-point accumulation: integer LDS atomics are exact, order independent (the painted
   // rows are bitwise reproducible) and much faster than float LDS atomics on this target.
-  // |sum| <= bound * 2^30 / bound < 2^31, and each addend is rounded to 1/scale.
+  // |sum| <= 0.99 * 2^31 (each K <= 1), and each addend is rounded to 1/scale.
   const float bound = sbound[0];
-  const float scale = bound > 0.f ? 1073741824.f / bound : 1.f;
+  const float scale = bound > 0.f ? 2126008811.f / bound : 1.f;
   int* irows = reinterpret_cast<int*>(rows);
-  // round/floor(offset + s) takes the values b and b + 1 only, offsets being in [0, 1)
-  const int b = (ORDER % 2) ? (int)rintf(s) : (int)floorf(s);
   for (int c = tid; c < total; c += PZ_NT) {
     int lo_i = 0, hi_i = nseg - 1;
     while (lo_i < hi_i) {
@@ -430,49 +368,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
     const int p1 = min(p0 + CHUNK, sstart[seg] + scount[seg]);
     const int hxr = seg / sy - rg.hi, hyr = seg % sy - rg.hi;  // home row relative to the block
-    // a chunk whose particles all share one cell (rows are sorted by z cell) is "dense"; a
-    // wavefront with enough dense chunks accumulates per cell and flushes windows, otherwise
-    // every particle scatters its stencil directly (wave-uniform choice: no divergence)
-    bool dense = false;
-    if (p1 - p0 >= 4)
-      dense = (__float_as_uint(sorted[p0].z) >> ZFRAC) == (__float_as_uint(sorted[p1 - 1].z) >> ZFRAC);
-    const bool use_window = __popcll(__ballot(dense)) >= 8;
-    if (use_window && W * W * W <= 64) {
-      const int wx0 = hxr + b + OMIN, wy0 = hyr + b + OMIN;
-      float acc[W][W][W];
-#pragma unroll
-      for (int a = 0; a < W; ++a)
-#pragma unroll
-        for (int bb = 0; bb < W; ++bb)
-#pragma unroll
-          for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = 0.f;
-      int zcur = -1;
-      for (int p = p0; p < p1; ++p) {
-        const float4 rec = sorted[p];
-        const unsigned zb = __float_as_uint(rec.z);
-        const int zc = (int)(zb >> ZFRAC);
-        if (zc != zcur) {
-          if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, irows, scale, wx0, wy0, zcur + b + OMIN);
-          zcur = zc;
-        }
-        const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
-        float wx[W], wy[W], wz[W];
-        window_weights<ORDER, W>(rec.x + s, b, wx);
-        window_weights<ORDER, W>(rec.y + s, b, wy);
-        window_weights<ORDER, W>(fz + s, b, wz);
-#pragma unroll
-        for (int a = 0; a < W; ++a) {
-          const float wa = rec.w * wx[a];
-#pragma unroll
-          for (int bb = 0; bb < W; ++bb) {
-            const float wab = wa * wy[bb];
-#pragma unroll
-            for (int cc = 0; cc < W; ++cc) acc[a][bb][cc] = fmaf(wab, wz[cc], acc[a][bb][cc]);
-          }
-        }
-      }
-      if (zcur >= 0) flush_window<W, RX, RY, NZ>(acc, irows, scale, wx0, wy0, zcur + b + OMIN);
-    } else {
+    {
       for (int p = p0; p < p1; ++p) {
         const float4 rec = sorted[p];
         const unsigned zb = __float_as_uint(rec.z);
@@ -634,12 +530,9 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
   if (np > 0)
     MP_LAUNCH("paint_rowsort_scatter", rowsort_scatter_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, bcursor, tmp_rec,
               tmp_key);
-  const size_t zlds = (size_t)(RS_NT / 64) * n[2] * sizeof(int);
-  if (zlds > 65536)
-    MP_CHECK_HIP(hipFuncSetAttribute((const void*)rowsort_fine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)zlds));
-  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), zlds, s, (const int*)bstart, tmp_rec,
+  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), 0, s, (const int*)bstart, tmp_rec,
             (const short*)tmp_key, n[1], n[2], nkeys, sorted, offsets, row_wabs);
-  sorted = tmp_rec;  // the z-sorted records end up in tmp_rec
+  sorted = tmp_rec;  // the y-sorted records end up in tmp_rec
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
   pz_rows(n[2] / 2, rg.rx, rg.ry);
