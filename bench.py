@@ -11,6 +11,11 @@ during the timed steps, against its algorithmic bytes per launch.  `cpu_baseline
 float64 NumPy oracle on a bounded sample (see its `sample` string).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+HBM traffic: `roofline.traffic` is read from profiles/hbm_traffic_512_tsc3.txt, written by
+`python bench.py --summarize-pmc FETCH_DIR WRITE_DIR --copy-bytes N` from two rocprofv3 counter runs
+(`rocprofv3 --pmc FETCH_SIZE ...` and `--pmc WRITE_SIZE ...` of `bench.py --copy-bytes N`),
+each scaled by what the same counter reads for a plain copy of N bytes in the same run.
 """
 import argparse
 import json
@@ -33,6 +38,12 @@ def parse():
     p.add_argument('--warmup', type=int, default=3)
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--seed', type=int, default=42)
+    p.add_argument('--copy-bytes', type=int, default=0,
+                   help='also run a plain device copy of this many bytes (counter calibration)')
+    p.add_argument('--summarize-pmc', nargs=2, metavar=('FETCH_DIR', 'WRITE_DIR'),
+                   help='write the HBM traffic file from two rocprofv3 --pmc output folders and exit')
+    p.add_argument('--summarize-trace', metavar='TRACE_DIR',
+                   help='write the kernel statistics summary from a rocprofv3 --kernel-trace --stats folder and exit')
     p.add_argument('--paint-path', type=int, default=None,
                    help='0 automatic, 1 tiled paint + z pass, 2 fused paint + z-FFT, 3 atomics + z pass')
     return p.parse_args()
@@ -57,6 +68,114 @@ def algorithmic_bytes(n=MESHSIZE, np_=NPART, ni=NI, nprod=len(ELLS)):
     }
 
 
+# timing name -> substring of the kernel symbol in rocprofv3 traces
+KERNEL_SYMBOLS = {
+    'paint_brick_count': 'brick_count_kernel', 'paint_brick_scatter': 'brick_scatter_kernel',
+    'paint_tile': 'paint_tile_kernel', 'paint_rowsort_hist': 'rowsort_hist_kernel',
+    'paint_rowsort_scatter': 'rowsort_scatter_kernel', 'paint_rowsort_fine': 'rowsort_fine_kernel',
+    'paint_zfft': 'paint_zfft_kernel', 'fft_zpass_r2c': 'zpass_r2c_kernel', 'fft_ypass': 'col_fft_kernel',
+    'fft_xpass_combine': 'xpass_combine_kernel', 'bin_poles': 'bin_poles_kernel',
+}
+COPY_SYMBOL = 'neg_kernel'  # torch.neg(src, out=dst): reads and writes copy_bytes each
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_512_tsc3.txt')
+TRACE_FILE = os.path.join(ROOT, 'profiles', 'kernels_512_tsc3.txt')
+PMC_COMMAND = ('rocprofv3 --pmc {counter} -d gpurun_out/pmc_{tag} -o run --output-format csv -- '
+               'python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --copy-bytes 1073741824')
+TRACE_COMMAND = ('rocprofv3 --kernel-trace --stats -d gpurun_out/kt_bench -o run --output-format csv -- '
+                 'python3 bench.py --steps 10 --warmup 3 --no-cpu-baseline')
+
+
+def _pmc_rows(folder, counter):
+    """(kernel name, dispatch id, value) of `counter` from the counter_collection.csv files under folder."""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(folder, '**', '*counter_collection.csv'), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get('Counter_Name') == counter:
+                    rows.append((r['Kernel_Name'], r.get('Dispatch_Id'), float(r['Counter_Value'])))
+    return rows
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_bytes, out=TRAFFIC_FILE):
+    """Mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, scaled by the copy kernel's
+    reading for its known byte count; written as `name fetch_bytes write_bytes traffic_bytes`."""
+    import collections
+    table = {}
+    for counter, folder in (('FETCH_SIZE', fetch_dir), ('WRITE_SIZE', write_dir)):
+        per = collections.defaultdict(lambda: collections.defaultdict(float))
+        for kname, disp, val in _pmc_rows(folder, counter):
+            per[kname][disp] += val
+        means = {k: sum(v.values()) / len(v) * 1024. for k, v in per.items()}
+        copy = [v for k, v in means.items() if COPY_SYMBOL in k]
+        if not copy:
+            raise SystemExit(f'no {COPY_SYMBOL} dispatch in {folder}')
+        table[counter] = (means, max(copy))
+    ffac = table['FETCH_SIZE'][1] / copy_bytes
+    wfac = table['WRITE_SIZE'][1] / copy_bytes
+    lines = [f'# HBM bytes per launch: rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by a {copy_bytes}-byte copy',
+             '# made by: ' + PMC_COMMAND.format(counter='FETCH_SIZE', tag='fetch'),
+             '#          ' + PMC_COMMAND.format(counter='WRITE_SIZE', tag='write'),
+             f'#          python bench.py --summarize-pmc {fetch_dir} {write_dir} --copy-bytes {copy_bytes}',
+             f'# copy read {table["FETCH_SIZE"][1]:.6g} B fetched (factor {ffac:.4f}), {table["WRITE_SIZE"][1]:.6g} B written (factor {wfac:.4f})',
+             '# name fetch_bytes write_bytes traffic_bytes']
+    for name, sym in KERNEL_SYMBOLS.items():
+        f = [v for k, v in table['FETCH_SIZE'][0].items() if sym in k]
+        w = [v for k, v in table['WRITE_SIZE'][0].items() if sym in k]
+        if f and w:
+            fb, wb = sum(f) / len(f) / ffac, sum(w) / len(w) / wfac
+            lines.append(f'{name} {fb:.0f} {wb:.0f} {fb + wb:.0f}')
+    with open(out, 'w') as fh:
+        fh.write('\n'.join(lines) + '\n')
+    print('\n'.join(lines))
+
+
+def summarize_trace(folder, out=TRACE_FILE):
+    """Per-kernel launch statistics of one rocprofv3 --kernel-trace --stats run, with the
+    algorithmic bytes and the bandwidth they imply."""
+    import csv
+    import glob
+    import re
+    path = glob.glob(os.path.join(folder, '**', '*kernel_stats.csv'), recursive=True)[0]
+    alg = algorithmic_bytes()
+    lines = ['# rocprofv3 kernel statistics of the 512^3 TSCx3 step (13 steps: 3 warm-up + 10 timed)',
+             '# made by: ' + TRACE_COMMAND, f'#          python bench.py --summarize-trace {folder}',
+             f'# {"kernel":58s} {"calls":>6s} {"avg_us":>10s} {"total_ms":>10s} {"share":>6s} {"alg_MB":>9s} {"alg_GB/s":>9s}']
+    other = [0, 0.]
+    for r in csv.DictReader(open(path)):
+        name = re.sub(r'\(.*', '', r['Name']).replace('void ', '')
+        if not name.startswith('mp::'):
+            other[0] += int(r['Calls'])
+            other[1] += float(r['TotalDurationNs']) * 1e-6
+            continue
+        avg = float(r['AverageNs']) * 1e-3
+        timing = [t for t, sym in KERNEL_SYMBOLS.items() if sym in name]
+        b = alg.get(timing[0]) if timing and not (timing[0] == 'fft_ypass' and 'true' in name) else None
+        gbs = f'{b / (avg * 1e-6) / 1e9:9.1f}' if b else f'{"":9s}'
+        mb = f'{b / 1e6:9.1f}' if b else f'{"":9s}'
+        lines.append(f'{name[:60]:60s} {int(r["Calls"]):6d} {avg:10.1f} {float(r["TotalDurationNs"]) * 1e-6:10.3f} '
+                     f'{float(r["Percentage"]):6.2f} {mb} {gbs}')
+    lines.append(f'# other kernels (torch and rocPRIM, mock generation at set-up): {other[0]} launches, {other[1]:.3f} ms')
+    lines.append('# bin_modes and the plain z-pass/c2r kernels run once at set-up (bin tables, Gaussian mock)')
+    with open(out, 'w') as fh:
+        fh.write('\n'.join(lines) + '\n')
+    print('\n'.join(lines))
+
+
+def read_traffic(path=TRAFFIC_FILE):
+    if not os.path.exists(path):
+        return {}
+    out = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith('#') or not line.strip():
+                continue
+            name, fb, wb, tb = line.split()
+            out[name] = float(tb)
+    return out
+
+
 def end_to_end_bytes(n=MESHSIZE, np_=NPART, ni=NI):
     """Algorithmic bytes of one mock: paint (16 B/particle + n_i 4N^3), n_i r2c (4N^3 + 8H),
     binning (n_i 8H), as counted in the project's baseline accounting."""
@@ -64,30 +183,40 @@ def end_to_end_bytes(n=MESHSIZE, np_=NPART, ni=NI):
     return np_ * 16 + ni * 4 * n**3 + ni * (4 * n**3 + 8 * H) + ni * 8 * H
 
 
-def cpu_baseline():
-    """Float64 NumPy oracle, one core, on one 256^3 mock with 2e6 particles (1/8 of the cells
-    and particles of the 512^3 workload); the 512^3 rate is taken as 1/8 of the sample rate."""
+def cpu_baseline(repeats=2):
+    """Float64 NumPy oracle, one core, on a 256^3 mock with 2e6 uniform particles (1/8 of the
+    cells and particles of the 512^3 workload), run `repeats` times; the 512^3 rate is taken as
+    1/8 of the best sample rate."""
     import numpy as np
     from oracle import power_oracle as po
     n, L, npart = 256, 1000., 2_000_000
     rng = np.random.default_rng(0)
     pos = rng.uniform(0., L, (npart, 3)).astype('f4').astype('f8')
     attrs = po.Attrs(n, L, L / 2.)
-    t0 = time.perf_counter()
-    mesh = po.paint(attrs, pos, None, resampler='tsc', interlacing=NI, compensate=True, out='complex')
-    mesh[0, 0, 0] = 0.
-    bin = po.BinPoles(attrs, edges={'step': STEP}, ells=ELLS)
-    po.spectrum_poles(attrs, bin, mesh, los='z')
-    po.box_normalization(attrs, np.ones(npart))
-    po.fkp_shotnoise(np.ones(npart))
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        mesh = po.paint(attrs, pos, None, resampler='tsc', interlacing=NI, compensate=True, out='complex')
+        mesh[0, 0, 0] = 0.
+        bin = po.BinPoles(attrs, edges={'step': STEP}, ells=ELLS)
+        po.spectrum_poles(attrs, bin, mesh, los='z')
+        po.box_normalization(attrs, np.ones(npart))
+        po.fkp_shotnoise(np.ones(npart))
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
     return {'value': 1. / (8. * dt), 'unit': 'mocks/s', 'cores': 1, 'kind': 'port',
-            'sample': f'float64 NumPy oracle, 1 core: one 256^3 TSCx3 mock with 2e6 particles in {dt:.1f} s '
-                      '(1/8 of the 512^3 cells and particles); value = sample rate / 8'}
+            'sample': f'float64 NumPy oracle, 1 core: a 256^3 TSCx3 mock with 2e6 particles (1/8 of the 512^3 cells '
+                      f'and particles), best of {repeats} runs {dt:.1f} s (total {sum(times):.1f} s); value = sample rate / 8'}
 
 
 def main():
     args = parse()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_bytes)
+        return
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); nothing is measured for more GPUs.', file=sys.stderr)
         sys.exit(2)
@@ -102,6 +231,12 @@ def main():
     attrs = MeshAttrs(meshsize=MESHSIZE, boxsize=BOXSIZE, boxcenter=BOXSIZE / 2.)
     particles = generate_lognormal_particles(attrs, size=NPART, seed=args.seed)
     bin = BinMesh2SpectrumPoles(attrs, edges={'step': STEP}, ells=ELLS)
+    if args.copy_bytes:
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        del src, dst
     torch.cuda.synchronize()
     print(f'# setup done: {particles.size} particles, {bin.nbins} bins', file=sys.stderr, flush=True)
 
@@ -146,6 +281,7 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     e2e_gbs = end_to_end_bytes() / (ms_per_step * 1e-3) / 1e9
     p0 = spec.get(0).value()
+    traffic = read_traffic()
     result = {
         'metric': 'P_ell(k) mocks/s at 512^3 TSCx3 (batched vs plain), HBM GB/s % of peak',
         'value': round(args.steps / elapsed, 4),
@@ -164,7 +300,7 @@ def main():
                    'ells': list(ELLS), 'kstep': STEP, 'nbins': bin.nbins, 'path': 'paint(out=complex) -> compute_mesh2_spectrum',
                    'paint_path': _lib.load().mp_set_paint_path(-1)},
         'roofline': {'kernel': dominant, 'bound': 'hbm', 'achieved': d['achieved_GBs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(d['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'frac': round(d['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': traffic.get(dominant),
                      'algorithmic_bytes_per_launch': alg[dominant], 'avg_ms': d['avg_ms']},
         'end_to_end': {'algorithmic_GB_per_mock': round(end_to_end_bytes() / 1e9, 3), 'GBs': round(e2e_gbs, 1),
                        'frac_of_peak': round(e2e_gbs / HBM_PEAK_GBS, 4)},

@@ -45,7 +45,7 @@ __global__ void combine_tables_kernel(int n0, int n1, int n2, int ni, int order,
   }
 }
 
-static int g_paint_path = 1;  // fused path opt-in until validated on the device
+static int g_paint_path = 0;  // automatic: fused paint + z-FFT where supported
 int paint_path_option() { return g_paint_path; }
 
 static bool use_fused(const int32_t n[3], int64_t np, int order, int ni) {
