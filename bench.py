@@ -64,6 +64,7 @@ def algorithmic_bytes(n=MESHSIZE, np_=NPART, ni=NI, nprod=len(ELLS)):
         'fft_zpass_r2c': ncell * 4 + H * 8,
         'fft_ypass': 2 * H * 8,
         'fft_xpass_combine': ni * H * 8 + H * 8,
+        'fft_xpass_bin': ni * H * 8,                      # n_i partial spectra in, per-bin sums out
         'bin_poles': H * 8,
     }
 
@@ -74,7 +75,8 @@ KERNEL_SYMBOLS = {
     'paint_tile': 'paint_tile_kernel', 'paint_rowsort_hist': 'rowsort_hist_kernel',
     'paint_rowsort_scatter': 'rowsort_scatter_kernel', 'paint_rowsort_fine': 'rowsort_fine_kernel',
     'paint_zfft': 'paint_zfft_kernel', 'fft_zpass_r2c': 'zpass_r2c_kernel', 'fft_ypass': 'col_fft_kernel',
-    'fft_xpass_combine': 'xpass_combine_kernel', 'bin_poles': 'bin_poles_kernel',
+    'fft_xpass_combine': 'xpass_kernel<512, false', 'fft_xpass_bin': 'xpass_kernel<512, true',
+    'bin_poles': 'bin_poles_kernel',
 }
 COPY_SYMBOL = 'neg_kernel'  # torch.neg(src, out=dst): reads and writes copy_bytes each
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_512_tsc3.txt')
@@ -242,7 +244,7 @@ def main():
 
     def step():
         mesh = particles.paint(resampler='tsc', interlacing=NI, compensate=True, out='complex')
-        mesh.value[0, 0, 0] = 0.  # mesh - mesh.mean() in Fourier space
+        mesh = mesh.at[0, 0, 0].set(0.)  # mesh - mesh.mean() in Fourier space
         spec = compute_mesh2_spectrum(mesh, bin=bin, los='z')
         return spec.clone(norm=compute_box2_normalization(particles, bin=bin),
                           num_shotnoise=compute_fkp2_shotnoise(particles, bin=bin))

@@ -58,6 +58,34 @@ int mp_paint_spectrum(void* stream, const void* positions, int positions_f64, co
                       const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
                       int interlacing, int compensate, void* out, int64_t out_pitch, void* workspace,
                       size_t workspace_bytes);
+/* The two halves of mp_paint_spectrum, so that a consumer can take the interlaced meshes
+ * before their combine.  mp_paint_partial: the scan over interlacing shifts of _paint
+ * (jaxpower/mesh.py:2738-2747) up to, not including, the x axis of each r2c: writes
+ * `interlacing` partial spectra, mesh j at partial + j * N0 * N1 * pitch, each transformed
+ * along z and y only. */
+int64_t mp_paint_partial_workspace_bytes(const int32_t meshsize[3], int64_t np, int order, int interlacing);
+int mp_paint_partial(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+                     const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
+                     int interlacing, void* partial, int64_t pitch, void* workspace, size_t workspace_bytes);
+/* mp_combine_partial: the x axis of the r2c of each partial spectrum, the phases
+ * exp(+i s_j (kx+ky+kz)) / n_i, the sum over shifts and the compensation
+ * (jaxpower/mesh.py:2739-2751, jaxpower/resamplers.py:16-26); zero_dc sets mode (0,0,0) to 0,
+ * as mesh.at[0, 0, 0].set(0.) does. */
+int64_t mp_combine_partial_workspace_bytes(const int32_t meshsize[3], int interlacing);
+int mp_combine_partial(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
+                       int interlacing, int compensate, int zero_dc, void* out, int64_t out_pitch, void* workspace,
+                       size_t workspace_bytes);
+/* mp_bin_poles_partial: compute_mesh2_spectrum with a global line of sight on the auto
+ * spectrum of the combined mesh (jaxpower/mesh2.py:346-369, BinMesh2SpectrumPoles.__call__
+ * mesh2.py:100-119), fused with mp_combine_partial so the combined mesh is never written:
+ * out[p][b][re, im] = sum over modes of bin b of w |A|^2 L_ell[p](mu), im = 0, with the same
+ * tables, thresholds and remove_zero rule as mp_bin_poles. */
+int64_t mp_bin_partial_workspace_bytes(const int32_t meshsize[3], int interlacing, int nprod, int nedges);
+int mp_bin_poles_partial(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
+                         int interlacing, int compensate, int zero_dc, const double* kx, const double* ky,
+                         const double* kz, const double* k2_thresholds, int nedges, const double los[3], int nprod,
+                         const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out, void* workspace,
+                         size_t workspace_bytes);
 /* Paint path of mp_paint_spectrum: 0 automatic (fused paint + z-FFT in LDS when the shape
  * allows), 1 tiled paint then z pass, 2 fused only, 3 per-particle atomics then z pass.
  * Returns the previous setting; a negative path only queries.  No counterpart in the

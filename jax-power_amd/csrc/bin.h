@@ -1,0 +1,53 @@
+// Shared pieces of the |k|-shell binning: geometry, exact k^2 in float64, bin search, and the
+// deterministic reduction of per-block partial histograms.
+#pragma once
+
+#include "common.h"
+
+namespace mp {
+
+constexpr int BIN_LDS_DOUBLES = 4096;  // 32 KiB histogram budget per block
+
+struct BinGeom {
+  int n0, n1, ncols;
+  int nedges;
+  const double* kx;
+  const double* ky;
+  const double* kz;
+  const double* thr;  // thr[e] = min{t : fl(sqrt(t)) >= edge e}
+};
+
+// (kx^2 + ky^2) + kz^2 exactly as IEEE float64 (no fused multiply-add), the order in which the
+// reference sums the squared components.
+__device__ __forceinline__ double k2_sum(double a2, double kz) {
+#pragma clang fp contract(off)
+  return a2 + kz * kz;
+}
+__device__ __forceinline__ double k2_xy(double kx, double ky) {
+#pragma clang fp contract(off)
+  return kx * kx + ky * ky;
+}
+
+// Number of thresholds <= k2: the mode is in bin (result - 1) when 1 <= result < nedges,
+// which is digitize(|k|, edges, right=False) of the reference.
+__device__ __forceinline__ int first_bin(const double* thr, int nedges, double k2) {
+  int lo = 0, hi = nedges;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (thr[mid] <= k2) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Move a bin position found for a nearby k2 to the one of k2 (either direction).
+__device__ __forceinline__ int walk_bin(const double* thr, int nedges, int b, double k2) {
+  while (b < nedges && thr[b] <= k2) ++b;
+  while (b > 0 && thr[b - 1] > k2) --b;
+  return b;
+}
+
+// out[e] = sum over blocks of partial[block][e], in a fixed order.
+int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out);
+
+}  // namespace mp

@@ -14,7 +14,7 @@
 // |k| increases, so the bin index only moves forward and the thread keeps FP64 running sums
 // in registers, flushing them into an LDS histogram when the bin changes.  Each block writes
 // its histogram once; a second kernel sums the block partials in a fixed order.
-#include "common.h"
+#include "bin.h"
 #include "sph.h"
 
 namespace mp {
@@ -23,16 +23,6 @@ constexpr int BIN_SEG = 16;
 constexpr int BIN_NT = 256;
 constexpr int BIN_MAX_PROD = 6;
 constexpr int BIN_MAX_MESH = 4;
-constexpr int BIN_LDS_DOUBLES = 4096;  // 32 KiB histogram budget per block
-
-struct BinGeom {
-  int n0, n1, ncols;
-  int nedges;
-  const double* kx;
-  const double* ky;
-  const double* kz;
-  const double* thr;
-};
 
 struct BinProducts {
   int nmesh, nprod;
@@ -43,25 +33,13 @@ struct BinProducts {
   int need_mu;
 };
 
-// (kx^2 + ky^2) + kz^2 exactly as IEEE float64 (no fused multiply-add).
-__device__ __forceinline__ double k2_sum(double a2, double kz) {
-#pragma clang fp contract(off)
-  return a2 + kz * kz;
-}
-__device__ __forceinline__ double k2_xy(double kx, double ky) {
-#pragma clang fp contract(off)
-  return kx * kx + ky * ky;
-}
-
-__device__ __forceinline__ int first_bin(const double* thr, int nedges, double k2) {
-  // number of thresholds <= k2
-  int lo = 0, hi = nedges;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (thr[mid] <= k2) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
+// v[m] for a run-time m without indexing a private array (which would live in scratch memory).
+__device__ __forceinline__ float2 pick_mesh(const float2 (&v)[BIN_MAX_MESH], int m) {
+  float2 r = v[0];
+#pragma unroll
+  for (int q = 1; q < BIN_MAX_MESH; ++q)
+    if (m == q) r = v[q];
+  return r;
 }
 
 // Per-block histogram of Sigma w * product * L_ell, two doubles (re, im) per (product, bin).
@@ -113,7 +91,7 @@ bin_poles_kernel(BinGeom g, BinProducts P, double* __restrict__ partial, double*
       float2 v[BIN_MAX_MESH];
 #pragma unroll
       for (int m = 0; m < BIN_MAX_MESH; ++m)
-        if (m < P.nmesh) v[m] = P.mesh[m][row * P.pitch[m] + iz];
+        v[m] = m < P.nmesh ? P.mesh[m][row * P.pitch[m] + iz] : make_float2(0.f, 0.f);
       const float w = iz > 0 ? 2.f : 1.f;
       float mu = 0.f;
       if (P.need_mu) {
@@ -125,8 +103,8 @@ bin_poles_kernel(BinGeom g, BinProducts P, double* __restrict__ partial, double*
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
         if (origin && P.rz[p]) continue;
-        float2 a = P.pi[p] < 0 ? make_float2(1.f, 0.f) : v[P.pi[p]];
-        float2 c = v[P.pj[p]];
+        float2 a = P.pi[p] < 0 ? make_float2(1.f, 0.f) : pick_mesh(v, P.pi[p]);
+        float2 c = pick_mesh(v, P.pj[p]);
         float re = a.x * c.x + a.y * c.y;  // conj(a) * c
         float im = a.x * c.y - a.y * c.x;
         float f = w * (P.ell[p] ? legendre(P.ell[p], mu) : 1.f);
@@ -164,6 +142,11 @@ reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int size
     __syncthreads();
   }
   if (threadIdx.x == 0) out[e] = sh[0];
+}
+
+int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out) {
+  MP_LAUNCH("bin_reduce", reduce_partials_kernel, dim3((unsigned)size), dim3(RED_NT), 0, s, partial, nblocks, size, out);
+  return MP_OK;
 }
 
 __global__ void fill_f64_kernel(double* p, int64_t n) {
@@ -234,8 +217,7 @@ static int launch_poles(hipStream_t s, const BinGeom& g, const BinProducts& P, d
     if (!partial) return set_error(MP_ERR_WORKSPACE, "bin: workspace too small");
     MP_LAUNCH("bin_poles", (bin_poles_kernel<NP, true>), dim3(blocks), dim3(BIN_NT), hsize * sizeof(double), s, g, P, partial,
               (double*)nullptr);
-    MP_LAUNCH("bin_reduce", reduce_partials_kernel, dim3((unsigned)hsize), dim3(RED_NT), 0, s, (const double*)partial, blocks, hsize,
-              out);
+    return reduce_partials(s, partial, blocks, hsize, out);
   } else {
     MP_LAUNCH("bin_zero", fill_f64_kernel, dim3((unsigned)ceil_div(hsize, 256)), dim3(256), 0, s, out, (int64_t)hsize);
     MP_LAUNCH("bin_poles", (bin_poles_kernel<NP, false>), dim3(blocks), dim3(BIN_NT), 0, s, g, P, (double*)nullptr, out);

@@ -56,8 +56,19 @@ int rfft3d(hipStream_t s, const float* in, float2* out, const int32_t n[3], int6
 // z and y passes only (the x pass is fused into a consumer).
 int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro);
 int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch);
-int combine_x(hipStream_t s, const float2* partial, int64_t mesh_stride, int ni, float2* out, const int32_t n[3],
-              int64_t in_pitch, int64_t out_pitch, const CombineTables& tab, float scale);
+// y pass of nmesh half spectra at data + m * mesh_stride, in one pipelined launch.
+int ypass_meshes(hipStream_t s, float2* data, int nmesh, int64_t mesh_stride, const int32_t n[3], int64_t pitch);
+// x pass of the interlaced paint (xpass.hip): combine of the n_i partial spectra after their z
+// and y passes (mesh j at partial + j * n0 * n1 * pitch), written out or binned.
+struct BinGeom;
+bool xpass_supported(const int32_t n[3]);
+size_t combine_partial_workspace_bytes(const int32_t n[3], int ni);
+int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
+                    int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes);
+size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges);
+int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
+                int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
+                double* out, void* ws, size_t ws_bytes);
 int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n[3], int64_t in_pitch, int64_t acc_pitch,
               const KPrologue& kp, int first);
 int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const int32_t n[3], float2* scratch,

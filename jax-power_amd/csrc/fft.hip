@@ -13,8 +13,7 @@
 //   y, x: tiles of (L along the FFT axis) x (C contiguous columns) are staged in LDS with the
 //      contiguous axis fastest (coalesced 128-byte row segments, conflict-free LDS access),
 //      transformed by a radix-8/4/2 Stockham FFT, and written back.
-//   The interlaced paint fuses its phase shifts, 1/n_i, and the window compensation into the
-//   x pass, which reads all n_i partial spectra and writes one combined spectrum.
+//   The x pass of the interlaced paint (combine of the n_i shifted meshes) is in xpass.hip.
 #include "common.h"
 #include "fft.h"
 #include "fft_core.h"
@@ -54,15 +53,6 @@ int ensure_twiddles() {
 }
 
 // ------------------------------------------------------------------ launch geometry
-
-// Column passes: C columns of length L per block (C*L*8 bytes of LDS = 64 KiB at L >= 512).
-template <int L>
-struct ColCfg {
-  static constexpr int C = L <= 512 ? 16 : 8192 / L;
-  static constexpr int NT0 = C * L / 8;
-  static constexpr int NT = NT0 < 64 ? 64 : NT0;
-  static constexpr size_t LDS = (size_t)C * L * sizeof(float2);
-};
 
 // z passes: C rows of M = Nz/2 complex per block, rows padded by one element in LDS.
 template <int M>
@@ -181,60 +171,55 @@ col_fft_kernel(const float2* in, float2* out, int ncols, int64_t in_fstride, int
   }
 }
 
-// Interlacing combine along x (the last forward pass of the painted meshes):
-//   out(kx, ky, kz) = scale * comp_x comp_y comp_z * sum_j FFT_x(in_j)(kx) ph_x[j,kx] ph_y[j,ky] ph_z[j,kz]
-// Block (chunk, y) handles columns kz in [c0, c0+C) of the y-th row for all kx.
-template <int L>
-__global__ void __launch_bounds__(ColCfg<L>::NT)
-xpass_combine_kernel(const float2* __restrict__ in, int64_t in_mesh_stride, int ni, float2* __restrict__ out,
-                     int ncols, int ny, int64_t in_fstride, int64_t in_ostride, int64_t out_fstride,
-                     int64_t out_ostride, CombineTables tab, float scale) {
-  using Cfg = ColCfg<L>;
-  constexpr int C = Cfg::C, NT = Cfg::NT;
-  constexpr int EPT = (C * L + NT - 1) / NT;
+// Pipelined column pass over several meshes: persistent blocks walk tiles (mesh, outer,
+// chunk of C columns); the global loads of the next tile are issued before the FFT of the
+// current one, and the tile is written back in the order it was read (in place allowed).
+struct ColPassArgs {
+  float2* data;
+  int64_t mesh_stride, fstride, ostride;  // between meshes, FFT rows, outer slices
+  int ncols, nouter, nchunks, ntiles;
+  const float2* tw;
+};
+
+template <int L, bool INV>
+__global__ void __launch_bounds__(XCfg<L>::NT)
+colpass_kernel(ColPassArgs a) {
+  using Cfg = XCfg<L>;
+  constexpr int C = Cfg::C, NT = Cfg::NT, EPT = Cfg::EPT;
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * C;
-  const int y = blockIdx.y;
-  float2 acc[EPT];
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
-  for (int j = 0; j < ni; ++j) {
-    const float2* src = in + j * in_mesh_stride + (int64_t)y * in_ostride + c0;
-    for (int idx = tid; idx < C * L; idx += NT) {
-      int c = idx % C, l = idx / C;
-      lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_lds<L, C, C, 1, true, NT, false>(lds, g_twiddle);
+  float2 reg[EPT];
+  auto base = [&](int tile) {
+    const int chunk = tile % a.nchunks, rest = tile / a.nchunks;
+    const int o = rest % a.nouter, m = rest / a.nouter;
+    return a.data + m * a.mesh_stride + (int64_t)o * a.ostride + chunk * C;
+  };
+  auto valid = [&](int tile, int cc) { return (tile % a.nchunks) * C + cc < a.ncols; };
+  auto load = [&](int tile) {
+    const float2* src = base(tile);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      int idx = tid + e * NT;
-      if (idx < C * L) {
-        int c = idx % C, l = idx / C;
-        float2 v = lds[l * C + c];
-        if (tab.phx) {
-          int kz = min(c0 + c, ncols - 1);
-          float2 ph = cmul(cmul(tab.phx[j * tab.stride + l], tab.phy[j * tab.stride + y]), tab.phz[j * tab.stride + kz]);
-          v = cmul(v, ph);
-        }
-        acc[e] = cadd(acc[e], v);
-      }
+      const int idx = tid + e * NT, cc = idx % C, l = idx / C;
+      reg[e] = valid(tile, cc) ? src[l * a.fstride + cc] : make_float2(0.f, 0.f);
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < a.ntiles) load(tile);
+  while (tile < a.ntiles) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];
+    __syncthreads();
+    const int next = tile + gridDim.x;
+    if (next < a.ntiles) load(next);
+    fft_lds<L, C, C, 1, true, NT, INV>(lds, a.tw);
+    float2* dst = base(tile);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = tid + e * NT, cc = idx % C, l = idx / C;
+      if (valid(tile, cc)) dst[l * a.fstride + cc] = lds[idx];
     }
     __syncthreads();
-  }
-  float2* dst = out + (int64_t)y * out_ostride + c0;
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    int idx = tid + e * NT;
-    if (idx < C * L) {
-      int c = idx % C, l = idx / C;
-      if (c0 + c < ncols) {
-        float s = scale;
-        if (tab.cx) s *= tab.cx[l] * tab.cy[y] * tab.cz[c0 + c];
-        dst[l * out_fstride + c] = cscale(acc[e], s);
-      }
-    }
+    tile = next;
   }
 }
 
@@ -297,16 +282,6 @@ static int launch_col(const char* name, const float2* in, float2* out, int ncols
 }
 
 template <int L>
-static int launch_combine(const float2* in, int64_t mstride, int ni, float2* out, int ncols, int ny, int64_t ifs, int64_t ios,
-                          int64_t ofs, int64_t oos, const CombineTables& tab, float scale, hipStream_t stream) {
-  using Cfg = ColCfg<L>;
-  dim3 grid((unsigned)ceil_div(ncols, Cfg::C), (unsigned)ny);
-  MP_LAUNCH("fft_xpass_combine", xpass_combine_kernel<L>, grid, dim3(Cfg::NT), Cfg::LDS, stream, in, mstride, ni, out, ncols, ny,
-            ifs, ios, ofs, oos, tab, scale);
-  return MP_OK;
-}
-
-template <int L>
 static int launch_ylm_acc(const float2* in, float2* acc, int ncols, int ny, int64_t ifs, int64_t ios, int64_t afs, int64_t aos,
                           const KPrologue& kp, int first, hipStream_t stream) {
   using Cfg = ColCfg<L>;
@@ -363,10 +338,21 @@ static int col_fft(int L, bool inv, const char* name, const float2* in, float2* 
   MP_POW2_SWITCH(L, launch_col_fwd, name, in, out, ncols, nouter, ifs, ios, ofs, oos, scale, s);
 }
 
-static int xpass_combine(int L, const float2* in, int64_t mstride, int ni, float2* out, int ncols, int ny, int64_t ifs,
-                         int64_t ios, int64_t ofs, int64_t oos, const CombineTables& tab, float scale, hipStream_t s) {
-  if (L < 2) return set_error(MP_ERR_UNSUPPORTED, "x length %d < 2", L);
-  MP_POW2_SWITCH(L, launch_combine, in, mstride, ni, out, ncols, ny, ifs, ios, ofs, oos, tab, scale, s);
+template <int L>
+static int launch_colpass(hipStream_t s, ColPassArgs& a) {
+  using Cfg = XCfg<L>;
+  auto kern = colpass_kernel<L, false>;
+  a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
+  int per = 0;
+  MP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, Cfg::NT, Cfg::LDS));
+  if (per < 1) per = 1;
+  int dev = 0, ncu = 0;
+  MP_CHECK_HIP(hipGetDevice(&dev));
+  MP_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int blocks = ncu * per;
+  if (blocks > a.ntiles) blocks = a.ntiles;
+  MP_LAUNCH("fft_ypass", kern, dim3((unsigned)blocks), dim3(Cfg::NT), Cfg::LDS, s, a);
+  return MP_OK;
 }
 
 static int xpass_ylm_acc(int L, const float2* in, float2* acc, int ncols, int ny, int64_t ifs, int64_t ios, int64_t afs,
@@ -415,6 +401,32 @@ int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], i
   return col_fft(n[1], false, "fft_ypass", out, out, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
 }
 
+int ypass_meshes(hipStream_t s, float2* data, int nmesh, int64_t mesh_stride, const int32_t n[3], int64_t pitch) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  ColPassArgs a{};
+  a.data = data;
+  a.mesh_stride = mesh_stride;
+  a.fstride = pitch;
+  a.ostride = (int64_t)n[1] * pitch;
+  a.ncols = n[2] / 2 + 1;
+  a.nouter = n[0];
+  a.tw = twiddle_device_ptr();
+  if (!a.tw) return set_error(MP_ERR_HIP, "ypass: twiddle table address");
+  switch (n[1]) {
+#define MP_YCASE(L)                                                  \
+  case L:                                                            \
+    a.ntiles = (int)ceil_div(a.ncols, XCfg<L>::C) * n[0] * nmesh;    \
+    return launch_colpass<L>(s, a);
+    MP_YCASE(2) MP_YCASE(4) MP_YCASE(8) MP_YCASE(16) MP_YCASE(32) MP_YCASE(64) MP_YCASE(128) MP_YCASE(256) MP_YCASE(512)
+    MP_YCASE(1024) MP_YCASE(2048)
+#undef MP_YCASE
+    default: return set_error(MP_ERR_UNSUPPORTED, "ypass: length %d", n[1]);
+  }
+}
+
 int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch) {
   int rc = check_shape(n);
   if (rc) return rc;
@@ -423,14 +435,6 @@ int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch) {
   const int ncols = n[2] / 2 + 1;
   const int64_t nx = n[0], ny = n[1];
   return col_fft(n[1], false, "fft_ypass", data, data, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
-}
-
-int combine_x(hipStream_t s, const float2* partial, int64_t mesh_stride, int ni, float2* out, const int32_t n[3],
-              int64_t in_pitch, int64_t out_pitch, const CombineTables& tab, float scale) {
-  const int ncols = n[2] / 2 + 1;
-  const int64_t ny = n[1];
-  return xpass_combine(n[0], partial, mesh_stride, ni, out, ncols, n[1], ny * in_pitch, in_pitch, ny * out_pitch, out_pitch,
-                       tab, scale, s);
 }
 
 int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n[3], int64_t in_pitch, int64_t acc_pitch,

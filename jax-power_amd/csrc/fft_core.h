@@ -68,6 +68,26 @@ __device__ __forceinline__ void dft(float2* v) {
   }
 }
 
+// Column passes: C columns of length L per block (C*L*8 bytes of LDS = 64 KiB at L >= 512).
+template <int L>
+struct ColCfg {
+  static constexpr int C = L <= 512 ? 16 : 8192 / L;
+  static constexpr int NT0 = C * L / 8;
+  static constexpr int NT = NT0 < 64 ? 64 : NT0;
+  static constexpr size_t LDS = (size_t)C * L * sizeof(float2);
+};
+
+// Tiles of the pipelined column passes: C columns x L rows, EPT = C*L/NT elements per thread
+// (a whole number for every power of two L in [2, 2048]).
+template <int L>
+struct XCfg {
+  static constexpr int C = L <= 512 ? (L >= 4 ? 16 : 64 / L) : 8192 / L;
+  static constexpr int NT0 = C * L / 8;
+  static constexpr int NT = NT0 < 64 ? 64 : NT0;
+  static constexpr int EPT = C * L / NT;
+  static constexpr size_t LDS = (size_t)C * L * sizeof(float2);
+};
+
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 constexpr int first_radix(int L) { return ilog2(L) % 3 == 0 ? 8 : (ilog2(L) % 3 == 1 ? 2 : 4); }
 

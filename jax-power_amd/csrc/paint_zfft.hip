@@ -196,17 +196,22 @@ __device__ int block_excl_scan(int* data, int n, int* wsum) {
   return wsum[RS_NT / 64];
 }
 
-// One workgroup per x bucket: counting sort by y; writes the row offsets and the sum of |w|
-// of every row.  tmp_rec[start, end) is free once the y scatter has read it, and receives
-// the final order.
-__device__ __forceinline__ float wave_sum_f32(float v) {
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
 }
 
+// One workgroup per x bucket: counting sort by y into ysorted (the order the paint reads);
+// writes the row offsets and a bound on the sum of |w| of every row.
 __global__ void __launch_bounds__(RS_NT)
-rowsort_fine_kernel(const int* __restrict__ bucket_start, float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
+rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
                     int nz, int64_t nkeys, float4* __restrict__ ysorted, int* __restrict__ offsets, float* __restrict__ row_wabs) {
   __shared__ int cnt[RS_MAX_KEYS + 1];
   __shared__ int wsum[RS_NT / 64 + 1];
@@ -226,19 +231,21 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, float4* __restrict__ t
     ysorted[start + r] = tmp_rec[i];
   }
   __syncthreads();  // cnt[y] now holds the end of row y; ysorted is visible to the workgroup
-  // per-row sum of |w| (bounds the fixed-point accumulation of the paint), and the y-sorted
-  // records copied back to tmp_rec, where the paint reads them
+  // Per-row bound on sum |w|, which sets the fixed-point scale of the paint.  It must not
+  // depend on the order of the row's records (which the atomics above leave unspecified), so
+  // it is an integer sum: q_i = ceil(|w_i| / m * 2^24) with m the row's largest |w|, and the
+  // bound is m * sum q_i / 2^24 >= sum |w|.
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int y = wave; y < ny; y += RS_NT / 64) {
     const int rs = start + (y ? cnt[y - 1] : 0), re = start + cnt[y];
-    float wa = 0.f;
-    for (int i = rs + lane; i < re; i += 64) {
-      float4 rec = ysorted[i];
-      tmp_rec[i] = rec;
-      wa += fabsf(rec.w);
-    }
-    wa = wave_sum_f32(wa);
-    if (lane == 0) row_wabs[(int64_t)hx * ny + y] = wa;
+    float m = 0.f;
+    for (int i = rs + lane; i < re; i += 64) m = fmaxf(m, fabsf(ysorted[i].w));
+    m = wave_max_f32(m);
+    unsigned long long q = 0;
+    if (m > 0.f)
+      for (int i = rs + lane; i < re; i += 64) q += (unsigned long long)ceilf(fabsf(ysorted[i].w) / m * 16777216.f);
+    q = wave_sum_u64(q);
+    if (lane == 0) row_wabs[(int64_t)hx * ny + y] = (float)((double)q * (double)m * (1.0 / 16777216.0));
   }
 }
 
@@ -264,7 +271,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   constexpr int LPC = M + 1;       // complex row pitch in LDS
   constexpr int NROW = RX * RY;
   constexpr int OMIN = -((ORDER - 1) / 2);
-  constexpr int CHUNK = 16;
+  constexpr int CHUNK = 4;  // particles per work item, loaded together before any is painted
   extern __shared__ float2 lds2[];
   float* rows = reinterpret_cast<float*>(lds2);
   __shared__ int sstart[PZ_MAX_SEG];
@@ -366,11 +373,16 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     }
     const int seg = lo_i;
     const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
-    const int p1 = min(p0 + CHUNK, sstart[seg] + scount[seg]);
+    const int cnt = min(CHUNK, sstart[seg] + scount[seg] - p0);
     const int hxr = seg / sy - rg.hi, hyr = seg % sy - rg.hi;  // home row relative to the block
+    float4 recs[CHUNK];
+#pragma unroll
+    for (int q = 0; q < CHUNK; ++q) recs[q] = q < cnt ? sorted[p0 + q] : make_float4(0.f, 0.f, 0.f, 0.f);
     {
-      for (int p = p0; p < p1; ++p) {
-        const float4 rec = sorted[p];
+#pragma unroll
+      for (int q = 0; q < CHUNK; ++q) {
+        if (q >= cnt) break;
+        const float4 rec = recs[q];
         const unsigned zb = __float_as_uint(rec.z);
         const int zc = (int)(zb >> ZFRAC);
         const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
@@ -532,7 +544,6 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
               tmp_key);
   MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), 0, s, (const int*)bstart, tmp_rec,
             (const short*)tmp_key, n[1], n[2], nkeys, sorted, offsets, row_wabs);
-  sorted = tmp_rec;  // the y-sorted records end up in tmp_rec
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
   pz_rows(n[2] / 2, rg.rx, rg.ry);

@@ -6,6 +6,7 @@
 // pass transforms all n_i partial spectra, applies exp(+i s_j (kx+ky+kz)) / n_i (mesh.py:
 // 2740-2744), sums them, and applies the compensation prod_a sinc(k_a/2pi)^-p
 // (resamplers.py:16-26) before the single write of the result.
+#include "bin.h"
 #include "common.h"
 #include "fft.h"
 #include "paint.h"
@@ -17,34 +18,6 @@ namespace mp {
 int fill_f32(hipStream_t s, float* p, int64_t n, float v);
 int read_mesh(hipStream_t s, const float* mesh, const GridMap& g, const void* pos, int pos_f64, int64_t np, int order, float* out);
 
-// Per-axis interlacing phases exp(i s_j k_circ) and compensation sinc(k/2pi)^-order, in FP64
-// then rounded to FP32.  k_circ = 2 pi fftfreq(n) (rfftfreq on the last axis), mesh.py:1605-1636.
-__global__ void combine_tables_kernel(int n0, int n1, int n2, int ni, int order, float2* __restrict__ ph, float* __restrict__ comp) {
-  const int ncols = n2 / 2 + 1;
-  const int len = n0 + n1 + ncols;
-  const int total = (ni + 1) * len;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  int j = t / len, r = t % len;
-  int axis = r < n0 ? 0 : (r < n0 + n1 ? 1 : 2);
-  int m = axis == 0 ? r : (axis == 1 ? r - n0 : r - n0 - n1);
-  int n = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
-  int f = (axis == 2 || m < (n + 1) / 2) ? m : m - n;
-  double x = (double)f / (double)n;  // fftfreq value
-  if (j < ni) {
-    double s = (double)j / (double)ni;
-    double a = 2.0 * M_PI * x * s;
-    ph[t] = make_float2((float)cos(a), (float)sin(a));
-  } else {
-    double c = 1.0;
-    if (f != 0 && order > 0) {
-      double px = M_PI * x;
-      c = pow(sin(px) / px, -(double)order);
-    }
-    comp[r] = (float)c;
-  }
-}
-
 static int g_paint_path = 0;  // automatic: fused paint + z-FFT where supported
 int paint_path_option() { return g_paint_path; }
 
@@ -54,83 +27,75 @@ static bool use_fused(const int32_t n[3], int64_t np, int order, int ni) {
   return paint_zfft_supported(n, np, order, ni);
 }
 
-size_t paint_spectrum_workspace_bytes(const int32_t n[3], int64_t np, int order, int ni) {
-  const int64_t nspec_ = (int64_t)n[0] * n[1] * spectrum_pitch(n[2]);
-  if (use_fused(n, np, order, ni)) {
-    const int64_t len = n[0] + n[1] + n[2] / 2 + 1;
-    size_t bytes = align_up((size_t)ni * nspec_ * 8, 256) + align_up((size_t)(ni + 1) * len * 8, 256) + align_up((size_t)len * 4, 256);
-    return bytes + paint_zfft_workspace_bytes(n, np) + 1024;
-  }
+// Workspace of paint_partial beyond its output (the n_i partial spectra).
+size_t paint_partial_workspace_bytes(const int32_t n[3], int64_t np, int order, int ni) {
+  if (use_fused(n, np, order, ni)) return paint_zfft_workspace_bytes(n, np) + 512;
   const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+  return align_up((size_t)ni * ncell * 4, 256) + paint_workspace_bytes(n, np, order, ni) + 512;
+}
+
+// The n_i shifted meshes painted and transformed along z and y: partial + j * n0 * n1 * pitch.
+int paint_partial(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
+                  float2* partial, int64_t pitch, void* ws, size_t ws_bytes) {
+  const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
+  MP_REQUIRE(partial, "paint_partial: null output");
+  MP_REQUIRE(pitch >= n[2] / 2 + 1, "paint_partial: pitch too small");
+  const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+  const int64_t nspec = (int64_t)n[0] * n[1] * pitch;
+  const bool fused = use_fused(n, np, order, ni);
+  if (paint_path_option() == 2 && !fused) return set_error(MP_ERR_UNSUPPORTED, "paint: fused path not available");
+  Workspace wsp(ws, ws_bytes);
+  int rc = MP_OK;
+  if (fused) {
+    size_t pbytes = paint_zfft_workspace_bytes(n, np);
+    void* pws = wsp.take<char>(pbytes);
+    if (!pws)
+      return set_error(MP_ERR_WORKSPACE, "paint: workspace of %zu bytes < %zu needed", ws_bytes,
+                       paint_partial_workspace_bytes(n, np, order, ni));
+    rc = paint_zfft(s, pos, pos_f64, w, np, g, order, ni, partial, pitch, nspec, pws, pbytes);
+    if (rc) return rc;
+    return ypass_meshes(s, partial, ni, nspec, n, pitch);
+  }
+  float* meshes = wsp.take<float>((size_t)ni * ncell);
+  size_t pbytes = paint_workspace_bytes(n, np, order, ni);
+  void* pws = wsp.take<char>(pbytes);
+  if (!meshes || !pws)
+    return set_error(MP_ERR_WORKSPACE, "paint: workspace of %zu bytes < %zu needed", ws_bytes,
+                     paint_partial_workspace_bytes(n, np, order, ni));
+  rc = paint_meshes(s, pos, pos_f64, w, np, g, order, ni, meshes, ncell, pws, pbytes, paint_path_option() == 3);
+  if (rc) return rc;
+  RealPrologue pro{};
+  for (int j = 0; j < ni; ++j) {
+    rc = rfft3d_zy(s, meshes + j * ncell, partial + j * nspec, n, pitch, pro);
+    if (rc) return rc;
+  }
+  return MP_OK;
+}
+
+size_t paint_spectrum_workspace_bytes(const int32_t n[3], int64_t np, int order, int ni) {
   const int64_t nspec = (int64_t)n[0] * n[1] * spectrum_pitch(n[2]);
-  const int64_t len = n[0] + n[1] + n[2] / 2 + 1;
-  size_t bytes = 0;
-  bytes += align_up((size_t)ni * ncell * 4, 256);
-  bytes += align_up((size_t)ni * nspec * 8, 256);
-  bytes += align_up((size_t)(ni + 1) * len * 8, 256);
-  bytes += align_up((size_t)len * 4, 256);
-  bytes += paint_workspace_bytes(n, np, order, ni);
-  return bytes + 1024;
+  return align_up((size_t)ni * nspec * 8, 256) + paint_partial_workspace_bytes(n, np, order, ni) +
+         combine_partial_workspace_bytes(n, ni) + 1024;
 }
 
 int paint_spectrum(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
                    int compensate, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes) {
   const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
   MP_REQUIRE(out_pitch >= n[2] / 2 + 1, "paint_spectrum: out_pitch too small");
-  const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+  if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "paint_spectrum: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
   const int64_t P = spectrum_pitch(n[2]);
   const int64_t nspec = (int64_t)n[0] * n[1] * P;
-  const int len = n[0] + n[1] + n[2] / 2 + 1;
-  const bool fused = use_fused(n, np, order, ni);
-  if (paint_path_option() == 2 && !fused) return set_error(MP_ERR_UNSUPPORTED, "paint_spectrum: fused path not available");
   Workspace wsp(ws, ws_bytes);
   float2* partial = wsp.take<float2>((size_t)ni * nspec);
-  float2* ph = wsp.take<float2>((size_t)(ni + 1) * len);
-  float* comp = wsp.take<float>((size_t)len);
-  int rc = MP_OK;
-  if (fused) {
-    size_t pbytes = paint_zfft_workspace_bytes(n, np);
-    void* pws = wsp.take<char>(pbytes);
-    if (!partial || !ph || !comp || !pws)
-      return set_error(MP_ERR_WORKSPACE, "paint_spectrum: workspace of %zu bytes < %zu needed", ws_bytes,
-                       paint_spectrum_workspace_bytes(n, np, order, ni));
-    rc = paint_zfft(s, pos, pos_f64, w, np, g, order, ni, partial, P, nspec, pws, pbytes);
-    if (rc) return rc;
-    for (int j = 0; j < ni; ++j) {
-      rc = ypass(s, partial + j * nspec, n, P);
-      if (rc) return rc;
-    }
-  } else {
-    float* meshes = wsp.take<float>((size_t)ni * ncell);
-    size_t pbytes = paint_workspace_bytes(n, np, order, ni);
-    void* pws = wsp.take<char>(pbytes);
-    if (!meshes || !partial || !ph || !comp || !pws)
-      return set_error(MP_ERR_WORKSPACE, "paint_spectrum: workspace of %zu bytes < %zu needed", ws_bytes,
-                       paint_spectrum_workspace_bytes(n, np, order, ni));
-    rc = paint_meshes(s, pos, pos_f64, w, np, g, order, ni, meshes, ncell, pws, pbytes, paint_path_option() == 3);
-    if (rc) return rc;
-    RealPrologue pro{};
-    for (int j = 0; j < ni; ++j) {
-      rc = rfft3d_zy(s, meshes + j * ncell, partial + j * nspec, n, P, pro);
-      if (rc) return rc;
-    }
-  }
-  const int total = (ni + 1) * len;
-  MP_LAUNCH("combine_tables", combine_tables_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, n[0], n[1], n[2], ni,
-            compensate ? order : 0, ph, comp);
-  CombineTables tab{};
-  tab.stride = len;
-  if (ni > 1) {
-    tab.phx = ph;
-    tab.phy = ph + n[0];
-    tab.phz = ph + n[0] + n[1];
-  }
-  if (compensate) {
-    tab.cx = comp;
-    tab.cy = comp + n[0];
-    tab.cz = comp + n[0] + n[1];
-  }
-  return combine_x(s, partial, nspec, ni, out, n, P, out_pitch, tab, 1.f / (float)ni);
+  const size_t pbytes = paint_partial_workspace_bytes(n, np, order, ni), cbytes = combine_partial_workspace_bytes(n, ni);
+  void* pws = wsp.take<char>(pbytes);
+  void* cws = wsp.take<char>(cbytes);
+  if (!partial || !pws || !cws)
+    return set_error(MP_ERR_WORKSPACE, "paint_spectrum: workspace of %zu bytes < %zu needed", ws_bytes,
+                     paint_spectrum_workspace_bytes(n, np, order, ni));
+  int rc = paint_partial(s, pos, pos_f64, w, np, g, order, ni, partial, P, pws, pbytes);
+  if (rc) return rc;
+  return combine_partial(s, partial, P, n, order, ni, compensate, 0, out, out_pitch, cws, cbytes);
 }
 
 // ------------------------------------------------------------------ FP64 reductions
@@ -195,6 +160,58 @@ MP_EXPORT int mp_paint_spectrum(void* stream, const void* positions, int positio
   GridMap g = make_map(meshsize, boxsize, boxcenter);
   return paint_spectrum((hipStream_t)stream, positions, positions_f64, weights, np, g, order, interlacing < 1 ? 1 : interlacing,
                         compensate, (float2*)out, out_pitch, workspace, workspace_bytes);
+}
+
+MP_EXPORT int64_t mp_paint_partial_workspace_bytes(const int32_t meshsize[3], int64_t np, int order, int interlacing) {
+  return (int64_t)paint_partial_workspace_bytes(meshsize, np, order, interlacing < 1 ? 1 : interlacing);
+}
+
+MP_EXPORT int mp_paint_partial(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+                               const int32_t meshsize[3], const double boxsize[3], const double boxcenter[3], int order,
+                               int interlacing, void* partial, int64_t pitch, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && boxsize && boxcenter && partial, "mp_paint_partial: null pointer");
+  MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
+  GridMap g = make_map(meshsize, boxsize, boxcenter);
+  return paint_partial((hipStream_t)stream, positions, positions_f64, weights, np, g, order, interlacing < 1 ? 1 : interlacing,
+                       (float2*)partial, pitch, workspace, workspace_bytes);
+}
+
+MP_EXPORT int64_t mp_combine_partial_workspace_bytes(const int32_t meshsize[3], int interlacing) {
+  return (int64_t)combine_partial_workspace_bytes(meshsize, interlacing < 1 ? 1 : interlacing);
+}
+
+MP_EXPORT int mp_combine_partial(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
+                                 int interlacing, int compensate, int zero_dc, void* out, int64_t out_pitch, void* workspace,
+                                 size_t workspace_bytes) {
+  MP_REQUIRE(meshsize, "mp_combine_partial: null meshsize");
+  MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
+  return combine_partial((hipStream_t)stream, (const float2*)partial, pitch, meshsize, order, interlacing < 1 ? 1 : interlacing,
+                         compensate, zero_dc, (float2*)out, out_pitch, workspace, workspace_bytes);
+}
+
+MP_EXPORT int64_t mp_bin_partial_workspace_bytes(const int32_t meshsize[3], int interlacing, int nprod, int nedges) {
+  return (int64_t)bin_partial_workspace_bytes(meshsize, interlacing < 1 ? 1 : interlacing, nprod, nedges);
+}
+
+MP_EXPORT int mp_bin_poles_partial(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
+                                   int interlacing, int compensate, int zero_dc, const double* kx, const double* ky,
+                                   const double* kz, const double* k2_thresholds, int nedges, const double los[3], int nprod,
+                                   const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out, void* workspace,
+                                   size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && k2_thresholds, "mp_bin_poles_partial: null pointer");
+  MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
+  MP_REQUIRE(nedges >= 2, "mp_bin_poles_partial: need at least 2 edges, got %d", nedges);
+  BinGeom g;
+  g.n0 = meshsize[0];
+  g.n1 = meshsize[1];
+  g.ncols = meshsize[2] / 2 + 1;
+  g.nedges = nedges;
+  g.kx = kx;
+  g.ky = ky;
+  g.kz = kz;
+  g.thr = k2_thresholds;
+  return bin_partial((hipStream_t)stream, (const float2*)partial, pitch, meshsize, order, interlacing < 1 ? 1 : interlacing,
+                     compensate, zero_dc, g, los, nprod, prod_ell, prod_remove_zero, out, workspace, workspace_bytes);
 }
 
 MP_EXPORT int64_t mp_ylm_workspace_bytes(const int32_t meshsize[3]) {

@@ -1,0 +1,485 @@
+// The x pass of an interlaced paint, with the interlacing combine folded in:
+//   A(kx, ky, kz) = comp(k) / n_i * sum_j exp(+i s_j (kx+ky+kz)_circ) FFT_x(partial_j)(kx, ky, kz)
+// (jaxpower/mesh.py:2739-2751, compensation resamplers.py:16-26).  Two consumers:
+//   * write: A is stored (the complex mesh of _paint(..., out='complex'));
+//   * bin:   A is never stored; |A|^2 w L_ell(mu) is summed into |k| shells in FP64, which is
+//            compute_mesh2_spectrum with a global line of sight on an auto spectrum
+//            (jaxpower/mesh2.py:346-369, BinMesh2SpectrumPoles.__call__ mesh2.py:100-119).
+//
+// Work decomposition: persistent blocks walk units (tile, shift j); a tile is C contiguous kz
+// columns of one ky row, for all L values of kx.  The global loads of the next unit are issued
+// before the FFT of the current one, so HBM latency overlaps the LDS work.  After the FFT each
+// thread owns one column and EPT consecutive kx of the tile for all shifts, so the combine
+// stays in registers, and in bin mode |k| is monotone along the thread's kx run: the FP64
+// shell sums stay in registers until the shell changes, then go to an LDS histogram that each
+// block writes once at the end.
+#include "bin.h"
+#include "common.h"
+#include "fft.h"
+#include "fft_core.h"
+#include "sph.h"
+
+#include <mutex>
+#include <vector>
+
+namespace mp {
+
+constexpr int XB_MAX_PROD = 6;
+
+struct XPassArgs {
+  const float2* in;
+  int64_t mesh_stride;  // between the partial spectra of consecutive shifts
+  int64_t fstride;      // between consecutive kx rows of the input
+  int64_t ostride;      // between consecutive ky rows of the input
+  int ni, ncols, ny, nchunks, ntiles;
+  CombineTables tab;
+  float scale;  // 1 / n_i
+  int zero_dc;  // the (0,0,0) mode of A is set to 0
+  const float2* tw;
+  // write mode
+  float2* out;
+  int64_t out_fstride, out_ostride;
+  // bin mode
+  BinGeom g;
+  int ell[XB_MAX_PROD], rz[XB_MAX_PROD];
+  float los[3];
+  int need_mu;
+  double* partial;  // [gridDim.x][nprod][nbins][2]
+};
+
+// Bytes of LDS the kernel needs beyond its tile: the per-kx and per-kz tables (TAB) and, in
+// bin mode, the thresholds and the histogram.
+__host__ __device__ inline size_t xpass_extra_lds(int L, int ncols, int ni, int nedges, int nprod, bool bin, bool tab) {
+  auto r8 = [](size_t b) { return (b + 7) & ~(size_t)7; };
+  size_t bytes = 0;
+  if (tab) {
+    bytes += r8((size_t)ni * L * 8) + r8((size_t)ni * ncols * 8) + r8((size_t)L * 4) + r8((size_t)ncols * 4);
+    if (bin) bytes += (size_t)L * 8 + (size_t)ncols * 8;
+  }
+  if (bin) bytes += (size_t)nedges * 8 + (size_t)nprod * (nedges - 1) * 8;
+  return bytes;
+}
+
+template <int L, bool BIN, int NP, bool TAB>
+__global__ void __launch_bounds__(XCfg<L>::NT)
+xpass_kernel(XPassArgs a) {
+  using Cfg = XCfg<L>;
+  constexpr int C = Cfg::C, NT = Cfg::NT;
+  constexpr int EPT = Cfg::EPT;
+  static_assert(EPT >= 1 && EPT * NT == C * L && (L / 2) % EPT == 0, "tile must split into monotone runs");
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int c = tid % C;           // this thread's column in the tile
+  const int l0 = (tid / C) * EPT;  // and its first kx
+  const int nedges = a.g.nedges, nbins = nedges - 1;
+  const bool phases = a.tab.phx != nullptr, comp = a.tab.cx != nullptr;
+  // tables indexed by kx or kz: copied to LDS (TAB) or read where they are
+  const float2* phx = a.tab.phx;
+  const float2* phz = a.tab.phz;
+  const float* cx = a.tab.cx;
+  const float* cz = a.tab.cz;
+  const double* kxt = a.g.kx;
+  const double* kzt = a.g.kz;
+  int phx_stride = a.tab.stride, phz_stride = a.tab.stride;
+  char* cur = reinterpret_cast<char*>(lds + C * L);
+  auto take = [&](size_t bytes) {
+    char* p = cur;
+    cur += (bytes + 7) & ~(size_t)7;
+    return p;
+  };
+  if constexpr (TAB) {
+    float2* sphx = reinterpret_cast<float2*>(take((size_t)a.ni * L * 8));
+    float2* sphz = reinterpret_cast<float2*>(take((size_t)a.ni * a.ncols * 8));
+    float* scx = reinterpret_cast<float*>(take((size_t)L * 4));
+    float* scz = reinterpret_cast<float*>(take((size_t)a.ncols * 4));
+    if (phases) {
+      for (int i = tid; i < a.ni * L; i += NT) sphx[i] = a.tab.phx[(i / L) * a.tab.stride + i % L];
+      for (int i = tid; i < a.ni * a.ncols; i += NT) sphz[i] = a.tab.phz[(i / a.ncols) * a.tab.stride + i % a.ncols];
+    }
+    if (comp) {
+      for (int i = tid; i < L; i += NT) scx[i] = a.tab.cx[i];
+      for (int i = tid; i < a.ncols; i += NT) scz[i] = a.tab.cz[i];
+    }
+    phx = sphx;
+    phz = sphz;
+    cx = scx;
+    cz = scz;
+    phx_stride = L;
+    phz_stride = a.ncols;
+    if constexpr (BIN) {
+      double* skx = reinterpret_cast<double*>(take((size_t)L * 8));
+      double* skz = reinterpret_cast<double*>(take((size_t)a.ncols * 8));
+      for (int i = tid; i < L; i += NT) skx[i] = a.g.kx[i];
+      for (int i = tid; i < a.ncols; i += NT) skz[i] = a.g.kz[i];
+      kxt = skx;
+      kzt = skz;
+    }
+  }
+  double* thr = reinterpret_cast<double*>(take(BIN ? (size_t)nedges * 8 : 0));
+  double* hist = reinterpret_cast<double*>(take(BIN ? (size_t)NP * nbins * 8 : 0));
+  if constexpr (BIN) {
+    for (int i = tid; i < nedges; i += NT) thr[i] = a.g.thr[i];
+    for (int i = tid; i < NP * nbins; i += NT) hist[i] = 0.;
+  }
+  float2 reg[EPT];
+  auto load = [&](int tile, int j) {
+    const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
+    const float2* src = a.in + j * a.mesh_stride + (int64_t)y * a.ostride + c0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = tid + e * NT, cc = idx % C, l = idx / C;
+      reg[e] = (c0 + cc < a.ncols) ? src[l * a.fstride + cc] : make_float2(0.f, 0.f);
+    }
+  };
+  float2 acc[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
+  int tile = blockIdx.x, j = 0;
+  if (tile < a.ntiles) load(tile, 0);
+  while (tile < a.ntiles) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];  // element (l, cc) at l * C + cc
+    __syncthreads();
+    int ntile = tile, nj = j + 1;
+    if (nj == a.ni) {
+      nj = 0;
+      ntile += gridDim.x;
+    }
+    if (ntile < a.ntiles) load(ntile, nj);
+    fft_lds<L, C, C, 1, true, NT, false>(lds, a.tw);
+    const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
+    const int kz = min(c0 + c, a.ncols - 1);
+    if (phases) {
+      const float2 pyz = cmul(a.tab.phy[j * a.tab.stride + y], phz[j * phz_stride + kz]);
+      const float2* px = phx + j * phx_stride + l0;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) acc[e] = cadd(acc[e], cmul(lds[(l0 + e) * C + c], cmul(px[e], pyz)));
+    } else {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) acc[e] = cadd(acc[e], lds[(l0 + e) * C + c]);
+    }
+    if (j == a.ni - 1) {
+      if (c0 + c < a.ncols) {
+        float s = a.scale;
+        if (comp) s *= a.tab.cy[y] * cz[kz];
+        if constexpr (!BIN) {
+          float2* dst = a.out + (int64_t)y * a.out_ostride + kz;
+#pragma unroll
+          for (int e = 0; e < EPT; ++e) {
+            const int l = l0 + e;
+            float2 v = cscale(acc[e], comp ? s * cx[l] : s);
+            if (a.zero_dc && l == 0 && y == 0 && kz == 0) v = make_float2(0.f, 0.f);
+            dst[l * a.out_fstride] = v;
+          }
+        } else {
+          const double kyv = a.g.ky[y], kzv = kzt[kz];
+          const float w = kz > 0 ? 2.f : 1.f;
+          const float fky = (float)kyv, fkz = (float)kzv;
+          const float lyz = fky * a.los[1] + fkz * a.los[2];
+          const float f2yz = fky * fky + fkz * fkz;
+          double sum[NP];
+#pragma unroll
+          for (int p = 0; p < NP; ++p) sum[p] = 0.;
+          int b = -1;
+#pragma unroll
+          for (int e = 0; e < EPT; ++e) {
+            const int l = l0 + e;
+            const double kxv = kxt[l];
+            const double k2 = k2_sum(k2_xy(kxv, kyv), kzv);
+            const int nb = e == 0 ? first_bin(thr, nedges, k2) : walk_bin(thr, nedges, b, k2);
+            if (nb != b) {
+              if (b >= 1 && b < nedges) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                  if (sum[p] != 0.) atomicAdd(&hist[p * nbins + b - 1], sum[p]);
+                  sum[p] = 0.;
+                }
+              }
+              b = nb;
+            }
+            if (b < 1 || b >= nedges) continue;
+            float2 A = cscale(acc[e], comp ? s * cx[l] : s);
+            const bool origin = (l == 0) & (y == 0) & (kz == 0);
+            if (origin && a.zero_dc) A = make_float2(0.f, 0.f);
+            const float p2 = A.x * A.x + A.y * A.y;
+            float mu = 0.f;
+            if (a.need_mu) {
+              const float fkx = (float)kxv;
+              const float kn2 = fkx * fkx + f2yz;
+              mu = kn2 > 0.f ? (fkx * a.los[0] + lyz) * rsqrtf(kn2) : 0.f;
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+              if (origin && a.rz[p]) continue;
+              const float f = w * (a.ell[p] ? legendre(a.ell[p], mu) : 1.f);
+              sum[p] += (double)(p2 * f);
+            }
+          }
+          if (b >= 1 && b < nedges) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+              if (sum[p] != 0.) atomicAdd(&hist[p * nbins + b - 1], sum[p]);
+          }
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    tile = ntile;
+    j = nj;
+  }
+  if constexpr (BIN) {
+    __syncthreads();
+    double* dst = a.partial + (int64_t)blockIdx.x * NP * nbins * 2;
+    for (int i = tid; i < NP * nbins; i += NT) {
+      dst[2 * i] = hist[i];
+      dst[2 * i + 1] = 0.;
+    }
+  }
+}
+
+// Per-axis interlacing phases exp(i s_j k_circ) and compensation sinc(k/2pi)^-order, in FP64
+// then rounded to FP32.  k_circ = 2 pi fftfreq(n) (rfftfreq on the last axis), mesh.py:1605-1636.
+__global__ void combine_tables_kernel(int n0, int n1, int n2, int ni, int order, float2* __restrict__ ph, float* __restrict__ comp) {
+  const int ncols = n2 / 2 + 1;
+  const int len = n0 + n1 + ncols;
+  const int total = (ni + 1) * len;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int j = t / len, r = t % len;
+  int axis = r < n0 ? 0 : (r < n0 + n1 ? 1 : 2);
+  int m = axis == 0 ? r : (axis == 1 ? r - n0 : r - n0 - n1);
+  int n = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+  int f = (axis == 2 || m < (n + 1) / 2) ? m : m - n;
+  double x = (double)f / (double)n;  // fftfreq value
+  if (j < ni) {
+    double s = (double)j / (double)ni;
+    double ang = 2.0 * M_PI * x * s;
+    ph[t] = make_float2((float)cos(ang), (float)sin(ang));
+  } else {
+    double cval = 1.0;
+    if (f != 0 && order > 0) {
+      double px = M_PI * x;
+      cval = pow(sin(px) / px, -(double)order);
+    }
+    comp[r] = (float)cval;
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+static int multiprocessors() {
+  static std::mutex mu;
+  static std::vector<int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+  if (!cache[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+    cache[dev] = n;
+  }
+  return cache[dev];
+}
+
+size_t xpass_tables_bytes(const int32_t n[3], int ni) {
+  const size_t len = (size_t)n[0] + n[1] + n[2] / 2 + 1;
+  return align_up((size_t)(ni + 1) * len * sizeof(float2), 256) + align_up(len * sizeof(float), 256);
+}
+
+// Builds the phase and compensation tables in the workspace.
+static int make_tables(hipStream_t s, const int32_t n[3], int order, int ni, int compensate, Workspace& wsp, CombineTables& tab) {
+  const int len = n[0] + n[1] + n[2] / 2 + 1;
+  float2* ph = wsp.take<float2>((size_t)(ni + 1) * len);
+  float* comp = wsp.take<float>((size_t)len);
+  if (!ph || !comp) return set_error(MP_ERR_WORKSPACE, "x pass: workspace too small for the tables");
+  const int total = (ni + 1) * len;
+  MP_LAUNCH("combine_tables", combine_tables_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, n[0], n[1], n[2], ni,
+            compensate ? order : 0, ph, comp);
+  tab = CombineTables{};
+  tab.stride = len;
+  if (ni > 1) {
+    tab.phx = ph;
+    tab.phy = ph + n[0];
+    tab.phz = ph + n[0] + n[1];
+  }
+  if (compensate) {
+    tab.cx = comp;
+    tab.cy = comp + n[0];
+    tab.cz = comp + n[0] + n[1];
+  }
+  return MP_OK;
+}
+
+template <int L, bool BIN, int NP, bool TAB>
+static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles) {
+  using Cfg = XCfg<L>;
+  auto kern = xpass_kernel<L, BIN, NP, TAB>;
+  a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
+  a.ntiles = a.nchunks * a.ny;
+  const size_t lds = Cfg::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, TAB);
+  if (lds > 160 * 1024) return set_error(MP_ERR_UNSUPPORTED, "x pass: %zu bytes of LDS needed", lds);
+  if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per = 0;
+  MP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, Cfg::NT, lds));
+  if (per < 1) per = 1;
+  int blocks = multiprocessors() * per;
+  if (blocks > a.ntiles) blocks = a.ntiles;
+  if (BIN) {
+    const size_t per_block = (size_t)NP * (a.g.nedges - 1) * 2;
+    const int cap = (int)(partial_doubles / per_block);
+    if (cap < 1) return set_error(MP_ERR_WORKSPACE, "x pass: no room for the partial histograms");
+    if (blocks > cap) blocks = cap;
+    a.partial = partial_ws;
+  }
+  *nblocks_out = blocks;
+  MP_LAUNCH(BIN ? "fft_xpass_bin" : "fft_xpass_combine", kern, dim3((unsigned)blocks), dim3(Cfg::NT), lds, s, a);
+  return MP_OK;
+}
+
+// The kx/kz tables go to LDS when they fit beside the tile.
+template <int L, bool BIN, int NP>
+static int launch_xpass(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles) {
+  const size_t with_tab = XCfg<L>::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, true);
+  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true>(s, a, nblocks_out, partial_ws, partial_doubles);
+  return launch_xpass_t<L, BIN, NP, false>(s, a, nblocks_out, partial_ws, partial_doubles);
+}
+
+template <int L>
+static int xpass_write_L(hipStream_t s, XPassArgs& a) {
+  int nb = 0;
+  return launch_xpass<L, false, 1>(s, a, &nb, nullptr, 0);
+}
+
+template <int L>
+static int xpass_bin_L(hipStream_t s, XPassArgs& a, int nprod, double* partial, size_t partial_doubles, int* nblocks) {
+  switch (nprod) {
+    case 1: return launch_xpass<L, true, 1>(s, a, nblocks, partial, partial_doubles);
+    case 2: return launch_xpass<L, true, 2>(s, a, nblocks, partial, partial_doubles);
+    case 3: return launch_xpass<L, true, 3>(s, a, nblocks, partial, partial_doubles);
+    case 4: return launch_xpass<L, true, 4>(s, a, nblocks, partial, partial_doubles);
+    case 5: return launch_xpass<L, true, 5>(s, a, nblocks, partial, partial_doubles);
+    case 6: return launch_xpass<L, true, 6>(s, a, nblocks, partial, partial_doubles);
+    default: return set_error(MP_ERR_INVALID, "x pass: %d products not in 1..%d", nprod, XB_MAX_PROD);
+  }
+}
+
+#define MP_XPASS_SWITCH(n, F, ...)                                                                      \
+  switch (n) {                                                                                          \
+    case 2: return F<2>(__VA_ARGS__);                                                                   \
+    case 4: return F<4>(__VA_ARGS__);                                                                   \
+    case 8: return F<8>(__VA_ARGS__);                                                                   \
+    case 16: return F<16>(__VA_ARGS__);                                                                 \
+    case 32: return F<32>(__VA_ARGS__);                                                                 \
+    case 64: return F<64>(__VA_ARGS__);                                                                 \
+    case 128: return F<128>(__VA_ARGS__);                                                               \
+    case 256: return F<256>(__VA_ARGS__);                                                               \
+    case 512: return F<512>(__VA_ARGS__);                                                               \
+    case 1024: return F<1024>(__VA_ARGS__);                                                             \
+    case 2048: return F<2048>(__VA_ARGS__);                                                             \
+    default: return set_error(MP_ERR_UNSUPPORTED, "x pass: length %d not a power of two in [2, 2048]", (int)(n)); \
+  }
+
+bool xpass_supported(const int32_t n[3]) {
+  for (int q = 0; q < 3; ++q)
+    if (!is_pow2(n[q]) || n[q] < 2 || n[q] > 2048) return false;
+  return true;
+}
+
+static void fill_common(XPassArgs& a, const float2* partial, int64_t pitch, const int32_t n[3], int ni, const CombineTables& tab,
+                        int zero_dc) {
+  a.in = partial;
+  a.ncols = n[2] / 2 + 1;
+  a.ny = n[1];
+  a.ostride = pitch;
+  a.fstride = (int64_t)n[1] * pitch;
+  a.mesh_stride = (int64_t)n[0] * n[1] * pitch;
+  a.ni = ni;
+  a.tab = tab;
+  a.scale = 1.f / (float)ni;
+  a.zero_dc = zero_dc;
+  a.tw = twiddle_device_ptr();
+}
+
+size_t combine_partial_workspace_bytes(const int32_t n[3], int ni) { return xpass_tables_bytes(n, ni) + 512; }
+
+int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
+                    int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes) {
+  MP_REQUIRE(partial && out, "combine: null pointer");
+  MP_REQUIRE(ni >= 1, "combine: interlacing %d < 1", ni);
+  if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "combine: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
+  MP_REQUIRE(pitch >= n[2] / 2 + 1 && out_pitch >= n[2] / 2 + 1, "combine: pitch < Nz/2+1");
+  int rc = ensure_twiddles();
+  if (rc) return rc;
+  Workspace wsp(ws, ws_bytes);
+  CombineTables tab;
+  rc = make_tables(s, n, order, ni, compensate, wsp, tab);
+  if (rc) return rc;
+  XPassArgs a{};
+  fill_common(a, partial, pitch, n, ni, tab, zero_dc);
+  if (!a.tw) return set_error(MP_ERR_HIP, "combine: twiddle table address");
+  a.out = out;
+  a.out_ostride = out_pitch;
+  a.out_fstride = (int64_t)n[1] * out_pitch;
+  MP_XPASS_SWITCH(n[0], xpass_write_L, s, a);
+}
+
+constexpr int XB_MAX_BLOCKS = 1024;  // partial histograms: the bin-mode grid is capped here
+
+size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges) {
+  const size_t blocks = XB_MAX_BLOCKS;
+  return xpass_tables_bytes(n, ni) + align_up(blocks * nprod * (nedges - 1) * 2 * sizeof(double), 256) + 512;
+}
+
+int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
+                int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
+                double* out, void* ws, size_t ws_bytes) {
+  MP_REQUIRE(partial && out && ell && remove_zero, "bin_partial: null pointer");
+  MP_REQUIRE(ni >= 1, "bin_partial: interlacing %d < 1", ni);
+  MP_REQUIRE(nprod >= 1 && nprod <= XB_MAX_PROD, "bin_partial: %d products not in 1..%d", nprod, XB_MAX_PROD);
+  if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "bin_partial: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
+  if (nprod * (g.nedges - 1) > BIN_LDS_DOUBLES) return set_error(MP_ERR_UNSUPPORTED, "bin_partial: too many bins for the LDS histogram");
+  MP_REQUIRE(pitch >= n[2] / 2 + 1, "bin_partial: pitch < Nz/2+1");
+  int rc = ensure_twiddles();
+  if (rc) return rc;
+  Workspace wsp(ws, ws_bytes);
+  CombineTables tab;
+  rc = make_tables(s, n, order, ni, compensate, wsp, tab);
+  if (rc) return rc;
+  const size_t pd = (size_t)XB_MAX_BLOCKS * nprod * (g.nedges - 1) * 2;
+  double* pbuf = wsp.take<double>(pd);
+  if (!pbuf) return set_error(MP_ERR_WORKSPACE, "bin_partial: workspace too small");
+  XPassArgs a{};
+  fill_common(a, partial, pitch, n, ni, tab, zero_dc);
+  if (!a.tw) return set_error(MP_ERR_HIP, "bin_partial: twiddle table address");
+  a.g = g;
+  a.need_mu = 0;
+  for (int p = 0; p < nprod; ++p) {
+    MP_REQUIRE(ell[p] >= 0 && ell[p] <= 10, "bin_partial: ell %d not in 0..10", ell[p]);
+    a.ell[p] = ell[p];
+    a.rz[p] = remove_zero[p];
+    if (ell[p]) a.need_mu = 1;
+  }
+  for (int q = 0; q < 3; ++q) a.los[q] = los ? (float)los[q] : 0.f;
+  int nblocks = 0;
+  switch (n[0]) {
+    case 2: rc = xpass_bin_L<2>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 4: rc = xpass_bin_L<4>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 8: rc = xpass_bin_L<8>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 16: rc = xpass_bin_L<16>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 32: rc = xpass_bin_L<32>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 64: rc = xpass_bin_L<64>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 128: rc = xpass_bin_L<128>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 256: rc = xpass_bin_L<256>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 512: rc = xpass_bin_L<512>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 1024: rc = xpass_bin_L<1024>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 2048: rc = xpass_bin_L<2048>(s, a, nprod, pbuf, pd, &nblocks); break;
+    default: return set_error(MP_ERR_UNSUPPORTED, "bin_partial: x length %d", n[0]);
+  }
+  if (rc) return rc;
+  return reduce_partials(s, pbuf, nblocks, nprod * (g.nedges - 1) * 2, out);
+}
+
+}  // namespace mp

@@ -374,7 +374,13 @@ class _UpdateRef:
         self.field, self.item = field, item
 
     def _op(self, name, other):
-        value = self.field.value.clone()
+        field = self.field
+        if (name == 'set' and isinstance(field, ComplexMeshField) and field._pending_combine() is not None
+                and _is_origin(self.item) and _is_zero_scalar(other)):
+            # mesh.at[0, 0, 0].set(0.) on a mesh whose interlacing combine is still pending:
+            # the combine (or its fused binning) zeroes the mode itself
+            return field._with_pending(field._pending_combine().with_zero_dc())
+        value = field.value.clone()
         other = other.value if isinstance(other, BaseMeshField) else other
         view = value[self.item]
         if name == 'set':
@@ -414,6 +420,24 @@ class _UpdateRef:
 
     def max(self, other):
         return self._op('max', other)
+
+
+def _is_origin(item):
+    if not isinstance(item, tuple) or len(item) != 3:
+        return False
+    try:
+        return all(int(i) == 0 and not isinstance(i, (slice, bool)) for i in item)
+    except TypeError:
+        return False
+
+
+def _is_zero_scalar(other):
+    if isinstance(other, torch.Tensor):
+        return other.numel() == 1 and not other.is_cuda and complex(other.item()) == 0
+    try:
+        return complex(other) == 0
+    except TypeError:
+        return False
 
 
 class _UpdateHelper:
@@ -606,8 +630,62 @@ def _read(mesh, positions, order):
     return out
 
 
+class _PendingCombine:
+    """The n_i interlaced meshes of _paint after their z and y FFT passes, waiting for the x
+    pass that combines them (mesh.py:2739-2751).  Holding them lets a global-line-of-sight
+    auto spectrum bin the combined mesh without ever writing it."""
+
+    def __init__(self, partial, pitch, order, interlacing, compensate, zero_dc=False):
+        self.partial, self.pitch = partial, int(pitch)
+        self.order, self.interlacing, self.compensate, self.zero_dc = int(order), int(interlacing), bool(compensate), bool(zero_dc)
+
+    def with_zero_dc(self):
+        return _PendingCombine(self.partial, self.pitch, self.order, self.interlacing, self.compensate, zero_dc=True)
+
+    def materialize(self, attrs):
+        device = self.partial.device
+        value = _empty_spectrum(attrs.meshsize, device)
+        n = _lib.i32x3(attrs.meshsize)
+        ws = _lib.workspace(_lib.load().mp_combine_partial_workspace_bytes(n, self.interlacing), device)
+        _lib.call('mp_combine_partial', _lib.stream_handle(device), _lib.ptr(self.partial), self.pitch, n, self.order,
+                  self.interlacing, int(self.compensate), int(self.zero_dc), _lib.ptr(value), _spectrum_pitch(value),
+                  _lib.ptr(ws), ws.numel())
+        return value
+
+
 class ComplexMeshField(BaseMeshField):
-    """Half-spectrum field (mesh.py:2118-2196)."""
+    """Half-spectrum field (mesh.py:2118-2196).
+
+    A mesh returned by an interlaced or compensated paint(out='complex') may hold its
+    interlaced meshes before their combine; `value` runs the combine on first access."""
+
+    def __getattr__(self, name):
+        if name == 'value':
+            pending = self.__dict__.get('_pending')
+            if pending is not None:
+                value = pending.materialize(self.__dict__['attrs'])
+                self.__dict__['value'] = value
+                del self.__dict__['_pending']
+                return value
+        raise AttributeError(f'{type(self).__name__!r} object has no attribute {name!r}')
+
+    @classmethod
+    def _from_pending(cls, pending, attrs):
+        new = cls.__new__(cls)
+        new.__dict__.update(attrs=attrs, _pending=pending)
+        return new
+
+    def _pending_combine(self):
+        return self.__dict__.get('_pending')
+
+    def _with_pending(self, pending):
+        return type(self)._from_pending(pending, self.attrs)
+
+    def clone(self, **kwargs):
+        pending = self._pending_combine()
+        if pending is not None and 'value' not in kwargs and 'attrs' not in kwargs and not kwargs:
+            return self._with_pending(pending)
+        return super().clone(**kwargs)
 
     @property
     def spacing(self):
@@ -799,12 +877,13 @@ def _paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compe
             mesh = mesh.r2c().apply(kernel_compensate)
             return mesh.c2r() if out == 'real' else mesh
         return mesh if out == 'real' else mesh.r2c()
-    value = _empty_spectrum(attrs.meshsize, device)
-    ws = _lib.workspace(_lib.load().mp_paint_spectrum_workspace_bytes(n, np_, order, ni), device)
-    _lib.call('mp_paint_spectrum', stream, _lib.ptr(pos), pos_f64, _lib.ptr(w), np_, n, box, center, order, ni,
-              int(bool(compensate)), _lib.ptr(value), _spectrum_pitch(value), _lib.ptr(ws), ws.numel())
+    pitch = int(_lib.load().mp_spectrum_pitch(int(attrs.meshsize[2])))
+    partial = torch.empty((ni, int(attrs.meshsize[0]), int(attrs.meshsize[1]), pitch), dtype=torch.complex64, device=device)
+    ws = _lib.workspace(_lib.load().mp_paint_partial_workspace_bytes(n, np_, order, ni), device)
+    _lib.call('mp_paint_partial', stream, _lib.ptr(pos), pos_f64, _lib.ptr(w), np_, n, box, center, order, ni,
+              _lib.ptr(partial), pitch, _lib.ptr(ws), ws.numel())
     del ws
-    mesh = ComplexMeshField(value, attrs=attrs)
+    mesh = ComplexMeshField._from_pending(_PendingCombine(partial, pitch, order, ni, compensate), attrs)
     if kernel_compensate is not None:
         mesh = mesh.apply(kernel_compensate)
     return mesh.c2r() if out == 'real' else mesh

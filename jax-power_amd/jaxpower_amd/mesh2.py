@@ -11,7 +11,7 @@ from . import _lib
 from .mesh import (MeshAttrs, BaseMeshField, RealMeshField, ComplexMeshField, ParticleField, FKPField, format_meshes,
                    compute_normalization, compute_box_normalization, _as_spectrum, _sum_f64, _as_device_tensor,
                    _default_device)
-from .types import Mesh2SpectrumPole, Mesh2SpectrumPoles
+from .types import Deferred, Mesh2SpectrumPole, Mesh2SpectrumPoles
 
 
 def _format_ells(ells):
@@ -101,6 +101,7 @@ class BinMesh2SpectrumPoles:
         t = self._tables
         _lib.call('mp_bin_modes', _lib.stream_handle(device), _lib.i32x3(mattrs.meshsize), _lib.ptr(t['kx']), _lib.ptr(t['ky']),
                   _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(edges), _lib.ptr(nmodes), _lib.ptr(ksum))
+        self._nmodes_dev = nmodes.to(torch.float64)
         self.nmodes = nmodes.cpu().numpy()
         with np.errstate(invalid='ignore', divide='ignore'):
             self.xavg = ksum.cpu().numpy() / self.nmodes
@@ -113,6 +114,10 @@ class BinMesh2SpectrumPoles:
     def products(self, meshes, products, los=(0., 0., 0.)):
         """Raw FP64 sums for products (i, j, ell, remove_zero) of conj(mesh_i) * mesh_j
         (i = -1: the constant 1) weighted by P_ell(k.los/|k|); returns complex (nprod, nbins)."""
+        return self._products_dev(meshes, products, los=los).cpu().numpy()
+
+    def _products_dev(self, meshes, products, los=(0., 0., 0.)):
+        """products() left on the device (complex128 tensor)."""
         device = meshes[0].device
         arrays, pitches = [], []
         for m in meshes:
@@ -129,8 +134,29 @@ class BinMesh2SpectrumPoles:
         _lib.call('mp_bin_poles', _lib.stream_handle(device), len(arrays), ptrs, pitch_arr, _lib.i32x3(self.mattrs.meshsize),
                   _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(self._edges1d),
                   _lib.f64x3(los), nprod, col(0), col(1), col(2), col(3), _lib.ptr(out), _lib.ptr(ws), ws.numel())
-        out = out.cpu().numpy()
-        return out[..., 0] + 1j * out[..., 1]
+        return torch.view_as_complex(out)
+
+    def _pending_products(self, pending, ells, remove_zero, los):
+        """_products_dev() of conj(A) A L_ell for the combined mesh A of a pending interlacing
+        combine, binned without writing A; None when the fused kernel does not cover the case."""
+        nprod = len(ells)
+        device = pending.partial.device
+        n = _lib.i32x3(self.mattrs.meshsize)
+        nedges = len(self._edges1d)
+        if nprod > 6 or nprod * (nedges - 1) > 4096:
+            return None
+        out = torch.empty((nprod, self.nbins, 2), dtype=torch.float64, device=device)
+        ws = _lib.workspace(_lib.load().mp_bin_partial_workspace_bytes(n, pending.interlacing, nprod, nedges), device)
+        col = lambda values: (_lib.ctypes.c_int32 * nprod)(*[int(v) for v in values])
+        t = self._tables
+        try:
+            _lib.call('mp_bin_poles_partial', _lib.stream_handle(device), _lib.ptr(pending.partial), pending.pitch, n,
+                      pending.order, pending.interlacing, int(pending.compensate), int(pending.zero_dc), _lib.ptr(t['kx']),
+                      _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), nedges, _lib.f64x3(los), nprod, col(ells),
+                      col(remove_zero), _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        except NotImplementedError:
+            return None
+        return torch.view_as_complex(out)
 
     def __call__(self, mesh, remove_zero=False):
         """Sum_{modes in bin} w * value / nmodes (mesh2.py:100-119)."""
@@ -203,23 +229,27 @@ def compute_mesh2_spectrum(*meshes, bin=None, los='z'):
                 bmeshes.append(C)
                 products.append((len(bmeshes) - 1, 0, 0, True))
                 scales.append(4. * np.pi)
-        sums = bin.products(bmeshes, products)
-        with np.errstate(invalid='ignore', divide='ignore'):
-            num = [s * scale / bin.nmodes for s, scale in zip(sums, scales)]
+        sums = bin._products_dev(bmeshes, products)
+        num = [sums[i] * (scale / bin._nmodes_dev) for i, scale in enumerate(scales)]
         if swap:
-            num = [np.conj(n) for n in num]
+            num = [torch.conj(n).resolve_conj() for n in num]
     else:  # fixed line of sight, mesh2.py:346-374
         A1 = _2c(meshes[0])
-        bmeshes = [A1.value]
-        j = 0
-        if not same:
-            bmeshes.append(_2c(meshes[1]).value)
-            j = 1
-        products = [(0, j, ell, ell != 0) for ell in ells]
-        sums = bin.products(bmeshes, products, los=vlos)
-        with np.errstate(invalid='ignore', divide='ignore'):
-            num = [s * (2 * ell + 1 if ell else 1) / bin.nmodes for s, ell in zip(sums, ells)]
-    num = [n.real if ell % 2 == 0 else n.imag for ell, n in zip(ells, num)]
+        sums = None
+        pending = A1._pending_combine() if same else None
+        if pending is not None:  # interlacing combine fused with the binning
+            sums = bin._pending_products(pending, ells, [ell != 0 for ell in ells], vlos)
+        if sums is None:
+            bmeshes = [A1.value]
+            j = 0
+            if not same:
+                bmeshes.append(_2c(meshes[1]).value)
+                j = 1
+            products = [(0, j, ell, ell != 0) for ell in ells]
+            sums = bin._products_dev(bmeshes, products, los=vlos)
+        num = [sums[i] * ((2 * ell + 1 if ell else 1) / bin._nmodes_dev) for i, ell in enumerate(ells)]
+    # left on the device: the spectrum reads them back on first access
+    num = [(n.real if ell % 2 == 0 else n.imag).contiguous() for ell, n in zip(ells, num)]
     poles = [Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num[i], num_shotnoise=0.,
                                norm=norm, volume=np.prod(mattrs.kfun) * bin.nmodes, ell=ell, attrs=attrs)
              for i, ell in enumerate(ells)]
@@ -246,7 +276,7 @@ def compute_mesh2(*meshes, bin=None, los='z'):
 
 def compute_box2_normalization(*inputs, bin=None):
     """(sum w)^2 / (N^3 Vcell) for a constant density (mesh2.py:505-513)."""
-    norm = float(compute_box_normalization(*format_meshes(*inputs, nmeshes=2)[0]))
+    norm = _deferred(compute_box_normalization(*format_meshes(*inputs, nmeshes=2)[0]))
     if bin is not None:
         return [norm] * len(bin.ells)
     return norm
@@ -268,7 +298,7 @@ def compute_fkp2_normalization(*fkps, bin=None, cellsize=10., split=None, fields
         alpha2 = fkps[0].data.sum() / fkps[0].randoms.sum()
         norm = norm + alpha2 * compute_normalization(fkps[1].data, fkps[0].randoms, **kwargs)
         norm = norm / 2
-    norm = float(norm)
+    norm = _deferred(norm)
     if bin is not None:
         return [norm] * len(bin.ells)
     return norm
@@ -283,9 +313,17 @@ def compute_fkp2_shotnoise(*fkps, bin=None, fields=None):
 
     if fields[1] == fields[0]:
         p = [get_particles(fkp) for fkp in fkps]
-        shotnoise = float(_sum_f64(p[0].weights, p[1].weights))
+        shotnoise = _sum_f64(p[0].weights, p[1].weights)
     else:
-        shotnoise = 0.
+        shotnoise = torch.zeros((), dtype=torch.float64, device=_default_device())
     if bin is not None:
-        return [shotnoise * (ell == 0) * np.ones_like(bin.xavg) for ell in bin.ells]
-    return shotnoise
+        ones = torch.ones(bin.nbins, dtype=torch.float64, device=shotnoise.device)
+        return [Deferred(shotnoise * ones) if ell == 0 else Deferred(0. * ones) for ell in bin.ells]
+    return Deferred(shotnoise)
+
+
+def _deferred(value):
+    """A float64 result left on the device until it is read (see types.Deferred)."""
+    if isinstance(value, torch.Tensor):
+        return Deferred(value.to(torch.float64))
+    return Deferred(torch.tensor(float(value), dtype=torch.float64))

@@ -2,20 +2,83 @@
 (``Mesh2SpectrumPole(s)``, constructed at jaxpower/mesh2.py:340-344, 370-374; registered at
 jaxpower/types.py:10-51, 93-94).
 
-A pole holds host float64 arrays: k (mode-averaged |k|), k_edges (nbins, 2), nmodes (int64),
+A pole holds float64 arrays: k (mode-averaged |k|), k_edges (nbins, 2), nmodes (int64),
 num_raw, num_shotnoise, norm, volume, and ``ell``.  ``value()`` is
-(num_raw - num_shotnoise) / norm.
+(num_raw - num_shotnoise) / norm.  Fields computed on the GPU are read back to the host on
+first access, so building and cloning a spectrum never waits for the device.
 """
 import numpy as np
 
 
+class Deferred:
+    """A float64 scalar or array computed on the GPU and read back on first use, so that the
+    estimator returns without waiting for the device (JAX arrays play this role in the
+    reference).  Converts like an array (``np.asarray``, ``float``) and supports arithmetic
+    through its host value."""
+
+    __array_ufunc__ = None  # numpy binary operators defer to the reflected methods below
+
+    def __init__(self, tensor):
+        self._tensor, self._host = tensor, None
+
+    def numpy(self):
+        if self._host is None:
+            self._host = self._tensor.detach().cpu().numpy()
+            self._tensor = None
+        return self._host
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.numpy()
+        return a.astype(dtype) if dtype is not None else a
+
+    def __float__(self):
+        return float(self.numpy())
+
+    def __len__(self):
+        return len(self.numpy())
+
+    def __getitem__(self, item):
+        return self.numpy()[item]
+
+    @property
+    def shape(self):
+        return self.numpy().shape
+
+    def __repr__(self):
+        return f'Deferred({self.numpy()!r})'
+
+
+def _binop(op, reflected=False):
+    def fn(self, other):
+        other = other.numpy() if isinstance(other, Deferred) else other
+        return op(other, self.numpy()) if reflected else op(self.numpy(), other)
+    return fn
+
+
+import operator as _operator  # noqa: E402
+
+for _name, _op in [('add', _operator.add), ('sub', _operator.sub), ('mul', _operator.mul), ('truediv', _operator.truediv),
+                   ('pow', _operator.pow), ('lt', _operator.lt), ('le', _operator.le), ('gt', _operator.gt),
+                   ('ge', _operator.ge), ('eq', _operator.eq), ('ne', _operator.ne)]:
+    setattr(Deferred, f'__{_name}__', _binop(_op))
+    if _name in ('add', 'sub', 'mul', 'truediv', 'pow'):
+        setattr(Deferred, f'__r{_name}__', _binop(_op, reflected=True))
+Deferred.__neg__ = lambda self: -self.numpy()
+Deferred.__abs__ = lambda self: np.abs(self.numpy())
+Deferred.__hash__ = None
+
+
 def _host(x):
+    if isinstance(x, Deferred):
+        return x.numpy()
     try:
         import torch
         if isinstance(x, torch.Tensor):
             return x.detach().cpu().numpy()
     except ImportError:  # pragma: no cover
         pass
+    if isinstance(x, (list, tuple)) and any(isinstance(v, Deferred) for v in x):
+        return np.asarray([_host(v) for v in x])
     return np.asarray(x)
 
 
@@ -26,17 +89,31 @@ class Mesh2SpectrumPole:
 
     def __init__(self, k=None, k_edges=None, nmodes=None, num_raw=None, num_shotnoise=None, norm=None, volume=None, ell=0,
                  attrs=None):
-        nbins = len(_host(k))
-        bcast = lambda v, dtype='f8': np.broadcast_to(_host(v).astype(dtype), (nbins,)).copy()
-        self._data = dict(k=_host(k).astype('f8'), k_edges=_host(k_edges).astype('f8'),
-                          nmodes=_host(nmodes).astype('i8') if np.issubdtype(_host(nmodes).dtype, np.integer) else bcast(nmodes),
-                          num_raw=bcast(num_raw), num_shotnoise=bcast(0. if num_shotnoise is None else num_shotnoise),
-                          norm=bcast(1. if norm is None else norm), volume=bcast(0. if volume is None else volume))
+        # fields may be device tensors or Deferred values: they are read back on first access
+        self._raw = dict(k=k, k_edges=k_edges, nmodes=nmodes, num_raw=num_raw, num_shotnoise=num_shotnoise, norm=norm,
+                         volume=volume)
+        self._cache = None
         self.ell = int(ell)
         self.attrs = dict(attrs or {})
 
+    @property
+    def _data(self):
+        if self._cache is None:
+            r = self._raw
+            k = _host(r['k']).astype('f8')
+            nbins = len(k)
+            bcast = lambda v, dtype='f8': np.broadcast_to(_host(v).astype(dtype), (nbins,)).copy()
+            nmodes = _host(r['nmodes'])
+            self._cache = dict(k=k, k_edges=_host(r['k_edges']).astype('f8'),
+                               nmodes=nmodes.astype('i8') if np.issubdtype(nmodes.dtype, np.integer) else bcast(nmodes),
+                               num_raw=bcast(r['num_raw']),
+                               num_shotnoise=bcast(0. if r['num_shotnoise'] is None else r['num_shotnoise']),
+                               norm=bcast(1. if r['norm'] is None else r['norm']),
+                               volume=bcast(0. if r['volume'] is None else r['volume']))
+        return self._cache
+
     def clone(self, **kwargs):
-        state = dict(self._data)
+        state = dict(self._raw if self._cache is None else self._cache)
         state.update(ell=self.ell, attrs=self.attrs)
         state.update(kwargs)
         return type(self)(**state)

@@ -133,3 +133,21 @@ def test_format_meshes():
     assert m == ['a', 'a'] and f == (0, 0)
     m, f = format_meshes('a', 'b', nmeshes=2)
     assert m == ['a', 'b'] and f == (0, 1)
+
+
+def test_deferred_values_in_containers():
+    # device results (here CPU tensors) and Deferred scalars are read back only on access
+    import torch
+    from jaxpower_amd import Mesh2SpectrumPole
+    from jaxpower_amd.types import Deferred
+    k = np.array([0.1, 0.2, 0.3])
+    norm = Deferred(torch.tensor(4., dtype=torch.float64))
+    sn = Deferred(torch.full((3,), 2., dtype=torch.float64))
+    pole = Mesh2SpectrumPole(k=torch.tensor(k), k_edges=np.column_stack([k - 0.05, k + 0.05]), nmodes=np.array([1, 2, 3]),
+                             num_raw=torch.tensor([10., 20., 30.], dtype=torch.float64), norm=1., ell=0)
+    assert pole._cache is None
+    pole = pole.clone(norm=norm, num_shotnoise=sn)
+    assert pole._cache is None and norm._host is None
+    assert np.allclose(pole.value(), (np.array([10., 20., 30.]) - 2.) / 4.)
+    assert float(norm) == 4. and np.isclose(norm, 4.) and norm * 2 == 8. and 1. / norm == 0.25 and np.allclose(sn + 1., 3.)
+    assert np.asarray(sn).shape == (3,) and len(sn) == 3 and sn[1] == 2.
