@@ -62,7 +62,7 @@ def algorithmic_bytes(n=MESHSIZE, np_=NPART, ni=NI, nprod=len(ELLS)):
         'paint_rowsort_fine': np_ * 18 + np_ * 16,         # records + keys in, sorted records out
         'paint_zfft': np_ * 16 + ni * H * 8,               # sorted records in, n_i z-transformed spectra out
         'fft_zpass_r2c': ncell * 4 + H * 8,
-        'fft_ypass': 2 * H * 8,
+        'fft_ypass': ni * 2 * H * 8,                      # the n_i partial spectra, in place, one launch
         'fft_xpass_combine': ni * H * 8 + H * 8,
         'fft_xpass_bin': ni * H * 8,                      # n_i partial spectra in, per-bin sums out
         'bin_poles': H * 8,

@@ -86,6 +86,26 @@ int mp_bin_poles_partial(void* stream, const void* partial, int64_t pitch, const
                          const double* kz, const double* k2_thresholds, int nedges, const double los[3], int nprod,
                          const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out, void* workspace,
                          size_t workspace_bytes);
+/* ---- Backward (d loss / d positions, weights) ---------------------------------------- */
+/* jax.grad of num_raw through the binning of compute_mesh2_spectrum (jaxpower/mesh2.py:
+ * 346-369) and the interlacing combine (jaxpower/mesh.py:2739-2751), transposed by hand:
+ * given coef[p][b] = d loss / d sums[p][b] (sums as written by mp_bin_poles_partial), writes
+ * the `interlacing` half spectra Y_j whose c2r (mp_irfft3d) is d loss / d delta_j, the
+ * derivative with respect to the j-th shifted painted mesh.  A is the combined spectrum
+ * (mp_combine_partial with the same order, interlacing, compensate and zero_dc). */
+int64_t mp_bin_adjoint_workspace_bytes(const int32_t meshsize[3], int interlacing);
+int mp_bin_adjoint(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], int order, int interlacing,
+                   int compensate, int zero_dc, const double* kx, const double* ky, const double* kz,
+                   const double* k2_thresholds, int nedges, const double los[3], int nprod, const int32_t* prod_ell,
+                   const int32_t* prod_remove_zero, const double* coef, void* Y, void* workspace, size_t workspace_bytes);
+/* Transpose of resamplers.paint (jaxpower/resamplers.py:73-103) with respect to weights and
+ * positions: grad_weights[i] = sum_j sum_stencil D_j K, grad_positions[i][a] =
+ * w_i sum_j sum_stencil D_j dK/dx_a (physical units; d round/floor = 0).  meshes holds the
+ * `interlacing` real meshes D_j contiguously. */
+int mp_read_grad(void* stream, const float* meshes, int interlacing, const int32_t meshsize[3], const double boxsize[3],
+                 const double boxcenter[3], const void* positions, int positions_f64, const float* weights, int64_t np,
+                 int order, float* grad_weights, float* grad_positions);
+
 /* Paint path of mp_paint_spectrum: 0 automatic (fused paint + z-FFT in LDS when the shape
  * allows), 1 tiled paint then z pass, 2 fused only, 3 per-particle atomics then z pass.
  * Returns the previous setting; a negative path only queries.  No counterpart in the

@@ -76,8 +76,10 @@ zpass_r2c_kernel(const float* __restrict__ in, float2* __restrict__ out, int64_t
   using Cfg = RowCfg<M>;
   constexpr int C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
   extern __shared__ float2 lds[];
+  __shared__ float2 stw[2 * M];
   const int tid = threadIdx.x;
   const int64_t row0 = (int64_t)blockIdx.x * C;
+  load_twiddles<2 * M, NT>(stw, g_twiddle);
   const float2* in2 = reinterpret_cast<const float2*>(in);
   for (int idx = tid; idx < C * M; idx += NT) {
     int c = idx / M, l = idx % M;
@@ -93,13 +95,13 @@ zpass_r2c_kernel(const float* __restrict__ in, float2* __restrict__ out, int64_t
     lds[c * LP + l] = v;
   }
   __syncthreads();
-  fft_lds<M, C, 1, LP, false, NT, false>(lds, g_twiddle);
+  fft_lds<M, C, 1, LP, false, NT, false, 2 * M>(lds, stw);
   constexpr int NK = M / 2 + 1;
   for (int idx = tid; idx < C * NK; idx += NT) {
     int c = idx / NK, k = idx % NK;
     int64_t row = row0 + c;
     if (row >= nrows) continue;
-    rfft_post(lds + c * LP, out + row * out_pitch, M, k, g_twiddle);
+    rfft_post<2 * M>(lds + c * LP, out + row * out_pitch, M, k, stw);
   }
 }
 
@@ -111,8 +113,11 @@ zpass_c2r_kernel(const float2* __restrict__ in, int64_t in_pitch, float* __restr
   using Cfg = RowCfg<M>;
   constexpr int C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
   extern __shared__ float2 lds[];
+  __shared__ float2 stw[2 * M];
   const int tid = threadIdx.x;
   const int64_t row0 = (int64_t)blockIdx.x * C;
+  load_twiddles<2 * M, NT>(stw, g_twiddle);
+  __syncthreads();
   constexpr int NK = M / 2 + 1;
   for (int idx = tid; idx < C * NK; idx += NT) {
     int c = idx / NK, k = idx % NK;
@@ -126,7 +131,7 @@ zpass_c2r_kernel(const float2* __restrict__ in, int64_t in_pitch, float* __restr
       } else {
         float2 xk = x[k], xmk = cconj(x[M - k]);
         float2 E = cscale(cadd(xk, xmk), 0.5f);
-        float2 O = cmul(cscale(csub(xk, xmk), 0.5f), cconj(g_twiddle[k * (TW_N / (2 * M))]));
+        float2 O = cmul(cscale(csub(xk, xmk), 0.5f), cconj(stw[k]));
         zk = make_float2(E.x - O.y, E.y + O.x);        // E + i O
         zmk = make_float2(E.x + O.y, -E.y + O.x);      // conj(E) + i conj(O)
       }
@@ -135,7 +140,7 @@ zpass_c2r_kernel(const float2* __restrict__ in, int64_t in_pitch, float* __restr
     if (k != 0 && 2 * k != M) lds[c * LP + (M - k)] = zmk;
   }
   __syncthreads();
-  fft_lds<M, C, 1, LP, false, NT, true>(lds, g_twiddle);
+  fft_lds<M, C, 1, LP, false, NT, true, 2 * M>(lds, stw);
   float2* out2 = reinterpret_cast<float2*>(out);
   for (int idx = tid; idx < C * M; idx += NT) {
     int c = idx / M, l = idx % M;
@@ -155,15 +160,17 @@ col_fft_kernel(const float2* in, float2* out, int ncols, int64_t in_fstride, int
   using Cfg = ColCfg<L>;
   constexpr int C = Cfg::C, NT = Cfg::NT;
   extern __shared__ float2 lds[];
+  __shared__ float2 stw[L];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * C;
+  load_twiddles<L, NT>(stw, g_twiddle);
   const float2* src = in + (int64_t)blockIdx.y * in_ostride + c0;
   for (int idx = tid; idx < C * L; idx += NT) {
     int c = idx % C, l = idx / C;
     lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<L, C, C, 1, true, NT, INV>(lds, g_twiddle);
+  fft_lds<L, C, C, 1, true, NT, INV, L>(lds, stw);
   float2* dst = out + (int64_t)blockIdx.y * out_ostride + c0;
   for (int idx = tid; idx < C * L; idx += NT) {
     int c = idx % C, l = idx / C;
@@ -187,7 +194,9 @@ colpass_kernel(ColPassArgs a) {
   using Cfg = XCfg<L>;
   constexpr int C = Cfg::C, NT = Cfg::NT, EPT = Cfg::EPT;
   extern __shared__ float2 lds[];
+  __shared__ float2 stw[L];
   const int tid = threadIdx.x;
+  load_twiddles<L, NT>(stw, a.tw);
   float2 reg[EPT];
   auto base = [&](int tile) {
     const int chunk = tile % a.nchunks, rest = tile / a.nchunks;
@@ -211,7 +220,7 @@ colpass_kernel(ColPassArgs a) {
     __syncthreads();
     const int next = tile + gridDim.x;
     if (next < a.ntiles) load(next);
-    fft_lds<L, C, C, 1, true, NT, INV>(lds, a.tw);
+    fft_lds<L, C, C, 1, true, NT, INV, L>(lds, stw);
     float2* dst = base(tile);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -231,16 +240,18 @@ xpass_ylm_acc_kernel(const float2* __restrict__ in, float2* __restrict__ acc, in
   using Cfg = ColCfg<L>;
   constexpr int C = Cfg::C, NT = Cfg::NT;
   extern __shared__ float2 lds[];
+  __shared__ float2 stw[L];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * C;
   const int y = blockIdx.y;
+  load_twiddles<L, NT>(stw, g_twiddle);
   const float2* src = in + (int64_t)y * in_ostride + c0;
   for (int idx = tid; idx < C * L; idx += NT) {
     int c = idx % C, l = idx / C;
     lds[l * C + c] = (c0 + c < ncols) ? src[l * in_fstride + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<L, C, C, 1, true, NT, false>(lds, g_twiddle);
+  fft_lds<L, C, C, 1, true, NT, false, L>(lds, stw);
   float2* dst = acc + (int64_t)y * acc_ostride + c0;
   for (int idx = tid; idx < C * L; idx += NT) {
     int c = idx % C, l = idx / C;

@@ -92,12 +92,13 @@ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 constexpr int first_radix(int L) { return ilog2(L) % 3 == 0 ? 8 : (ilog2(L) % 3 == 1 ? 2 : 4); }
 
 // One Stockham stage over C independent length-L transforms held in LDS; element (l, c)
-// lives at buf[l * LS + c * CS].  CFAST: consecutive threads take consecutive c.
-template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int R, int NS>
+// lives at buf[l * LS + c * CS].  CFAST: consecutive threads take consecutive c.  tw is a
+// table of TWN twiddles exp(-2 pi i e / TWN) (TWN a multiple of L).
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int R, int NS, int TWN>
 __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict__ tw) {
   constexpr int NB = C * (L / R);
   constexpr int BPT = (NB + NT - 1) / NT;
-  constexpr int TWSTEP = TW_N / (NS * R);
+  constexpr int TWSTEP = TWN / (NS * R);
   float2 v[BPT][R];
   const int tid = threadIdx.x;
 #pragma unroll
@@ -135,23 +136,36 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict_
   __syncthreads();
 }
 
-template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int NS>
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int NS, int TWN>
 __device__ __forceinline__ void fft_stages(float2* buf, const float2* __restrict__ tw) {
   if constexpr (NS < L) {
     constexpr int R = NS == 1 ? first_radix(L) : 8;
-    fft_stage<L, C, LS, CS, CFAST, NT, INV, R, NS>(buf, tw);
-    fft_stages<L, C, LS, CS, CFAST, NT, INV, NS * R>(buf, tw);
+    fft_stage<L, C, LS, CS, CFAST, NT, INV, R, NS, TWN>(buf, tw);
+    fft_stages<L, C, LS, CS, CFAST, NT, INV, NS * R, TWN>(buf, tw);
   }
 }
 
-// Full in-place FFT of C columns in LDS (callers sync before; this syncs after).
-template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV>
+// Full in-place FFT of C columns in LDS (callers sync before; this syncs after), with a
+// TWN-entry twiddle table (normally an LDS copy made by load_twiddles).
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int TWN = TW_N>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
-  fft_stages<L, C, LS, CS, CFAST, NT, INV, 1>(buf, tw);
+  static_assert(TWN % L == 0, "twiddle table must hold the L-th roots of unity");
+  fft_stages<L, C, LS, CS, CFAST, NT, INV, 1, TWN>(buf, tw);
+}
+
+// Copies the TWN-entry table exp(-2 pi i e / TWN) out of the global TW_N-entry table (the
+// caller synchronizes before use).  Twiddles read from LDS keep global-memory latency out of
+// every butterfly stage.
+template <int TWN, int NT>
+__device__ __forceinline__ void load_twiddles(float2* stw, const float2* __restrict__ gtw) {
+  static_assert(TW_N % TWN == 0, "twiddle table length must divide TW_N");
+  for (int i = threadIdx.x; i < TWN; i += NT) stw[i] = gtw[i * (TW_N / TWN)];
 }
 
 // Post-processing of the r2c z pass for output index k in [0, M/2] of one row: the row holds
 // Z = FFT_M(x[2m] + i x[2m+1]); writes X[k] and X[M-k] of the N = 2M point real transform.
+// tw: table of TWN twiddles, TWN a multiple of 2M.
+template <int TWN = TW_N>
 __device__ __forceinline__ void rfft_post(const float2* row, float2* o, int M, int k, const float2* __restrict__ tw) {
   float2 zk = row[k];
   if (k == 0) {
@@ -163,7 +177,7 @@ __device__ __forceinline__ void rfft_post(const float2* row, float2* o, int M, i
   float2 E = cscale(cadd(zk, zmk), 0.5f);
   float2 D = cscale(csub(zk, zmk), 0.5f);
   float2 O = make_float2(D.y, -D.x);                // -i * D
-  float2 t = cmul(tw[k * (TW_N / (2 * M))], O);    // exp(-2 pi i k / (2M)) * O
+  float2 t = cmul(tw[k * (TWN / (2 * M))], O);     // exp(-2 pi i k / (2M)) * O
   o[k] = cadd(E, t);
   if (2 * k != M) o[M - k] = cconj(csub(E, t));
 }

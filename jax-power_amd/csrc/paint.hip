@@ -17,6 +17,7 @@
 // Meshes whose axes do not tile (size not a multiple of 4, or between 17 and 31) use a
 // per-particle kernel with float atomics on a zero-filled mesh.
 #include "paint.h"
+#include "resampler.h"
 
 namespace mp {
 
@@ -28,19 +29,6 @@ constexpr int MAX_TILED_NI = 3;
 constexpr int SCAN_NT = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_BLOCK = SCAN_NT * SCAN_ITEMS;
-
-template <int ORDER>
-__host__ __device__ __forceinline__ float kern1d(float s) {
-  if constexpr (ORDER == 1) {
-    return 1.f;
-  } else if constexpr (ORDER == 2) {
-    return 1.f - s;
-  } else if constexpr (ORDER == 3) {
-    return s <= 0.5f ? 0.75f - s * s : 0.5f * (1.5f - s) * (1.5f - s);
-  } else {
-    return s <= 1.f ? (4.f - 6.f * s * s + 3.f * s * s * s) * (1.f / 6.f) : (2.f - s) * (2.f - s) * (2.f - s) * (1.f / 6.f);
-  }
-}
 
 __device__ __forceinline__ int gcd_u32(unsigned a, unsigned b) {
   while (b) {
@@ -57,22 +45,6 @@ __device__ __forceinline__ int coprime_stride(int total) {
   unsigned s = ((unsigned)((double)total * 0.6180339887498949)) | 1u;
   while (gcd_u32(s, (unsigned)total) != 1) s += 2;
   return (int)(s % (unsigned)total);
-}
-
-__host__ __device__ __forceinline__ int pos_mod(int a, int n) {
-  int r = a % n;
-  return r < 0 ? r + n : r;
-}
-
-template <typename T>
-__device__ __forceinline__ T round_or_floor(T u, bool odd) {
-  return odd ? rint(u) : floor(u);
-}
-
-template <typename T>
-__device__ __forceinline__ void to_grid(const T* __restrict__ pos, int64_t i, const GridMap& g, T xg[3]) {
-#pragma unroll
-  for (int a = 0; a < 3; ++a) xg[a] = (pos[3 * i + a] + (T)g.shift[a]) / (T)g.cell[a];
 }
 
 __global__ void fill_f32_kernel(float* __restrict__ p, int64_t n, float v) {

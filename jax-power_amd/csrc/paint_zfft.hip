@@ -280,7 +280,9 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __shared__ float swabs[PZ_MAX_SEG];
   __shared__ float sbound[1];
   __shared__ float tbound[NROW];
+  __shared__ float2 stw[NZ];
   const int tid = threadIdx.x;
+  load_twiddles<NZ, PZ_NT>(stw, tw);
 
   // XCD-aware work index: blocks sharing an x strip (hence most of their particles) and the
   // shifts of one block run on the same XCD, i.e. share an L2.
@@ -414,14 +416,14 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __syncthreads();
 
   // z-axis r2c of the NROW rows held in LDS (row r: complex elements lds2[r*LPC + l])
-  fft_lds<M, NROW, 1, LPC, false, PZ_NT, false>(lds2, tw);
+  fft_lds<M, NROW, 1, LPC, false, PZ_NT, false, NZ>(lds2, stw);
   constexpr int NK = M / 2 + 1;
   float2* dst = out + j * mesh_stride;
   for (int idx = tid; idx < NROW * NK; idx += PZ_NT) {
     int r = idx / NK, kk = idx % NK;
     int rx = r / RY, ry = r % RY;
     float2* o = dst + ((int64_t)(x0 + rx) * rg.n[1] + (y0 + ry)) * pitch;
-    rfft_post(lds2 + r * LPC, o, M, kk, tw);
+    rfft_post<NZ>(lds2 + r * LPC, o, M, kk, stw);
   }
 }
 
@@ -431,7 +433,7 @@ template <int M>
 struct PZCfg;
 template <> struct PZCfg<1024> { static constexpr int RX = 4, RY = 4; };
 template <> struct PZCfg<512> { static constexpr int RX = 8, RY = 4; };
-template <> struct PZCfg<256> { static constexpr int RX = 8, RY = 8; };
+template <> struct PZCfg<256> { static constexpr int RX = 8, RY = 4; };
 template <> struct PZCfg<128> { static constexpr int RX = 16, RY = 8; };
 template <> struct PZCfg<64> { static constexpr int RX = 16, RY = 16; };
 template <> struct PZCfg<32> { static constexpr int RX = 16, RY = 16; };

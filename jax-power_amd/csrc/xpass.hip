@@ -68,7 +68,9 @@ xpass_kernel(XPassArgs a) {
   constexpr int EPT = Cfg::EPT;
   static_assert(EPT >= 1 && EPT * NT == C * L && (L / 2) % EPT == 0, "tile must split into monotone runs");
   extern __shared__ float2 lds[];
+  __shared__ float2 stw[L];
   const int tid = threadIdx.x;
+  load_twiddles<L, NT>(stw, a.tw);
   const int c = tid % C;           // this thread's column in the tile
   const int l0 = (tid / C) * EPT;  // and its first kx
   const int nedges = a.g.nedges, nbins = nedges - 1;
@@ -146,7 +148,7 @@ xpass_kernel(XPassArgs a) {
       ntile += gridDim.x;
     }
     if (ntile < a.ntiles) load(ntile, nj);
-    fft_lds<L, C, C, 1, true, NT, false>(lds, a.tw);
+    fft_lds<L, C, C, 1, true, NT, false, L>(lds, stw);
     const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
     const int kz = min(c0 + c, a.ncols - 1);
     if (phases) {
@@ -290,7 +292,7 @@ size_t xpass_tables_bytes(const int32_t n[3], int ni) {
 }
 
 // Builds the phase and compensation tables in the workspace.
-static int make_tables(hipStream_t s, const int32_t n[3], int order, int ni, int compensate, Workspace& wsp, CombineTables& tab) {
+int combine_tables(hipStream_t s, const int32_t n[3], int order, int ni, int compensate, Workspace& wsp, CombineTables& tab) {
   const int len = n[0] + n[1] + n[2] / 2 + 1;
   float2* ph = wsp.take<float2>((size_t)(ni + 1) * len);
   float* comp = wsp.take<float>((size_t)len);
@@ -415,7 +417,7 @@ int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const i
   if (rc) return rc;
   Workspace wsp(ws, ws_bytes);
   CombineTables tab;
-  rc = make_tables(s, n, order, ni, compensate, wsp, tab);
+  rc = combine_tables(s, n, order, ni, compensate, wsp, tab);
   if (rc) return rc;
   XPassArgs a{};
   fill_common(a, partial, pitch, n, ni, tab, zero_dc);
@@ -446,7 +448,7 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   if (rc) return rc;
   Workspace wsp(ws, ws_bytes);
   CombineTables tab;
-  rc = make_tables(s, n, order, ni, compensate, wsp, tab);
+  rc = combine_tables(s, n, order, ni, compensate, wsp, tab);
   if (rc) return rc;
   const size_t pd = (size_t)XB_MAX_BLOCKS * nprod * (g.nedges - 1) * 2;
   double* pbuf = wsp.take<double>(pd);

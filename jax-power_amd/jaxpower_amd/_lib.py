@@ -43,6 +43,11 @@ SIGNATURES = {
     'mp_bin_partial_workspace_bytes': (c_int64, [P_I32, c_int, c_int, c_int]),
     'mp_bin_poles_partial': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_int, P_F64, c_int, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
+    'mp_bin_adjoint_workspace_bytes': (c_int64, [P_I32, c_int]),
+    'mp_bin_adjoint': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_int, P_F64, c_int, P_I32, P_I32, c_void_p, c_void_p, c_void_p, c_size_t]),
+    'mp_read_grad': (c_int, [c_void_p, c_void_p, c_int, P_I32, P_F64, P_F64, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p,
+                             c_void_p]),
     'mp_read': (c_int, [c_void_p, c_void_p, P_I32, P_F64, P_F64, c_void_p, c_int, c_int64, c_int, c_void_p]),
     'mp_spectrum_pitch': (c_int64, [ctypes.c_int32]),
     'mp_rfft3d': (c_int, [c_void_p, c_void_p, c_void_p, P_I32, c_int64]),

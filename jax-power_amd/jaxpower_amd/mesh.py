@@ -850,6 +850,21 @@ for _name in ['meshsize', 'boxsize', 'boxcenter', 'dtype', 'fft_backend', 'cells
     setattr(ParticleField, _name, property(lambda self, _n=_name: getattr(self.attrs, _n)))
 
 
+def _paint_partial(attrs, positions, weights, order, ni, compensate):
+    """The n_i interlaced meshes after their z and y FFT passes (mp_paint_partial)."""
+    device = positions.device
+    pos = positions.contiguous()
+    n = _lib.i32x3(attrs.meshsize)
+    np_ = pos.shape[0]
+    pitch = int(_lib.load().mp_spectrum_pitch(int(attrs.meshsize[2])))
+    partial = torch.empty((ni, int(attrs.meshsize[0]), int(attrs.meshsize[1]), pitch), dtype=torch.complex64, device=device)
+    ws = _lib.workspace(_lib.load().mp_paint_partial_workspace_bytes(n, np_, order, ni), device)
+    _lib.call('mp_paint_partial', _lib.stream_handle(device), _lib.ptr(pos), int(pos.dtype == torch.float64), _lib.ptr(weights),
+              np_, n, _lib.f64x3(attrs.boxsize), _lib.f64x3(attrs.boxcenter), order, ni, _lib.ptr(partial), pitch, _lib.ptr(ws),
+              ws.numel())
+    return _PendingCombine(partial, pitch, order, ni, compensate)
+
+
 def _paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compensate=False, out='real'):
     """_paint, mesh.py:2673-2754, single device."""
     resampler = resamplers.get_resampler(resampler)
@@ -877,13 +892,7 @@ def _paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compe
             mesh = mesh.r2c().apply(kernel_compensate)
             return mesh.c2r() if out == 'real' else mesh
         return mesh if out == 'real' else mesh.r2c()
-    pitch = int(_lib.load().mp_spectrum_pitch(int(attrs.meshsize[2])))
-    partial = torch.empty((ni, int(attrs.meshsize[0]), int(attrs.meshsize[1]), pitch), dtype=torch.complex64, device=device)
-    ws = _lib.workspace(_lib.load().mp_paint_partial_workspace_bytes(n, np_, order, ni), device)
-    _lib.call('mp_paint_partial', stream, _lib.ptr(pos), pos_f64, _lib.ptr(w), np_, n, box, center, order, ni,
-              _lib.ptr(partial), pitch, _lib.ptr(ws), ws.numel())
-    del ws
-    mesh = ComplexMeshField._from_pending(_PendingCombine(partial, pitch, order, ni, compensate), attrs)
+    mesh = ComplexMeshField._from_pending(_paint_partial(attrs, pos, w, order, ni, compensate), attrs)
     if kernel_compensate is not None:
         mesh = mesh.apply(kernel_compensate)
     return mesh.c2r() if out == 'real' else mesh
