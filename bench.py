@@ -44,6 +44,8 @@ def parse():
                    help='write the HBM traffic file from two rocprofv3 --pmc output folders and exit')
     p.add_argument('--summarize-trace', metavar='TRACE_DIR',
                    help='write the kernel statistics summary from a rocprofv3 --kernel-trace --stats folder and exit')
+    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024'],
+                   help='box512: the headline plain box; fkp1024: survey FKP field, 1024^3, local line of sight')
     p.add_argument('--paint-path', type=int, default=None,
                    help='0 automatic, 1 tiled paint + z pass, 2 fused paint + z-FFT, 3 atomics + z pass')
     return p.parse_args()
@@ -211,8 +213,78 @@ def cpu_baseline(repeats=2):
                       f'and particles), best of {repeats} runs {dt:.1f} s (total {sum(times):.1f} s); value = sample rate / 8'}
 
 
+def run_fkp1024(args):
+    """Survey FKP measurement: 5e6 data and 1e8 randoms uniform in the shell 1000 <= r < 2000
+    Mpc/h within 60 degrees of the z axis (observer at the origin), FKP weights
+    1/(1 + nbar 1e4), 1024^3 mesh with boxpad 2, TSC with 3-fold interlacing, compensated,
+    los='firstpoint', ells 0, 2, 4 (real spherical harmonics), k step 0.005, FKP normalization
+    on the cellsize-10 regridded mesh and shot noise.  One step = paint + spectrum +
+    normalization + shot noise.  Prints one JSON line."""
+    import numpy as np
+    import torch
+    from jaxpower_amd import (ParticleField, FKPField, BinMesh2SpectrumPoles, compute_mesh2_spectrum, get_mesh_attrs,
+                              compute_fkp2_normalization, compute_fkp2_shotnoise, _lib)
+    from jaxpower_amd.mock import generate_shell_cap_positions, shell_cap_volume
+    torch.cuda.set_device(0)
+    nd, nr = 5_000_000, 100_000_000
+    dpos = generate_shell_cap_positions(nd, seed=args.seed, device='cuda')
+    rpos = generate_shell_cap_positions(nr, seed=args.seed + 1, device='cuda')
+    nbar = nd / shell_cap_volume()
+    wfkp = 1. / (1. + nbar * 1e4)
+    attrs = get_mesh_attrs(dpos.cpu().numpy()[::50], rpos.cpu().numpy()[::1000], boxpad=2., meshsize=1024)
+    data = ParticleField(dpos, torch.full((nd,), wfkp, device='cuda'), attrs=attrs)
+    randoms = ParticleField(rpos, torch.full((nr,), wfkp, device='cuda'), attrs=attrs)
+    fkp = FKPField(data, randoms)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': STEP}, ells=ELLS)
+    torch.cuda.synchronize()
+    print(f'# fkp setup done: boxsize {attrs.boxsize[0]:.1f}, {bin.nbins} bins', file=sys.stderr, flush=True)
+
+    def step():
+        mesh = fkp.paint(resampler='tsc', interlacing=NI, compensate=True, out='complex')
+        spec = compute_mesh2_spectrum(mesh, bin=bin, los='firstpoint')
+        return spec.clone(norm=compute_fkp2_normalization(fkp, bin=bin), num_shotnoise=compute_fkp2_shotnoise(fkp, bin=bin))
+
+    for _ in range(args.warmup):
+        spec = step()
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    lib.mp_timing_reset()
+    lib.mp_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        spec = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.mp_timing_enable(0)
+    buf = _lib.ctypes.create_string_buffer(1 << 16)
+    lib.mp_timing_report(buf, len(buf))
+    kernels = {}
+    for line in buf.value.decode().splitlines():
+        name, total, count = line.split()
+        kernels[name] = {'avg_ms': round(float(total) / int(count), 4), 'launches_per_step': int(count) / args.steps,
+                         'share': round(float(total) / (elapsed * 1e3), 4)}
+    p0 = spec.get(0).value()
+    result = {'metric': 'FKP P_ell(k) local line of sight, measurements/s', 'value': round(args.steps / elapsed, 4),
+              'unit': 'measurements/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+              'ms_per_step': round(elapsed * 1e3 / args.steps, 3), 'higher_is_better': True, 'scaling': 'weak',
+              'vs_baseline': None, 'dtype': 'f32 (f64 bin sums)',
+              'data': 'synthetic: 5e6 data and 1e8 randoms uniform in a shell-cap, FKP weights',
+              'config': {'workload': 'FKP survey field, 1024^3, local line of sight (firstpoint)', 'meshsize': 1024,
+                         'boxsize': float(attrs.boxsize[0]), 'data': nd, 'randoms': nr, 'resampler': 'tsc', 'interlacing': NI,
+                         'compensate': True, 'los': 'firstpoint', 'ells': list(ELLS), 'kstep': STEP, 'nbins': bin.nbins},
+              'kernels': kernels,
+              'check': {'p0_finite': bool(np.all(np.isfinite(p0[1:]))), 'p0_first_bins': [float(x) for x in p0[1:4]]}}
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
+    if args.workload == 'fkp1024':
+        if args.gpus != 1:
+            print('bench.py runs on one GPU only.', file=sys.stderr)
+            sys.exit(2)
+        run_fkp1024(args)
+        return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace)
         return

@@ -79,14 +79,16 @@ struct ColCfg {
 
 // Tiles of the pipelined column passes: C columns x L rows, EPT = C*L/NT elements per thread
 // (a whole number for every power of two L in [2, 2048]).
-template <int L>
-struct XCfg {
-  static constexpr int C = L <= 512 ? (L >= 4 ? 16 : 64 / L) : 8192 / L;
+template <int L, int CC>
+struct XCfgC {
+  static constexpr int C = CC;
   static constexpr int NT0 = C * L / 8;
   static constexpr int NT = NT0 < 64 ? 64 : NT0;
   static constexpr int EPT = C * L / NT;
   static constexpr size_t LDS = (size_t)C * L * sizeof(float2);
 };
+template <int L>
+using XCfg = XCfgC<L, L <= 512 ? (L >= 4 ? 16 : 64 / L) : 8192 / L>;
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 constexpr int first_radix(int L) { return ilog2(L) % 3 == 0 ? 8 : (ilog2(L) % 3 == 1 ? 2 : 4); }
@@ -165,15 +167,17 @@ __device__ __forceinline__ void load_twiddles(float2* stw, const float2* __restr
 // Post-processing of the r2c z pass for output index k in [0, M/2] of one row: the row holds
 // Z = FFT_M(x[2m] + i x[2m+1]); writes X[k] and X[M-k] of the N = 2M point real transform.
 // tw: table of TWN twiddles, TWN a multiple of 2M.
+// Element m of the row is row[m * stride].
 template <int TWN = TW_N>
-__device__ __forceinline__ void rfft_post(const float2* row, float2* o, int M, int k, const float2* __restrict__ tw) {
-  float2 zk = row[k];
+__device__ __forceinline__ void rfft_post(const float2* row, float2* o, int M, int k, const float2* __restrict__ tw,
+                                          int stride = 1) {
+  float2 zk = row[k * stride];
   if (k == 0) {
     o[0] = make_float2(zk.x + zk.y, 0.f);
     o[M] = make_float2(zk.x - zk.y, 0.f);
     return;
   }
-  float2 zmk = cconj(row[M - k]);
+  float2 zmk = cconj(row[(M - k) * stride]);
   float2 E = cscale(cadd(zk, zmk), 0.5f);
   float2 D = cscale(csub(zk, zmk), 0.5f);
   float2 O = make_float2(D.y, -D.x);                // -i * D

@@ -20,8 +20,10 @@ namespace mp {
 
 constexpr int ZFRAC = 20;                 // fractional bits of z in the record (Nz <= 2048)
 constexpr int PZ_NT = 1024;
-constexpr int PZ_MAX_SEG = 512;
-constexpr size_t PZ_LDS_BUDGET = 132 * 1024 + 1024;
+// Segment table (home rows of a block's neighbourhood): the reach of a stencil is at most
+// p + 1 <= 5 rows per axis, so (RX + 5) (RY + 5) rounded to a wave.
+__host__ __device__ constexpr int pz_seg_capacity(int rx, int ry) { return ((rx + 5) * (ry + 5) + 63) / 64 * 64; }
+constexpr size_t PZ_LDS_BUDGET = 140 * 1024;  // dynamic part; the static tables add at most 20 KiB
 
 int exclusive_scan_i32(hipStream_t s, int* data, int64_t n, int* tmp, int* copy);
 size_t scan_workspace_bytes(int64_t n);
@@ -268,16 +270,20 @@ __global__ void __launch_bounds__(PZ_NT)
 paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ offsets, const float* __restrict__ row_wabs, RowGeom rg,
                   float2* __restrict__ out, int64_t pitch, int64_t mesh_stride, const float2* __restrict__ tw) {
   constexpr int NZ = 2 * M;
-  constexpr int LPC = M + 1;       // complex row pitch in LDS
+  // LDS layout: complex element l (reals z = 2l, 2l+1) of row r at lds2[l * RP + r], rows
+  // fastest with one pad column, so the column FFT is bank-conflict free and a stencil add at
+  // (r, z) lands in bank (z + 2r) mod 64
+  constexpr int RP = RX * RY + 1;
   constexpr int NROW = RX * RY;
   constexpr int OMIN = -((ORDER - 1) / 2);
   constexpr int CHUNK = 4;  // particles per work item, loaded together before any is painted
   extern __shared__ float2 lds2[];
   float* rows = reinterpret_cast<float*>(lds2);
-  __shared__ int sstart[PZ_MAX_SEG];
-  __shared__ int scount[PZ_MAX_SEG];
-  __shared__ int spre[PZ_MAX_SEG + 2];
-  __shared__ float swabs[PZ_MAX_SEG];
+  constexpr int SEG = pz_seg_capacity(RX, RY);
+  __shared__ int sstart[SEG];
+  __shared__ int scount[SEG];
+  __shared__ int spre[SEG + 2];
+  __shared__ float swabs[SEG];
   __shared__ float sbound[1];
   __shared__ float tbound[NROW];
   __shared__ float2 stw[NZ];
@@ -297,16 +303,22 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   const int x0 = bx * RX, y0 = by * RY;
   const float s = (float)j / (float)rg.ni;
 
-  for (int i = tid; i < NROW * LPC; i += PZ_NT) lds2[i] = make_float2(0.f, 0.f);
+  for (int i = tid; i < M * RP; i += PZ_NT) lds2[i] = make_float2(0.f, 0.f);
 
   // neighbourhood rows: homes in [x0 - hi, x0 + RX - 1 - lo]; each row's particles are
   // contiguous and sorted by z cell.  Work items are chunks of up to CHUNK consecutive
   // particles of one row.
-  const int sx = RX + rg.hi - rg.lo, sy = RY + rg.hi - rg.lo;
+  // rows reached from home row h at this shift: h + [lo, hi].  The stencil origin of x~ + s,
+  // x~ in [h, h+1), is h or h+1 for s < 1/2 with round, h+1 or h+2 for s > 1/2; at s = 1/2
+  // float rounding of x~ + s can give any of the three.  With floor it is h, or h and h+1 for
+  // s > 0.
+  const int lo = OMIN + ((ORDER % 2) && 2 * j > rg.ni ? 1 : 0);
+  const int hi = (ORDER - 1 + OMIN) + ((ORDER % 2) ? (2 * j >= rg.ni ? 2 : 1) : (j > 0 ? 1 : 0));
+  const int sx = RX + hi - lo, sy = RY + hi - lo;
   const int nseg = sx * sy;
   for (int i = tid; i < nseg; i += PZ_NT) {
     int ix = i / sy, iy = i % sy;
-    int hx = pmod(x0 - rg.hi + ix, rg.n[0]), hy = pmod(y0 - rg.hi + iy, rg.n[1]);
+    int hx = pmod(x0 - hi + ix, rg.n[0]), hy = pmod(y0 - hi + iy, rg.n[1]);
     int64_t key = (int64_t)hx * rg.n[1] + hy;
     int st = offsets[key];
     int c = offsets[key + 1] - st;
@@ -317,7 +329,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   }
   __syncthreads();
   if (tid < 64) {
-    constexpr int PER = PZ_MAX_SEG / 64;
+    constexpr int PER = SEG / 64;
     int local[PER];
     int sum = 0;
 #pragma unroll
@@ -344,7 +356,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   // bound on |cell value| for the fixed-point accumulation: for each row of the block, the sum
   // of |w| over the home rows that can reach it; the block uses the largest
   if (tid < NROW) {
-    const int rx = tid / RY, ry = tid % RY, span = rg.hi - rg.lo;
+    const int rx = tid / RY, ry = tid % RY, span = hi - lo;
     float sum = 0.f;
     for (int a = 0; a <= span; ++a)
       for (int bb = 0; bb <= span; ++bb) sum += swabs[(rx + a) * sy + (ry + bb)];
@@ -376,7 +388,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     const int seg = lo_i;
     const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
     const int cnt = min(CHUNK, sstart[seg] + scount[seg] - p0);
-    const int hxr = seg / sy - rg.hi, hyr = seg % sy - rg.hi;  // home row relative to the block
+    const int hxr = seg / sy - hi, hyr = seg % sy - hi;  // home row relative to the block
     float4 recs[CHUNK];
 #pragma unroll
     for (int q = 0; q < CHUNK; ++q) recs[q] = q < cnt ? sorted[p0 + q] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -399,10 +411,13 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
 #pragma unroll
           for (int oy = 0; oy < ORDER; ++oy) {
             if (!(cy[oy] >= 0 && cy[oy] < RY)) continue;
-            int* row = irows + (cx[ox] * RY + cy[oy]) * (NZ + 2);
+            int* row = irows + 2 * (cx[ox] * RY + cy[oy]);
             const float kxy = rec.w * kx[ox] * ky[oy] * scale;
 #pragma unroll
-            for (int oz = 0; oz < ORDER; ++oz) atomicAdd(row + (cz[oz] & (NZ - 1)), __float2int_rn(kxy * kz[oz]));
+            for (int oz = 0; oz < ORDER; ++oz) {
+              const int z = cz[oz] & (NZ - 1);
+              atomicAdd(row + 2 * RP * (z >> 1) + (z & 1), __float2int_rn(kxy * kz[oz]));
+            }
           }
         }
       }
@@ -411,19 +426,19 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __syncthreads();
   {
     const float inv = 1.f / scale;
-    for (int i = tid; i < NROW * 2 * LPC; i += PZ_NT) rows[i] = (float)irows[i] * inv;
+    for (int i = tid; i < M * RP * 2; i += PZ_NT) rows[i] = (float)irows[i] * inv;
   }
   __syncthreads();
 
-  // z-axis r2c of the NROW rows held in LDS (row r: complex elements lds2[r*LPC + l])
-  fft_lds<M, NROW, 1, LPC, false, PZ_NT, false, NZ>(lds2, stw);
+  // z-axis r2c of the NROW rows held in LDS
+  fft_lds<M, NROW, RP, 1, true, PZ_NT, false, NZ>(lds2, stw);
   constexpr int NK = M / 2 + 1;
   float2* dst = out + j * mesh_stride;
   for (int idx = tid; idx < NROW * NK; idx += PZ_NT) {
     int r = idx / NK, kk = idx % NK;
     int rx = r / RY, ry = r % RY;
     float2* o = dst + ((int64_t)(x0 + rx) * rg.n[1] + (y0 + ry)) * pitch;
-    rfft_post<NZ>(lds2 + r * LPC, o, M, kk, stw);
+    rfft_post<NZ>(lds2 + r, o, M, kk, stw, RP);
   }
 }
 
@@ -471,7 +486,7 @@ bool paint_zfft_supported(const int32_t n[3], int64_t np, int order, int ni) {
   reach_of(order, ni, lo, hi);
   // neighbourhood rows must not alias through the periodic wrap, and fit the segment table
   if (rx + hi - lo > n[0] || ry + hi - lo > n[1]) return false;
-  return (rx + hi - lo) * (ry + hi - lo) <= PZ_MAX_SEG;
+  return (rx + hi - lo) * (ry + hi - lo) <= pz_seg_capacity(rx, ry);
 }
 
 size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np) {
@@ -489,7 +504,7 @@ template <int ORDER, int M>
 static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs, const RowGeom& rg, float2* out,
                      int64_t pitch, int64_t mesh_stride) {
   constexpr int RX = PZCfg<M>::RX, RY = PZCfg<M>::RY;
-  constexpr size_t lds = (size_t)RX * RY * (M + 1) * sizeof(float2);
+  constexpr size_t lds = (size_t)M * (RX * RY + 1) * sizeof(float2);
   static_assert(lds <= PZ_LDS_BUDGET, "LDS budget");
   auto kern = paint_zfft_kernel<ORDER, M, RX, RY>;
   if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

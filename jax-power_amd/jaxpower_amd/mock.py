@@ -93,3 +93,24 @@ def generate_lognormal_particles(mattrs, nbar=None, size=None, power=None, seed=
         perm = torch.randperm(pos.shape[0], generator=g, device=device)
         pos = pos[perm]
     return ParticleField(pos, attrs=mattrs)
+
+
+def generate_shell_cap_positions(size, rmin=1000., rmax=2000., cap_degrees=60., seed=42, dtype=torch.float32, device=None):
+    """(size, 3) positions uniform in volume in the shell rmin <= r < rmax, within cap_degrees of
+    the +z axis, with the observer at the origin (the survey geometry of an FKP measurement;
+    no counterpart in the reference)."""
+    device = device or _default_device()
+    g = _generator(seed, device)
+    u = torch.rand((int(size), 3), generator=g, device=device, dtype=torch.float64)
+    r = (rmin**3 + u[:, 0] * (rmax**3 - rmin**3))**(1. / 3.)
+    cmin = np.cos(np.radians(cap_degrees))
+    cost = cmin + u[:, 1] * (1. - cmin)
+    sint = torch.sqrt(torch.clamp(1. - cost**2, min=0.))
+    phi = 2. * np.pi * u[:, 2]
+    pos = torch.stack([r * sint * torch.cos(phi), r * sint * torch.sin(phi), r * cost], dim=1)
+    return pos.to(dtype)
+
+
+def shell_cap_volume(rmin=1000., rmax=2000., cap_degrees=60.):
+    """Volume of the shell-cap region of generate_shell_cap_positions."""
+    return 2. * np.pi * (1. - np.cos(np.radians(cap_degrees))) * (rmax**3 - rmin**3) / 3.
