@@ -44,8 +44,9 @@ def parse():
                    help='write the HBM traffic file from two rocprofv3 --pmc output folders and exit')
     p.add_argument('--summarize-trace', metavar='TRACE_DIR',
                    help='write the kernel statistics summary from a rocprofv3 --kernel-trace --stats folder and exit')
-    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024'],
-                   help='box512: the headline plain box; fkp1024: survey FKP field, 1024^3, local line of sight')
+    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024', 'batch256'],
+                   help='box512: the headline plain box; fkp1024: survey FKP field, 1024^3, local line of sight; '
+                        'batch256: 64 lognormal mocks at 256^3 with uneven counts, batched (HIP graph) vs per-mock calls')
     p.add_argument('--paint-path', type=int, default=None,
                    help='0 automatic, 1 tiled paint + z pass, 2 fused paint + z-FFT, 3 atomics + z pass')
     return p.parse_args()
@@ -277,8 +278,62 @@ def run_fkp1024(args):
     print(json.dumps(result))
 
 
+def run_batch256(args):
+    """64 lognormal mocks (the 512^3 recipe at 256^3, boxsize 1000) with particle counts drawn
+    uniformly in [5e5, 4e6], CIC and TSC, 2-fold interlacing, compensated, DC removed, ells
+    0, 2, 4, k step 0.005, box normalization and shot noise: mocks/s of the batch replayed as
+    one captured HIP graph, and of the same sequence as per-mock eager calls.  One step = the
+    64 mocks.  Prints one JSON line."""
+    import numpy as np
+    import torch
+    from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, Mesh2SpectrumBatch, generate_lognormal_particles, _lib
+    torch.cuda.set_device(0)
+    nmock = 64
+    attrs = MeshAttrs(meshsize=256, boxsize=1000., boxcenter=500.)
+    counts = np.random.default_rng(args.seed).integers(500_000, 4_000_001, nmock)
+    fields = [generate_lognormal_particles(attrs, size=int(c), seed=args.seed + 1 + i) for i, c in enumerate(counts)]
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': STEP}, ells=ELLS)
+    torch.cuda.synchronize()
+    print(f'# batch setup done: {sum(f.size for f in fields)} particles in {nmock} mocks', file=sys.stderr, flush=True)
+    out = {}
+    for resampler in ('cic', 'tsc'):
+        rates = {}
+        for mode, graph in (('per_mock', False), ('batched_graph', True)):
+            batch = Mesh2SpectrumBatch(fields, bin, resampler=resampler, interlacing=2, compensate=True, graph=graph)
+            for _ in range(max(args.warmup, 1)):
+                specs = batch()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                specs = batch()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            rates[mode] = {'mocks_per_s': round(nmock * args.steps / el, 2), 'ms_per_mock': round(el * 1e3 / (nmock * args.steps), 4)}
+            print(f'# {resampler} {mode}: {rates[mode]}', file=sys.stderr, flush=True)
+            del batch
+        rates['speedup'] = round(rates['batched_graph']['mocks_per_s'] / rates['per_mock']['mocks_per_s'], 3)
+        out[resampler] = rates
+    buf = _lib.ctypes.create_string_buffer(1 << 14)
+    _lib.load().mp_paint_tuning_report(buf, len(buf))
+    result = {'metric': 'mocks/s, 64 mocks at 256^3 (batched vs per-mock)', 'value': out['tsc']['batched_graph']['mocks_per_s'],
+              'unit': 'mocks/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+              'ms_per_step': round(1e3 * nmock / out['tsc']['batched_graph']['mocks_per_s'], 3), 'higher_is_better': True,
+              'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 (f64 bin sums)',
+              'data': 'synthetic: 64 seeded lognormal Poisson mocks, counts uniform in [5e5, 4e6]',
+              'config': {'workload': '64 lognormal mocks, 256^3, boxsize 1000, interlacing 2, compensated, los z', 'meshsize': 256,
+                         'mocks': nmock, 'particles': int(sum(f.size for f in fields)), 'ells': list(ELLS), 'kstep': STEP},
+              'results': out, 'tuning': buf.value.decode().strip().splitlines()}
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
+    if args.workload == 'batch256':
+        if args.gpus != 1:
+            print('bench.py runs on one GPU only.', file=sys.stderr)
+            sys.exit(2)
+        run_batch256(args)
+        return
     if args.workload == 'fkp1024':
         if args.gpus != 1:
             print('bench.py runs on one GPU only.', file=sys.stderr)
@@ -381,6 +436,9 @@ def main():
         'kernels': table,
         'check': {'p0_finite': bool(np.all(np.isfinite(p0))), 'p0_first_bins': [float(x) for x in p0[1:4]]},
     }
+    tbuf = _lib.ctypes.create_string_buffer(1 << 14)
+    _lib.load().mp_paint_tuning_report(tbuf, len(tbuf))
+    result['tuning'] = tbuf.value.decode().strip().splitlines()
     if not args.no_cpu_baseline:
         print('# cpu baseline ...', file=sys.stderr, flush=True)
         result['cpu_baseline'] = cpu_baseline()

@@ -111,6 +111,14 @@ int mp_read_grad(void* stream, const float* meshes, int interlacing, const int32
  * Returns the previous setting; a negative path only queries.  No counterpart in the
  * reference (an implementation choice). */
 int mp_set_paint_path(int path);
+/* Autotuner of the fused paint (no counterpart in the reference): on the first paint of each
+ * (meshsize, order, interlacing) case, the row-block shapes that fit are timed on the actual
+ * particles and the fastest is kept for the process (skipped under graph capture and below
+ * 2^18 particles).  enable >= 0 sets it (default 1; 0 takes the first shape that fits;
+ * 100 + q forces candidate q where it fits, for tests); returns the previous setting.  The
+ * report writes one line per tuned case ("... rows=RXxRY ms: ...") and returns their count. */
+int mp_set_paint_tuning(int enable);
+int mp_paint_tuning_report(char* buf, int64_t buflen);
 
 /* resamplers.read via RealMeshField.read (jaxpower/resamplers.py:106-131,
  * jaxpower/mesh.py:2065-2115): out[i] = sum_stencil mesh * K. */

@@ -28,4 +28,9 @@ int paint_zfft(hipStream_t s, const void* pos, int pos_f64, const float* w, int6
 // paint + z pass, 2 fused only (error when unsupported), 3 per-particle atomics + z pass.
 int paint_path_option();
 
+// Autotuner of the fused paint's row-block shape (paint_zfft.hip): enable >= 0 sets it,
+// returns the previous setting; the report lists each tuned case with the timings.
+int paint_tuning_option(int enable);
+int paint_tuning_report(char* buf, int64_t len);
+
 }  // namespace mp

@@ -16,6 +16,11 @@
 #include "fft_core.h"
 #include "paint.h"
 
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
 namespace mp {
 
 constexpr int ZFRAC = 20;                 // fractional bits of z in the record (Nz <= 2048)
@@ -444,16 +449,30 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
 
 // ------------------------------------------------------------------ host side
 
-template <int M>
-struct PZCfg;
-template <> struct PZCfg<1024> { static constexpr int RX = 4, RY = 4; };
-template <> struct PZCfg<512> { static constexpr int RX = 8, RY = 4; };
-template <> struct PZCfg<256> { static constexpr int RX = 8, RY = 4; };
-template <> struct PZCfg<128> { static constexpr int RX = 16, RY = 8; };
-template <> struct PZCfg<64> { static constexpr int RX = 16, RY = 16; };
-template <> struct PZCfg<32> { static constexpr int RX = 16, RY = 16; };
-template <> struct PZCfg<16> { static constexpr int RX = 16, RY = 16; };
-template <> struct PZCfg<8> { static constexpr int RX = 16, RY = 16; };
+// Row-block shapes (RX, RY) tried per z half-length M; the first is the default.  Each fits
+// M (RX RY + 1) complex in at most 140 KiB of LDS; the half-size shapes let two blocks share
+// a CU.
+struct PzShape {
+  int rx, ry;
+};
+static const PzShape kPz1024[] = {{4, 4}, {4, 2}};
+static const PzShape kPz512[] = {{8, 4}, {4, 4}};
+static const PzShape kPz256[] = {{8, 4}, {8, 8}, {4, 4}};
+static const PzShape kPz128[] = {{8, 8}, {16, 8}, {8, 4}};
+static const PzShape kPz64[] = {{16, 8}, {16, 16}, {8, 8}};
+static const PzShape kPzSmall[] = {{16, 16}, {16, 8}, {8, 8}};
+
+static const PzShape* pz_candidates(int M, int& count) {
+  switch (M) {
+    case 1024: count = 2; return kPz1024;
+    case 512: count = 2; return kPz512;
+    case 256: count = 3; return kPz256;
+    case 128: count = 3; return kPz128;
+    case 64: count = 3; return kPz64;
+    case 32: case 16: case 8: count = 3; return kPzSmall;
+    default: count = 0; return nullptr;
+  }
+}
 
 static void reach_of(int order, int ni, int& lo, int& hi) {
   int omin = -((order - 1) / 2), omax = order - 1 - (order - 1) / 2;
@@ -462,31 +481,23 @@ static void reach_of(int order, int ni, int& lo, int& hi) {
   hi = i0max + omax;  // offsets are stored strictly below 1, so no margin is needed
 }
 
-static bool pz_rows(int M, int& rx, int& ry) {
-  switch (M) {
-    case 1024: rx = PZCfg<1024>::RX; ry = PZCfg<1024>::RY; return true;
-    case 512: rx = PZCfg<512>::RX; ry = PZCfg<512>::RY; return true;
-    case 256: rx = PZCfg<256>::RX; ry = PZCfg<256>::RY; return true;
-    case 128: rx = PZCfg<128>::RX; ry = PZCfg<128>::RY; return true;
-    case 64: rx = PZCfg<64>::RX; ry = PZCfg<64>::RY; return true;
-    case 32: rx = PZCfg<32>::RX; ry = PZCfg<32>::RY; return true;
-    case 16: rx = PZCfg<16>::RX; ry = PZCfg<16>::RY; return true;
-    case 8: rx = PZCfg<8>::RX; ry = PZCfg<8>::RY; return true;
-    default: return false;
-  }
+static bool pz_shape_ok(const int32_t n[3], const PzShape& sh, int order, int ni) {
+  if (n[0] % sh.rx || n[1] % sh.ry) return false;
+  int lo, hi;
+  reach_of(order, ni, lo, hi);
+  // neighbourhood rows must not alias through the periodic wrap, and fit the segment table
+  if (sh.rx + hi - lo > n[0] || sh.ry + hi - lo > n[1]) return false;
+  return (sh.rx + hi - lo) * (sh.ry + hi - lo) <= pz_seg_capacity(sh.rx, sh.ry);
 }
 
 bool paint_zfft_supported(const int32_t n[3], int64_t np, int order, int ni) {
   if (order < 1 || order > 4 || ni < 1 || np >= (int64_t)INT32_MAX) return false;
   if (!is_pow2(n[2])) return false;
-  int rx, ry;
-  if (!pz_rows(n[2] / 2, rx, ry)) return false;
-  if (n[0] % rx || n[1] % ry) return false;
-  int lo, hi;
-  reach_of(order, ni, lo, hi);
-  // neighbourhood rows must not alias through the periodic wrap, and fit the segment table
-  if (rx + hi - lo > n[0] || ry + hi - lo > n[1]) return false;
-  return (rx + hi - lo) * (ry + hi - lo) <= pz_seg_capacity(rx, ry);
+  int count = 0;
+  const PzShape* c = pz_candidates(n[2] / 2, count);
+  for (int q = 0; q < count; ++q)
+    if (pz_shape_ok(n, c[q], order, ni)) return true;
+  return false;
 }
 
 size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np) {
@@ -500,10 +511,9 @@ size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np) {
   return bytes + 256;
 }
 
-template <int ORDER, int M>
+template <int ORDER, int M, int RX, int RY>
 static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs, const RowGeom& rg, float2* out,
                      int64_t pitch, int64_t mesh_stride) {
-  constexpr int RX = PZCfg<M>::RX, RY = PZCfg<M>::RY;
   constexpr size_t lds = (size_t)M * (RX * RY + 1) * sizeof(float2);
   static_assert(lds <= PZ_LDS_BUDGET, "LDS budget");
   auto kern = paint_zfft_kernel<ORDER, M, RX, RY>;
@@ -517,20 +527,175 @@ static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, co
   return MP_OK;
 }
 
+template <int ORDER, int M>
+static int launch_pz_shape(const PzShape& sh, hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs,
+                           const RowGeom& rg, float2* out, int64_t pitch, int64_t mesh_stride) {
+  constexpr bool small = M <= 32;
+#define MP_PZ(RX_, RY_)                                                                                    \
+  if (sh.rx == RX_ && sh.ry == RY_) return launch_pz<ORDER, M, RX_, RY_>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+  if constexpr (M == 1024) { MP_PZ(4, 4) MP_PZ(4, 2) }
+  if constexpr (M == 512) { MP_PZ(8, 4) MP_PZ(4, 4) }
+  if constexpr (M == 256) { MP_PZ(8, 4) MP_PZ(8, 8) MP_PZ(4, 4) }
+  if constexpr (M == 128) { MP_PZ(8, 8) MP_PZ(16, 8) MP_PZ(8, 4) }
+  if constexpr (M == 64) { MP_PZ(16, 8) MP_PZ(16, 16) MP_PZ(8, 8) }
+  if constexpr (small) { MP_PZ(16, 16) MP_PZ(16, 8) MP_PZ(8, 8) }
+#undef MP_PZ
+  return set_error(MP_ERR_UNSUPPORTED, "paint_zfft: no %dx%d row blocks for Nz/2 = %d", sh.rx, sh.ry, M);
+}
+
 template <int ORDER>
-static int dispatch_pz(int M, hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs, const RowGeom& rg,
-                       float2* out, int64_t pitch, int64_t mesh_stride) {
+static int dispatch_pz(int M, const PzShape& sh, hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs,
+                       const RowGeom& rg, float2* out, int64_t pitch, int64_t mesh_stride) {
   switch (M) {
-    case 1024: return launch_pz<ORDER, 1024>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 512: return launch_pz<ORDER, 512>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 256: return launch_pz<ORDER, 256>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 128: return launch_pz<ORDER, 128>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 64: return launch_pz<ORDER, 64>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 32: return launch_pz<ORDER, 32>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 16: return launch_pz<ORDER, 16>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 8: return launch_pz<ORDER, 8>(s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 1024: return launch_pz_shape<ORDER, 1024>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 512: return launch_pz_shape<ORDER, 512>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 256: return launch_pz_shape<ORDER, 256>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 128: return launch_pz_shape<ORDER, 128>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 64: return launch_pz_shape<ORDER, 64>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 32: return launch_pz_shape<ORDER, 32>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 16: return launch_pz_shape<ORDER, 16>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 8: return launch_pz_shape<ORDER, 8>(sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
     default: return set_error(MP_ERR_UNSUPPORTED, "paint_zfft: Nz/2 = %d", M);
   }
+}
+
+static int launch_pz_any(int order, int M, const PzShape& sh, hipStream_t s, const float4* sorted, const int* offsets,
+                         const float* row_wabs, RowGeom rg, float2* out, int64_t pitch, int64_t mesh_stride) {
+  rg.rx = sh.rx;
+  rg.ry = sh.ry;
+  rg.nbx = rg.n[0] / sh.rx;
+  rg.nby = rg.n[1] / sh.ry;
+  switch (order) {
+    case 1: return dispatch_pz<1>(M, sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 2: return dispatch_pz<2>(M, sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    case 3: return dispatch_pz<3>(M, sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+    default: return dispatch_pz<4>(M, sh, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+  }
+}
+
+// ------------------------------------------------------------------ autotuner
+// On the first paint of a (device, meshsize, order, interlacing) case, every supported row-block
+// shape is timed on the actual sorted particles (two launches each, HIP events) and the
+// fastest is kept for the process.  Skipped while the stream is being captured into a graph,
+// and for small catalogues, which take the default shape.
+
+struct PzTuneKey {
+  int dev, n0, n1, n2, order, ni;
+  bool operator<(const PzTuneKey& o) const {
+    const int a[6] = {dev, n0, n1, n2, order, ni}, b[6] = {o.dev, o.n0, o.n1, o.n2, o.order, o.ni};
+    for (int q = 0; q < 6; ++q)
+      if (a[q] != b[q]) return a[q] < b[q];
+    return false;
+  }
+};
+struct PzTuneEntry {
+  PzShape shape;
+  float ms[4];
+  int count;
+};
+static std::mutex g_pz_tune_mu;
+static std::map<PzTuneKey, PzTuneEntry> g_pz_tuned;
+static int g_pz_tune = 1;
+constexpr int64_t PZ_TUNE_MIN_NP = 1 << 18;
+
+static int pz_choose(hipStream_t s, int order, int M, const int32_t n[3], int ni, int64_t np, const float4* sorted,
+                     const int* offsets, const float* row_wabs, const RowGeom& rg, float2* out, int64_t pitch, int64_t mesh_stride,
+                     PzShape& chosen) {
+  int count = 0;
+  const PzShape* c = pz_candidates(M, count);
+  int first = -1, nok = 0;
+  for (int q = 0; q < count; ++q)
+    if (pz_shape_ok(n, c[q], order, ni)) {
+      if (first < 0) first = q;
+      ++nok;
+    }
+  if (first < 0) return set_error(MP_ERR_UNSUPPORTED, "paint_zfft: no row-block shape fits (%d, %d, %d)", n[0], n[1], n[2]);
+  chosen = c[first];
+  if (g_pz_tune >= 100) {  // a forced candidate (tests of every shape)
+    const int q = g_pz_tune - 100;
+    if (q < count && pz_shape_ok(n, c[q], order, ni)) chosen = c[q];
+    return MP_OK;
+  }
+  int dev = 0;
+  MP_CHECK_HIP(hipGetDevice(&dev));
+  PzTuneKey key{dev, n[0], n[1], n[2], order, ni};
+  {
+    std::lock_guard<std::mutex> lock(g_pz_tune_mu);
+    auto it = g_pz_tuned.find(key);
+    if (it != g_pz_tuned.end()) {
+      chosen = it->second.shape;
+      return MP_OK;
+    }
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  MP_CHECK_HIP(hipStreamIsCapturing(s, &cap));
+  if (!g_pz_tune || nok < 2 || np < PZ_TUNE_MIN_NP || cap != hipStreamCaptureStatusNone || timing_enabled()) return MP_OK;
+  PzTuneEntry entry{};
+  entry.count = count;
+  float best = 1e30f;
+  hipEvent_t e0, e1;
+  MP_CHECK_HIP(hipEventCreate(&e0));
+  MP_CHECK_HIP(hipEventCreate(&e1));
+  int rc = MP_OK;
+  for (int q = 0; q < count && rc == MP_OK; ++q) {
+    entry.ms[q] = -1.f;
+    if (!pz_shape_ok(n, c[q], order, ni)) continue;
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 2 && rc == MP_OK; ++rep) {
+      if (hipEventRecord(e0, s) != hipSuccess) { rc = set_error(MP_ERR_HIP, "paint_zfft tuning: event record"); break; }
+      rc = launch_pz_any(order, M, c[q], s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = set_error(MP_ERR_HIP, "paint_zfft tuning: event timing");
+        break;
+      }
+      tmin = ms < tmin ? ms : tmin;
+    }
+    entry.ms[q] = tmin;
+    if (tmin < best) {
+      best = tmin;
+      chosen = c[q];
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  entry.shape = chosen;
+  std::lock_guard<std::mutex> lock(g_pz_tune_mu);
+  g_pz_tuned[key] = entry;
+  return MP_OK;
+}
+
+int paint_tuning_option(int enable) {
+  int old = g_pz_tune;
+  if (enable >= 0) g_pz_tune = enable >= 100 ? enable : (enable ? 1 : 0);
+  return old;
+}
+
+int paint_tuning_report(char* buf, int64_t len) {
+  std::lock_guard<std::mutex> lock(g_pz_tune_mu);
+  std::string text;
+  char line[512];
+  for (const auto& kv : g_pz_tuned) {
+    const PzTuneKey& k = kv.first;
+    const PzTuneEntry& e = kv.second;
+    int count = 0;
+    const PzShape* c = pz_candidates(k.n2 / 2, count);
+    int off = snprintf(line, sizeof(line), "paint_zfft n=%dx%dx%d order=%d interlacing=%d rows=%dx%d ms:", k.n0, k.n1, k.n2,
+                       k.order, k.ni, e.shape.rx, e.shape.ry);
+    for (int q = 0; q < count && off < (int)sizeof(line) - 32; ++q)
+      off += snprintf(line + off, sizeof(line) - off, " %dx%d=%.4f", c[q].rx, c[q].ry, e.ms[q]);
+    text += line;
+    text += "\n";
+  }
+  if (buf && len > 0) {
+    size_t ncopy = text.size() < (size_t)(len - 1) ? text.size() : (size_t)(len - 1);
+    memcpy(buf, text.data(), ncopy);
+    buf[ncopy] = 0;
+  }
+  return (int)g_pz_tuned.size();
 }
 
 template <typename T>
@@ -563,18 +728,13 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
             (const short*)tmp_key, n[1], n[2], nkeys, sorted, offsets, row_wabs);
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
-  pz_rows(n[2] / 2, rg.rx, rg.ry);
-  rg.nbx = n[0] / rg.rx;
-  rg.nby = n[1] / rg.ry;
   reach_of(order, ni, rg.lo, rg.hi);
   rg.ni = ni;
   const int M = n[2] / 2;
-  switch (order) {
-    case 1: return dispatch_pz<1>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 2: return dispatch_pz<2>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    case 3: return dispatch_pz<3>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-    default: return dispatch_pz<4>(M, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-  }
+  PzShape shape{0, 0};
+  rc = pz_choose(s, order, M, n, ni, np, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride, shape);
+  if (rc) return rc;
+  return launch_pz_any(order, M, shape, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
 }
 
 // ni z-transformed half-spectrum rows (pitch `pitch`, mesh j at out + j * mesh_stride).

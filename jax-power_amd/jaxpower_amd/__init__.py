@@ -12,6 +12,7 @@ from .mesh2 import (BinMesh2SpectrumPoles, compute_mesh2, compute_mesh2_spectrum
 from .mock import generate_uniform_particles, generate_gaussian_mesh, generate_lognormal_particles
 from .types import Mesh2SpectrumPole, Mesh2SpectrumPoles, read as read_observable, write as write_observable
 from .autodiff import mesh2_spectrum_poles
+from .batch import Mesh2SpectrumBatch
 from . import resamplers
 
 __all__ = ['MeshAttrs', 'BaseMeshField', 'RealMeshField', 'ComplexMeshField', 'ParticleField', 'FKPField', 'r2c', 'c2r',
@@ -19,4 +20,4 @@ __all__ = ['MeshAttrs', 'BaseMeshField', 'RealMeshField', 'ComplexMeshField', 'P
            'compute_box_normalization', 'BinMesh2SpectrumPoles', 'compute_mesh2', 'compute_mesh2_spectrum',
            'compute_fkp2_normalization', 'compute_box2_normalization', 'compute_fkp2_shotnoise',
            'generate_uniform_particles', 'generate_gaussian_mesh', 'generate_lognormal_particles', 'Mesh2SpectrumPole',
-           'Mesh2SpectrumPoles', 'mesh2_spectrum_poles', 'resamplers']
+           'Mesh2SpectrumPoles', 'mesh2_spectrum_poles', 'Mesh2SpectrumBatch', 'resamplers']

@@ -63,6 +63,8 @@ SIGNATURES = {
     'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),
     'mp_set_paint_path': (c_int, [c_int]),
+    'mp_set_paint_tuning': (c_int, [c_int]),
+    'mp_paint_tuning_report': (c_int, [c_char_p, c_int64]),
 }
 
 

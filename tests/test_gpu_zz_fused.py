@@ -59,7 +59,7 @@ def test_fused_matches_separate_path_f32_positions():
 def test_fused_rejects_unsupported_shape(fused_path):
     from jaxpower_amd import MeshAttrs, ParticleField
     say('fused unsupported shape')
-    attrs = MeshAttrs(meshsize=(32, 16, 64), boxsize=100.)
+    attrs = MeshAttrs(meshsize=(32, 4, 64), boxsize=100.)  # no row-block shape divides 4 rows
     p = ParticleField(torch.rand((100, 3), device='cuda') * 100., attrs=attrs)
     with pytest.raises(NotImplementedError):
         p.paint(resampler='tsc', interlacing=3, compensate=True, out='complex')
@@ -83,3 +83,29 @@ def test_fused_dense_clumps(fused_path, resampler):
     want = po.paint(oattrs, pos, w.astype('f8'), resampler=resampler, interlacing=3, compensate=True, out='complex')
     err = np.abs(got - want).max() / np.abs(want).max()
     assert err < 1e-5, err  # many float32 weights summed per cell
+
+
+@pytest.mark.parametrize('meshsize', [(64, 64, 512), (32, 32, 256), (64, 32, 128), (32, 32, 64), (32, 16, 32)])
+@pytest.mark.parametrize('resampler', ['cic', 'tsc', 'pcs'])
+def test_fused_every_row_block_shape(fused_path, meshsize, resampler):
+    """Every row-block shape the autotuner may pick (forced one at a time) paints the oracle's mesh."""
+    from jaxpower_amd import MeshAttrs, ParticleField, _lib
+    say('fused shapes', meshsize, resampler)
+    rng = np.random.default_rng(31)
+    attrs = MeshAttrs(meshsize=meshsize, boxsize=np.array(meshsize) * 5., boxcenter=0.)
+    lo = attrs.boxcenter - attrs.boxsize / 2.
+    pos = rng.uniform(0., 1., (4000, 3)) * attrs.boxsize + lo
+    w = rng.uniform(0.5, 1.5, 4000).astype('f4')
+    p = ParticleField(torch.from_numpy(pos).cuda(), torch.from_numpy(w).cuda(), attrs=attrs)
+    want = po.paint(po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter), pos, w.astype('f8'), resampler=resampler,
+                    interlacing=3, compensate=True, out='complex')
+    lib = _lib.load()
+    old = lib.mp_set_paint_tuning(-1)
+    try:
+        for q in range(3):
+            lib.mp_set_paint_tuning(100 + q)
+            got = p.paint(resampler=resampler, interlacing=3, compensate=True, out='complex').value.cpu().numpy()
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err < 1e-5, (q, err)
+    finally:
+        lib.mp_set_paint_tuning(old)
