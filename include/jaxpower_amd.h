@@ -160,6 +160,16 @@ int64_t mp_ylm_workspace_bytes(const int32_t meshsize[3]);
 int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t meshsize[3], const double boxsize[3],
                       const double boxcenter[3], int ell, void* out, int64_t out_pitch, void* workspace,
                       size_t workspace_bytes);
+/* The ell > 0 terms of the local line-of-sight estimator for one ell, fused
+ * (jaxpower/mesh2.py:311-333): out[p][b][re, im] = sum over modes of bin b of conj(C_ell) A0 w
+ * (p = 0, mode k = 0 excluded) and, when with_ell0 != 0, |A0|^2 w (p = 1), with
+ * C_ell = sum_m r2c(rmesh Y_lm(xhat)) Y_lm(khat) never stored.  rmesh: the real mesh (N0, N1, N2);
+ * A0: its half spectrum (or the second field's for a cross spectrum). */
+int64_t mp_ylm_bin_workspace_bytes(const int32_t meshsize[3], int ell, int nedges);
+int mp_ylm_bin(void* stream, const float* rmesh, const void* A0, int64_t a0_pitch, const int32_t meshsize[3],
+               const double boxsize[3], const double boxcenter[3], int ell, int with_ell0, const double* kx,
+               const double* ky, const double* kz, const double* k2_thresholds, int nedges, double* out, void* workspace,
+               size_t workspace_bytes);
 
 /* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
  *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */

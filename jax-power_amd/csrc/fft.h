@@ -56,6 +56,10 @@ int rfft3d(hipStream_t s, const float* in, float2* out, const int32_t n[3], int6
 // z and y passes only (the x pass is fused into a consumer).
 int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro);
 int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch);
+// z and y passes of rho * Y_lm(xhat) for every m of pro.ell (pro.kind = PRO_YLM): half spectrum
+// of m at Z + (m + ell) * mesh_stride.
+int rfft_zy_ylm(hipStream_t s, const float* in, float2* Z, int64_t mesh_stride, const int32_t n[3], int64_t pitch,
+                const RealPrologue& pro);
 // y pass of nmesh half spectra at data + m * mesh_stride, in one pipelined launch.
 int ypass_meshes(hipStream_t s, float2* data, int nmesh, int64_t mesh_stride, const int32_t n[3], int64_t pitch);
 // x pass of the interlaced paint (xpass.hip): combine of the n_i partial spectra after their z
@@ -69,6 +73,12 @@ size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int ne
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
                 double* out, void* ws, size_t ws_bytes);
+// Local line of sight for one ell, fused (xpass.hip): z/y passes of rho Y_lm(xhat) for all m,
+// then the x pass forms C = sum_m FFT(.) Y_lm(khat) per tile and bins conj(C) A0 w (and |A0|^2 w
+// when with0) without writing C; out[p][b][re, im] as mp_bin_poles writes it.
+size_t ylm_bin_workspace_bytes(const int32_t n[3], int ell, int nedges);
+int ylm_bin(hipStream_t s, const float* rmesh, const float2* A0, int64_t a0_pitch, const int32_t n[3], const double boxsize[3],
+            const double boxcenter[3], int ell, int with0, const BinGeom& g, double* out, void* ws, size_t ws_bytes);
 int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n[3], int64_t in_pitch, int64_t acc_pitch,
               const KPrologue& kp, int first);
 int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const int32_t n[3], float2* scratch,

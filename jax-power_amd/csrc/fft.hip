@@ -105,6 +105,59 @@ zpass_r2c_kernel(const float* __restrict__ in, float2* __restrict__ out, int64_t
   }
 }
 
+// z pass of rho * Y_lm(xhat) for every m = -ell..ell of pro.ell: the real rows are read once
+// and kept in registers; for each m they are multiplied by Y_lm of the node direction, staged
+// in LDS, transformed, and written to out + (m + ell) * mesh_stride.
+template <int M>
+__global__ void __launch_bounds__(RowCfg<M>::NT)
+zpass_r2c_ylm_kernel(const float* __restrict__ in, float2* __restrict__ out, int64_t nrows, int64_t out_pitch, int64_t mesh_stride,
+                     RealPrologue pro) {
+  using Cfg = RowCfg<M>;
+  constexpr int C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
+  constexpr int EPT = (C * M + NT - 1) / NT;
+  extern __shared__ float2 lds[];
+  __shared__ float2 stw[2 * M];
+  const int tid = threadIdx.x;
+  const int64_t row0 = (int64_t)blockIdx.x * C;
+  load_twiddles<2 * M, NT>(stw, g_twiddle);
+  const float2* in2 = reinterpret_cast<const float2*>(in);
+  float2 reg[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int idx = tid + e * NT;
+    const int64_t row = row0 + idx / M;
+    reg[e] = (idx < C * M && row < nrows) ? in2[row * M + idx % M] : make_float2(0.f, 0.f);
+  }
+  constexpr int NK = M / 2 + 1;
+  for (int m = -pro.ell; m <= pro.ell; ++m) {
+    RealPrologue p = pro;
+    p.m = m;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = tid + e * NT;
+      if (idx < C * M) {
+        const int c = idx / M, l = idx % M;
+        const int64_t row = row0 + c;
+        float2 v = reg[e];
+        if (row < nrows) {
+          v.x *= p.eval(row, 2 * l);
+          v.y *= p.eval(row, 2 * l + 1);
+        }
+        lds[c * LP + l] = v;
+      }
+    }
+    __syncthreads();
+    fft_lds<M, C, 1, LP, false, NT, false, 2 * M>(lds, stw);
+    float2* o = out + (int64_t)(m + pro.ell) * mesh_stride;
+    for (int idx = tid; idx < C * NK; idx += NT) {
+      const int c = idx / NK, k = idx % NK;
+      const int64_t row = row0 + c;
+      if (row < nrows) rfft_post<2 * M>(lds + c * LP, o + row * out_pitch, M, k, stw);
+    }
+    __syncthreads();
+  }
+}
+
 // Half-complex rows -> real rows, times `scale`.  Imaginary parts of the k=0 and k=M entries
 // are ignored, as numpy's irfftn does on its last axis.
 template <int M>
@@ -276,6 +329,15 @@ static int launch_zpass_r2c(const float* in, float2* out, int64_t nrows, int64_t
 }
 
 template <int M>
+static int launch_zpass_r2c_ylm(const float* in, float2* out, int64_t nrows, int64_t pitch, int64_t mstride, const RealPrologue& pro,
+                                hipStream_t stream) {
+  using Cfg = RowCfg<M>;
+  dim3 grid((unsigned)ceil_div(nrows, Cfg::C));
+  MP_LAUNCH("fft_zpass_r2c_ylm", zpass_r2c_ylm_kernel<M>, grid, dim3(Cfg::NT), Cfg::LDS, stream, in, out, nrows, pitch, mstride, pro);
+  return MP_OK;
+}
+
+template <int M>
 static int launch_zpass_c2r(const float2* in, int64_t pitch, float* out, int64_t nrows, float scale, hipStream_t stream) {
   using Cfg = RowCfg<M>;
   dim3 grid((unsigned)ceil_div(nrows, Cfg::C));
@@ -436,6 +498,20 @@ int ypass_meshes(hipStream_t s, float2* data, int nmesh, int64_t mesh_stride, co
 #undef MP_YCASE
     default: return set_error(MP_ERR_UNSUPPORTED, "ypass: length %d", n[1]);
   }
+}
+
+int rfft_zy_ylm(hipStream_t s, const float* in, float2* Z, int64_t mesh_stride, const int32_t n[3], int64_t pitch,
+                const RealPrologue& pro) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2;
+  if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  const int64_t nrows = (int64_t)n[0] * n[1];
+  rc = [&]() -> int { MP_POW2_SWITCH(M, launch_zpass_r2c_ylm, in, Z, nrows, pitch, mesh_stride, pro, s); }();
+  if (rc) return rc;
+  return ypass_meshes(s, Z, 2 * pro.ell + 1, mesh_stride, n, pitch);
 }
 
 int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch) {

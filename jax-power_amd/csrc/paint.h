@@ -24,6 +24,12 @@ size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np);
 int paint_zfft(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
                float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes);
 
+// Real-space paint from the row sort (paint_zfft.hip): any mesh up to 2048 per axis; its
+// workspace is paint_zfft_workspace_bytes.
+bool paint_rows_supported(const int32_t n[3], int64_t np, int order, int ni);
+int paint_rows(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
+               float* meshes, int64_t mesh_stride, void* ws, size_t ws_bytes);
+
 // Path used by the interlaced spectrum paint: 0 automatic (fused when supported), 1 separate
 // paint + z pass, 2 fused only (error when unsupported), 3 per-particle atomics + z pass.
 int paint_path_option();

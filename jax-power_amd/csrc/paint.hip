@@ -412,7 +412,15 @@ static bool use_tiled(const int32_t n[3], int64_t np, int order, int ni, TileGeo
   return make_tile_geom(n, order, ni, tg);
 }
 
+// The row paint serves every shape it supports unless the path option asks for the tiled (1)
+// or atomic (3) paint.
+static bool use_rows(const int32_t n[3], int64_t np, int order, int ni) {
+  const int opt = paint_path_option();
+  return opt != 1 && opt != 3 && paint_rows_supported(n, np, order, ni);
+}
+
 size_t paint_workspace_bytes(const int32_t n[3], int64_t np, int order, int ni) {
+  if (use_rows(n, np, order, ni)) return paint_zfft_workspace_bytes(n, np);
   TileGeom tg;
   if (!use_tiled(n, np, order, ni, tg)) return 256;
   int64_t nbt = (int64_t)tg.nb[0] * tg.nb[1] * tg.nb[2];
@@ -503,6 +511,11 @@ int paint_meshes(hipStream_t s, const void* pos, int pos_f64, const float* w, in
   MP_REQUIRE(np == 0 || pos, "null positions");
   MP_REQUIRE(meshes, "null meshes");
   for (int a = 0; a < 3; ++a) MP_REQUIRE(g.n[a] >= 1 && g.n[a] < 32768, "meshsize[%d] = %d out of range", a, g.n[a]);
+  if (!force_atomic && use_rows(g.n, np, order, ni)) {
+    if (ws_bytes < paint_zfft_workspace_bytes(g.n, np))
+      return set_error(MP_ERR_WORKSPACE, "paint: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(g.n, np));
+    return paint_rows(s, pos, pos_f64, w, np, g, order, ni, meshes, mesh_stride, ws, ws_bytes);
+  }
   if (pos_f64)
     return paint_typed<double>(s, (const double*)pos, w, np, g, order, ni, meshes, mesh_stride, ws, ws_bytes, force_atomic);
   return paint_typed<float>(s, (const float*)pos, w, np, g, order, ni, meshes, mesh_stride, ws, ws_bytes, force_atomic);

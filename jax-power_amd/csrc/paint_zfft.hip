@@ -10,7 +10,7 @@
 //
 // Record layout of a sorted particle: float4(fx, fy, zbits, w): fx, fy in [0, 1) are the
 // offsets from the home cell along x and y, zbits is z~ (wrapped into [0, Nz)) as unsigned
-// fixed point with ZFRAC fractional bits, w the weight.
+// fixed point with zfrac = 32 - ceil(log2 Nz) fractional bits (rounded to nearest), w the weight.
 #include "common.h"
 #include "fft.h"
 #include "fft_core.h"
@@ -23,7 +23,12 @@
 
 namespace mp {
 
-constexpr int ZFRAC = 20;                 // fractional bits of z in the record (Nz <= 2048)
+// Fractional bits of z in a record: all the bits of a 32-bit word that the cell index leaves.
+static int zfrac_of(int nz) {
+  int b = 1;
+  while ((1 << b) < nz) ++b;
+  return 32 - b;
+}
 constexpr int PZ_NT = 1024;
 // Segment table (home rows of a block's neighbourhood): the reach of a stencil is at most
 // p + 1 <= 5 rows per axis, so (RX + 5) (RY + 5) rounded to a wave.
@@ -40,7 +45,14 @@ struct RowGeom {
   int nbx, nby;        // blocks per axis
   int lo, hi;          // stencil reach relative to the home cell
   int ni;
+  int zfrac;           // fractional bits of z in the records
 };
+
+// Home z cell and offset in [0, 1] of a record's z word (the offset rounds to float).
+__device__ __forceinline__ void z_of_record(unsigned zb, int zfrac, int& zc, float& fz) {
+  zc = (int)(zb >> zfrac);
+  fz = (float)(zb << (32 - zfrac)) * 2.3283064365386963e-10f;  // 2^-32
+}
 
 template <int ORDER>
 __host__ __device__ __forceinline__ float kern1(float s) {
@@ -95,7 +107,7 @@ constexpr int RS_MAX_KEYS = 2048;
 
 template <typename T>
 __device__ __forceinline__ void row_record(const T* __restrict__ pos, const float* __restrict__ w, int64_t i, const GridMap& g,
-                                           int& hx, int& hy, float4& rec) {
+                                           int zfrac, int& hx, int& hy, float4& rec) {
   T xg[3];
   grid_coords(pos, i, g, xg);
   T fx = floor(xg[0]), fy = floor(xg[1]);
@@ -106,34 +118,52 @@ __device__ __forceinline__ void row_record(const T* __restrict__ pos, const floa
   float oy = fminf(fmaxf((float)(xg[1] - fy), 0.f), 0.99999994f);
   // z wrapped into [0, Nz) as fixed point
   T z = xg[2] - floor(xg[2] / (T)g.n[2]) * (T)g.n[2];
-  double zs = (double)z * (double)(1u << ZFRAC);
-  double zmax = (double)g.n[2] * (double)(1u << ZFRAC) - 1.;
+  const double zone = (double)(1ull << zfrac);
+  double zs = rint((double)z * zone);
+  double zmax = (double)g.n[2] * zone - 1.;
   unsigned zbits = (unsigned)fmin(fmax(zs, 0.), zmax);
   rec = make_float4(ox, oy, __uint_as_float(zbits), w ? w[i] : 1.f);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(RS_NT)
-rowsort_hist_kernel(const T* __restrict__ pos, int64_t np, GridMap g, int* __restrict__ bucket_total) {
-  __shared__ int hist[RS_MAX_KEYS];
-  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
-  __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * RS_CHUNK;
-  const int64_t i1 = i0 + RS_CHUNK < np ? i0 + RS_CHUNK : np;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += RS_NT) {
-    T xg[3];
-    grid_coords(pos, i, g, xg);
-    atomicAdd(&hist[pmod((int)floor(xg[0]), g.n[0])], 1);
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT)
-    if (hist[b]) atomicAdd(&bucket_total[b], hist[b]);
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(RS_NT)
-rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int* __restrict__ bucket_cursor,
-                       float4* __restrict__ tmp_rec, short* __restrict__ tmp_key) {
+rowsort_hist_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int* __restrict__ bucket_total,
+                    unsigned* __restrict__ wmax_bits) {
+  __shared__ int hist[RS_MAX_KEYS];
+  __shared__ float wmax[RS_NT / 64];
+  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * RS_CHUNK;
+  const int64_t i1 = i0 + RS_CHUNK < np ? i0 + RS_CHUNK : np;
+  float m = w ? 0.f : 1.f;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += RS_NT) {
+    T xg[3];
+    grid_coords(pos, i, g, xg);
+    atomicAdd(&hist[pmod((int)floor(xg[0]), g.n[0])], 1);
+    if (w) m = fmaxf(m, fabsf(w[i]));
+  }
+  m = wave_max_f32(m);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT)
+    if (hist[b]) atomicAdd(&bucket_total[b], hist[b]);
+  if (threadIdx.x == 0) {
+    float bm = 0.f;
+    for (int q = 0; q < RS_NT / 64; ++q) bm = fmaxf(bm, wmax[q]);
+    atomicMax(wmax_bits, __float_as_uint(bm));  // non-negative floats order like their bits
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RS_NT)
+rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int zfrac,
+                       int* __restrict__ bucket_cursor, float4* __restrict__ tmp_rec, short* __restrict__ tmp_key) {
   __shared__ int hist[RS_MAX_KEYS];
   __shared__ int base[RS_MAX_KEYS];
   for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
@@ -161,7 +191,7 @@ rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, i
     if (i < i1) {
       int hx, hy;
       float4 rec;
-      row_record(pos, w, i, g, hx, hy, rec);
+      row_record(pos, w, i, g, zfrac, hx, hy, rec);
       int64_t slot = (int64_t)base[hx] + rank[q];
       tmp_rec[slot] = rec;
       tmp_key[slot] = (short)hy;
@@ -203,28 +233,22 @@ __device__ int block_excl_scan(int* data, int n, int* wsum) {
   return wsum[RS_NT / 64];
 }
 
-__device__ __forceinline__ float wave_max_f32(float v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
-  return v;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
 
 // One workgroup per x bucket: counting sort by y into ysorted (the order the paint reads);
 // writes the row offsets and a bound on the sum of |w| of every row.
 __global__ void __launch_bounds__(RS_NT)
 rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
-                    int nz, int64_t nkeys, float4* __restrict__ ysorted, int* __restrict__ offsets, float* __restrict__ row_wabs) {
+                    int nz, int64_t nkeys, const unsigned* __restrict__ wmax_bits, float4* __restrict__ ysorted,
+                    int* __restrict__ offsets, float* __restrict__ row_wabs) {
   __shared__ int cnt[RS_MAX_KEYS + 1];
+  __shared__ unsigned long long qsum[RS_MAX_KEYS];
   __shared__ int wsum[RS_NT / 64 + 1];
   const int hx = blockIdx.x;
   const int start = bucket_start[hx], end = bucket_start[hx + 1];
-  for (int y = threadIdx.x; y < ny; y += RS_NT) cnt[y] = 0;
+  for (int y = threadIdx.x; y < ny; y += RS_NT) {
+    cnt[y] = 0;
+    qsum[y] = 0ull;
+  }
   __syncthreads();
   for (int i = start + threadIdx.x; i < end; i += RS_NT) atomicAdd(&cnt[tmp_key[i]], 1);
   __syncthreads();
@@ -232,35 +256,35 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restri
   for (int y = threadIdx.x; y < ny; y += RS_NT) offsets[(int64_t)hx * ny + y] = start + cnt[y];
   if (hx == 0 && threadIdx.x == 0) offsets[nkeys] = bucket_start[gridDim.x];
   __syncthreads();
-  for (int i = start + threadIdx.x; i < end; i += RS_NT) {
-    int y = tmp_key[i];
-    int r = atomicAdd(&cnt[y], 1);
-    ysorted[start + r] = tmp_rec[i];
-  }
-  __syncthreads();  // cnt[y] now holds the end of row y; ysorted is visible to the workgroup
   // Per-row bound on sum |w|, which sets the fixed-point scale of the paint.  It must not
-  // depend on the order of the row's records (which the atomics above leave unspecified), so
-  // it is an integer sum: q_i = ceil(|w_i| / m * 2^24) with m the row's largest |w|, and the
-  // bound is m * sum q_i / 2^24 >= sum |w|.
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int y = wave; y < ny; y += RS_NT / 64) {
-    const int rs = start + (y ? cnt[y - 1] : 0), re = start + cnt[y];
-    float m = 0.f;
-    for (int i = rs + lane; i < re; i += 64) m = fmaxf(m, fabsf(ysorted[i].w));
-    m = wave_max_f32(m);
-    unsigned long long q = 0;
-    if (m > 0.f)
-      for (int i = rs + lane; i < re; i += 64) q += (unsigned long long)ceilf(fabsf(ysorted[i].w) / m * 16777216.f);
-    q = wave_sum_u64(q);
-    if (lane == 0) row_wabs[(int64_t)hx * ny + y] = (float)((double)q * (double)m * (1.0 / 16777216.0));
+  // depend on the order of the row's records (which the atomics leave unspecified), so it is
+  // an integer sum: q_i = ceil(|w_i| / m * 2^24) with m the largest |w| of the catalogue, and
+  // the bound is m * sum q_i / 2^24 >= sum |w|.
+  const float m = __uint_as_float(*wmax_bits);
+  const float qs = m > 0.f ? 16777216.f / m : 0.f;
+  for (int i = start + threadIdx.x; i < end; i += RS_NT) {
+    const int y = tmp_key[i];
+    const int r = atomicAdd(&cnt[y], 1);
+    const float4 rec = tmp_rec[i];
+    ysorted[start + r] = rec;
+    atomicAdd(&qsum[y], (unsigned long long)ceilf(fabsf(rec.w) * qs));
   }
+  __syncthreads();
+  for (int y = threadIdx.x; y < ny; y += RS_NT)
+    row_wabs[(int64_t)hx * ny + y] = (float)((double)qsum[y] * (double)m * (1.0 / 16777216.0));
 }
 
 // Stencil cells base + round/floor(u) + OMIN + o and weights K(|cell - u|), o in [0, ORDER).
+// base is relative to an even origin, so that an exact half rounds to the even mesh cell as
+// round-half-even on the mesh coordinate does (only NGP, whose kernel is discontinuous there,
+// depends on it).
 template <int ORDER>
 __device__ __forceinline__ void stencil_1d(float u, int base, int* cell, float* k) {
   constexpr int OMIN = -((ORDER - 1) / 2);
-  const float r = (ORDER % 2) ? rintf(u) : floorf(u);
+  float r = (ORDER % 2) ? rintf(u) : floorf(u);
+  if constexpr (ORDER == 1) {
+    if (fabsf(u - r) == 0.5f && ((base + (int)r) & 1)) r = 2.f * u - r;
+  }
   const int i0 = base + (int)r + OMIN;
 #pragma unroll
   for (int o = 0; o < ORDER; ++o) {
@@ -402,9 +426,9 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
       for (int q = 0; q < CHUNK; ++q) {
         if (q >= cnt) break;
         const float4 rec = recs[q];
-        const unsigned zb = __float_as_uint(rec.z);
-        const int zc = (int)(zb >> ZFRAC);
-        const float fz = (float)(zb & ((1u << ZFRAC) - 1u)) * (1.f / (float)(1u << ZFRAC));
+        int zc;
+        float fz;
+        z_of_record(__float_as_uint(rec.z), rg.zfrac, zc, fz);
         int cx[ORDER], cy[ORDER], cz[ORDER];
         float kx[ORDER], ky[ORDER], kz[ORDER];
         stencil_1d<ORDER>(rec.x + s, hxr, cx, kx);
@@ -444,6 +468,152 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     int rx = r / RY, ry = r % RY;
     float2* o = dst + ((int64_t)(x0 + rx) * rg.n[1] + (y0 + ry)) * pitch;
     rfft_post<NZ>(lds2 + r, o, M, kk, stw, RP);
+  }
+}
+
+// Real-space paint from the row sort (any mesh shape up to 2048 per axis): one workgroup per
+// (block of RX x RY rows, shift); the rows (full z extent, int32 fixed point) accumulate in
+// LDS like the fused paint, then are written to the real mesh.  Row blocks at the upper mesh
+// edge may be partial.
+constexpr int PR_NT = 512;
+
+template <int ORDER, int RX, int RY>
+__global__ void __launch_bounds__(PR_NT)
+paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ offsets, const float* __restrict__ row_wabs,
+                  RowGeom rg, float* __restrict__ out, int64_t mesh_stride) {
+  constexpr int NROW = RX * RY;
+  constexpr int CHUNK = 4;
+  constexpr int OMIN = -((ORDER - 1) / 2);
+  constexpr int SEG = pz_seg_capacity(RX, RY);
+  extern __shared__ int irows[];  // row r, cell z at irows[r * rs + z]
+  __shared__ int sstart[SEG];
+  __shared__ int scount[SEG];
+  __shared__ int spre[SEG + 2];
+  __shared__ float swabs[SEG];
+  __shared__ float tbound[NROW];
+  __shared__ float sbound[1];
+  const int tid = threadIdx.x;
+  const int n2 = rg.n[2], rs = n2 + 1;
+  const int id = blockIdx.x;
+  const int j = id % rg.ni;
+  const int rest = id / rg.ni;
+  const int by = rest % rg.nby, bx = rest / rg.nby;
+  const int x0 = bx * RX, y0 = by * RY;
+  const float s = (float)j / (float)rg.ni;
+  for (int i = tid; i < NROW * rs; i += PR_NT) irows[i] = 0;
+  const int lo = OMIN + ((ORDER % 2) && 2 * j > rg.ni ? 1 : 0);
+  const int hi = (ORDER - 1 + OMIN) + ((ORDER % 2) ? (2 * j >= rg.ni ? 2 : 1) : (j > 0 ? 1 : 0));
+  const int sx = RX + hi - lo, sy = RY + hi - lo;
+  const int nseg = sx * sy;
+  for (int i = tid; i < nseg; i += PR_NT) {
+    const int ix = i / sy, iy = i % sy;
+    const int hx = pmod(x0 - hi + ix, rg.n[0]), hy = pmod(y0 - hi + iy, rg.n[1]);
+    const int64_t key = (int64_t)hx * rg.n[1] + hy;
+    const int st = offsets[key];
+    const int c = offsets[key + 1] - st;
+    sstart[i] = st;
+    scount[i] = c;
+    spre[i + 1] = (c + CHUNK - 1) / CHUNK;
+    swabs[i] = row_wabs[key];
+  }
+  __syncthreads();
+  if (tid < 64) {
+    constexpr int PER = SEG / 64;
+    int local[PER];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = tid * PER + q;
+      local[q] = i < nseg ? spre[i + 1] : 0;
+      sum += local[q];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d *= 2) {
+      const int v = __shfl_up(incl, d, 64);
+      if (tid >= d) incl += v;
+    }
+    int run = incl - sum;
+    if (tid == 0) spre[0] = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = tid * PER + q;
+      run += local[q];
+      if (i < nseg) spre[i + 1] = run;
+    }
+  }
+  if (tid < NROW) {
+    const int rx = tid / RY, ry = tid % RY, span = hi - lo;
+    float sum = 0.f;
+    for (int a = 0; a <= span; ++a)
+      for (int bb = 0; bb <= span; ++bb) sum += swabs[(rx + a) * sy + (ry + bb)];
+    tbound[tid] = sum;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    float m = 0.f;
+    for (int i = tid; i < NROW; i += 64) m = fmaxf(m, tbound[i]);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    if (tid == 0) sbound[0] = m;
+  }
+  __syncthreads();
+  const int total = spre[nseg];
+  const float bound = sbound[0];
+  const float scale = bound > 0.f ? 2126008811.f / bound : 1.f;
+  const int rxmax = min(RX, rg.n[0] - x0), rymax = min(RY, rg.n[1] - y0);  // partial blocks at the edge
+  for (int c = tid; c < total; c += PR_NT) {
+    int lo_i = 0, hi_i = nseg - 1;
+    while (lo_i < hi_i) {
+      const int mid = (lo_i + hi_i + 1) >> 1;
+      if (spre[mid] <= c) lo_i = mid;
+      else hi_i = mid - 1;
+    }
+    const int seg = lo_i;
+    const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
+    const int cnt = min(CHUNK, sstart[seg] + scount[seg] - p0);
+    const int hxr = seg / sy - hi, hyr = seg % sy - hi;
+    float4 recs[CHUNK];
+#pragma unroll
+    for (int q = 0; q < CHUNK; ++q) recs[q] = q < cnt ? sorted[p0 + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < CHUNK; ++q) {
+      if (q >= cnt) break;
+      const float4 rec = recs[q];
+      int zc;
+      float fz;
+      z_of_record(__float_as_uint(rec.z), rg.zfrac, zc, fz);
+      int cx[ORDER], cy[ORDER], cz[ORDER];
+      float kx[ORDER], ky[ORDER], kz[ORDER];
+      stencil_1d<ORDER>(rec.x + s, hxr, cx, kx);
+      stencil_1d<ORDER>(rec.y + s, hyr, cy, ky);
+      stencil_1d<ORDER>(fz + s, zc, cz, kz);
+#pragma unroll
+      for (int ox = 0; ox < ORDER; ++ox) {
+        if (!(cx[ox] >= 0 && cx[ox] < rxmax)) continue;
+#pragma unroll
+        for (int oy = 0; oy < ORDER; ++oy) {
+          if (!(cy[oy] >= 0 && cy[oy] < rymax)) continue;
+          int* row = irows + (cx[ox] * RY + cy[oy]) * rs;
+          const float kxy = rec.w * kx[ox] * ky[oy] * scale;
+#pragma unroll
+          for (int oz = 0; oz < ORDER; ++oz) {
+            int z = cz[oz];
+            z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+            atomicAdd(row + z, __float2int_rn(kxy * kz[oz]));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const float inv = 1.f / scale;
+  float* dst = out + j * mesh_stride;
+  for (int r = 0; r < NROW; ++r) {
+    const int rx = r / RY, ry = r % RY;
+    if (rx >= rxmax || ry >= rymax) continue;
+    float* o = dst + ((int64_t)(x0 + rx) * rg.n[1] + (y0 + ry)) * n2;
+    for (int z = tid; z < n2; z += PR_NT) o[z] = (float)irows[r * rs + z] * inv;
   }
 }
 
@@ -504,7 +674,7 @@ size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np) {
   int64_t nkeys = (int64_t)n[0] * n[1];
   size_t npp = (size_t)(np > 0 ? np : 1);
   size_t bytes = align_up((size_t)(nkeys + 1) * 4, 256);          // row offsets
-  bytes += 2 * align_up((size_t)(n[0] + 1) * 4, 256);             // bucket start / cursor
+  bytes += 2 * align_up((size_t)(n[0] + 2) * 4, 256);             // bucket start / cursor (+ max |w|)
   bytes += scan_workspace_bytes(n[0] + 1);
   bytes += align_up(npp * 16, 256) * 2 + align_up(npp * 2, 256);  // tmp records, sorted records, tmp keys
   bytes += align_up((size_t)nkeys * 4, 256);                      // per-row sum of |w|
@@ -698,43 +868,125 @@ int paint_tuning_report(char* buf, int64_t len) {
   return (int)g_pz_tuned.size();
 }
 
+// The row sort shared by the fused paint and the real-space row paint: records grouped by home
+// (x, y) row, row offsets, and the per-row bound on sum |w|.
+struct RowSorted {
+  const float4* sorted;
+  const int* offsets;
+  const float* row_wabs;
+};
+
 template <typename T>
-static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t np, const GridMap& g, int order, int ni,
-                            float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes) {
+static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, const GridMap& g, void* ws, size_t ws_bytes,
+                    RowSorted& out) {
   const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
   const int64_t nkeys = (int64_t)n[0] * n[1];
-  MP_REQUIRE(n[0] <= RS_MAX_KEYS && n[1] <= RS_MAX_KEYS, "paint_zfft: meshsize beyond %d", RS_MAX_KEYS);
+  MP_REQUIRE(n[0] <= RS_MAX_KEYS && n[1] <= RS_MAX_KEYS, "row sort: meshsize beyond %d", RS_MAX_KEYS);
   Workspace wsp(ws, ws_bytes);
   int* offsets = wsp.take<int>(nkeys + 1);
   int* bstart = wsp.take<int>(n[0] + 1);
-  int* bcursor = wsp.take<int>(n[0] + 1);
+  int* bcursor = wsp.take<int>(n[0] + 2);
   int* scan_tmp = (int*)wsp.take<char>(scan_workspace_bytes(n[0] + 1));
   float4* tmp_rec = wsp.take<float4>(np > 0 ? np : 1);
   float4* sorted = wsp.take<float4>(np > 0 ? np : 1);
   short* tmp_key = wsp.take<short>(np > 0 ? np : 1);
   float* row_wabs = wsp.take<float>(nkeys);
   if (!offsets || !bstart || !bcursor || !scan_tmp || !tmp_rec || !sorted || !tmp_key || !row_wabs)
-    return set_error(MP_ERR_WORKSPACE, "paint_zfft: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(n, np));
+    return set_error(MP_ERR_WORKSPACE, "row sort: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(n, np));
+  unsigned* wmax = reinterpret_cast<unsigned*>(bcursor + n[0] + 1);  // past the n0 + 1 cursors the scan writes
   int rc = fill_i32_dev(s, bstart, n[0] + 1, 0);
   if (rc) return rc;
+  rc = fill_i32_dev(s, reinterpret_cast<int*>(wmax), 1, 0);
+  if (rc) return rc;
   const unsigned nchunk = (unsigned)ceil_div(np, RS_CHUNK);
-  if (np > 0) MP_LAUNCH("paint_rowsort_hist", rowsort_hist_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, np, g, bstart);
+  if (np > 0)
+    MP_LAUNCH("paint_rowsort_hist", rowsort_hist_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, bstart, wmax);
   rc = exclusive_scan_i32(s, bstart, n[0] + 1, scan_tmp, bcursor);
   if (rc) return rc;
   if (np > 0)
-    MP_LAUNCH("paint_rowsort_scatter", rowsort_scatter_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, bcursor, tmp_rec,
-              tmp_key);
+    MP_LAUNCH("paint_rowsort_scatter", rowsort_scatter_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, zfrac_of(n[2]),
+              bcursor, tmp_rec, tmp_key);
   MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), 0, s, (const int*)bstart, tmp_rec,
-            (const short*)tmp_key, n[1], n[2], nkeys, sorted, offsets, row_wabs);
+            (const short*)tmp_key, n[1], n[2], nkeys, (const unsigned*)wmax, sorted, offsets, row_wabs);
+  out.sorted = sorted;
+  out.offsets = offsets;
+  out.row_wabs = row_wabs;
+  return MP_OK;
+}
+
+template <typename T>
+static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t np, const GridMap& g, int order, int ni,
+                            float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes) {
+  const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
+  RowSorted rs;
+  int rc = row_sort<T>(s, pos, w, np, g, ws, ws_bytes, rs);
+  if (rc) return rc;
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
   reach_of(order, ni, rg.lo, rg.hi);
   rg.ni = ni;
+  rg.zfrac = zfrac_of(n[2]);
   const int M = n[2] / 2;
   PzShape shape{0, 0};
-  rc = pz_choose(s, order, M, n, ni, np, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride, shape);
+  rc = pz_choose(s, order, M, n, ni, np, rs.sorted, rs.offsets, rs.row_wabs, rg, out, pitch, mesh_stride, shape);
   if (rc) return rc;
-  return launch_pz_any(order, M, shape, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+  return launch_pz_any(order, M, shape, s, rs.sorted, rs.offsets, rs.row_wabs, rg, out, pitch, mesh_stride);
+}
+
+template <int ORDER, int RX, int RY>
+static int launch_rows(hipStream_t s, const RowSorted& rs, RowGeom rg, float* meshes, int64_t mesh_stride) {
+  rg.rx = RX;
+  rg.ry = RY;
+  rg.nbx = (int)ceil_div(rg.n[0], RX);
+  rg.nby = (int)ceil_div(rg.n[1], RY);
+  const size_t lds = (size_t)RX * RY * (rg.n[2] + 1) * sizeof(int);
+  auto kern = paint_rows_kernel<ORDER, RX, RY>;
+  if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const unsigned grid = (unsigned)((int64_t)rg.nbx * rg.nby * rg.ni);
+  MP_LAUNCH("paint_rows", kern, dim3(grid), dim3(PR_NT), lds, s, rs.sorted, rs.offsets, rs.row_wabs, rg, meshes, mesh_stride);
+  return MP_OK;
+}
+
+// Rows per block of the row paint: 8x8 rows up to Nz = 256, 4x4 above (at most 131 KiB of LDS).
+static bool rows_big(int nz) { return nz > 256; }
+
+bool paint_rows_supported(const int32_t n[3], int64_t np, int order, int ni) {
+  if (order < 1 || order > 4 || ni < 1 || np >= (int64_t)INT32_MAX) return false;
+  if (n[0] > RS_MAX_KEYS || n[1] > RS_MAX_KEYS || n[2] > 2048 || n[2] < 4) return false;
+  const int r = rows_big(n[2]) ? 4 : 8;
+  int lo, hi;
+  reach_of(order, ni, lo, hi);
+  return r + hi - lo <= n[0] && r + hi - lo <= n[1];
+}
+
+template <typename T>
+static int paint_rows_typed(hipStream_t s, const T* pos, const float* w, int64_t np, const GridMap& g, int order, int ni,
+                            float* meshes, int64_t mesh_stride, void* ws, size_t ws_bytes) {
+  RowSorted rs;
+  int rc = row_sort<T>(s, pos, w, np, g, ws, ws_bytes, rs);
+  if (rc) return rc;
+  RowGeom rg;
+  for (int a = 0; a < 3; ++a) rg.n[a] = g.n[a];
+  reach_of(order, ni, rg.lo, rg.hi);
+  rg.ni = ni;
+  rg.zfrac = zfrac_of(g.n[2]);
+  const bool big = rows_big(g.n[2]);
+#define MP_ROWS(O)                                                                                   \
+  case O:                                                                                            \
+    return big ? launch_rows<O, 4, 4>(s, rs, rg, meshes, mesh_stride) : launch_rows<O, 8, 8>(s, rs, rg, meshes, mesh_stride);
+  switch (order) {
+    MP_ROWS(1) MP_ROWS(2) MP_ROWS(3) MP_ROWS(4)
+    default: return set_error(MP_ERR_INVALID, "resampler order %d not in 1..4", order);
+  }
+#undef MP_ROWS
+}
+
+// ni real meshes (mesh j at meshes + j * mesh_stride) painted from the row sort.
+int paint_rows(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
+               float* meshes, int64_t mesh_stride, void* ws, size_t ws_bytes) {
+  if (!paint_rows_supported(g.n, np, order, ni)) return set_error(MP_ERR_UNSUPPORTED, "row paint: shape not supported");
+  if (pos_f64) return paint_rows_typed<double>(s, (const double*)pos, w, np, g, order, ni, meshes, mesh_stride, ws, ws_bytes);
+  return paint_rows_typed<float>(s, (const float*)pos, w, np, g, order, ni, meshes, mesh_stride, ws, ws_bytes);
 }
 
 // ni z-transformed half-spectrum rows (pitch `pitch`, mesh j at out + j * mesh_stride).

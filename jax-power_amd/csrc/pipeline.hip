@@ -259,6 +259,32 @@ MP_EXPORT int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t 
   return MP_OK;
 }
 
+MP_EXPORT int64_t mp_ylm_bin_workspace_bytes(const int32_t meshsize[3], int ell, int nedges) {
+  return (int64_t)ylm_bin_workspace_bytes(meshsize, ell, nedges);
+}
+
+// The ell > 0 terms of the local line-of-sight estimator (jaxpower/mesh2.py:311-333) for one
+// ell, fused: sum over bins of conj(C_ell) A0 w with C_ell = sum_m r2c(rho Y_lm(xhat)) Y_lm(khat),
+// and with_ell0 != 0 also |A0|^2 w; C_ell is never stored.
+MP_EXPORT int mp_ylm_bin(void* stream, const float* rmesh, const void* A0, int64_t a0_pitch, const int32_t meshsize[3],
+                         const double boxsize[3], const double boxcenter[3], int ell, int with_ell0, const double* kx,
+                         const double* ky, const double* kz, const double* k2_thresholds, int nedges, double* out, void* workspace,
+                         size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && boxsize && boxcenter && kx && ky && kz && k2_thresholds, "mp_ylm_bin: null pointer");
+  MP_REQUIRE(nedges >= 2, "mp_ylm_bin: need at least 2 edges, got %d", nedges);
+  BinGeom g;
+  g.n0 = meshsize[0];
+  g.n1 = meshsize[1];
+  g.ncols = meshsize[2] / 2 + 1;
+  g.nedges = nedges;
+  g.kx = kx;
+  g.ky = ky;
+  g.kz = kz;
+  g.thr = k2_thresholds;
+  return ylm_bin((hipStream_t)stream, rmesh, (const float2*)A0, a0_pitch, meshsize, boxsize, boxcenter, ell, with_ell0, g, out,
+                 workspace, workspace_bytes);
+}
+
 MP_EXPORT int64_t mp_reduce_workspace_bytes(void) { return RED_BLOCKS * sizeof(double) + 256; }
 
 // out[0] = sum_i a_i * b_i (b given), sum_i a_i^2 (square), or sum_i a_i, accumulated in FP64.

@@ -26,6 +26,8 @@ namespace mp {
 
 constexpr int XB_MAX_PROD = 6;
 
+static int multiprocessors();
+
 struct XPassArgs {
   const float2* in;
   int64_t mesh_stride;  // between the partial spectra of consecutive shifts
@@ -246,6 +248,159 @@ xpass_kernel(XPassArgs a) {
   }
 }
 
+// Local line of sight, one ell: C(k) = sum_m FFT_x(Z_m)(k) Y_lm(khat) over the 2 ell + 1 partial
+// spectra Z_m (rho Y_lm(xhat) after their z and y passes), never written: each tile's C is
+// multiplied by the combined mesh A0 at the same modes and binned in FP64 as
+// conj(C) A0 w (remove_zero at k = 0), and optionally |A0|^2 w (the ell = 0 term of an auto
+// spectrum).  jaxpower/mesh2.py:304-333.
+struct YlmBinArgs {
+  const float2* Z;
+  int64_t mesh_stride, fstride, ostride;  // of Z (and of A0 with a0_fstride, a0_ostride)
+  const float2* A0;
+  int64_t a0_fstride, a0_ostride;
+  int nm, ell, ncols, ny, nchunks, ntiles;
+  KPrologue kp;
+  const float2* tw;
+  BinGeom g;
+  int with0;
+  double* partial;  // [gridDim.x][NPP][nbins][2]
+};
+
+template <int L, int NPP>
+__global__ void __launch_bounds__(XPCfg<L>::NT)
+xpass_ylm_bin_kernel(YlmBinArgs a) {
+  using Cfg = XPCfg<L>;
+  constexpr int C = Cfg::C, NT = Cfg::NT;
+  constexpr int EPT = Cfg::EPT;
+  static_assert(EPT >= 1 && EPT * NT == C * L && (L / 2) % EPT == 0, "tile must split into monotone runs");
+  extern __shared__ float2 lds[];
+  __shared__ float2 stw[L];
+  const int tid = threadIdx.x;
+  load_twiddles<L, NT>(stw, a.tw);
+  const int c = tid % C;
+  const int l0 = (tid / C) * EPT;
+  const int nedges = a.g.nedges, nbins = nedges - 1;
+  double* thr = reinterpret_cast<double*>(lds + C * L);
+  double* hist = thr + nedges;
+  for (int i = tid; i < nedges; i += NT) thr[i] = a.g.thr[i];
+  for (int i = tid; i < NPP * nbins * 2; i += NT) hist[i] = 0.;
+  float2 reg[EPT];
+  auto load = [&](int tile, int m) {
+    const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
+    const float2* src = a.Z + m * a.mesh_stride + (int64_t)y * a.ostride + c0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = tid + e * NT, cc = idx % C, l = idx / C;
+      reg[e] = (c0 + cc < a.ncols) ? src[l * a.fstride + cc] : make_float2(0.f, 0.f);
+    }
+  };
+  float2 acc[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
+  int tile = blockIdx.x, m = 0;
+  if (tile < a.ntiles) load(tile, 0);
+  while (tile < a.ntiles) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];
+    __syncthreads();
+    int ntile = tile, nm = m + 1;
+    if (nm == a.nm) {
+      nm = 0;
+      ntile += gridDim.x;
+    }
+    if (ntile < a.ntiles) load(ntile, nm);
+    fft_lds<L, C, C, 1, true, NT, false, L>(lds, stw);
+    const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
+    const int kz = min(c0 + c, a.ncols - 1);
+    KPrologue kp = a.kp;
+    kp.m = m - a.ell;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) acc[e] = cadd(acc[e], cscale(lds[(l0 + e) * C + c], kp.eval(l0 + e, y, kz)));
+    if (m == a.nm - 1) {
+      if (c0 + c < a.ncols) {
+        const double kyv = a.g.ky[y], kzv = a.g.kz[kz];
+        const float w = kz > 0 ? 2.f : 1.f;
+        double sum[NPP][2];
+#pragma unroll
+        for (int p = 0; p < NPP; ++p) sum[p][0] = sum[p][1] = 0.;
+        int b = -1;
+        const float2* a0 = a.A0 + (int64_t)y * a.a0_ostride + kz;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          const int l = l0 + e;
+          const double k2 = k2_sum(k2_xy(a.g.kx[l], kyv), kzv);
+          const int nb = e == 0 ? first_bin(thr, nedges, k2) : walk_bin(thr, nedges, b, k2);
+          if (nb != b) {
+            if (b >= 1 && b < nedges) {
+#pragma unroll
+              for (int p = 0; p < NPP; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                  if (sum[p][q] != 0.) atomicAdd(&hist[(p * nbins + b - 1) * 2 + q], sum[p][q]);
+                  sum[p][q] = 0.;
+                }
+            }
+            b = nb;
+          }
+          if (b < 1 || b >= nedges) continue;
+          const float2 A = a0[l * a.a0_fstride];
+          const float2 Cv = acc[e];
+          const bool origin = (l == 0) & (y == 0) & (kz == 0);
+          if (!origin) {  // remove_zero for the ell > 0 product
+            sum[0][0] += (double)((Cv.x * A.x + Cv.y * A.y) * w);
+            sum[0][1] += (double)((Cv.x * A.y - Cv.y * A.x) * w);
+          }
+          if (NPP > 1) sum[NPP - 1][0] += (double)((A.x * A.x + A.y * A.y) * w);
+        }
+        if (b >= 1 && b < nedges) {
+#pragma unroll
+          for (int p = 0; p < NPP; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+              if (sum[p][q] != 0.) atomicAdd(&hist[(p * nbins + b - 1) * 2 + q], sum[p][q]);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    tile = ntile;
+    m = nm;
+  }
+  __syncthreads();
+  double* dst = a.partial + (int64_t)blockIdx.x * NPP * nbins * 2;
+  for (int i = tid; i < NPP * nbins * 2; i += NT) dst[i] = hist[i];
+}
+
+template <int L, int NPP>
+static int launch_ylm_bin(hipStream_t s, YlmBinArgs& a, double* partial, size_t partial_doubles, int* nblocks_out) {
+  using Cfg = XPCfg<L>;
+  auto kern = xpass_ylm_bin_kernel<L, NPP>;
+  a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
+  a.ntiles = a.nchunks * a.ny;
+  const int nbins = a.g.nedges - 1;
+  const size_t lds = Cfg::LDS + (size_t)a.g.nedges * 8 + (size_t)NPP * nbins * 2 * 8;
+  if (lds > 150 * 1024) return set_error(MP_ERR_UNSUPPORTED, "ylm bin: %zu bytes of LDS needed", lds);
+  if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per = 0;
+  MP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, Cfg::NT, lds));
+  if (per < 1) per = 1;
+  int blocks = multiprocessors() * per;
+  if (blocks > a.ntiles) blocks = a.ntiles;
+  const int cap = (int)(partial_doubles / ((size_t)NPP * nbins * 2));
+  if (cap < 1) return set_error(MP_ERR_WORKSPACE, "ylm bin: no room for the partial histograms");
+  if (blocks > cap) blocks = cap;
+  a.partial = partial;
+  *nblocks_out = blocks;
+  MP_LAUNCH("fft_xpass_ylm_bin", kern, dim3((unsigned)blocks), dim3(Cfg::NT), lds, s, a);
+  return MP_OK;
+}
+
+template <int L>
+static int ylm_bin_L(hipStream_t s, YlmBinArgs& a, double* partial, size_t pd, int* nblocks) {
+  return a.with0 ? launch_ylm_bin<L, 2>(s, a, partial, pd, nblocks) : launch_ylm_bin<L, 1>(s, a, partial, pd, nblocks);
+}
+
 // Per-axis interlacing phases exp(i s_j k_circ) and compensation sinc(k/2pi)^-order, in FP64
 // then rounded to FP32.  k_circ = 2 pi fftfreq(n) (rfftfreq on the last axis), mesh.py:1605-1636.
 __global__ void combine_tables_kernel(int n0, int n1, int n2, int ni, int order, float2* __restrict__ ph, float* __restrict__ comp) {
@@ -373,6 +528,22 @@ static int xpass_bin_L(hipStream_t s, XPassArgs& a, int nprod, double* partial, 
   }
 }
 
+#define MP_XPASS_SWITCH2(n, F, ...)                                        \
+  switch (n) {                                                             \
+    case 2: rc = F<2>(__VA_ARGS__); break;                                 \
+    case 4: rc = F<4>(__VA_ARGS__); break;                                 \
+    case 8: rc = F<8>(__VA_ARGS__); break;                                 \
+    case 16: rc = F<16>(__VA_ARGS__); break;                               \
+    case 32: rc = F<32>(__VA_ARGS__); break;                               \
+    case 64: rc = F<64>(__VA_ARGS__); break;                               \
+    case 128: rc = F<128>(__VA_ARGS__); break;                             \
+    case 256: rc = F<256>(__VA_ARGS__); break;                             \
+    case 512: rc = F<512>(__VA_ARGS__); break;                             \
+    case 1024: rc = F<1024>(__VA_ARGS__); break;                           \
+    case 2048: rc = F<2048>(__VA_ARGS__); break;                           \
+    default: return set_error(MP_ERR_UNSUPPORTED, "x length %d", (int)(n)); \
+  }
+
 #define MP_XPASS_SWITCH(n, F, ...)                                                                      \
   switch (n) {                                                                                          \
     case 2: return F<2>(__VA_ARGS__);                                                                   \
@@ -487,6 +658,67 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   }
   if (rc) return rc;
   return reduce_partials(s, pbuf, nblocks, nprod * (g.nedges - 1) * 2, out);
+}
+
+size_t ylm_bin_workspace_bytes(const int32_t n[3], int ell, int nedges) {
+  const size_t nspec = (size_t)n[0] * n[1] * spectrum_pitch(n[2]);
+  return align_up((size_t)(2 * ell + 1) * nspec * 8, 256) + align_up((size_t)XB_MAX_BLOCKS * 2 * (nedges - 1) * 2 * 8, 256) + 512;
+}
+
+int ylm_bin(hipStream_t s, const float* rmesh, const float2* A0, int64_t a0_pitch, const int32_t n[3], const double boxsize[3],
+            const double boxcenter[3], int ell, int with0, const BinGeom& g, double* out, void* ws, size_t ws_bytes) {
+  MP_REQUIRE(rmesh && A0 && out, "ylm_bin: null pointer");
+  MP_REQUIRE(ell >= 1 && ell <= MAX_ELL, "ylm_bin: ell %d not in 1..%d", ell, MAX_ELL);
+  if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "ylm_bin: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
+  if ((with0 ? 2 : 1) * (g.nedges - 1) * 2 > BIN_LDS_DOUBLES) return set_error(MP_ERR_UNSUPPORTED, "ylm_bin: too many bins");
+  MP_REQUIRE(a0_pitch >= n[2] / 2 + 1, "ylm_bin: A0 pitch < Nz/2+1");
+  const int64_t P = spectrum_pitch(n[2]);
+  const int64_t nspec = (int64_t)n[0] * n[1] * P;
+  const int nm = 2 * ell + 1;
+  Workspace wsp(ws, ws_bytes);
+  float2* Z = wsp.take<float2>((size_t)nm * nspec);
+  const size_t pd = (size_t)XB_MAX_BLOCKS * 2 * (g.nedges - 1) * 2;
+  double* pbuf = wsp.take<double>(pd);
+  if (!Z || !pbuf) return set_error(MP_ERR_WORKSPACE, "ylm_bin: workspace of %zu bytes < %zu", ws_bytes, ylm_bin_workspace_bytes(n, ell, g.nedges));
+  RealPrologue pro{};
+  pro.kind = PRO_YLM;
+  pro.ell = ell;
+  pro.ny = n[1];
+  pro.nz = n[2];
+  pro.x0 = (float)(boxcenter[0] - boxsize[0] / 2.);
+  pro.y0 = (float)(boxcenter[1] - boxsize[1] / 2.);
+  pro.z0 = (float)(boxcenter[2] - boxsize[2] / 2.);
+  pro.dx = (float)(boxsize[0] / n[0]);
+  pro.dy = (float)(boxsize[1] / n[1]);
+  pro.dz = (float)(boxsize[2] / n[2]);
+  int rc = rfft_zy_ylm(s, rmesh, Z, nspec, n, P, pro);
+  if (rc) return rc;
+  YlmBinArgs a{};
+  a.Z = Z;
+  a.mesh_stride = nspec;
+  a.ostride = P;
+  a.fstride = (int64_t)n[1] * P;
+  a.A0 = A0;
+  a.a0_ostride = a0_pitch;
+  a.a0_fstride = (int64_t)n[1] * a0_pitch;
+  a.nm = nm;
+  a.ell = ell;
+  a.ncols = n[2] / 2 + 1;
+  a.ny = n[1];
+  a.kp.ell = ell;
+  a.kp.nx = n[0];
+  a.kp.ny = n[1];
+  a.kp.kfx = (float)(2. * M_PI / boxsize[0]);
+  a.kp.kfy = (float)(2. * M_PI / boxsize[1]);
+  a.kp.kfz = (float)(2. * M_PI / boxsize[2]);
+  a.tw = twiddle_device_ptr();
+  if (!a.tw) return set_error(MP_ERR_HIP, "ylm_bin: twiddle table address");
+  a.g = g;
+  a.with0 = with0 ? 1 : 0;
+  int nblocks = 0;
+  MP_XPASS_SWITCH2(n[0], ylm_bin_L, s, a, pbuf, pd, &nblocks);
+  if (rc) return rc;
+  return reduce_partials(s, pbuf, nblocks, (with0 ? 2 : 1) * (g.nedges - 1) * 2, out);
 }
 
 }  // namespace mp

@@ -213,24 +213,29 @@ def compute_mesh2_spectrum(*meshes, bin=None, los='z'):
             meshes = meshes[::-1]
         rmesh1 = meshes[0]
         A0 = _2c(rmesh1 if same else meshes[1])
-        bmeshes, products, scales = [A0.value], [], []
-        if 0 in ells:
-            if same:
-                products.append((0, 0, 0, False))
-            else:
-                bmeshes.append(_2c(rmesh1).value)
-                products.append((len(bmeshes) - 1, 0, 0, False))
-            scales.append(1.)
-        if nonzeroells:
-            rreal = _2r(rmesh1)
+        rreal = _2r(rmesh1) if nonzeroells else None
+        sums = _ylm_bin_all(bin, rreal, A0, nonzeroells, with0=(0 in ells and same)) if nonzeroells else None
+        if sums is not None:  # fused: C_ell never stored
+            if 0 in ells and 0 not in sums:
+                sums[0] = bin._products_dev([A0.value, _2c(rmesh1).value], [(1, 0, 0, False)])[0]
+            num = [sums[ell] * ((4. * np.pi if ell else 1.) / bin._nmodes_dev) for ell in ells]
+        else:
+            bmeshes, products, scales = [A0.value], [], []
+            if 0 in ells:
+                if same:
+                    products.append((0, 0, 0, False))
+                else:
+                    bmeshes.append(_2c(rmesh1).value)
+                    products.append((len(bmeshes) - 1, 0, 0, False))
+                scales.append(1.)
             for ell in nonzeroells:
                 C = mattrs.create('complex', device=A0.value.device).value
                 _ylm_accumulate(rreal, ell, C)
                 bmeshes.append(C)
                 products.append((len(bmeshes) - 1, 0, 0, True))
                 scales.append(4. * np.pi)
-        sums = bin._products_dev(bmeshes, products)
-        num = [sums[i] * (scale / bin._nmodes_dev) for i, scale in enumerate(scales)]
+            sums = bin._products_dev(bmeshes, products)
+            num = [sums[i] * (scale / bin._nmodes_dev) for i, scale in enumerate(scales)]
         if swap:
             num = [torch.conj(n).resolve_conj() for n in num]
     else:  # fixed line of sight, mesh2.py:346-374
@@ -254,6 +259,40 @@ def compute_mesh2_spectrum(*meshes, bin=None, los='z'):
                                norm=norm, volume=np.prod(mattrs.kfun) * bin.nmodes, ell=ell, attrs=attrs)
              for i, ell in enumerate(ells)]
     return Mesh2SpectrumPoles(poles, attrs=attrs)
+
+
+def _ylm_bin_all(bin, rreal, A0, ells, with0=False, memory_fraction=0.5):
+    """{ell: complex sums (nbins,)} of conj(C_ell) A0 over every ell > 0 with the fused kernel
+    (mp_ylm_bin), plus |A0|^2 under key 0 when with0; None when the fused kernel does not apply
+    (shape, bins, or its 2 ell + 1 partial spectra would take more than memory_fraction of the
+    free device memory), in which case the caller stores C_ell."""
+    a = rreal.attrs
+    device = rreal.value.device
+    n = _lib.i32x3(a.meshsize)
+    nedges = len(bin._edges1d)
+    lib = _lib.load()
+    need = max(int(lib.mp_ylm_bin_workspace_bytes(n, ell, nedges)) for ell in ells)
+    if need > memory_fraction * torch.cuda.mem_get_info(device)[0]:
+        return None
+    value = rreal.value.contiguous()
+    a0, pitch = _as_spectrum(A0.value)
+    t = bin._tables
+    out = {}
+    for i, ell in enumerate(ells):
+        w0 = bool(with0 and i == 0)
+        res = torch.empty((2 if w0 else 1, bin.nbins, 2), dtype=torch.float64, device=device)
+        ws = _lib.workspace(lib.mp_ylm_bin_workspace_bytes(n, ell, nedges), device)
+        try:
+            _lib.call('mp_ylm_bin', _lib.stream_handle(device), _lib.ptr(value), _lib.ptr(a0), pitch, n, _lib.f64x3(a.boxsize),
+                      _lib.f64x3(a.boxcenter), ell, int(w0), _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']),
+                      _lib.ptr(t['thr']), nedges, _lib.ptr(res), _lib.ptr(ws), ws.numel())
+        except NotImplementedError:
+            return None
+        res = torch.view_as_complex(res)
+        out[ell] = res[0]
+        if w0:
+            out[0] = res[1]
+    return out
 
 
 def _ylm_accumulate(rmesh, ell, out):

@@ -136,6 +136,39 @@ def test_paint_float32_positions_and_unit_weights():
     assert np.isclose(float(mesh.sum()), 20000., rtol=1e-5)
 
 
+@pytest.mark.parametrize('ni', [1, 2, 3])
+def test_ngp_exact_ties_round_half_even(ni):
+    """Particles exactly on cell centres and cell boundaries: NGP (and the interlaced shift of
+    1/2) lands on exact halves, which jnp.round sends to the even cell.  Checked for the
+    real-space row paint and the fused paint + z-FFT path."""
+    from jaxpower_amd import _lib, MeshAttrs, ParticleField
+    say('ngp ties', ni)
+    shape = (16, 16, 32)
+    attrs = MeshAttrs(meshsize=shape, boxsize=(160., 160., 320.), boxcenter=(80., 80., 160.))
+    axes = [np.arange(n) * 10. for n in shape]
+    centres = np.stack(np.meshgrid(*axes, indexing='ij'), -1).reshape(-1, 3)
+    pos = np.concatenate([centres, centres + 5., centres + np.array([5., 0., 5.])])
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.5, 1.5, len(pos)).astype('f4')
+    xg = _grid(attrs, pos)
+    assert np.all(xg * 2. == np.round(xg * 2.))  # exact halves
+    out = torch.empty((ni,) + shape, dtype=torch.float32, device=DEV)
+    n = _lib.i32x3(shape)
+    ws = _lib.workspace(_lib.load().mp_paint_workspace_bytes(n, len(pos), 1, ni), DEV)
+    dpos, dw = to_dev(pos), to_dev(w)  # held until the call has run
+    _lib.call('mp_paint', _lib.stream_handle(), _lib.ptr(dpos), 1, _lib.ptr(dw), len(pos), n,
+              _lib.f64x3(attrs.boxsize), _lib.f64x3(attrs.boxcenter), 1, ni, _lib.ptr(out), _lib.ptr(ws), ws.numel())
+    got = host(out)
+    for j in range(ni):
+        want = po.paint_grid(shape, xg + j / ni, w.astype('f8'), 1)
+        assert np.abs(got[j] - want).max() <= 2e-6 * np.abs(want).max(), j
+    p = ParticleField(to_dev(pos), to_dev(w), attrs=attrs)
+    got = host(p.paint(resampler='ngp', interlacing=ni if ni > 1 else 0, compensate=True, out='complex').value)
+    want = po.paint(po.Attrs(shape, attrs.boxsize, attrs.boxcenter), pos, w.astype('f8'), resampler='ngp',
+                    interlacing=ni if ni > 1 else 0, compensate=True, out='complex')
+    assert np.abs(got - want).max() <= 3e-6 * np.abs(want).max()
+
+
 @pytest.mark.parametrize('resampler', ['cic', 'tsc', 'pcs'])
 @pytest.mark.parametrize('interlacing', [0, 2, 3])
 @pytest.mark.parametrize('compensate', [False, True])

@@ -156,3 +156,32 @@ def test_bin_poles_partial_abi():
     assert _bin_abi(lib, stream, partial, pitch, n, order, ni, 1, 1, tb, 5000, los, ells, rz, out, ws) == -2
     assert _bin_abi(lib, stream, partial, pitch, n, 7, ni, 1, 1, t, nedges, los, ells, rz, out, ws) == -1
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize('los,cross', [('firstpoint', False), ('endpoint', True), ('firstpoint', True)])
+def test_local_los_fused_matches_stored(los, cross, monkeypatch):
+    """The fused local line of sight (mp_ylm_bin: C_ell never stored) against the path that
+    stores C_ell and bins it with mp_bin_poles, and against the oracle."""
+    from jaxpower_amd import MeshAttrs, RealMeshField, BinMesh2SpectrumPoles, compute_mesh2_spectrum
+    from jaxpower_amd import mesh2 as m2
+    say('local fused', los, cross)
+    rng = np.random.default_rng(44)
+    shape = (64, 32, 32)
+    attrs = MeshAttrs(meshsize=shape, boxsize=(1280., 640., 640.), boxcenter=(500., -200., 1400.))
+    m1 = rng.normal(size=shape).astype('f4')
+    m2_ = (0.6 * m1 + 0.5 * rng.normal(size=shape)).astype('f4')
+    ells = (0, 2, 4)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.005}, ells=ells)
+    meshes = [RealMeshField(to_dev(m1), attrs=attrs)] + ([RealMeshField(to_dev(m2_), attrs=attrs)] if cross else [])
+    fused = compute_mesh2_spectrum(*meshes, bin=bin, los=los)
+    orig = m2._ylm_bin_all
+    monkeypatch.setattr(m2, '_ylm_bin_all', lambda *args, **kw: None)
+    stored = compute_mesh2_spectrum(*meshes, bin=bin, los=los)
+    monkeypatch.setattr(m2, '_ylm_bin_all', orig)
+    got = np.array([fused.get(ell).values('num_raw') for ell in ells])
+    ref = np.array([stored.get(ell).values('num_raw') for ell in ells])
+    assert_poles_close(got, ref, rtol=1e-5, label='fused vs stored')
+    oattrs = po.Attrs(shape, attrs.boxsize, attrs.boxcenter)
+    want = po.spectrum_poles(oattrs, po.BinPoles(oattrs, edges={'step': 0.005}, ells=ells), m1.astype('f8'),
+                             m2_.astype('f8') if cross else None, los=los)
+    assert_poles_close(got, want['num_raw'], rtol=2e-5, label='fused vs oracle')

@@ -32,9 +32,9 @@ def test_host_queries_without_device():
     lib = _lib.load()
     assert lib.mp_spectrum_pitch(512) == 272 and lib.mp_spectrum_pitch(64) == 48 and lib.mp_spectrum_pitch(8) == 16
     n = _lib.i32x3([512, 512, 512])
-    tiled = lib.mp_paint_workspace_bytes(n, 16_000_000, 3, 3)
-    assert tiled >= 16_000_000 * 16 + 4 * (128**3)
-    assert lib.mp_paint_workspace_bytes(_lib.i32x3([20, 20, 20]), 1000, 3, 3) == 256  # atomic path needs none
+    rows = lib.mp_paint_workspace_bytes(n, 16_000_000, 3, 3)
+    assert rows >= 16_000_000 * 34  # row sort: two 16-byte records and a 2-byte key per particle
+    assert lib.mp_paint_workspace_bytes(_lib.i32x3([6, 6, 6]), 1000, 3, 3) == 256  # atomic path needs none
     assert lib.mp_bin_workspace_bytes(3, 161) >= 1024 * 3 * 160 * 16
     # argument errors are reported without touching a device
     rc = lib.mp_bin_poles(None, 0, None, None, n, None, None, None, None, 1, None, 0, None, None, None, None, None, None, 0)
