@@ -77,7 +77,7 @@ KERNEL_SYMBOLS = {
     'paint_brick_count': 'brick_count_kernel', 'paint_brick_scatter': 'brick_scatter_kernel',
     'paint_tile': 'paint_tile_kernel', 'paint_rowsort_hist': 'rowsort_hist_kernel',
     'paint_rowsort_scatter': 'rowsort_scatter_kernel', 'paint_rowsort_fine': 'rowsort_fine_kernel',
-    'paint_zfft': 'paint_zfft_kernel', 'fft_zpass_r2c': 'zpass_r2c_kernel', 'fft_ypass': 'col_fft_kernel',
+    'paint_zfft': 'paint_zfft_kernel', 'fft_zpass_r2c': 'zpass_r2c_kernel', 'fft_ypass': 'colpass_kernel<512, false',
     'fft_xpass_combine': 'xpass_kernel<512, false', 'fft_xpass_bin': 'xpass_kernel<512, true',
     'bin_poles': 'bin_poles_kernel',
 }

@@ -274,23 +274,47 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restri
     row_wabs[(int64_t)hx * ny + y] = (float)((double)qsum[y] * (double)m * (1.0 / 16777216.0));
 }
 
-// Stencil cells base + round/floor(u) + OMIN + o and weights K(|cell - u|), o in [0, ORDER).
-// base is relative to an even origin, so that an exact half rounds to the even mesh cell as
-// round-half-even on the mesh coordinate does (only NGP, whose kernel is discontinuous there,
-// depends on it).
+// Stencil of coordinate u (relative to cell base): returns the first cell,
+// base + round/floor(u) + OMIN, and sets k[o] = K(|first + o - base - u|), o in [0, ORDER), in
+// closed form in t = u - round/floor(u) (resamplers.py:63-69 evaluate K piecewise in |d|; the
+// polynomials are the same).  base is relative to an even origin, so that an exact half rounds
+// to the even mesh cell as round-half-even on the mesh coordinate does (only NGP, whose kernel
+// is discontinuous there, depends on it).
 template <int ORDER>
-__device__ __forceinline__ void stencil_1d(float u, int base, int* cell, float* k) {
-  constexpr int OMIN = -((ORDER - 1) / 2);
-  float r = (ORDER % 2) ? rintf(u) : floorf(u);
+__device__ __forceinline__ int stencil_1d(float u, int base, float* k) {
   if constexpr (ORDER == 1) {
+    float r = rintf(u);
     if (fabsf(u - r) == 0.5f && ((base + (int)r) & 1)) r = 2.f * u - r;
+    k[0] = 1.f;
+    return base + (int)r;
+  } else if constexpr (ORDER == 2) {
+    const float r = floorf(u), t = u - r;
+    k[0] = 1.f - t;
+    k[1] = t;
+    return base + (int)r;
+  } else if constexpr (ORDER == 3) {
+    const float r = rintf(u), t = u - r;  // t in [-1/2, 1/2]
+    const float a = 0.5f - t, b = 0.5f + t;
+    k[0] = 0.5f * a * a;
+    k[1] = 0.75f - t * t;
+    k[2] = 0.5f * b * b;
+    return base + (int)r - 1;
+  } else {
+    const float r = floorf(u), t = u - r, t1 = 1.f - t;  // t in [0, 1)
+    const float t2 = t * t, t12 = t1 * t1;
+    k[0] = t12 * t1 * (1.f / 6.f);
+    k[1] = (4.f - 6.f * t2 + 3.f * t2 * t) * (1.f / 6.f);
+    k[2] = (4.f - 6.f * t12 + 3.f * t12 * t1) * (1.f / 6.f);
+    k[3] = t2 * t * (1.f / 6.f);
+    return base + (int)r - 1;
   }
-  const int i0 = base + (int)r + OMIN;
-#pragma unroll
-  for (int o = 0; o < ORDER; ++o) {
-    cell[o] = i0 + o;
-    k[o] = kern1<ORDER>(fabsf(r + (float)(OMIN + o) - u));
-  }
+}
+
+// Fixed-point addend: round to nearest (halves up), in one instruction.
+__device__ __forceinline__ int to_fixed(float v) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
 }
 
 // One workgroup: RX x RY rows of shifted mesh j, painted in LDS, then z-FFT'd (r2c) in LDS.
@@ -335,8 +359,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   for (int i = tid; i < M * RP; i += PZ_NT) lds2[i] = make_float2(0.f, 0.f);
 
   // neighbourhood rows: homes in [x0 - hi, x0 + RX - 1 - lo]; each row's particles are
-  // contiguous and sorted by z cell.  Work items are chunks of up to CHUNK consecutive
-  // particles of one row.
+  // contiguous.  Work items are chunks of up to CHUNK consecutive particles of one row.
   // rows reached from home row h at this shift: h + [lo, hi].  The stencil origin of x~ + s,
   // x~ in [h, h+1), is h or h+1 for s < 1/2 with round, h+1 or h+2 for s > 1/2; at s = 1/2
   // float rounding of x~ + s can give any of the three.  With floor it is h, or h and h+1 for
@@ -429,24 +452,32 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
         int zc;
         float fz;
         z_of_record(__float_as_uint(rec.z), rg.zfrac, zc, fz);
-        int cx[ORDER], cy[ORDER], cz[ORDER];
         float kx[ORDER], ky[ORDER], kz[ORDER];
-        stencil_1d<ORDER>(rec.x + s, hxr, cx, kx);
-        stencil_1d<ORDER>(rec.y + s, hyr, cy, ky);
-        stencil_1d<ORDER>(fz + s, zc, cz, kz);
+        const int cx0 = stencil_1d<ORDER>(rec.x + s, hxr, kx);
+        const int cy0 = stencil_1d<ORDER>(rec.y + s, hyr, ky);
+        const int cz0 = stencil_1d<ORDER>(fz + s, zc, kz);
+        // cell (cx0 + ox, cy0 + oy, z) is word 2 ((cx0 + ox) RY + cy0 + oy) + zoff(z): one base
+        // per z cell (wrapped into the row), and the (ox, oy) part a constant offset.  The
+        // dynamic LDS follows the static tables, so a base below the first row stays >= 0.
+        int* zbase[ORDER];
+#pragma unroll
+        for (int oz = 0; oz < ORDER; ++oz) {
+          const int z = (cz0 + oz) & (NZ - 1);
+          int w = (z & ~1) * RP + (z & 1) + 2 * (cx0 * RY + cy0);
+          asm volatile("" : "+v"(w));  // computed once: the atomics below use it with constant offsets
+          zbase[oz] = irows + w;
+        }
+        const float ws = rec.w * scale;
 #pragma unroll
         for (int ox = 0; ox < ORDER; ++ox) {
-          if (!(cx[ox] >= 0 && cx[ox] < RX)) continue;
+          if ((unsigned)(cx0 + ox) >= (unsigned)RX) continue;
+          const float wx = ws * kx[ox];
 #pragma unroll
           for (int oy = 0; oy < ORDER; ++oy) {
-            if (!(cy[oy] >= 0 && cy[oy] < RY)) continue;
-            int* row = irows + 2 * (cx[ox] * RY + cy[oy]);
-            const float kxy = rec.w * kx[ox] * ky[oy] * scale;
+            if ((unsigned)(cy0 + oy) >= (unsigned)RY) continue;
+            const float kxy = wx * ky[oy];
 #pragma unroll
-            for (int oz = 0; oz < ORDER; ++oz) {
-              const int z = cz[oz] & (NZ - 1);
-              atomicAdd(row + 2 * RP * (z >> 1) + (z & 1), __float2int_rn(kxy * kz[oz]));
-            }
+            for (int oz = 0; oz < ORDER; ++oz) atomicAdd(zbase[oz] + 2 * (ox * RY + oy), to_fixed(kxy * kz[oz]));
           }
         }
       }
@@ -583,25 +614,30 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
       int zc;
       float fz;
       z_of_record(__float_as_uint(rec.z), rg.zfrac, zc, fz);
-      int cx[ORDER], cy[ORDER], cz[ORDER];
       float kx[ORDER], ky[ORDER], kz[ORDER];
-      stencil_1d<ORDER>(rec.x + s, hxr, cx, kx);
-      stencil_1d<ORDER>(rec.y + s, hyr, cy, ky);
-      stencil_1d<ORDER>(fz + s, zc, cz, kz);
+      const int cx0 = stencil_1d<ORDER>(rec.x + s, hxr, kx);
+      const int cy0 = stencil_1d<ORDER>(rec.y + s, hyr, ky);
+      const int cz0 = stencil_1d<ORDER>(fz + s, zc, kz);
+      int zoff[ORDER];
+#pragma unroll
+      for (int oz = 0; oz < ORDER; ++oz) {
+        const int z = cz0 + oz;
+        zoff[oz] = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+      }
+      const float ws = rec.w * scale;
 #pragma unroll
       for (int ox = 0; ox < ORDER; ++ox) {
-        if (!(cx[ox] >= 0 && cx[ox] < rxmax)) continue;
+        const int cx = cx0 + ox;
+        if (!(cx >= 0 && cx < rxmax)) continue;
+        const float wx = ws * kx[ox];
 #pragma unroll
         for (int oy = 0; oy < ORDER; ++oy) {
-          if (!(cy[oy] >= 0 && cy[oy] < rymax)) continue;
-          int* row = irows + (cx[ox] * RY + cy[oy]) * rs;
-          const float kxy = rec.w * kx[ox] * ky[oy] * scale;
+          const int cy = cy0 + oy;
+          if (!(cy >= 0 && cy < rymax)) continue;
+          int* row = irows + (cx * RY + cy) * rs;
+          const float kxy = wx * ky[oy];
 #pragma unroll
-          for (int oz = 0; oz < ORDER; ++oz) {
-            int z = cz[oz];
-            z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
-            atomicAdd(row + z, __float2int_rn(kxy * kz[oz]));
-          }
+          for (int oz = 0; oz < ORDER; ++oz) atomicAdd(row + zoff[oz], to_fixed(kxy * kz[oz]));
         }
       }
     }

@@ -67,6 +67,15 @@ __host__ __device__ inline size_t xpass_extra_lds(int L, int ncols, int ni, int 
 template <int L>
 using XPCfg = XCfgC<L, L < 512 ? (L >= 4 ? 16 : 64 / L) : 4096 / L>;
 
+// First tile of a persistent block.  Blocks are dispatched to the 8 XCDs round-robin; here
+// the blocks of one XCD take a contiguous run of tiles, so the two column chunks that share a
+// 128-byte line of the input meet in that XCD's L2 instead of being fetched by two XCDs.
+// Advancing by gridDim.x keeps the runs contiguous.
+__device__ __forceinline__ int xcd_first_tile() {
+  const int g = gridDim.x, b = blockIdx.x;
+  return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
+}
+
 template <int L, bool BIN, int NP, bool TAB>
 __global__ void __launch_bounds__(XPCfg<L>::NT)
 xpass_kernel(XPassArgs a) {
@@ -143,7 +152,7 @@ xpass_kernel(XPassArgs a) {
   float2 acc[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
-  int tile = blockIdx.x, j = 0;
+  int tile = xcd_first_tile(), j = 0;
   if (tile < a.ntiles) load(tile, 0);
   while (tile < a.ntiles) {
 #pragma unroll
@@ -297,7 +306,7 @@ xpass_ylm_bin_kernel(YlmBinArgs a) {
   float2 acc[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) acc[e] = make_float2(0.f, 0.f);
-  int tile = blockIdx.x, m = 0;
+  int tile = xcd_first_tile(), m = 0;
   if (tile < a.ntiles) load(tile, 0);
   while (tile < a.ntiles) {
 #pragma unroll
