@@ -153,6 +153,36 @@ int mp_bin_poles(void* stream, int nmesh, const void* const* meshes, const int64
                  const int32_t* prod_j, const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out,
                  void* workspace, size_t workspace_bytes);
 
+/* ---- correlation function ---------------------------------------------------------- */
+/* _make_edges2(kind='real') -> _get_bin_attrs without weights (jaxpower/mesh2.py:25-62,
+ * 120-158; jaxpower/mesh.py:2831-2863): nmodes[b] (exact cell counts) and ssum[b] = sum |s|
+ * for bins b = 0..nedges-2 over the full real mesh.  sx, sy, sz: device float64 separations
+ * fftfreq(N) * cellsize * N per index; s2_thresholds[e] = min{t : fl(sqrt(t)) >= edge[e]}. */
+int mp_bin_real_modes(void* stream, const int32_t meshsize[3], const double* sx, const double* sy, const double* sz,
+                      const double* s2_thresholds, int nedges, int64_t* nmodes, double* ssum);
+/* BinMesh2CorrelationPoles.__call__ with the Legendre weights of compute_mesh2_correlation
+ * (jaxpower/mesh2.py:175-184, 490-498): out[e * nbins + b] = sum over the cells of bin b of
+ * mesh * P_ell[e](mu_s), mu_s = s.los/|s| (0 at s = 0), the (0,0,0) cell skipped when
+ * remove_zero[e]; FP64 sums, at most 6 ells. */
+int64_t mp_bin_real_workspace_bytes(const int32_t meshsize[3], int nell, int nedges);
+int mp_bin_real_poles(void* stream, const float* mesh, const int32_t meshsize[3], const double* sx, const double* sy,
+                      const double* sz, const double* s2_thresholds, int nedges, const double los[3], int nell,
+                      const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace, size_t workspace_bytes);
+/* The spectrum products of compute_mesh2_correlation (jaxpower/mesh2.py:429-431, 447, 476-478):
+ * out = conj(a) * b on half spectra, or |a|^2 (imaginary part 0) when b is NULL; out may be a. */
+int mp_spectrum_product(void* stream, const void* a, int64_t a_pitch, const void* b, int64_t b_pitch,
+                        const int32_t meshsize[3], void* out, int64_t out_pitch);
+/* Real-space Y_lm weights of the local line-of-sight correlation (jaxpower/mesh2.py:447-450,
+ * jaxpower/utils.py:190-334): out (+)= in * Y_lm(c) with c = (cx[i], cy[j], cz[k]) (positions
+ * for xhat, separations for shat), Y = 0 at the zero vector; ell <= 4. */
+int mp_mul_ylm(void* stream, const float* in, float* out, const int32_t meshsize[3], const double* cx, const double* cy,
+               const double* cz, int ell, int m, int accumulate);
+/* mp_combine_partial writing |A|^2 (imaginary part 0) instead of A: the auto-correlation
+ * input Aell = |A|^2 of jaxpower/mesh2.py:476, without storing A. */
+int mp_combine_partial_power(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
+                             int interlacing, int compensate, int zero_dc, void* out, int64_t out_pitch, void* workspace,
+                             size_t workspace_bytes);
+
 /* ---- local line of sight ---------------------------------------------------------- */
 /* One term of the firstpoint estimator (jaxpower/mesh2.py:319-329):
  * out = sum_m r2c(rmesh * Y_lm(xhat)) * Y_lm(khat), ell in 1..4, x measured from the origin. */

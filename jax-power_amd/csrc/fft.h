@@ -68,7 +68,7 @@ struct BinGeom;
 bool xpass_supported(const int32_t n[3]);
 size_t combine_partial_workspace_bytes(const int32_t n[3], int ni);
 int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
-                    int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes);
+                    int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes, int power = 0);
 size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges);
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,

@@ -37,6 +37,7 @@ struct XPassArgs {
   CombineTables tab;
   float scale;  // 1 / n_i
   int zero_dc;  // the (0,0,0) mode of A is set to 0
+  int power;    // write mode: store |A|^2 (imaginary part 0) instead of A
   const float2* tw;
   // write mode
   float2* out;
@@ -187,6 +188,7 @@ xpass_kernel(XPassArgs a) {
             const int l = l0 + e;
             float2 v = cscale(acc[e], comp ? s * cx[l] : s);
             if (a.zero_dc && l == 0 && y == 0 && kz == 0) v = make_float2(0.f, 0.f);
+            if (a.power) v = make_float2(v.x * v.x + v.y * v.y, 0.f);
             dst[l * a.out_fstride] = v;
           }
         } else {
@@ -593,7 +595,7 @@ static void fill_common(XPassArgs& a, const float2* partial, int64_t pitch, cons
 size_t combine_partial_workspace_bytes(const int32_t n[3], int ni) { return xpass_tables_bytes(n, ni) + 512; }
 
 int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
-                    int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes) {
+                    int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes, int power) {
   MP_REQUIRE(partial && out, "combine: null pointer");
   MP_REQUIRE(ni >= 1, "combine: interlacing %d < 1", ni);
   if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "combine: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
@@ -610,6 +612,7 @@ int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const i
   a.out = out;
   a.out_ostride = out_pitch;
   a.out_fstride = (int64_t)n[1] * out_pitch;
+  a.power = power;
   MP_XPASS_SWITCH(n[0], xpass_write_L, s, a);
 }
 

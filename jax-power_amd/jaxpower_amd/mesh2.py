@@ -10,8 +10,8 @@ import torch
 from . import _lib
 from .mesh import (MeshAttrs, BaseMeshField, RealMeshField, ComplexMeshField, ParticleField, FKPField, format_meshes,
                    compute_normalization, compute_box_normalization, _as_spectrum, _sum_f64, _as_device_tensor,
-                   _default_device)
-from .types import Deferred, Mesh2SpectrumPole, Mesh2SpectrumPoles
+                   _default_device, _empty_spectrum, _spectrum_pitch, _c2r, _r2c)
+from .types import Deferred, Mesh2SpectrumPole, Mesh2SpectrumPoles, Mesh2CorrelationPole, Mesh2CorrelationPoles
 
 
 def _format_ells(ells):
@@ -47,12 +47,20 @@ def _kvectors(mattrs):
     return toret
 
 
-def _find_unique_edges(mattrs, xmin=0., xmax=np.inf, device=None):
-    """mesh.py:2788-2809: edges at mid-points between the distinct |k| of the lattice."""
-    kx, ky, kz = (torch.from_numpy(k).to(device) for k in _kvectors(mattrs))
+def _svectors(mattrs):
+    """Per-axis separations fftfreq(n) * cellsize * n, float64 (xcoords(kind='separation'),
+    mesh.py:361-403, 1573-1603)."""
+    return [np.fft.fftfreq(int(n)) * mattrs.cellsize[axis] * int(n) for axis, n in enumerate(mattrs.meshsize)]
+
+
+def _find_unique_edges(mattrs, xmin=0., xmax=np.inf, device=None, vectors=None, x0=None):
+    """mesh.py:2788-2809: edges at mid-points between the distinct norms of the lattice
+    vectors (wavenumbers by default, x0 the fundamental mode)."""
+    vectors = _kvectors(mattrs) if vectors is None else vectors
+    kx, ky, kz = (torch.from_numpy(k).to(device) for k in vectors)
     x2 = (kx[:, None, None]**2 + ky[None, :, None]**2 + kz[None, None, :]**2).reshape(-1)
     x2 = x2[(x2 >= xmin**2) & (x2 <= xmax**2)]
-    x0 = float(mattrs.kfun.min())
+    x0 = float(mattrs.kfun.min() if x0 is None else x0)
     keys = (x2 / (0.5 * x0)**2 + 0.5).to(torch.int64)
     keys, inverse = torch.unique(keys, return_inverse=True)
     first = torch.full((len(keys),), x2.numel(), dtype=torch.int64, device=x2.device)
@@ -306,11 +314,194 @@ def _ylm_accumulate(rmesh, ell, out):
               _lib.f64x3(a.boxcenter), ell, _lib.ptr(out), pitch, _lib.ptr(ws), ws.numel())
 
 
+class BinMesh2CorrelationPoles:
+    """Binning operator for mesh to correlation function (mesh2.py:120-184): separation shells
+    over the full real mesh, every cell weighted 1.  Edges as for the spectrum with the
+    smallest cellsize as the unit (mesh2.py:25-47, kind='real').  Only the default
+    configuration-space basis is on this path."""
+
+    def __init__(self, mattrs, edges=None, ells=0, mode_oversampling=0, basis=None, klimit=None, batch_size=None, device=None):
+        if not isinstance(mattrs, MeshAttrs):
+            mattrs = mattrs.attrs
+        if mode_oversampling:
+            raise NotImplementedError('mode_oversampling > 0 is not implemented on this path')
+        if basis is not None:
+            raise NotImplementedError(f"basis={basis!r} is not implemented on this path (only the default, basis=None)")
+        device = device or _default_device()
+        if edges is None:
+            edges = {}
+        if isinstance(edges, dict):
+            step = edges.get('step', None)
+            vec0 = float(np.min(mattrs.cellsize))
+            vecmax = vec0 * np.min(mattrs.meshsize) / 2.
+            if step is None:
+                edges = _find_unique_edges(mattrs, edges.get('min', 0.), edges.get('max', np.inf), device=device,
+                                           vectors=_svectors(mattrs), x0=vec0)
+            else:
+                edges = np.arange(edges.get('min', 0.), edges.get('max', vecmax), step)
+        edges = np.asarray(edges, dtype='f8')
+        if edges.ndim == 2:
+            assert np.allclose(edges[1:, 0], edges[:-1, 1])
+            edges = np.append(edges[:, 0], edges[-1, 1])
+        if len(edges) < 2:
+            raise ValueError('need at least two edges')
+        self.mattrs = mattrs
+        self.ells = _format_ells(ells)
+        self.basis, self.klimit, self.batch_size = basis, klimit, batch_size
+        self._edges1d = edges
+        self.edges = np.column_stack([edges[:-1], edges[1:]])
+        self._device = device
+        sx, sy, sz = _svectors(mattrs)
+        self._tables = dict(sx=torch.from_numpy(sx).to(device), sy=torch.from_numpy(sy).to(device),
+                            sz=torch.from_numpy(sz).to(device), thr=torch.from_numpy(_k2_thresholds(edges)).to(device))
+        nbins = len(edges) - 1
+        nmodes = torch.empty(nbins, dtype=torch.int64, device=device)
+        ssum = torch.empty(nbins, dtype=torch.float64, device=device)
+        t = self._tables
+        _lib.call('mp_bin_real_modes', _lib.stream_handle(device), _lib.i32x3(mattrs.meshsize), _lib.ptr(t['sx']),
+                  _lib.ptr(t['sy']), _lib.ptr(t['sz']), _lib.ptr(t['thr']), len(edges), _lib.ptr(nmodes), _lib.ptr(ssum))
+        self._nmodes_dev = nmodes.to(torch.float64)
+        self.nmodes = nmodes.cpu().numpy()
+        with np.errstate(invalid='ignore', divide='ignore'):
+            self.xavg = ssum.cpu().numpy() / self.nmodes
+
+    @property
+    def nbins(self):
+        return len(self.xavg)
+
+    def _sums_dev(self, value, ells, remove_zero, los=(0., 0., 0.)):
+        """FP64 sums over each separation bin of value * P_ell(mu_s) for each ell, (nell, nbins),
+        left on the device; value is a real (N0, N1, N2) float32 tensor."""
+        value = value.contiguous()
+        if value.dtype != torch.float32:
+            value = value.to(torch.float32)
+        device = value.device
+        nell = len(ells)
+        n = _lib.i32x3(self.mattrs.meshsize)
+        nedges = len(self._edges1d)
+        out = torch.empty((nell, self.nbins), dtype=torch.float64, device=device)
+        ws = _lib.workspace(_lib.load().mp_bin_real_workspace_bytes(n, nell, nedges), device)
+        col = lambda values: (_lib.ctypes.c_int32 * nell)(*[int(v) for v in values])
+        t = self._tables
+        _lib.call('mp_bin_real_poles', _lib.stream_handle(device), _lib.ptr(value), n, _lib.ptr(t['sx']), _lib.ptr(t['sy']),
+                  _lib.ptr(t['sz']), _lib.ptr(t['thr']), nedges, _lib.f64x3(los), nell, col(ells), col(remove_zero),
+                  _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        return out
+
+    def __call__(self, mesh, ell=0, remove_zero=False):
+        """Sum_{cells in bin} value / nmodes (mesh2.py:175-184); mesh is real."""
+        value = mesh.value if isinstance(mesh, BaseMeshField) else _as_device_tensor(mesh, device=self._device)
+        if value.is_complex():
+            raise ValueError('BinMesh2CorrelationPoles bins a real (configuration-space) mesh')
+        s = self._sums_dev(value, [0], [bool(remove_zero)])[0].cpu().numpy()
+        with np.errstate(invalid='ignore', divide='ignore'):
+            return s / self.nmodes
+
+
+def _product_spectrum(A1, A2=None):
+    """Half spectrum |A1|^2 (A2 None) or conj(A1) A2 on the device; a pending interlacing
+    combine is combined straight into |A|^2 without storing A."""
+    mattrs = A1.attrs
+    n = _lib.i32x3(mattrs.meshsize)
+    if A2 is None:
+        pending = A1._pending_combine()
+        if pending is not None:
+            device = pending.partial.device
+            out = _empty_spectrum(mattrs.meshsize, device)
+            ws = _lib.workspace(_lib.load().mp_combine_partial_workspace_bytes(n, pending.interlacing), device)
+            _lib.call('mp_combine_partial_power', _lib.stream_handle(device), _lib.ptr(pending.partial), pending.pitch, n,
+                      pending.order, pending.interlacing, int(pending.compensate), int(pending.zero_dc), _lib.ptr(out),
+                      _spectrum_pitch(out), _lib.ptr(ws), ws.numel())
+            return out
+    a, pa = _as_spectrum(A1.value)
+    out = _empty_spectrum(mattrs.meshsize, a.device)
+    if A2 is None:
+        b, pb = None, 0
+    else:
+        b, pb = _as_spectrum(A2.value)
+    _lib.call('mp_spectrum_product', _lib.stream_handle(a.device), _lib.ptr(a), pa, _lib.ptr(b) if b is not None else None, pb, n,
+              _lib.ptr(out), _spectrum_pitch(out))
+    return out
+
+
+def _mul_ylm(value, out, mattrs, coords, ell, m, accumulate):
+    """out (+)= value * Y_lm(coords) on real meshes; coords are three device float64 tables."""
+    _lib.call('mp_mul_ylm', _lib.stream_handle(value.device), _lib.ptr(value), _lib.ptr(out), _lib.i32x3(mattrs.meshsize),
+              _lib.ptr(coords[0]), _lib.ptr(coords[1]), _lib.ptr(coords[2]), int(ell), int(m), int(accumulate))
+
+
+def compute_mesh2_correlation(*meshes, bin=None, los='z'):
+    """Correlation-function multipoles from one (auto) or two (cross) meshes (mesh2.py:377-504):
+    xi = c2r(|A|^2) (or c2r(conj(A1) A2)) binned in separation shells with L_ell(mu_s) for a
+    fixed line of sight; for a local (firstpoint/endpoint) line of sight the ell > 0 terms are
+    4 pi sum_m c2r(conj(r2c(rho Y_lm(xhat))) A0) Y_lm(shat)."""
+    if not isinstance(bin, BinMesh2CorrelationPoles):
+        raise ValueError(f'bin must be BinMesh2CorrelationPoles, not {type(bin)}')
+    meshes, fields = format_meshes(*meshes, nmeshes=2)
+    mattrs = meshes[0].attrs
+    vcell = float(np.prod(mattrs.cellsize))
+    norm = np.prod(mattrs.meshsize, dtype='f8') / vcell * np.ones_like(bin.xavg)
+    los, vlos, swap = _format_los(los)
+    attrs = dict(los=vlos.tolist() if vlos is not None else los)
+    ells = bin.ells
+    same = fields[1] == fields[0]
+    nonzeroells = [ell for ell in ells if ell != 0]
+
+    if vlos is None:  # local (varying) line of sight, mesh2.py:416-469
+        if swap:
+            meshes = meshes[::-1]
+        rmesh1 = meshes[0]
+        A0 = _2c(rmesh1 if same else meshes[1])
+        num = {}
+        if 0 in ells:
+            P = _product_spectrum(A0) if same else _product_spectrum(_2c(rmesh1), A0)
+            num[0] = bin._sums_dev(_c2r(P, mattrs.meshsize), [0], [False])[0]
+            del P
+        if nonzeroells:
+            rreal = _2r(rmesh1).value.contiguous()
+            if rreal.dtype != torch.float32:
+                rreal = rreal.to(torch.float32)
+            device = rreal.device
+            offset = mattrs.boxcenter - mattrs.boxsize / 2.
+            xs = [torch.from_numpy(np.arange(int(n)) * float(d) + float(o)).to(device)
+                  for n, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]  # positions (mesh.py:1594-1603)
+            ss = [bin._tables['sx'], bin._tables['sy'], bin._tables['sz']]
+            a0 = A0.value
+            weighted = torch.empty_like(rreal)
+            for ell in nonzeroells:
+                carry = torch.zeros_like(rreal)
+                for m in range(-ell, ell + 1):
+                    _mul_ylm(rreal, weighted, mattrs, xs, ell, m, False)
+                    B = _r2c(weighted, mattrs.meshsize)
+                    _lib.call('mp_spectrum_product', _lib.stream_handle(device), _lib.ptr(B), _spectrum_pitch(B),
+                              _lib.ptr(_as_spectrum(a0)[0]), _as_spectrum(a0)[1], _lib.i32x3(mattrs.meshsize), _lib.ptr(B),
+                              _spectrum_pitch(B))
+                    _mul_ylm(_c2r(B, mattrs.meshsize), carry, mattrs, ss, ell, m, True)
+                    del B
+                num[ell] = bin._sums_dev(carry, [0], [True])[0] * (4. * np.pi)
+                del carry
+        num = [num[ell] / (bin._nmodes_dev * vcell) for ell in ells]
+    else:  # fixed line of sight, mesh2.py:471-504
+        A1 = _2c(meshes[0])
+        P = _product_spectrum(A1) if same else _product_spectrum(A1, _2c(meshes[1]))
+        xi = _c2r(P, mattrs.meshsize)
+        del P
+        sums = bin._sums_dev(xi, ells, [ell != 0 for ell in ells], los=vlos)
+        num = [sums[i] * ((2 * ell + 1) / (bin._nmodes_dev * vcell)) for i, ell in enumerate(ells)]
+    num = [n.contiguous() for n in num]
+    poles = [Mesh2CorrelationPole(s=bin.xavg, s_edges=bin.edges, nmodes=bin.nmodes, num_raw=num[i], num_shotnoise=0., norm=norm,
+                                  volume=vcell * bin.nmodes, ell=ell, attrs=attrs)
+             for i, ell in enumerate(ells)]
+    return Mesh2CorrelationPoles(poles, attrs=attrs)
+
+
 def compute_mesh2(*meshes, bin=None, los='z'):
     """Dispatch on the binning type (mesh2.py:225-250)."""
     if isinstance(bin, BinMesh2SpectrumPoles):
         return compute_mesh2_spectrum(*meshes, bin=bin, los=los)
-    raise ValueError(f'bin must be BinMesh2SpectrumPoles on this path, not {type(bin)}')
+    if isinstance(bin, BinMesh2CorrelationPoles):
+        return compute_mesh2_correlation(*meshes, bin=bin, los=los)
+    raise ValueError(f'bin must be BinMesh2SpectrumPoles or BinMesh2CorrelationPoles, not {type(bin)}')
 
 
 def compute_box2_normalization(*inputs, bin=None):

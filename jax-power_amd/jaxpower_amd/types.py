@@ -1,6 +1,7 @@
-"""Power-spectrum multipole containers with the API jax-power uses from lsstypes
-(``Mesh2SpectrumPole(s)``, constructed at jaxpower/mesh2.py:340-344, 370-374; registered at
-jaxpower/types.py:10-51, 93-94).
+"""Power-spectrum and correlation-function multipole containers with the API jax-power uses
+from lsstypes (``Mesh2SpectrumPole(s)``, constructed at jaxpower/mesh2.py:340-344, 370-374;
+``Mesh2CorrelationPole(s)``, mesh2.py:467-468, 502-503; registered at jaxpower/types.py:10-51,
+93-96).
 
 A pole holds float64 arrays: k (mode-averaged |k|), k_edges (nbins, 2), nmodes (int64),
 num_raw, num_shotnoise, norm, volume, and ``ell``.  ``value()`` is
@@ -82,34 +83,42 @@ def _host(x):
     return np.asarray(x)
 
 
-class Mesh2SpectrumPole:
+class _MeshPole:
+    """One multipole of a binned two-point measurement; ``_x`` names its coordinate ('k' for
+    spectra, 's' for correlation functions)."""
 
-    _coords_names = ('k',)
+    _x = 'k'
     _values_names = ('num_raw', 'num_shotnoise', 'norm', 'nmodes', 'volume')
 
-    def __init__(self, k=None, k_edges=None, nmodes=None, num_raw=None, num_shotnoise=None, norm=None, volume=None, ell=0,
-                 attrs=None):
+    def __init__(self, ell=0, attrs=None, **fields):
+        names = self._field_names()
+        unknown = set(fields) - set(names)
+        if unknown:
+            raise TypeError(f'{type(self).__name__}: unexpected fields {sorted(unknown)}')
         # fields may be device tensors or Deferred values: they are read back on first access
-        self._raw = dict(k=k, k_edges=k_edges, nmodes=nmodes, num_raw=num_raw, num_shotnoise=num_shotnoise, norm=norm,
-                         volume=volume)
+        self._raw = {name: fields.get(name) for name in names}
         self._cache = None
         self.ell = int(ell)
         self.attrs = dict(attrs or {})
 
+    @classmethod
+    def _field_names(cls):
+        return (cls._x, cls._x + '_edges', 'nmodes', 'num_raw', 'num_shotnoise', 'norm', 'volume')
+
     @property
     def _data(self):
         if self._cache is None:
-            r = self._raw
-            k = _host(r['k']).astype('f8')
-            nbins = len(k)
+            r, x = self._raw, self._x
+            xv = _host(r[x]).astype('f8')
+            nbins = len(xv)
             bcast = lambda v, dtype='f8': np.broadcast_to(_host(v).astype(dtype), (nbins,)).copy()
             nmodes = _host(r['nmodes'])
-            self._cache = dict(k=k, k_edges=_host(r['k_edges']).astype('f8'),
-                               nmodes=nmodes.astype('i8') if np.issubdtype(nmodes.dtype, np.integer) else bcast(nmodes),
-                               num_raw=bcast(r['num_raw']),
-                               num_shotnoise=bcast(0. if r['num_shotnoise'] is None else r['num_shotnoise']),
-                               norm=bcast(1. if r['norm'] is None else r['norm']),
-                               volume=bcast(0. if r['volume'] is None else r['volume']))
+            self._cache = {x: xv, x + '_edges': _host(r[x + '_edges']).astype('f8'),
+                           'nmodes': nmodes.astype('i8') if np.issubdtype(nmodes.dtype, np.integer) else bcast(nmodes),
+                           'num_raw': bcast(r['num_raw']),
+                           'num_shotnoise': bcast(0. if r['num_shotnoise'] is None else r['num_shotnoise']),
+                           'norm': bcast(1. if r['norm'] is None else r['norm']),
+                           'volume': bcast(0. if r['volume'] is None else r['volume'])}
         return self._cache
 
     def clone(self, **kwargs):
@@ -122,11 +131,11 @@ class Mesh2SpectrumPole:
         d = self._data
         return (d['num_raw'] - d['num_shotnoise']) / d['norm']
 
-    def coords(self, name='k'):
-        return self._data[name]
+    def coords(self, name=None):
+        return self._data[name or self._x]
 
-    def edges(self, name='k'):
-        return self._data[name + '_edges']
+    def edges(self, name=None):
+        return self._data[(name or self._x) + '_edges']
 
     def values(self, name='value'):
         if name == 'value':
@@ -137,26 +146,46 @@ class Mesh2SpectrumPole:
 
     @property
     def shape(self):
-        return self._data['k'].shape
+        return self._data[self._x].shape
 
-    def select(self, k=None):
-        """Keep bins with kmin <= k < kmax (k given as (kmin, kmax)) or a slice of bins."""
-        if k is None:
+    def select(self, **kwargs):
+        """Keep bins with xmin <= x < xmax (x given as (xmin, xmax) under the coordinate's
+        name, e.g. k=(0.01, 0.2)) or a slice of bins."""
+        sel = kwargs.pop(self._x, None)
+        if kwargs:
+            raise TypeError(f'select: unexpected arguments {sorted(kwargs)}')
+        if sel is None:
             return self
-        if isinstance(k, slice):
+        if isinstance(sel, slice):
             mask = np.zeros(self.shape, dtype=bool)
-            mask[k] = True
+            mask[sel] = True
         else:
-            kk = self._data['k']
-            mask = (kk >= k[0]) & (kk < k[1])
+            xx = self._data[self._x]
+            mask = (xx >= sel[0]) & (xx < sel[1])
         data = {name: v[mask] for name, v in self._data.items()}
         return type(self)(**data, ell=self.ell, attrs=self.attrs)
 
     def __repr__(self):
-        return f'Mesh2SpectrumPole(ell={self.ell}, nbins={len(self._data["k"])})'
+        return f'{type(self).__name__}(ell={self.ell}, nbins={len(self._data[self._x])})'
 
 
-class Mesh2SpectrumPoles:
+class Mesh2SpectrumPole(_MeshPole):
+    """Power-spectrum multipole: k (mode-averaged |k|), k_edges, nmodes, num_raw,
+    num_shotnoise, norm, volume, ell (mesh2.py:340-344, 370-374)."""
+
+    _x = 'k'
+
+
+class Mesh2CorrelationPole(_MeshPole):
+    """Correlation-function multipole: s (cell-averaged |s|), s_edges, nmodes, num_raw,
+    num_shotnoise, norm, volume, ell (mesh2.py:467-468, 502-503)."""
+
+    _x = 's'
+
+
+class _MeshPoles:
+
+    _pole = Mesh2SpectrumPole
 
     def __init__(self, poles, attrs=None):
         self._branches = list(poles)
@@ -170,7 +199,7 @@ class Mesh2SpectrumPoles:
         if ells is None:
             ells = kwargs.get('ell')
         if isinstance(ells, (list, tuple)):
-            return Mesh2SpectrumPoles([self.get(ell) for ell in ells])
+            return type(self)([self.get(ell) for ell in ells])
         for p in self._branches:
             if p.ell == ells:
                 return p
@@ -198,10 +227,10 @@ class Mesh2SpectrumPoles:
     def values(self, name='value'):
         return [p.values(name) for p in self._branches]
 
-    def coords(self, name='k'):
+    def coords(self, name=None):
         return self._branches[0].coords(name)
 
-    def edges(self, name='k'):
+    def edges(self, name=None):
         return self._branches[0].edges(name)
 
     def select(self, **kwargs):
@@ -214,24 +243,35 @@ class Mesh2SpectrumPoles:
             for name, v in p._data.items():
                 arrays[f'ell{p.ell}_{name}'] = v
         arrays['ells'] = np.array(self.ells)
+        arrays['kind'] = np.array(type(self).__name__)
         los = self.attrs.get('los')
         if los is not None:
             arrays['los'] = np.asarray(los)
         np.savez(fn, **arrays)
 
     def __repr__(self):
-        return f'Mesh2SpectrumPoles(ells={self.ells})'
+        return f'{type(self).__name__}(ells={self.ells})'
+
+
+class Mesh2SpectrumPoles(_MeshPoles):
+    _pole = Mesh2SpectrumPole
+
+
+class Mesh2CorrelationPoles(_MeshPoles):
+    _pole = Mesh2CorrelationPole
 
 
 def read(fn):
-    """Read a Mesh2SpectrumPoles written by ``write``."""
+    """Read a Mesh2SpectrumPoles or Mesh2CorrelationPoles written by ``write``."""
     with np.load(fn, allow_pickle=False) as data:
+        kind = str(data['kind']) if 'kind' in data.files else 'Mesh2SpectrumPoles'
+        cls = {'Mesh2SpectrumPoles': Mesh2SpectrumPoles, 'Mesh2CorrelationPoles': Mesh2CorrelationPoles}[kind]
         attrs = {'los': data['los'].tolist()} if 'los' in data.files else {}
         poles = []
         for ell in data['ells']:
-            kw = {name: data[f'ell{ell}_{name}'] for name in ['k', 'k_edges', 'nmodes', 'num_raw', 'num_shotnoise', 'norm', 'volume']}
-            poles.append(Mesh2SpectrumPole(**kw, ell=int(ell), attrs=attrs))
-    return Mesh2SpectrumPoles(poles, attrs=attrs)
+            kw = {name: data[f'ell{ell}_{name}'] for name in cls._pole._field_names()}
+            poles.append(cls._pole(**kw, ell=int(ell), attrs=attrs))
+    return cls(poles, attrs=attrs)
 
 
 def write(fn, observable):

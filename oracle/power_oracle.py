@@ -291,10 +291,15 @@ def make_edges(attrs, edges=None):
 
 
 def find_unique_edges(attrs, xmin=0., xmax=np.inf):
-    """mesh.py:2788-2809 (single device)."""
-    kx, ky, kz = attrs.kcoords('wavenumber')
+    """mesh.py:2788-2809 (single device), on the wavenumbers."""
+    return unique_edges(attrs.kcoords('wavenumber'), attrs.kfun.min(), xmin, xmax)
+
+
+def unique_edges(vec, x0, xmin=0., xmax=np.inf):
+    """_find_unique_edges, mesh.py:2788-2809: bin edges midway between the distinct norms of
+    the lattice vectors vec (three 1-D coordinate arrays), x0 the smallest spacing."""
+    kx, ky, kz = vec
     x2 = (kx[:, None, None]**2 + ky[None, :, None]**2 + kz[None, None, :]**2).ravel()
-    x0 = attrs.kfun.min()
     x2 = x2[(x2 >= xmin**2) & (x2 <= xmax**2)]
     _, index = np.unique(np.int64(x2 / (0.5 * x0)**2 + 0.5), return_index=True)
     x = np.sqrt(x2[index])
@@ -536,4 +541,158 @@ def local_los_bruteforce(attrs, rmesh, ell):
                 ph = np.exp(-2j * np.pi * (a * ix[:, 0] / nx + b * ix[:, 1] / ny + c * ix[:, 2] / nz))
                 C = (2 * ell + 1) * np.sum(rho * pl * ph)
                 out[a, b, c] = np.conj(C) * A0[a, b, c]
+    return out
+
+
+# ----------------------------------------------------------------------------
+# Correlation function multipoles (mesh2.py:120-184, 377-504)
+# ----------------------------------------------------------------------------
+
+def separation_coords(attrs):
+    """xcoords(kind='separation'): fftfreq(N) * cellsize * N per axis (mesh.py:1573-1603,
+    361-403)."""
+    return [np.fft.fftfreq(int(n)) * c * n for n, c in zip(attrs.meshsize, attrs.cellsize)]
+
+
+def snorm(attrs):
+    """|s| on the full real mesh, summed x, y, z in that order (mesh2.py:54)."""
+    sx, sy, sz = separation_coords(attrs)
+    s2 = sx[:, None, None]**2 + sy[None, :, None]**2
+    s2 = s2 + sz[None, None, :]**2
+    return np.sqrt(s2)
+
+
+def make_sedges(attrs, edges=None):
+    """_make_edges2(kind='real'), mesh2.py:25-47: vec0 = smallest cellsize, default max
+    vec0 * min(N) / 2; arrays pass through."""
+    if edges is None:
+        edges = {}
+    if isinstance(edges, dict):
+        step = edges.get('step', None)
+        vec0 = attrs.cellsize.min()
+        vecmax = vec0 * np.min(attrs.meshsize) / 2.
+        if step is None:
+            return unique_edges(separation_coords(attrs), vec0, edges.get('min', 0.), edges.get('max', np.inf))
+        return np.arange(edges.get('min', 0.), edges.get('max', vecmax), step)
+    edges = np.asarray(edges, dtype='f8')
+    if edges.ndim == 2:
+        edges = np.append(edges[:, 0], edges[-1, 1])
+    return edges
+
+
+class BinCorrelationPoles:
+    """BinMesh2CorrelationPoles (mesh2.py:120-184), basis=None, mode_oversampling=0: every
+    cell of the real mesh weighted 1, bins by digitize(|s|, edges, right=False)."""
+
+    def __init__(self, attrs, edges=None, ells=0):
+        self.attrs = attrs
+        edges = make_sedges(attrs, edges)
+        snrm = snorm(attrs)
+        self.ibin = np.digitize(snrm, edges, right=False)  # mesh.py:2840
+        length = len(edges) + 1
+        self.nmodes = np.bincount(self.ibin.ravel(), minlength=length)[1:-1].astype('i8')
+        xsum = np.bincount(self.ibin.ravel(), weights=snrm.ravel(), minlength=length)[1:-1]
+        with np.errstate(invalid='ignore', divide='ignore'):
+            self.xavg = xsum / self.nmodes  # mesh2.py:62
+        self.edges = np.column_stack([edges[:-1], edges[1:]])
+        self.ells = tuple(sorted(ells if np.ndim(ells) else (ells,)))
+
+    def __call__(self, value, remove_zero=False):
+        """mesh2.py:175-181 with _bincount, mesh.py:2874-2910."""
+        value = np.array(value)
+        if remove_zero:
+            value[0, 0, 0] = 0.
+        n = len(self.xavg) + 2
+        s = np.bincount(self.ibin.ravel(), weights=value.ravel(), minlength=n)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            return s[1:-1] / self.nmodes
+
+
+def correlation_poles(attrs, bin, mesh1, mesh2=None, los='z'):
+    """compute_mesh2_correlation, mesh2.py:377-504 (basis=None).  mesh1/mesh2 are real
+    (N,N,N) or complex half spectra.  Returns dict with s, nmodes, num_raw (per ell), norm."""
+    ells = bin.ells
+    same = mesh2 is None
+    vcell = np.prod(attrs.cellsize)
+    norm = np.prod(attrs.meshsize) / vcell * np.ones_like(bin.xavg)  # mesh2.py:400
+
+    def c(m):
+        return m if np.iscomplexobj(m) else r2c(m)
+
+    def r(m):
+        return c2r(m, attrs.meshsize) if np.iscomplexobj(m) else m
+
+    num = []
+    if isinstance(los, str) and los in ('firstpoint', 'local', 'endpoint'):  # mesh2.py:416-469
+        meshes = [mesh1, mesh1 if same else mesh2]
+        if los == 'endpoint':
+            meshes = meshes[::-1]
+        rmesh1 = meshes[0]
+        A0 = c(rmesh1 if same else meshes[1])
+        if 0 in ells:
+            Aell = (A0.real**2 + A0.imag**2) if same else np.conj(c(rmesh1)) * A0
+            num.append(bin(c2r(Aell, attrs.meshsize)))
+        rmesh1 = r(rmesh1)
+        xs = attrs.rcoords()
+        xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+        ss = separation_coords(attrs)
+        sx, sy, sz = ss[0][:, None, None], ss[1][None, :, None], ss[2][None, None, :]
+        for ell in ells:
+            if ell == 0:
+                continue
+            C = 0.
+            for m in range(-ell, ell + 1):
+                B = r2c(rmesh1 * real_ylm(ell, m, xx, xy, xz))
+                C = C + c2r(np.conj(B) * A0, attrs.meshsize) * real_ylm(ell, m, sx, sy, sz)
+            num.append(4. * np.pi * bin(C, remove_zero=True))
+        los_attr = 'firstpoint' if los == 'local' else los
+    else:  # mesh2.py:471-504
+        vlos = los_vector(los)
+        A1 = c(mesh1)
+        Aell = c2r((A1.real**2 + A1.imag**2) if same else np.conj(A1) * c(mesh2), attrs.meshsize)
+        if 0 in ells:
+            num.append(bin(Aell))
+        ss = separation_coords(attrs)
+        sx, sy, sz = ss[0][:, None, None], ss[1][None, :, None], ss[2][None, None, :]
+        sn = np.sqrt(sx**2 + sy**2 + sz**2)
+        sn[0, 0, 0] = 1.
+        mu = (sx * vlos[0] + sy * vlos[1] + sz * vlos[2]) / sn
+        for ell in ells:
+            if ell == 0:
+                continue
+            num.append((2 * ell + 1) * bin(Aell * legendre(ell, mu), remove_zero=True))
+        los_attr = vlos
+    num = [np.real(n) for n in num]
+    return dict(s=bin.xavg, s_edges=bin.edges, nmodes=bin.nmodes, num_raw=[n / vcell for n in num], norm=norm,
+                volume=vcell * bin.nmodes, ells=ells, los=los_attr)
+
+
+def circular_correlation_bruteforce(mesh1, mesh2=None):
+    """xi(s) = sum_x mesh1(x) mesh2(x + s), periodic, by explicit shifts (the definition the
+    c2r(conj(A1) A2) route must reproduce); for small meshes only."""
+    mesh2 = mesh1 if mesh2 is None else mesh2
+    out = np.empty(mesh1.shape)
+    for idx in np.ndindex(*mesh1.shape):
+        out[idx] = np.sum(mesh1 * np.roll(mesh2, [-i for i in idx], axis=(0, 1, 2)))
+    return out
+
+
+def local_correlation_bruteforce(attrs, rmesh, ell):
+    """sum over cells x and lags s of rho(x) rho(x + s) (2l+1)/(4pi) P_l(xhat . shat), per
+    lag s, with xhat = 0 at the origin (utils.py:252-254): the local line-of-sight term of
+    mesh2.py:445-460 written without Y_lm (addition theorem); small meshes only."""
+    xs = attrs.rcoords()
+    X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
+    xn = np.linalg.norm(X, axis=-1)
+    xhat = np.where(xn[..., None] > 0, X / np.where(xn > 0, xn, 1.)[..., None], 0.)
+    ss = separation_coords(attrs)
+    out = np.zeros(rmesh.shape)
+    for idx in np.ndindex(*rmesh.shape):
+        svec = np.array([ss[a][idx[a]] for a in range(3)])
+        sn = np.linalg.norm(svec)
+        if sn == 0.:
+            continue
+        mu = xhat @ (svec / sn)
+        w = np.where(xn > 0, (2 * ell + 1) / (4 * np.pi) * legendre(ell, mu), 0.)
+        out[idx] = np.sum(rmesh * w * np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2)))
     return out

@@ -364,3 +364,92 @@ def test_read_compensated_plane_wave():
         # at nodes the CIC kernel is exact; TSC spreads 3/4, 1/8, 1/8 -> multiply by its transfer
         transfer = 1. if order == 2 else 0.75 + 0.25 * np.cos(k)
         assert np.allclose(got, expected * transfer, atol=1e-12)
+
+
+# ---------------------------------------------------------------- correlation function
+
+def test_correlation_c2r_is_circular_correlation():
+    """c2r(conj(A1) A2) is sum_x m1(x) m2(x + s), the definition of the estimator's xi
+    (mesh2.py:472-476); pins the c2r normalisation and the conjugation side."""
+    rng = np.random.default_rng(3)
+    shape = (6, 4, 8)
+    m1, m2 = rng.normal(size=shape), rng.normal(size=shape)
+    got = po.c2r(np.conj(po.r2c(m1)) * po.r2c(m2), shape)
+    assert np.allclose(got, po.circular_correlation_bruteforce(m1, m2), rtol=0, atol=1e-12)
+    auto = po.c2r(np.abs(po.r2c(m1))**2, shape)
+    assert np.allclose(auto, po.circular_correlation_bruteforce(m1), rtol=0, atol=1e-12)
+    assert np.isclose(auto[0, 0, 0], np.sum(m1**2))  # zero lag
+
+
+def _lags(attrs):
+    """Lag vectors (i, j, k) -> separation, from the index definition of fftfreq."""
+    out = {}
+    for idx in np.ndindex(*attrs.meshsize):
+        out[idx] = np.array([(i if i < n / 2 else i - n) * c for i, n, c in zip(idx, attrs.meshsize, attrs.cellsize)])
+    return out
+
+
+def test_correlation_nmodes_and_edges_bruteforce():
+    """Separation bins: counts and mean |s| against an explicit loop over lags."""
+    attrs = po.Attrs((6, 8, 4), (60., 80., 40.), 0.)
+    bin = po.BinCorrelationPoles(attrs, edges={'step': 7.}, ells=(0,))
+    edges = np.append(bin.edges[:, 0], bin.edges[-1, 1])
+    assert np.isclose(edges[-1] - edges[-2], 7.) and edges[0] == 0. and edges[-1] < 10. * 4 / 2 + 7.
+    count, ssum = np.zeros(len(bin.nmodes), 'i8'), np.zeros(len(bin.nmodes))
+    for idx, s in _lags(attrs).items():
+        sn = np.linalg.norm(s)
+        b = np.searchsorted(edges, sn, side='right') - 1
+        if 0 <= b < len(count):
+            count[b] += 1
+            ssum[b] += sn
+    assert np.array_equal(count, bin.nmodes)
+    ok = count > 0
+    assert np.allclose(ssum[ok] / count[ok], bin.xavg[ok], rtol=1e-14)
+
+
+@pytest.mark.parametrize('los', ['x', 'z', (0.6, 0., 0.8)])
+def test_correlation_global_los_bruteforce(los):
+    """Global line of sight: (2l+1) <xi(s) L_l(mu_s)> over separation shells, xi from explicit
+    shifts and L_l from scipy, against the oracle (mesh2.py:465-504)."""
+    rng = np.random.default_rng(5)
+    attrs = po.Attrs((6, 6, 6), (60., 60., 60.), (5., -3., 2.))
+    mesh = rng.normal(size=attrs.meshsize)
+    ells = (0, 2, 4)
+    bin = po.BinCorrelationPoles(attrs, edges={'step': 9.}, ells=ells)
+    got = po.correlation_poles(attrs, bin, mesh, los=los)
+    xi = po.circular_correlation_bruteforce(mesh)
+    vlos = po.los_vector(los)
+    vlos = vlos / np.linalg.norm(vlos)
+    edges = np.append(bin.edges[:, 0], bin.edges[-1, 1])
+    vcell = np.prod(attrs.cellsize)
+    for ill, ell in enumerate(ells):
+        acc, cnt = np.zeros(len(bin.nmodes)), np.zeros(len(bin.nmodes))
+        for idx, s in _lags(attrs).items():
+            sn = np.linalg.norm(s)
+            b = np.searchsorted(edges, sn, side='right') - 1
+            if not 0 <= b < len(acc):
+                continue
+            cnt[b] += 1
+            if ell and sn == 0.:
+                continue  # remove_zero for l > 0
+            mu = s @ vlos / sn if sn > 0 else 0.
+            acc[b] += xi[idx] * special.eval_legendre(ell, mu)
+        want = (2 * ell + 1) * acc / cnt / vcell
+        assert np.allclose(got['num_raw'][ill], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    assert np.allclose(got['norm'], np.prod(attrs.meshsize) / vcell)
+
+
+def test_correlation_local_los_bruteforce():
+    """Local (firstpoint) line of sight: the Y_lm route of mesh2.py:445-460 against the
+    explicit double sum over cells and lags with P_l(xhat . shat)."""
+    rng = np.random.default_rng(9)
+    attrs = po.Attrs((6, 6, 6), (60., 60., 60.), (70., -20., 40.))
+    mesh = rng.normal(size=attrs.meshsize)
+    ells = (0, 2, 4)
+    bin = po.BinCorrelationPoles(attrs, edges={'step': 9.}, ells=ells)
+    got = po.correlation_poles(attrs, bin, mesh, los='firstpoint')
+    vcell = np.prod(attrs.cellsize)
+    assert np.allclose(got['num_raw'][0], bin(po.circular_correlation_bruteforce(mesh)) / vcell, rtol=1e-12)
+    for ill, ell in enumerate(ells[1:], start=1):
+        want = 4. * np.pi * bin(po.local_correlation_bruteforce(attrs, mesh, ell), remove_zero=True) / vcell
+        assert np.allclose(got['num_raw'][ill], want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
