@@ -44,7 +44,7 @@ def parse():
                    help='write the HBM traffic file from two rocprofv3 --pmc output folders and exit')
     p.add_argument('--summarize-trace', metavar='TRACE_DIR',
                    help='write the kernel statistics summary from a rocprofv3 --kernel-trace --stats folder and exit')
-    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024', 'batch256'],
+    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024', 'batch256', 'corr512'],
                    help='box512: the headline plain box; fkp1024: survey FKP field, 1024^3, local line of sight; '
                         'batch256: 64 lognormal mocks at 256^3 with uneven counts, batched (HIP graph) vs per-mock calls')
     p.add_argument('--paint-path', type=int, default=None,
@@ -278,6 +278,57 @@ def run_fkp1024(args):
     print(json.dumps(result))
 
 
+def run_corr512(args):
+    """Correlation-function multipoles of the 512^3 TSCx3 lognormal mock (the headline box):
+    paint(out='complex'), DC removed, compute_mesh2_correlation with los 'z', ells 0, 2, 4,
+    separation bins of 4 Mpc/h up to 200 Mpc/h (the binning of the reference's
+    tests/test_mesh2.py:130-132).  One step = paint + correlation.  Prints one JSON line."""
+    import numpy as np
+    import torch
+    from jaxpower_amd import MeshAttrs, BinMesh2CorrelationPoles, compute_mesh2_correlation, generate_lognormal_particles, _lib
+    torch.cuda.set_device(0)
+    attrs = MeshAttrs(meshsize=MESHSIZE, boxsize=BOXSIZE, boxcenter=BOXSIZE / 2.)
+    particles = generate_lognormal_particles(attrs, size=NPART, seed=args.seed)
+    bin = BinMesh2CorrelationPoles(attrs, edges={'step': 4., 'max': 200.}, ells=ELLS)
+    torch.cuda.synchronize()
+    print(f'# corr setup done: {particles.size} particles, {bin.nbins} bins', file=sys.stderr, flush=True)
+
+    def step():
+        mesh = particles.paint(resampler='tsc', interlacing=NI, compensate=True, out='complex').at[0, 0, 0].set(0.)
+        return compute_mesh2_correlation(mesh, bin=bin, los='z')
+
+    for _ in range(args.warmup):
+        xi = step()
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    lib.mp_timing_reset()
+    lib.mp_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        xi = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.mp_timing_enable(0)
+    buf = _lib.ctypes.create_string_buffer(1 << 16)
+    lib.mp_timing_report(buf, len(buf))
+    kernels = {}
+    for line in buf.value.decode().splitlines():
+        name, total, count = line.split()
+        kernels[name] = {'avg_ms': round(float(total) / int(count), 4), 'launches_per_step': int(count) / args.steps,
+                         'share': round(float(total) / (elapsed * 1e3), 4)}
+    x0 = xi.get(0).value()
+    result = {'metric': 'xi_ell(s) mocks/s at 512^3 TSCx3', 'value': round(args.steps / elapsed, 4), 'unit': 'mocks/s',
+              'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(elapsed * 1e3 / args.steps, 3),
+              'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32 (f64 bin sums)',
+              'data': 'synthetic: seeded lognormal Poisson mock, P_G(k)=2e4 (k/0.1)^-1.5 exp(-(k/0.5)^2)',
+              'config': {'workload': 'plain periodic box, correlation multipoles', 'meshsize': MESHSIZE, 'boxsize': BOXSIZE,
+                         'particles': particles.size, 'resampler': 'tsc', 'interlacing': NI, 'compensate': True, 'los': 'z',
+                         'ells': list(ELLS), 'sstep': 4., 'smax': 200., 'nbins': bin.nbins},
+              'kernels': kernels,
+              'check': {'xi0_finite': bool(np.all(np.isfinite(x0[1:]))), 'xi0_bins_2_4': [float(v) for v in x0[2:5]]}}
+    print(json.dumps(result))
+
+
 def run_batch256(args):
     """64 lognormal mocks (the 512^3 recipe at 256^3, boxsize 1000) with particle counts drawn
     uniformly in [5e5, 4e6], CIC and TSC, 2-fold interlacing, compensated, DC removed, ells
@@ -333,6 +384,12 @@ def main():
             print('bench.py runs on one GPU only.', file=sys.stderr)
             sys.exit(2)
         run_batch256(args)
+        return
+    if args.workload == 'corr512':
+        if args.gpus != 1:
+            print('bench.py runs on one GPU only.', file=sys.stderr)
+            sys.exit(2)
+        run_corr512(args)
         return
     if args.workload == 'fkp1024':
         if args.gpus != 1:

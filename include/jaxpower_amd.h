@@ -168,6 +168,14 @@ int64_t mp_bin_real_workspace_bytes(const int32_t meshsize[3], int nell, int ned
 int mp_bin_real_poles(void* stream, const float* mesh, const int32_t meshsize[3], const double* sx, const double* sy,
                       const double* sz, const double* s2_thresholds, int nedges, const double los[3], int nell,
                       const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace, size_t workspace_bytes);
+/* mp_irfft3d then mp_bin_real_poles in one go (jaxpower/mesh2.py:477-498: c2r of the product
+ * spectrum, then the multipole binning), the last inverse pass fused with the binning so the
+ * real mesh is never stored: out[e * nbins + b] = sum over bin b of irfftn(in) P_ell[e](mu_s). */
+int64_t mp_irfft3d_bin_real_workspace_bytes(const int32_t meshsize[3], int nell, int nedges);
+int mp_irfft3d_bin_real(void* stream, const void* in, int64_t in_pitch, const int32_t meshsize[3], const double* sx,
+                        const double* sy, const double* sz, const double* s2_thresholds, int nedges, const double los[3],
+                        int nell, const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace,
+                        size_t workspace_bytes);
 /* The spectrum products of compute_mesh2_correlation (jaxpower/mesh2.py:429-431, 447, 476-478):
  * out = conj(a) * b on half spectra, or |a|^2 (imaginary part 0) when b is NULL; out may be a. */
 int mp_spectrum_product(void* stream, const void* a, int64_t a_pitch, const void* b, int64_t b_pitch,

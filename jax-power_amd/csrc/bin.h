@@ -47,6 +47,23 @@ __device__ __forceinline__ int walk_bin(const double* thr, int nedges, int b, do
   return b;
 }
 
+// Separation-shell binning of a real (configuration-space) mesh: geometry and the multipoles
+// asked for (sum of value * P_ell(mu_s) per bin, the (0,0,0) cell skipped where rz is set).
+constexpr int CB_MAX_ELL = 6;
+constexpr int CB_MAX_BLOCKS = 1024;
+struct RealBinArgs {
+  const float* mesh;  // (n0, n1, n2), z fastest; unused by the fused z pass
+  int n0, n1, n2, nedges;
+  const double* sx;
+  const double* sy;
+  const double* sz;
+  const double* thr;  // thr[e] = min{t : fl(sqrt(t)) >= edge e}
+  int nell;
+  int ell[CB_MAX_ELL], rz[CB_MAX_ELL];
+  float los[3];
+  int need_mu;
+};
+
 // out[e] = sum over blocks of partial[block][e], in a fixed order.
 int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out);
 

@@ -17,6 +17,7 @@
 // LDS histogram that each block writes once, and a second kernel sums the block partials in a
 // fixed order.
 #include "bin.h"
+#include "fft.h"
 #include "sph.h"
 
 namespace mp {
@@ -25,21 +26,6 @@ int fill_i32_dev(hipStream_t s, int* p, int64_t n, int v);
 
 constexpr int CB_SEG = 32;
 constexpr int CB_NT = 256;
-constexpr int CB_MAX_ELL = 6;
-constexpr int CB_MAX_BLOCKS = 1024;
-
-struct RealBinArgs {
-  const float* mesh;  // (n0, n1, n2), z fastest
-  int n0, n1, n2, nedges;
-  const double* sx;
-  const double* sy;
-  const double* sz;
-  const double* thr;
-  int nell;
-  int ell[CB_MAX_ELL], rz[CB_MAX_ELL];
-  float los[3];
-  int need_mu;
-};
 
 static int real_bin_blocks(const RealBinArgs& a) {
   const int64_t nseg = ceil_div(a.n2, CB_SEG);
@@ -91,7 +77,11 @@ bin_real_kernel(RealBinArgs a, double* __restrict__ partial) {
   extern __shared__ double hist[];
   const int nbins = a.nedges - 1;
   const int hsize = NE * nbins;
+  double* thr = hist + hsize;  // thresholds and z separations, read per cell
+  double* szt = thr + a.nedges;
   for (int i = threadIdx.x; i < hsize; i += CB_NT) hist[i] = 0.;
+  for (int i = threadIdx.x; i < a.nedges; i += CB_NT) thr[i] = a.thr[i];
+  for (int i = threadIdx.x; i < a.n2; i += CB_NT) szt[i] = a.sz[i];
   __syncthreads();
   const int nseg = (a.n2 + CB_SEG - 1) / CB_SEG;
   const int64_t nwork = (int64_t)a.n0 * a.n1 * nseg;
@@ -102,7 +92,7 @@ bin_real_kernel(RealBinArgs a, double* __restrict__ partial) {
     const int z0 = seg * CB_SEG, z1 = min(z0 + CB_SEG, a.n2);
     const double sxv = a.sx[ix], syv = a.sy[iy];
     const double a2 = k2_xy(sxv, syv);
-    int b = first_bin(a.thr, a.nedges, k2_sum(a2, a.sz[z0]));
+    int b = first_bin(thr, a.nedges, k2_sum(a2, szt[z0]));
     double acc[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) acc[e] = 0.;
@@ -111,8 +101,8 @@ bin_real_kernel(RealBinArgs a, double* __restrict__ partial) {
     const float fa2 = fsx * fsx + fsy * fsy;
     const float* src = a.mesh + row * a.n2;
     for (int iz = z0; iz < z1; ++iz) {
-      const double szv = a.sz[iz];
-      const int nb = walk_bin(a.thr, a.nedges, b, k2_sum(a2, szv));
+      const double szv = szt[iz];
+      const int nb = walk_bin(thr, a.nedges, b, k2_sum(a2, szv));
       if (nb != b) {
         if (b >= 1 && b < a.nedges) {
 #pragma unroll
@@ -198,6 +188,23 @@ static int real_args(RealBinArgs& a, const float* mesh, const int32_t n[3], cons
   return MP_OK;
 }
 
+static int real_ells(RealBinArgs& a, const double los[3], int nell, const int32_t* ells, const int32_t* remove_zero) {
+  MP_REQUIRE(ells && remove_zero, "correlation binning: null ells");
+  MP_REQUIRE(nell >= 1 && nell <= CB_MAX_ELL, "correlation binning: %d ells not in 1..%d", nell, CB_MAX_ELL);
+  MP_REQUIRE(((size_t)nell * (a.nedges - 1) + a.nedges + a.n2) * sizeof(double) <= 96 * 1024,
+             "correlation binning: too many bins for the LDS histogram");
+  a.nell = nell;
+  a.need_mu = 0;
+  for (int e = 0; e < nell; ++e) {
+    MP_REQUIRE(ells[e] >= 0 && ells[e] <= 10, "correlation binning: ell %d not in 0..10", ells[e]);
+    a.ell[e] = ells[e];
+    a.rz[e] = remove_zero[e];
+    if (ells[e]) a.need_mu = 1;
+  }
+  for (int d = 0; d < 3; ++d) a.los[d] = los ? (float)los[d] : 0.f;
+  return MP_OK;
+}
+
 template <int NE>
 static int launch_real_bin(hipStream_t s, const RealBinArgs& a, double* out, void* ws, size_t ws_bytes) {
   const int hsize = NE * (a.nedges - 1);
@@ -205,7 +212,9 @@ static int launch_real_bin(hipStream_t s, const RealBinArgs& a, double* out, voi
   Workspace w(ws, ws_bytes);
   double* partial = w.take<double>((size_t)blocks * hsize);
   if (!partial) return set_error(MP_ERR_WORKSPACE, "correlation binning: workspace too small");
-  MP_LAUNCH("bin_real", bin_real_kernel<NE>, dim3(blocks), dim3(CB_NT), hsize * sizeof(double), s, a, partial);
+  const size_t lds = ((size_t)hsize + a.nedges + a.n2) * sizeof(double);
+  if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)bin_real_kernel<NE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  MP_LAUNCH("bin_real", bin_real_kernel<NE>, dim3(blocks), dim3(CB_NT), lds, s, a, partial);
   return reduce_partials(s, partial, blocks, hsize, out);
 }
 
@@ -243,18 +252,9 @@ MP_EXPORT int mp_bin_real_poles(void* stream, const float* mesh, const int32_t m
   RealBinArgs a{};
   int rc = real_args(a, mesh, meshsize, sx, sy, sz, s2_thresholds, nedges);
   if (rc) return rc;
-  MP_REQUIRE(mesh && out && ells && remove_zero, "mp_bin_real_poles: null pointer");
-  MP_REQUIRE(nell >= 1 && nell <= CB_MAX_ELL, "mp_bin_real_poles: %d ells not in 1..%d", nell, CB_MAX_ELL);
-  MP_REQUIRE((size_t)nell * (nedges - 1) * sizeof(double) <= 64 * 1024, "mp_bin_real_poles: too many bins for the LDS histogram");
-  a.nell = nell;
-  a.need_mu = 0;
-  for (int e = 0; e < nell; ++e) {
-    MP_REQUIRE(ells[e] >= 0 && ells[e] <= 10, "mp_bin_real_poles: ell %d not in 0..10", ells[e]);
-    a.ell[e] = ells[e];
-    a.rz[e] = remove_zero[e];
-    if (ells[e]) a.need_mu = 1;
-  }
-  for (int d = 0; d < 3; ++d) a.los[d] = los ? (float)los[d] : 0.f;
+  MP_REQUIRE(mesh && out, "mp_bin_real_poles: null pointer");
+  rc = real_ells(a, los, nell, ells, remove_zero);
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   switch (nell) {
     case 1: return launch_real_bin<1>(s, a, out, workspace, workspace_bytes);
@@ -264,6 +264,23 @@ MP_EXPORT int mp_bin_real_poles(void* stream, const float* mesh, const int32_t m
     case 5: return launch_real_bin<5>(s, a, out, workspace, workspace_bytes);
     default: return launch_real_bin<6>(s, a, out, workspace, workspace_bytes);
   }
+}
+
+MP_EXPORT int64_t mp_irfft3d_bin_real_workspace_bytes(const int32_t meshsize[3], int nell, int nedges) {
+  return (int64_t)irfft3d_bin_real_workspace_bytes(meshsize, nell, nedges);
+}
+
+MP_EXPORT int mp_irfft3d_bin_real(void* stream, const void* in, int64_t in_pitch, const int32_t meshsize[3], const double* sx,
+                                  const double* sy, const double* sz, const double* s2_thresholds, int nedges, const double los[3],
+                                  int nell, const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace,
+                                  size_t workspace_bytes) {
+  RealBinArgs a{};
+  int rc = real_args(a, nullptr, meshsize, sx, sy, sz, s2_thresholds, nedges);
+  if (rc) return rc;
+  rc = real_ells(a, los, nell, ells, remove_zero);
+  if (rc) return rc;
+  MP_REQUIRE(in && out, "mp_irfft3d_bin_real: null pointer");
+  return irfft3d_bin_real((hipStream_t)stream, (const float2*)in, in_pitch, meshsize, a, out, workspace, workspace_bytes);
 }
 
 MP_EXPORT int mp_spectrum_product(void* stream, const void* a, int64_t a_pitch, const void* b, int64_t b_pitch,

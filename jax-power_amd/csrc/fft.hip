@@ -14,9 +14,11 @@
 //      contiguous axis fastest (coalesced 128-byte row segments, conflict-free LDS access),
 //      transformed by a radix-8/4/2 Stockham FFT, and written back.
 //   The x pass of the interlaced paint (combine of the n_i shifted meshes) is in xpass.hip.
+#include "bin.h"
 #include "common.h"
 #include "fft.h"
 #include "fft_core.h"
+#include "sph.h"
 
 #include <cmath>
 #include <mutex>
@@ -200,6 +202,117 @@ zpass_c2r_kernel(const float2* __restrict__ in, int64_t in_pitch, float* __restr
     int64_t row = row0 + c;
     if (row < nrows) out2[row * M + l] = cscale(lds[c * LP + l], scale);
   }
+}
+
+// The z c2r pass fused with the separation-shell binning of the real rows it produces (the
+// correlation function of a product spectrum, jaxpower/mesh2.py:471-498): the rows never
+// leave LDS.  Each thread then sweeps RUN consecutive complex elements (2 RUN reals) of one
+// row, keeping FP64 running sums per multipole until the bin changes (along a row |s| rises
+// to the middle and falls after it, so the bin moves by small steps either way), then adds
+// them into an LDS histogram.  Persistent blocks; each writes its histogram once.
+template <int M, int NE>
+__global__ void __launch_bounds__(RowCfg<M>::NT)
+zpass_c2r_bin_kernel(const float2* __restrict__ in, int64_t in_pitch, int64_t nrows, float scale, RealBinArgs a,
+                     double* __restrict__ partial) {
+  using Cfg = RowCfg<M>;
+  constexpr int C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
+  constexpr int TPR = NT / C;
+  constexpr int RUN = (M + TPR - 1) / TPR;
+  extern __shared__ float2 lds[];
+  double* hist = reinterpret_cast<double*>(lds + C * LP);
+  double* thr = hist + NE * (a.nedges - 1);  // thresholds and z separations, read per element
+  double* szt = thr + a.nedges;
+  __shared__ float2 stw[2 * M];
+  const int tid = threadIdx.x;
+  const int nbins = a.nedges - 1, hsize = NE * nbins;
+  load_twiddles<2 * M, NT>(stw, g_twiddle);
+  for (int i = tid; i < hsize; i += NT) hist[i] = 0.;
+  for (int i = tid; i < a.nedges; i += NT) thr[i] = a.thr[i];
+  for (int i = tid; i < 2 * M; i += NT) szt[i] = a.sz[i];
+  __syncthreads();
+  constexpr int NK = M / 2 + 1;
+  const int64_t ngroups = (nrows + C - 1) / C;
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t row0 = grp * C;
+    for (int idx = tid; idx < C * NK; idx += NT) {
+      int c = idx / NK, k = idx % NK;
+      int64_t row = row0 + c;
+      float2 zk = make_float2(0.f, 0.f), zmk = make_float2(0.f, 0.f);
+      if (row < nrows) {
+        const float2* x = in + row * in_pitch;
+        if (k == 0) {
+          float x0 = x[0].x, xm = x[M].x;
+          zk = make_float2(0.5f * (x0 + xm), 0.5f * (x0 - xm));
+        } else {
+          float2 xk = x[k], xmk = cconj(x[M - k]);
+          float2 E = cscale(cadd(xk, xmk), 0.5f);
+          float2 O = cmul(cscale(csub(xk, xmk), 0.5f), cconj(stw[k]));
+          zk = make_float2(E.x - O.y, E.y + O.x);
+          zmk = make_float2(E.x + O.y, -E.y + O.x);
+        }
+      }
+      lds[c * LP + k] = zk;
+      if (k != 0 && 2 * k != M) lds[c * LP + (M - k)] = zmk;
+    }
+    __syncthreads();
+    fft_lds<M, C, 1, LP, false, NT, true, 2 * M>(lds, stw);
+    const int c = tid / TPR, l0 = (tid % TPR) * RUN;
+    const int64_t row = row0 + c;
+    if (c < C && row < nrows && l0 < M) {
+      const int ix = (int)(row / a.n1), iy = (int)(row % a.n1);
+      const double sxv = a.sx[ix], syv = a.sy[iy];
+      const double a2 = k2_xy(sxv, syv);
+      int b = first_bin(thr, a.nedges, k2_sum(a2, szt[2 * l0]));
+      double acc[NE];
+#pragma unroll
+      for (int e = 0; e < NE; ++e) acc[e] = 0.;
+      const float fsx = (float)sxv, fsy = (float)syv;
+      const float lxy = fsx * a.los[0] + fsy * a.los[1];
+      const float fa2 = fsx * fsx + fsy * fsy;
+      const int l1 = min(l0 + RUN, M);
+      for (int l = l0; l < l1; ++l) {
+        const float2 v2 = lds[c * LP + l];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int iz = 2 * l + h;
+          const double szv = szt[iz];
+          const int nb = walk_bin(thr, a.nedges, b, k2_sum(a2, szv));
+          if (nb != b) {
+            if (b >= 1 && b < a.nedges) {
+#pragma unroll
+              for (int e = 0; e < NE; ++e) {
+                if (acc[e] != 0.) atomicAdd(&hist[e * nbins + b - 1], acc[e]);
+                acc[e] = 0.;
+              }
+            }
+            b = nb;
+          }
+          if (b < 1 || b >= a.nedges) continue;
+          const float v = (h ? v2.y : v2.x) * scale;
+          float mu = 0.f;
+          if (a.need_mu) {
+            const float fsz = (float)szv;
+            const float s2 = fa2 + fsz * fsz;
+            mu = s2 > 0.f ? (lxy + fsz * a.los[2]) * rsqrtf(s2) : 0.f;
+          }
+          const bool origin = (ix == 0) & (iy == 0) & (iz == 0);
+#pragma unroll
+          for (int e = 0; e < NE; ++e) {
+            if (origin && a.rz[e]) continue;
+            acc[e] += (double)(v * (a.ell[e] ? legendre(a.ell[e], mu) : 1.f));
+          }
+        }
+      }
+      if (b >= 1 && b < a.nedges) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+          if (acc[e] != 0.) atomicAdd(&hist[e * nbins + b - 1], acc[e]);
+      }
+    }
+    __syncthreads();
+  }
+  double* dst = partial + (int64_t)blockIdx.x * hsize;
+  for (int i = tid; i < hsize; i += NT) dst[i] = hist[i];
 }
 
 // ------------------------------------------------------------------ column passes
@@ -548,6 +661,74 @@ int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const
   // the z pass returns M * (x_even + i x_odd) for M = Nz/2 (unnormalised M-point inverse)
   const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)M));
   return zpass_c2r(M, scratch, scratch_pitch, out, nx * ny, scale, s);
+}
+
+template <int M, int NE>
+static int launch_c2r_bin_ne(const float2* in, int64_t pitch, int64_t nrows, float scale, const RealBinArgs& a, double* partial,
+                             int blocks, hipStream_t s) {
+  using Cfg = RowCfg<M>;
+  const size_t lds = Cfg::LDS + ((size_t)NE * (a.nedges - 1) + a.nedges + 2 * M) * sizeof(double);
+  auto kern = zpass_c2r_bin_kernel<M, NE>;
+  if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  MP_LAUNCH("fft_zpass_c2r_bin", kern, dim3((unsigned)blocks), dim3(Cfg::NT), lds, s, in, pitch, nrows, scale, a, partial);
+  return MP_OK;
+}
+
+static int c2r_bin_blocks(int M, int64_t nrows) {
+  const int C = M <= 256 ? 16 : 4096 / M;
+  const int64_t g = ceil_div(nrows, C);
+  return (int)(g < CB_MAX_BLOCKS ? (g < 1 ? 1 : g) : CB_MAX_BLOCKS);
+}
+
+template <int M>
+static int launch_c2r_bin(const float2* in, int64_t pitch, int64_t nrows, float scale, const RealBinArgs& a, double* partial,
+                          int blocks, hipStream_t s) {
+  switch (a.nell) {
+    case 1: return launch_c2r_bin_ne<M, 1>(in, pitch, nrows, scale, a, partial, blocks, s);
+    case 2: return launch_c2r_bin_ne<M, 2>(in, pitch, nrows, scale, a, partial, blocks, s);
+    case 3: return launch_c2r_bin_ne<M, 3>(in, pitch, nrows, scale, a, partial, blocks, s);
+    case 4: return launch_c2r_bin_ne<M, 4>(in, pitch, nrows, scale, a, partial, blocks, s);
+    case 5: return launch_c2r_bin_ne<M, 5>(in, pitch, nrows, scale, a, partial, blocks, s);
+    default: return launch_c2r_bin_ne<M, 6>(in, pitch, nrows, scale, a, partial, blocks, s);
+  }
+}
+
+size_t irfft3d_bin_real_workspace_bytes(const int32_t n[3], int nell, int nedges) {
+  const size_t H = (size_t)n[0] * n[1] * spectrum_pitch(n[2]) * sizeof(float2);
+  const int blocks = c2r_bin_blocks(n[2] / 2, (int64_t)n[0] * n[1]);
+  return align_up(H, 256) + align_up((size_t)blocks * (nell < 1 ? 1 : nell) * (nedges > 1 ? nedges - 1 : 1) * sizeof(double), 256) +
+         512;
+}
+
+// c2r (x and y inverse passes, then the z c2r pass fused with the separation binning of
+// the real mesh): out[e * nbins + b] = sum over cells of bin b of irfftn(in) P_ell[e](mu_s).
+int irfft3d_bin_real(hipStream_t s, const float2* in, int64_t in_pitch, const int32_t n[3], const RealBinArgs& a, double* out,
+                     void* ws, size_t ws_bytes) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2, ncols = M + 1;
+  if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  MP_REQUIRE(in_pitch >= ncols, "irfft3d_bin_real: in_pitch too small");
+  const int64_t nx = n[0], ny = n[1];
+  const int64_t sp = spectrum_pitch(n[2]);
+  const int blocks = c2r_bin_blocks(M, nx * ny);
+  const int hsize = a.nell * (a.nedges - 1);
+  Workspace w(ws, ws_bytes);
+  float2* scratch = w.take<float2>((size_t)nx * ny * sp);
+  double* partial = w.take<double>((size_t)blocks * hsize);
+  if (!scratch || !partial)
+    return set_error(MP_ERR_WORKSPACE, "irfft3d_bin_real: workspace of %zu bytes < %zu", ws_bytes,
+                     irfft3d_bin_real_workspace_bytes(n, a.nell, a.nedges));
+  rc = col_fft(n[0], true, "fft_xpass_inv", in, scratch, ncols, ny, ny * in_pitch, in_pitch, ny * sp, sp, 1.f, s);
+  if (rc) return rc;
+  rc = col_fft(n[1], true, "fft_ypass_inv", scratch, scratch, ncols, nx, sp, ny * sp, sp, ny * sp, 1.f, s);
+  if (rc) return rc;
+  const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)M));
+  rc = [&]() -> int { MP_POW2_SWITCH(M, launch_c2r_bin, scratch, sp, nx * ny, scale, a, partial, blocks, s); }();
+  if (rc) return rc;
+  return reduce_partials(s, partial, blocks, hsize, out);
 }
 
 int64_t spectrum_pitch(int nz) { return (int64_t)align_up((size_t)(nz / 2 + 1), 16); }

@@ -13,12 +13,22 @@ namespace mp {
 
 constexpr int MAX_ELL = 4;
 
-// P_l(mu) by the three-term recurrence (closed forms of utils.py:341-360 up to rounding).
+// P_l(mu): the closed forms of utils.py:341-360 for l <= 4 (the multipoles measured in
+// practice), the three-term recurrence with tabulated 1/(n+1) above.
 __host__ __device__ __forceinline__ float legendre(int ell, float mu) {
-  if (ell == 0) return 1.f;
+  const float m2 = mu * mu;
+  switch (ell) {
+    case 0: return 1.f;
+    case 1: return mu;
+    case 2: return 1.5f * m2 - 0.5f;
+    case 3: return mu * (2.5f * m2 - 1.5f);
+    case 4: return (4.375f * m2 - 3.75f) * m2 + 0.375f;
+    default: break;
+  }
+  constexpr float inv[11] = {1.f, 0.5f, 1.f / 3.f, 0.25f, 0.2f, 1.f / 6.f, 1.f / 7.f, 0.125f, 1.f / 9.f, 0.1f, 1.f / 11.f};
   float p0 = 1.f, p1 = mu;
   for (int n = 1; n < ell; ++n) {
-    float p2 = ((2 * n + 1) * mu * p1 - n * p0) / (n + 1);
+    float p2 = ((2 * n + 1) * mu * p1 - n * p0) * inv[n < 10 ? n : 10];
     p0 = p1;
     p1 = p2;
   }

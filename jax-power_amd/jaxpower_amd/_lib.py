@@ -63,6 +63,9 @@ SIGNATURES = {
     'mp_bin_real_workspace_bytes': (c_int64, [P_I32, c_int, c_int]),
     'mp_bin_real_poles': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_int, P_I32,
                                   P_I32, c_void_p, c_void_p, c_size_t]),
+    'mp_irfft3d_bin_real_workspace_bytes': (c_int64, [P_I32, c_int, c_int]),
+    'mp_irfft3d_bin_real': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_int,
+                                    P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_spectrum_product': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, P_I32, c_void_p, c_int64]),
     'mp_mul_ylm': (c_int, [c_void_p, c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int]),
     'mp_ylm_workspace_bytes': (c_int64, [P_I32]),

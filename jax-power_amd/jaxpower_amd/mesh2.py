@@ -388,6 +388,23 @@ class BinMesh2CorrelationPoles:
                   _lib.ptr(out), _lib.ptr(ws), ws.numel())
         return out
 
+    def _spectrum_sums_dev(self, spectrum, ells, remove_zero, los=(0., 0., 0.)):
+        """_sums_dev() of irfftn(spectrum) for a half spectrum, with the last pass of the
+        inverse FFT fused with the binning (the real mesh is never stored)."""
+        value, pitch = _as_spectrum(spectrum)
+        device = value.device
+        nell = len(ells)
+        n = _lib.i32x3(self.mattrs.meshsize)
+        nedges = len(self._edges1d)
+        out = torch.empty((nell, self.nbins), dtype=torch.float64, device=device)
+        ws = _lib.workspace(_lib.load().mp_irfft3d_bin_real_workspace_bytes(n, nell, nedges), device)
+        col = lambda values: (_lib.ctypes.c_int32 * nell)(*[int(v) for v in values])
+        t = self._tables
+        _lib.call('mp_irfft3d_bin_real', _lib.stream_handle(device), _lib.ptr(value), pitch, n, _lib.ptr(t['sx']), _lib.ptr(t['sy']),
+                  _lib.ptr(t['sz']), _lib.ptr(t['thr']), nedges, _lib.f64x3(los), nell, col(ells), col(remove_zero),
+                  _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        return out
+
     def __call__(self, mesh, ell=0, remove_zero=False):
         """Sum_{cells in bin} value / nmodes (mesh2.py:175-184); mesh is real."""
         value = mesh.value if isinstance(mesh, BaseMeshField) else _as_device_tensor(mesh, device=self._device)
@@ -455,7 +472,7 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
         num = {}
         if 0 in ells:
             P = _product_spectrum(A0) if same else _product_spectrum(_2c(rmesh1), A0)
-            num[0] = bin._sums_dev(_c2r(P, mattrs.meshsize), [0], [False])[0]
+            num[0] = bin._spectrum_sums_dev(P, [0], [False])[0]
             del P
         if nonzeroells:
             rreal = _2r(rmesh1).value.contiguous()
@@ -484,9 +501,8 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
     else:  # fixed line of sight, mesh2.py:471-504
         A1 = _2c(meshes[0])
         P = _product_spectrum(A1) if same else _product_spectrum(A1, _2c(meshes[1]))
-        xi = _c2r(P, mattrs.meshsize)
+        sums = bin._spectrum_sums_dev(P, ells, [ell != 0 for ell in ells], los=vlos)
         del P
-        sums = bin._sums_dev(xi, ells, [ell != 0 for ell in ells], los=vlos)
         num = [sums[i] * ((2 * ell + 1) / (bin._nmodes_dev * vcell)) for i, ell in enumerate(ells)]
     num = [n.contiguous() for n in num]
     poles = [Mesh2CorrelationPole(s=bin.xavg, s_edges=bin.edges, nmodes=bin.nmodes, num_raw=num[i], num_shotnoise=0., norm=norm,

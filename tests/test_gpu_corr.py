@@ -144,3 +144,23 @@ def test_correlation_bin_call_plain_mean():
         got = bin(to_dev(value), remove_zero=rz)
         want = obin(value.astype('f8'), remove_zero=rz)
         assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize('meshsize', [(32, 32, 64), (16, 8, 2), (8, 16, 1024)])
+def test_fused_c2r_binning_matches_stored_real_mesh(meshsize):
+    """The z c2r pass fused with the separation binning equals c2r to a stored real mesh
+    followed by the binning kernel."""
+    from jaxpower_amd import MeshAttrs, BinMesh2CorrelationPoles
+    from jaxpower_amd.mesh import _c2r, _r2c
+    say('corr fused c2r', meshsize)
+    attrs = MeshAttrs(meshsize=meshsize, boxsize=np.array(meshsize) * 10., boxcenter=0.)
+    rng = np.random.default_rng(13)
+    x = rng.normal(size=meshsize).astype('f4')
+    P = _r2c(to_dev(x), attrs.meshsize)
+    ells, rz = (0, 2, 4, 1), (False, True, True, False)
+    bin = BinMesh2CorrelationPoles(attrs, edges={'step': 13., 'max': 100.}, ells=ells)
+    los = (0.3, -0.5, np.sqrt(1 - 0.34))
+    fused = host(bin._spectrum_sums_dev(P, ells, rz, los=los))
+    stored = host(bin._sums_dev(_c2r(P, attrs.meshsize), ells, rz, los=los))
+    scale = np.abs(stored).max()
+    assert np.abs(fused - stored).max() <= 1e-6 * scale
