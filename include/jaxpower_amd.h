@@ -170,9 +170,10 @@ int mp_bin_real_poles(void* stream, const float* mesh, const int32_t meshsize[3]
                       const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace, size_t workspace_bytes);
 /* mp_irfft3d then mp_bin_real_poles in one go (jaxpower/mesh2.py:477-498: c2r of the product
  * spectrum, then the multipole binning), the last inverse pass fused with the binning so the
- * real mesh is never stored: out[e * nbins + b] = sum over bin b of irfftn(in) P_ell[e](mu_s). */
+ * real mesh is never stored: out[e * nbins + b] = sum over bin b of irfftn(in) P_ell[e](mu_s).
+ * x_done != 0: `in` already carries its (unnormalised) inverse FFT along x. */
 int64_t mp_irfft3d_bin_real_workspace_bytes(const int32_t meshsize[3], int nell, int nedges);
-int mp_irfft3d_bin_real(void* stream, const void* in, int64_t in_pitch, const int32_t meshsize[3], const double* sx,
+int mp_irfft3d_bin_real(void* stream, const void* in, int64_t in_pitch, int x_done, const int32_t meshsize[3], const double* sx,
                         const double* sy, const double* sz, const double* s2_thresholds, int nedges, const double los[3],
                         int nell, const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace,
                         size_t workspace_bytes);
@@ -186,10 +187,11 @@ int mp_spectrum_product(void* stream, const void* a, int64_t a_pitch, const void
 int mp_mul_ylm(void* stream, const float* in, float* out, const int32_t meshsize[3], const double* cx, const double* cy,
                const double* cz, int ell, int m, int accumulate);
 /* mp_combine_partial writing |A|^2 (imaginary part 0) instead of A: the auto-correlation
- * input Aell = |A|^2 of jaxpower/mesh2.py:476, without storing A. */
+ * input Aell = |A|^2 of jaxpower/mesh2.py:476, without storing A; with inverse_x != 0 the
+ * (unnormalised) inverse FFT along x of |A|^2 is written instead (the first pass of its c2r). */
 int mp_combine_partial_power(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
-                             int interlacing, int compensate, int zero_dc, void* out, int64_t out_pitch, void* workspace,
-                             size_t workspace_bytes);
+                             int interlacing, int compensate, int zero_dc, int inverse_x, void* out, int64_t out_pitch,
+                             void* workspace, size_t workspace_bytes);
 
 /* ---- local line of sight ---------------------------------------------------------- */
 /* One term of the firstpoint estimator (jaxpower/mesh2.py:319-329):

@@ -270,17 +270,17 @@ MP_EXPORT int64_t mp_irfft3d_bin_real_workspace_bytes(const int32_t meshsize[3],
   return (int64_t)irfft3d_bin_real_workspace_bytes(meshsize, nell, nedges);
 }
 
-MP_EXPORT int mp_irfft3d_bin_real(void* stream, const void* in, int64_t in_pitch, const int32_t meshsize[3], const double* sx,
-                                  const double* sy, const double* sz, const double* s2_thresholds, int nedges, const double los[3],
-                                  int nell, const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace,
-                                  size_t workspace_bytes) {
+MP_EXPORT int mp_irfft3d_bin_real(void* stream, const void* in, int64_t in_pitch, int x_done, const int32_t meshsize[3],
+                                  const double* sx, const double* sy, const double* sz, const double* s2_thresholds, int nedges,
+                                  const double los[3], int nell, const int32_t* ells, const int32_t* remove_zero, double* out,
+                                  void* workspace, size_t workspace_bytes) {
   RealBinArgs a{};
   int rc = real_args(a, nullptr, meshsize, sx, sy, sz, s2_thresholds, nedges);
   if (rc) return rc;
   rc = real_ells(a, los, nell, ells, remove_zero);
   if (rc) return rc;
   MP_REQUIRE(in && out, "mp_irfft3d_bin_real: null pointer");
-  return irfft3d_bin_real((hipStream_t)stream, (const float2*)in, in_pitch, meshsize, a, out, workspace, workspace_bytes);
+  return irfft3d_bin_real((hipStream_t)stream, (const float2*)in, in_pitch, meshsize, a, out, workspace, workspace_bytes, x_done);
 }
 
 MP_EXPORT int mp_spectrum_product(void* stream, const void* a, int64_t a_pitch, const void* b, int64_t b_pitch,

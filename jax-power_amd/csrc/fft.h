@@ -86,6 +86,6 @@ int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const
 struct RealBinArgs;
 size_t irfft3d_bin_real_workspace_bytes(const int32_t n[3], int nell, int nedges);
 int irfft3d_bin_real(hipStream_t s, const float2* in, int64_t in_pitch, const int32_t n[3], const RealBinArgs& a, double* out,
-                     void* ws, size_t ws_bytes);
+                     void* ws, size_t ws_bytes, int x_done = 0);
 
 }  // namespace mp

@@ -703,7 +703,7 @@ size_t irfft3d_bin_real_workspace_bytes(const int32_t n[3], int nell, int nedges
 // c2r (x and y inverse passes, then the z c2r pass fused with the separation binning of
 // the real mesh): out[e * nbins + b] = sum over cells of bin b of irfftn(in) P_ell[e](mu_s).
 int irfft3d_bin_real(hipStream_t s, const float2* in, int64_t in_pitch, const int32_t n[3], const RealBinArgs& a, double* out,
-                     void* ws, size_t ws_bytes) {
+                     void* ws, size_t ws_bytes, int x_done) {
   int rc = check_shape(n);
   if (rc) return rc;
   rc = ensure_twiddles();
@@ -721,9 +721,13 @@ int irfft3d_bin_real(hipStream_t s, const float2* in, int64_t in_pitch, const in
   if (!scratch || !partial)
     return set_error(MP_ERR_WORKSPACE, "irfft3d_bin_real: workspace of %zu bytes < %zu", ws_bytes,
                      irfft3d_bin_real_workspace_bytes(n, a.nell, a.nedges));
-  rc = col_fft(n[0], true, "fft_xpass_inv", in, scratch, ncols, ny, ny * in_pitch, in_pitch, ny * sp, sp, 1.f, s);
-  if (rc) return rc;
-  rc = col_fft(n[1], true, "fft_ypass_inv", scratch, scratch, ncols, nx, sp, ny * sp, sp, ny * sp, 1.f, s);
+  if (x_done) {  // the input already carries its inverse FFT along x
+    rc = col_fft(n[1], true, "fft_ypass_inv", in, scratch, ncols, nx, in_pitch, ny * in_pitch, sp, ny * sp, 1.f, s);
+  } else {
+    rc = col_fft(n[0], true, "fft_xpass_inv", in, scratch, ncols, ny, ny * in_pitch, in_pitch, ny * sp, sp, 1.f, s);
+    if (rc) return rc;
+    rc = col_fft(n[1], true, "fft_ypass_inv", scratch, scratch, ncols, nx, sp, ny * sp, sp, ny * sp, 1.f, s);
+  }
   if (rc) return rc;
   const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)M));
   rc = [&]() -> int { MP_POW2_SWITCH(M, launch_c2r_bin, scratch, sp, nx * ny, scale, a, partial, blocks, s); }();

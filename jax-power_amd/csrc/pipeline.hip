@@ -190,12 +190,12 @@ MP_EXPORT int mp_combine_partial(void* stream, const void* partial, int64_t pitc
 }
 
 MP_EXPORT int mp_combine_partial_power(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
-                                       int interlacing, int compensate, int zero_dc, void* out, int64_t out_pitch, void* workspace,
-                                       size_t workspace_bytes) {
+                                       int interlacing, int compensate, int zero_dc, int inverse_x, void* out, int64_t out_pitch,
+                                       void* workspace, size_t workspace_bytes) {
   MP_REQUIRE(meshsize, "mp_combine_partial_power: null meshsize");
   MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
   return combine_partial((hipStream_t)stream, (const float2*)partial, pitch, meshsize, order, interlacing < 1 ? 1 : interlacing,
-                         compensate, zero_dc, (float2*)out, out_pitch, workspace, workspace_bytes, 1);
+                         compensate, zero_dc, (float2*)out, out_pitch, workspace, workspace_bytes, inverse_x ? 2 : 1);
 }
 
 MP_EXPORT int64_t mp_bin_partial_workspace_bytes(const int32_t meshsize[3], int interlacing, int nprod, int nedges) {

@@ -37,7 +37,7 @@ struct XPassArgs {
   CombineTables tab;
   float scale;  // 1 / n_i
   int zero_dc;  // the (0,0,0) mode of A is set to 0
-  int power;    // write mode: store |A|^2 (imaginary part 0) instead of A
+  int power;    // write mode: 1 stores |A|^2 (imaginary part 0) instead of A, 2 its inverse FFT along x
   const float2* tw;
   // write mode
   float2* out;
@@ -178,7 +178,31 @@ xpass_kernel(XPassArgs a) {
       for (int e = 0; e < EPT; ++e) acc[e] = cadd(acc[e], lds[(l0 + e) * C + c]);
     }
     if (j == a.ni - 1) {
-      if (c0 + c < a.ncols) {
+      bool done = false;
+      if constexpr (!BIN) {
+        if (a.power == 2) {  // |A|^2, then its inverse FFT along x in the same tile
+          const bool col_ok = c0 + c < a.ncols;
+          float s = a.scale;
+          if (comp) s *= a.tab.cy[y] * cz[kz];
+          __syncthreads();  // the combine above read the tile
+#pragma unroll
+          for (int e = 0; e < EPT; ++e) {
+            const int l = l0 + e;
+            float2 v = cscale(acc[e], comp ? s * cx[l] : s);
+            if (a.zero_dc && l == 0 && y == 0 && kz == 0) v = make_float2(0.f, 0.f);
+            lds[l * C + c] = make_float2(col_ok ? v.x * v.x + v.y * v.y : 0.f, 0.f);
+          }
+          __syncthreads();
+          fft_lds<L, C, C, 1, true, NT, true, L>(lds, stw);
+          if (col_ok) {
+            float2* dst = a.out + (int64_t)y * a.out_ostride + kz;
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) dst[(l0 + e) * a.out_fstride] = lds[(l0 + e) * C + c];
+          }
+          done = true;
+        }
+      }
+      if (!done && c0 + c < a.ncols) {
         float s = a.scale;
         if (comp) s *= a.tab.cy[y] * cz[kz];
         if constexpr (!BIN) {

@@ -40,8 +40,8 @@ SIGNATURES = {
     'mp_combine_partial_workspace_bytes': (c_int64, [P_I32, c_int]),
     'mp_combine_partial': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_void_p, c_int64, c_void_p,
                                    c_size_t]),
-    'mp_combine_partial_power': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_void_p, c_int64,
-                                         c_void_p, c_size_t]),
+    'mp_combine_partial_power': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                         c_int64, c_void_p, c_size_t]),
     'mp_bin_partial_workspace_bytes': (c_int64, [P_I32, c_int, c_int, c_int]),
     'mp_bin_poles_partial': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_int, P_F64, c_int, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
@@ -64,8 +64,8 @@ SIGNATURES = {
     'mp_bin_real_poles': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_int, P_I32,
                                   P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_irfft3d_bin_real_workspace_bytes': (c_int64, [P_I32, c_int, c_int]),
-    'mp_irfft3d_bin_real': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_int,
-                                    P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
+    'mp_irfft3d_bin_real': (c_int, [c_void_p, c_void_p, c_int64, c_int, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64,
+                                    c_int, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_spectrum_product': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, P_I32, c_void_p, c_int64]),
     'mp_mul_ylm': (c_int, [c_void_p, c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int]),
     'mp_ylm_workspace_bytes': (c_int64, [P_I32]),

@@ -388,9 +388,10 @@ class BinMesh2CorrelationPoles:
                   _lib.ptr(out), _lib.ptr(ws), ws.numel())
         return out
 
-    def _spectrum_sums_dev(self, spectrum, ells, remove_zero, los=(0., 0., 0.)):
+    def _spectrum_sums_dev(self, spectrum, ells, remove_zero, los=(0., 0., 0.), x_done=False):
         """_sums_dev() of irfftn(spectrum) for a half spectrum, with the last pass of the
-        inverse FFT fused with the binning (the real mesh is never stored)."""
+        inverse FFT fused with the binning (the real mesh is never stored); x_done: the
+        spectrum already carries its inverse FFT along x."""
         value, pitch = _as_spectrum(spectrum)
         device = value.device
         nell = len(ells)
@@ -400,7 +401,8 @@ class BinMesh2CorrelationPoles:
         ws = _lib.workspace(_lib.load().mp_irfft3d_bin_real_workspace_bytes(n, nell, nedges), device)
         col = lambda values: (_lib.ctypes.c_int32 * nell)(*[int(v) for v in values])
         t = self._tables
-        _lib.call('mp_irfft3d_bin_real', _lib.stream_handle(device), _lib.ptr(value), pitch, n, _lib.ptr(t['sx']), _lib.ptr(t['sy']),
+        _lib.call('mp_irfft3d_bin_real', _lib.stream_handle(device), _lib.ptr(value), pitch, int(x_done), n, _lib.ptr(t['sx']),
+                  _lib.ptr(t['sy']),
                   _lib.ptr(t['sz']), _lib.ptr(t['thr']), nedges, _lib.f64x3(los), nell, col(ells), col(remove_zero),
                   _lib.ptr(out), _lib.ptr(ws), ws.numel())
         return out
@@ -415,9 +417,10 @@ class BinMesh2CorrelationPoles:
             return s / self.nmodes
 
 
-def _product_spectrum(A1, A2=None):
-    """Half spectrum |A1|^2 (A2 None) or conj(A1) A2 on the device; a pending interlacing
-    combine is combined straight into |A|^2 without storing A."""
+def _product_spectrum(A1, A2=None, inverse_x=False):
+    """(P, x_done): the half spectrum P = |A1|^2 (A2 None) or conj(A1) A2 on the device.  A
+    pending interlacing combine is combined straight into |A|^2 without storing A, and with
+    inverse_x P then also carries its inverse FFT along x (x_done True)."""
     mattrs = A1.attrs
     n = _lib.i32x3(mattrs.meshsize)
     if A2 is None:
@@ -427,9 +430,9 @@ def _product_spectrum(A1, A2=None):
             out = _empty_spectrum(mattrs.meshsize, device)
             ws = _lib.workspace(_lib.load().mp_combine_partial_workspace_bytes(n, pending.interlacing), device)
             _lib.call('mp_combine_partial_power', _lib.stream_handle(device), _lib.ptr(pending.partial), pending.pitch, n,
-                      pending.order, pending.interlacing, int(pending.compensate), int(pending.zero_dc), _lib.ptr(out),
-                      _spectrum_pitch(out), _lib.ptr(ws), ws.numel())
-            return out
+                      pending.order, pending.interlacing, int(pending.compensate), int(pending.zero_dc), int(inverse_x),
+                      _lib.ptr(out), _spectrum_pitch(out), _lib.ptr(ws), ws.numel())
+            return out, bool(inverse_x)
     a, pa = _as_spectrum(A1.value)
     out = _empty_spectrum(mattrs.meshsize, a.device)
     if A2 is None:
@@ -438,7 +441,7 @@ def _product_spectrum(A1, A2=None):
         b, pb = _as_spectrum(A2.value)
     _lib.call('mp_spectrum_product', _lib.stream_handle(a.device), _lib.ptr(a), pa, _lib.ptr(b) if b is not None else None, pb, n,
               _lib.ptr(out), _spectrum_pitch(out))
-    return out
+    return out, False
 
 
 def _mul_ylm(value, out, mattrs, coords, ell, m, accumulate):
@@ -471,8 +474,8 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
         A0 = _2c(rmesh1 if same else meshes[1])
         num = {}
         if 0 in ells:
-            P = _product_spectrum(A0) if same else _product_spectrum(_2c(rmesh1), A0)
-            num[0] = bin._spectrum_sums_dev(P, [0], [False])[0]
+            P, x_done = _product_spectrum(A0, inverse_x=True) if same else _product_spectrum(_2c(rmesh1), A0)
+            num[0] = bin._spectrum_sums_dev(P, [0], [False], x_done=x_done)[0]
             del P
         if nonzeroells:
             rreal = _2r(rmesh1).value.contiguous()
@@ -500,8 +503,8 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
         num = [num[ell] / (bin._nmodes_dev * vcell) for ell in ells]
     else:  # fixed line of sight, mesh2.py:471-504
         A1 = _2c(meshes[0])
-        P = _product_spectrum(A1) if same else _product_spectrum(A1, _2c(meshes[1]))
-        sums = bin._spectrum_sums_dev(P, ells, [ell != 0 for ell in ells], los=vlos)
+        P, x_done = _product_spectrum(A1, inverse_x=True) if same else _product_spectrum(A1, _2c(meshes[1]))
+        sums = bin._spectrum_sums_dev(P, ells, [ell != 0 for ell in ells], los=vlos, x_done=x_done)
         del P
         num = [sums[i] * ((2 * ell + 1) / (bin._nmodes_dev * vcell)) for i, ell in enumerate(ells)]
     num = [n.contiguous() for n in num]
