@@ -160,45 +160,6 @@ rowsort_hist_kernel(const T* __restrict__ pos, const float* __restrict__ w, int6
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(RS_NT)
-rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int zfrac,
-                       int* __restrict__ bucket_cursor, float4* __restrict__ tmp_rec, short* __restrict__ tmp_key) {
-  __shared__ int hist[RS_MAX_KEYS];
-  __shared__ int base[RS_MAX_KEYS];
-  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
-  __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * RS_CHUNK;
-  const int64_t i1 = i0 + RS_CHUNK < np ? i0 + RS_CHUNK : np;
-  constexpr int PER = RS_CHUNK / RS_NT;
-  int rank[PER];
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    int64_t i = i0 + threadIdx.x + (int64_t)q * RS_NT;
-    if (i < i1) {
-      T xg[3];
-      grid_coords(pos, i, g, xg);
-      rank[q] = atomicAdd(&hist[pmod((int)floor(xg[0]), g.n[0])], 1);
-    }
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < g.n[0]; b += RS_NT)
-    if (hist[b]) base[b] = atomicAdd(&bucket_cursor[b], hist[b]);
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    int64_t i = i0 + threadIdx.x + (int64_t)q * RS_NT;
-    if (i < i1) {
-      int hx, hy;
-      float4 rec;
-      row_record(pos, w, i, g, zfrac, hx, hy, rec);
-      int64_t slot = (int64_t)base[hx] + rank[q];
-      tmp_rec[slot] = rec;
-      tmp_key[slot] = (short)hy;
-    }
-  }
-}
-
 __device__ __forceinline__ int wave_scan_incl(int v) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -233,6 +194,71 @@ __device__ int block_excl_scan(int* data, int n, int* wsum) {
   return wsum[RS_NT / 64];
 }
 
+
+// Level A scatter: each block builds the records of RS_SCHUNK particles once, orders them by x
+// bucket in LDS, reserves one range per (block, bucket) in the bucket arrays, and writes each
+// bucket's run with consecutive lanes, so the global writes are contiguous runs rather than
+// scattered 16-byte records.
+constexpr int RS_SCHUNK = 6144;
+
+template <typename T>
+__global__ void __launch_bounds__(RS_NT)
+rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int zfrac,
+                       int* __restrict__ bucket_cursor, float4* __restrict__ tmp_rec, short* __restrict__ tmp_key) {
+  __shared__ int hist[RS_MAX_KEYS];
+  __shared__ int lstart[RS_MAX_KEYS];
+  __shared__ int gbase[RS_MAX_KEYS];
+  __shared__ int wsum[RS_NT / 64 + 1];
+  extern __shared__ float4 srec[];  // RS_SCHUNK records, then their y keys and x buckets
+  short* skey = reinterpret_cast<short*>(srec + RS_SCHUNK);
+  short* sbkt = skey + RS_SCHUNK;
+  const int n0 = g.n[0];
+  for (int b = threadIdx.x; b < n0; b += RS_NT) hist[b] = 0;
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * RS_SCHUNK;
+  const int cnt = (int)(i0 + RS_SCHUNK < np ? RS_SCHUNK : np - i0);
+  constexpr int PER = RS_SCHUNK / RS_NT;
+  int rank[PER], hxq[PER], hyq[PER];
+  float4 recq[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int t = threadIdx.x + q * RS_NT;
+    if (t < cnt) {
+      row_record(pos, w, i0 + t, g, zfrac, hxq[q], hyq[q], recq[q]);
+      rank[q] = atomicAdd(&hist[hxq[q]], 1);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < n0; b += RS_NT) {
+    lstart[b] = hist[b];
+    if (hist[b]) gbase[b] = atomicAdd(&bucket_cursor[b], hist[b]);
+  }
+  __syncthreads();
+  for (int b0 = 0; b0 < n0; b0 += 2 * RS_NT) {  // exclusive scan of the bucket counts
+    const int m = min(2 * RS_NT, n0 - b0);
+    const int carry = b0 ? lstart[b0 - 1] + hist[b0 - 1] : 0;
+    block_excl_scan(lstart + b0, m, wsum);
+    for (int b = threadIdx.x; b < m; b += RS_NT) lstart[b0 + b] += carry;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int t = threadIdx.x + q * RS_NT;
+    if (t < cnt) {
+      const int slot = lstart[hxq[q]] + rank[q];
+      srec[slot] = recq[q];
+      skey[slot] = (short)hyq[q];
+      sbkt[slot] = (short)hxq[q];
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < cnt; t += RS_NT) {
+    const int b = sbkt[t];
+    const int64_t dst = (int64_t)gbase[b] + (t - lstart[b]);
+    tmp_rec[dst] = srec[t];
+    tmp_key[dst] = skey[t];
+  }
+}
 
 // One workgroup per x bucket: counting sort by y into ysorted (the order the paint reads);
 // writes the row offsets and a bound on the sum of |w| of every row.
@@ -939,9 +965,13 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
     MP_LAUNCH("paint_rowsort_hist", rowsort_hist_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, bstart, wmax);
   rc = exclusive_scan_i32(s, bstart, n[0] + 1, scan_tmp, bcursor);
   if (rc) return rc;
-  if (np > 0)
-    MP_LAUNCH("paint_rowsort_scatter", rowsort_scatter_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, zfrac_of(n[2]),
-              bcursor, tmp_rec, tmp_key);
+  if (np > 0) {
+    constexpr size_t slds = (size_t)RS_SCHUNK * (sizeof(float4) + 2 * sizeof(short));
+    auto skern = rowsort_scatter_kernel<T>;
+    MP_CHECK_HIP(hipFuncSetAttribute((const void*)skern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));
+    MP_LAUNCH("paint_rowsort_scatter", skern, dim3((unsigned)ceil_div(np, RS_SCHUNK)), dim3(RS_NT), slds, s, pos, w, np, g,
+              zfrac_of(n[2]), bcursor, tmp_rec, tmp_key);
+  }
   MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), 0, s, (const int*)bstart, tmp_rec,
             (const short*)tmp_key, n[1], n[2], nkeys, (const unsigned*)wmax, sorted, offsets, row_wabs);
   out.sorted = sorted;
