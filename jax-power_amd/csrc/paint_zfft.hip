@@ -261,14 +261,20 @@ rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, i
 }
 
 // One workgroup per x bucket: counting sort by y into ysorted (the order the paint reads);
-// writes the row offsets and a bound on the sum of |w| of every row.
+// writes the row offsets and a bound on the sum of |w| of every row.  The bucket is moved in
+// sub-chunks of RS_SCHUNK records: each is ordered by y in LDS and every y run is written
+// contiguously after the records of that row already placed.
 __global__ void __launch_bounds__(RS_NT)
 rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
                     int nz, int64_t nkeys, const unsigned* __restrict__ wmax_bits, float4* __restrict__ ysorted,
                     int* __restrict__ offsets, float* __restrict__ row_wabs) {
-  __shared__ int cnt[RS_MAX_KEYS + 1];
+  __shared__ int cnt[RS_MAX_KEYS];     // whole bucket: counts, then the next free slot of each row
+  __shared__ int lcnt[RS_MAX_KEYS];    // sub-chunk counts
+  __shared__ int lstart[RS_MAX_KEYS];  // sub-chunk start of each row in LDS
   __shared__ unsigned long long qsum[RS_MAX_KEYS];
   __shared__ int wsum[RS_NT / 64 + 1];
+  extern __shared__ float4 srec[];  // RS_SCHUNK records, then their y keys
+  short* skey = reinterpret_cast<short*>(srec + RS_SCHUNK);
   const int hx = blockIdx.x;
   const int start = bucket_start[hx], end = bucket_start[hx + 1];
   for (int y = threadIdx.x; y < ny; y += RS_NT) {
@@ -279,21 +285,54 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restri
   for (int i = start + threadIdx.x; i < end; i += RS_NT) atomicAdd(&cnt[tmp_key[i]], 1);
   __syncthreads();
   block_excl_scan(cnt, ny, wsum);
-  for (int y = threadIdx.x; y < ny; y += RS_NT) offsets[(int64_t)hx * ny + y] = start + cnt[y];
+  for (int y = threadIdx.x; y < ny; y += RS_NT) {
+    offsets[(int64_t)hx * ny + y] = start + cnt[y];
+    cnt[y] += start;
+  }
   if (hx == 0 && threadIdx.x == 0) offsets[nkeys] = bucket_start[gridDim.x];
-  __syncthreads();
   // Per-row bound on sum |w|, which sets the fixed-point scale of the paint.  It must not
   // depend on the order of the row's records (which the atomics leave unspecified), so it is
   // an integer sum: q_i = ceil(|w_i| / m * 2^24) with m the largest |w| of the catalogue, and
   // the bound is m * sum q_i / 2^24 >= sum |w|.
   const float m = __uint_as_float(*wmax_bits);
   const float qs = m > 0.f ? 16777216.f / m : 0.f;
-  for (int i = start + threadIdx.x; i < end; i += RS_NT) {
-    const int y = tmp_key[i];
-    const int r = atomicAdd(&cnt[y], 1);
-    const float4 rec = tmp_rec[i];
-    ysorted[start + r] = rec;
-    atomicAdd(&qsum[y], (unsigned long long)ceilf(fabsf(rec.w) * qs));
+  constexpr int PER = RS_SCHUNK / RS_NT;
+  for (int sub = start; sub < end; sub += RS_SCHUNK) {
+    const int n = min(RS_SCHUNK, end - sub);
+    for (int y = threadIdx.x; y < ny; y += RS_NT) lcnt[y] = 0;
+    __syncthreads();
+    int rank[PER], key[PER];
+    float4 rec[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int t = threadIdx.x + q * RS_NT;
+      if (t < n) {
+        key[q] = tmp_key[sub + t];
+        rec[q] = tmp_rec[sub + t];
+        rank[q] = atomicAdd(&lcnt[key[q]], 1);
+        atomicAdd(&qsum[key[q]], (unsigned long long)ceilf(fabsf(rec[q].w) * qs));
+      }
+    }
+    __syncthreads();
+    for (int y = threadIdx.x; y < ny; y += RS_NT) lstart[y] = lcnt[y];
+    __syncthreads();
+    block_excl_scan(lstart, ny, wsum);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int t = threadIdx.x + q * RS_NT;
+      if (t < n) {
+        const int slot = lstart[key[q]] + rank[q];
+        srec[slot] = rec[q];
+        skey[slot] = (short)key[q];
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += RS_NT) {
+      const int y = skey[t];
+      ysorted[cnt[y] + (t - lstart[y])] = srec[t];
+    }
+    __syncthreads();
+    for (int y = threadIdx.x; y < ny; y += RS_NT) cnt[y] += lcnt[y];
   }
   __syncthreads();
   for (int y = threadIdx.x; y < ny; y += RS_NT)
@@ -972,7 +1011,9 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
     MP_LAUNCH("paint_rowsort_scatter", skern, dim3((unsigned)ceil_div(np, RS_SCHUNK)), dim3(RS_NT), slds, s, pos, w, np, g,
               zfrac_of(n[2]), bcursor, tmp_rec, tmp_key);
   }
-  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), 0, s, (const int*)bstart, tmp_rec,
+  constexpr size_t flds = (size_t)RS_SCHUNK * (sizeof(float4) + sizeof(short));
+  MP_CHECK_HIP(hipFuncSetAttribute((const void*)rowsort_fine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
+  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), flds, s, (const int*)bstart, tmp_rec,
             (const short*)tmp_key, n[1], n[2], nkeys, (const unsigned*)wmax, sorted, offsets, row_wabs);
   out.sorted = sorted;
   out.offsets = offsets;
