@@ -124,27 +124,31 @@ zpass_r2c_ylm_kernel(const float* __restrict__ in, float2* __restrict__ out, int
   load_twiddles<2 * M, NT>(stw, g_twiddle);
   const float2* in2 = reinterpret_cast<const float2*>(in);
   float2 reg[EPT];
+  float px[EPT], py[EPT];  // node x, y of each element's row, computed once for every m
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int idx = tid + e * NT;
     const int64_t row = row0 + idx / M;
-    reg[e] = (idx < C * M && row < nrows) ? in2[row * M + idx % M] : make_float2(0.f, 0.f);
+    const bool ok = idx < C * M && row < nrows;
+    reg[e] = ok ? in2[row * M + idx % M] : make_float2(0.f, 0.f);
+    const int r = ok ? (int)row : 0;
+    const int ix = r / pro.ny, iy = r - ix * pro.ny;
+    px[e] = pro.x0 + pro.dx * ix;
+    py[e] = pro.y0 + pro.dy * iy;
   }
   constexpr int NK = M / 2 + 1;
   for (int m = -pro.ell; m <= pro.ell; ++m) {
-    RealPrologue p = pro;
-    p.m = m;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int idx = tid + e * NT;
       if (idx < C * M) {
         const int c = idx / M, l = idx % M;
-        const int64_t row = row0 + c;
         float2 v = reg[e];
-        if (row < nrows) {
-          v.x *= p.eval(row, 2 * l);
-          v.y *= p.eval(row, 2 * l + 1);
-        }
+        // opaque per m: otherwise the compiler hoists every loop-invariant term of every
+        // Y_lm case out of the m loop and runs out of registers
+        asm volatile("" : "+v"(px[e]), "+v"(py[e]));
+        v.x *= ylm_of_vector(pro.ell, m, px[e], py[e], pro.z0 + pro.dz * (2 * l));
+        v.y *= ylm_of_vector(pro.ell, m, px[e], py[e], pro.z0 + pro.dz * (2 * l + 1));
         lds[c * LP + l] = v;
       }
     }
@@ -621,6 +625,7 @@ int rfft_zy_ylm(hipStream_t s, const float* in, float2* Z, int64_t mesh_stride, 
   if (rc) return rc;
   const int M = n[2] / 2;
   if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  MP_REQUIRE(pro.ell >= 1 && pro.ell <= MAX_ELL, "rfft_zy_ylm: ell %d not in 1..%d", pro.ell, MAX_ELL);
   const int64_t nrows = (int64_t)n[0] * n[1];
   rc = [&]() -> int { MP_POW2_SWITCH(M, launch_zpass_r2c_ylm, in, Z, nrows, pitch, mesh_stride, pro, s); }();
   if (rc) return rc;

@@ -67,6 +67,11 @@ __host__ __device__ inline size_t xpass_extra_lds(int L, int ncols, int ni, int 
 // latency than one block with twice the columns (measured on the 512^3 x pass).
 template <int L>
 using XPCfg = XCfgC<L, L < 512 ? (L >= 4 ? 16 : 64 / L) : 4096 / L>;
+// Register budget of the local line-of-sight binning x pass: at least 4 waves per SIMD
+// (<= 128 VGPRs), so that two 8-wave blocks share a CU; unconstrained it took 160 VGPRs at
+// L = 1024 and ran 1.3x slower (measured on the 1024^3 FKP spectrum).  The combine kernel
+// keeps its registers: capped it spills and slows down.
+constexpr int XP_WAVES_PER_SIMD = 4;
 
 // First tile of a persistent block.  Blocks are dispatched to the 8 XCDs round-robin; here
 // the blocks of one XCD take a contiguous run of tiles, so the two column chunks that share a
@@ -302,7 +307,7 @@ struct YlmBinArgs {
 };
 
 template <int L, int NPP>
-__global__ void __launch_bounds__(XPCfg<L>::NT)
+__global__ void __launch_bounds__(XPCfg<L>::NT, XP_WAVES_PER_SIMD)
 xpass_ylm_bin_kernel(YlmBinArgs a) {
   using Cfg = XPCfg<L>;
   constexpr int C = Cfg::C, NT = Cfg::NT;

@@ -1016,9 +1016,11 @@ def compute_box_normalization(*inputs):
     attrs = inputs[0].attrs
     size = float(np.prod(attrs.meshsize))
     norm = 1.
+    sums = {}  # an auto normalisation passes the same field twice: sum it once
     for inp in inputs:
-        s = inp.sum() if isinstance(inp, ParticleField) else _sum_f64(inp.value.reshape(-1))
-        norm = norm * s / size
+        if id(inp) not in sums:
+            sums[id(inp)] = inp.sum() if isinstance(inp, ParticleField) else _sum_f64(inp.value.reshape(-1))
+        norm = norm * sums[id(inp)] / size
     return norm * size * float(np.prod(attrs.cellsize))**(1 - len(inputs))
 
 
