@@ -10,8 +10,10 @@ painting, FFTs, readout and FP64 reductions run in libjaxpower_amd.so.  Half spe
 (N0, N1, N2//2+1) complex64 views of buffers whose rows are padded to a multiple of 16
 elements, so every row starts on a 128-byte boundary.
 """
+import json
 import math
 import operator
+import os
 from functools import lru_cache
 
 import numpy as np
@@ -19,6 +21,48 @@ import torch
 
 from . import _lib
 from . import resamplers
+
+
+def _save_npz(fn, state, attrs):
+    """npz with the arrays of ``state`` and the attributes (mesh.py:1352-1390): the attributes
+    are stored as a JSON string, so reading them back executes nothing.  h5py is not part of
+    this build, so only '.npz' is written."""
+    fn = str(fn)
+    if not fn.endswith('.npz'):
+        raise ValueError(f'extension of {fn!r} not supported here (npz only)')
+    dirname = os.path.dirname(fn)
+    if dirname:
+        os.makedirs(dirname, exist_ok=True)
+    arrays = {}
+    for name, value in state.items():
+        arrays[name] = value.detach().cpu().numpy() if isinstance(value, torch.Tensor) else np.asarray(value)
+    arrays['attrs'] = np.array(json.dumps(attrs))
+    np.savez(fn, **arrays)
+
+
+def _load_npz(fn, allow_pickle=False):
+    """(arrays, attrs) of a file written by ``_save_npz``.  Files written by the reference keep
+    their attributes as a pickled dict (mesh.py:1413-1416): those are read only with an explicit
+    allow_pickle=True, for files from a trusted source."""
+    fn = str(fn)
+    if not fn.endswith('.npz'):
+        raise ValueError(f'extension of {fn!r} not supported here (npz only)')
+    try:
+        with np.load(fn, allow_pickle=allow_pickle) as data:
+            raw = data['attrs']
+            attrs = json.loads(str(raw)) if raw.dtype.kind == 'U' else raw[()]
+            arrays = {name: data[name] for name in data.files if name != 'attrs'}
+    except ValueError as exc:
+        if not allow_pickle and 'allow_pickle' in str(exc):
+            raise ValueError(f'{fn}: attributes are pickled (a file written by jax-power); pass allow_pickle=True '
+                             'only for a trusted file') from exc
+        raise
+    return arrays, dict(attrs)
+
+
+def _attrs_from_state(state):
+    keys = ('meshsize', 'boxsize', 'boxcenter', 'dtype', 'fft_backend')
+    return MeshAttrs.from_state({k: state[k] for k in keys if k in state})
 
 
 def _default_device():
@@ -490,6 +534,16 @@ class BaseMeshField:
     def __getitem__(self, item):
         return self.value[item]
 
+    def save(self, fn):
+        """Save the mesh (value and attrs) to an npz file (mesh.py:1783-1787)."""
+        _save_npz(fn, {'value': self.value}, self.attrs.__getstate__())
+
+    @classmethod
+    def load(cls, fn, device=None, allow_pickle=False):
+        """Mesh saved by ``save`` (mesh.py:1789-1797)."""
+        arrays, attrs = _load_npz(fn, allow_pickle=allow_pickle)
+        return cls(arrays['value'], attrs=_attrs_from_state(attrs), device=device)
+
     @property
     def at(self):
         return _UpdateHelper(self)
@@ -538,20 +592,6 @@ class BaseMeshField:
 
     def ravel(self):
         return self.value.reshape(-1)
-
-    def save(self, fn):
-        """npz: value and attrs state (mesh.py:1784-1788)."""
-        state = self.attrs.__getstate__()
-        np.savez(fn, value=np.asarray(self), **{f'attrs_{k}': np.asarray(v) for k, v in state.items()})
-
-    @classmethod
-    def load(cls, fn):
-        with np.load(fn, allow_pickle=False) as data:
-            state = {k[6:]: data[k] for k in data.files if k.startswith('attrs_')}
-            state['dtype'] = str(state['dtype'])
-            state['fft_backend'] = str(state['fft_backend'])
-            attrs = MeshAttrs(**state)
-            return cls(data['value'], attrs=attrs)
 
 
 def _binary(name, op):
@@ -770,6 +810,25 @@ class ParticleField:
         """Single device: particles already sit with their mesh."""
         return self.clone(**kwargs)
 
+    def save(self, fn):
+        """Save positions, weights, extra fields and attrs to an npz file (mesh.py:2646-2658)."""
+        state = {'positions': self.positions, 'weights': self.weights}
+        for name, value in self.extra.items():
+            if name in state:
+                raise ValueError(f'extra field name {name} conflicts with reserved field names')
+            state[name] = value
+        attrs = self.attrs.__getstate__()
+        if self.extra:
+            attrs['extra'] = list(self.extra)
+        _save_npz(fn, state, attrs)
+
+    @classmethod
+    def load(cls, fn, device=None, allow_pickle=False):
+        """Particles saved by ``save`` (mesh.py:2660-2670)."""
+        arrays, attrs = _load_npz(fn, allow_pickle=allow_pickle)
+        extra = {name: arrays.pop(name) for name in attrs.pop('extra', [])}
+        return cls(arrays['positions'], arrays['weights'], attrs=_attrs_from_state(attrs), extra=extra, device=device)
+
     def __getitem__(self, name):
         return self.clone(positions=self.positions[name], weights=self.weights[name],
                           extra={k: v[name] for k, v in self.extra.items()})
@@ -821,21 +880,6 @@ class ParticleField:
         attrs = self.attrs if dtype is None else self.attrs.clone(dtype=dtype)
         return _paint(attrs, self.positions, self.weights, resampler=resampler, interlacing=interlacing,
                       compensate=compensate, out=out)
-
-    def save(self, fn):
-        state = self.attrs.__getstate__()
-        arrays = dict(positions=self.positions.cpu().numpy(), weights=self.weights.cpu().numpy())
-        arrays.update({f'extra_{k}': v.cpu().numpy() for k, v in self.extra.items()})
-        np.savez(fn, **arrays, **{f'attrs_{k}': np.asarray(v) for k, v in state.items()})
-
-    @classmethod
-    def load(cls, fn):
-        with np.load(fn, allow_pickle=False) as data:
-            state = {k[6:]: data[k] for k in data.files if k.startswith('attrs_')}
-            state['dtype'] = str(state['dtype'])
-            state['fft_backend'] = str(state['fft_backend'])
-            extra = {k[6:]: data[k] for k in data.files if k.startswith('extra_')}
-            return cls(data['positions'], data['weights'], attrs=MeshAttrs(**state), extra=extra)
 
 
 def _scale_weights(w, factor):

@@ -151,3 +151,44 @@ def test_deferred_values_in_containers():
     assert np.allclose(pole.value(), (np.array([10., 20., 30.]) - 2.) / 4.)
     assert float(norm) == 4. and np.isclose(norm, 4.) and norm * 2 == 8. and 1. / norm == 0.25 and np.allclose(sn + 1., 3.)
     assert np.asarray(sn).shape == (3,) and len(sn) == 3 and sn[1] == 2.
+
+
+def test_mesh_and_particle_npz_round_trip(tmp_path):
+    """save/load (mesh.py:1352-1450, 1783-1797, 2646-2670) on the host: values, attrs and extra
+    fields come back; attrs are JSON, so loading needs no pickle."""
+    import torch
+    from jaxpower_amd import MeshAttrs, RealMeshField, ComplexMeshField, ParticleField
+    attrs = MeshAttrs(meshsize=(4, 6, 8), boxsize=(40., 60., 80.), boxcenter=(1., 2., 3.))
+    rng = np.random.default_rng(0)
+    value = rng.normal(size=(4, 6, 8)).astype('f4')
+    mesh = RealMeshField(torch.from_numpy(value), attrs=attrs, device='cpu')
+    mesh.save(tmp_path / 'sub' / 'mesh.npz')
+    back = RealMeshField.load(tmp_path / 'sub' / 'mesh.npz', device='cpu')
+    assert np.array_equal(np.asarray(back), value)
+    assert back.attrs == attrs
+    cvalue = (rng.normal(size=(4, 6, 5)) + 1j * rng.normal(size=(4, 6, 5))).astype('c8')
+    cmesh = ComplexMeshField(torch.from_numpy(cvalue), attrs=attrs, device='cpu')
+    cmesh.save(tmp_path / 'cmesh.npz')
+    assert np.array_equal(np.asarray(ComplexMeshField.load(tmp_path / 'cmesh.npz', device='cpu')), cvalue)
+    pos = rng.uniform(0., 40., (100, 3)).astype('f4')
+    p = ParticleField(torch.from_numpy(pos), torch.ones(100), attrs=attrs, extra={'id': torch.arange(100)}, device='cpu')
+    p.save(tmp_path / 'particles.npz')
+    q = ParticleField.load(tmp_path / 'particles.npz', device='cpu')
+    assert np.array_equal(q.positions.numpy(), pos) and np.array_equal(q.extra['id'].numpy(), np.arange(100))
+    assert q.attrs == attrs
+    with pytest.raises(ValueError):
+        mesh.save(tmp_path / 'mesh.h5')
+
+
+def test_load_reference_style_file_needs_allow_pickle(tmp_path):
+    """A file written the way jax-power writes it (attrs as a dict, pickled by numpy) loads only
+    with an explicit allow_pickle=True."""
+    from jaxpower_amd import RealMeshField
+    value = np.arange(64, dtype='f4').reshape(4, 4, 4)
+    state = dict(meshsize=(4, 4, 4), boxsize=(10., 10., 10.), boxcenter=(0., 0., 0.), dtype='<f4', fft_backend='auto')
+    fn = tmp_path / 'ref.npz'
+    np.savez(fn, value=value, attrs=state)
+    with pytest.raises(ValueError, match='allow_pickle'):
+        RealMeshField.load(fn, device='cpu')
+    back = RealMeshField.load(fn, device='cpu', allow_pickle=True)
+    assert np.array_equal(np.asarray(back), value)
