@@ -42,9 +42,13 @@ class Mesh2SpectrumBatch:
     ``compute_mesh2_spectrum(mesh, bin=bin, los=los)``, then box normalization and shot noise.
 
     ``graph=True`` captures the whole batch into one HIP graph on the first call and replays it
-    afterwards; ``graph=False`` runs the same sequence eagerly (the per-mock reference)."""
+    afterwards; ``graph=False`` runs the same sequence eagerly (the per-mock reference).
+    ``streams > 1`` issues the mocks round-robin on that many streams forked from the caller's
+    stream (and joined back), so the kernels of independent mocks overlap on the GPU; in the
+    captured graph they become parallel branches."""
 
-    def __init__(self, fields, bin, resampler='cic', interlacing=0, compensate=False, los='z', remove_dc=True, graph=True):
+    def __init__(self, fields, bin, resampler='cic', interlacing=0, compensate=False, los='z', remove_dc=True, graph=True,
+                 streams=1):
         self.fields = list(fields)
         if not self.fields or not all(isinstance(f, ParticleField) for f in self.fields):
             raise ValueError('fields must be a non-empty list of ParticleField')
@@ -53,7 +57,8 @@ class Mesh2SpectrumBatch:
             raise ValueError('all fields of a batch must share one MeshAttrs')
         self.bin, self.resampler, self.interlacing, self.compensate = bin, resampler, interlacing, compensate
         self.los, self.remove_dc, self.graph = los, remove_dc, graph
-        self._graph, self._outputs = None, None
+        self.streams = max(1, int(streams))
+        self._graph, self._outputs, self._side = None, None, None
 
     def _one(self, p):
         mesh = p.paint(resampler=self.resampler, interlacing=self.interlacing, compensate=self.compensate, out='complex')
@@ -63,7 +68,21 @@ class Mesh2SpectrumBatch:
         return spec.clone(norm=compute_box2_normalization(p, bin=self.bin), num_shotnoise=compute_fkp2_shotnoise(p, bin=self.bin))
 
     def _run(self):
-        return [self._one(p) for p in self.fields]
+        if self.streams == 1:
+            return [self._one(p) for p in self.fields]
+        main = torch.cuda.current_stream()
+        if self._side is None:
+            self._side = [torch.cuda.Stream() for _ in range(self.streams)]  # default priority
+        out = [None] * len(self.fields)
+        for side in self._side:
+            side.wait_stream(main)
+        for k, side in enumerate(self._side):
+            with torch.cuda.stream(side):
+                for i in range(k, len(self.fields), self.streams):
+                    out[i] = self._one(self.fields[i])
+        for side in self._side:
+            main.wait_stream(side)
+        return out
 
     def capture(self):
         """Warm up (tuning, twiddles, allocator) on a side stream, then capture the batch."""

@@ -57,3 +57,26 @@ def test_batch_graph_replays_match_eager(resampler, interlacing):
     omesh[0, 0, 0] = 0.
     want = po.spectrum_poles(oattrs, po.BinPoles(oattrs, edges={'step': 0.02}, ells=ells), omesh, los='z')
     assert_poles_close([ref.get(ell).values('num_raw') for ell in ells], want['num_raw'], label='batch vs oracle')
+
+
+def test_batch_streams_graph_matches_eager():
+    """Mocks issued on several forked streams (parallel branches of the captured graph) give
+    the per-mock spectra; FP64 bin sums may differ in their last bits where blocks of other
+    mocks change the order of LDS additions."""
+    from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, Mesh2SpectrumBatch
+    say('batch streams')
+    attrs = MeshAttrs(meshsize=(32, 64, 32), boxsize=(320., 640., 320.), boxcenter=0.)
+    ells = (0, 2, 4)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=ells)
+    fields = _fields(attrs, [3000, 1200, 5000, 800, 2600], 7)
+    eager = Mesh2SpectrumBatch(fields, bin, resampler='tsc', interlacing=2, compensate=True, graph=False)()
+    for graph in (False, True):
+        batch = Mesh2SpectrumBatch(fields, bin, resampler='tsc', interlacing=2, compensate=True, graph=graph, streams=3)
+        for _ in range(2):
+            got = batch()
+            torch.cuda.synchronize()
+            for g, e in zip(got, eager):
+                a, b = _values(g, ells), _values(e, ells)
+                assert np.array_equal(np.isnan(a), np.isnan(b))
+                ok = ~np.isnan(b)
+                assert np.allclose(a[ok], b[ok], rtol=1e-12, atol=1e-12 * np.nanmax(np.abs(b)))
