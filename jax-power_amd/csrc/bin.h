@@ -3,6 +3,7 @@
 #pragma once
 
 #include "common.h"
+#include "sph.h"
 
 namespace mp {
 
@@ -51,6 +52,7 @@ __device__ __forceinline__ int walk_bin(const double* thr, int nedges, int b, do
 // asked for (sum of value * P_ell(mu_s) per bin, the (0,0,0) cell skipped where rz is set).
 constexpr int CB_MAX_ELL = 6;
 constexpr int CB_MAX_BLOCKS = 1024;
+static_assert(CB_MAX_ELL <= LEG_MAX, "Legendre table too short");
 struct RealBinArgs {
   const float* mesh;  // (n0, n1, n2), z fastest; unused by the fused z pass
   int n0, n1, n2, nedges;
@@ -60,6 +62,7 @@ struct RealBinArgs {
   const double* thr;  // thr[e] = min{t : fl(sqrt(t)) >= edge e}
   int nell;
   int ell[CB_MAX_ELL], rz[CB_MAX_ELL];
+  LegendreTable leg;
   float los[3];
   int need_mu;
 };

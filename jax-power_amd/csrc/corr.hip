@@ -122,10 +122,11 @@ bin_real_kernel(RealBinArgs a, double* __restrict__ partial) {
         mu = s2 > 0.f ? (lxy + fsz * a.los[2]) * rsqrtf(s2) : 0.f;
       }
       const bool origin = (ix == 0) & (iy == 0) & (iz == 0);
+      const float m2 = mu * mu;
 #pragma unroll
       for (int e = 0; e < NE; ++e) {
         if (origin && a.rz[e]) continue;
-        acc[e] += (double)(v * (a.ell[e] ? legendre(a.ell[e], mu) : 1.f));
+        acc[e] += (double)(v * legendre_eval(a.leg, e, mu, m2));
       }
     }
     if (b >= 1 && b < a.nedges) {
@@ -199,6 +200,7 @@ static int real_ells(RealBinArgs& a, const double los[3], int nell, const int32_
     MP_REQUIRE(ells[e] >= 0 && ells[e] <= 10, "correlation binning: ell %d not in 0..10", ells[e]);
     a.ell[e] = ells[e];
     a.rz[e] = remove_zero[e];
+    legendre_table_set(a.leg, e, ells[e]);
     if (ells[e]) a.need_mu = 1;
   }
   for (int d = 0; d < 3; ++d) a.los[d] = los ? (float)los[d] : 0.f;

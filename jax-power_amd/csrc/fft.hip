@@ -300,10 +300,11 @@ zpass_c2r_bin_kernel(const float2* __restrict__ in, int64_t in_pitch, int64_t nr
             mu = s2 > 0.f ? (lxy + fsz * a.los[2]) * rsqrtf(s2) : 0.f;
           }
           const bool origin = (ix == 0) & (iy == 0) & (iz == 0);
+          const float m2 = mu * mu;
 #pragma unroll
           for (int e = 0; e < NE; ++e) {
             if (origin && a.rz[e]) continue;
-            acc[e] += (double)(v * (a.ell[e] ? legendre(a.ell[e], mu) : 1.f));
+            acc[e] += (double)(v * legendre_eval(a.leg, e, mu, m2));
           }
         }
       }
@@ -379,7 +380,9 @@ colpass_kernel(ColPassArgs a) {
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int idx = tid + e * NT, cc = idx % C, l = idx / C;
-      reg[e] = valid(tile, cc) ? src[l * a.fstride + cc] : make_float2(0.f, 0.f);
+      if (!valid(tile, cc)) reg[e] = make_float2(0.f, 0.f);
+      else if constexpr (C >= 16) reg[e] = load_stream(src + l * a.fstride + cc);
+      else reg[e] = src[l * a.fstride + cc];
     }
   };
   int tile = blockIdx.x;
@@ -395,7 +398,9 @@ colpass_kernel(ColPassArgs a) {
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int idx = tid + e * NT, cc = idx % C, l = idx / C;
-      if (valid(tile, cc)) dst[l * a.fstride + cc] = lds[idx];
+      if (!valid(tile, cc)) continue;
+      if constexpr (C >= 16) store_stream(dst + l * a.fstride + cc, lds[idx]);
+      else dst[l * a.fstride + cc] = lds[idx];
     }
     __syncthreads();
     tile = next;

@@ -400,6 +400,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   constexpr int SEG = pz_seg_capacity(RX, RY);
   __shared__ int sstart[SEG];
   __shared__ int scount[SEG];
+  __shared__ int shome[SEG];  // home row relative to the block, (x << 16) | (y & 0xffff)
   __shared__ int spre[SEG + 2];
   __shared__ float swabs[SEG];
   __shared__ float sbound[1];
@@ -441,6 +442,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     int c = offsets[key + 1] - st;
     sstart[i] = st;
     scount[i] = c;
+    shome[i] = ((ix - hi) * 65536) | ((iy - hi) & 0xffff);
     spre[i + 1] = (c + CHUNK - 1) / CHUNK;
     swabs[i] = row_wabs[key];
   }
@@ -505,7 +507,8 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     const int seg = lo_i;
     const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
     const int cnt = min(CHUNK, sstart[seg] + scount[seg] - p0);
-    const int hxr = seg / sy - hi, hyr = seg % sy - hi;  // home row relative to the block
+    const int home = shome[seg];
+    const int hxr = home >> 16, hyr = (int)(short)(home & 0xffff);  // home row relative to the block
     float4 recs[CHUNK];
 #pragma unroll
     for (int q = 0; q < CHUNK; ++q) recs[q] = q < cnt ? sorted[p0 + q] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -584,6 +587,7 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   extern __shared__ int irows[];  // row r, cell z at irows[r * rs + z]
   __shared__ int sstart[SEG];
   __shared__ int scount[SEG];
+  __shared__ int shome[SEG];  // home row relative to the block, (x << 16) | (y & 0xffff)
   __shared__ int spre[SEG + 2];
   __shared__ float swabs[SEG];
   __shared__ float tbound[NROW];
@@ -609,6 +613,7 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     const int c = offsets[key + 1] - st;
     sstart[i] = st;
     scount[i] = c;
+    shome[i] = ((ix - hi) * 65536) | ((iy - hi) & 0xffff);
     spre[i + 1] = (c + CHUNK - 1) / CHUNK;
     swabs[i] = row_wabs[key];
   }
@@ -668,7 +673,8 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     const int seg = lo_i;
     const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
     const int cnt = min(CHUNK, sstart[seg] + scount[seg] - p0);
-    const int hxr = seg / sy - hi, hyr = seg % sy - hi;
+    const int home = shome[seg];
+    const int hxr = home >> 16, hyr = (int)(short)(home & 0xffff);  // home row relative to the block
     float4 recs[CHUNK];
 #pragma unroll
     for (int q = 0; q < CHUNK; ++q) recs[q] = q < cnt ? sorted[p0 + q] : make_float4(0.f, 0.f, 0.f, 0.f);

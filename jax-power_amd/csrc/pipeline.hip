@@ -107,10 +107,26 @@ __global__ void __launch_bounds__(RED_NT)
 dot_partial_kernel(const float* __restrict__ a, const float* __restrict__ b, int64_t n, int square, double* __restrict__ partial) {
   __shared__ double sh[RED_NT];
   double s = 0.;
-  for (int64_t i = (int64_t)blockIdx.x * RED_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_NT) {
-    double x = (double)a[i];
-    s += b ? x * (double)b[i] : (square ? x * x : x);
+  auto term = [&](float u, float v) {
+    const double x = (double)u;
+    return b ? x * (double)v : (square ? x * x : x);
+  };
+  const int64_t first = (int64_t)blockIdx.x * RED_NT + threadIdx.x, stride = (int64_t)gridDim.x * RED_NT;
+  int64_t done = 0;
+  // 16-byte loads where both vectors allow them (weights of particle fields are), the rest one
+  // element at a time; the grid size fixes the summation order, so sums are reproducible
+  if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0) {
+    const int64_t n4 = n / 4;
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    for (int64_t i = first; i < n4; i += stride) {
+      const float4 u = a4[i];
+      const float4 v = b ? b4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      s += (term(u.x, v.x) + term(u.y, v.y)) + (term(u.z, v.z) + term(u.w, v.w));
+    }
+    done = n4 * 4;
   }
+  for (int64_t i = done + first; i < n; i += stride) s += term(a[i], b ? b[i] : 0.f);
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int d = RED_NT / 2; d > 0; d >>= 1) {

@@ -35,6 +35,33 @@ __host__ __device__ __forceinline__ float legendre(int ell, float mu) {
   return p1;
 }
 
+// P_l(mu) for a fixed list of multipoles without a branch per mode: for l <= 5,
+// P_l(mu) = mu^(l mod 2) (c0 + c1 mu^2 + c2 mu^4), the closed forms above in Horner form;
+// higher l take legendre().
+constexpr int LEG_MAX = 6;
+struct LegendreTable {
+  float c0[LEG_MAX], c1[LEG_MAX], c2[LEG_MAX];
+  int odd[LEG_MAX], ell[LEG_MAX];
+};
+
+inline void legendre_table_set(LegendreTable& t, int p, int ell) {
+  static const float coef[6][3] = {{1.f, 0.f, 0.f},     {1.f, 0.f, 0.f},     {-0.5f, 1.5f, 0.f},
+                                   {-1.5f, 2.5f, 0.f},  {0.375f, -3.75f, 4.375f}, {1.875f, -8.75f, 7.875f}};
+  const int i = ell <= 5 ? ell : 0;
+  t.c0[p] = coef[i][0];
+  t.c1[p] = coef[i][1];
+  t.c2[p] = coef[i][2];
+  t.odd[p] = ell & 1;
+  t.ell[p] = ell;
+}
+
+// m2 = mu * mu
+__device__ __forceinline__ float legendre_eval(const LegendreTable& t, int p, float mu, float m2) {
+  if (t.ell[p] > 5) return legendre(t.ell[p], mu);
+  const float q = fmaf(fmaf(t.c2[p], m2, t.c1[p]), m2, t.c0[p]);
+  return t.odd[p] ? mu * q : q;
+}
+
 __host__ __device__ __forceinline__ float ylm_unit(int ell, int m, float x, float y, float z) {
   switch (ell) {
     case 0:

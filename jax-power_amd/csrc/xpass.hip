@@ -45,10 +45,12 @@ struct XPassArgs {
   // bin mode
   BinGeom g;
   int ell[XB_MAX_PROD], rz[XB_MAX_PROD];
+  LegendreTable leg;
   float los[3];
   int need_mu;
   double* partial;  // [gridDim.x][nprod][nbins][2]
 };
+static_assert(XB_MAX_PROD <= LEG_MAX, "Legendre table too short");
 
 // Bytes of LDS the kernel needs beyond its tile: the per-kx and per-kz tables (TAB) and, in
 // bin mode, the thresholds and the histogram.
@@ -257,10 +259,11 @@ xpass_kernel(XPassArgs a) {
               const float kn2 = fkx * fkx + f2yz;
               mu = kn2 > 0.f ? (fkx * a.los[0] + lyz) * rsqrtf(kn2) : 0.f;
             }
+            const float m2 = mu * mu;
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
               if (origin && a.rz[p]) continue;
-              const float f = w * (a.ell[p] ? legendre(a.ell[p], mu) : 1.f);
+              const float f = w * legendre_eval(a.leg, p, mu, m2);
               sum[p] += (double)(p2 * f);
             }
           }
@@ -679,6 +682,7 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
     MP_REQUIRE(ell[p] >= 0 && ell[p] <= 10, "bin_partial: ell %d not in 0..10", ell[p]);
     a.ell[p] = ell[p];
     a.rz[p] = remove_zero[p];
+    legendre_table_set(a.leg, p, ell[p]);
     if (ell[p]) a.need_mu = 1;
   }
   for (int q = 0; q < 3; ++q) a.los[q] = los ? (float)los[q] : 0.f;
