@@ -211,6 +211,25 @@ int mp_ylm_bin(void* stream, const float* rmesh, const void* A0, int64_t a0_pitc
                const double* ky, const double* kz, const double* k2_thresholds, int nedges, double* out, void* workspace,
                size_t workspace_bytes);
 
+/* ---- bispectrum, Scoccimarro basis, global line of sight ------------------------------ */
+/* The shell meshes of BinMesh3SpectrumPoles.__call__ (jaxpower/mesh3.py:300-305), with the
+ * Legendre factor of the third mesh of compute_mesh3_spectrum (mesh3.py:585-590):
+ * out + q N0 N1 N2 = irfftn(A(k) [bin(k) == b0 + q] P_ell(k.los/|k|)) for q < nshell, float32
+ * (N0, N1, N2) meshes; A == NULL gives the shell indicators whose triangle sums are the mode
+ * counts (mesh3.py:229-236).  Bins as in mp_bin_poles (k tables and squared-|k| thresholds). */
+size_t mp_bispec_shells_workspace_bytes(const int32_t meshsize[3]);
+int mp_bispec_shells(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double* kx, const double* ky,
+                     const double* kz, const double* k2_thresholds, int nedges, int b0, int nshell, const double los[3], int ell,
+                     float* out, void* workspace, size_t workspace_bytes);
+/* The closed-triangle reduction of BinMesh3SpectrumPoles.bin_reduce_meshs with
+ * reduce = prod(meshes).sum() (jaxpower/mesh3.py:238-275, 307-311): out[t] = sum over the ncells
+ * cells of s1[b1] s2[b2] s3[b3] for (b1, b2, b3) = tri[3t .. 3t+2] (a device int32 array sorted by
+ * (b1, b2)); shell q of a stack at stack + q * ncells; FP64 across cells. */
+size_t mp_bispec_triangles_workspace_bytes(int ntri);
+int mp_bispec_triangles(void* stream, const float* s1, const float* s2, const float* s3, int64_t ncells, int nshell1,
+                        int nshell2, int nshell3, const int32_t* tri, int ntri, double* out, void* workspace,
+                        size_t workspace_bytes);
+
 /* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
  *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */
 int64_t mp_reduce_workspace_bytes(void);

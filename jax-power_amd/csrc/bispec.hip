@@ -1,0 +1,247 @@
+// Bispectrum multipoles in the Scoccimarro basis with a global line of sight
+// (jaxpower/mesh3.py:180-313 BinMesh3SpectrumPoles, 509-629 compute_mesh3_spectrum):
+//   * shells: for each |k| shell b of a half spectrum A, the real mesh
+//       I_b(x) = irfftn(A(k) [bin(k) == b] P_ell(k.los/|k|))
+//     (mesh3.py:300-305; P_ell of the third mesh, mesh3.py:585-590; A absent: the shell
+//     indicator, whose triangle sums count the closed triangles, mesh3.py:229-236);
+//   * triangles: S(b1, b2, b3) = sum_x I1_b1(x) I2_b2(x) I3_b3(x) (mesh3.py:307-311), in FP32
+//     per cell and FP64 across cells.
+//
+// The shell meshes are made one at a time in a single complex scratch mesh (split, then an
+// in-place inverse FFT), so the workspace is one half spectrum whatever the number of shells.
+// The triangle pass is a persistent sweep over cell tiles: the block stages every shell of
+// its tile in LDS, and each wave owns a run of triangles (sorted by (b1, b2), so the product
+// I1_b1 I2_b2 of a lane's cells is reused along b3) with the lane's running sums in registers.
+#include "bin.h"
+#include "common.h"
+#include "fft.h"
+#include "sph.h"
+
+namespace mp {
+
+constexpr int SH_NT = 256;
+constexpr int TR_NT = 512;                // 8 waves: 2 per SIMD, up to 256 VGPRs each
+constexpr int TR_WAVES = TR_NT / 64;
+constexpr int TR_TILE = 128;              // cells per tile: 2 per lane
+constexpr int TR_PER_WAVE = 32;           // triangles per wave and launch
+constexpr int TR_FLUSH = 32;              // tiles summed in FP32 (64 cells a lane) before the FP64 add
+constexpr int TR_MAX_SHELLS = 96;         // LDS: (distinct stacks x shells) x TR_TILE floats
+constexpr int TR_MAX_BLOCKS = 1024;
+
+static_assert(TR_TILE == 2 * 64, "two cells per lane");
+static_assert(TR_PER_WAVE <= 64, "lane k gathers the total of the wave's triangle k");
+
+// out = A(k) [bin(k) == b] f(k) on the half spectrum (f = P_ell(mu) when ell_on, else 1).
+__global__ void __launch_bounds__(SH_NT)
+shell_split_kernel(const float2* __restrict__ A, int64_t pitch, int n0, int n1, int nc, const double* __restrict__ kx,
+                   const double* __restrict__ ky, const double* __restrict__ kz, const double* __restrict__ thr, int nedges,
+                   int b, float lx, float ly, float lz, LegendreTable leg, int ell_on, float2* __restrict__ out,
+                   int64_t out_pitch) {
+  const int64_t total = (int64_t)n0 * n1 * nc;
+  for (int64_t i = (int64_t)blockIdx.x * SH_NT + threadIdx.x; i < total; i += (int64_t)gridDim.x * SH_NT) {
+    const int iz = (int)(i % nc);
+    const int64_t row = i / nc;
+    const int iy = (int)(row % n1), ix = (int)(row / n1);
+    const double kxv = kx[ix], kyv = ky[iy], kzv = kz[iz];
+    const int nb = first_bin(thr, nedges, k2_sum(k2_xy(kxv, kyv), kzv)) - 1;
+    float2 v = make_float2(0.f, 0.f);
+    if (nb == b) {
+      v = A ? A[row * pitch + iz] : make_float2(1.f, 0.f);
+      if (ell_on) {
+        const float fx = (float)kxv, fy = (float)kyv, fz = (float)kzv;
+        const float kn2 = fx * fx + fy * fy + fz * fz;
+        const float mu = kn2 > 0.f ? (fx * lx + fy * ly + fz * lz) * rsqrtf(kn2) : 0.f;
+        const float f = legendre_eval(leg, 0, mu, mu * mu);
+        v = make_float2(v.x * f, v.y * f);
+      }
+    }
+    out[row * out_pitch + iz] = v;
+  }
+}
+
+struct TriArgs {
+  const float* stack[3];  // distinct shell stacks (up to three), shell q of stack s at stack[s] + q * shell_stride
+  int nshell[3];
+  int lds_base[3];        // first LDS row of each stack
+  int role[3];            // stack of each triangle side
+  int nstack;
+  int64_t shell_stride, ncells;
+  const int* tri;         // (b1, b2, b3) per triangle, sorted by (b1, b2)
+  int ntri;
+  double* partial;        // [gridDim.x][ntri]
+};
+
+__global__ void __launch_bounds__(TR_NT)
+triangle_sum_kernel(TriArgs a) {
+  extern __shared__ float srow[];  // row r (a shell of a stack) of the tile at srow[r * TR_TILE]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int t0 = (blockIdx.y * TR_WAVES + wave) * TR_PER_WAVE;  // this wave's first triangle
+  const int nrows = a.lds_base[a.nstack - 1] + a.nshell[a.nstack - 1];
+  const float* s1 = srow + a.lds_base[a.role[0]] * TR_TILE;
+  const float* s2 = srow + a.lds_base[a.role[1]] * TR_TILE;
+  const float* s3 = srow + a.lds_base[a.role[2]] * TR_TILE;
+  float acc[TR_PER_WAVE];  // this lane's FP32 sums, one per triangle of the wave
+#pragma unroll
+  for (int k = 0; k < TR_PER_WAVE; ++k) acc[k] = 0.f;
+  double own = 0.;  // FP64 total of triangle t0 + lane, gathered at every flush
+  auto flush = [&]() {
+#pragma unroll
+    for (int k = 0; k < TR_PER_WAVE; ++k) {
+      double v = (double)acc[k];
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+      if (lane == k) own += v;
+      acc[k] = 0.f;
+    }
+  };
+  const int64_t ntiles = ceil_div(a.ncells, TR_TILE);
+  int since_flush = 0;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t c0 = tile * TR_TILE;
+    for (int i = tid; i < nrows * TR_TILE; i += TR_NT) {
+      const int r = i / TR_TILE, c = i % TR_TILE;
+      int s = 0;
+      while (s + 1 < a.nstack && r >= a.lds_base[s + 1]) ++s;
+      const int q = r - a.lds_base[s];
+      srow[i] = c0 + c < a.ncells ? a.stack[s][q * a.shell_stride + c0 + c] : 0.f;
+    }
+    __syncthreads();
+    int pb1 = -1, pb2 = -1;
+    float p0 = 0.f, p1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < TR_PER_WAVE; ++k) {
+      const int t = t0 + k;
+      if (t < a.ntri) {
+        const int b1 = a.tri[3 * t], b2 = a.tri[3 * t + 1], b3 = a.tri[3 * t + 2];
+        const bool ok = (unsigned)b1 < (unsigned)a.nshell[a.role[0]] && (unsigned)b2 < (unsigned)a.nshell[a.role[1]] &&
+                        (unsigned)b3 < (unsigned)a.nshell[a.role[2]];  // a triangle with a shell out of range sums to 0
+        if (ok) {
+          if (b1 != pb1 || b2 != pb2) {
+            const float2 u = *reinterpret_cast<const float2*>(s1 + b1 * TR_TILE + 2 * lane);
+            const float2 v = *reinterpret_cast<const float2*>(s2 + b2 * TR_TILE + 2 * lane);
+            p0 = u.x * v.x;
+            p1 = u.y * v.y;
+            pb1 = b1;
+            pb2 = b2;
+          }
+          const float2 w = *reinterpret_cast<const float2*>(s3 + b3 * TR_TILE + 2 * lane);
+          acc[k] = fmaf(p1, w.y, fmaf(p0, w.x, acc[k]));
+        }
+      }
+    }
+    if (++since_flush == TR_FLUSH) {
+      flush();
+      since_flush = 0;
+    }
+    __syncthreads();
+  }
+  flush();
+  if (lane < TR_PER_WAVE && t0 + lane < a.ntri) a.partial[(int64_t)blockIdx.x * a.ntri + t0 + lane] = own;
+}
+
+static int multiprocessor_count(int* ncu) {
+  int dev = 0;
+  MP_CHECK_HIP(hipGetDevice(&dev));
+  MP_CHECK_HIP(hipDeviceGetAttribute(ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  return MP_OK;
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+MP_EXPORT size_t mp_bispec_shells_workspace_bytes(const int32_t meshsize[3]) {
+  if (!meshsize) return 0;
+  return align_up((size_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * sizeof(float2), 256) + 256;
+}
+
+// Real meshes of the shells b0 .. b0 + nshell - 1 (bins of the edges whose squared-|k|
+// thresholds are thr): out + q * N0 N1 N2 = irfftn(A [bin == b0 + q] P_ell(k.los/|k|)); A
+// null: the shell indicators.  los is used when ell > 0.
+MP_EXPORT int mp_bispec_shells(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double* kx, const double* ky,
+                               const double* kz, const double* thr, int nedges, int b0, int nshell, const double los[3], int ell,
+                               float* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && thr && out, "mp_bispec_shells: null pointer");
+  MP_REQUIRE(nedges >= 2 && b0 >= 0 && nshell >= 1 && b0 + nshell <= nedges - 1, "mp_bispec_shells: shells %d..%d outside the %d bins",
+             b0, b0 + nshell - 1, nedges - 1);
+  MP_REQUIRE(ell >= 0 && ell <= 10, "mp_bispec_shells: ell %d not in 0..10", ell);
+  MP_REQUIRE(ell == 0 || los, "mp_bispec_shells: ell > 0 needs a line of sight");
+  MP_REQUIRE(!A || pitch >= meshsize[2] / 2 + 1, "mp_bispec_shells: pitch too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_bispec_shells: workspace too small");
+  LegendreTable leg{};
+  legendre_table_set(leg, 0, ell);
+  const float lx = ell ? (float)los[0] : 0.f, ly = ell ? (float)los[1] : 0.f, lz = ell ? (float)los[2] : 0.f;
+  const int nc = meshsize[2] / 2 + 1;
+  const int64_t total = (int64_t)meshsize[0] * meshsize[1] * nc;
+  int ncu = 0;
+  int rc = multiprocessor_count(&ncu);
+  if (rc) return rc;
+  const int blocks = (int)std::min<int64_t>(ceil_div(total, SH_NT), (int64_t)ncu * 16);
+  const int64_t cells = (int64_t)meshsize[0] * meshsize[1] * meshsize[2];
+  for (int q = 0; q < nshell; ++q) {
+    MP_LAUNCH("bispec_shell_split", shell_split_kernel, dim3((unsigned)blocks), dim3(SH_NT), 0, s, (const float2*)A, pitch, meshsize[0],
+              meshsize[1], nc, kx, ky, kz, thr, nedges, b0 + q, lx, ly, lz, leg, ell > 0 ? 1 : 0, scratch, sp);
+    rc = irfft3d(s, scratch, sp, out + q * cells, meshsize, scratch, sp);  // in place: each pass rewrites the tiles it read
+    if (rc) return rc;
+  }
+  return MP_OK;
+}
+
+MP_EXPORT size_t mp_bispec_triangles_workspace_bytes(int ntri) {
+  return align_up((size_t)TR_MAX_BLOCKS * (ntri > 0 ? ntri : 1) * sizeof(double), 256) + 256;
+}
+
+// S[t] = sum over the ncells cells of s1[b1](x) s2[b2](x) s3[b3](x), (b1, b2, b3) = tri[3t ..
+// 3t + 2] (a device array, sorted by (b1, b2); a triangle naming a shell beyond its stack
+// sums to 0); shell q of stack s at s + q * ncells.  Stacks given by equal pointers share
+// their staging.
+MP_EXPORT int mp_bispec_triangles(void* stream, const float* s1, const float* s2, const float* s3, int64_t ncells, int nshell1,
+                                  int nshell2, int nshell3, const int32_t* tri, int ntri, double* out, void* workspace,
+                                  size_t workspace_bytes) {
+  MP_REQUIRE(s1 && s2 && s3 && tri && out, "mp_bispec_triangles: null pointer");
+  MP_REQUIRE(ncells >= 1 && ntri >= 1, "mp_bispec_triangles: empty input");
+  MP_REQUIRE(nshell1 >= 1 && nshell2 >= 1 && nshell3 >= 1, "mp_bispec_triangles: no shells");
+  hipStream_t s = (hipStream_t)stream;
+  TriArgs a{};
+  const float* ptrs[3] = {s1, s2, s3};
+  const int ns[3] = {nshell1, nshell2, nshell3};
+  int rows = 0;
+  for (int r = 0; r < 3; ++r) {
+    int found = -1;
+    for (int q = 0; q < a.nstack; ++q)
+      if (a.stack[q] == ptrs[r]) found = q;
+    if (found >= 0) {
+      MP_REQUIRE(a.nshell[found] == ns[r], "mp_bispec_triangles: one stack given with two shell counts");
+    } else {
+      found = a.nstack++;
+      a.stack[found] = ptrs[r];
+      a.nshell[found] = ns[r];
+      a.lds_base[found] = rows;
+      rows += ns[r];
+    }
+    a.role[r] = found;
+  }
+  if (rows > TR_MAX_SHELLS) return set_error(MP_ERR_UNSUPPORTED, "mp_bispec_triangles: %d shell rows > %d", rows, TR_MAX_SHELLS);
+  a.shell_stride = ncells;
+  a.ncells = ncells;
+  a.tri = tri;
+  a.ntri = ntri;
+  Workspace ws(workspace, workspace_bytes);
+  int ncu = 0;
+  int rc = multiprocessor_count(&ncu);
+  if (rc) return rc;
+  const int groups = (int)ceil_div(ntri, TR_WAVES * TR_PER_WAVE);
+  int bx = (int)std::min<int64_t>(ceil_div(ncells, TR_TILE), (int64_t)std::max(1, 2 * ncu / groups));
+  bx = std::min(bx, TR_MAX_BLOCKS);
+  a.partial = ws.take<double>((size_t)bx * ntri);
+  if (!a.partial) return set_error(MP_ERR_WORKSPACE, "mp_bispec_triangles: workspace too small");
+  const size_t lds = (size_t)rows * TR_TILE * sizeof(float);
+  if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)triangle_sum_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  MP_LAUNCH("bispec_triangles", triangle_sum_kernel, dim3((unsigned)bx, (unsigned)groups), dim3(TR_NT), lds, s, a);
+  return reduce_partials(s, a.partial, bx, ntri, out);
+}

@@ -63,6 +63,12 @@ SIGNATURES = {
     'mp_bin_real_workspace_bytes': (c_int64, [P_I32, c_int, c_int]),
     'mp_bin_real_poles': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_int, P_I32,
                                   P_I32, c_void_p, c_void_p, c_size_t]),
+    'mp_bispec_shells_workspace_bytes': (c_size_t, [P_I32]),
+    'mp_bispec_shells': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, P_F64,
+                                 c_int, c_void_p, c_void_p, c_size_t]),
+    'mp_bispec_triangles_workspace_bytes': (c_size_t, [c_int]),
+    'mp_bispec_triangles': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_int, c_void_p,
+                                    c_void_p, c_size_t]),
     'mp_irfft3d_bin_real_workspace_bytes': (c_int64, [P_I32, c_int, c_int]),
     'mp_irfft3d_bin_real': (c_int, [c_void_p, c_void_p, c_int64, c_int, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64,
                                     c_int, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),

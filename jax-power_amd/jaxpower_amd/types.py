@@ -183,6 +183,39 @@ class Mesh2CorrelationPole(_MeshPole):
     _x = 's'
 
 
+class Mesh3SpectrumPole(_MeshPole):
+    """Bispectrum multipole (mesh3.py:625-627): k (ntri, 3) shell-averaged |k| of the three
+    sides, k_edges (ntri, 3, 2), nmodes (closed-triangle counts, float), num_raw,
+    num_shotnoise, norm, volume, ell, and the binning ``basis``."""
+
+    _x = 'k'
+
+    def __init__(self, ell=0, attrs=None, basis='scoccimarro', **fields):
+        super().__init__(ell=ell, attrs=attrs, **fields)
+        self.basis = basis
+
+    def clone(self, **kwargs):
+        kwargs.setdefault('basis', self.basis)
+        return super().clone(**kwargs)
+
+    def select(self, **kwargs):
+        """Keep triangles whose three sides all satisfy kmin <= k < kmax (k=(kmin, kmax)),
+        or a slice of triangles."""
+        sel = kwargs.pop('k', None)
+        if kwargs:
+            raise TypeError(f'select: unexpected arguments {sorted(kwargs)}')
+        if sel is None:
+            return self
+        if isinstance(sel, slice):
+            mask = np.zeros(len(self._data['k']), dtype=bool)
+            mask[sel] = True
+        else:
+            kk = self._data['k']
+            mask = np.all((kk >= sel[0]) & (kk < sel[1]), axis=-1)
+        data = {name: v[mask] for name, v in self._data.items()}
+        return type(self)(**data, ell=self.ell, attrs=self.attrs, basis=self.basis)
+
+
 class _MeshPoles:
 
     _pole = Mesh2SpectrumPole
@@ -236,9 +269,9 @@ class _MeshPoles:
     def select(self, **kwargs):
         return type(self)([p.select(**kwargs) for p in self._branches], attrs=self.attrs)
 
-    def write(self, fn):
+    def write(self, fn, extra=None):
         """npz with one group of arrays per ell."""
-        arrays = {}
+        arrays = dict(extra or {})
         for p in self._branches:
             for name, v in p._data.items():
                 arrays[f'ell{p.ell}_{name}'] = v
@@ -261,16 +294,25 @@ class Mesh2CorrelationPoles(_MeshPoles):
     _pole = Mesh2CorrelationPole
 
 
+class Mesh3SpectrumPoles(_MeshPoles):
+    _pole = Mesh3SpectrumPole
+
+    def write(self, fn):
+        super().write(fn, extra={'basis': np.array(self._branches[0].basis if self._branches else 'scoccimarro')})
+
+
 def read(fn):
-    """Read a Mesh2SpectrumPoles or Mesh2CorrelationPoles written by ``write``."""
+    """Read a Mesh2SpectrumPoles, Mesh2CorrelationPoles or Mesh3SpectrumPoles written by ``write``."""
     with np.load(fn, allow_pickle=False) as data:
         kind = str(data['kind']) if 'kind' in data.files else 'Mesh2SpectrumPoles'
-        cls = {'Mesh2SpectrumPoles': Mesh2SpectrumPoles, 'Mesh2CorrelationPoles': Mesh2CorrelationPoles}[kind]
+        cls = {'Mesh2SpectrumPoles': Mesh2SpectrumPoles, 'Mesh2CorrelationPoles': Mesh2CorrelationPoles,
+               'Mesh3SpectrumPoles': Mesh3SpectrumPoles}[kind]
         attrs = {'los': data['los'].tolist()} if 'los' in data.files else {}
+        extra = {'basis': str(data['basis'])} if 'basis' in data.files else {}
         poles = []
         for ell in data['ells']:
             kw = {name: data[f'ell{ell}_{name}'] for name in cls._pole._field_names()}
-            poles.append(cls._pole(**kw, ell=int(ell), attrs=attrs))
+            poles.append(cls._pole(**kw, ell=int(ell), attrs=attrs, **extra))
     return cls(poles, attrs=attrs)
 
 

@@ -696,3 +696,131 @@ def local_correlation_bruteforce(attrs, rmesh, ell):
         w = np.where(xn > 0, (2 * ell + 1) / (4 * np.pi) * legendre(ell, mu), 0.)
         out[idx] = np.sum(rmesh * w * np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2)))
     return out
+
+
+# ----------------------------------------------------------------------------
+# Bispectrum, Scoccimarro basis, global line of sight (mesh3.py:23-177, 180-313, 509-629)
+# ----------------------------------------------------------------------------
+
+class BinBispectrumPoles:
+    """BinMesh3SpectrumPoles with basis 'scoccimarro' or 'scoccimarro-diagonal'
+    (mesh3.py:23-177, 180-313).  Side i bins its modes as BinPoles does; triangles are the
+    lexicographic (b1, b2, b3) combinations (mesh3.py:107-124) that pass the mask, by default
+    mid1 <= mid2 <= mid3, |mid1 - mid2| <= mid3 <= mid1 + mid2 and the sum of the upper edges
+    <= 2 k_nyq (mesh3.py:31-35); mask_edges='' keeps every combination.  nmodes are the
+    closed-triangle counts N^6 sum_x prod_i c2r([bin_i == b_i]) (mesh3.py:229-236)."""
+
+    def __init__(self, attrs, edges=None, ells=0, mask_edges=None, diagonal=False):
+        ndim = 3
+        if not isinstance(edges, (list, tuple)):
+            edges = [edges] * ndim
+        self.attrs = attrs
+        self.axes = [BinPoles(attrs, edges=e) for e in edges[:ndim]]
+        self.axes += [self.axes[-1]] * (ndim - len(self.axes))
+        self.ibin1d = [ax.ibin - 1 for ax in self.axes]  # mesh3.py:92-93
+        self.edges1d = [ax.edges for ax in self.axes]
+        self.xavg1d = [ax.xavg for ax in self.axes]
+        self.nmodes1d = [ax.nmodes for ax in self.axes]
+        self.ells = [int(ell) for ell in np.atleast_1d(ells)]
+
+        def prod(arrays):
+            if diagonal:
+                return np.column_stack([np.asarray(arrays[0])] * ndim)
+            grid = np.meshgrid(*arrays, indexing='ij')
+            return np.column_stack([g.ravel() for g in grid])
+
+        lo = prod([e[:, 0] for e in self.edges1d])
+        hi = prod([e[:, 1] for e in self.edges1d])
+        edges3 = np.stack([lo, hi], axis=-1)
+        mid = edges3.mean(axis=-1)
+        nyq = attrs.kfun.min() * np.min(attrs.meshsize) / 2.
+        if mask_edges is None:
+            m1, m2, m3 = mid.T
+            mask = (m1 <= m2) & (m2 <= m3) & (m3 >= np.abs(m1 - m2)) & (m3 <= np.abs(m1 + m2))
+            mask &= edges3[:, :, 1].sum(axis=1) <= 2 * nyq
+        else:
+            assert mask_edges == '', 'the oracle takes the default mask or none'
+            mask = np.ones(len(edges3), dtype=bool)
+        self.edges = edges3[mask]
+        self.xavg = prod(self.xavg1d)[mask]
+        self.iedges = prod([np.arange(len(x)) for x in self.xavg1d])[mask]
+        ones = np.ones(self.ibin1d[0].shape, dtype='f8')
+        self.nmodes = self.reduce(ones, ones, ones) * float(np.prod(attrs.meshsize))**2
+
+    def shells(self, axis, value):
+        """c2r(value [bin == b]) for every shell b of side axis (mesh3.py:300-305)."""
+        return [c2r(value * (self.ibin1d[axis] == b), self.attrs.meshsize) for b in range(len(self.xavg1d[axis]))]
+
+    def reduce(self, A1, A2, A3):
+        """sum_x c2r(A1 [b1]) c2r(A2 [b2]) c2r(A3 [b3]) per triangle (mesh3.py:238-275, 307-311)."""
+        s = [self.shells(i, A) for i, A in enumerate((A1, A2, A3))]
+        return np.array([np.sum(s[0][b1] * s[1][b2] * s[2][b3]) for b1, b2, b3 in self.iedges])
+
+
+def bispectrum_poles(attrs, bin, mesh1, mesh2=None, mesh3=None, los='x'):
+    """compute_mesh3_spectrum, Scoccimarro basis, global line of sight (mesh3.py:509-629):
+    num_raw = (2 ell + 1) N^6 reduce(A1, A2, A3 P_ell(mu)) / nmodes (real part; odd ell keep the
+    imaginary part of a real sum, i.e. 0); norm = N^3 / Vcell^2.  Missing meshes repeat the last
+    one given (mesh.py:3138-3156)."""
+    meshes = [mesh1, mesh2, mesh3]
+    for i in (1, 2):
+        if meshes[i] is None:
+            meshes[i] = meshes[i - 1]
+    A = [m if np.iscomplexobj(m) else r2c(m) for m in meshes]
+    vlos = los_vector(los)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    kn = np.sqrt(kx**2 + ky**2 + kz**2)
+    kn[0, 0, 0] = 1.
+    mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / kn
+    n6 = float(np.prod(attrs.meshsize))**2
+    num = []
+    for ell in bin.ells:
+        if ell % 2:
+            num.append(np.zeros(len(bin.iedges)))
+            continue
+        num.append((2 * ell + 1) * bin.reduce(A[0], A[1], A[2] * legendre(ell, mu)) * n6 / bin.nmodes)
+    norm = np.prod(attrs.meshsize) / np.prod(attrs.cellsize)**2
+    return dict(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num, norm=norm, ells=bin.ells, los=vlos)
+
+
+def _full_grid_bins(attrs, edges):
+    """Shell index (digitize - 1) of every mode of the full FFT grid, and the integer mode
+    indices (fftfreq order) per axis."""
+    n = [int(x) for x in attrs.meshsize]
+    kk = [np.fft.fftfreq(n[a]) * attrs.kfun[a] * n[a] for a in range(3)]
+    k2 = kk[0][:, None, None]**2 + kk[1][None, :, None]**2
+    k2 = k2 + kk[2][None, None, :]**2
+    return np.digitize(np.sqrt(k2), edges, right=False) - 1, kk
+
+
+def bispectrum_bruteforce(attrs, bin, rmeshes, ell=0, los='z'):
+    """Closed-triangle sums over pairs of full-grid modes: for each triangle (b1, b2, b3),
+    sum of F1(k1) F2(k2) F3(k3) P_ell(mu3) over k1 in shell b1, k2 in shell b2 and
+    k3 = -(k1 + k2) (mod N) in shell b3, with F = fftn of the real meshes (counts with F = 1
+    and ell = 0).  This is what N^6 sum_x prod c2r(...) must equal; small meshes only."""
+    n = np.array([int(x) for x in attrs.meshsize])
+    edges = np.append(bin.edges1d[0][:, 0], bin.edges1d[0][-1, 1])
+    assert all(np.array_equal(e, bin.edges1d[0]) for e in bin.edges1d), 'one set of edges for all sides'
+    ib, kk = _full_grid_bins(attrs, edges)
+    F = [np.fft.fftn(m) if m is not None else np.ones(tuple(n)) for m in rmeshes]
+    vlos = los_vector(los)
+    idx = np.stack(np.meshgrid(*[np.arange(x) for x in n], indexing='ij'), axis=-1).reshape(-1, 3)
+    ibf = ib.ravel()
+    valid = ibf >= 0
+    out = np.zeros(len(bin.iedges), dtype=complex)
+    key = {tuple(t): i for i, t in enumerate(bin.iedges)}
+    i1 = np.flatnonzero(valid)
+    for a in i1:
+        k1 = idx[a]
+        k3 = (-(k1[None, :] + idx[i1])) % n
+        b3 = ib[k3[:, 0], k3[:, 1], k3[:, 2]]
+        kv = np.stack([kk[0][k3[:, 0]], kk[1][k3[:, 1]], kk[2][k3[:, 2]]], axis=-1)
+        kn = np.linalg.norm(kv, axis=-1)
+        mu = np.where(kn > 0, kv @ vlos / np.where(kn > 0, kn, 1.), 0.)
+        f = F[0].ravel()[a] * F[1].ravel()[i1] * F[2][k3[:, 0], k3[:, 1], k3[:, 2]] * legendre(ell, mu)
+        for j, (bb2, bb3) in enumerate(zip(ibf[i1], b3)):
+            t = key.get((ibf[a], bb2, bb3))
+            if t is not None:
+                out[t] += f[j]
+    return out

@@ -1,0 +1,148 @@
+"""GPU parity of the bispectrum multipoles in the Scoccimarro basis (BinMesh3SpectrumPoles,
+compute_mesh3_spectrum; jaxpower/mesh3.py:23-313, 509-629) against the float64 oracle, itself
+pinned to brute-force closed-triangle sums in test_oracle.py.
+
+Tolerances: edges and shell-averaged |k| exact to 1e-12.  Triangle counts and multipoles come
+from float32 meshes, float32 inverse FFTs and float32 products per cell (FP64 across cells);
+each triangle sum is compared as |d| <= 1e-5 |want| + ftol max|want| per ell, ftol covering
+the float32 rounding of the cell products, which is relative to the sum of |products| rather
+than to their (partly cancelling) total.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import power_oracle as po
+
+from test_gpu_parity import DEV, host, say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_close(got, want, rtol=1e-5, ftol=2e-6, label=''):
+    got, want = np.asarray(got, dtype='f8'), np.asarray(want, dtype='f8')
+    nan = np.isnan(want)
+    assert np.array_equal(nan, np.isnan(got)), f'{label}: NaN pattern differs'
+    g, w = got[~nan], want[~nan]
+    floor = ftol * np.abs(w).max()
+    err = np.abs(g - w)
+    assert np.all(err <= rtol * np.abs(w) + floor), f'{label}: max err {err.max():.3e}, floor {floor:.3e}, max {np.abs(w).max():.3e}'
+
+
+def _mock(attrs, size, seed):
+    rng = np.random.default_rng(seed)
+    lo = attrs.boxcenter - attrs.boxsize / 2.
+    pos = rng.uniform(0., 1., (size, 3)) * attrs.boxsize + lo
+    n = 2 * size // 3
+    centers = rng.uniform(0., 1., (6, 3)) * attrs.boxsize + lo
+    pos[:n] = centers[rng.integers(0, 6, n)] + rng.normal(0., 0.05, (n, 3)) * attrs.boxsize
+    pos[:n] = lo + np.mod(pos[:n] - lo, attrs.boxsize)
+    w = rng.uniform(0.5, 1.5, size).astype('f4')
+    return pos, w
+
+
+@pytest.mark.parametrize('meshsize,boxsize,edges,basis', [
+    ((32, 32, 32), 320., {'step': 0.06}, 'scoccimarro'),
+    ((32, 16, 64), (320., 160., 640.), {'step': 0.05}, 'scoccimarro'),
+    ((16, 16, 16), 160., {'step': 0.04}, 'scoccimarro-diagonal'),
+])
+def test_bispec_bin_setup_matches_oracle(meshsize, boxsize, edges, basis):
+    from jaxpower_amd import MeshAttrs, BinMesh3SpectrumPoles
+    say('bispec bin', meshsize, basis)
+    attrs = MeshAttrs(meshsize=meshsize, boxsize=boxsize, boxcenter=0.)
+    bin = BinMesh3SpectrumPoles(attrs, edges=edges, ells=(0, 2), basis=basis)
+    obin = po.BinBispectrumPoles(po.Attrs(meshsize, boxsize, 0.), edges=edges, ells=(0, 2), diagonal='diagonal' in basis)
+    assert np.array_equal(bin._iedges, obin.iedges)
+    assert np.array_equal(bin.edges, obin.edges)
+    assert np.allclose(bin.xavg, obin.xavg, rtol=1e-12, atol=0.)
+    for nm in bin.nmodes:
+        assert_close(nm, obin.nmodes, rtol=1e-5, ftol=1e-6, label='nmodes')
+
+
+@pytest.mark.parametrize('resampler,interlacing,los,ells', [('tsc', 2, 'z', (0, 2)), ('cic', 0, 'x', (0, 1, 4)),
+                                                             ('pcs', 3, (0.6, 0., 0.8), (0, 2))])
+def test_bispectrum_matches_oracle(resampler, interlacing, los, ells):
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_mesh3_spectrum
+    say('bispec', resampler, interlacing, los, ells)
+    attrs = MeshAttrs(meshsize=32, boxsize=400., boxcenter=(0., 50., 100.))
+    pos, w = _mock(attrs, 30000, 5)
+    mesh = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler=resampler, interlacing=interlacing, compensate=True,
+                                                                   out='complex')
+    bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.05}, ells=ells, basis='scoccimarro')
+    spec = compute_mesh3_spectrum(mesh, bin=bin, los=los)
+    oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter)
+    omesh = po.paint(oattrs, pos, w.astype('f8'), resampler=resampler, interlacing=interlacing, compensate=True, out='complex')
+    obin = po.BinBispectrumPoles(oattrs, edges={'step': 0.05}, ells=ells)
+    want = po.bispectrum_poles(oattrs, obin, omesh, los=los)
+    for i, ell in enumerate(ells):
+        got = spec.get(ell).values('num_raw')
+        if ell % 2:
+            assert np.all(got == 0.)
+            continue
+        assert_close(got, want['num_raw'][i], label=f'{resampler} {los} ell {ell}')
+    assert np.allclose(spec.get(ells[0]).values('norm'), want['norm'])
+    assert spec.get(0).coords('k').shape == (len(obin.iedges), 3)
+    assert spec.get(0).edges('k').shape == (len(obin.iedges), 3, 2)
+    assert spec.get(0).basis == 'scoccimarro'
+
+
+def test_bispectrum_cross_and_real_input():
+    """Three different meshes (real input converted by r2c on the GPU), non-cubic mesh."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_mesh3_spectrum
+    say('bispec cross')
+    attrs = MeshAttrs(meshsize=(32, 16, 64), boxsize=(320., 160., 640.), boxcenter=0.)
+    fields = [_mock(attrs, 20000, s) for s in (11, 12, 13)]
+    meshes = [ParticleField(to_dev(p), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=0, out='real') for p, w in fields]
+    bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.05}, ells=(0, 2), basis='scoccimarro')
+    spec = compute_mesh3_spectrum(*meshes, bin=bin, los='y')
+    oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter)
+    om = [po.paint(oattrs, p, w.astype('f8'), resampler='cic') for p, w in fields]
+    want = po.bispectrum_poles(oattrs, po.BinBispectrumPoles(oattrs, edges={'step': 0.05}, ells=(0, 2)), *om, los='y')
+    for i, ell in enumerate((0, 2)):
+        assert_close(spec.get(ell).values('num_raw'), want['num_raw'][i], label=f'cross ell {ell}')
+
+
+def test_bispec_bin_call_remove_zero():
+    from jaxpower_amd import MeshAttrs, BinMesh3SpectrumPoles
+    from jaxpower_amd.mesh import _r2c
+    say('bispec bin call')
+    attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
+    rng = np.random.default_rng(3)
+    x = (rng.normal(size=(16, 16, 16)) + 0.5 * rng.normal(size=(16, 16, 16))**2).astype('f4')
+    A = _r2c(to_dev(x), attrs.meshsize)
+    bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.06}, basis='scoccimarro')
+    oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, 0.)
+    obin = po.BinBispectrumPoles(oattrs, edges={'step': 0.06})
+    oA = po.r2c(x.astype('f8'))
+    for rz in (False, True):
+        got = bin(A, A, A, remove_zero=rz)
+        a = oA.copy()
+        if rz:
+            a[0, 0, 0] = 0.
+        want = obin.reduce(a, a, a) * 16.**6
+        assert_close(got, want, label=f'remove_zero={rz}')
+
+
+def test_bispectrum_write_read_and_box_normalization(tmp_path):
+    from jaxpower_amd import (MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_mesh3_spectrum, compute_box3_normalization,
+                              read_observable)
+    say('bispec io')
+    attrs = MeshAttrs(meshsize=16, boxsize=200., boxcenter=0.)
+    pos, w = _mock(attrs, 5000, 21)
+    p = ParticleField(to_dev(pos), to_dev(w), attrs=attrs)
+    bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.06}, ells=(0, 2), basis='scoccimarro')
+    spec = compute_mesh3_spectrum(p.paint(resampler='tsc', interlacing=2, compensate=True, out='complex'), bin=bin, los='z')
+    norm = compute_box3_normalization(p, bin=bin)
+    ow = w.astype('f8')
+    size = 16**3
+    want = (ow.sum() / size)**3 * size * ((200. / 16)**3)**(-2)  # mesh.py:3121-3129 with three inputs
+    assert len(norm) == 2 and np.allclose([float(n) for n in norm], want, rtol=1e-6)
+    spec = spec.clone(norm=[float(n) for n in norm])
+    fn = tmp_path / 'b.npz'
+    spec.write(fn)
+    back = read_observable(fn)
+    assert type(back).__name__ == 'Mesh3SpectrumPoles' and back.get(2).basis == 'scoccimarro'
+    for ell in (0, 2):
+        assert np.array_equal(back.get(ell).values('num_raw'), spec.get(ell).values('num_raw'))
+        assert np.array_equal(back.get(ell).coords('k'), spec.get(ell).coords('k'))
+        assert np.allclose(back.get(ell).value(), spec.get(ell).values('num_raw') / want)

@@ -453,3 +453,59 @@ def test_correlation_local_los_bruteforce():
     for ill, ell in enumerate(ells[1:], start=1):
         want = 4. * np.pi * bin(po.local_correlation_bruteforce(attrs, mesh, ell), remove_zero=True) / vcell
         assert np.allclose(got['num_raw'][ill], want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+
+
+# ---------------------------------------------------------------- bispectrum (Scoccimarro basis)
+
+@pytest.mark.parametrize('meshsize,boxsize,step', [((8, 8, 8), 80., 0.06), ((8, 6, 8), (80., 60., 80.), 0.05)])
+def test_bispectrum_counts_bruteforce(meshsize, boxsize, step):
+    """nmodes = N^6 sum_x prod c2r(shell indicators) equals the count of closed full-grid
+    triangles k1 + k2 + k3 = 0 (mod N) with k_i in shell b_i (mesh3.py:229-236)."""
+    attrs = po.Attrs(meshsize, boxsize, 0.)
+    bin = po.BinBispectrumPoles(attrs, edges={'step': step}, ells=0)
+    counts = po.bispectrum_bruteforce(attrs, bin, [None, None, None])
+    assert len(bin.iedges) > 3
+    assert np.allclose(bin.nmodes, counts.real, rtol=1e-12, atol=1e-9)
+    assert np.sum(bin.nmodes > 0.5) >= 3
+
+
+def test_bispectrum_mask_and_triangle_order():
+    """Default mask (mesh3.py:31-35): sorted sides, triangle inequality on the bin centres and
+    the sum of upper edges within 2 k_nyq; triangles in lexicographic order; mask_edges=''
+    keeps every combination; the diagonal basis keeps (b, b, b)."""
+    attrs = po.Attrs(16, 160., 0.)
+    bin = po.BinBispectrumPoles(attrs, edges={'step': 0.05})
+    mid = bin.edges.mean(axis=-1)
+    nyq = attrs.kfun.min() * 8
+    assert np.all(mid[:, 0] <= mid[:, 1]) and np.all(mid[:, 1] <= mid[:, 2])
+    assert np.all(mid[:, 2] <= mid[:, 0] + mid[:, 1]) and np.all(mid[:, 2] >= np.abs(mid[:, 0] - mid[:, 1]))
+    assert np.all(bin.edges[:, :, 1].sum(axis=1) <= 2 * nyq)
+    assert [tuple(t) for t in bin.iedges] == sorted(tuple(t) for t in bin.iedges)
+    nb = len(bin.xavg1d[0])
+    full = po.BinBispectrumPoles(attrs, edges={'step': 0.05}, mask_edges='')
+    assert len(full.iedges) == nb**3 and len(bin.iedges) < nb**3
+    diag = po.BinBispectrumPoles(attrs, edges={'step': 0.05}, mask_edges='', diagonal=True)
+    assert np.array_equal(diag.iedges, np.repeat(np.arange(nb)[:, None], 3, axis=1))
+
+
+@pytest.mark.parametrize('los,cross', [('z', False), ((0.6, 0., 0.8), False), ('x', True)])
+def test_bispectrum_poles_bruteforce(los, cross):
+    """compute_mesh3_spectrum's FFT route equals the explicit sum over closed triangles of
+    F1 F2 F3 P_ell(mu3), divided by the triangle counts, for ell = 0, 2, 4 (odd ell: 0)."""
+    attrs = po.Attrs((8, 8, 8), 80., 0.)
+    rng = np.random.default_rng(4)
+    meshes = [rng.normal(size=(8, 8, 8)) + 0.3 * rng.normal(size=(8, 8, 8))**2 for _ in range(3)]
+    if not cross:
+        meshes = [meshes[0]] * 3
+    bin = po.BinBispectrumPoles(attrs, edges={'step': 0.07}, ells=(0, 1, 2, 4))
+    res = po.bispectrum_poles(attrs, bin, *meshes, los=los)
+    counts = po.bispectrum_bruteforce(attrs, bin, [None, None, None]).real
+    for i, ell in enumerate(bin.ells):
+        if ell % 2:
+            assert np.all(res['num_raw'][i] == 0.)
+            continue
+        b = po.bispectrum_bruteforce(attrs, bin, meshes, ell=ell, los=los)
+        want = (2 * ell + 1) * b.real / counts
+        scale = np.abs(want).max()
+        assert np.allclose(res['num_raw'][i], want, rtol=1e-9, atol=1e-10 * scale), ell
+    assert np.isclose(res['norm'], 512 / 1000.**2)
