@@ -44,9 +44,10 @@ def parse():
                    help='write the HBM traffic file from two rocprofv3 --pmc output folders and exit')
     p.add_argument('--summarize-trace', metavar='TRACE_DIR',
                    help='write the kernel statistics summary from a rocprofv3 --kernel-trace --stats folder and exit')
-    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024', 'batch256', 'corr512'],
+    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024', 'batch256', 'corr512', 'window256'],
                    help='box512: the headline plain box; fkp1024: survey FKP field, 1024^3, local line of sight; '
-                        'batch256: 64 lognormal mocks at 256^3 with uneven counts, batched (HIP graph) vs per-mock calls')
+                        'batch256: 64 lognormal mocks at 256^3 with uneven counts, batched (HIP graph) vs per-mock calls; '
+                        'corr512: correlation multipoles of the 512^3 mock; window256: window matrix of a survey selection at 256^3')
     p.add_argument('--paint-path', type=int, default=None,
                    help='0 automatic, 1 tiled paint + z pass, 2 fused paint + z-FFT, 3 atomics + z pass')
     return p.parse_args()
@@ -329,6 +330,83 @@ def run_corr512(args):
     print(json.dumps(result))
 
 
+def window_bytes(n):
+    """Compulsory HBM bytes per launch of the window-matrix kernels at n^3 (one input shell)."""
+    H = n * n * (n // 2 + 1)
+    return {'window_shell': H * 8, 'window_xpass_inv': 2 * H * 8, 'window_ypass_inv': 2 * H * 8,
+            'window_zpass': 2 * H * 8 + n**3 * 4, 'window_ypass': 2 * H * 8, 'window_xpass': 2 * H * 8, 'bin_poles': H * 8}
+
+
+def run_window256(args):
+    """Window matrix of a survey selection (compute_mesh2_spectrum_window, global line of sight
+    'x', theory on the mesh): 256^3 mesh, boxsize 2000, selection = CIC paint of 1e6 points of
+    a Gaussian blob squashed along y and z (the selection of the reference's
+    tests/test_mesh2.py:577-607), input shells linspace(0.01, 0.1, 20) for ellin = 0, 2, 4,
+    output ells 0, 2, 4 in k-bins of 0.005.  One step = the whole matrix (57 columns).  Prints
+    one JSON line."""
+    import numpy as np
+    import torch
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window, _lib
+    torch.cuda.set_device(0)
+    n, boxsize, size = 256, 2000., 1_000_000
+    attrs = MeshAttrs(meshsize=n, boxsize=boxsize, boxcenter=0.)
+    rng = np.random.default_rng(args.seed)
+    pos = np.array([1., 0.2, 0.2]) * 0.2 * rng.normal(size=(size, 3))
+    w = np.all((pos > -0.2) & (pos < 0.2), axis=-1).astype('f4')
+    pos = (pos * boxsize).astype('f4')
+    sel = ParticleField(torch.from_numpy(pos).cuda(), torch.from_numpy(w).cuda(), attrs=attrs).paint(
+        resampler='cic', interlacing=1, compensate=False)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': STEP}, ells=ELLS)
+    edgesin = np.linspace(0.01, 0.1, 20)
+    ellsin = (0, 2, 4)
+    torch.cuda.synchronize()
+    print(f'# window setup done: {bin.nbins} output bins, {3 * (len(edgesin) - 1)} columns', file=sys.stderr, flush=True)
+
+    def step():
+        return compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los='x')
+
+    for _ in range(args.warmup):
+        wm = step()
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    lib.mp_timing_reset()
+    lib.mp_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        wm = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.mp_timing_enable(0)
+    buf = _lib.ctypes.create_string_buffer(1 << 16)
+    lib.mp_timing_report(buf, len(buf))
+    alg = window_bytes(n)
+    kernels = {}
+    for line in buf.value.decode().splitlines():
+        name, total, count = line.split()
+        avg = float(total) / int(count)
+        kernels[name] = {'avg_ms': round(avg, 5), 'launches_per_step': int(count) / args.steps,
+                         'share': round(float(total) / (elapsed * 1e3), 4)}
+        if name in alg:
+            kernels[name]['achieved_GBs'] = round(alg[name] / (avg * 1e-3) / 1e9, 1)
+    top = max((k for k in kernels if k in alg), key=lambda k: kernels[k]['share'])
+    ncol = wm.value().shape[1]
+    value = wm.value()
+    result = {'metric': 'window-matrix columns/s at 256^3 (global line of sight)', 'value': round(ncol * args.steps / elapsed, 2),
+              'unit': 'columns/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+              'ms_per_step': round(elapsed * 1e3 / args.steps, 3), 'higher_is_better': True, 'scaling': 'weak',
+              'vs_baseline': None, 'dtype': 'f32 (f64 bin sums)',
+              'data': 'synthetic: CIC paint of 1e6 Gaussian-blob points (seeded)',
+              'config': {'workload': 'window matrix, survey selection, global line of sight x', 'meshsize': n, 'boxsize': boxsize,
+                         'edgesin': 'linspace(0.01, 0.1, 20)', 'ellsin': list(ellsin), 'ells': list(ELLS), 'kstep': STEP,
+                         'columns': ncol, 'rows': value.shape[0]},
+              'roofline': {'kernel': top, 'bound': 'hbm', 'achieved': kernels[top]['achieved_GBs'], 'peak': HBM_PEAK_GBS,
+                           'unit': 'GB/s', 'frac': round(kernels[top]['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': None,
+                           'algorithmic_bytes_per_launch': alg[top], 'avg_ms': kernels[top]['avg_ms']},
+              'kernels': kernels,
+              'check': {'finite': bool(np.all(np.isfinite(value[:, :]))), 'w00_diag_max': float(np.abs(value).max())}}
+    print(json.dumps(result))
+
+
 def run_batch256(args):
     """64 lognormal mocks (the 512^3 recipe at 256^3, boxsize 1000) with particle counts drawn
     uniformly in [5e5, 4e6], CIC and TSC, 2-fold interlacing, compensated, DC removed, ells
@@ -391,6 +469,12 @@ def main():
             print('bench.py runs on one GPU only.', file=sys.stderr)
             sys.exit(2)
         run_corr512(args)
+        return
+    if args.workload == 'window256':
+        if args.gpus != 1:
+            print('bench.py runs on one GPU only.', file=sys.stderr)
+            sys.exit(2)
+        run_window256(args)
         return
     if args.workload == 'fkp1024':
         if args.gpus != 1:

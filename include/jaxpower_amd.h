@@ -230,6 +230,16 @@ int mp_bispec_triangles(void* stream, const float* s1, const float* s2, const fl
                         int nshell2, int nshell3, const int32_t* tri, int ntri, double* out, void* workspace,
                         size_t workspace_bytes);
 
+/* ---- window matrix, global line of sight ----------------------------------------------- */
+/* One input column of compute_mesh2_spectrum_window without flags (jaxpower/mesh2.py:1097-1099,
+ * selection Q of mesh2.py:1022-1029): the half spectrum out = A, or r2c(Q irfftn(A)) when Q is
+ * not NULL, with A = [k2_lo <= k^2 < k2_hi] P_ellin(k.los/|k|) scale (mu = 0 at k = 0), k^2 as in
+ * mp_bin_poles (thresholds from the input edges e0, e1 the same way).  Q: real (N0, N1, N2)
+ * float32 mesh; the output multipoles are binned with mp_bin_poles (prod_i = -1). */
+int mp_window_shell(void* stream, const float* Q, const int32_t meshsize[3], const double* kx, const double* ky,
+                    const double* kz, double k2_lo, double k2_hi, int ellin, const double los[3], float scale, void* out,
+                    int64_t out_pitch);
+
 /* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
  *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */
 int64_t mp_reduce_workspace_bytes(void);

@@ -83,6 +83,9 @@ int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n
               const KPrologue& kp, int first);
 int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const int32_t n[3], float2* scratch,
             int64_t scratch_pitch);
+// In place on a half spectrum: data <- r2c(Q * scale * irfftn(data)), Q a real (N0, N1, N2)
+// mesh; the real mesh is formed and multiplied inside the z pass only.
+int convolve_real(hipStream_t s, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale);
 struct RealBinArgs;
 size_t irfft3d_bin_real_workspace_bytes(const int32_t n[3], int nell, int nedges);
 int irfft3d_bin_real(hipStream_t s, const float2* in, int64_t in_pitch, const int32_t n[3], const RealBinArgs& a, double* out,

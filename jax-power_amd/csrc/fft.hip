@@ -208,6 +208,62 @@ zpass_c2r_kernel(const float2* __restrict__ in, int64_t in_pitch, float* __restr
   }
 }
 
+// The z pass of a convolution by a real selection mesh, in place: each half-complex row
+// (already through its inverse x and y passes) is taken back to real space in LDS (as in
+// zpass_c2r_kernel), multiplied point by point by the matching real row of Q times `scale`,
+// and transformed forward again (as in zpass_r2c_kernel).  The real row is never written.
+template <int M>
+__global__ void __launch_bounds__(RowCfg<M>::NT)
+zpass_mul_real_kernel(float2* __restrict__ data, int64_t pitch, const float* __restrict__ Q, int64_t nrows, float scale) {
+  using Cfg = RowCfg<M>;
+  constexpr int C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
+  extern __shared__ float2 lds[];
+  __shared__ float2 stw[2 * M];
+  const int tid = threadIdx.x;
+  const int64_t row0 = (int64_t)blockIdx.x * C;
+  load_twiddles<2 * M, NT>(stw, g_twiddle);
+  __syncthreads();
+  constexpr int NK = M / 2 + 1;
+  for (int idx = tid; idx < C * NK; idx += NT) {
+    int c = idx / NK, k = idx % NK;
+    int64_t row = row0 + c;
+    float2 zk = make_float2(0.f, 0.f), zmk = make_float2(0.f, 0.f);
+    if (row < nrows) {
+      const float2* x = data + row * pitch;
+      if (k == 0) {
+        float x0 = x[0].x, xm = x[M].x;
+        zk = make_float2(0.5f * (x0 + xm), 0.5f * (x0 - xm));
+      } else {
+        float2 xk = x[k], xmk = cconj(x[M - k]);
+        float2 E = cscale(cadd(xk, xmk), 0.5f);
+        float2 O = cmul(cscale(csub(xk, xmk), 0.5f), cconj(stw[k]));
+        zk = make_float2(E.x - O.y, E.y + O.x);
+        zmk = make_float2(E.x + O.y, -E.y + O.x);
+      }
+    }
+    lds[c * LP + k] = zk;
+    if (k != 0 && 2 * k != M) lds[c * LP + (M - k)] = zmk;
+  }
+  __syncthreads();
+  fft_lds<M, C, 1, LP, false, NT, true, 2 * M>(lds, stw);
+  const float2* q2 = reinterpret_cast<const float2*>(Q);
+  for (int idx = tid; idx < C * M; idx += NT) {
+    int c = idx / M, l = idx % M;
+    int64_t row = row0 + c;
+    const float2 qv = row < nrows ? q2[row * M + l] : make_float2(0.f, 0.f);
+    const float2 v = lds[c * LP + l];
+    lds[c * LP + l] = make_float2(v.x * (qv.x * scale), v.y * (qv.y * scale));
+  }
+  __syncthreads();
+  fft_lds<M, C, 1, LP, false, NT, false, 2 * M>(lds, stw);
+  for (int idx = tid; idx < C * NK; idx += NT) {
+    int c = idx / NK, k = idx % NK;
+    int64_t row = row0 + c;
+    if (row >= nrows) continue;
+    rfft_post<2 * M>(lds + c * LP, data + row * pitch, M, k, stw);
+  }
+}
+
 // The z c2r pass fused with the separation-shell binning of the real rows it produces (the
 // correlation function of a product spectrum, jaxpower/mesh2.py:471-498): the rows never
 // leave LDS.  Each thread then sweeps RUN consecutive complex elements (2 RUN reals) of one
@@ -467,6 +523,14 @@ static int launch_zpass_c2r(const float2* in, int64_t pitch, float* out, int64_t
   return MP_OK;
 }
 
+template <int M>
+static int launch_zpass_mul_real(float2* data, int64_t pitch, const float* Q, int64_t nrows, float scale, hipStream_t stream) {
+  using Cfg = RowCfg<M>;
+  dim3 grid((unsigned)ceil_div(nrows, Cfg::C));
+  MP_LAUNCH("window_zpass", zpass_mul_real_kernel<M>, grid, dim3(Cfg::NT), Cfg::LDS, stream, data, pitch, Q, nrows, scale);
+  return MP_OK;
+}
+
 template <int L, bool INV>
 static int launch_col(const char* name, const float2* in, float2* out, int ncols, int64_t nouter, int64_t ifs, int64_t ios,
                       int64_t ofs, int64_t oos, float scale, hipStream_t stream) {
@@ -671,6 +735,28 @@ int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const
   // the z pass returns M * (x_even + i x_odd) for M = Nz/2 (unnormalised M-point inverse)
   const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)M));
   return zpass_c2r(M, scratch, scratch_pitch, out, nx * ny, scale, s);
+}
+
+int convolve_real(hipStream_t s, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2, ncols = M + 1;
+  if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  if (pitch < ncols) return set_error(MP_ERR_INVALID, "pitch %lld < Nz/2+1", (long long)pitch);
+  const int64_t nx = n[0], ny = n[1];
+  rc = col_fft(n[0], true, "window_xpass_inv", data, data, ncols, ny, ny * pitch, pitch, ny * pitch, pitch, 1.f, s);
+  if (rc) return rc;
+  rc = col_fft(n[1], true, "window_ypass_inv", data, data, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
+  if (rc) return rc;
+  // irfftn normalisation 1/(N0 N1 Nz) with the z pass returning M (x_even + i x_odd), M = Nz/2
+  const float zscale = (float)((double)scale / ((double)n[0] * (double)n[1] * (double)M));
+  rc = [&]() -> int { MP_POW2_SWITCH(M, launch_zpass_mul_real, data, pitch, Q, nx * ny, zscale, s); }();
+  if (rc) return rc;
+  rc = col_fft(n[1], false, "window_ypass", data, data, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
+  if (rc) return rc;
+  return col_fft(n[0], false, "window_xpass", data, data, ncols, ny, ny * pitch, pitch, ny * pitch, pitch, 1.f, s);
 }
 
 template <int M, int NE>

@@ -936,8 +936,8 @@ static int pz_choose(hipStream_t s, int order, int M, const int32_t n[3], int ni
       chosen = c[q];
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   entry.shape = chosen;
   std::lock_guard<std::mutex> lock(g_pz_tune_mu);

@@ -11,7 +11,8 @@ from . import _lib
 from .mesh import (MeshAttrs, BaseMeshField, RealMeshField, ComplexMeshField, ParticleField, FKPField, format_meshes,
                    compute_normalization, compute_box_normalization, _as_spectrum, _sum_f64, _as_device_tensor,
                    _default_device, _empty_spectrum, _spectrum_pitch, _c2r, _r2c)
-from .types import Deferred, Mesh2SpectrumPole, Mesh2SpectrumPoles, Mesh2CorrelationPole, Mesh2CorrelationPoles
+from .types import (Deferred, Mesh2SpectrumPole, Mesh2SpectrumPoles, Mesh2CorrelationPole, Mesh2CorrelationPoles,
+                    WindowMatrix, _host)
 
 
 def _format_ells(ells):
@@ -512,6 +513,93 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
                                   volume=vcell * bin.nmodes, ell=ell, attrs=attrs)
              for i, ell in enumerate(ells)]
     return Mesh2CorrelationPoles(poles, attrs=attrs)
+
+
+def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='z', buffer=None, batch_size=None, pbar=False,
+                                  norm=None, flags=tuple()):
+    """Window matrix of the power-spectrum multipoles from a selection mesh, global line of
+    sight, theory defined on the mesh (mesh2.py:921-1118 without flags, 1453-1482).
+
+    For each input multipole ellin and input shell [e0, e1): A = [e0 <= |k| < e1]
+    P_ellin(mu) N^3, convolved by the selection as r2c(Q c2r(A)) with Q = c2r(A1 conj(A2)) / N^3
+    (nothing for a periodic box, given as a MeshAttrs), then binned as
+    (2l+1)/(1+odd) bin(A (P_l(mu) + odd P_l(-mu)), remove_zero=True) N^3/V_cell / norm / N^3.
+    The real part of every row is kept (mesh2.py:1466-1472), and odd output multipoles vanish.
+    ``buffer``, ``batch_size`` and ``pbar`` only steer the reference's memory use and progress
+    display and are accepted for compatibility; the flags 'smooth' and 'infinite' and the local
+    line of sight are not implemented on this path."""
+    if not isinstance(bin, BinMesh2SpectrumPoles):
+        raise ValueError(f'bin must be BinMesh2SpectrumPoles, not {type(bin)}')
+    flags = tuple(flags or ())
+    if flags:
+        raise NotImplementedError(f'window flags {flags} are not implemented on this path (mesh-defined theory only)')
+    periodic = isinstance(meshes[0], MeshAttrs)
+    if periodic:
+        mattrs = meshes[0]
+        assert all(m is None or m is mattrs for m in meshes[1:]), 'a periodic window takes one MeshAttrs'
+    else:
+        meshes, fields = format_meshes(*meshes, nmeshes=2)
+        mattrs = meshes[0].attrs
+    los, vlos, swap = _format_los(los)
+    if vlos is None:
+        raise NotImplementedError('window matrix with a local (firstpoint/endpoint) line of sight is not implemented on this path')
+    ells = bin.ells
+    n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
+    _norm = n3 / float(np.prod(mattrs.cellsize))
+    if norm is None:
+        norm = _norm
+    norm = np.broadcast_to(np.asarray(_host(norm), dtype='f8').ravel() if np.ndim(_host(norm)) else float(_host(norm)),
+                           (len(ells),)).copy()
+    rnorm = _norm / norm / n3
+    if hasattr(edgesin, 'ells') and hasattr(edgesin, 'edges'):  # a theory container (mesh2.py:994-996)
+        ellsin = edgesin.ells
+        edgesin = edgesin.edges('k')
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    ellsin = [int(ellsin)] if np.ndim(ellsin) == 0 else [int(ell) for ell in ellsin]
+    device = bin._device
+    Q = None
+    if not periodic:  # Q N^3 = c2r(A1 conj(A2)), mesh2.py:1022-1029
+        A1 = _2c(meshes[0])
+        P, _ = _product_spectrum(A1) if fields[1] == fields[0] else _product_spectrum(_2c(meshes[1]), A1)
+        Q = _c2r(P, mattrs.meshsize)
+        del P
+        device = Q.device
+    # shell membership e0 <= |k| < e1 as k^2 >= t(e0) and k^2 < t(e1), t as for the bin edges
+    thr = _k2_thresholds(edgesin.ravel()).reshape(edgesin.shape)
+    B = _empty_spectrum(mattrs.meshsize, device)
+    pitch = _spectrum_pitch(B)
+    scale = n3 if Q is None else 1.  # the N^3 of A cancels the 1/N^3 of Q
+    even = [ell for ell in ells if ell % 2 == 0]
+    products = [(-1, 0, ell, True) for ell in even]
+    t = bin._tables
+    stream = _lib.stream_handle(device)
+    n = _lib.i32x3(mattrs.meshsize)
+    cols = []
+    for ellin in ellsin:
+        for t_lo, t_hi in thr:
+            _lib.call('mp_window_shell', stream, _lib.ptr(Q) if Q is not None else None, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
+                      _lib.ptr(t['kz']), float(t_lo), float(t_hi), int(ellin), _lib.f64x3(vlos), scale, _lib.ptr(B), pitch)
+            cols.append(bin._products_dev([B], products, los=vlos) if products else None)
+    nbins = bin.nbins
+    ncol = len(cols)
+    sums = torch.stack(cols).cpu().numpy() if products else np.zeros((ncol, 0, nbins), dtype='c16')
+    wmat = np.zeros((ncol, len(ells), nbins), dtype='f8')
+    with np.errstate(invalid='ignore', divide='ignore'):
+        for ie, ell in enumerate(even):
+            ill = ells.index(ell)
+            wmat[:, ill, :] = (2 * ell + 1) * rnorm[ill] * sums[:, ie, :].real / bin.nmodes
+    wmat = wmat.reshape(ncol, -1).T
+    attrs = dict(los=vlos.tolist())
+    zeros = np.zeros(nbins)
+    observable = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=zeros,
+                                                       num_shotnoise=zeros, norm=np.full(nbins, norm[ill]), ell=ell, attrs=attrs)
+                                     for ill, ell in enumerate(ells)], attrs=attrs)
+    kin = edgesin.mean(axis=-1)
+    theory = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=kin, k_edges=edgesin, nmodes=np.zeros(len(kin), dtype='i8'),
+                                                   num_raw=np.zeros(len(kin)), ell=ell) for ell in ellsin])
+    return WindowMatrix(observable=observable, theory=theory, value=wmat)
 
 
 def compute_mesh2(*meshes, bin=None, los='z'):

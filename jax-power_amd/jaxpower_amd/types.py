@@ -301,6 +301,45 @@ class Mesh3SpectrumPoles(_MeshPoles):
         super().write(fn, extra={'basis': np.array(self._branches[0].basis if self._branches else 'scoccimarro')})
 
 
+class WindowMatrix:
+    """Window matrix W mapping a theory (input multipoles on input k bins) to the binned
+    observable: observable = W . theory (mesh2.py:1453-1482).  ``observable`` holds the output
+    poles (k, edges, nmodes, norm; num_raw zero), ``theory`` the input poles (k = mid-points of
+    the input edges), ``value()`` the (n_ells n_bins, n_ellsin n_in) matrix, ordered by ell then
+    bin on both sides."""
+
+    def __init__(self, observable, theory, value):
+        self.observable = observable
+        self.theory = theory
+        self._value = np.asarray(value)
+
+    def value(self):
+        return self._value
+
+    @property
+    def shape(self):
+        return self._value.shape
+
+    def dot(self, theory, return_type=None):
+        """W . theory: ``theory`` is a poles container (its concatenated ``value()``) or a flat
+        array; returns the observable poles with that value (num_raw = value x norm), or the
+        flat array when return_type is 'array'."""
+        vec = np.asarray(theory.value() if hasattr(theory, 'value') else theory, dtype='f8').ravel()
+        out = self._value @ vec
+        if return_type == 'array':
+            return out
+        poles, start = [], 0
+        for p in self.observable:
+            size = p.shape[0]
+            norm = p.values('norm')
+            poles.append(p.clone(num_raw=out[start:start + size] * norm, num_shotnoise=np.zeros(size)))
+            start += size
+        return type(self.observable)(poles, attrs=self.observable.attrs)
+
+    def __repr__(self):
+        return f'WindowMatrix(shape={self._value.shape}, ells={self.observable.ells}, ellsin={self.theory.ells})'
+
+
 def read(fn):
     """Read a Mesh2SpectrumPoles, Mesh2CorrelationPoles or Mesh3SpectrumPoles written by ``write``."""
     with np.load(fn, allow_pickle=False) as data:

@@ -824,3 +824,68 @@ def bispectrum_bruteforce(attrs, bin, rmeshes, ell=0, los='z'):
             if t is not None:
                 out[t] += f[j]
     return out
+
+
+# ----------------------------------------------------------------------------
+# Window matrix, global line of sight (mesh2.py:921-1118, 1453-1482)
+# ----------------------------------------------------------------------------
+
+def spectrum_window(attrs, bin, edgesin, ellsin=0, meshes=None, los='z', norm=None):
+    """compute_mesh2_spectrum_window with a global line of sight and no flags
+    (mesh2.py:921-1118, 1453-1482): for each input multipole ellin and input shell
+    [e0, e1), the mesh-defined theory A = [e0 <= |k| < e1] P_ellin(mu) N^3 is (for a
+    selection) convolved as r2c(Q c2r(A)) with Q = c2r(A1 conj(A2)) / N^3 (mesh2.py:1022-1029,
+    1106-1107), then binned as (2l+1)/(1+odd) bin(A (P_l(mu) + odd P_l(-mu)), remove_zero=True)
+    rnorm (mesh2.py:1037-1045), rnorm = N^3/V_cell / norm / N^3 (mesh2.py:973-979).
+    meshes: None (periodic box, mesh2.py:1030-1034) or a list of one or two real meshes /
+    half spectra.  Returns the (n_ells n_bins, n_ellsin n_in) real matrix: the reference
+    keeps the real part of every row (mesh2.py:1466-1472)."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    ellsin = (ellsin,) if np.ndim(ellsin) == 0 else tuple(ellsin)
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    vlos = los_vector(los)
+    Q = None
+    if meshes is not None:
+        cm = [m if np.iscomplexobj(m) else r2c(m) for m in meshes]
+        Q = c2r(cm[0] * np.conj(cm[-1]), attrs.meshsize) / n3
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    kn = knorm(attrs)
+    mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / np.where(kn == 0., 1., kn)
+    cols = []
+    for ellin in ellsin:
+        legin = legendre(ellin, mu)
+        for e0, e1 in edgesin:
+            A = ((kn >= e0) & (kn < e1)) * legin * n3
+            if Q is not None:
+                A = r2c(Q * c2r(A, attrs.meshsize))
+            col = []
+            for ill, ell in enumerate(ells):
+                odd = ell % 2
+                leg = legendre(ell, mu) + (legendre(ell, -mu) if odd else 0.)
+                col.append((2 * ell + 1) / (1 + odd) * bin(A * leg, remove_zero=True) * rnorm[ill])
+            cols.append(np.concatenate(col))
+    return np.real(np.array(cols).T)
+
+
+def convolve_bruteforce(Q, A_full):
+    """fftn(Q ifftn(A)) by the convolution theorem written out: (1/N^3) sum_q Qhat(k - q) A(q),
+    with Qhat the direct DFT of Q (explicit exponential sums, no FFT).  Full grids; tiny
+    meshes only."""
+    n = Q.shape
+    W = [np.exp(-2j * np.pi * np.outer(np.arange(m), np.arange(m)) / m) for m in n]
+    Qhat = np.einsum('ia,jb,kc,abc->ijk', W[0], W[1], W[2], Q)
+    out = np.zeros(n, dtype=complex)
+    idx = [np.arange(m) for m in n]
+    for q in np.ndindex(*n):
+        if A_full[q] == 0:
+            continue
+        shifted = Qhat[np.ix_((idx[0] - q[0]) % n[0], (idx[1] - q[1]) % n[1], (idx[2] - q[2]) % n[2])]
+        out += shifted * A_full[q]
+    return out / np.prod(n)

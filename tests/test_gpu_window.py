@@ -1,0 +1,122 @@
+"""GPU parity of the power-spectrum window matrix (compute_mesh2_spectrum_window, global line
+of sight, theory defined on the mesh; jaxpower/mesh2.py:921-1118, 1453-1482) against the
+float64 oracle.
+
+Tolerance: |dW| <= 1e-5 |W| + 1e-5 max|W| entry by entry.  The periodic window involves no FFT
+(float32 Legendre factors, FP64 bin sums); with a selection every column goes through a float32
+c2r and r2c, whose rounding is ~1e-7 of the largest mode of the convolved shell, spread over all
+modes.  Either way the terms summed are O(max|W|) and carry float32 rounding, so blocks that
+vanish by symmetry (e.g. ell = 0 from ellin = 2 on full shells) hold residues ~1e-8 max|W|:
+hence the floor relative to the whole matrix.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import power_oracle as po
+
+from test_gpu_parity import DEV, host, say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_window_close(got, want, nout, nin, rtol=1e-5, ftol=1e-5, label=''):
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.shape == want.shape, (got.shape, want.shape)
+    nell, nellin = want.shape[0] // nout, want.shape[1] // nin
+    floor = ftol * np.nanmax(np.abs(want))
+    for i in range(nell):
+        for j in range(nellin):
+            g = got[i * nout:(i + 1) * nout, j * nin:(j + 1) * nin]
+            w = want[i * nout:(i + 1) * nout, j * nin:(j + 1) * nin]
+            nan = np.isnan(w)
+            assert np.array_equal(nan, np.isnan(g)), f'{label} block ({i}, {j}): NaN pattern differs'
+            err = np.abs(g - w)[~nan]
+            bad = err > rtol * np.abs(w[~nan]) + floor
+            assert not np.any(bad), f'{label} block ({i}, {j}): max err {err.max():.3e}, floor {floor:.3e}'
+
+
+def _survey_selection(attrs, size, seed, scale=0.2):
+    """Gaussian blob of points, squashed along y and z and cut to the box (the selection of
+    the reference's tests/test_mesh2.py:528-560)."""
+    rng = np.random.default_rng(seed)
+    pos = np.array([1., 0.2, 0.2]) * scale * rng.normal(size=(size, 3))
+    mask = np.all((pos > -scale) & (pos < scale), axis=-1)
+    pos = pos * attrs.boxsize + attrs.boxcenter
+    return pos, mask.astype('f4')
+
+
+@pytest.mark.parametrize('meshsize,boxsize,edgesin,los', [
+    (32, 320., 'common', 'z'),
+    (32, 320., np.arange(0.01, 0.3, 0.013), 'x'),
+    ((16, 32, 64), (320., 400., 640.), np.arange(0., 0.25, 0.02), [0.3, -0.4, np.sqrt(0.75)]),
+])
+def test_window_periodic(meshsize, boxsize, edgesin, los):
+    from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window periodic', meshsize, boxsize, los)
+    attrs = MeshAttrs(meshsize=meshsize, boxsize=boxsize, boxcenter=0.)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    if isinstance(edgesin, str):
+        edgesin = bin.edges
+    ellsin = (0, 2, 4)
+    wm = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los)
+    oattrs = po.Attrs(meshsize, boxsize, 0.)
+    obin = po.BinPoles(oattrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    want = po.spectrum_window(oattrs, obin, edgesin, ellsin=ellsin, los=los)
+    nin = len(wm.theory.get(0).coords())
+    assert_window_close(wm.value(), want, len(obin.nmodes), nin, label='periodic')
+    assert wm.observable.ells == (0, 2, 4) and wm.theory.ells == ellsin
+
+
+@pytest.mark.parametrize('cross,los,norm', [(False, 'x', None), (True, [0.6, 0.8, 0.], 2.5e-3), (False, 'z', [1e-3, 2e-3, 4e-3])])
+def test_window_selection(cross, los, norm):
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window selection', cross, los, norm)
+    attrs = MeshAttrs(meshsize=32, boxsize=500., boxcenter=1000.)
+    pos, w = _survey_selection(attrs, 200000, 42)
+    sel1 = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+    meshes = [sel1]
+    if cross:
+        pos2, w2 = _survey_selection(attrs, 150000, 7, scale=0.25)
+        meshes.append(ParticleField(to_dev(pos2), to_dev(w2), attrs=attrs).paint(resampler='tsc', interlacing=1, compensate=False))
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    edgesin = np.arange(0.01, 0.2, 0.02)
+    ellsin = (0, 2, 4)
+    wm = compute_mesh2_spectrum_window(*meshes, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, norm=norm)
+    oattrs = po.Attrs(32, 500., 1000.)
+    obin = po.BinPoles(oattrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    omeshes = [host(m.value).astype('f8') for m in meshes]
+    want = po.spectrum_window(oattrs, obin, edgesin, ellsin=ellsin, meshes=omeshes, los=los, norm=norm)
+    nout, nin = len(obin.nmodes), len(edgesin) - 1
+    assert_window_close(wm.value(), want, nout, nin, label='selection')
+    # W . theory for a theory piecewise constant on the input shells
+    rng = np.random.default_rng(1)
+    theta = rng.uniform(0.5, 2., want.shape[1])
+    obs = wm.dot(theta)
+    got = np.concatenate([p.value() for p in obs])
+    ref = want @ theta
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+
+
+def test_window_constant_selection_is_periodic():
+    """A constant selection (Q = 1) must give the periodic window: the convolution path against
+    the direct one on the GPU itself."""
+    from jaxpower_amd import MeshAttrs, RealMeshField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window constant selection')
+    attrs = MeshAttrs(meshsize=32, boxsize=400., boxcenter=0.)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.025}, ells=(0, 2, 4))
+    ones = RealMeshField(torch.ones((32, 32, 32), dtype=torch.float32, device=DEV), attrs=attrs)
+    edgesin = np.arange(0., 0.3, 0.03)
+    w1 = compute_mesh2_spectrum_window(ones, edgesin=edgesin, ellsin=(0, 2), bin=bin, los='y').value()
+    w0 = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=(0, 2), bin=bin, los='y').value()
+    assert_window_close(w1, w0, bin.nbins, len(edgesin) - 1, label='constant selection')
+
+
+def test_window_unsupported_raise():
+    from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.05}, ells=(0, 2))
+    with pytest.raises(NotImplementedError):
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=0)
+    with pytest.raises(NotImplementedError):
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('infinite',))

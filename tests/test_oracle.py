@@ -509,3 +509,77 @@ def test_bispectrum_poles_bruteforce(los, cross):
         scale = np.abs(want).max()
         assert np.allclose(res['num_raw'][i], want, rtol=1e-9, atol=1e-10 * scale), ell
     assert np.isclose(res['norm'], 512 / 1000.**2)
+
+
+# ---------------------------------------------------------------- window matrix (global line of sight)
+
+def test_window_periodic_matches_common_edges_route():
+    """mesh2.py:1091-1095: with input edges equal to the output edges, the reference bins
+    P_ellin(mu) N^3 over the whole mesh once and puts the result on the diagonal; the per-shell
+    route of the oracle must give that matrix (zeros off the diagonal, the DC mode dropped)."""
+    a = po.Attrs(16, 400., 0.)
+    bin = po.BinPoles(a, edges={'step': 2. * np.pi / 400.}, ells=(0, 2, 4))
+    ellsin = (0, 2, 4)
+    W = po.spectrum_window(a, bin, bin.edges, ellsin=ellsin)
+    nb = len(bin.nmodes)
+    kx, ky, kz = a.kcoords('wavenumber')
+    k2 = kx[:, None, None]**2 + ky[None, :, None]**2 + kz[None, None, :]**2
+    kn = np.sqrt(k2)
+    mu = np.broadcast_to(kz[None, None, :], kn.shape) / np.where(kn == 0, 1., kn)
+    n3 = np.prod(a.meshsize)
+    for iin, ellin in enumerate(ellsin):
+        for il, ell in enumerate(bin.ells):
+            block = W[il * nb:(il + 1) * nb, iin * nb:(iin + 1) * nb]
+            diag = (2 * ell + 1) * bin(po.legendre(ellin, mu) * po.legendre(ell, mu) * n3, remove_zero=True).real / n3
+            np.testing.assert_allclose(np.diag(block), diag, rtol=1e-12, atol=1e-14)
+            np.testing.assert_array_equal(block - np.diag(np.diag(block)), 0.)
+    # monopole to monopole: 1 in every bin, minus the DC mode in the first
+    block = W[:nb, :nb]
+    np.testing.assert_allclose(np.diag(block), 1. - (np.arange(nb) == 0) / bin.nmodes[0], rtol=1e-13)
+
+
+def test_window_selection_vs_bruteforce_convolution():
+    """r2c(Q c2r(A)) with Q = c2r(A1 conj A2)/N^3 against the convolution written out:
+    Q(s) from explicit circular shifts, A on the full grid from fftfreq wavenumbers (even ellin:
+    A is real and even, hence hermitian), Qhat by direct DFT sums."""
+    rng = np.random.default_rng(3)
+    n = 6
+    a = po.Attrs(n, 60., 0.)
+    bin = po.BinPoles(a, edges={'step': 2. * np.pi / 60.}, ells=(0, 2, 4))
+    m1, m2 = rng.normal(size=(2, n, n, n))
+    edgesin = np.array([[0., 0.15], [0.15, 0.25], [0.25, 0.4]])
+    ellsin = (0, 2)
+    W = po.spectrum_window(a, bin, edgesin, ellsin=ellsin, meshes=[m1, m2], los='y')
+    n3 = n**3
+    Q = po.circular_correlation_bruteforce(m2, m1) / n3
+    kk = [np.fft.fftfreq(n) * a.kfun[i] * n for i in range(3)]
+    kn = np.sqrt(kk[0][:, None, None]**2 + kk[1][None, :, None]**2 + kk[2][None, None, :]**2)
+    mu = np.broadcast_to(kk[1][None, :, None], kn.shape) / np.where(kn == 0, 1., kn)
+    nb = len(bin.nmodes)
+    kxh, kyh, kzh = a.kcoords('wavenumber')
+    knh = np.sqrt(kxh[:, None, None]**2 + kyh[None, :, None]**2 + kzh[None, None, :]**2)
+    muh = np.broadcast_to(kyh[None, :, None], knh.shape) / np.where(knh == 0, 1., knh)
+    for iin, ellin in enumerate(ellsin):
+        for ie, (e0, e1) in enumerate(edgesin):
+            A_full = ((kn >= e0) & (kn < e1)) * po.legendre(ellin, mu) * n3
+            B = po.convolve_bruteforce(Q, A_full)[:, :, :n // 2 + 1]
+            for il, ell in enumerate(bin.ells):
+                want = (2 * ell + 1) * bin(B * po.legendre(ell, muh), remove_zero=True).real / n3
+                got = W[il * nb:(il + 1) * nb, iin * len(edgesin) + ie]
+                np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.max(np.abs(W)))
+
+
+def test_window_constant_selection_is_periodic():
+    """A constant selection mesh gives Q = 1 (mesh2.py:1022-1029), so the window must be the
+    periodic one; with norm given, the matrix scales by N^3/V_cell / norm (mesh2.py:973-979)."""
+    a = po.Attrs(12, 300., 0.)
+    bin = po.BinPoles(a, edges={'step': 0.03}, ells=(0, 2))
+    edgesin = np.arange(0., 0.2, 0.025)
+    W0 = po.spectrum_window(a, bin, edgesin, ellsin=(0, 2, 4), los=[0.6, 0., 0.8])
+    W1 = po.spectrum_window(a, bin, edgesin, ellsin=(0, 2, 4), meshes=[np.ones(tuple(a.meshsize))], los=[0.6, 0., 0.8])
+    np.testing.assert_allclose(W1, W0, rtol=1e-9, atol=1e-12)
+    norm = 7.5
+    W2 = po.spectrum_window(a, bin, edgesin, ellsin=(0, 2, 4), los=[0.6, 0., 0.8], norm=norm)
+    _norm = np.prod(a.meshsize) / np.prod(a.cellsize)
+    np.testing.assert_allclose(W2, W0 * _norm / norm, rtol=1e-13)
+    assert np.abs(np.diag(W0[:len(bin.nmodes), :])).max() > 0.5
