@@ -333,8 +333,9 @@ def run_corr512(args):
 def window_bytes(n):
     """Compulsory HBM bytes per launch of the window-matrix kernels at n^3 (one input shell)."""
     H = n * n * (n // 2 + 1)
-    return {'window_shell': H * 8, 'window_xpass_inv': 2 * H * 8, 'window_ypass_inv': 2 * H * 8,
-            'window_zpass': 2 * H * 8 + n**3 * 4, 'window_ypass': 2 * H * 8, 'window_xpass': 2 * H * 8, 'bin_poles': H * 8}
+    return {'window_xpass_inv': H * 8,                     # shell generated, spectrum written
+            'window_ypass_inv': 2 * H * 8, 'window_zpass': 2 * H * 8 + n**3 * 4, 'window_ypass': 2 * H * 8,
+            'fft_xpass_bin': H * 8}                          # spectrum read, per-bin sums out
 
 
 def run_window256(args):

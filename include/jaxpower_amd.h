@@ -240,6 +240,18 @@ int mp_window_shell(void* stream, const float* Q, const int32_t meshsize[3], con
                     const double* kz, double k2_lo, double k2_hi, int ellin, const double los[3], float scale, void* out,
                     int64_t out_pitch);
 
+/* mp_window_shell with a selection, then the output binning, for ncol columns in one call
+ * (jaxpower/mesh2.py:1037-1045, 1099): column c is the input shell k2_lo[c] <= k^2 < k2_hi[c] of
+ * multipole ellin[c] (k2_lo, k2_hi, ellin: HOST arrays of ncol entries) and
+ * out[c][p][b][re, im] = sum over the modes of bin b, mode k = 0 excluded, of
+ * w Re(B) P_ell[p](k.los/|k|) (imaginary slots 0), B = r2c(Q irfftn(A)); neither A nor B is
+ * written.  Power-of-two meshsizes, nell <= 6; bins as in mp_bin_poles. */
+int64_t mp_window_bin_workspace_bytes(const int32_t meshsize[3], int nell, int nedges);
+int mp_window_bin(void* stream, const float* Q, const int32_t meshsize[3], const double* kx, const double* ky,
+                  const double* kz, const double* k2_thresholds, int nedges, int ncol, const double* k2_lo,
+                  const double* k2_hi, const int32_t* ellin, const double los[3], float scale, int nell, const int32_t* ells,
+                  double* out, void* workspace, size_t workspace_bytes);
+
 /* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
  *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */
 int64_t mp_reduce_workspace_bytes(void);

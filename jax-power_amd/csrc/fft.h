@@ -72,7 +72,7 @@ int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const i
 size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges);
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
-                double* out, void* ws, size_t ws_bytes);
+                double* out, void* ws, size_t ws_bytes, int linear = 0);
 // Local line of sight for one ell, fused (xpass.hip): z/y passes of rho Y_lm(xhat) for all m,
 // then the x pass forms C = sum_m FFT(.) Y_lm(khat) per tile and bins conj(C) A0 w (and |A0|^2 w
 // when with0) without writing C; out[p][b][re, im] as mp_bin_poles writes it.
@@ -83,6 +83,38 @@ int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n
               const KPrologue& kp, int first);
 int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const int32_t n[3], float2* scratch,
             int64_t scratch_pitch);
+// |k| shell of the window matrix (jaxpower/mesh2.py:1097-1098): the half-spectrum value
+// [t_lo <= k^2 < t_hi] P_ell(k.los/|k|) scale at mode (ix, iy, iz), k^2 summed in float64 as the
+// binning does (so shell membership is digitize of the same edges), mu = 0 at k = 0.
+struct ShellGen {
+  const double* kx;
+  const double* ky;
+  const double* kz;
+  double t_lo, t_hi;
+  double los[3];
+  LegendreTable leg;
+  int ell_on;
+  float scale;
+  __device__ __forceinline__ float eval(int ix, int iy, int iz) const {
+    const double kxv = kx[ix], kyv = ky[iy], kzv = kz[iz];
+    double k2;
+    {
+#pragma clang fp contract(off)
+      k2 = (kxv * kxv + kyv * kyv) + kzv * kzv;
+    }
+    if (!(k2 >= t_lo && k2 < t_hi)) return 0.f;
+    float v = scale;
+    if (ell_on) {
+      const double mu = k2 > 0. ? (kxv * los[0] + kyv * los[1] + kzv * los[2]) / sqrt(k2) : 0.;
+      const float m = (float)mu;
+      v *= legendre_eval(leg, 0, m, m * m);
+    }
+    return v;
+  }
+};
+// data <- the z and y passes of r2c(Q * scale * irfftn(S)), S the shell of g, with the inverse
+// x pass generating S instead of reading it; the forward x pass is left to a consumer.
+int convolve_shell_zy(hipStream_t s, const ShellGen& g, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale);
 // In place on a half spectrum: data <- r2c(Q * scale * irfftn(data)), Q a real (N0, N1, N2)
 // mesh; the real mesh is formed and multiplied inside the z pass only.
 int convolve_real(hipStream_t s, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale);

@@ -405,6 +405,32 @@ col_fft_kernel(const float2* in, float2* out, int ncols, int64_t in_fstride, int
   }
 }
 
+// Inverse FFT along x of a generated half spectrum (the window's |k| shell): as
+// col_fft_kernel<L, true> with the input of column c, row l of slice blockIdx.y computed from
+// the mode indices (l, blockIdx.y, c) instead of read.
+template <int L>
+__global__ void __launch_bounds__(ColCfg<L>::NT)
+xpass_inv_shell_kernel(ShellGen g, float2* out, int ncols, int64_t out_fstride, int64_t out_ostride) {
+  using Cfg = ColCfg<L>;
+  constexpr int C = Cfg::C, NT = Cfg::NT;
+  extern __shared__ float2 lds[];
+  __shared__ float2 stw[L];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C, y = blockIdx.y;
+  load_twiddles<L, NT>(stw, g_twiddle);
+  for (int idx = tid; idx < C * L; idx += NT) {
+    int c = idx % C, l = idx / C;
+    lds[l * C + c] = make_float2((c0 + c < ncols) ? g.eval(l, y, c0 + c) : 0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<L, C, C, 1, true, NT, true, L>(lds, stw);
+  float2* dst = out + (int64_t)y * out_ostride + c0;
+  for (int idx = tid; idx < C * L; idx += NT) {
+    int c = idx % C, l = idx / C;
+    if (c0 + c < ncols) dst[l * out_fstride + c] = lds[l * C + c];
+  }
+}
+
 // Pipelined column pass over several meshes: persistent blocks walk tiles (mesh, outer,
 // chunk of C columns); the global loads of the next tile are issued before the FFT of the
 // current one, and the tile is written back in the order it was read (in place allowed).
@@ -520,6 +546,14 @@ static int launch_zpass_c2r(const float2* in, int64_t pitch, float* out, int64_t
   using Cfg = RowCfg<M>;
   dim3 grid((unsigned)ceil_div(nrows, Cfg::C));
   MP_LAUNCH("fft_zpass_c2r", zpass_c2r_kernel<M>, grid, dim3(Cfg::NT), Cfg::LDS, stream, in, pitch, out, nrows, scale);
+  return MP_OK;
+}
+
+template <int L>
+static int launch_xpass_inv_shell(const ShellGen& g, float2* out, int ncols, int ny, int64_t ofs, int64_t oos, hipStream_t stream) {
+  using Cfg = ColCfg<L>;
+  dim3 grid((unsigned)ceil_div(ncols, Cfg::C), (unsigned)ny);
+  MP_LAUNCH("window_xpass_inv", xpass_inv_shell_kernel<L>, grid, dim3(Cfg::NT), Cfg::LDS, stream, g, out, ncols, ofs, oos);
   return MP_OK;
 }
 
@@ -735,6 +769,25 @@ int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const
   // the z pass returns M * (x_even + i x_odd) for M = Nz/2 (unnormalised M-point inverse)
   const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)M));
   return zpass_c2r(M, scratch, scratch_pitch, out, nx * ny, scale, s);
+}
+
+int convolve_shell_zy(hipStream_t s, const ShellGen& g, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2, ncols = M + 1;
+  if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  if (pitch < ncols) return set_error(MP_ERR_INVALID, "pitch %lld < Nz/2+1", (long long)pitch);
+  const int64_t nx = n[0], ny = n[1];
+  rc = [&]() -> int { MP_POW2_SWITCH(n[0], launch_xpass_inv_shell, g, data, ncols, (int)ny, ny * pitch, pitch, s); }();
+  if (rc) return rc;
+  rc = col_fft(n[1], true, "window_ypass_inv", data, data, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
+  if (rc) return rc;
+  const float zscale = (float)((double)scale / ((double)n[0] * (double)n[1] * (double)M));
+  rc = [&]() -> int { MP_POW2_SWITCH(M, launch_zpass_mul_real, data, pitch, Q, nx * ny, zscale, s); }();
+  if (rc) return rc;
+  return col_fft(n[1], false, "window_ypass", data, data, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
 }
 
 int convolve_real(hipStream_t s, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale) {

@@ -38,6 +38,7 @@ struct XPassArgs {
   float scale;  // 1 / n_i
   int zero_dc;  // the (0,0,0) mode of A is set to 0
   int power;    // write mode: 1 stores |A|^2 (imaginary part 0) instead of A, 2 its inverse FFT along x
+  int linear;   // bin mode: bins Re(A) (a linear functional of the mesh) instead of |A|^2
   const float2* tw;
   // write mode
   float2* out;
@@ -252,7 +253,7 @@ xpass_kernel(XPassArgs a) {
             float2 A = cscale(acc[e], comp ? s * cx[l] : s);
             const bool origin = (l == 0) & (y == 0) & (kz == 0);
             if (origin && a.zero_dc) A = make_float2(0.f, 0.f);
-            const float p2 = A.x * A.x + A.y * A.y;
+            const float p2 = a.linear ? A.x : A.x * A.x + A.y * A.y;
             float mu = 0.f;
             if (a.need_mu) {
               const float fkx = (float)kxv;
@@ -501,10 +502,11 @@ int combine_tables(hipStream_t s, const int32_t n[3], int order, int ni, int com
   float* comp = wsp.take<float>((size_t)len);
   if (!ph || !comp) return set_error(MP_ERR_WORKSPACE, "x pass: workspace too small for the tables");
   const int total = (ni + 1) * len;
-  MP_LAUNCH("combine_tables", combine_tables_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, n[0], n[1], n[2], ni,
-            compensate ? order : 0, ph, comp);
   tab = CombineTables{};
   tab.stride = len;
+  if (ni == 1 && !compensate) return MP_OK;  // no phase and no compensation: nothing to tabulate
+  MP_LAUNCH("combine_tables", combine_tables_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, n[0], n[1], n[2], ni,
+            compensate ? order : 0, ph, comp);
   if (ni > 1) {
     tab.phx = ph;
     tab.phy = ph + n[0];
@@ -657,7 +659,7 @@ size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int ne
 
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
-                double* out, void* ws, size_t ws_bytes) {
+                double* out, void* ws, size_t ws_bytes, int linear) {
   MP_REQUIRE(partial && out && ell && remove_zero, "bin_partial: null pointer");
   MP_REQUIRE(ni >= 1, "bin_partial: interlacing %d < 1", ni);
   MP_REQUIRE(nprod >= 1 && nprod <= XB_MAX_PROD, "bin_partial: %d products not in 1..%d", nprod, XB_MAX_PROD);
@@ -678,6 +680,7 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   if (!a.tw) return set_error(MP_ERR_HIP, "bin_partial: twiddle table address");
   a.g = g;
   a.need_mu = 0;
+  a.linear = linear;
   for (int p = 0; p < nprod; ++p) {
     MP_REQUIRE(ell[p] >= 0 && ell[p] <= 10, "bin_partial: ell %d not in 0..10", ell[p]);
     a.ell[p] = ell[p];

@@ -568,8 +568,6 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
         device = Q.device
     # shell membership e0 <= |k| < e1 as k^2 >= t(e0) and k^2 < t(e1), t as for the bin edges
     thr = _k2_thresholds(edgesin.ravel()).reshape(edgesin.shape)
-    B = _empty_spectrum(mattrs.meshsize, device)
-    pitch = _spectrum_pitch(B)
     scale = n3 if Q is None else 1.  # the N^3 of A cancels the 1/N^3 of Q
     even = [ell for ell in ells if ell % 2 == 0]
     products = [(-1, 0, ell, True) for ell in even]
@@ -577,11 +575,27 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
     stream = _lib.stream_handle(device)
     n = _lib.i32x3(mattrs.meshsize)
     cols = []
-    for ellin in ellsin:
-        for t_lo, t_hi in thr:
-            _lib.call('mp_window_shell', stream, _lib.ptr(Q) if Q is not None else None, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
-                      _lib.ptr(t['kz']), float(t_lo), float(t_hi), int(ellin), _lib.f64x3(vlos), scale, _lib.ptr(B), pitch)
-            cols.append(bin._products_dev([B], products, los=vlos) if products else None)
+    nedges = len(bin._edges1d)
+    fused = bool(Q is not None and products and len(even) <= 6 and len(even) * (nedges - 1) <= 4096
+                 and all(int(m) >= 2 and int(m) & (int(m) - 1) == 0 for m in mattrs.meshsize))
+    B = None if fused else _empty_spectrum(mattrs.meshsize, device)
+    pitch = _spectrum_pitch(B) if B is not None else 0
+    if fused:  # every column in one call, four launches each, neither the shell nor its convolution written
+        ncol = len(ellsin) * len(thr)
+        ws = _lib.workspace(_lib.load().mp_window_bin_workspace_bytes(n, len(even), nedges), device)
+        out = torch.empty((ncol, len(even), bin.nbins, 2), dtype=torch.float64, device=device)
+        dbl = lambda v: (_lib.ctypes.c_double * ncol)(*[float(x) for x in v])
+        _lib.call('mp_window_bin', stream, _lib.ptr(Q), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']),
+                  nedges, ncol, dbl(list(thr[:, 0]) * len(ellsin)), dbl(list(thr[:, 1]) * len(ellsin)),
+                  (_lib.ctypes.c_int32 * ncol)(*[ellin for ellin in ellsin for _ in thr]), _lib.f64x3(vlos), scale, len(even),
+                  (_lib.ctypes.c_int32 * len(even))(*even), _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        cols = list(torch.view_as_complex(out))
+    else:
+        for ellin in ellsin:
+            for t_lo, t_hi in thr:
+                _lib.call('mp_window_shell', stream, _lib.ptr(Q) if Q is not None else None, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
+                          _lib.ptr(t['kz']), float(t_lo), float(t_hi), int(ellin), _lib.f64x3(vlos), scale, _lib.ptr(B), pitch)
+                cols.append(bin._products_dev([B], products, los=vlos) if products else None)
     nbins = bin.nbins
     ncol = len(cols)
     sums = torch.stack(cols).cpu().numpy() if products else np.zeros((ncol, 0, nbins), dtype='c16')

@@ -120,3 +120,40 @@ def test_window_unsupported_raise():
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=0)
     with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('infinite',))
+
+
+def test_window_fused_column_matches_written_spectrum():
+    """mp_window_bin (shell generated in the inverse x pass, binned in the forward x pass) against
+    mp_window_shell (the convolved spectrum written out) followed by mp_bin_poles, through the
+    C-ABI, on the same selection and shell."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, _lib
+    from jaxpower_amd.mesh import _empty_spectrum, _spectrum_pitch, _c2r
+    from jaxpower_amd.mesh2 import _k2_thresholds
+    say('window fused column vs written spectrum')
+    attrs = MeshAttrs(meshsize=(32, 64, 32), boxsize=(400., 800., 400.), boxcenter=0.)
+    pos, w = _survey_selection(attrs, 100000, 5)
+    sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+    A = sel.r2c().value
+    Q = _c2r((A * A.conj()).to(torch.complex64), attrs.meshsize)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    t = bin._tables
+    n = _lib.i32x3(attrs.meshsize)
+    los = _lib.f64x3([0., 0.6, 0.8])
+    t_lo, t_hi = _k2_thresholds(np.array([0.03, 0.07]))
+    ells = (_lib.ctypes.c_int32 * 3)(0, 2, 4)
+    ws = _lib.workspace(_lib.load().mp_window_bin_workspace_bytes(n, 3, len(bin._edges1d)), Q.device)
+    outs = torch.empty((2, 3, bin.nbins, 2), dtype=torch.float64, device=Q.device)
+    _lib.call('mp_window_bin', _lib.stream_handle(Q.device), _lib.ptr(Q), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
+              _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(bin._edges1d), 2, (_lib.ctypes.c_double * 2)(t_lo, t_lo),
+              (_lib.ctypes.c_double * 2)(t_hi, t_hi), (_lib.ctypes.c_int32 * 2)(0, 2), los, 1., 3, ells, _lib.ptr(outs),
+              _lib.ptr(ws), ws.numel())
+    for ic, ellin in enumerate((0, 2)):
+        out = outs[ic]
+        B = _empty_spectrum(attrs.meshsize, Q.device)
+        _lib.call('mp_window_shell', _lib.stream_handle(Q.device), _lib.ptr(Q), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
+                  _lib.ptr(t['kz']), float(t_lo), float(t_hi), ellin, los, 1., _lib.ptr(B), _spectrum_pitch(B))
+        ref = bin.products([B], [(-1, 0, ell, True) for ell in (0, 2, 4)], los=[0., 0.6, 0.8])
+        got = host(out)[..., 0]
+        floor = 1e-6 * np.abs(ref.real).max()
+        assert np.all(np.abs(got - ref.real) <= 1e-6 * np.abs(ref.real) + floor), (ellin, np.abs(got - ref.real).max(), floor)
+        assert np.all(host(out)[..., 1] == 0.)
