@@ -137,16 +137,31 @@ def summarize_pmc(fetch_dir, write_dir, copy_bytes, out=TRAFFIC_FILE):
     print('\n'.join(lines))
 
 
-def summarize_trace(folder, out=TRACE_FILE):
+WINDOW_TRACE_FILE = os.path.join(ROOT, 'profiles', 'kernels_window_256.txt')
+WINDOW_TRACE_COMMAND = ('rocprofv3 --kernel-trace --stats -d gpurun_out/kt_window -o run --output-format csv -- '
+                        'python3 bench.py --workload window256 --steps 5 --warmup 1 --no-cpu-baseline')
+WINDOW_SYMBOLS = {'window_xpass_inv': 'xpass_inv_shell_kernel', 'window_ypass_inv': 'col_fft_kernel<256, true>',
+                  'window_zpass': 'zpass_mul_real_kernel', 'window_ypass': 'col_fft_kernel<256, false>',
+                  'fft_xpass_bin': 'xpass_kernel<256, true'}
+
+
+def summarize_trace(folder, out=TRACE_FILE, workload='box512'):
     """Per-kernel launch statistics of one rocprofv3 --kernel-trace --stats run, with the
     algorithmic bytes and the bandwidth they imply."""
     import csv
     import glob
     import re
     path = glob.glob(os.path.join(folder, '**', '*kernel_stats.csv'), recursive=True)[0]
-    alg = algorithmic_bytes()
-    lines = ['# rocprofv3 kernel statistics of the 512^3 TSCx3 step (13 steps: 3 warm-up + 10 timed)',
-             '# made by: ' + TRACE_COMMAND, f'#          python bench.py --summarize-trace {folder}',
+    if workload == 'window256':
+        alg, symbols, out = window_bytes(256), WINDOW_SYMBOLS, WINDOW_TRACE_FILE
+        title = ('# rocprofv3 kernel statistics of the 256^3 window matrix (57 columns per step, 6 steps: 1 warm-up + 5 timed;'
+                 ' the col_fft lines include the 3 passes of each selection FFT at set-up)')
+        command = WINDOW_TRACE_COMMAND
+    else:
+        alg, symbols = algorithmic_bytes(), KERNEL_SYMBOLS
+        title = '# rocprofv3 kernel statistics of the 512^3 TSCx3 step (13 steps: 3 warm-up + 10 timed)'
+        command = TRACE_COMMAND
+    lines = [title, '# made by: ' + command, f'#          python bench.py --workload {workload} --summarize-trace {folder}',
              f'# {"kernel":58s} {"calls":>6s} {"avg_us":>10s} {"total_ms":>10s} {"share":>6s} {"alg_MB":>9s} {"alg_GB/s":>9s}']
     other = [0, 0.]
     for r in csv.DictReader(open(path)):
@@ -156,7 +171,7 @@ def summarize_trace(folder, out=TRACE_FILE):
             other[1] += float(r['TotalDurationNs']) * 1e-6
             continue
         avg = float(r['AverageNs']) * 1e-3
-        timing = [t for t, sym in KERNEL_SYMBOLS.items() if sym in name]
+        timing = [t for t, sym in symbols.items() if sym in name]
         b = alg.get(timing[0]) if timing and not (timing[0] == 'fft_ypass' and 'true' in name) else None
         gbs = f'{b / (avg * 1e-6) / 1e9:9.1f}' if b else f'{"":9s}'
         mb = f'{b / 1e6:9.1f}' if b else f'{"":9s}'
@@ -392,6 +407,18 @@ def run_window256(args):
     top = max((k for k in kernels if k in alg), key=lambda k: kernels[k]['share'])
     ncol = wm.value().shape[1]
     value = wm.value()
+    cpu = None
+    if not args.no_cpu_baseline:  # float64 oracle, one core: 2 input shells x 3 ellsin of the same matrix
+        from oracle import power_oracle as po
+        oattrs = po.Attrs(n, boxsize, 0.)
+        obin = po.BinPoles(oattrs, edges={'step': STEP}, ells=ELLS)
+        host_sel = sel.value.cpu().numpy().astype('f8')
+        t0 = time.perf_counter()
+        po.spectrum_window(oattrs, obin, edgesin[:3], ellsin=ellsin, meshes=[host_sel], los='x')
+        dt = time.perf_counter() - t0
+        cpu = {'value': round(6 / dt, 4), 'unit': 'columns/s', 'cores': 1, 'kind': 'port',
+               'sample': f'float64 NumPy oracle, 1 core: 6 columns (2 input shells x ellin 0, 2, 4) of the same 256^3 '
+                         f'matrix, including the selection product once, {dt:.1f} s'}
     result = {'metric': 'window-matrix columns/s at 256^3 (global line of sight)', 'value': round(ncol * args.steps / elapsed, 2),
               'unit': 'columns/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
               'ms_per_step': round(elapsed * 1e3 / args.steps, 3), 'higher_is_better': True, 'scaling': 'weak',
@@ -403,8 +430,8 @@ def run_window256(args):
               'roofline': {'kernel': top, 'bound': 'hbm', 'achieved': kernels[top]['achieved_GBs'], 'peak': HBM_PEAK_GBS,
                            'unit': 'GB/s', 'frac': round(kernels[top]['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': None,
                            'algorithmic_bytes_per_launch': alg[top], 'avg_ms': kernels[top]['avg_ms']},
-              'kernels': kernels,
-              'check': {'finite': bool(np.all(np.isfinite(value[:, :]))), 'w00_diag_max': float(np.abs(value).max())}}
+              'cpu_baseline': cpu, 'kernels': kernels,
+              'check': {'finite': bool(np.all(np.isfinite(value[:, :]))), 'w_max': float(np.abs(value).max())}}
     print(json.dumps(result))
 
 
@@ -459,6 +486,12 @@ def run_batch256(args):
 
 def main():
     args = parse()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, workload=args.workload)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_bytes)
+        return
     if args.workload == 'batch256':
         if args.gpus != 1:
             print('bench.py runs on one GPU only.', file=sys.stderr)
@@ -482,12 +515,6 @@ def main():
             print('bench.py runs on one GPU only.', file=sys.stderr)
             sys.exit(2)
         run_fkp1024(args)
-        return
-    if args.summarize_trace:
-        summarize_trace(args.summarize_trace)
-        return
-    if args.summarize_pmc:
-        summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_bytes)
         return
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); nothing is measured for more GPUs.', file=sys.stderr)

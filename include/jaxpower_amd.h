@@ -252,6 +252,21 @@ int mp_window_bin(void* stream, const float* Q, const int32_t meshsize[3], const
                   const double* k2_hi, const int32_t* ellin, const double los[3], float scale, int nell, const int32_t* ells,
                   double* out, void* workspace, size_t workspace_bytes);
 
+/* compute_mesh2_spectrum_window(flags=('infinite',)) with a global line of sight
+ * (jaxpower/mesh2.py:1078-1092): column c takes the input shell [k0[c], k1[c]) of multipole
+ * ellin[c] (HOST arrays; edges, not thresholds) to configuration space,
+ * A(s) = xi_ellin(|s|) P_ellin(s.los/|s|) scale (times the real mesh Q unless NULL), with
+ * xi_ell = BesselIntegral(..., method='exact', mode='backward').w (jaxpower/utils.py:470-501),
+ * then bins r2c(A) as mp_window_bin does.  Equal cellsizes, power-of-two meshsizes, nell <= 6. */
+int64_t mp_window_tophat_workspace_bytes(const int32_t meshsize[3], int nell, int nedges);
+int mp_window_bin_tophat(void* stream, const float* Q, const int32_t meshsize[3], double cellsize, const double* kx,
+                         const double* ky, const double* kz, const double* k2_thresholds, int nedges, int ncol,
+                         const double* k0, const double* k1, const int32_t* ellin, const double los[3], double scale,
+                         int nell, const int32_t* ells, double* out, void* workspace, size_t workspace_bytes);
+/* Host-side value of xi_ell(s) for the shell [k0, k1), ell in 0..5 (NaN otherwise): the
+ * configuration-space weights above, for checking them without a device. */
+double mp_tophat_shell_xi(int ell, double k0, double k1, double s);
+
 /* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
  *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */
 int64_t mp_reduce_workspace_bytes(void);

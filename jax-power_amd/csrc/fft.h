@@ -8,18 +8,35 @@ namespace mp {
 
 constexpr int TW_N = 4096;  // twiddle table length (largest supported transform length is 2048)
 
-enum { PRO_NONE = 0, PRO_YLM = 1 };
+enum { PRO_NONE = 0, PRO_YLM = 1, PRO_TOPHAT = 2 };
 
 // Real-space prologue of the z pass: multiplies each real input by Y_lm(xhat) of the node
-// position (local line of sight, jaxpower/mesh2.py:319-323), or by 1.
+// position (local line of sight, jaxpower/mesh2.py:319-323), by the configuration-space shell
+// of the window matrix, ttab[|n|^2] P_ell(n.los/|n|) with n the separation index vector
+// (fftfreq order, equal cellsizes; jaxpower/mesh2.py:1078-1092), or by 1.
 struct RealPrologue {
-  int kind;  // PRO_NONE or PRO_YLM
+  int kind;  // PRO_NONE, PRO_YLM or PRO_TOPHAT
   int ell, m;
   int ny, nz;
   float x0, y0, z0;  // position of node (0,0,0): boxcenter - boxsize/2
   float dx, dy, dz;  // cellsize
+  // PRO_TOPHAT
+  int nx;
+  const float* ttab;
+  float lx, ly, lz;
+  LegendreTable leg;
   __device__ __forceinline__ float eval(int64_t row, int z) const {
     int ix = (int)(row / ny), iy = (int)(row % ny);
+    if (kind == PRO_TOPHAT) {
+      const int fx = ix < (nx + 1) / 2 ? ix : ix - nx, fy = iy < (ny + 1) / 2 ? iy : iy - ny, fz = z < (nz + 1) / 2 ? z : z - nz;
+      const int n2 = fx * fx + fy * fy + fz * fz;
+      float v = ttab[n2];
+      if (ell) {
+        const float mu = n2 ? ((float)fx * lx + (float)fy * ly + (float)fz * lz) * rsqrtf((float)n2) : 0.f;
+        v *= legendre_eval(leg, 0, mu, mu * mu);
+      }
+      return v;
+    }
     float px = x0 + dx * ix, py = y0 + dy * iy, pz = z0 + dz * z;
     return ylm_of_vector(ell, m, px, py, pz);
   }

@@ -11,6 +11,13 @@
 // space, multiplies by the row of Q and transforms forward again without writing the real row,
 // the forward y pass runs in place, and the forward x pass bins Re(B) P_ell in FP64 without
 // writing B.  mp_window_shell writes A (periodic box) or B as a half spectrum instead.
+//
+// With an infinite-resolution theory (flags=('infinite',), mesh2.py:1078-1092) each column is
+// the real mesh xi_ellin(|s|) P_ellin(mu_s) V (times Q), transformed and binned:
+// mp_window_bin_tophat tabulates xi per integer |n|^2 (equal cellsizes: |s| = cell |n|), the
+// z pass generates the real rows from the table and multiplies them by Q as it loads them, then
+// the y pass and the binning x pass follow.
+#include "bessel.h"
 #include "bin.h"
 #include "common.h"
 #include "fft.h"
@@ -73,6 +80,101 @@ MP_EXPORT int mp_window_shell(void* stream, const float* Q, const int32_t meshsi
             out_pitch);
   if (!Q) return MP_OK;
   return convolve_real(s, o, out_pitch, Q, meshsize, 1.f);
+}
+
+namespace mp {
+
+constexpr int TT_NT = 256;
+
+// ttab[n2] = xi_ell(cell sqrt(n2)) scale for n2 = 0 .. n2max (the separation shell of the
+// input bin [k0, k1)).
+__global__ void __launch_bounds__(TT_NT)
+tophat_table_kernel(int ell, double k0, double k1, double cell, double scale, int n2max, float* __restrict__ ttab) {
+  const int i = blockIdx.x * TT_NT + threadIdx.x;
+  if (i > n2max) return;
+  ttab[i] = (float)(tophat_shell_xi(ell, k0, k1, cell * sqrt((double)i)) * scale);
+}
+
+static int64_t tophat_n2max(const int32_t n[3]) {
+  int64_t m = 0;
+  for (int a = 0; a < 3; ++a) m += (int64_t)(n[a] / 2) * (n[a] / 2);
+  return m;
+}
+
+}  // namespace mp
+
+// Host evaluation of the configuration-space shell xi_ell(s) of [k0, k1) used by
+// mp_window_bin_tophat (the BesselIntegral weights of jaxpower/utils.py:470-501).
+MP_EXPORT double mp_tophat_shell_xi(int ell, double k0, double k1, double s) {
+  if (ell < 0 || ell > 5) return NAN;
+  return tophat_shell_xi(ell, k0, k1, s);
+}
+
+MP_EXPORT int64_t mp_window_tophat_workspace_bytes(const int32_t meshsize[3], int nell, int nedges) {
+  if (!meshsize || nell < 1 || nedges < 2) return 0;
+  const size_t spec = align_up((size_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * sizeof(float2), 256);
+  const size_t tab = align_up((size_t)(tophat_n2max(meshsize) + 1) * sizeof(float), 256);
+  return (int64_t)(spec + tab + bin_partial_workspace_bytes(meshsize, 1, nell, nedges) + 512);
+}
+
+// Columns of the window matrix with an infinite-resolution theory: column c is the input shell
+// [k0[c], k1[c]) of multipole ellin[c] (three HOST arrays; edges, not thresholds), taken to
+// configuration space as xi_ellin(|s|) P_ellin(s.los/|s|) scale (times the real mesh Q when not
+// NULL), then r2c and binned as in mp_window_bin: out[c][p][b][re, im].  Equal cellsizes on the
+// three axes, power-of-two meshsizes.
+MP_EXPORT int mp_window_bin_tophat(void* stream, const float* Q, const int32_t meshsize[3], double cellsize, const double* kx,
+                                   const double* ky, const double* kz, const double* k2_thresholds, int nedges, int ncol,
+                                   const double* k0, const double* k1, const int32_t* ellin, const double los[3], double scale,
+                                   int nell, const int32_t* ells, double* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && k2_thresholds && ells && out && los && k0 && k1 && ellin, "mp_window_bin_tophat: null pointer");
+  MP_REQUIRE(ncol >= 0 && nell >= 1 && nell <= 6 && nedges >= 2, "mp_window_bin_tophat: bad sizes");
+  MP_REQUIRE(cellsize > 0., "mp_window_bin_tophat: cellsize %g <= 0", cellsize);
+  for (int c = 0; c < ncol; ++c) MP_REQUIRE(ellin[c] >= 0 && ellin[c] <= 5, "mp_window_bin_tophat: ellin %d not in 0..5", ellin[c]);
+  if (!xpass_supported(meshsize))
+    return set_error(MP_ERR_UNSUPPORTED, "mp_window_bin_tophat: meshsize (%d, %d, %d) not supported", meshsize[0], meshsize[1],
+                     meshsize[2]);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  const int64_t n2max = tophat_n2max(meshsize);
+  Workspace ws(workspace, workspace_bytes);
+  float2* spec = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  float* ttab = ws.take<float>((size_t)n2max + 1);
+  const size_t rest = workspace_bytes > ws.used ? workspace_bytes - ws.used : 0;
+  if (!spec || !ttab || rest < bin_partial_workspace_bytes(meshsize, 1, nell, nedges))
+    return set_error(MP_ERR_WORKSPACE, "mp_window_bin_tophat: workspace of %zu bytes < %lld", workspace_bytes,
+                     (long long)mp_window_tophat_workspace_bytes(meshsize, nell, nedges));
+  BinGeom geo{};
+  geo.n0 = meshsize[0];
+  geo.n1 = meshsize[1];
+  geo.ncols = meshsize[2] / 2 + 1;
+  geo.nedges = nedges;
+  geo.kx = kx;
+  geo.ky = ky;
+  geo.kz = kz;
+  geo.thr = k2_thresholds;
+  int32_t rz[6] = {1, 1, 1, 1, 1, 1};
+  char* base = static_cast<char*>(workspace) + ws.used;
+  const int64_t per_col = (int64_t)nell * (nedges - 1) * 2;
+  for (int c = 0; c < ncol; ++c) {
+    MP_LAUNCH("window_tophat_table", tophat_table_kernel, dim3((unsigned)ceil_div(n2max + 1, TT_NT)), dim3(TT_NT), 0, s, ellin[c],
+              k0[c], k1[c], cellsize, scale, (int)n2max, ttab);
+    RealPrologue pro{};
+    pro.kind = PRO_TOPHAT;
+    pro.ell = ellin[c];
+    pro.nx = meshsize[0];
+    pro.ny = meshsize[1];
+    pro.nz = meshsize[2];
+    pro.ttab = ttab;
+    pro.lx = (float)los[0];
+    pro.ly = (float)los[1];
+    pro.lz = (float)los[2];
+    legendre_table_set(pro.leg, 0, ellin[c]);
+    int rc = rfft3d_zy(s, Q, spec, meshsize, sp, pro);
+    if (rc) return rc;
+    rc = bin_partial(s, spec, sp, meshsize, 0, 1, 0, 0, geo, los, nell, ells, rz, out + c * per_col, base, rest, 1);
+    if (rc) return rc;
+  }
+  return MP_OK;
 }
 
 MP_EXPORT int64_t mp_window_bin_workspace_bytes(const int32_t meshsize[3], int nell, int nedges) {

@@ -525,14 +525,18 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
     (nothing for a periodic box, given as a MeshAttrs), then binned as
     (2l+1)/(1+odd) bin(A (P_l(mu) + odd P_l(-mu)), remove_zero=True) N^3/V_cell / norm / N^3.
     The real part of every row is kept (mesh2.py:1466-1472), and odd output multipoles vanish.
+    With flags=('infinite',) each input shell is taken to configuration space analytically,
+    A = BesselIntegral(shell, |s|, ellin).w P_ellin(mu_s) V, times Q for a selection, then r2c
+    and binned the same way (mesh2.py:1078-1092).
     ``buffer``, ``batch_size`` and ``pbar`` only steer the reference's memory use and progress
-    display and are accepted for compatibility; the flags 'smooth' and 'infinite' and the local
-    line of sight are not implemented on this path."""
+    display and are accepted for compatibility; the flag 'smooth' and the local line of sight
+    are not implemented on this path."""
     if not isinstance(bin, BinMesh2SpectrumPoles):
         raise ValueError(f'bin must be BinMesh2SpectrumPoles, not {type(bin)}')
     flags = tuple(flags or ())
-    if flags:
-        raise NotImplementedError(f'window flags {flags} are not implemented on this path (mesh-defined theory only)')
+    infinite = 'infinite' in flags
+    if set(flags) - {'infinite'}:
+        raise NotImplementedError(f'window flags {flags} are not implemented on this path (mesh-defined or infinite theory only)')
     periodic = isinstance(meshes[0], MeshAttrs)
     if periodic:
         mattrs = meshes[0]
@@ -566,6 +570,9 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
         Q = _c2r(P, mattrs.meshsize)
         del P
         device = Q.device
+    if infinite:  # the theory shells taken to configuration space analytically, mesh2.py:1078-1092
+        wmat = _window_infinite(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device)
+        return _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm)
     # shell membership e0 <= |k| < e1 as k^2 >= t(e0) and k^2 < t(e1), t as for the bin edges
     thr = _k2_thresholds(edgesin.ravel()).reshape(edgesin.shape)
     scale = n3 if Q is None else 1.  # the N^3 of A cancels the 1/N^3 of Q
@@ -604,7 +611,49 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
         for ie, ell in enumerate(even):
             ill = ells.index(ell)
             wmat[:, ill, :] = (2 * ell + 1) * rnorm[ill] * sums[:, ie, :].real / bin.nmodes
-    wmat = wmat.reshape(ncol, -1).T
+    return _window_matrix(wmat.reshape(ncol, -1).T, bin, edgesin, ellsin, vlos, norm)
+
+
+def _window_infinite(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
+    """(n_ells n_bins, n_ellsin n_in) matrix of the infinite-resolution theory
+    (mesh2.py:1078-1092) through mp_window_bin_tophat."""
+    cell = np.asarray(mattrs.cellsize, dtype='f8')
+    if not np.all(cell == cell[0]):
+        raise NotImplementedError("window flags=('infinite',) needs equal cellsizes on this path")
+    if not all(int(m) >= 2 and int(m) & (int(m) - 1) == 0 for m in mattrs.meshsize):
+        raise NotImplementedError("window flags=('infinite',) needs power-of-two meshsizes on this path")
+    ells = bin.ells
+    even = [ell for ell in ells if ell % 2 == 0]
+    nbins, nedges = bin.nbins, len(bin._edges1d)
+    ncol = len(ellsin) * len(edgesin)
+    wmat = np.zeros((ncol, len(ells), nbins), dtype='f8')
+    if even:
+        if len(even) > 6 or len(even) * (nedges - 1) > 4096:
+            raise NotImplementedError('too many multipoles or bins for the window binning pass')
+        n = _lib.i32x3(mattrs.meshsize)
+        volume = float(np.prod(mattrs.boxsize, dtype='f8'))
+        scale = volume if Q is None else volume / float(np.prod(mattrs.meshsize, dtype='f8'))  # Q = c2r(A1 conj(A2)) / N^3
+        ws = _lib.workspace(_lib.load().mp_window_tophat_workspace_bytes(n, len(even), nedges), device)
+        out = torch.empty((ncol, len(even), nbins, 2), dtype=torch.float64, device=device)
+        dbl = lambda v: (_lib.ctypes.c_double * ncol)(*[float(x) for x in v])
+        t = bin._tables
+        _lib.call('mp_window_bin_tophat', _lib.stream_handle(device), _lib.ptr(Q) if Q is not None else None, n, float(cell[0]),
+                  _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), nedges, ncol,
+                  dbl(list(edgesin[:, 0]) * len(ellsin)), dbl(list(edgesin[:, 1]) * len(ellsin)),
+                  (_lib.ctypes.c_int32 * ncol)(*[ellin for ellin in ellsin for _ in edgesin]), _lib.f64x3(vlos), scale, len(even),
+                  (_lib.ctypes.c_int32 * len(even))(*even), _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        sums = out[..., 0].cpu().numpy()
+        with np.errstate(invalid='ignore', divide='ignore'):
+            for ie, ell in enumerate(even):
+                ill = ells.index(ell)
+                wmat[:, ill, :] = (2 * ell + 1) * rnorm[ill] * sums[:, ie, :] / bin.nmodes
+    return wmat.reshape(ncol, -1).T
+
+
+def _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm):
+    """WindowMatrix container of mesh2.py:1463-1482."""
+    ells = bin.ells
+    nbins = bin.nbins
     attrs = dict(los=vlos.tolist())
     zeros = np.zeros(nbins)
     observable = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=zeros,

@@ -889,3 +889,78 @@ def convolve_bruteforce(Q, A_full):
         shifted = Qhat[np.ix_((idx[0] - q[0]) % n[0], (idx[1] - q[1]) % n[1], (idx[2] - q[2]) % n[2])]
         out += shifted * A_full[q]
     return out / np.prod(n)
+
+
+def tophat_bessel(ell, x):
+    """F_ell(x) = x^-3 int_0^x t^2 j_ell(t) dt by the reference's closed forms for x > 0.1 and
+    its low-x series otherwise (utils.py:403-415, 453-462); Si from scipy.special.sici
+    (utils.py:370-372)."""
+    x = np.asarray(x, dtype='f8')
+    si = special.sici(np.where(x > 0.1, x, 1.))[0]
+    c, s = np.cos(x), np.sin(x)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        big = {0: lambda: (-c / x + s / x**2) / x,
+               1: lambda: (-s - 2 * c / x) / x**2 + 2 / x**3,
+               2: lambda: (x * c - 4 * s + 3 * si) / x**3,
+               3: lambda: 8 / x**3 + (x**2 * s + 7 * x * c - 15 * s) / x**4,
+               4: lambda: (-x**3 * c + 11 * x**2 * s + 15 * x**2 * si / 2 + 105 * x * c / 2 - 105 * s / 2) / x**5,
+               5: lambda: 16 / x**3 + (-x**4 * s - 16 * x**3 * c + 105 * x**2 * s + 315 * x * c - 315 * s) / x**6}[ell]()
+    small = {0: -x**10 / 518918400 + x**8 / 3991680 - x**6 / 45360 + x**4 / 840 - x**2 / 30 + 1 / 3,
+             1: x**9 / 47900160 - x**7 / 453600 + x**5 / 6720 - x**3 / 180 + x / 12,
+             2: x**10 / 674593920 - x**8 / 5488560 + x**6 / 68040 - x**4 / 1470 + x**2 / 75,
+             3: -x**9 / 77837760 + x**7 / 831600 - x**5 / 15120 + x**3 / 630,
+             4: -x**10 / 1264863600 + x**8 / 11891880 - x**6 / 187110 + x**4 / 6615,
+             5: x**9 / 194594400 - x**7 / 2702700 + x**5 / 83160}[ell]
+    return np.where(x > 0.1, big, small)
+
+
+def tophat_shell_xi(ell, k0, k1, s):
+    """BesselIntegral(edges=(k0, k1), s, ell, method='exact', mode='backward').w
+    (utils.py:470-501): (-1)^(ell//2) / (2 pi)^3 * 4 pi [F(k1 s) k1^3 - F(k0 s) k0^3]."""
+    return (-1)**(ell // 2) / (2. * np.pi)**3 * 4. * np.pi * (tophat_bessel(ell, k1 * s) * k1**3 - tophat_bessel(ell, k0 * s) * k0**3)
+
+
+def spectrum_window_infinite(attrs, bin, edgesin, ellsin=0, meshes=None, los='z', norm=None):
+    """compute_mesh2_spectrum_window(flags=('infinite',)) with a global line of sight
+    (mesh2.py:1078-1092): for each input shell the theory is taken to configuration space
+    analytically, A = BesselIntegral(shell, |s|, ellin).w P_ellin(mu_s) V (s the separation
+    vectors, mu_s = 0 at s = 0), multiplied by Q for a selection, then binned as in
+    spectrum_window after r2c."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    ellsin = (ellsin,) if np.ndim(ellsin) == 0 else tuple(ellsin)
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    vlos = los_vector(los)
+    Q = None
+    if meshes is not None:
+        cm = [m if np.iscomplexobj(m) else r2c(m) for m in meshes]
+        Q = c2r(cm[0] * np.conj(cm[-1]), attrs.meshsize) / n3
+    sx, sy, sz = separation_coords(attrs)
+    sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
+    sn = np.sqrt(sx**2 + sy**2 + sz**2)
+    smu = (sx * vlos[0] + sy * vlos[1] + sz * vlos[2]) / np.where(sn == 0., 1., sn)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    kn = knorm(attrs)
+    mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / np.where(kn == 0., 1., kn)
+    volume = np.prod(attrs.boxsize)
+    cols = []
+    for ellin in ellsin:
+        legin = legendre(ellin, smu)
+        for e0, e1 in edgesin:
+            A = tophat_shell_xi(ellin, e0, e1, sn) * legin * volume
+            if Q is not None:
+                A = A * Q
+            A = r2c(A)
+            col = []
+            for ill, ell in enumerate(ells):
+                odd = ell % 2
+                leg = legendre(ell, mu) + (legendre(ell, -mu) if odd else 0.)
+                col.append((2 * ell + 1) / (1 + odd) * bin(A * leg, remove_zero=True) * rnorm[ill])
+            cols.append(np.concatenate(col))
+    return np.real(np.array(cols).T)

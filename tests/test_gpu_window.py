@@ -119,7 +119,7 @@ def test_window_unsupported_raise():
     with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=0)
     with pytest.raises(NotImplementedError):
-        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('infinite',))
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('smooth',))
 
 
 def test_window_fused_column_matches_written_spectrum():
@@ -157,3 +157,26 @@ def test_window_fused_column_matches_written_spectrum():
         floor = 1e-6 * np.abs(ref.real).max()
         assert np.all(np.abs(got - ref.real) <= 1e-6 * np.abs(ref.real) + floor), (ellin, np.abs(got - ref.real).max(), floor)
         assert np.all(host(out)[..., 1] == 0.)
+
+
+@pytest.mark.parametrize('periodic,los', [(True, 'z'), (False, 'x'), (False, [0.36, 0.48, 0.8])])
+def test_window_infinite(periodic, los):
+    """flags=('infinite',) (mesh2.py:1078-1092): configuration-space shells from the closed-form
+    Bessel integrals, times the selection, r2c and binned, against the oracle."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window infinite', periodic, los)
+    attrs = MeshAttrs(meshsize=32, boxsize=500., boxcenter=1000.)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    edgesin = np.arange(0.01, 0.2, 0.03)
+    ellsin = (0, 2, 4)
+    oattrs = po.Attrs(32, 500., 1000.)
+    obin = po.BinPoles(oattrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    if periodic:
+        wm = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('infinite',))
+        want = po.spectrum_window_infinite(oattrs, obin, edgesin, ellsin=ellsin, los=los)
+    else:
+        pos, w = _survey_selection(attrs, 200000, 11)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('infinite',))
+        want = po.spectrum_window_infinite(oattrs, obin, edgesin, ellsin=ellsin, meshes=[host(sel.value).astype('f8')], los=los)
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='infinite')

@@ -192,3 +192,19 @@ def test_load_reference_style_file_needs_allow_pickle(tmp_path):
         RealMeshField.load(fn, device='cpu')
     back = RealMeshField.load(fn, device='cpu', allow_pickle=True)
     assert np.array_equal(np.asarray(back), value)
+
+
+def test_tophat_shell_xi_host_matches_oracle():
+    """The library's float64 configuration-space shell (closed-form Bessel integrals, its own
+    sine integral) against the oracle's (scipy's sici), across both sides of x = 0.1 and the
+    sine-integral series/asymptotic switch at x = 20."""
+    from jaxpower_amd import _lib
+    from oracle import power_oracle as po
+    lib = _lib.load()
+    k0, k1 = 0.05, 0.08
+    for ell in range(6):
+        for s in np.concatenate([[0., 0.5, 1.3], np.geomspace(2., 2000., 57), [250., 251., 400.]]):
+            got = lib.mp_tophat_shell_xi(ell, k0, k1, float(s))
+            want = float(po.tophat_shell_xi(ell, k0, k1, s))
+            scale = k1**3 / (2. * np.pi)**3 * 4. * np.pi * 1e-9  # ~1e-9 of the shell's peak value
+            assert abs(got - want) <= 1e-9 * abs(want) + scale * (1. if ell <= 2 else 1e3), (ell, s, got, want)

@@ -583,3 +583,39 @@ def test_window_constant_selection_is_periodic():
     _norm = np.prod(a.meshsize) / np.prod(a.cellsize)
     np.testing.assert_allclose(W2, W0 * _norm / norm, rtol=1e-13)
     assert np.abs(np.diag(W0[:len(bin.nmodes), :])).max() > 0.5
+
+
+@pytest.mark.parametrize('ell', [0, 1, 2, 3, 4, 5])
+def test_tophat_bessel_vs_quadrature(ell):
+    """The closed forms and low-x series of utils.py:403-415 are x^-3 int_0^x t^2 j_ell(t) dt
+    (numerical quadrature of scipy's spherical Bessel functions), on both sides of the x = 0.1
+    switch.  Just above the switch the reference's closed forms cancel catastrophically for
+    ell >= 3 (terms ~x^-(ell+1) summing to ~x^ell), which the oracle reproduces: below x = 3
+    the check is loose (1e-6) for ell >= 3, and at x = 0.1001 it is skipped for ell >= 4."""
+    from scipy import integrate
+    cases = [(x, 1e-9 if (x < 0.1 or x >= 3. or ell <= 2) else 1e-6) for x in (0.03, 0.0999, 0.7, 3., 11.3, 40., 95.)]
+    if ell <= 3:
+        cases.append((0.1001, 1e-9 if ell <= 2 else 1e-6))
+    for x, rtol in cases:
+        want = integrate.quad(lambda t: t**2 * special.spherical_jn(ell, t), 0., x, limit=400, epsabs=1e-14, epsrel=1e-12)[0] / x**3
+        got = po.tophat_bessel(ell, x)
+        assert abs(got - want) <= rtol * max(abs(want), 1e-6 / x**2), (ell, x, got, want)
+
+
+def test_window_infinite_close_to_mesh_window():
+    """The configuration-space (infinite-resolution) theory of a well-resolved shell, sampled on
+    the mesh and binned, must nearly reproduce the mesh-defined window on the shell's own bins:
+    1 for ell = ellin = 0 and ell = ellin = 2 (a sign error in (-1)^(ell/2), a missing 4 pi or
+    (2 pi)^3 fails this), and little leakage from ellin = 0 into ell = 2."""
+    a = po.Attrs(48, 1000., 0.)
+    kf = 2. * np.pi / 1000.
+    bin = po.BinPoles(a, edges=np.arange(0., 0.13, 4 * kf), ells=(0, 2))
+    edgesin = np.array([[12 * kf, 16 * kf]])
+    nb = len(bin.nmodes)
+    Wi = po.spectrum_window_infinite(a, bin, edgesin, ellsin=(0, 2), los='z')
+    Wm = po.spectrum_window(a, bin, edgesin, ellsin=(0, 2), los='z')
+    b = 3  # the output bin [12 kf, 16 kf)
+    assert abs(Wm[b, 0] - 1.) < 1e-12 and abs(Wm[nb + b, 1] - 1.) < 1e-2  # lattice average of 5 P_2^2
+    assert abs(Wi[b, 0] - 1.) < 0.05, Wi[b, 0]
+    assert abs(Wi[nb + b, 1] - 1.) < 0.08, Wi[nb + b, 1]
+    assert abs(Wi[nb + b, 0]) < 0.05
