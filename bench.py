@@ -178,7 +178,9 @@ def summarize_trace(folder, out=TRACE_FILE, workload='box512'):
         lines.append(f'{name[:60]:60s} {int(r["Calls"]):6d} {avg:10.1f} {float(r["TotalDurationNs"]) * 1e-6:10.3f} '
                      f'{float(r["Percentage"]):6.2f} {mb} {gbs}')
     lines.append(f'# other kernels (torch and rocPRIM, mock generation at set-up): {other[0]} launches, {other[1]:.3f} ms')
-    lines.append('# bin_modes and the plain z-pass/c2r kernels run once at set-up (bin tables, Gaussian mock)')
+    lines.append('# bin_modes and the plain z-pass/c2r kernels run once at set-up (bin tables, Gaussian mock)' if workload != 'window256'
+                 else '# bin_modes runs once at set-up (bin tables); the plain z-pass r2c/c2r and spectrum_product are the selection\'s '
+                      'Q = c2r(|A|^2) once per step')
     with open(out, 'w') as fh:
         fh.write('\n'.join(lines) + '\n')
     print('\n'.join(lines))
