@@ -267,6 +267,12 @@ int mp_window_bin_tophat(void* stream, const float* Q, const int32_t meshsize[3]
  * configuration-space weights above, for checking them without a device. */
 double mp_tophat_shell_xi(int ell, double k0, double k1, double s);
 
+/* Host arrays, for the smooth window (compute_mesh2_spectrum_window(flags=('smooth',)),
+ * jaxpower/mesh2.py:1045-1076): out[i] = xi_ell(s[i]) of the shell [k0, k1) as above, and
+ * out[i] = j_ell(x[i]) by the closed forms / series of jaxpower/utils.py:418-450; ell in 0..5. */
+int mp_tophat_shell_xi_array(int ell, double k0, double k1, const double* s, int64_t n, double* out);
+int mp_spherical_jn_array(int ell, const double* x, int64_t n, double* out);
+
 /* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
  *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */
 int64_t mp_reduce_workspace_bytes(void);

@@ -66,6 +66,32 @@ __host__ __device__ inline double tophat_bessel(int ell, double x) {
   }
 }
 
+// Spherical Bessel function j_ell(x), ell = 0..5: the closed forms for x > 0.1 and the Taylor
+// series below, the split of jaxpower/utils.py:418-450.
+__host__ __device__ inline double spherical_jn(int ell, double x) {
+  if (x > 0.1) {
+    const double s = sin(x), c = cos(x);
+    const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+    switch (ell) {
+      case 0: return s / x;
+      case 1: return -c / x + s / x2;
+      case 2: return (-1. / x + 3. / x3) * s - 3. * c / x2;
+      case 3: return (-6. / x2 + 15. / x4) * s + (1. / x - 15. / x3) * c;
+      case 4: return (10. / x2 - 105. / x4) * c + (1. / x - 45. / x3 + 105. / (x4 * x)) * s;
+      default: return (15. / x2 - 420. / x4 + 945. / (x4 * x2)) * s + (-1. / x + 105. / x3 - 945. / (x4 * x)) * c;
+    }
+  }
+  const double x2 = x * x;
+  switch (ell) {
+    case 0: return 1. + x2 * (-1. / 6. + x2 * (1. / 120. + x2 * (-1. / 5040. + x2 * (1. / 362880. - x2 / 39916800.))));
+    case 1: return x * (1. / 3. + x2 * (-1. / 30. + x2 * (1. / 840. + x2 * (-1. / 45360. + x2 / 3991680.))));
+    case 2: return x2 * (1. / 15. + x2 * (-1. / 210. + x2 * (1. / 7560. + x2 * (-1. / 498960. + x2 / 51891840.))));
+    case 3: return x * x2 * (1. / 105. + x2 * (-1. / 1890. + x2 * (1. / 83160. - x2 / 6486480.)));
+    case 4: return x2 * x2 * (1. / 945. + x2 * (-1. / 20790. + x2 * (1. / 1081080. - x2 / 97297200.)));
+    default: return x * x2 * x2 * (1. / 10395. + x2 * (-1. / 270270. + x2 / 16216200.));
+  }
+}
+
 // xi_ell(s) of the shell [k0, k1), without the volume factor.
 __host__ __device__ inline double tophat_shell_xi(int ell, double k0, double k1, double s) {
   const double sign = ((ell / 2) & 1) ? -1. : 1.;

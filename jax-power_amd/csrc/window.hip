@@ -110,6 +110,23 @@ MP_EXPORT double mp_tophat_shell_xi(int ell, double k0, double k1, double s) {
   return tophat_shell_xi(ell, k0, k1, s);
 }
 
+// Host arrays: out[i] = xi_ell(s[i]) of the shell [k0, k1) (as mp_tophat_shell_xi), and
+// out[i] = j_ell(x[i]) (jaxpower/utils.py:418-450), ell in 0..5; the Hankel sums of the smooth
+// window (jaxpower/mesh2.py:1057-1068) are formed from these on the host.
+MP_EXPORT int mp_tophat_shell_xi_array(int ell, double k0, double k1, const double* s, int64_t n, double* out) {
+  MP_REQUIRE(ell >= 0 && ell <= 5, "mp_tophat_shell_xi_array: ell %d not in 0..5", ell);
+  MP_REQUIRE(n >= 0 && (n == 0 || (s && out)), "mp_tophat_shell_xi_array: null pointer");
+  for (int64_t i = 0; i < n; ++i) out[i] = tophat_shell_xi(ell, k0, k1, s[i]);
+  return MP_OK;
+}
+
+MP_EXPORT int mp_spherical_jn_array(int ell, const double* x, int64_t n, double* out) {
+  MP_REQUIRE(ell >= 0 && ell <= 5, "mp_spherical_jn_array: ell %d not in 0..5", ell);
+  MP_REQUIRE(n >= 0 && (n == 0 || (x && out)), "mp_spherical_jn_array: null pointer");
+  for (int64_t i = 0; i < n; ++i) out[i] = spherical_jn(ell, x[i]);
+  return MP_OK;
+}
+
 MP_EXPORT int64_t mp_window_tophat_workspace_bytes(const int32_t meshsize[3], int nell, int nedges) {
   if (!meshsize || nell < 1 || nedges < 2) return 0;
   const size_t spec = align_up((size_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * sizeof(float2), 256);

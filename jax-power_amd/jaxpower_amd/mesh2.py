@@ -529,14 +529,15 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
     A = BesselIntegral(shell, |s|, ellin).w P_ellin(mu_s) V, times Q for a selection, then r2c
     and binned the same way (mesh2.py:1078-1092).
     ``buffer``, ``batch_size`` and ``pbar`` only steer the reference's memory use and progress
-    display and are accepted for compatibility; the flag 'smooth' and the local line of sight
-    are not implemented on this path."""
+    display and are accepted for compatibility.  With flags=('smooth',) the selection
+    multipoles binned in separation shells are Hankel-transformed against each input shell
+    (mesh2.py:1045-1076).  The local line of sight is not implemented on this path."""
     if not isinstance(bin, BinMesh2SpectrumPoles):
         raise ValueError(f'bin must be BinMesh2SpectrumPoles, not {type(bin)}')
     flags = tuple(flags or ())
-    infinite = 'infinite' in flags
-    if set(flags) - {'infinite'}:
-        raise NotImplementedError(f'window flags {flags} are not implemented on this path (mesh-defined or infinite theory only)')
+    infinite, smooth = 'infinite' in flags, 'smooth' in flags
+    if set(flags) - {'infinite', 'smooth'}:
+        raise NotImplementedError(f'window flags {flags} are not implemented on this path')
     periodic = isinstance(meshes[0], MeshAttrs)
     if periodic:
         mattrs = meshes[0]
@@ -570,6 +571,9 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
         Q = _c2r(P, mattrs.meshsize)
         del P
         device = Q.device
+    if smooth:  # Hankel transforms of the binned selection multipoles, mesh2.py:1045-1076
+        wmat = _window_smooth(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device)
+        return _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm)
     if infinite:  # the theory shells taken to configuration space analytically, mesh2.py:1078-1092
         wmat = _window_infinite(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device)
         return _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm)
@@ -648,6 +652,60 @@ def _window_infinite(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
                 ill = ells.index(ell)
                 wmat[:, ill, :] = (2 * ell + 1) * rnorm[ill] * sums[:, ie, :] / bin.nmodes
     return wmat.reshape(ncol, -1).T
+
+
+def _host_array_fn(name, ell, x, *args):
+    x = np.ascontiguousarray(x, dtype='f8')
+    out = np.empty_like(x)
+    _lib.call(name, int(ell), *args, x.ctypes.data, x.size, out.ctypes.data)
+    return out
+
+
+def _window_smooth(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
+    """(n_ells n_bins, n_ellsin n_in) matrix of the smooth window (mesh2.py:1045-1076).  The
+    selection (Q, or 1) times P_l(mu_s) P_lin(mu_s) binned in separation shells for every pair
+    (l, lin) is a combination of the shell sums of Q P_L(mu_s), L = |l - lin| .. l + lin (the
+    Legendre series of the product), which one binning pass of the selection provides; the
+    Hankel sums over the few hundred shells are then formed on the host."""
+    from numpy.polynomial import legendre as npleg
+    ells = bin.ells
+    if max(list(ells) + list(ellsin)) > 5:
+        raise NotImplementedError("window flags=('smooth',) supports multipoles up to 5 on this path")
+    sbin = BinMesh2CorrelationPoles(mattrs, edges=None, device=device)
+    value = Q if Q is not None else torch.ones(tuple(int(n) for n in mattrs.meshsize), dtype=torch.float32, device=device)
+    qscale = 1. / float(np.prod(mattrs.meshsize, dtype='f8')) if Q is not None else 1.  # Q = c2r(A1 conj(A2)) / N^3
+    unit = lambda ell: np.eye(ell + 1)[ell]
+    series = {(ell, ellin): npleg.legmul(unit(ell), unit(ellin)) for ellin in ellsin for ell in ells}
+    Ls = sorted({L for c in series.values() for L, v in enumerate(c) if v != 0.})
+    sums = {}
+    for i in range(0, len(Ls), 6):
+        chunk = Ls[i:i + 6]
+        out = sbin._sums_dev(value, chunk, [False] * len(chunk), los=vlos).cpu().numpy() * qscale
+        sums.update(zip(chunk, out))
+    empty = sbin.nmodes == 0
+    with np.errstate(invalid='ignore', divide='ignore'):
+        savg = np.where(empty, 0., sbin.xavg)
+        kout = np.where(bin.nmodes == 0, 0., bin.xavg)
+    snm = sbin.nmodes.astype('f8')
+    volume = float(np.prod(mattrs.boxsize, dtype='f8'))
+    jn = {ell: _host_array_fn('mp_spherical_jn_array', ell, np.outer(kout, savg)) for ell in ells}
+    blocks = []
+    for ellin in ellsin:
+        tmp = []
+        for ill, ell in enumerate(ells):
+            with np.errstate(invalid='ignore', divide='ignore'):
+                Qs = sum(v * sums[L] for L, v in enumerate(series[ell, ellin]) if v != 0.) / snm
+            if ell != 0:
+                Qs[0] = 0.
+            Qs = np.where(empty, 0., Qs)
+            rows = []
+            for e0, e1 in edgesin:
+                xi = _host_array_fn('mp_tophat_shell_xi_array', ellin, savg, float(e0), float(e1))
+                spec = (-1)**(ell // 2) * (jn[ell] @ (snm * xi * volume * Qs))
+                rows.append((2 * ell + 1) * spec * rnorm[ill])
+            tmp.append(np.array(rows))
+        blocks.append(np.concatenate(tmp, axis=-1))
+    return np.real(np.concatenate(blocks, axis=0).T)
 
 
 def _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm):

@@ -964,3 +964,72 @@ def spectrum_window_infinite(attrs, bin, edgesin, ellsin=0, meshes=None, los='z'
                 col.append((2 * ell + 1) / (1 + odd) * bin(A * leg, remove_zero=True) * rnorm[ill])
             cols.append(np.concatenate(col))
     return np.real(np.array(cols).T)
+
+
+def spherical_jn(ell, x):
+    """get_spherical_jn (utils.py:418-450): closed forms for x > 0.1, Taylor series below."""
+    x = np.asarray(x, dtype='f8')
+    s, c = np.sin(x), np.cos(x)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        big = {0: lambda: s / x,
+               1: lambda: -c / x + s / x**2,
+               2: lambda: (-1 / x + 3 / x**3) * s - 3 * c / x**2,
+               3: lambda: (-6 / x**2 + 15 / x**4) * s + (1 / x - 15 / x**3) * c,
+               4: lambda: (10 / x**2 - 105 / x**4) * c + (1 / x - 45 / x**3 + 105 / x**5) * s,
+               5: lambda: (15 / x**2 - 420 / x**4 + 945 / x**6) * s + (-1 / x + 105 / x**3 - 945 / x**5) * c}[ell]()
+    small = {0: -x**10 / 39916800 + x**8 / 362880 - x**6 / 5040 + x**4 / 120 - x**2 / 6 + 1,
+             1: x**9 / 3991680 - x**7 / 45360 + x**5 / 840 - x**3 / 30 + x / 3,
+             2: x**10 / 51891840 - x**8 / 498960 + x**6 / 7560 - x**4 / 210 + x**2 / 15,
+             3: -x**9 / 6486480 + x**7 / 83160 - x**5 / 1890 + x**3 / 105,
+             4: -x**10 / 97297200 + x**8 / 1081080 - x**6 / 20790 + x**4 / 945,
+             5: x**9 / 16216200 - x**7 / 270270 + x**5 / 10395}[ell]
+    return np.where(x > 0.1, big, small)
+
+
+def spectrum_window_smooth(attrs, bin, edgesin, ellsin=0, meshes=None, los='z', norm=None):
+    """compute_mesh2_spectrum_window(flags=('smooth',)) with a global line of sight
+    (mesh2.py:1045-1076): the selection (Q, or 1 for a periodic box) times P_ell(mu_s)
+    P_ellin(mu_s) is binned in separation shells with the default (unique) edges, the first
+    shell zeroed for ell > 0 and empty shells dropped; each input shell then gives
+    (2l+1) rnorm (-1)^(l//2) sum_s nmodes(s) j_l(k savg(s)) xi_ellin(savg(s)) V Qs(s) at the
+    output k = xavg (0 for empty output bins)."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    ellsin = (ellsin,) if np.ndim(ellsin) == 0 else tuple(ellsin)
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    vlos = los_vector(los)
+    Q = 1.
+    if meshes is not None:
+        cm = [m if np.iscomplexobj(m) else r2c(m) for m in meshes]
+        Q = c2r(cm[0] * np.conj(cm[-1]), attrs.meshsize) / n3
+    sbin = BinCorrelationPoles(attrs, edges=None)
+    with np.errstate(invalid='ignore'):
+        kout = np.where(bin.nmodes == 0, 0., bin.xavg)
+    sx, sy, sz = separation_coords(attrs)
+    sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
+    sn = np.sqrt(sx**2 + sy**2 + sz**2)
+    smu = (sx * vlos[0] + sy * vlos[1] + sz * vlos[2]) / np.where(sn == 0., 1., sn)
+    volume = np.prod(attrs.boxsize)
+    blocks = []
+    for ellin in ellsin:
+        tmp = []
+        for ill, ell in enumerate(ells):
+            Qs = sbin(Q * legendre(ell, smu) * legendre(ellin, smu))
+            if ell != 0:
+                Qs[0] = 0.
+            empty = sbin.nmodes == 0
+            Qs = np.where(empty, 0., Qs)
+            savg = np.where(empty, 0., sbin.xavg)
+            rows = []
+            for e0, e1 in edgesin:
+                tophat_Qs = tophat_shell_xi(ellin, e0, e1, savg) * volume * Qs
+                spec = (-1)**(ell // 2) * np.sum(sbin.nmodes[None, :] * spherical_jn(ell, kout[:, None] * savg[None, :]) * tophat_Qs[None, :], axis=-1)
+                rows.append((2 * ell + 1) * spec * rnorm[ill])
+            tmp.append(np.array(rows))
+        blocks.append(np.concatenate(tmp, axis=-1))
+    return np.real(np.concatenate(blocks, axis=0).T)

@@ -119,7 +119,7 @@ def test_window_unsupported_raise():
     with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=0)
     with pytest.raises(NotImplementedError):
-        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('smooth',))
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('fftlog',))
 
 
 def test_window_fused_column_matches_written_spectrum():
@@ -180,3 +180,27 @@ def test_window_infinite(periodic, los):
         wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('infinite',))
         want = po.spectrum_window_infinite(oattrs, obin, edgesin, ellsin=ellsin, meshes=[host(sel.value).astype('f8')], los=los)
     assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='infinite')
+
+
+@pytest.mark.parametrize('periodic,los', [(True, 'z'), (False, 'x'), (False, [0.36, 0.48, 0.8])])
+def test_window_smooth(periodic, los):
+    """flags=('smooth',) (mesh2.py:1045-1076): the selection times P_l P_lin binned in
+    separation shells (one GPU pass over the Legendre series of the products), Hankel-summed
+    against each input shell, against the oracle."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window smooth', periodic, los)
+    attrs = MeshAttrs(meshsize=32, boxsize=500., boxcenter=1000.)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    edgesin = np.arange(0.01, 0.2, 0.03)
+    ellsin = (0, 2, 4)
+    oattrs = po.Attrs(32, 500., 1000.)
+    obin = po.BinPoles(oattrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    if periodic:
+        wm = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('smooth',))
+        want = po.spectrum_window_smooth(oattrs, obin, edgesin, ellsin=ellsin, los=los)
+    else:
+        pos, w = _survey_selection(attrs, 200000, 13)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('smooth',))
+        want = po.spectrum_window_smooth(oattrs, obin, edgesin, ellsin=ellsin, meshes=[host(sel.value).astype('f8')], los=los)
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='smooth')

@@ -208,3 +208,23 @@ def test_tophat_shell_xi_host_matches_oracle():
             want = float(po.tophat_shell_xi(ell, k0, k1, s))
             scale = k1**3 / (2. * np.pi)**3 * 4. * np.pi * 1e-9  # ~1e-9 of the shell's peak value
             assert abs(got - want) <= 1e-9 * abs(want) + scale * (1. if ell <= 2 else 1e3), (ell, s, got, want)
+
+
+def test_spherical_jn_and_shell_arrays_host():
+    """The library's host arrays of j_ell and of the shell xi_ell (used by the smooth window)
+    against the oracle's restatement of the reference's closed forms."""
+    from jaxpower_amd import _lib
+    from jaxpower_amd.mesh2 import _host_array_fn
+    from oracle import power_oracle as po
+    _lib.load()
+    x = np.concatenate([np.linspace(0., 0.0999, 5), np.geomspace(0.2, 300., 40)])
+    for ell in range(6):
+        got = _host_array_fn('mp_spherical_jn_array', ell, x)
+        want = po.spherical_jn(ell, x)
+        # above x = 0.1 the closed forms sum terms up to ~(2l+1)!! x^-(l+1): rounding scales with them
+        dfact = np.prod(np.arange(2 * ell + 1, 0, -2, dtype='f8'))
+        atol = np.where(x > 0.1, 1e-14 * dfact / np.maximum(x, 0.1)**(ell + 1), 1e-15)
+        assert np.all(np.abs(got - want) <= 1e-10 * np.abs(want) + atol), ell
+        got = _host_array_fn('mp_tophat_shell_xi_array', ell, x * 10., 0.02, 0.05)
+        want = po.tophat_shell_xi(ell, 0.02, 0.05, x * 10.)
+        assert np.allclose(got, want, rtol=1e-8, atol=1e-9 * np.abs(want).max()), ell

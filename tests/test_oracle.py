@@ -619,3 +619,34 @@ def test_window_infinite_close_to_mesh_window():
     assert abs(Wi[b, 0] - 1.) < 0.05, Wi[b, 0]
     assert abs(Wi[nb + b, 1] - 1.) < 0.08, Wi[nb + b, 1]
     assert abs(Wi[nb + b, 0]) < 0.05
+
+
+@pytest.mark.parametrize('ell', [0, 1, 2, 3, 4, 5])
+def test_spherical_jn_vs_scipy(ell):
+    """utils.py:418-450 closed forms and series against scipy.special.spherical_jn.  Just above
+    the x = 0.1 switch the reference's closed forms cancel for ell >= 3 (terms ~x^-(ell+1)
+    summing to ~x^ell); there only the well-conditioned points are held to 1e-9."""
+    x = np.concatenate([np.linspace(0., 0.0999, 7), [0.5, 2., 9.7, 33., 150.]])
+    want = special.spherical_jn(ell, x)
+    got = po.spherical_jn(ell, x)
+    rtol = np.where((x < 0.1) | (x >= 2.) | (ell <= 3), 1e-9, 1e-5)
+    assert np.all(np.abs(got - want) <= rtol * np.abs(want) + 1e-15), (ell, np.max(np.abs(got - want)))
+
+
+def test_window_smooth_close_to_infinite():
+    """The smooth window (Hankel transform of the binned selection multipoles,
+    mesh2.py:1045-1076) approximates the infinite-resolution one: on a periodic box the
+    monopole-to-monopole block must agree within a few per cent on the shell's own bins, and
+    for a constant selection mesh Q = 1 the smooth window equals the periodic one exactly."""
+    a = po.Attrs(32, 800., 0.)
+    kf = 2. * np.pi / 800.
+    bin = po.BinPoles(a, edges=np.arange(0., 0.2, 3 * kf), ells=(0, 2))
+    edgesin = np.array([[6 * kf, 9 * kf], [9 * kf, 12 * kf]])
+    Ws = po.spectrum_window_smooth(a, bin, edgesin, ellsin=(0, 2), los='z')
+    Wi = po.spectrum_window_infinite(a, bin, edgesin, ellsin=(0, 2), los='z')
+    nb = len(bin.nmodes)
+    for col, b in ((0, 2), (1, 3)):
+        assert abs(Ws[b, col] - Wi[b, col]) < 0.08 * abs(Wi[b, col]), (col, Ws[b, col], Wi[b, col])
+    assert abs(Ws[nb + 2, 2] - Wi[nb + 2, 2]) < 0.15 * abs(Wi[nb + 2, 2])
+    W1 = po.spectrum_window_smooth(a, bin, edgesin, ellsin=(0, 2), meshes=[np.ones((32,) * 3)], los='z')
+    np.testing.assert_allclose(W1, Ws, rtol=1e-9, atol=1e-12 * np.abs(Ws).max())
