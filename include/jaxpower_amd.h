@@ -267,6 +267,20 @@ int mp_window_bin_tophat(void* stream, const float* Q, const int32_t meshsize[3]
  * configuration-space weights above, for checking them without a device. */
 double mp_tophat_shell_xi(int ell, double k0, double k1, double s);
 
+/* compute_mesh2_spectrum_window(flags=('infinite',)) with a local (firstpoint) line of sight,
+ * theory line of sight 'firstpoint', wide-angle order 0 (jaxpower/mesh2.py:1235-1260), for one
+ * output (ell, m) and input multipole ellin: column c (input shell [k0[c], k1[c]), HOST arrays)
+ * gives out[c][b][re, im] = sum over bin b (k = 0 excluded when remove_zero) of
+ * w Re(B) Y_ell,m(khat), B = r2c(xi_ellin(|s|) scale Qs), with Qs the real selection term
+ * V_cell sum_m1 Y_ellin,m1(shat) c2r(conj(r2c(rho Y_ell,m Y_ellin,m1)) A0) the caller forms with
+ * mp_mul_ylm, mp_rfft3d, mp_spectrum_product and mp_irfft3d.  Workspace:
+ * mp_window_tophat_workspace_bytes(meshsize, 1, nedges).  Equal cellsizes, power-of-two
+ * meshsizes, ell <= 4. */
+int mp_window_bin_tophat_local(void* stream, const float* Qs, const int32_t meshsize[3], double cellsize, const double* kx,
+                               const double* ky, const double* kz, const double* k2_thresholds, int nedges, int ncol,
+                               const double* k0, const double* k1, int ellin, int ell, int m, int remove_zero, double scale,
+                               double* out, void* workspace, size_t workspace_bytes);
+
 /* Host arrays, for the smooth window (compute_mesh2_spectrum_window(flags=('smooth',)),
  * jaxpower/mesh2.py:1045-1076): out[i] = xi_ell(s[i]) of the shell [k0, k1) as above, and
  * out[i] = j_ell(x[i]) by the closed forms / series of jaxpower/utils.py:418-450; ell in 0..5. */

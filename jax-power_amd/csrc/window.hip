@@ -246,3 +246,63 @@ MP_EXPORT int mp_window_bin(void* stream, const float* Q, const int32_t meshsize
   }
   return MP_OK;
 }
+
+// Local (firstpoint) line of sight with an infinite-resolution theory (jaxpower/mesh2.py:1235-1260,
+// wide-angle order 0), one output (ell, m) and one input multipole ellin: for column c (input
+// shell [k0[c], k1[c]), HOST arrays), out[c][b][re, im] = sum over the modes of bin b (mode
+// k = 0 excluded when remove_zero) of w Re(B) Y_ell,m(khat), B = r2c(xi_ellin(|s|) scale Qs),
+// Qs the real (N0, N1, N2) selection term of this (ell, m) and ellin.  Equal cellsizes,
+// power-of-two meshsizes, ell <= 4.
+MP_EXPORT int mp_window_bin_tophat_local(void* stream, const float* Qs, const int32_t meshsize[3], double cellsize, const double* kx,
+                                         const double* ky, const double* kz, const double* k2_thresholds, int nedges, int ncol,
+                                         const double* k0, const double* k1, int ellin, int ell, int m, int remove_zero,
+                                         double scale, double* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(Qs && meshsize && kx && ky && kz && k2_thresholds && out && k0 && k1, "mp_window_bin_tophat_local: null pointer");
+  MP_REQUIRE(ncol >= 0 && nedges >= 2, "mp_window_bin_tophat_local: bad sizes");
+  MP_REQUIRE(cellsize > 0., "mp_window_bin_tophat_local: cellsize %g <= 0", cellsize);
+  MP_REQUIRE(ellin >= 0 && ellin <= 5, "mp_window_bin_tophat_local: ellin %d not in 0..5", ellin);
+  MP_REQUIRE(ell >= 0 && ell <= 4 && m >= -ell && m <= ell, "mp_window_bin_tophat_local: (ell, m) = (%d, %d) not supported", ell, m);
+  if (!xpass_supported(meshsize))
+    return set_error(MP_ERR_UNSUPPORTED, "mp_window_bin_tophat_local: meshsize (%d, %d, %d) not supported", meshsize[0], meshsize[1],
+                     meshsize[2]);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  const int64_t n2max = tophat_n2max(meshsize);
+  Workspace ws(workspace, workspace_bytes);
+  float2* spec = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  float* ttab = ws.take<float>((size_t)n2max + 1);
+  const size_t rest = workspace_bytes > ws.used ? workspace_bytes - ws.used : 0;
+  if (!spec || !ttab || rest < bin_partial_workspace_bytes(meshsize, 1, 1, nedges))
+    return set_error(MP_ERR_WORKSPACE, "mp_window_bin_tophat_local: workspace of %zu bytes < %lld", workspace_bytes,
+                     (long long)mp_window_tophat_workspace_bytes(meshsize, 1, nedges));
+  BinGeom geo{};
+  geo.n0 = meshsize[0];
+  geo.n1 = meshsize[1];
+  geo.ncols = meshsize[2] / 2 + 1;
+  geo.nedges = nedges;
+  geo.kx = kx;
+  geo.ky = ky;
+  geo.kz = kz;
+  geo.thr = k2_thresholds;
+  const int32_t ells[1] = {ell}, ms[1] = {m}, rz[1] = {remove_zero ? 1 : 0};
+  const double zero_los[3] = {0., 0., 0.};
+  char* base = static_cast<char*>(workspace) + ws.used;
+  const int64_t per_col = (int64_t)(nedges - 1) * 2;
+  for (int c = 0; c < ncol; ++c) {
+    MP_LAUNCH("window_tophat_table", tophat_table_kernel, dim3((unsigned)ceil_div(n2max + 1, TT_NT)), dim3(TT_NT), 0, s, ellin, k0[c],
+              k1[c], cellsize, scale, (int)n2max, ttab);
+    RealPrologue pro{};
+    pro.kind = PRO_TOPHAT;
+    pro.ell = 0;  // isotropic: the angular part sits in Qs
+    pro.nx = meshsize[0];
+    pro.ny = meshsize[1];
+    pro.nz = meshsize[2];
+    pro.ttab = ttab;
+    legendre_table_set(pro.leg, 0, 0);
+    int rc = rfft3d_zy(s, Qs, spec, meshsize, sp, pro);
+    if (rc) return rc;
+    rc = bin_partial(s, spec, sp, meshsize, 0, 1, 0, 0, geo, zero_los, 1, ells, rz, out + c * per_col, base, rest, 1, ms);
+    if (rc) return rc;
+  }
+  return MP_OK;
+}

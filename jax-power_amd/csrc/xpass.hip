@@ -39,6 +39,8 @@ struct XPassArgs {
   int zero_dc;  // the (0,0,0) mode of A is set to 0
   int power;    // write mode: 1 stores |A|^2 (imaginary part 0) instead of A, 2 its inverse FFT along x
   int linear;   // bin mode: bins Re(A) (a linear functional of the mesh) instead of |A|^2
+  int ylm;      // bin mode: Y_lm(khat) weights instead of P_ell(mu)
+  int ylm_m[XB_MAX_PROD];  // bin mode with YLM: the weight of product p is Y_{ell[p], ylm_m[p]}(khat)
   const float2* tw;
   // write mode
   float2* out;
@@ -85,7 +87,7 @@ __device__ __forceinline__ int xcd_first_tile() {
   return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
 }
 
-template <int L, bool BIN, int NP, bool TAB>
+template <int L, bool BIN, int NP, bool TAB, bool YLM = false>
 __global__ void __launch_bounds__(XPCfg<L>::NT)
 xpass_kernel(XPassArgs a) {
   using Cfg = XPCfg<L>;
@@ -254,18 +256,27 @@ xpass_kernel(XPassArgs a) {
             const bool origin = (l == 0) & (y == 0) & (kz == 0);
             if (origin && a.zero_dc) A = make_float2(0.f, 0.f);
             const float p2 = a.linear ? A.x : A.x * A.x + A.y * A.y;
-            float mu = 0.f;
-            if (a.need_mu) {
-              const float fkx = (float)kxv;
-              const float kn2 = fkx * fkx + f2yz;
-              mu = kn2 > 0.f ? (fkx * a.los[0] + lyz) * rsqrtf(kn2) : 0.f;
-            }
-            const float m2 = mu * mu;
+            if constexpr (YLM) {  // real spherical harmonics of khat (local line of sight)
 #pragma unroll
-            for (int p = 0; p < NP; ++p) {
-              if (origin && a.rz[p]) continue;
-              const float f = w * legendre_eval(a.leg, p, mu, m2);
-              sum[p] += (double)(p2 * f);
+              for (int p = 0; p < NP; ++p) {
+                if (origin && a.rz[p]) continue;
+                const float f = w * ylm_of_vector(a.ell[p], a.ylm_m[p], (float)kxv, fky, fkz);
+                sum[p] += (double)(p2 * f);
+              }
+            } else {
+              float mu = 0.f;
+              if (a.need_mu) {
+                const float fkx = (float)kxv;
+                const float kn2 = fkx * fkx + f2yz;
+                mu = kn2 > 0.f ? (fkx * a.los[0] + lyz) * rsqrtf(kn2) : 0.f;
+              }
+              const float m2 = mu * mu;
+#pragma unroll
+              for (int p = 0; p < NP; ++p) {
+                if (origin && a.rz[p]) continue;
+                const float f = w * legendre_eval(a.leg, p, mu, m2);
+                sum[p] += (double)(p2 * f);
+              }
             }
           }
           if (b >= 1 && b < nedges) {
@@ -520,10 +531,10 @@ int combine_tables(hipStream_t s, const int32_t n[3], int order, int ni, int com
   return MP_OK;
 }
 
-template <int L, bool BIN, int NP, bool TAB>
+template <int L, bool BIN, int NP, bool TAB, bool YLM = false>
 static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles) {
   using Cfg = XPCfg<L>;
-  auto kern = xpass_kernel<L, BIN, NP, TAB>;
+  auto kern = xpass_kernel<L, BIN, NP, TAB, YLM>;
   a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
   a.ntiles = a.nchunks * a.ny;
   const size_t lds = Cfg::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, TAB);
@@ -547,11 +558,11 @@ static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double*
 }
 
 // The kx/kz tables go to LDS when they fit beside the tile.
-template <int L, bool BIN, int NP>
+template <int L, bool BIN, int NP, bool YLM = false>
 static int launch_xpass(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles) {
   const size_t with_tab = XPCfg<L>::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, true);
-  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true>(s, a, nblocks_out, partial_ws, partial_doubles);
-  return launch_xpass_t<L, BIN, NP, false>(s, a, nblocks_out, partial_ws, partial_doubles);
+  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true, YLM>(s, a, nblocks_out, partial_ws, partial_doubles);
+  return launch_xpass_t<L, BIN, NP, false, YLM>(s, a, nblocks_out, partial_ws, partial_doubles);
 }
 
 template <int L>
@@ -562,6 +573,10 @@ static int xpass_write_L(hipStream_t s, XPassArgs& a) {
 
 template <int L>
 static int xpass_bin_L(hipStream_t s, XPassArgs& a, int nprod, double* partial, size_t partial_doubles, int* nblocks) {
+  if (a.ylm) {
+    if (nprod != 1) return set_error(MP_ERR_INVALID, "x pass: Y_lm binning takes one product, got %d", nprod);
+    return launch_xpass<L, true, 1, true>(s, a, nblocks, partial, partial_doubles);
+  }
   switch (nprod) {
     case 1: return launch_xpass<L, true, 1>(s, a, nblocks, partial, partial_doubles);
     case 2: return launch_xpass<L, true, 2>(s, a, nblocks, partial, partial_doubles);
@@ -659,7 +674,7 @@ size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int ne
 
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
-                double* out, void* ws, size_t ws_bytes, int linear) {
+                double* out, void* ws, size_t ws_bytes, int linear, const int32_t* ylm_m) {
   MP_REQUIRE(partial && out && ell && remove_zero, "bin_partial: null pointer");
   MP_REQUIRE(ni >= 1, "bin_partial: interlacing %d < 1", ni);
   MP_REQUIRE(nprod >= 1 && nprod <= XB_MAX_PROD, "bin_partial: %d products not in 1..%d", nprod, XB_MAX_PROD);
@@ -681,8 +696,14 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   a.g = g;
   a.need_mu = 0;
   a.linear = linear;
+  a.ylm = ylm_m != nullptr;
   for (int p = 0; p < nprod; ++p) {
     MP_REQUIRE(ell[p] >= 0 && ell[p] <= 10, "bin_partial: ell %d not in 0..10", ell[p]);
+    if (ylm_m) {
+      MP_REQUIRE(ell[p] <= 4 && ylm_m[p] >= -ell[p] && ylm_m[p] <= ell[p], "bin_partial: (ell, m) = (%d, %d) not supported",
+                 ell[p], ylm_m[p]);
+      a.ylm_m[p] = ylm_m[p];
+    }
     a.ell[p] = ell[p];
     a.rz[p] = remove_zero[p];
     legendre_table_set(a.leg, p, ell[p]);

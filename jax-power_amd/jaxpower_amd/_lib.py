@@ -90,6 +90,8 @@ SIGNATURES = {
                                      P_F64, P_I32, P_F64, c_double, c_int, P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_tophat_shell_xi_array': (c_int, [c_int, c_double, c_double, c_void_p, c_int64, c_void_p]),
     'mp_spherical_jn_array': (c_int, [c_int, c_void_p, c_int64, c_void_p]),
+    'mp_window_bin_tophat_local': (c_int, [c_void_p, c_void_p, P_I32, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                           P_F64, P_F64, c_int, c_int, c_int, c_int, c_double, c_void_p, c_void_p, c_size_t]),
     'mp_reduce_workspace_bytes': (c_int64, []),
     'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),

@@ -542,12 +542,16 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
     if periodic:
         mattrs = meshes[0]
         assert all(m is None or m is mattrs for m in meshes[1:]), 'a periodic window takes one MeshAttrs'
+        fields = (0, 0)
     else:
         meshes, fields = format_meshes(*meshes, nmeshes=2)
         mattrs = meshes[0].attrs
     los, vlos, swap = _format_los(los)
     if vlos is None:
-        raise NotImplementedError('window matrix with a local (firstpoint/endpoint) line of sight is not implemented on this path')
+        if not infinite:
+            raise NotImplementedError("window matrix with a local (firstpoint/endpoint) line of sight: only flags=('infinite',) "
+                                      "is implemented on this path")
+        return _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm)
     ells = bin.ells
     n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
     _norm = n3 / float(np.prod(mattrs.cellsize))
@@ -652,6 +656,108 @@ def _window_infinite(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
                 ill = ells.index(ell)
                 wmat[:, ill, :] = (2 * ell + 1) * rnorm[ill] * sums[:, ie, :] / bin.nmodes
     return wmat.reshape(ncol, -1).T
+
+
+def _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm):
+    """compute_mesh2_spectrum_window(flags=('infinite',)) with a local (firstpoint/endpoint) line
+    of sight, theory line of sight 'firstpoint', wide-angle order 0 (mesh2.py:1133-1156,
+    1235-1260, 1463-1482).  For each input multipole lin and output (l, m) the selection term
+    Qs = V_cell sum_m1 Y_lin,m1(shat) 4 pi / (2 lin + 1) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0) is
+    formed once on the device; every input shell then takes one z pass (the shell's xi_lin(|s|)
+    generated from a table and multiplied into Qs as it is loaded), one y pass and one binning x
+    pass with Y_lm(khat) weights."""
+    theory_los = 'firstpoint'
+    if isinstance(ellsin, tuple) and len(ellsin) == 2 and isinstance(ellsin[1], str):
+        ellsin, theory_los = ellsin
+    if theory_los != 'firstpoint':
+        raise NotImplementedError(f'window theory line of sight {theory_los!r} is not implemented on this path')
+    ellsin = [ellsin] if np.ndim(ellsin) == 0 and not isinstance(ellsin, tuple) else list(ellsin)
+    pairs = []
+    for ellin in ellsin:
+        ellin, wa = ellin if isinstance(ellin, tuple) else (ellin, 0)
+        if wa != 0:
+            raise NotImplementedError('wide-angle orders > 0 are not implemented on this path')
+        pairs.append(int(ellin))
+    ellsin = pairs
+    ells = bin.ells
+    if max(ells) > 4 or max(ellsin) > 4:
+        raise NotImplementedError('local line-of-sight window multipoles up to 4 on this path')
+    cell = np.asarray(mattrs.cellsize, dtype='f8')
+    if not np.all(cell == cell[0]):
+        raise NotImplementedError("window flags=('infinite',) needs equal cellsizes on this path")
+    if not all(int(m) >= 2 and int(m) & (int(m) - 1) == 0 for m in mattrs.meshsize):
+        raise NotImplementedError("window flags=('infinite',) needs power-of-two meshsizes on this path")
+    n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
+    _norm = n3 / float(np.prod(mattrs.cellsize))
+    if norm is None:
+        norm = _norm
+    norm = np.broadcast_to(np.asarray(_host(norm), dtype='f8').ravel() if np.ndim(_host(norm)) else float(_host(norm)),
+                           (len(ells),)).copy()
+    rnorm = _norm / norm / n3
+    if hasattr(edgesin, 'ells') and hasattr(edgesin, 'edges'):
+        edgesin = edgesin.edges('k')
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    device = bin._device
+    shape = tuple(int(n) for n in mattrs.meshsize)
+    if periodic:
+        rmesh = torch.ones(shape, dtype=torch.float32, device=device)
+        A0 = _r2c(rmesh, mattrs.meshsize)
+    else:
+        if swap:
+            meshes = meshes[::-1]
+        rmesh = _2r(meshes[0]).value.contiguous().to(torch.float32)
+        A0 = _2c(meshes[0] if fields[1] == fields[0] else meshes[1]).value
+        device = rmesh.device
+    a0, a0_pitch = _as_spectrum(A0)
+    offset = mattrs.boxcenter - mattrs.boxsize / 2.
+    xs = [torch.from_numpy(np.arange(int(n)) * float(d) + float(o)).to(device)
+          for n, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]  # positions (mesh.py:1594-1603)
+    ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]
+    n = _lib.i32x3(mattrs.meshsize)
+    stream = _lib.stream_handle(device)
+    nbins, nedges, nin = bin.nbins, len(bin._edges1d), len(edgesin)
+    ws = _lib.workspace(_lib.load().mp_window_tophat_workspace_bytes(n, 1, nedges), device)
+    t = bin._tables
+    k0 = (_lib.ctypes.c_double * nin)(*[float(e) for e in edgesin[:, 0]])
+    k1 = (_lib.ctypes.c_double * nin)(*[float(e) for e in edgesin[:, 1]])
+    base = torch.empty_like(rmesh)
+    tmp = torch.empty_like(rmesh)
+    Qs = torch.empty_like(rmesh)
+    # per (ellin, l, m): the binned sums of every input shell, summed over m on the host
+    out = torch.empty((len(ellsin), len(ells), 2 * max(ells) + 1, nin, nbins, 2), dtype=torch.float64, device=device)
+    for iin, ellin in enumerate(ellsin):
+        for ill, ell in enumerate(ells):
+            for m in range(-ell, ell + 1):
+                _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
+                for m1 in range(-ellin, ellin + 1):
+                    _mul_ylm(base, tmp, mattrs, xs, ellin, m1, False)
+                    B = _r2c(tmp, mattrs.meshsize)
+                    _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
+                              _spectrum_pitch(B))
+                    _mul_ylm(_c2r(B, mattrs.meshsize), Qs, mattrs, ss, ellin, m1, m1 != -ellin)
+                    del B
+                scale = 4. * np.pi / (2 * ellin + 1) * float(np.prod(mattrs.cellsize))
+                _lib.call('mp_window_bin_tophat_local', stream, _lib.ptr(Qs), n, float(cell[0]), _lib.ptr(t['kx']), _lib.ptr(t['ky']),
+                          _lib.ptr(t['kz']), _lib.ptr(t['thr']), nedges, nin, k0, k1, int(ellin), int(ell), int(m), int(ell == 0),
+                          scale, _lib.ptr(out[iin, ill, m + ell]), _lib.ptr(ws), ws.numel())
+    sums = out[..., 0].cpu().numpy()
+    wmat = np.zeros((len(ellsin), nin, len(ells), nbins))
+    for ill, ell in enumerate(ells):
+        wmat[:, :, ill, :] = sums[:, ill, :2 * ell + 1].sum(axis=1)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        wmat = wmat * (4. * np.pi * rnorm)[None, None, :, None] / bin.nmodes
+    wmat = wmat.reshape(len(ellsin) * nin, -1).T
+    attrs = dict(los='firstpoint')
+    zeros = np.zeros(nbins)
+    observable = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=zeros,
+                                                       num_shotnoise=zeros, norm=np.full(nbins, norm[ill]), ell=ell, attrs=attrs)
+                                     for ill, ell in enumerate(ells)], attrs=attrs)
+    kin = edgesin.mean(axis=-1)
+    theory = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=kin, k_edges=edgesin, nmodes=np.zeros(len(kin), dtype='i8'),
+                                                   num_raw=np.zeros(len(kin)), ell=ell) for ell in ellsin])
+    return WindowMatrix(observable=observable, theory=theory, value=wmat)
 
 
 def _host_array_fn(name, ell, x, *args):

@@ -1033,3 +1033,82 @@ def spectrum_window_smooth(attrs, bin, edgesin, ellsin=0, meshes=None, los='z', 
             tmp.append(np.array(rows))
         blocks.append(np.concatenate(tmp, axis=-1))
     return np.real(np.concatenate(blocks, axis=0).T)
+
+
+def window_local_selection(attrs, rmesh, ell, m, ellin):
+    """Q_lm(s) of the local (firstpoint) window with a configuration-space theory
+    (mesh2.py:1241-1247, wide-angle order 0): V_cell sum_m1 Y_lin,m1(shat) 4 pi / (2 lin + 1)
+    c2r(conj(r2c(rho Y_lm(xhat) Y_lin,m1(xhat))) A0), A0 = r2c(rho); positions from the
+    origin, Y = 0 at the zero vector."""
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    sx, sy, sz = separation_coords(attrs)
+    sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
+    A0 = r2c(rmesh)
+    base = rmesh * real_ylm(ell, m, xx, xy, xz)
+    Qs = 0.
+    for m1 in range(-ellin, ellin + 1):
+        Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(base * real_ylm(ellin, m1, xx, xy, xz))) * A0, attrs.meshsize)
+        Qs = Qs + real_ylm(ellin, m1, sx, sy, sz) * Q * np.prod(attrs.cellsize)
+    return Qs
+
+
+def window_local_selection_bruteforce(attrs, rmesh, ell, m, ellin):
+    """The same Q_lm(s) as an explicit double sum over cells x and lags s,
+    V_cell sum_x rho(x) rho(x + s) Y_lm(xhat) P_lin(xhat . shat) (addition theorem; Y and P
+    taken as 0 at the zero vector), independent of the Y_lin,m1 conventions; tiny meshes."""
+    n = [int(v) for v in attrs.meshsize]
+    xs = attrs.rcoords()
+    ss = separation_coords(attrs)
+    X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
+    xn = np.linalg.norm(X, axis=-1)
+    xhat = X / np.where(xn == 0, 1., xn)[..., None]
+    ylm = real_ylm(ell, m, X[..., 0], X[..., 1], X[..., 2])
+    out = np.zeros(n)
+    for idx in np.ndindex(*n):
+        svec = np.array([ss[a][idx[a]] for a in range(3)])
+        sn = np.linalg.norm(svec)
+        if sn == 0.:
+            mu = np.zeros(n)
+        else:
+            mu = xhat @ (svec / sn)
+        leg = np.where((xn == 0) | (sn == 0.), 0., legendre(ellin, mu))  # the Y are 0 at the zero vector
+        shifted = np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2))  # rho(x + s)
+        out[idx] = np.sum(rmesh * shifted * ylm * leg)
+    return out * np.prod(attrs.cellsize)
+
+
+def spectrum_window_local_infinite(attrs, bin, edgesin, ellsin=0, mesh=None, norm=None):
+    """compute_mesh2_spectrum_window(flags=('infinite',)) with the local (firstpoint) line of
+    sight and theory line of sight 'firstpoint', wide-angle order 0 (mesh2.py:1235-1260): for each
+    input multipole lin, output (l, m) and input shell, 4 pi bin(Y_lm(khat) r2c(xi_lin(|s|) Q_lm(s)),
+    remove_zero = (l == 0)) rnorm, summed over m; mesh None is the periodic box (rho = 1).
+    Returns the (n_ells n_bins, n_ellsin n_in) real matrix."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    ellsin = (ellsin,) if np.ndim(ellsin) == 0 else tuple(ellsin)
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    rmesh = np.ones(tuple(int(v) for v in attrs.meshsize)) if mesh is None else (c2r(mesh, attrs.meshsize) if np.iscomplexobj(mesh) else mesh)
+    sx, sy, sz = separation_coords(attrs)
+    sn = np.sqrt(sx[:, None, None]**2 + sy[None, :, None]**2 + sz[None, None, :]**2)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    nb = len(bin.xavg)
+    blocks = []
+    for ellin in ellsin:
+        W = np.zeros((len(edgesin), len(ells) * nb))
+        tophats = [tophat_shell_xi(ellin, e0, e1, sn) for e0, e1 in edgesin]
+        for ill, ell in enumerate(ells):
+            for m in range(-ell, ell + 1):
+                Qs = window_local_selection(attrs, rmesh, ell, m, ellin)
+                ykm = real_ylm(ell, m, kx, ky, kz)
+                for ie, tophat in enumerate(tophats):
+                    spec = 4. * np.pi * bin(ykm * r2c(tophat * Qs), remove_zero=ell == 0) * rnorm[ill]
+                    W[ie, ill * nb:(ill + 1) * nb] += np.real(spec)
+        blocks.append(W)
+    return np.concatenate(blocks, axis=0).T

@@ -204,3 +204,29 @@ def test_window_smooth(periodic, los):
         wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('smooth',))
         want = po.spectrum_window_smooth(oattrs, obin, edgesin, ellsin=ellsin, meshes=[host(sel.value).astype('f8')], los=los)
     assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='smooth')
+
+
+@pytest.mark.parametrize('periodic,los,ellsin', [(True, 'firstpoint', (0, 2)), (False, 'firstpoint', (0, 2, 4)),
+                                                 (False, 'endpoint', ((0, 2), 'firstpoint'))])
+def test_window_local_infinite(periodic, los, ellsin):
+    """flags=('infinite',) with a local line of sight (mesh2.py:1235-1260): per input multipole and
+    output (l, m) the selection term sum_m1 Y_lin,m1(shat) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0),
+    then per input shell xi_lin(|s|) times it, r2c, Y_lm(khat)-weighted binning, summed over m;
+    against the oracle (whose Y_lm sign conventions may differ: only the sum over m is compared)."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window local infinite', periodic, los, ellsin)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=(0., 0., 0.) if periodic else (900., 300., 500.))
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    edgesin = np.arange(0.01, 0.12, 0.025)
+    oattrs = po.Attrs(16, 400., (0., 0., 0.) if periodic else (900., 300., 500.))
+    obin = po.BinPoles(oattrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    oellsin = ellsin[0] if isinstance(ellsin[-1], str) else ellsin
+    if periodic:
+        wm = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('infinite',))
+        want = po.spectrum_window_local_infinite(oattrs, obin, edgesin, ellsin=oellsin)
+    else:
+        pos, w = _survey_selection(attrs, 100000, 17, scale=0.3)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('infinite',))
+        want = po.spectrum_window_local_infinite(oattrs, obin, edgesin, ellsin=oellsin, mesh=host(sel.value).astype('f8'))
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='local infinite')

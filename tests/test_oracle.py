@@ -650,3 +650,17 @@ def test_window_smooth_close_to_infinite():
     assert abs(Ws[nb + 2, 2] - Wi[nb + 2, 2]) < 0.15 * abs(Wi[nb + 2, 2])
     W1 = po.spectrum_window_smooth(a, bin, edgesin, ellsin=(0, 2), meshes=[np.ones((32,) * 3)], los='z')
     np.testing.assert_allclose(W1, Ws, rtol=1e-9, atol=1e-12 * np.abs(Ws).max())
+
+
+@pytest.mark.parametrize('ell,m,ellin', [(0, 0, 0), (2, 1, 0), (0, 0, 2), (2, -2, 2), (4, 3, 2)])
+def test_window_local_selection_vs_bruteforce(ell, m, ellin):
+    """Q_lm(s) of the local-line-of-sight window (sum over m1 of Y_lin,m1(shat) times the
+    r2c/c2r cross-correlation of rho Y_lm Y_lin,m1 with rho) against the explicit double sum
+    over cells and lags with P_lin(xhat . shat): pins the Y normalisation, the conjugation and
+    the direction of the lag, independently of the Y sign conventions."""
+    rng = np.random.default_rng(5)
+    a = po.Attrs(6, 60., 20.)
+    rho = rng.uniform(0.5, 1.5, (6, 6, 6))
+    got = po.window_local_selection(a, rho, ell, m, ellin)
+    want = po.window_local_selection_bruteforce(a, rho, ell, m, ellin)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
