@@ -281,6 +281,25 @@ int mp_window_bin_tophat_local(void* stream, const float* Qs, const int32_t mesh
                                const double* k0, const double* k1, int ellin, int ell, int m, int remove_zero, double scale,
                                double* out, void* workspace, size_t workspace_bytes);
 
+/* compute_mesh2_spectrum_window without flags, local (firstpoint) line of sight, theory line of
+ * sight 'firstpoint', wide-angle order 0 (jaxpower/mesh2.py:1262-1305), in three pieces the caller
+ * chains per input shell and output (ell, m):
+ *   mp_window_shell_ylm: out = c2r([k2_lo <= k^2 < k2_hi] Y_ell,m(khat) scale), a real mesh (the
+ *     xi of mesh2.py:1276; the closed right edge |k| <= e1 is k^2 < t(nextafter(e1)));
+ *   mp_mul_acc: out (+)= a * b on real meshes (Aell += xi * Q, mesh2.py:1277-1281);
+ *   mp_bin_ylm_real: out[b][re, im] = sum over bin b (k = 0 excluded when remove_zero) of
+ *     w Re(r2c(mesh)) Y_ell,m(khat) (mesh2.py:1283-1285), the spectrum binned in its x pass.
+ * ell <= 4; mp_bin_ylm_real needs power-of-two meshsizes. */
+int64_t mp_window_shell_ylm_workspace_bytes(const int32_t meshsize[3]);
+int mp_window_shell_ylm(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                        double k2_lo, double k2_hi, int ell, int m, float scale, float* out, void* workspace,
+                        size_t workspace_bytes);
+int mp_mul_acc(void* stream, const float* a, const float* b, float* out, int64_t n, int accumulate);
+int64_t mp_bin_ylm_real_workspace_bytes(const int32_t meshsize[3], int nedges);
+int mp_bin_ylm_real(void* stream, const float* mesh, const int32_t meshsize[3], const double* kx, const double* ky,
+                    const double* kz, const double* k2_thresholds, int nedges, int ell, int m, int remove_zero, double* out,
+                    void* workspace, size_t workspace_bytes);
+
 /* Host arrays, for the smooth window (compute_mesh2_spectrum_window(flags=('smooth',)),
  * jaxpower/mesh2.py:1045-1076): out[i] = xi_ell(s[i]) of the shell [k0, k1) as above, and
  * out[i] = j_ell(x[i]) by the closed forms / series of jaxpower/utils.py:418-450; ell in 0..5. */

@@ -112,6 +112,7 @@ struct ShellGen {
   LegendreTable leg;
   int ell_on;
   float scale;
+  int ylm_on, ylm_ell, ylm_m;  // weight Y_lm(khat) instead of P_ell(mu) (local line of sight)
   __device__ __forceinline__ float eval(int ix, int iy, int iz) const {
     const double kxv = kx[ix], kyv = ky[iy], kzv = kz[iz];
     double k2;
@@ -121,7 +122,9 @@ struct ShellGen {
     }
     if (!(k2 >= t_lo && k2 < t_hi)) return 0.f;
     float v = scale;
-    if (ell_on) {
+    if (ylm_on) {
+      v *= ylm_of_vector(ylm_ell, ylm_m, (float)kxv, (float)kyv, (float)kzv);
+    } else if (ell_on) {
       const double mu = k2 > 0. ? (kxv * los[0] + kyv * los[1] + kzv * los[2]) / sqrt(k2) : 0.;
       const float m = (float)mu;
       v *= legendre_eval(leg, 0, m, m * m);
@@ -132,6 +135,9 @@ struct ShellGen {
 // data <- the z and y passes of r2c(Q * scale * irfftn(S)), S the shell of g, with the inverse
 // x pass generating S instead of reading it; the forward x pass is left to a consumer.
 int convolve_shell_zy(hipStream_t s, const ShellGen& g, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale);
+// out = irfftn(S), S the shell of g, with the inverse x pass generating S (scratch: a half
+// spectrum of row pitch scratch_pitch).
+int irfft3d_shell(hipStream_t s, const ShellGen& g, float* out, const int32_t n[3], float2* scratch, int64_t scratch_pitch);
 // In place on a half spectrum: data <- r2c(Q * scale * irfftn(data)), Q a real (N0, N1, N2)
 // mesh; the real mesh is formed and multiplied inside the z pass only.
 int convolve_real(hipStream_t s, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale);

@@ -790,6 +790,23 @@ int convolve_shell_zy(hipStream_t s, const ShellGen& g, float2* data, int64_t pi
   return col_fft(n[1], false, "window_ypass", data, data, ncols, nx, pitch, ny * pitch, pitch, ny * pitch, 1.f, s);
 }
 
+int irfft3d_shell(hipStream_t s, const ShellGen& g, float* out, const int32_t n[3], float2* scratch, int64_t sp) {
+  int rc = check_shape(n);
+  if (rc) return rc;
+  rc = ensure_twiddles();
+  if (rc) return rc;
+  const int M = n[2] / 2, ncols = M + 1;
+  if (M > 1024) return set_error(MP_ERR_UNSUPPORTED, "z length %d too large", 2 * M);
+  if (sp < ncols) return set_error(MP_ERR_INVALID, "pitch %lld < Nz/2+1", (long long)sp);
+  const int64_t nx = n[0], ny = n[1];
+  rc = [&]() -> int { MP_POW2_SWITCH(n[0], launch_xpass_inv_shell, g, scratch, ncols, (int)ny, ny * sp, sp, s); }();
+  if (rc) return rc;
+  rc = col_fft(n[1], true, "window_ypass_inv", scratch, scratch, ncols, nx, sp, ny * sp, sp, ny * sp, 1.f, s);
+  if (rc) return rc;
+  const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)M));
+  return zpass_c2r(M, scratch, sp, out, nx * ny, scale, s);
+}
+
 int convolve_real(hipStream_t s, float2* data, int64_t pitch, const float* Q, const int32_t n[3], float scale) {
   int rc = check_shape(n);
   if (rc) return rc;

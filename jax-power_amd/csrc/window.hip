@@ -306,3 +306,118 @@ MP_EXPORT int mp_window_bin_tophat_local(void* stream, const float* Qs, const in
   }
   return MP_OK;
 }
+
+namespace mp {
+
+constexpr int MA_NT = 256;
+
+__global__ void __launch_bounds__(MA_NT)
+mul_acc_kernel(const float4* __restrict__ a, const float4* __restrict__ b, float4* __restrict__ out, int64_t n4, int accumulate) {
+  for (int64_t i = (int64_t)blockIdx.x * MA_NT + threadIdx.x; i < n4; i += (int64_t)gridDim.x * MA_NT) {
+    const float4 x = a[i], y = b[i];
+    float4 v = make_float4(x.x * y.x, x.y * y.y, x.z * y.z, x.w * y.w);
+    if (accumulate) {
+      const float4 o = out[i];
+      v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+    }
+    out[i] = v;
+  }
+}
+
+__global__ void __launch_bounds__(MA_NT)
+mul_acc_tail_kernel(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, int64_t start, int64_t n,
+                    int accumulate) {
+  const int64_t i = start + (int64_t)blockIdx.x * MA_NT + threadIdx.x;
+  if (i < n) out[i] = accumulate ? out[i] + a[i] * b[i] : a[i] * b[i];
+}
+
+}  // namespace mp
+
+// The real-space products of the local window with the theory on the mesh
+// (jaxpower/mesh2.py:1277-1281, Aell += xi * Q): out = a * b, or out += a * b when accumulate.
+// Device arrays of n floats, 16-byte aligned.
+MP_EXPORT int mp_mul_acc(void* stream, const float* a, const float* b, float* out, int64_t n, int accumulate) {
+  MP_REQUIRE(a && b && out && n >= 0, "mp_mul_acc: null pointer");
+  MP_REQUIRE(((uintptr_t)a | (uintptr_t)b | (uintptr_t)out) % 16 == 0, "mp_mul_acc: arrays must be 16-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n4 = n / 4;
+  int dev = 0, ncu = 0;
+  MP_CHECK_HIP(hipGetDevice(&dev));
+  MP_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  if (n4 > 0) {
+    const int blocks = (int)std::min<int64_t>(ceil_div(n4, MA_NT), (int64_t)ncu * 8);
+    MP_LAUNCH("window_mul_acc", mul_acc_kernel, dim3((unsigned)blocks), dim3(MA_NT), 0, s, (const float4*)a, (const float4*)b,
+              (float4*)out, n4, accumulate ? 1 : 0);
+  }
+  if (n > 4 * n4)
+    MP_LAUNCH("window_mul_acc", mul_acc_tail_kernel, dim3(1), dim3(MA_NT), 0, s, a, b, out, 4 * n4, n, accumulate ? 1 : 0);
+  return MP_OK;
+}
+
+MP_EXPORT int64_t mp_window_shell_ylm_workspace_bytes(const int32_t meshsize[3]) {
+  if (!meshsize) return 0;
+  return (int64_t)(align_up((size_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * sizeof(float2), 256) + 256);
+}
+
+// The configuration-space shell of the local window with the theory on the mesh
+// (jaxpower/mesh2.py:1276): out = c2r([k2_lo <= k^2 < k2_hi] Y_ell,m(khat) scale), a real
+// (N0, N1, N2) mesh; the shell is generated inside the inverse x pass.  ell <= 4.
+MP_EXPORT int mp_window_shell_ylm(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                                  double k2_lo, double k2_hi, int ell, int m, float scale, float* out, void* workspace,
+                                  size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && out, "mp_window_shell_ylm: null pointer");
+  MP_REQUIRE(ell >= 0 && ell <= 4 && m >= -ell && m <= ell, "mp_window_shell_ylm: (ell, m) = (%d, %d) not supported", ell, m);
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_window_shell_ylm: workspace too small");
+  ShellGen g = make_shell(kx, ky, kz, k2_lo, k2_hi, 0, nullptr, scale);
+  g.ylm_on = 1;
+  g.ylm_ell = ell;
+  g.ylm_m = m;
+  return irfft3d_shell((hipStream_t)stream, g, out, meshsize, scratch, sp);
+}
+
+MP_EXPORT int64_t mp_bin_ylm_real_workspace_bytes(const int32_t meshsize[3], int nedges) {
+  if (!meshsize || nedges < 2) return 0;
+  const size_t spec = align_up((size_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * sizeof(float2), 256);
+  return (int64_t)(spec + bin_partial_workspace_bytes(meshsize, 1, 1, nedges) + 256);
+}
+
+// out[b][re, im] = sum over the modes of bin b (k = 0 excluded when remove_zero) of
+// w Re(r2c(mesh)) Y_ell,m(khat): one output (ell, m) term of the local window with the theory on
+// the mesh (jaxpower/mesh2.py:1283-1285); the half spectrum is binned in its x pass, not written.
+MP_EXPORT int mp_bin_ylm_real(void* stream, const float* mesh, const int32_t meshsize[3], const double* kx, const double* ky,
+                              const double* kz, const double* k2_thresholds, int nedges, int ell, int m, int remove_zero, double* out,
+                              void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(mesh && meshsize && kx && ky && kz && k2_thresholds && out, "mp_bin_ylm_real: null pointer");
+  MP_REQUIRE(ell >= 0 && ell <= 4 && m >= -ell && m <= ell, "mp_bin_ylm_real: (ell, m) = (%d, %d) not supported", ell, m);
+  MP_REQUIRE(nedges >= 2, "mp_bin_ylm_real: need at least two edges");
+  if (!xpass_supported(meshsize))
+    return set_error(MP_ERR_UNSUPPORTED, "mp_bin_ylm_real: meshsize (%d, %d, %d) not supported", meshsize[0], meshsize[1], meshsize[2]);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* spec = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  const size_t rest = workspace_bytes > ws.used ? workspace_bytes - ws.used : 0;
+  if (!spec || rest < bin_partial_workspace_bytes(meshsize, 1, 1, nedges))
+    return set_error(MP_ERR_WORKSPACE, "mp_bin_ylm_real: workspace of %zu bytes < %lld", workspace_bytes,
+                     (long long)mp_bin_ylm_real_workspace_bytes(meshsize, nedges));
+  RealPrologue pro{};
+  pro.kind = PRO_NONE;
+  int rc = rfft3d_zy(s, mesh, spec, meshsize, sp, pro);
+  if (rc) return rc;
+  BinGeom geo{};
+  geo.n0 = meshsize[0];
+  geo.n1 = meshsize[1];
+  geo.ncols = meshsize[2] / 2 + 1;
+  geo.nedges = nedges;
+  geo.kx = kx;
+  geo.ky = ky;
+  geo.kz = kz;
+  geo.thr = k2_thresholds;
+  const int32_t ells[1] = {ell}, ms[1] = {m}, rz[1] = {remove_zero ? 1 : 0};
+  const double zero_los[3] = {0., 0., 0.};
+  char* base = static_cast<char*>(workspace) + ws.used;
+  return bin_partial(s, spec, sp, meshsize, 0, 1, 0, 0, geo, zero_los, 1, ells, rz, out, base, rest, 1, ms);
+}

@@ -92,6 +92,13 @@ SIGNATURES = {
     'mp_spherical_jn_array': (c_int, [c_int, c_void_p, c_int64, c_void_p]),
     'mp_window_bin_tophat_local': (c_int, [c_void_p, c_void_p, P_I32, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                            P_F64, P_F64, c_int, c_int, c_int, c_int, c_double, c_void_p, c_void_p, c_size_t]),
+    'mp_mul_acc': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int]),
+    'mp_window_shell_ylm_workspace_bytes': (c_int64, [P_I32]),
+    'mp_window_shell_ylm': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_float,
+                                    c_void_p, c_void_p, c_size_t]),
+    'mp_bin_ylm_real_workspace_bytes': (c_int64, [P_I32, c_int]),
+    'mp_bin_ylm_real': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                                c_void_p, c_void_p, c_size_t]),
     'mp_reduce_workspace_bytes': (c_int64, []),
     'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),

@@ -548,10 +548,9 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
         mattrs = meshes[0].attrs
     los, vlos, swap = _format_los(los)
     if vlos is None:
-        if not infinite:
-            raise NotImplementedError("window matrix with a local (firstpoint/endpoint) line of sight: only flags=('infinite',) "
-                                      "is implemented on this path")
-        return _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm)
+        if smooth:
+            raise NotImplementedError("window flags=('smooth',) with a local line of sight is not implemented on this path")
+        return _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm, infinite)
     ells = bin.ells
     n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
     _norm = n3 / float(np.prod(mattrs.cellsize))
@@ -658,14 +657,20 @@ def _window_infinite(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
     return wmat.reshape(ncol, -1).T
 
 
-def _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm):
-    """compute_mesh2_spectrum_window(flags=('infinite',)) with a local (firstpoint/endpoint) line
-    of sight, theory line of sight 'firstpoint', wide-angle order 0 (mesh2.py:1133-1156,
-    1235-1260, 1463-1482).  For each input multipole lin and output (l, m) the selection term
+def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm, infinite):
+    """compute_mesh2_spectrum_window with a local (firstpoint/endpoint) line of sight, theory line
+    of sight 'firstpoint', wide-angle order 0 (mesh2.py:1133-1156, 1235-1305, 1463-1482).
+
+    flags=('infinite',): for each input multipole lin and output (l, m) the selection term
     Qs = V_cell sum_m1 Y_lin,m1(shat) 4 pi / (2 lin + 1) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0) is
     formed once on the device; every input shell then takes one z pass (the shell's xi_lin(|s|)
     generated from a table and multiplied into Qs as it is loaded), one y pass and one binning x
-    pass with Y_lm(khat) weights."""
+    pass with Y_lm(khat) weights.
+
+    No flags (theory on the mesh): the terms Q_lm,m1 = c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0) are
+    kept for every (l, m, m1); each closed input shell gives xi_m1 = c2r(shell Y_lin,m1(khat))
+    (generated in the inverse x pass), then A_lm = sum_m1 xi_m1 Q_lm,m1 and one binning pass of
+    r2c(A_lm) Y_lm(khat) per (l, m)."""
     theory_los = 'firstpoint'
     if isinstance(ellsin, tuple) and len(ellsin) == 2 and isinstance(ellsin[1], str):
         ellsin, theory_los = ellsin
@@ -683,10 +688,10 @@ def _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bi
     if max(ells) > 4 or max(ellsin) > 4:
         raise NotImplementedError('local line-of-sight window multipoles up to 4 on this path')
     cell = np.asarray(mattrs.cellsize, dtype='f8')
-    if not np.all(cell == cell[0]):
+    if infinite and not np.all(cell == cell[0]):
         raise NotImplementedError("window flags=('infinite',) needs equal cellsizes on this path")
     if not all(int(m) >= 2 and int(m) & (int(m) - 1) == 0 for m in mattrs.meshsize):
-        raise NotImplementedError("window flags=('infinite',) needs power-of-two meshsizes on this path")
+        raise NotImplementedError('a local line-of-sight window needs power-of-two meshsizes on this path')
     n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
     _norm = n3 / float(np.prod(mattrs.cellsize))
     if norm is None:
@@ -727,7 +732,9 @@ def _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bi
     Qs = torch.empty_like(rmesh)
     # per (ellin, l, m): the binned sums of every input shell, summed over m on the host
     out = torch.empty((len(ellsin), len(ells), 2 * max(ells) + 1, nin, nbins, 2), dtype=torch.float64, device=device)
-    for iin, ellin in enumerate(ellsin):
+    if not infinite:
+        _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, bin, out)
+    for iin, ellin in enumerate(ellsin if infinite else ()):
         for ill, ell in enumerate(ells):
             for m in range(-ell, ell + 1):
                 _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
@@ -746,6 +753,8 @@ def _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bi
     wmat = np.zeros((len(ellsin), nin, len(ells), nbins))
     for ill, ell in enumerate(ells):
         wmat[:, :, ill, :] = sums[:, ill, :2 * ell + 1].sum(axis=1)
+    if not infinite:  # the 4 pi / (2 lin + 1) of Q_lm,m1 (mesh2.py:1269)
+        wmat *= np.array([4. * np.pi / (2 * ellin + 1) for ellin in ellsin])[:, None, None, None]
     with np.errstate(invalid='ignore', divide='ignore'):
         wmat = wmat * (4. * np.pi * rnorm)[None, None, :, None] / bin.nmodes
     wmat = wmat.reshape(len(ellsin) * nin, -1).T
@@ -758,6 +767,53 @@ def _window_local_infinite(meshes, fields, periodic, mattrs, edgesin, ellsin, bi
     theory = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=kin, k_edges=edgesin, nmodes=np.zeros(len(kin), dtype='i8'),
                                                    num_raw=np.zeros(len(kin)), ell=ell) for ell in ellsin])
     return WindowMatrix(observable=observable, theory=theory, value=wmat)
+
+
+def _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, bin, out):
+    """Binned sums of the local window with the theory on the mesh (mesh2.py:1262-1305) into
+    out[iin, ill, m + l, ie] (without the 4 pi / (2 lin + 1) of Q and the 4 pi rnorm / nmodes)."""
+    device = rmesh.device
+    n = _lib.i32x3(mattrs.meshsize)
+    stream = _lib.stream_handle(device)
+    ells = bin.ells
+    t = bin._tables
+    nedges = len(bin._edges1d)
+    ncell = rmesh.numel()
+    ws_shell = _lib.workspace(_lib.load().mp_window_shell_ylm_workspace_bytes(n), device)
+    ws_bin = _lib.workspace(_lib.load().mp_bin_ylm_real_workspace_bytes(n, nedges), device)
+    # shell [e0, e1] closed on the right (mesh2.py:1276): k >= e0 and not k >= nextafter(e1)
+    lo = _k2_thresholds(edgesin[:, 0])
+    hi = _k2_thresholds(np.nextafter(edgesin[:, 1], np.inf))
+    base = torch.empty_like(rmesh)
+    tmp = torch.empty_like(rmesh)
+    acc = torch.empty_like(rmesh)
+    for iin, ellin in enumerate(ellsin):
+        nm1 = 2 * ellin + 1
+        Q = {}
+        for ell in ells:
+            for m in range(-ell, ell + 1):
+                _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
+                for m1 in range(-ellin, ellin + 1):
+                    _mul_ylm(base, tmp, mattrs, xs, ellin, m1, False)
+                    B = _r2c(tmp, mattrs.meshsize)
+                    _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
+                              _spectrum_pitch(B))
+                    Q[ell, m, m1] = _c2r(B, mattrs.meshsize)
+                    del B
+        xi = torch.empty((nm1,) + tuple(rmesh.shape), dtype=torch.float32, device=device)
+        for ie in range(len(edgesin)):
+            for m1 in range(-ellin, ellin + 1):
+                _lib.call('mp_window_shell_ylm', stream, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), float(lo[ie]),
+                          float(hi[ie]), int(ellin), int(m1), 1., _lib.ptr(xi[m1 + ellin]), _lib.ptr(ws_shell), ws_shell.numel())
+            for ill, ell in enumerate(ells):
+                for m in range(-ell, ell + 1):
+                    for m1 in range(-ellin, ellin + 1):
+                        _lib.call('mp_mul_acc', stream, _lib.ptr(xi[m1 + ellin]), _lib.ptr(Q[ell, m, m1]), _lib.ptr(acc), ncell,
+                                  int(m1 != -ellin))
+                    _lib.call('mp_bin_ylm_real', stream, _lib.ptr(acc), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']),
+                              _lib.ptr(t['thr']), nedges, int(ell), int(m), int(ell == 0), _lib.ptr(out[iin, ill, m + ell, ie]),
+                              _lib.ptr(ws_bin), ws_bin.numel())
+        del Q, xi
 
 
 def _host_array_fn(name, ell, x, *args):

@@ -1112,3 +1112,88 @@ def spectrum_window_local_infinite(attrs, bin, edgesin, ellsin=0, mesh=None, nor
                     W[ie, ill * nb:(ill + 1) * nb] += np.real(spec)
         blocks.append(W)
     return np.concatenate(blocks, axis=0).T
+
+
+def window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1):
+    """A_lm(s) = sum_m1 xi_m1(s) Q_lm,m1(s) of the local (firstpoint) window with the theory on the
+    mesh (mesh2.py:1267-1281, wide-angle order 0): xi_m1 = c2r([e0 <= |k| <= e1] Y_lin,m1(khat))
+    and Q_lm,m1 = 4 pi / (2 lin + 1) c2r(conj(r2c(rho Y_lm(xhat) Y_lin,m1(xhat))) A0)."""
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    kn = knorm(attrs)
+    shell = (kn >= e0) & (kn <= e1)
+    A0 = r2c(rmesh)
+    base = rmesh * real_ylm(ell, m, xx, xy, xz)
+    out = 0.
+    for m1 in range(-ellin, ellin + 1):
+        xi = c2r(shell * real_ylm(ellin, m1, kx, ky, kz), attrs.meshsize)
+        Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(base * real_ylm(ellin, m1, xx, xy, xz))) * A0, attrs.meshsize)
+        out = out + xi * Q
+    return out
+
+
+def window_local_terms_bruteforce(attrs, rmesh, ell, m, ellin, e0, e1):
+    """The same A_lm(s) from the addition theorem, without any Y_lin,m1:
+    N^-3 sum_x sum_k rho(x) rho(x + s) Y_lm(xhat) [e0 <= |k| <= e1] P_lin(xhat . khat) cos(k . s)
+    over the full k grid (the shell and P_lin are even in k), the P and Y taken as 0 at zero
+    vectors; tiny meshes only."""
+    n = [int(v) for v in attrs.meshsize]
+    xs = attrs.rcoords()
+    X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1).reshape(-1, 3)
+    xn = np.linalg.norm(X, axis=-1)
+    xhat = X / np.where(xn == 0, 1., xn)[:, None]
+    ylm = real_ylm(ell, m, X[:, 0], X[:, 1], X[:, 2])
+    kk = [np.fft.fftfreq(n[a]) * attrs.kfun[a] * n[a] for a in range(3)]
+    K = np.stack(np.meshgrid(*kk, indexing='ij'), axis=-1).reshape(-1, 3)
+    kn = np.linalg.norm(K, axis=-1)
+    # the half-spectrum shell uses |k| summed x, y, z as knorm does; on the full grid that is the same
+    sel = (kn >= e0) & (kn <= e1)
+    K, kn = K[sel], kn[sel]
+    khat = K / np.where(kn == 0, 1., kn)[:, None]
+    leg = legendre(ellin, xhat @ khat.T)  # (x, k)
+    leg[xn == 0, :] = 0.
+    leg[:, kn == 0] = 0.
+    idx = [np.arange(v) for v in n]
+    grid = np.stack(np.meshgrid(*idx, indexing='ij'), axis=-1).reshape(-1, 3)
+    ss = [np.fft.fftfreq(n[a]) * attrs.cellsize[a] * n[a] for a in range(3)]
+    flat = rmesh.ravel()
+    out = np.zeros(n)
+    for sidx in np.ndindex(*n):
+        svec = np.array([ss[a][sidx[a]] for a in range(3)])
+        shifted = np.roll(rmesh, [-i for i in sidx], axis=(0, 1, 2)).ravel()  # rho(x + s)
+        w = flat * shifted * ylm  # (x,)
+        out[sidx] = np.sum((w @ leg) * np.cos(K @ svec)) / np.prod(n)
+    return out
+
+
+def spectrum_window_local(attrs, bin, edgesin, ellsin=0, mesh=None, norm=None):
+    """compute_mesh2_spectrum_window without flags, local (firstpoint) line of sight, theory line
+    of sight 'firstpoint', wide-angle order 0 (mesh2.py:1262-1305): for each input multipole lin
+    and closed input shell [e0, e1], sum over (l, m) of 4 pi bin(r2c(A_lm) Y_lm(khat),
+    remove_zero = (l == 0)) rnorm.  mesh None is the periodic box (rho = 1)."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    ellsin = (ellsin,) if np.ndim(ellsin) == 0 else tuple(ellsin)
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    rmesh = np.ones(tuple(int(v) for v in attrs.meshsize)) if mesh is None else (c2r(mesh, attrs.meshsize) if np.iscomplexobj(mesh) else mesh)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    nb = len(bin.xavg)
+    blocks = []
+    for ellin in ellsin:
+        W = np.zeros((len(edgesin), len(ells) * nb))
+        for ie, (e0, e1) in enumerate(edgesin):
+            for ill, ell in enumerate(ells):
+                acc = 0.
+                for m in range(-ell, ell + 1):
+                    acc = acc + r2c(window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1)) * real_ylm(ell, m, kx, ky, kz)
+                W[ie, ill * nb:(ill + 1) * nb] = np.real(4. * np.pi * bin(acc, remove_zero=ell == 0) * rnorm[ill])
+        blocks.append(W)
+    return np.concatenate(blocks, axis=0).T

@@ -117,7 +117,9 @@ def test_window_unsupported_raise():
     attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
     bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.05}, ells=(0, 2))
     with pytest.raises(NotImplementedError):
-        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=0)
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=0, flags=('smooth',))
+    with pytest.raises(NotImplementedError):
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=((0, 2), 'local'))
     with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('fftlog',))
 
@@ -230,3 +232,29 @@ def test_window_local_infinite(periodic, los, ellsin):
         wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('infinite',))
         want = po.spectrum_window_local_infinite(oattrs, obin, edgesin, ellsin=oellsin, mesh=host(sel.value).astype('f8'))
     assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='local infinite')
+
+
+@pytest.mark.parametrize('periodic,los,ellsin', [(True, 'firstpoint', (0, 2)), (False, 'firstpoint', (0, 2)),
+                                                 (False, 'endpoint', (2,))])
+def test_window_local_mesh_theory(periodic, los, ellsin):
+    """No flags with a local line of sight (mesh2.py:1262-1305): xi_m1 = c2r(closed shell
+    Y_lin,m1(khat)), A_lm = sum_m1 xi_m1 c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0), binned with
+    Y_lm(khat) and summed over m, against the oracle."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window local mesh theory', periodic, los, ellsin)
+    center = (0., 0., 0.) if periodic else (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    kf = 2. * np.pi / 400.
+    edgesin = np.array([[kf, 3 * kf], [3 * kf, 5 * kf], [5 * kf, 8 * kf]])  # closed right edges on lattice shells
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    if periodic:
+        wm = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los)
+        want = po.spectrum_window_local(oattrs, obin, edgesin, ellsin=ellsin)
+    else:
+        pos, w = _survey_selection(attrs, 100000, 19, scale=0.3)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los)
+        want = po.spectrum_window_local(oattrs, obin, edgesin, ellsin=ellsin, mesh=host(sel.value).astype('f8'))
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin), label='local mesh theory')

@@ -664,3 +664,19 @@ def test_window_local_selection_vs_bruteforce(ell, m, ellin):
     got = po.window_local_selection(a, rho, ell, m, ellin)
     want = po.window_local_selection_bruteforce(a, rho, ell, m, ellin)
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
+
+
+@pytest.mark.parametrize('ell,m,ellin', [(0, 0, 0), (2, -1, 0), (0, 0, 2), (2, 2, 2)])
+def test_window_local_terms_vs_bruteforce(ell, m, ellin):
+    """A_lm(s) = sum_m1 c2r(shell Y_lin,m1(khat)) Q_lm,m1(s) of the local window with the theory
+    on the mesh, against the addition-theorem double sum over cells and modes with
+    P_lin(xhat . khat): pins the pairing of the two Y_lin,m1, the conjugation, the lag
+    direction and the closed right edge of the shell."""
+    rng = np.random.default_rng(8)
+    a = po.Attrs(6, 60., 20.)
+    rho = rng.uniform(0.5, 1.5, (6, 6, 6))
+    kf = 2. * np.pi / 60.
+    e0, e1 = kf, 2. * kf  # closed: |k| = 2 kf is in, as in mesh2.py:1276
+    got = po.window_local_terms(a, rho, ell, m, ellin, e0, e1)
+    want = po.window_local_terms_bruteforce(a, rho, ell, m, ellin, e0, e1)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
