@@ -548,9 +548,7 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
         mattrs = meshes[0].attrs
     los, vlos, swap = _format_los(los)
     if vlos is None:
-        if smooth:
-            raise NotImplementedError("window flags=('smooth',) with a local line of sight is not implemented on this path")
-        return _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm, infinite)
+        return _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm, infinite, smooth)
     ells = bin.ells
     n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
     _norm = n3 / float(np.prod(mattrs.cellsize))
@@ -657,7 +655,7 @@ def _window_infinite(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
     return wmat.reshape(ncol, -1).T
 
 
-def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm, infinite):
+def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, norm, infinite, smooth=False):
     """compute_mesh2_spectrum_window with a local (firstpoint/endpoint) line of sight, theory line
     of sight 'firstpoint', wide-angle order 0 (mesh2.py:1133-1156, 1235-1305, 1463-1482).
 
@@ -670,7 +668,11 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     No flags (theory on the mesh): the terms Q_lm,m1 = c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0) are
     kept for every (l, m, m1); each closed input shell gives xi_m1 = c2r(shell Y_lin,m1(khat))
     (generated in the inverse x pass), then A_lm = sum_m1 xi_m1 Q_lm,m1 and one binning pass of
-    r2c(A_lm) Y_lm(khat) per (l, m)."""
+    r2c(A_lm) Y_lm(khat) per (l, m).
+
+    flags=('smooth',): per (lin, l) the selection terms times Y_lm(shat) Y_lin,m1(shat), summed
+    over (m, m1) on the device and binned once in separation shells, then Hankel-summed against
+    each input shell on the host (mesh2.py:1195-1233)."""
     theory_los = 'firstpoint'
     if isinstance(ellsin, tuple) and len(ellsin) == 2 and isinstance(ellsin[1], str):
         ellsin, theory_los = ellsin
@@ -690,7 +692,7 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     cell = np.asarray(mattrs.cellsize, dtype='f8')
     if infinite and not np.all(cell == cell[0]):
         raise NotImplementedError("window flags=('infinite',) needs equal cellsizes on this path")
-    if not all(int(m) >= 2 and int(m) & (int(m) - 1) == 0 for m in mattrs.meshsize):
+    if not smooth and not all(int(m) >= 2 and int(m) & (int(m) - 1) == 0 for m in mattrs.meshsize):
         raise NotImplementedError('a local line-of-sight window needs power-of-two meshsizes on this path')
     n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
     _norm = n3 / float(np.prod(mattrs.cellsize))
@@ -720,6 +722,9 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     xs = [torch.from_numpy(np.arange(int(n)) * float(d) + float(o)).to(device)
           for n, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]  # positions (mesh.py:1594-1603)
     ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]
+    if smooth:
+        wmat = _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm)
+        return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
     n = _lib.i32x3(mattrs.meshsize)
     stream = _lib.stream_handle(device)
     nbins, nedges, nin = bin.nbins, len(bin._edges1d), len(edgesin)
@@ -816,6 +821,53 @@ def _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, 
         del Q, xi
 
 
+def _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm):
+    """(n_ells n_bins, n_ellsin n_in) matrix of the local smooth window (mesh2.py:1195-1233)."""
+    device = rmesh.device
+    n = _lib.i32x3(mattrs.meshsize)
+    stream = _lib.stream_handle(device)
+    ells = bin.ells
+    sbin = BinMesh2CorrelationPoles(mattrs, edges=None, device=device)
+    empty = sbin.nmodes == 0
+    with np.errstate(invalid='ignore', divide='ignore'):
+        savg = np.where(empty, 0., sbin.xavg)
+        kout = np.where(bin.nmodes == 0, 0., bin.xavg)
+    snm = sbin.nmodes.astype('f8')
+    vcell = float(np.prod(mattrs.cellsize, dtype='f8'))
+    jn = {ell: _host_array_fn('mp_spherical_jn_array', ell, np.outer(kout, savg)) for ell in ells}
+    base = torch.empty_like(rmesh)
+    tmp = torch.empty_like(rmesh)
+    acc = torch.empty_like(rmesh)
+    blocks = []
+    for ellin in ellsin:
+        rows = np.zeros((len(edgesin), len(ells), bin.nbins))
+        xi = [_host_array_fn('mp_tophat_shell_xi_array', ellin, savg, float(e0), float(e1)) for e0, e1 in edgesin]
+        for ill, ell in enumerate(ells):
+            first = True
+            for m1 in range(-ellin, ellin + 1):
+                for m in range(-ell, ell + 1):
+                    _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
+                    _mul_ylm(base, tmp, mattrs, xs, ellin, m1, False)
+                    B = _r2c(tmp, mattrs.meshsize)
+                    _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
+                              _spectrum_pitch(B))
+                    Q = _c2r(B, mattrs.meshsize)
+                    del B
+                    _mul_ylm(Q, tmp, mattrs, ss, ell, m, False)
+                    _mul_ylm(tmp, acc, mattrs, ss, ellin, m1, not first)
+                    first = False
+            with np.errstate(invalid='ignore', divide='ignore'):
+                Qs = sbin._sums_dev(acc, [0], [False])[0].cpu().numpy() / snm
+            Qs *= 4. * np.pi * 4. * np.pi / (2 * ellin + 1) * vcell
+            if ell != 0:
+                Qs[0] = 0.
+            Qs = np.where(empty, 0., Qs)
+            for ie in range(len(edgesin)):
+                rows[ie, ill] = (-1)**(ell // 2) * (jn[ell] @ (snm * xi[ie] * Qs)) * rnorm[ill]
+        blocks.append(rows.reshape(len(edgesin), -1))
+    return np.concatenate(blocks, axis=0).T
+
+
 def _host_array_fn(name, ell, x, *args):
     x = np.ascontiguousarray(x, dtype='f8')
     out = np.empty_like(x)
@@ -870,11 +922,11 @@ def _window_smooth(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
     return np.real(np.concatenate(blocks, axis=0).T)
 
 
-def _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm):
+def _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm, los=None):
     """WindowMatrix container of mesh2.py:1463-1482."""
     ells = bin.ells
     nbins = bin.nbins
-    attrs = dict(los=vlos.tolist())
+    attrs = dict(los=vlos.tolist() if los is None else los)
     zeros = np.zeros(nbins)
     observable = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=zeros,
                                                        num_shotnoise=zeros, norm=np.full(nbins, norm[ill]), ell=ell, attrs=attrs)

@@ -1197,3 +1197,80 @@ def spectrum_window_local(attrs, bin, edgesin, ellsin=0, mesh=None, norm=None):
                 W[ie, ill * nb:(ill + 1) * nb] = np.real(4. * np.pi * bin(acc, remove_zero=ell == 0) * rnorm[ill])
         blocks.append(W)
     return np.concatenate(blocks, axis=0).T
+
+
+def window_local_smooth_terms(attrs, rmesh, ell, ellin):
+    """sum_{m, m1} 4 pi / (2 lin + 1) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0)(s) Y_lm(shat) Y_lin,m1(shat)
+    of the local smooth window (mesh2.py:1205-1210, wide-angle order 0), before the separation
+    binning, the 4 pi and V_cell."""
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    sx, sy, sz = separation_coords(attrs)
+    sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
+    A0 = r2c(rmesh)
+    out = 0.
+    for m1 in range(-ellin, ellin + 1):
+        for m in range(-ell, ell + 1):
+            Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(rmesh * real_ylm(ell, m, xx, xy, xz) * real_ylm(ellin, m1, xx, xy, xz))) * A0,
+                                                  attrs.meshsize)
+            out = out + Q * real_ylm(ell, m, sx, sy, sz) * real_ylm(ellin, m1, sx, sy, sz)
+    return out
+
+
+def window_local_smooth_terms_bruteforce(attrs, rmesh, ell, ellin):
+    """The same sum from the addition theorem: (2l+1)/(4 pi) sum_x rho(x) rho(x+s) P_l(xhat.shat)
+    P_lin(xhat.shat), the products taken as 0 where x or s is the zero vector; tiny meshes."""
+    n = [int(v) for v in attrs.meshsize]
+    xs = attrs.rcoords()
+    ss = separation_coords(attrs)
+    X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
+    xn = np.linalg.norm(X, axis=-1)
+    xhat = X / np.where(xn == 0, 1., xn)[..., None]
+    out = np.zeros(n)
+    for idx in np.ndindex(*n):
+        svec = np.array([ss[a][idx[a]] for a in range(3)])
+        sn = np.linalg.norm(svec)
+        if sn == 0.:
+            continue
+        mu = xhat @ (svec / sn)
+        f = np.where(xn == 0, 0., legendre(ell, mu) * legendre(ellin, mu))
+        shifted = np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2))
+        out[idx] = (2 * ell + 1) / (4. * np.pi) * np.sum(rmesh * shifted * f)
+    return out
+
+
+def spectrum_window_local_smooth(attrs, bin, edgesin, ellsin=0, mesh=None, norm=None):
+    """compute_mesh2_spectrum_window(flags=('smooth',)) with a local (firstpoint) line of sight,
+    theory line of sight 'firstpoint', wide-angle order 0 (mesh2.py:1195-1233): Qs(s) =
+    4 pi V_cell sbin(sum_{m, m1} Q_lm,m1 Y_lm(shat) Y_lin,m1(shat)), first shell zeroed for l > 0,
+    then (-1)^(l//2) sum_s nmodes(s) j_l(k savg) xi_lin(savg) Qs(s) rnorm per input shell."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    ellsin = (ellsin,) if np.ndim(ellsin) == 0 else tuple(ellsin)
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    rmesh = np.ones(tuple(int(v) for v in attrs.meshsize)) if mesh is None else (c2r(mesh, attrs.meshsize) if np.iscomplexobj(mesh) else mesh)
+    sbin = BinCorrelationPoles(attrs, edges=None)
+    with np.errstate(invalid='ignore'):
+        kout = np.where(bin.nmodes == 0, 0., bin.xavg)
+    empty = sbin.nmodes == 0
+    savg = np.where(empty, 0., sbin.xavg)
+    nb = len(bin.xavg)
+    blocks = []
+    for ellin in ellsin:
+        W = np.zeros((len(edgesin), len(ells) * nb))
+        for ill, ell in enumerate(ells):
+            Qs = 4. * np.pi * sbin(window_local_smooth_terms(attrs, rmesh, ell, ellin)) * np.prod(attrs.cellsize)
+            if ell != 0:
+                Qs[0] = 0.
+            Qs = np.where(empty, 0., Qs)
+            for ie, (e0, e1) in enumerate(edgesin):
+                tq = tophat_shell_xi(ellin, e0, e1, savg) * Qs
+                spec = (-1)**(ell // 2) * np.sum(sbin.nmodes[None, :] * spherical_jn(ell, kout[:, None] * savg[None, :]) * tq[None, :], axis=-1)
+                W[ie, ill * nb:(ill + 1) * nb] = spec * rnorm[ill]
+        blocks.append(W)
+    return np.concatenate(blocks, axis=0).T

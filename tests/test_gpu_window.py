@@ -117,8 +117,6 @@ def test_window_unsupported_raise():
     attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
     bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.05}, ells=(0, 2))
     with pytest.raises(NotImplementedError):
-        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=0, flags=('smooth',))
-    with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=((0, 2), 'local'))
     with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('fftlog',))
@@ -258,3 +256,27 @@ def test_window_local_mesh_theory(periodic, los, ellsin):
         wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los)
         want = po.spectrum_window_local(oattrs, obin, edgesin, ellsin=ellsin, mesh=host(sel.value).astype('f8'))
     assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin), label='local mesh theory')
+
+
+@pytest.mark.parametrize('periodic,los,ellsin', [(True, 'firstpoint', (0, 2)), (False, 'firstpoint', (0, 2, 4))])
+def test_window_local_smooth(periodic, los, ellsin):
+    """flags=('smooth',) with a local line of sight (mesh2.py:1195-1233): the selection terms times
+    Y_lm(shat) Y_lin,m1(shat) summed over (m, m1), binned in separation shells, Hankel-summed
+    against each input shell, against the oracle."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window local smooth', periodic, los, ellsin)
+    center = (0., 0., 0.) if periodic else (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    edgesin = np.arange(0.01, 0.12, 0.025)
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    if periodic:
+        wm = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('smooth',))
+        want = po.spectrum_window_local_smooth(oattrs, obin, edgesin, ellsin=ellsin)
+    else:
+        pos, w = _survey_selection(attrs, 100000, 23, scale=0.3)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los=los, flags=('smooth',))
+        want = po.spectrum_window_local_smooth(oattrs, obin, edgesin, ellsin=ellsin, mesh=host(sel.value).astype('f8'))
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='local smooth')

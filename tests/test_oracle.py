@@ -680,3 +680,15 @@ def test_window_local_terms_vs_bruteforce(ell, m, ellin):
     got = po.window_local_terms(a, rho, ell, m, ellin, e0, e1)
     want = po.window_local_terms_bruteforce(a, rho, ell, m, ellin, e0, e1)
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
+
+
+@pytest.mark.parametrize('ell,ellin', [(0, 0), (2, 0), (0, 2), (2, 2), (4, 2)])
+def test_window_local_smooth_terms_vs_bruteforce(ell, ellin):
+    """sum_{m, m1} Q_lm,m1(s) Y_lm(shat) Y_lin,m1(shat) of the local smooth window against the
+    addition-theorem sum over cells with P_l(xhat.shat) P_lin(xhat.shat)."""
+    rng = np.random.default_rng(9)
+    a = po.Attrs(6, 60., 20.)
+    rho = rng.uniform(0.5, 1.5, (6, 6, 6))
+    got = po.window_local_smooth_terms(a, rho, ell, ellin)
+    want = po.window_local_smooth_terms_bruteforce(a, rho, ell, ellin)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
