@@ -300,6 +300,24 @@ int mp_bin_ylm_real(void* stream, const float* mesh, const int32_t meshsize[3], 
                     const double* kz, const double* k2_thresholds, int nedges, int ell, int m, int remove_zero, double* out,
                     void* workspace, size_t workspace_bytes);
 
+/* compute_mesh2_spectrum_mean (jaxpower/mesh2.py:1485-1680) for a theory tabulated per multipole:
+ * P(k) = tvals[#(tkeys <= k^2) - 1] when that index is in [0, ntab), else 0 (tkeys: nkeys
+ * float64 squared-edge thresholds, or the sorted distinct k^2 of the mesh for a theory given as
+ * a function of |k|; device arrays).
+ *   mp_spectrum_mean_bin, global line of sight, one input multipole: out[p][b][re, im] = sum over
+ *     bin b (k = 0 included) of w Re(r2c(Q irfftn(P P_ellin(mu) scale))) P_ell[p](mu); Q a real
+ *     mesh; workspace mp_window_bin_workspace_bytes(meshsize, nell, nedges);
+ *   mp_theory_ylm_real: out = c2r(P Y_ell,m(khat) scale), the xi of the local line of sight
+ *     (mesh2.py:1640-1645), finished with mp_mul_acc and mp_bin_ylm_real; workspace
+ *     mp_window_shell_ylm_workspace_bytes(meshsize). */
+int mp_spectrum_mean_bin(void* stream, const float* Q, const int32_t meshsize[3], const double* kx, const double* ky,
+                         const double* kz, const double* k2_thresholds, int nedges, int nkeys, const double* tkeys, int ntab,
+                         const float* tvals, int ellin, const double los[3], float scale, int nell, const int32_t* ells,
+                         double* out, void* workspace, size_t workspace_bytes);
+int mp_theory_ylm_real(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz, int nkeys,
+                       const double* tkeys, int ntab, const float* tvals, int ell, int m, float scale, float* out, void* workspace,
+                       size_t workspace_bytes);
+
 /* Host arrays, for the smooth window (compute_mesh2_spectrum_window(flags=('smooth',)),
  * jaxpower/mesh2.py:1045-1076): out[i] = xi_ell(s[i]) of the shell [k0, k1) as above, and
  * out[i] = j_ell(x[i]) by the closed forms / series of jaxpower/utils.py:418-450; ell in 0..5. */

@@ -113,6 +113,12 @@ struct ShellGen {
   int ell_on;
   float scale;
   int ylm_on, ylm_ell, ylm_m;  // weight Y_lm(khat) instead of P_ell(mu) (local line of sight)
+  // tabulated radial theory instead of a shell (tn > 0): tvals[#(tkeys <= k^2) - 1] when that
+  // index is in [0, tn), else 0 (digitize on the squared-edge thresholds, or an exact lookup in
+  // the sorted distinct k^2 of the mesh)
+  const double* tkeys;
+  const float* tvals;
+  int tnkeys, tn;
   __device__ __forceinline__ float eval(int ix, int iy, int iz) const {
     const double kxv = kx[ix], kyv = ky[iy], kzv = kz[iz];
     double k2;
@@ -120,8 +126,19 @@ struct ShellGen {
 #pragma clang fp contract(off)
       k2 = (kxv * kxv + kyv * kyv) + kzv * kzv;
     }
-    if (!(k2 >= t_lo && k2 < t_hi)) return 0.f;
     float v = scale;
+    if (tn > 0) {
+      int lo = 0, hi = tnkeys;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (tkeys[mid] <= k2) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < 1 || lo > tn) return 0.f;
+      v *= tvals[lo - 1];
+    } else if (!(k2 >= t_lo && k2 < t_hi)) {
+      return 0.f;
+    }
     if (ylm_on) {
       v *= ylm_of_vector(ylm_ell, ylm_m, (float)kxv, (float)kyv, (float)kzv);
     } else if (ell_on) {

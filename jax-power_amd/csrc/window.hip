@@ -421,3 +421,70 @@ MP_EXPORT int mp_bin_ylm_real(void* stream, const float* mesh, const int32_t mes
   char* base = static_cast<char*>(workspace) + ws.used;
   return bin_partial(s, spec, sp, meshsize, 0, 1, 0, 0, geo, zero_los, 1, ells, rz, out, base, rest, 1, ms);
 }
+
+// The mean spectrum of a theory through a selection (jaxpower/mesh2.py:1485-1680), global line of
+// sight, one input multipole: the theory P(k) = tvals[#(tkeys <= k^2) - 1] (0 outside
+// [0, ntab)) times P_ellin(k.los/|k|) scale is convolved as r2c(Q irfftn(.)) and binned:
+// out[p][b][re, im] = sum over the modes of bin b (k = 0 included) of w Re(B) P_ell[p](mu).
+// tkeys (nkeys float64) and tvals (ntab float32) are device arrays; Q a real mesh.
+MP_EXPORT int mp_spectrum_mean_bin(void* stream, const float* Q, const int32_t meshsize[3], const double* kx, const double* ky,
+                                   const double* kz, const double* k2_thresholds, int nedges, int nkeys, const double* tkeys, int ntab,
+                                   const float* tvals, int ellin, const double los[3], float scale, int nell, const int32_t* ells,
+                                   double* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(Q && meshsize && kx && ky && kz && k2_thresholds && tkeys && tvals && ells && out && los, "mp_spectrum_mean_bin: null pointer");
+  MP_REQUIRE(nkeys >= 1 && ntab >= 1 && ntab <= nkeys, "mp_spectrum_mean_bin: %d values for %d keys", ntab, nkeys);
+  MP_REQUIRE(ellin >= 0 && ellin <= 5 && nell >= 1 && nell <= 6 && nedges >= 2, "mp_spectrum_mean_bin: bad multipoles or bins");
+  if (!xpass_supported(meshsize))
+    return set_error(MP_ERR_UNSUPPORTED, "mp_spectrum_mean_bin: meshsize (%d, %d, %d) not supported", meshsize[0], meshsize[1], meshsize[2]);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* spec = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  const size_t rest = workspace_bytes > ws.used ? workspace_bytes - ws.used : 0;
+  if (!spec || rest < bin_partial_workspace_bytes(meshsize, 1, nell, nedges))
+    return set_error(MP_ERR_WORKSPACE, "mp_spectrum_mean_bin: workspace of %zu bytes < %lld", workspace_bytes,
+                     (long long)mp_window_bin_workspace_bytes(meshsize, nell, nedges));
+  ShellGen g = make_shell(kx, ky, kz, 0., 0., ellin, los, scale);
+  g.tkeys = tkeys;
+  g.tvals = tvals;
+  g.tnkeys = nkeys;
+  g.tn = ntab;
+  int rc = convolve_shell_zy(s, g, spec, sp, Q, meshsize, 1.f);
+  if (rc) return rc;
+  BinGeom geo{};
+  geo.n0 = meshsize[0];
+  geo.n1 = meshsize[1];
+  geo.ncols = meshsize[2] / 2 + 1;
+  geo.nedges = nedges;
+  geo.kx = kx;
+  geo.ky = ky;
+  geo.kz = kz;
+  geo.thr = k2_thresholds;
+  int32_t rz[6] = {0, 0, 0, 0, 0, 0};
+  char* base = static_cast<char*>(workspace) + ws.used;
+  return bin_partial(s, spec, sp, meshsize, 0, 1, 0, 0, geo, los, nell, ells, rz, out, base, rest, 1);
+}
+
+// out = c2r(P(k) Y_ell,m(khat) scale) with the tabulated theory P of mp_spectrum_mean_bin: the
+// xi of the local line-of-sight mean (jaxpower/mesh2.py:1640-1645); workspace as
+// mp_window_shell_ylm.
+MP_EXPORT int mp_theory_ylm_real(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                                 int nkeys, const double* tkeys, int ntab, const float* tvals, int ell, int m, float scale, float* out,
+                                 void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && tkeys && tvals && out, "mp_theory_ylm_real: null pointer");
+  MP_REQUIRE(nkeys >= 1 && ntab >= 1 && ntab <= nkeys, "mp_theory_ylm_real: %d values for %d keys", ntab, nkeys);
+  MP_REQUIRE(ell >= 0 && ell <= 4 && m >= -ell && m <= ell, "mp_theory_ylm_real: (ell, m) = (%d, %d) not supported", ell, m);
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_theory_ylm_real: workspace too small");
+  ShellGen g = make_shell(kx, ky, kz, 0., 0., 0, nullptr, scale);
+  g.ylm_on = 1;
+  g.ylm_ell = ell;
+  g.ylm_m = m;
+  g.tkeys = tkeys;
+  g.tvals = tvals;
+  g.tnkeys = nkeys;
+  g.tn = ntab;
+  return irfft3d_shell((hipStream_t)stream, g, out, meshsize, scratch, sp);
+}

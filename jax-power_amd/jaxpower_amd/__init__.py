@@ -10,7 +10,8 @@ from .mesh import (MeshAttrs, BaseMeshField, RealMeshField, ComplexMeshField, Pa
                    read, paint, fftfreq, get_mesh_attrs, next_fft_size, compute_normalization, compute_box_normalization,
                    get_halo_for_new_mattrs)
 from .mesh2 import (BinMesh2SpectrumPoles, BinMesh2CorrelationPoles, compute_mesh2, compute_mesh2_spectrum,
-                    compute_mesh2_correlation, compute_mesh2_spectrum_window, compute_fkp2_normalization,
+                    compute_mesh2_correlation, compute_mesh2_spectrum_window, compute_mesh2_spectrum_mean,
+                    compute_fkp2_normalization,
                     compute_box2_normalization, compute_fkp2_shotnoise)
 from .mesh3 import BinMesh3SpectrumPoles, compute_mesh3, compute_mesh3_spectrum, compute_box3_normalization
 from .mock import generate_uniform_particles, generate_gaussian_mesh, generate_lognormal_particles
@@ -28,4 +29,4 @@ __all__ = ['MeshAttrs', 'BaseMeshField', 'RealMeshField', 'ComplexMeshField', 'P
            'generate_uniform_particles', 'generate_gaussian_mesh', 'generate_lognormal_particles', 'Mesh2SpectrumPole',
            'Mesh2SpectrumPoles', 'mesh2_spectrum_poles', 'Mesh2SpectrumBatch', 'resamplers', 'BinMesh3SpectrumPoles',
            'compute_mesh3', 'compute_mesh3_spectrum', 'compute_box3_normalization', 'Mesh3SpectrumPole', 'Mesh3SpectrumPoles',
-           'compute_mesh2_spectrum_window', 'WindowMatrix']
+           'compute_mesh2_spectrum_window', 'compute_mesh2_spectrum_mean', 'WindowMatrix']

@@ -868,6 +868,158 @@ def _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, b
     return np.concatenate(blocks, axis=0).T
 
 
+def _theory_tables(mattrs, theory, device):
+    """Parse the theory of compute_mesh2_spectrum_mean (mesh2.py:1549-1567) into
+    (theory_los, {(ell, wa): values}, keys): piecewise-constant values on the squared-edge
+    thresholds keys (Interpolator1D of order 0, utils.py:507-537), or, for callables of |k|,
+    values at the sorted distinct k^2 of the mesh."""
+    poles, kin, theory_los = theory, None, 'firstpoint'
+    if isinstance(poles, tuple) and isinstance(poles[-1], str):
+        theory_los = poles[-1]
+        poles = poles[0] if len(poles) == 2 else poles[:-1]
+    if isinstance(poles, tuple):
+        kin, poles = poles
+    if hasattr(poles, 'ells') and hasattr(poles, 'get'):  # a poles container with k_edges
+        e = poles.edges('k')
+        kin = np.append(e[:, 0], e[-1, 1])
+        poles = {ell: poles.get(ell).value() for ell in poles.ells}
+    if isinstance(poles, list):
+        poles = {ell: pole for ell, pole in zip((0, 2, 4), poles)}
+    if not isinstance(poles, dict):
+        raise NotImplementedError('theory must be given per multipole (dict, list or poles container) on this path')
+    poles = {ell if isinstance(ell, tuple) else (ell, 0): pole for ell, pole in poles.items()}
+    if any(wa != 0 for _, wa in poles):
+        raise NotImplementedError('wide-angle orders > 0 are not implemented on this path')
+    if all(callable(p) for p in poles.values()):
+        kx, ky, kz = (torch.from_numpy(k).to(device) for k in _kvectors(mattrs))
+        k2 = (kx[:, None, None]**2 + ky[None, :, None]**2) + kz[None, None, :]**2
+        keys = torch.unique(k2.reshape(-1)).cpu().numpy()  # sorted distinct k^2, summed as the kernels do
+        del k2
+        knorm = np.sqrt(keys)
+        values = {key: np.asarray(p(knorm), dtype='f8') * np.ones_like(knorm) for key, p in poles.items()}
+        return theory_los, values, keys, len(keys)
+    if kin is None:
+        raise ValueError('tabulated poles need their k (or k edges)')
+    kin = np.asarray(kin, dtype='f8')
+    values = {key: np.asarray(p, dtype='f8') for key, p in poles.items()}
+    n = len(next(iter(values.values())))
+    if len(kin) != n + 1:  # points: mid-point edges, ends extrapolated (utils.py:515-517)
+        tmp = (kin[:-1] + kin[1:]) / 2.
+        kin = np.concatenate([[tmp[0] - (kin[1] - kin[0])], tmp, [tmp[-1] + (kin[-1] - kin[-2])]])
+    return theory_los, values, _k2_thresholds(kin), n
+
+
+def compute_mesh2_spectrum_mean(*meshes, theory, bin=None, los='z'):
+    """Mean power-spectrum multipoles of a theory seen through a selection (mesh2.py:1485-1680),
+    for a theory given per multipole (dict/list of arrays on k or k edges, a poles container, or
+    callables of |k|; wide-angle order 0).  Global line of sight: the theory P(k) L_lin(mu) rnorm
+    on the mesh is convolved as r2c(Q c2r(.)), Q = c2r(A1 conj(A2)) (N^3 for a periodic box given
+    as a MeshAttrs), and binned with (2l+1) P_l(mu) in one fused pass per input multipole.  Local
+    (firstpoint/endpoint) line of sight with theory line of sight 'firstpoint':
+    4 pi bin(sum_m r2c(Q_m) Y_lm(khat)) with Q_m = sum_{lin, m1} 4 pi/(2 lin + 1)
+    c2r(P_lin rnorm Y_lin,m1(khat)) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0)."""
+    if not isinstance(bin, BinMesh2SpectrumPoles):
+        raise ValueError(f'bin must be BinMesh2SpectrumPoles, not {type(bin)}')
+    periodic = isinstance(meshes[0], MeshAttrs)
+    if periodic:
+        mattrs = meshes[0]
+        fields = (0, 0)
+    else:
+        meshes, fields = format_meshes(*meshes, nmeshes=2)
+        mattrs = meshes[0].attrs
+    if not all(int(m) >= 2 and int(m) & (int(m) - 1) == 0 for m in mattrs.meshsize):
+        raise NotImplementedError('compute_mesh2_spectrum_mean needs power-of-two meshsizes on this path')
+    device = bin._device
+    n3 = float(np.prod(mattrs.meshsize, dtype='f8'))
+    vcell = float(np.prod(mattrs.cellsize, dtype='f8'))
+    norm = n3 / vcell
+    rnorm = 1. / vcell
+    los, vlos, swap = _format_los(los)
+    attrs = dict(los=vlos.tolist() if vlos is not None else los)
+    ells = bin.ells
+    theory_los, values, keys, ntab = _theory_tables(mattrs, theory, device)
+    shape = tuple(int(v) for v in mattrs.meshsize)
+    n = _lib.i32x3(mattrs.meshsize)
+    t = bin._tables
+    nedges = len(bin._edges1d)
+    dkeys = torch.from_numpy(np.ascontiguousarray(keys, dtype='f8')).to(device)
+    dvals = {key: torch.from_numpy(np.ascontiguousarray(v, dtype='f4')).to(device) for key, v in values.items()}
+    stream = _lib.stream_handle(device)
+    sums = np.zeros((len(ells), bin.nbins))
+    if vlos is not None:
+        if periodic:
+            Q = torch.full(shape, n3, dtype=torch.float32, device=device)
+        else:
+            A1 = _2c(meshes[0])
+            P, _ = _product_spectrum(A1) if fields[1] == fields[0] else _product_spectrum(_2c(meshes[1]), A1)
+            Q = _c2r(P, mattrs.meshsize)
+            del P
+        even = [ell for ell in ells if ell % 2 == 0]
+        if even:
+            ws = _lib.workspace(_lib.load().mp_window_bin_workspace_bytes(n, len(even), nedges), device)
+            out = torch.empty((len(values), len(even), bin.nbins, 2), dtype=torch.float64, device=Q.device)
+            for i, ((ellin, _), v) in enumerate(dvals.items()):
+                _lib.call('mp_spectrum_mean_bin', stream, _lib.ptr(Q), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']),
+                          _lib.ptr(t['thr']), nedges, len(keys), _lib.ptr(dkeys), ntab, _lib.ptr(v), int(ellin), _lib.f64x3(vlos),
+                          rnorm, len(even), (_lib.ctypes.c_int32 * len(even))(*even), _lib.ptr(out[i]), _lib.ptr(ws), ws.numel())
+            tot = out[..., 0].sum(dim=0).cpu().numpy() if len(values) > 1 else out[0, ..., 0].cpu().numpy()
+            with np.errstate(invalid='ignore', divide='ignore'):
+                for ie, ell in enumerate(even):  # odd ells: P_l(mu) + P_l(-mu) = 0 (mesh2.py:1599-1602)
+                    sums[ells.index(ell)] = (2 * ell + 1) * tot[ie] / bin.nmodes
+    else:
+        if theory_los != 'firstpoint':
+            raise NotImplementedError(f'theory line of sight {theory_los!r} is not implemented on this path')
+        if max(ells) > 4 or max(ell for ell, _ in values) > 4:
+            raise NotImplementedError('local line-of-sight multipoles up to 4 on this path')
+        if periodic:
+            rmesh = torch.ones(shape, dtype=torch.float32, device=device)
+            A0 = _r2c(rmesh, mattrs.meshsize)
+        else:
+            if swap:
+                meshes = meshes[::-1]
+            rmesh = _2r(meshes[0]).value.contiguous().to(torch.float32)
+            A0 = _2c(meshes[0] if fields[1] == fields[0] else meshes[1]).value
+            device = rmesh.device
+        a0, a0_pitch = _as_spectrum(A0)
+        offset = mattrs.boxcenter - mattrs.boxsize / 2.
+        xs = [torch.from_numpy(np.arange(int(m)) * float(d) + float(o)).to(device)
+              for m, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]
+        ws_xi = _lib.workspace(_lib.load().mp_window_shell_ylm_workspace_bytes(n), device)
+        ws_bin = _lib.workspace(_lib.load().mp_bin_ylm_real_workspace_bytes(n, nedges), device)
+        xi = {}
+        for (ell1, _), v in dvals.items():
+            for m1 in range(-ell1, ell1 + 1):
+                xi[ell1, m1] = torch.empty(shape, dtype=torch.float32, device=device)
+                _lib.call('mp_theory_ylm_real', stream, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), len(keys),
+                          _lib.ptr(dkeys), ntab, _lib.ptr(v), int(ell1), int(m1), rnorm * 4. * np.pi / (2 * ell1 + 1),
+                          _lib.ptr(xi[ell1, m1]), _lib.ptr(ws_xi), ws_xi.numel())
+        base, tmp, acc = torch.empty_like(rmesh), torch.empty_like(rmesh), torch.empty_like(rmesh)
+        out = torch.empty((len(ells), 2 * max(ells) + 1, bin.nbins, 2), dtype=torch.float64, device=device)
+        for ill, ell in enumerate(ells):
+            for m in range(-ell, ell + 1):
+                _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
+                first = True
+                for (ell1, m1), x in xi.items():
+                    _mul_ylm(base, tmp, mattrs, xs, ell1, m1, False)
+                    B = _r2c(tmp, mattrs.meshsize)
+                    _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
+                              _spectrum_pitch(B))
+                    Qt = _c2r(B, mattrs.meshsize)
+                    del B
+                    _lib.call('mp_mul_acc', stream, _lib.ptr(x), _lib.ptr(Qt), _lib.ptr(acc), rmesh.numel(), int(not first))
+                    first = False
+                _lib.call('mp_bin_ylm_real', stream, _lib.ptr(acc), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']),
+                          _lib.ptr(t['thr']), nedges, int(ell), int(m), 0, _lib.ptr(out[ill, m + ell]), _lib.ptr(ws_bin), ws_bin.numel())
+        host_out = out[..., 0].cpu().numpy()
+        with np.errstate(invalid='ignore', divide='ignore'):
+            for ill, ell in enumerate(ells):
+                sums[ill] = 4. * np.pi * host_out[ill, :2 * ell + 1].sum(axis=0) / bin.nmodes
+    poles = [Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=sums[ill], num_shotnoise=np.zeros(bin.nbins),
+                               norm=np.full(bin.nbins, norm), volume=np.prod(mattrs.kfun) * bin.nmodes, ell=ell, attrs=attrs)
+             for ill, ell in enumerate(ells)]
+    return Mesh2SpectrumPoles(poles, attrs=attrs)
+
+
 def _host_array_fn(name, ell, x, *args):
     x = np.ascontiguousarray(x, dtype='f8')
     out = np.empty_like(x)

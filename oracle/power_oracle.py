@@ -1274,3 +1274,63 @@ def spectrum_window_local_smooth(attrs, bin, edgesin, ellsin=0, mesh=None, norm=
                 W[ie, ill * nb:(ill + 1) * nb] = spec * rnorm[ill]
         blocks.append(W)
     return np.concatenate(blocks, axis=0).T
+
+
+def theory_on_mesh(attrs, kin, values, edges):
+    """Interpolator1D(kin, |k|, edges=edges) of order 0 applied to values (utils.py:507-537):
+    piecewise constant by digitize(right=False) on the edges (points: mid-point edges with the
+    ends extrapolated), 0 outside."""
+    kin = np.asarray(kin, dtype='f8')
+    if not edges:
+        tmp = (kin[:-1] + kin[1:]) / 2.
+        kin = np.concatenate([[tmp[0] - (kin[1] - kin[0])], tmp, [tmp[-1] + (kin[-1] - kin[-2])]])
+    idx = np.digitize(knorm(attrs), kin, right=False) - 1
+    mask = (idx >= 0) & (idx <= len(kin) - 2)
+    return np.where(mask, np.asarray(values, dtype='f8')[np.where(mask, idx, 0)], 0.)
+
+
+def spectrum_mean(attrs, bin, kin, poles, edges=True, mesh=None, los='z'):
+    """compute_mesh2_spectrum_mean (mesh2.py:1485-1680) for a theory tabulated per multipole
+    (poles: dict ell -> values on kin), wide-angle order 0: global line of sight
+    num_l = (2l+1) bin(r2c(Q c2r(P rnorm)) P_l(mu)) with P = sum_lin P_lin(|k|) L_lin(mu),
+    Q = c2r(A conj A) (times N^3 for the periodic box, mesh None); local (firstpoint) line of
+    sight num_l = 4 pi bin(sum_m r2c(Q_m) Y_lm(khat)) with
+    Q_m = sum_{lin, m1} 4 pi/(2 lin + 1) c2r(P_lin rnorm Y_lin,m1(khat)) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0).
+    Returns num_raw per output ell (norm = N^3 / V_cell)."""
+    n3 = float(np.prod(attrs.meshsize))
+    rnorm = 1. / np.prod(attrs.cellsize)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    kn = knorm(attrs)
+    rmesh = np.ones(tuple(int(v) for v in attrs.meshsize)) if mesh is None else (c2r(mesh, attrs.meshsize) if np.iscomplexobj(mesh) else mesh)
+    tables = {ell: theory_on_mesh(attrs, kin, values, edges) for ell, values in poles.items()}
+    num = []
+    if not (isinstance(los, str) and los in ('firstpoint', 'local', 'endpoint')):
+        vlos = los_vector(los)
+        mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / np.where(kn == 0., 1., kn)
+        A = sum(tables[ell] * legendre(ell, mu) for ell in tables) * rnorm
+        if mesh is None:
+            A = A * n3
+        else:
+            A1 = r2c(rmesh)
+            A = r2c(c2r(A1 * np.conj(A1), attrs.meshsize) * c2r(A, attrs.meshsize))
+        for ell in bin.ells:
+            odd = ell % 2
+            leg = legendre(ell, mu) + (legendre(ell, -mu) if odd else 0.)
+            num.append(np.real((2 * ell + 1) / (1 + odd) * bin(A * leg)))
+        return num
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    A0 = r2c(rmesh)
+    for ell in bin.ells:
+        acc = 0.
+        for m in range(-ell, ell + 1):
+            Q = 0.
+            for ell1, table in tables.items():
+                for m1 in range(-ell1, ell1 + 1):
+                    xi = c2r(table * rnorm * real_ylm(ell1, m1, kx, ky, kz), attrs.meshsize)
+                    Q = Q + 4. * np.pi / (2 * ell1 + 1) * xi * c2r(np.conj(r2c(rmesh * real_ylm(ell, m, xx, xy, xz) * real_ylm(ell1, m1, xx, xy, xz))) * A0,
+                                                                 attrs.meshsize)
+            acc = acc + r2c(Q) * real_ylm(ell, m, kx, ky, kz)
+        num.append(np.real(4. * np.pi * bin(acc)))
+    return num

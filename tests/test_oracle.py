@@ -692,3 +692,33 @@ def test_window_local_smooth_terms_vs_bruteforce(ell, ellin):
     got = po.window_local_smooth_terms(a, rho, ell, ellin)
     want = po.window_local_smooth_terms_bruteforce(a, rho, ell, ellin)
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
+
+
+@pytest.mark.parametrize('los', ['x', 'firstpoint'])
+def test_spectrum_mean_equals_window_times_theory(los):
+    """For a theory tabulated on the input shells, the mean spectrum (mesh2.py:1485-1680) is the
+    window (mesh2.py:921-1305, pinned above) applied to it, in every output bin that does not hold
+    the k = 0 mode (the window drops it, the mean keeps it): two restatements of different code
+    paths of the reference checked against each other."""
+    rng = np.random.default_rng(12)
+    a = po.Attrs(8, 160., (300., 100., 200.))
+    kf = 2. * np.pi / 160.
+    bin = po.BinPoles(a, edges=np.arange(0., 4.5 * kf, kf), ells=(0, 2))
+    edgesin = np.column_stack([np.arange(0., 4.) * kf, np.arange(1., 5.) * kf])
+    edges1d = np.append(edgesin[:, 0], edgesin[-1, 1])
+    poles = {0: rng.uniform(1., 2., 4), 2: rng.uniform(-0.5, 0.5, 4)}
+    rho = rng.uniform(0.5, 1.5, (8, 8, 8))
+    num = po.spectrum_mean(a, bin, edges1d, poles, edges=True, mesh=rho, los=los)
+    norm = np.prod(a.meshsize) / np.prod(a.cellsize)
+    if los == 'firstpoint':
+        # closed right edges in the window: move them down by one ulp to match digitize's half-open shells
+        wedges = np.column_stack([edgesin[:, 0], np.nextafter(edgesin[:, 1], 0.)])
+        W = po.spectrum_window_local(a, bin, wedges, ellsin=(0, 2), mesh=rho)
+    else:
+        W = po.spectrum_window(a, bin, edgesin, ellsin=(0, 2), meshes=[rho], los=los)
+    theta = np.concatenate([poles[0], poles[2]])
+    got = W @ theta  # num_raw / norm with the window's default norm
+    nb = len(bin.nmodes)
+    for il in range(2):
+        want = num[il] / norm
+        np.testing.assert_allclose(got[il * nb + 1:(il + 1) * nb], want[1:], rtol=1e-9, atol=1e-12 * np.abs(want).max())
