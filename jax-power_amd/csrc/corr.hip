@@ -140,6 +140,72 @@ bin_real_kernel(RealBinArgs a, double* __restrict__ partial) {
   for (int i = threadIdx.x; i < hsize; i += CB_NT) dst[i] = hist[i];
 }
 
+// The same binning for more bins than an LDS histogram holds (e.g. the unique separation shells
+// of a large mesh): thresholds read from global memory and the running sums added straight to
+// out with FP64 atomics (out zeroed beforehand).  The order of the additions is not fixed, so the
+// sums can differ run to run in their last bits.
+template <int NE>
+__global__ void __launch_bounds__(CB_NT)
+bin_real_global_kernel(RealBinArgs a, double* __restrict__ out) {
+  const int nbins = a.nedges - 1;
+  const int nseg = (a.n2 + CB_SEG - 1) / CB_SEG;
+  const int64_t nwork = (int64_t)a.n0 * a.n1 * nseg;
+  for (int64_t q = (int64_t)blockIdx.x * CB_NT + threadIdx.x; q < nwork; q += (int64_t)gridDim.x * CB_NT) {
+    const int64_t row = q / nseg;
+    const int seg = (int)(q % nseg);
+    const int ix = (int)(row / a.n1), iy = (int)(row % a.n1);
+    const int z0 = seg * CB_SEG, z1 = min(z0 + CB_SEG, a.n2);
+    const double sxv = a.sx[ix], syv = a.sy[iy];
+    const double a2 = k2_xy(sxv, syv);
+    int b = first_bin(a.thr, a.nedges, k2_sum(a2, a.sz[z0]));
+    double acc[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) acc[e] = 0.;
+    const float fsx = (float)sxv, fsy = (float)syv;
+    const float lxy = fsx * a.los[0] + fsy * a.los[1];
+    const float fa2 = fsx * fsx + fsy * fsy;
+    const float* src = a.mesh + row * a.n2;
+    for (int iz = z0; iz < z1; ++iz) {
+      const double szv = a.sz[iz];
+      const int nb = walk_bin(a.thr, a.nedges, b, k2_sum(a2, szv));
+      if (nb != b) {
+        if (b >= 1 && b < a.nedges) {
+#pragma unroll
+          for (int e = 0; e < NE; ++e) {
+            if (acc[e] != 0.) atomicAdd(&out[e * nbins + b - 1], acc[e]);
+            acc[e] = 0.;
+          }
+        }
+        b = nb;
+      }
+      if (b < 1 || b >= a.nedges) continue;
+      const float v = src[iz];
+      float mu = 0.f;
+      if (a.need_mu) {
+        const float fsz = (float)szv;
+        const float s2 = fa2 + fsz * fsz;
+        mu = s2 > 0.f ? (lxy + fsz * a.los[2]) * rsqrtf(s2) : 0.f;
+      }
+      const bool origin = (ix == 0) & (iy == 0) & (iz == 0);
+      const float m2 = mu * mu;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        if (origin && a.rz[e]) continue;
+        acc[e] += (double)(v * legendre_eval(a.leg, e, mu, m2));
+      }
+    }
+    if (b >= 1 && b < a.nedges) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+        if (acc[e] != 0.) atomicAdd(&out[e * nbins + b - 1], acc[e]);
+    }
+  }
+}
+
+static bool real_bin_fits_lds(int nell, int nedges, int n2) {
+  return ((size_t)nell * (nedges - 1) + nedges + n2) * sizeof(double) <= 96 * 1024;
+}
+
 // out = conj(a) * b on half spectra, or |a|^2 (imaginary part 0) when b is null; out may be a.
 __global__ void spectrum_product_kernel(const float2* a, int64_t a_pitch, const float2* b, int64_t b_pitch,
                                         int64_t nrows, int ncols, float2* out, int64_t out_pitch) {
@@ -192,8 +258,6 @@ static int real_args(RealBinArgs& a, const float* mesh, const int32_t n[3], cons
 static int real_ells(RealBinArgs& a, const double los[3], int nell, const int32_t* ells, const int32_t* remove_zero) {
   MP_REQUIRE(ells && remove_zero, "correlation binning: null ells");
   MP_REQUIRE(nell >= 1 && nell <= CB_MAX_ELL, "correlation binning: %d ells not in 1..%d", nell, CB_MAX_ELL);
-  MP_REQUIRE(((size_t)nell * (a.nedges - 1) + a.nedges + a.n2) * sizeof(double) <= 96 * 1024,
-             "correlation binning: too many bins for the LDS histogram");
   a.nell = nell;
   a.need_mu = 0;
   for (int e = 0; e < nell; ++e) {
@@ -211,6 +275,12 @@ template <int NE>
 static int launch_real_bin(hipStream_t s, const RealBinArgs& a, double* out, void* ws, size_t ws_bytes) {
   const int hsize = NE * (a.nedges - 1);
   const int blocks = real_bin_blocks(a);
+  if (!real_bin_fits_lds(NE, a.nedges, a.n2)) {
+    int rc = fill_i32_dev(s, reinterpret_cast<int*>(out), 2 * (int64_t)hsize, 0);
+    if (rc) return rc;
+    MP_LAUNCH("bin_real", bin_real_global_kernel<NE>, dim3(blocks), dim3(CB_NT), 0, s, a, out);
+    return MP_OK;
+  }
   Workspace w(ws, ws_bytes);
   double* partial = w.take<double>((size_t)blocks * hsize);
   if (!partial) return set_error(MP_ERR_WORKSPACE, "correlation binning: workspace too small");
@@ -245,6 +315,7 @@ MP_EXPORT int64_t mp_bin_real_workspace_bytes(const int32_t meshsize[3], int nel
   a.n0 = meshsize[0];
   a.n1 = meshsize[1];
   a.n2 = meshsize[2];
+  if (!real_bin_fits_lds(nell < 1 ? 1 : nell, nedges, a.n2)) return 256;  // global atomics: no partial histograms
   return (int64_t)(align_up((size_t)real_bin_blocks(a) * (nell < 1 ? 1 : nell) * (nedges - 1 > 0 ? nedges - 1 : 1) * sizeof(double), 256) + 256);
 }
 
@@ -282,6 +353,7 @@ MP_EXPORT int mp_irfft3d_bin_real(void* stream, const void* in, int64_t in_pitch
   rc = real_ells(a, los, nell, ells, remove_zero);
   if (rc) return rc;
   MP_REQUIRE(in && out, "mp_irfft3d_bin_real: null pointer");
+  MP_REQUIRE(real_bin_fits_lds(nell, nedges, meshsize[2]), "mp_irfft3d_bin_real: too many bins for the LDS histogram");
   return irfft3d_bin_real((hipStream_t)stream, (const float2*)in, in_pitch, meshsize, a, out, workspace, workspace_bytes, x_done);
 }
 

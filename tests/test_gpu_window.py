@@ -334,3 +334,32 @@ def test_spectrum_mean_callable_theory():
     for ell in (0, 2):
         x, y = a.get(ell).values('num_raw'), b.get(ell).values('num_raw')
         assert np.allclose(x, y, rtol=1e-6, atol=1e-6 * np.abs(a.get(0).values('num_raw')).max()), ell
+
+
+def test_separation_binning_many_shells():
+    """Separation binning with more shells than an LDS histogram holds (the unique shells of a
+    128^3 mesh, the default of the smooth window) goes through FP64 global atomics: against the
+    oracle's bincount; and the smooth window at the reference's test size (64^3,
+    tests/test_mesh2.py:528-560)."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2CorrelationPoles, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('separation binning, unique shells at 128^3')
+    big = MeshAttrs(meshsize=128, boxsize=500., boxcenter=1000.)
+    sbin = BinMesh2CorrelationPoles(big, edges=None)
+    osbin = po.BinCorrelationPoles(po.Attrs(128, 500., 1000.), edges=None)
+    assert sbin.nbins > 8000 and np.array_equal(sbin.nmodes, osbin.nmodes)
+    rng = np.random.default_rng(2)
+    mesh = rng.normal(size=(128, 128, 128)).astype('f4')
+    got = sbin(to_dev(mesh))
+    want = osbin(mesh.astype('f8'))
+    ok = osbin.nmodes > 0
+    assert np.allclose(got[ok], want[ok], rtol=1e-6, atol=1e-6 * np.abs(want[ok]).max())
+    attrs = MeshAttrs(meshsize=64, boxsize=500., boxcenter=1000.)
+    oattrs = po.Attrs(64, 500., 1000.)
+    pos, w = _survey_selection(attrs, 300000, 31)
+    sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.01}, ells=(0, 2, 4))
+    edgesin = np.array([[0.1, 0.11]])
+    wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=(0, 2, 4), bin=bin, los='x', flags=('smooth',))
+    obin = po.BinPoles(oattrs, edges={'step': 0.01}, ells=(0, 2, 4))
+    want = po.spectrum_window_smooth(oattrs, obin, edgesin, ellsin=(0, 2, 4), meshes=[host(sel.value).astype('f8')], los='x')
+    assert_window_close(wm.value(), want, len(obin.nmodes), 1, label='smooth 64^3')
