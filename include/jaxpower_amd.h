@@ -318,6 +318,15 @@ int mp_theory_ylm_real(void* stream, const int32_t meshsize[3], const double* kx
                        const double* tkeys, int ntab, const float* tvals, int ell, int m, float scale, float* out, void* workspace,
                        size_t workspace_bytes);
 
+/* Host: real Gaunt coefficient int Y_l1,m1 Y_l2,m2 Y_l3,m3 dOmega with the library's own real
+ * spherical harmonics (the coefficients of jaxpower/mesh2.py:1365-1372, jaxpower/utils.py:540-600,
+ * in this library's per-m sign convention, which cancels in the estimator); l <= 4, NaN otherwise. */
+double mp_real_gaunt(int l1, int m1, int l2, int m2, int l3, int m3);
+
+/* y += alpha x on real meshes (device arrays of n floats): the Gaunt-weighted sums of the window
+ * with the 'local' theory line of sight (jaxpower/mesh2.py:1398-1402). */
+int mp_axpy(void* stream, float alpha, const float* x, float* y, int64_t n);
+
 /* Host arrays, for the smooth window (compute_mesh2_spectrum_window(flags=('smooth',)),
  * jaxpower/mesh2.py:1045-1076): out[i] = xi_ell(s[i]) of the shell [k0, k1) as above, and
  * out[i] = j_ell(x[i]) by the closed forms / series of jaxpower/utils.py:418-450; ell in 0..5. */

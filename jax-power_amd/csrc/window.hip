@@ -488,3 +488,78 @@ MP_EXPORT int mp_theory_ylm_real(void* stream, const int32_t meshsize[3], const 
   g.tn = ntab;
   return irfft3d_shell((hipStream_t)stream, g, out, meshsize, scratch, sp);
 }
+
+namespace mp {
+
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton iteration on P_n, the textbook method).
+static void gauss_legendre(int n, double* x, double* w) {
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5)), pp = 0.;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1., p2 = 0.;
+      for (int j = 1; j <= n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2. * j - 1.) * z * p2 - (j - 1.) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.);
+      const double dz = p1 / pp;
+      z -= dz;
+      if (fabs(dz) < 1e-15) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2. / ((1. - z * z) * pp * pp);
+  }
+}
+
+}  // namespace mp
+
+// Real Gaunt coefficient int Y_l1,m1 Y_l2,m2 Y_l3,m3 dOmega with this library's real spherical
+// harmonics (the ones every Y_lm kernel evaluates), by Gauss-Legendre quadrature in cos(theta)
+// times a uniform rule in phi, exact for l1 + l2 + l3 <= 12; l <= 4.  The real Gaunt products of
+// jaxpower/mesh2.py:1365-1372 (jaxpower/utils.py:540-600) pair each Y with the same Y elsewhere in
+// the estimator, so computing them with the library's own Y keeps the result independent of the
+// per-m sign convention.
+MP_EXPORT double mp_real_gaunt(int l1, int m1, int l2, int m2, int l3, int m3) {
+  const int ls[3] = {l1, l2, l3}, ms[3] = {m1, m2, m3};
+  for (int i = 0; i < 3; ++i)
+    if (ls[i] < 0 || ls[i] > 4 || ms[i] < -ls[i] || ms[i] > ls[i]) return NAN;
+  constexpr int NT = 16, NP = 32;
+  double x[NT], w[NT];
+  gauss_legendre(NT, x, w);
+  double sum = 0.;
+  for (int i = 0; i < NT; ++i) {
+    const double ct = x[i], st = sqrt(1. - ct * ct);
+    for (int j = 0; j < NP; ++j) {
+      const double phi = 2. * 3.14159265358979323846 * j / NP;
+      const float ux = (float)(st * cos(phi)), uy = (float)(st * sin(phi)), uz = (float)ct;
+      double f = 1.;
+      for (int q = 0; q < 3; ++q) f *= (double)ylm_unit(ls[q], ms[q], ux, uy, uz);
+      sum += w[i] * f;
+    }
+  }
+  return sum * 2. * 3.14159265358979323846 / NP;
+}
+
+namespace mp {
+
+__global__ void __launch_bounds__(MA_NT)
+axpy_kernel(float alpha, const float* __restrict__ x, float* __restrict__ y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * MA_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * MA_NT) y[i] += alpha * x[i];
+}
+
+}  // namespace mp
+
+// y += alpha x on real meshes of n floats: the Gaunt-weighted sums of the local-theory window
+// (jaxpower/mesh2.py:1398-1402, Qs[p] += rg Y_p,mp(shat) Q).
+MP_EXPORT int mp_axpy(void* stream, float alpha, const float* x, float* y, int64_t n) {
+  MP_REQUIRE(x && y && n >= 0, "mp_axpy: null pointer");
+  if (n == 0) return MP_OK;
+  int dev = 0, ncu = 0;
+  MP_CHECK_HIP(hipGetDevice(&dev));
+  MP_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int blocks = (int)std::min<int64_t>(ceil_div(n, MA_NT), (int64_t)ncu * 8);
+  MP_LAUNCH("window_axpy", axpy_kernel, dim3((unsigned)blocks), dim3(MA_NT), 0, (hipStream_t)stream, alpha, x, y, n);
+  return MP_OK;
+}

@@ -103,6 +103,8 @@ SIGNATURES = {
                                      c_void_p, c_int, P_F64, c_float, c_int, P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_theory_ylm_real': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                    c_float, c_void_p, c_void_p, c_size_t]),
+    'mp_real_gaunt': (c_double, [c_int, c_int, c_int, c_int, c_int, c_int]),
+    'mp_axpy': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int64]),
     'mp_reduce_workspace_bytes': (c_int64, []),
     'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),

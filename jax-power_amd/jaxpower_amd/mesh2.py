@@ -676,8 +676,10 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     theory_los = 'firstpoint'
     if isinstance(ellsin, tuple) and len(ellsin) == 2 and isinstance(ellsin[1], str):
         ellsin, theory_los = ellsin
-    if theory_los != 'firstpoint':
+    if theory_los not in ('firstpoint', 'local'):
         raise NotImplementedError(f'window theory line of sight {theory_los!r} is not implemented on this path')
+    if theory_los == 'local' and not infinite:
+        raise NotImplementedError("window theory line of sight 'local' is implemented for flags=('infinite',) only on this path")
     ellsin = [ellsin] if np.ndim(ellsin) == 0 and not isinstance(ellsin, tuple) else list(ellsin)
     pairs = []
     for ellin in ellsin:
@@ -724,6 +726,10 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]
     if smooth:
         wmat = _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm)
+        return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
+    if theory_los == 'local':
+        rmesh2 = rmesh if (periodic or fields[1] == fields[0]) else _2r(meshes[1]).value.contiguous().to(torch.float32)
+        wmat = _window_local_theory_infinite(rmesh, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, float(cell[0]))
         return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
     n = _lib.i32x3(mattrs.meshsize)
     stream = _lib.stream_handle(device)
@@ -819,6 +825,76 @@ def _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, 
                               _lib.ptr(t['thr']), nedges, int(ell), int(m), int(ell == 0), _lib.ptr(out[iin, ill, m + ell, ie]),
                               _lib.ptr(ws_bin), ws_bin.numel())
         del Q, xi
+
+
+_LOCAL_THEORY_COEFF2 = {(0, 0): [(0, 1.), (4, -7. / 18.)], (0, 2): [(2, 1. / 2.), (4, -5. / 18.)], (2, 2): [(4, 35. / 18.)]}
+_LOCAL_THEORY_COEFF2[2, 0] = _LOCAL_THEORY_COEFF2[0, 2]  # the local-theory multipoles, mesh2.py:1449-1452
+
+
+def _window_local_theory_infinite(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, cell):
+    """(n_ells n_bins, n_ellsin n_in) matrix of the window with the 'local' theory line of sight and
+    an infinite-resolution theory (mesh2.py:1382-1413, 1446-1457).  For (l1, l2) in {2, 0}^2 and
+    output (l, m) the Gaunt-weighted selection terms
+    Qs_p = V_cell sum G(p mp; l1 m1; l2 m2) Y_p,mp(shat) (4 pi)^2/((2l1+1)(2l2+1))
+    c2r(conj(r2c(rho1 Y_lm Y_l1,m1)) r2c(rho2 Y_l2,m2)) are formed on the device (Gaunt
+    coefficients in the library's own Y convention), then each p takes the shells through
+    mp_window_bin_tophat_local with Bessel order p; the input multipoles combine the (l1, l2)
+    blocks as mesh2.py:1454-1457 does."""
+    lib = _lib.load()
+    device = rmesh1.device
+    ells = bin.ells
+    n = _lib.i32x3(mattrs.meshsize)
+    stream = _lib.stream_handle(device)
+    ncell = rmesh1.numel()
+    nbins, nedges, nin = bin.nbins, len(bin._edges1d), len(edgesin)
+    t = bin._tables
+    ws = _lib.workspace(lib.mp_window_tophat_workspace_bytes(n, 1, nedges), device)
+    k0 = (_lib.ctypes.c_double * nin)(*[float(e) for e in edgesin[:, 0]])
+    k1 = (_lib.ctypes.c_double * nin)(*[float(e) for e in edgesin[:, 1]])
+    vcell = float(np.prod(mattrs.cellsize, dtype='f8'))
+    base, tmp, tmp2 = torch.empty_like(rmesh1), torch.empty_like(rmesh1), torch.empty_like(rmesh1)
+    out = torch.empty((nin, nbins, 2), dtype=torch.float64, device=device)
+    blocks = {}
+    for ell1, ell2 in ((2, 2), (2, 0), (0, 2), (0, 0)):
+        G2 = {}
+        for m2 in range(-ell2, ell2 + 1):
+            _mul_ylm(rmesh2, tmp, mattrs, xs, ell2, m2, False)
+            G2[m2] = _r2c(tmp, mattrs.meshsize)
+        ps = [p for p in (0, 2, 4) if abs(lib.mp_real_gaunt(p, 0, ell1, 0, ell2, 0)) > 1e-6]
+        W = np.zeros((nin, len(ells), nbins))
+        for ill, ell in enumerate(ells):
+            for m in range(-ell, ell + 1):
+                Qs = {p: torch.zeros_like(rmesh1) for p in ps}
+                _mul_ylm(rmesh1, base, mattrs, xs, ell, m, False)
+                for m1 in range(-ell1, ell1 + 1):
+                    _mul_ylm(base, tmp, mattrs, xs, ell1, m1, False)
+                    F = _r2c(tmp, mattrs.meshsize)
+                    for m2 in range(-ell2, ell2 + 1):
+                        B = _empty_spectrum(mattrs.meshsize, device)
+                        _lib.call('mp_spectrum_product', stream, _lib.ptr(F), _spectrum_pitch(F), _lib.ptr(G2[m2]), _spectrum_pitch(G2[m2]),
+                                  n, _lib.ptr(B), _spectrum_pitch(B))
+                        Q = _c2r(B, mattrs.meshsize)
+                        del B
+                        for p in ps:
+                            for mp in range(-p, p + 1):
+                                rg = lib.mp_real_gaunt(p, mp, ell1, m1, ell2, m2)
+                                if abs(rg) > 1e-6:
+                                    _mul_ylm(Q, tmp2, mattrs, ss, p, mp, False)
+                                    _lib.call('mp_axpy', stream, float(rg * (4. * np.pi)**2 / ((2 * ell1 + 1) * (2 * ell2 + 1))),
+                                              _lib.ptr(tmp2), _lib.ptr(Qs[p]), ncell)
+                    del F
+                for p in ps:
+                    _lib.call('mp_window_bin_tophat_local', stream, _lib.ptr(Qs[p]), n, cell, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
+                              _lib.ptr(t['kz']), _lib.ptr(t['thr']), nedges, nin, k0, k1, int(p), int(ell), int(m), int(ell == 0),
+                              vcell, _lib.ptr(out), _lib.ptr(ws), ws.numel())
+                    W[:, ill, :] += out[..., 0].cpu().numpy()
+        with np.errstate(invalid='ignore', divide='ignore'):
+            blocks[ell1, ell2] = W * (4. * np.pi * rnorm)[None, :, None] / bin.nmodes
+    wmat = np.zeros((len(ellsin), nin, len(ells), nbins))
+    for illin, ellin in enumerate(ellsin):
+        for key, lst in _LOCAL_THEORY_COEFF2.items():
+            wmat[illin] += sum(c * (ell == ellin) for ell, c in lst) * blocks[key]
+    return wmat.reshape(len(ellsin) * nin, -1).T
 
 
 def _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm):

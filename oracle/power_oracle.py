@@ -19,6 +19,7 @@ Conventions followed (all from the reference):
 Pinning of every function is in tests/test_oracle.py; a function listed as
 "parity unpinned" there has no independent check.
 """
+from functools import lru_cache
 from itertools import product
 import math
 
@@ -1334,3 +1335,77 @@ def spectrum_mean(attrs, bin, kin, poles, edges=True, mesh=None, los='z'):
             acc = acc + r2c(Q) * real_ylm(ell, m, kx, ky, kz)
         num.append(np.real(4. * np.pi * bin(acc)))
     return num
+
+
+@lru_cache(maxsize=None)
+def real_gaunt(l1, m1, l2, m2, l3, m3):
+    """int Y_l1,m1 Y_l2,m2 Y_l3,m3 dOmega with the oracle's real_ylm (utils.py:540-600 computes it
+    with sympy in the reference's convention, which differs per m by (-1)^|m|); Gauss-Legendre in
+    cos(theta) times a uniform rule in phi, exact for l1 + l2 + l3 <= 12."""
+    x, w = np.polynomial.legendre.leggauss(16)
+    phi = 2. * np.pi * np.arange(32) / 32
+    st = np.sqrt(1. - x**2)
+    ux, uy, uz = st[:, None] * np.cos(phi)[None, :], st[:, None] * np.sin(phi)[None, :], np.broadcast_to(x[:, None], (16, 32))
+    f = real_ylm(l1, m1, ux, uy, uz) * real_ylm(l2, m2, ux, uy, uz) * real_ylm(l3, m3, ux, uy, uz)
+    return float(np.sum(w[:, None] * f) * 2. * np.pi / 32)
+
+
+LOCAL_THEORY_COEFF2 = {(0, 0): [(0, 1.), (4, -7. / 18.)], (0, 2): [(2, 1. / 2.), (4, -5. / 18.)], (2, 2): [(4, 35. / 18.)]}
+LOCAL_THEORY_COEFF2[2, 0] = LOCAL_THEORY_COEFF2[0, 2]
+
+
+def spectrum_window_local_theory_infinite(attrs, bin, edgesin, ellsin=(0, 2, 4), mesh=None, mesh2=None, norm=None):
+    """compute_mesh2_spectrum_window(flags=('infinite',)) with a local line of sight and theory line
+    of sight 'local' (mesh2.py:1382-1413, 1446-1457): for (l1, l2) in {2, 0}^2 and output (l, m),
+    Qs_p = V_cell sum_{m1, m2, mp} G(p mp; l1 m1; l2 m2) Y_p,mp(shat)
+    (4 pi)^2/((2 l1 + 1)(2 l2 + 1)) c2r(conj(r2c(rho1 Y_lm Y_l1,m1)) r2c(rho2 Y_l2,m2)), then per
+    input shell 4 pi bin(Y_lm(khat) r2c(sum_p xi_p(|s|) Qs_p), remove_zero = (l == 0)) rnorm, and
+    the input multipoles are the combinations of mesh2.py:1449-1457."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    ones = np.ones(tuple(int(v) for v in attrs.meshsize))
+    rho1 = ones if mesh is None else mesh
+    rho2 = rho1 if mesh2 is None else mesh2
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    sx, sy, sz = separation_coords(attrs)
+    sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
+    sn = np.sqrt(sx**2 + sy**2 + sz**2)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    nb = len(bin.xavg)
+    tmp = {}
+    for ell1, ell2 in product((2, 0), (2, 0)):
+        W = np.zeros((len(edgesin), len(ells) * nb))
+        G2 = {m2: r2c(rho2 * real_ylm(ell2, m2, xx, xy, xz)) for m2 in range(-ell2, ell2 + 1)}
+        for ill, ell in enumerate(ells):
+            ps = [p for p in (0, 2, 4) if abs(real_gaunt(p, 0, ell1, 0, ell2, 0)) > 1e-10]
+            for m in range(-ell, ell + 1):
+                Qs = {p: 0. for p in ps}
+                for m1 in range(-ell1, ell1 + 1):
+                    F = r2c(rho1 * real_ylm(ell, m, xx, xy, xz) * real_ylm(ell1, m1, xx, xy, xz))
+                    for m2 in range(-ell2, ell2 + 1):
+                        Q = (4. * np.pi)**2 / ((2 * ell1 + 1) * (2 * ell2 + 1)) * c2r(np.conj(F) * G2[m2], attrs.meshsize)
+                        for p in ps:
+                            for mp in range(-p, p + 1):
+                                rg = real_gaunt(p, mp, ell1, m1, ell2, m2)
+                                if abs(rg) > 1e-10:
+                                    Qs[p] = Qs[p] + rg * real_ylm(p, mp, sx, sy, sz) * Q
+                ykm = real_ylm(ell, m, kx, ky, kz)
+                for ie, (e0, e1) in enumerate(edgesin):
+                    xi = sum(tophat_shell_xi(p, e0, e1, sn) * Qs[p] * np.prod(attrs.cellsize) for p in ps)
+                    spec = 4. * np.pi * bin(ykm * r2c(xi), remove_zero=ell == 0) * rnorm[ill]
+                    W[ie, ill * nb:(ill + 1) * nb] += np.real(spec)
+        tmp[ell1, ell2] = W
+    out = np.zeros((len(ellsin), len(edgesin), len(ells) * nb))
+    for illin, ellin in enumerate(ellsin):
+        for key, lst in LOCAL_THEORY_COEFF2.items():
+            coeff = sum(c * (ell == ellin) for ell, c in lst)
+            out[illin] += coeff * tmp[key]
+    return out.reshape(len(ellsin) * len(edgesin), -1).T

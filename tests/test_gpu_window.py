@@ -363,3 +363,30 @@ def test_separation_binning_many_shells():
     obin = po.BinPoles(oattrs, edges={'step': 0.01}, ells=(0, 2, 4))
     want = po.spectrum_window_smooth(oattrs, obin, edgesin, ellsin=(0, 2, 4), meshes=[host(sel.value).astype('f8')], los='x')
     assert_window_close(wm.value(), want, len(obin.nmodes), 1, label='smooth 64^3')
+
+
+@pytest.mark.parametrize('periodic', [True, False])
+def test_window_local_theory_infinite(periodic):
+    """flags=('infinite',) with a local line of sight and theory line of sight 'local'
+    (mesh2.py:1382-1413, 1446-1457): Gaunt-weighted selection terms, Bessel shells of order p,
+    Y_lm(khat) binning, combined into the input multipoles; against the oracle, whose Y_lm and
+    Gaunt coefficients follow another sign convention (only the convention-free result is compared)."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window local theory infinite', periodic)
+    center = (0., 0., 0.) if periodic else (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    edgesin = np.arange(0.01, 0.12, 0.025)
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    if periodic:
+        wm = compute_mesh2_spectrum_window(attrs, edgesin=edgesin, ellsin=((0, 2, 4), 'local'), bin=bin, los='firstpoint',
+                                           flags=('infinite',))
+        want = po.spectrum_window_local_theory_infinite(oattrs, obin, edgesin, ellsin=(0, 2, 4))
+    else:
+        pos, w = _survey_selection(attrs, 100000, 37, scale=0.3)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=((0, 2, 4), 'local'), bin=bin, los='firstpoint',
+                                           flags=('infinite',))
+        want = po.spectrum_window_local_theory_infinite(oattrs, obin, edgesin, ellsin=(0, 2, 4), mesh=host(sel.value).astype('f8'))
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='local theory infinite')

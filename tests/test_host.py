@@ -228,3 +228,24 @@ def test_spherical_jn_and_shell_arrays_host():
         got = _host_array_fn('mp_tophat_shell_xi_array', ell, x * 10., 0.02, 0.05)
         want = po.tophat_shell_xi(ell, 0.02, 0.05, x * 10.)
         assert np.allclose(got, want, rtol=1e-8, atol=1e-9 * np.abs(want).max()), ell
+
+
+def test_real_gaunt_host_invariants():
+    """The library's real Gaunt coefficients (its own Y_lm convention): the same convention-free
+    invariant as the oracle's, the m-independent G(0 0; l m; l m) = 1/sqrt(4 pi), and |G| equal to
+    the oracle's coefficient by coefficient (conventions differ by signs only)."""
+    from jaxpower_amd import _lib
+    from oracle import power_oracle as po
+    lib = _lib.load()
+    for l in (2, 4):
+        for m in range(-l, l + 1):
+            assert abs(lib.mp_real_gaunt(0, 0, l, m, l, m) - 0.5 / np.sqrt(np.pi)) < 1e-6
+    for l1, l2, l3 in [(0, 2, 2), (2, 2, 2), (4, 2, 2), (4, 4, 2)]:
+        tot_lib = tot_or = 0.
+        for a in range(-l1, l1 + 1):
+            for b in range(-l2, l2 + 1):
+                for c in range(-l3, l3 + 1):
+                    g = lib.mp_real_gaunt(l1, a, l2, b, l3, c)
+                    tot_lib += g * g
+                    tot_or += po.real_gaunt(l1, a, l2, b, l3, c)**2
+        assert abs(tot_lib - tot_or) < 1e-6 * max(tot_or, 1e-3), (l1, l2, l3, tot_lib, tot_or)

@@ -722,3 +722,30 @@ def test_spectrum_mean_equals_window_times_theory(los):
     for il in range(2):
         want = num[il] / norm
         np.testing.assert_allclose(got[il * nb + 1:(il + 1) * nb], want[1:], rtol=1e-9, atol=1e-12 * np.abs(want).max())
+
+
+def test_real_gaunt_vs_reference_table():
+    """The oracle's real Gaunt coefficients against entries of the reference's table
+    (jaxpower/utils.py:600-), whose Y_lm differ from the oracle's by (-1)^|m| per factor, and the
+    convention-free invariant sum_{m1 m2 m3} G^2 = (2l1+1)(2l2+1)(2l3+1)/(4 pi) (l1 l2 l3; 0 0 0)^2."""
+    from scipy.special import factorial
+    table = {((0, 0), (2, -2), (2, -2)): 0.28209479177387814, ((2, -2), (2, -1), (2, 1)): 0.15607834722743974,
+             ((2, -1), (2, -1), (2, 0)): 0.0901118757864343, ((2, 0), (2, 2), (2, 2)): -0.18022375157286857,
+             ((2, 1), (2, 1), (2, 2)): 0.15607834722743988, ((4, -4), (2, -2), (2, 2)): 0.23841361350444812,
+             ((4, -3), (2, -2), (2, 1)): 0.16858388283618375}
+    for key, value in table.items():
+        sign = (-1)**sum(abs(m) for _, m in key)
+        got = po.real_gaunt(*[v for lm in key for v in lm])
+        assert abs(sign * got - value) < 1e-12, (key, got, value)
+
+    def three_j_000(a, b, c):
+        g = (a + b + c) // 2
+        if (a + b + c) % 2 or c > a + b or c < abs(a - b):
+            return 0.
+        return (-1)**g * np.sqrt(factorial(2 * g - 2 * a) * factorial(2 * g - 2 * b) * factorial(2 * g - 2 * c) / factorial(2 * g + 1)) \
+            * factorial(g) / (factorial(g - a) * factorial(g - b) * factorial(g - c))
+
+    for l1, l2, l3 in [(0, 2, 2), (2, 2, 2), (4, 2, 2), (2, 2, 0), (4, 4, 2)]:
+        tot = sum(po.real_gaunt(l1, a, l2, b, l3, c)**2 for a in range(-l1, l1 + 1) for b in range(-l2, l2 + 1) for c in range(-l3, l3 + 1))
+        want = (2 * l1 + 1) * (2 * l2 + 1) * (2 * l3 + 1) / (4. * np.pi) * three_j_000(l1, l2, l3)**2
+        assert abs(tot - want) < 1e-12, (l1, l2, l3, tot, want)
