@@ -678,8 +678,8 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
         ellsin, theory_los = ellsin
     if theory_los not in ('firstpoint', 'local'):
         raise NotImplementedError(f'window theory line of sight {theory_los!r} is not implemented on this path')
-    if theory_los == 'local' and not infinite:
-        raise NotImplementedError("window theory line of sight 'local' is implemented for flags=('infinite',) only on this path")
+    if theory_los == 'local' and not (infinite or smooth):
+        raise NotImplementedError("window theory line of sight 'local' needs flags=('infinite',) or ('smooth',) on this path")
     ellsin = [ellsin] if np.ndim(ellsin) == 0 and not isinstance(ellsin, tuple) else list(ellsin)
     pairs = []
     for ellin in ellsin:
@@ -724,12 +724,12 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     xs = [torch.from_numpy(np.arange(int(n)) * float(d) + float(o)).to(device)
           for n, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]  # positions (mesh.py:1594-1603)
     ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]
-    if smooth:
-        wmat = _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm)
-        return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
     if theory_los == 'local':
         rmesh2 = rmesh if (periodic or fields[1] == fields[0]) else _2r(meshes[1]).value.contiguous().to(torch.float32)
-        wmat = _window_local_theory_infinite(rmesh, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, float(cell[0]))
+        wmat = _window_local_theory(rmesh, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, float(cell[0]), smooth)
+        return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
+    if smooth:
+        wmat = _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm)
         return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
     n = _lib.i32x3(mattrs.meshsize)
     stream = _lib.stream_handle(device)
@@ -831,7 +831,7 @@ _LOCAL_THEORY_COEFF2 = {(0, 0): [(0, 1.), (4, -7. / 18.)], (0, 2): [(2, 1. / 2.)
 _LOCAL_THEORY_COEFF2[2, 0] = _LOCAL_THEORY_COEFF2[0, 2]  # the local-theory multipoles, mesh2.py:1449-1452
 
 
-def _window_local_theory_infinite(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, cell):
+def _window_local_theory(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, cell, smooth=False):
     """(n_ells n_bins, n_ellsin n_in) matrix of the window with the 'local' theory line of sight and
     an infinite-resolution theory (mesh2.py:1382-1413, 1446-1457).  For (l1, l2) in {2, 0}^2 and
     output (l, m) the Gaunt-weighted selection terms
@@ -839,7 +839,9 @@ def _window_local_theory_infinite(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsi
     c2r(conj(r2c(rho1 Y_lm Y_l1,m1)) r2c(rho2 Y_l2,m2)) are formed on the device (Gaunt
     coefficients in the library's own Y convention), then each p takes the shells through
     mp_window_bin_tophat_local with Bessel order p; the input multipoles combine the (l1, l2)
-    blocks as mesh2.py:1454-1457 does."""
+    blocks as mesh2.py:1454-1457 does.  With smooth (mesh2.py:1346-1380) the terms are multiplied
+    by Y_lm(shat) instead, summed over m, binned once per p in separation shells and
+    Hankel-summed on the host."""
     lib = _lib.load()
     device = rmesh1.device
     ells = bin.ells
@@ -853,7 +855,16 @@ def _window_local_theory_infinite(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsi
     k1 = (_lib.ctypes.c_double * nin)(*[float(e) for e in edgesin[:, 1]])
     vcell = float(np.prod(mattrs.cellsize, dtype='f8'))
     base, tmp, tmp2 = torch.empty_like(rmesh1), torch.empty_like(rmesh1), torch.empty_like(rmesh1)
+    tmp3 = torch.empty_like(rmesh1)
     out = torch.empty((nin, nbins, 2), dtype=torch.float64, device=device)
+    if smooth:
+        sbin = BinMesh2CorrelationPoles(mattrs, edges=None, device=device)
+        empty = sbin.nmodes == 0
+        with np.errstate(invalid='ignore', divide='ignore'):
+            savg = np.where(empty, 0., sbin.xavg)
+            kout = np.where(bin.nmodes == 0, 0., bin.xavg)
+        snm = sbin.nmodes.astype('f8')
+        jn = {ell: _host_array_fn('mp_spherical_jn_array', ell, np.outer(kout, savg)) for ell in ells}
     blocks = {}
     for ell1, ell2 in ((2, 2), (2, 0), (0, 2), (0, 0)):
         G2 = {}
@@ -863,8 +874,11 @@ def _window_local_theory_infinite(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsi
         ps = [p for p in (0, 2, 4) if abs(lib.mp_real_gaunt(p, 0, ell1, 0, ell2, 0)) > 1e-6]
         W = np.zeros((nin, len(ells), nbins))
         for ill, ell in enumerate(ells):
-            for m in range(-ell, ell + 1):
+            if smooth:
                 Qs = {p: torch.zeros_like(rmesh1) for p in ps}
+            for m in range(-ell, ell + 1):
+                if not smooth:
+                    Qs = {p: torch.zeros_like(rmesh1) for p in ps}
                 _mul_ylm(rmesh1, base, mattrs, xs, ell, m, False)
                 for m1 in range(-ell1, ell1 + 1):
                     _mul_ylm(base, tmp, mattrs, xs, ell1, m1, False)
@@ -880,14 +894,31 @@ def _window_local_theory_infinite(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsi
                                 rg = lib.mp_real_gaunt(p, mp, ell1, m1, ell2, m2)
                                 if abs(rg) > 1e-6:
                                     _mul_ylm(Q, tmp2, mattrs, ss, p, mp, False)
+                                    if smooth:
+                                        _mul_ylm(tmp2, tmp3, mattrs, ss, ell, m, False)
                                     _lib.call('mp_axpy', stream, float(rg * (4. * np.pi)**2 / ((2 * ell1 + 1) * (2 * ell2 + 1))),
-                                              _lib.ptr(tmp2), _lib.ptr(Qs[p]), ncell)
+                                              _lib.ptr(tmp3 if smooth else tmp2), _lib.ptr(Qs[p]), ncell)
                     del F
+                if smooth:
+                    continue
                 for p in ps:
                     _lib.call('mp_window_bin_tophat_local', stream, _lib.ptr(Qs[p]), n, cell, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
                               _lib.ptr(t['kz']), _lib.ptr(t['thr']), nedges, nin, k0, k1, int(p), int(ell), int(m), int(ell == 0),
                               vcell, _lib.ptr(out), _lib.ptr(ws), ws.numel())
                     W[:, ill, :] += out[..., 0].cpu().numpy()
+            if smooth:  # Hankel sums of the binned terms (mesh2.py:1366-1380)
+                for p in ps:
+                    with np.errstate(invalid='ignore', divide='ignore'):
+                        Qp = sbin._sums_dev(Qs[p], [0], [False])[0].cpu().numpy() / snm * (4. * np.pi * vcell)
+                    if ell != 0:
+                        Qp[0] = 0.
+                    Qp = np.where(empty, 0., Qp)
+                    for ie, (e0, e1) in enumerate(edgesin):
+                        xi = _host_array_fn('mp_tophat_shell_xi_array', p, savg, float(e0), float(e1))
+                        W[ie, ill] += (-1)**(ell // 2) * (jn[ell] @ (snm * xi * Qp)) * rnorm[ill]
+        if smooth:
+            blocks[ell1, ell2] = W
+            continue
         with np.errstate(invalid='ignore', divide='ignore'):
             blocks[ell1, ell2] = W * (4. * np.pi * rnorm)[None, :, None] / bin.nmodes
     wmat = np.zeros((len(ellsin), nin, len(ells), nbins))

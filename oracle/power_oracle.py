@@ -1409,3 +1409,65 @@ def spectrum_window_local_theory_infinite(attrs, bin, edgesin, ellsin=(0, 2, 4),
             coeff = sum(c * (ell == ellin) for ell, c in lst)
             out[illin] += coeff * tmp[key]
     return out.reshape(len(ellsin) * len(edgesin), -1).T
+
+
+def spectrum_window_local_theory_smooth(attrs, bin, edgesin, ellsin=(0, 2, 4), mesh=None, mesh2=None, norm=None):
+    """compute_mesh2_spectrum_window(flags=('smooth',)) with a local line of sight and theory line of
+    sight 'local' (mesh2.py:1346-1380, 1446-1457): per (l1, l2) and output l,
+    Q_p = 4 pi V_cell sbin(sum_{m, m1, m2, mp} G(p mp; l1 m1; l2 m2) Y_lm(shat) Y_p,mp(shat)
+    (4 pi)^2/((2 l1 + 1)(2 l2 + 1)) c2r(conj(r2c(rho1 Y_lm Y_l1,m1)) r2c(rho2 Y_l2,m2))), first shell
+    zeroed for l > 0, Hankel-summed with xi_p of each input shell, then combined into the input
+    multipoles."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    ones = np.ones(tuple(int(v) for v in attrs.meshsize))
+    rho1 = ones if mesh is None else mesh
+    rho2 = rho1 if mesh2 is None else mesh2
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    sx, sy, sz = separation_coords(attrs)
+    sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
+    sbin = BinCorrelationPoles(attrs, edges=None)
+    with np.errstate(invalid='ignore'):
+        kout = np.where(bin.nmodes == 0, 0., bin.xavg)
+    empty = sbin.nmodes == 0
+    savg = np.where(empty, 0., sbin.xavg)
+    nb = len(bin.xavg)
+    tmp = {}
+    for ell1, ell2 in product((2, 0), (2, 0)):
+        W = np.zeros((len(edgesin), len(ells) * nb))
+        G2 = {m2: r2c(rho2 * real_ylm(ell2, m2, xx, xy, xz)) for m2 in range(-ell2, ell2 + 1)}
+        for ill, ell in enumerate(ells):
+            ps = [p for p in (0, 2, 4) if abs(real_gaunt(p, 0, ell1, 0, ell2, 0)) > 1e-10]
+            acc = {p: 0. for p in ps}
+            for m in range(-ell, ell + 1):
+                for m1 in range(-ell1, ell1 + 1):
+                    F = r2c(rho1 * real_ylm(ell, m, xx, xy, xz) * real_ylm(ell1, m1, xx, xy, xz))
+                    for m2 in range(-ell2, ell2 + 1):
+                        Q = (4. * np.pi)**2 / ((2 * ell1 + 1) * (2 * ell2 + 1)) * c2r(np.conj(F) * G2[m2], attrs.meshsize)
+                        for p in ps:
+                            for mp in range(-p, p + 1):
+                                rg = real_gaunt(p, mp, ell1, m1, ell2, m2)
+                                if abs(rg) > 1e-10:
+                                    acc[p] = acc[p] + rg * real_ylm(ell, m, sx, sy, sz) * real_ylm(p, mp, sx, sy, sz) * Q
+            for p in ps:
+                Qp = 4. * np.pi * sbin(acc[p]) * np.prod(attrs.cellsize)
+                if ell != 0:
+                    Qp[0] = 0.
+                Qp = np.where(empty, 0., Qp)
+                for ie, (e0, e1) in enumerate(edgesin):
+                    tq = tophat_shell_xi(p, e0, e1, savg) * Qp
+                    spec = (-1)**(ell // 2) * np.sum(sbin.nmodes[None, :] * spherical_jn(ell, kout[:, None] * savg[None, :]) * tq[None, :], axis=-1)
+                    W[ie, ill * nb:(ill + 1) * nb] += spec * rnorm[ill]
+        tmp[ell1, ell2] = W
+    out = np.zeros((len(ellsin), len(edgesin), len(ells) * nb))
+    for illin, ellin in enumerate(ellsin):
+        for key, lst in LOCAL_THEORY_COEFF2.items():
+            out[illin] += sum(c * (ell == ellin) for ell, c in lst) * tmp[key]
+    return out.reshape(len(ellsin) * len(edgesin), -1).T

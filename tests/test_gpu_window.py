@@ -390,3 +390,26 @@ def test_window_local_theory_infinite(periodic):
                                            flags=('infinite',))
         want = po.spectrum_window_local_theory_infinite(oattrs, obin, edgesin, ellsin=(0, 2, 4), mesh=host(sel.value).astype('f8'))
     assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='local theory infinite')
+
+
+@pytest.mark.parametrize('periodic', [True, False])
+def test_window_local_theory_smooth(periodic):
+    """flags=('smooth',) with the 'local' theory line of sight (mesh2.py:1346-1380, 1446-1457)
+    against the oracle (another Y_lm and Gaunt sign convention)."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window local theory smooth', periodic)
+    center = (0., 0., 0.) if periodic else (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    edgesin = np.arange(0.01, 0.12, 0.025)
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    mesh = None
+    inputs = [attrs]
+    if not periodic:
+        pos, w = _survey_selection(attrs, 100000, 41, scale=0.3)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        inputs, mesh = [sel], host(sel.value).astype('f8')
+    wm = compute_mesh2_spectrum_window(*inputs, edgesin=edgesin, ellsin=((0, 2, 4), 'local'), bin=bin, los='firstpoint', flags=('smooth',))
+    want = po.spectrum_window_local_theory_smooth(oattrs, obin, edgesin, ellsin=(0, 2, 4), mesh=mesh)
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='local theory smooth')
