@@ -323,6 +323,13 @@ int mp_theory_ylm_real(void* stream, const int32_t meshsize[3], const double* kx
  * in this library's per-m sign convention, which cancels in the estimator); l <= 4, NaN otherwise. */
 double mp_real_gaunt(int l1, int m1, int l2, int m2, int l3, int m3);
 
+/* The shell of the window with the 'local' theory line of sight and the theory on the mesh
+ * (jaxpower/mesh2.py:1436): out = c2r([k2_lo <= k^2 < k2_hi] Y_l1,m1(khat) Y_l2,m2(-khat) scale),
+ * a real mesh; workspace mp_window_shell_ylm_workspace_bytes(meshsize); l1, l2 <= 4. */
+int mp_window_shell_ylm2(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                         double k2_lo, double k2_hi, int l1, int m1, int l2, int m2, float scale, float* out, void* workspace,
+                         size_t workspace_bytes);
+
 /* y += alpha x on real meshes (device arrays of n floats): the Gaunt-weighted sums of the window
  * with the 'local' theory line of sight (jaxpower/mesh2.py:1398-1402). */
 int mp_axpy(void* stream, float alpha, const float* x, float* y, int64_t n);

@@ -113,6 +113,7 @@ struct ShellGen {
   int ell_on;
   float scale;
   int ylm_on, ylm_ell, ylm_m;  // weight Y_lm(khat) instead of P_ell(mu) (local line of sight)
+  int ylm2_on, ylm2_ell, ylm2_m;  // and a second factor Y_l2,m2(-khat) ('local' theory line of sight)
   // tabulated radial theory instead of a shell (tn > 0): tvals[#(tkeys <= k^2) - 1] when that
   // index is in [0, tn), else 0 (digitize on the squared-edge thresholds, or an exact lookup in
   // the sorted distinct k^2 of the mesh)
@@ -141,6 +142,7 @@ struct ShellGen {
     }
     if (ylm_on) {
       v *= ylm_of_vector(ylm_ell, ylm_m, (float)kxv, (float)kyv, (float)kzv);
+      if (ylm2_on) v *= ylm_of_vector(ylm2_ell, ylm2_m, -(float)kxv, -(float)kyv, -(float)kzv);
     } else if (ell_on) {
       const double mu = k2 > 0. ? (kxv * los[0] + kyv * los[1] + kzv * los[2]) / sqrt(k2) : 0.;
       const float m = (float)mu;

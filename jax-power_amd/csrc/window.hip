@@ -563,3 +563,26 @@ MP_EXPORT int mp_axpy(void* stream, float alpha, const float* x, float* y, int64
   MP_LAUNCH("window_axpy", axpy_kernel, dim3((unsigned)blocks), dim3(MA_NT), 0, (hipStream_t)stream, alpha, x, y, n);
   return MP_OK;
 }
+
+// The configuration-space shell of the window with the 'local' theory line of sight and the
+// theory on the mesh (jaxpower/mesh2.py:1436): out = c2r([k2_lo <= k^2 < k2_hi] Y_l1,m1(khat)
+// Y_l2,m2(-khat) scale), a real mesh; workspace as mp_window_shell_ylm.  l1, l2 <= 4.
+MP_EXPORT int mp_window_shell_ylm2(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                                   double k2_lo, double k2_hi, int l1, int m1, int l2, int m2, float scale, float* out,
+                                   void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && out, "mp_window_shell_ylm2: null pointer");
+  MP_REQUIRE(l1 >= 0 && l1 <= 4 && m1 >= -l1 && m1 <= l1 && l2 >= 0 && l2 <= 4 && m2 >= -l2 && m2 <= l2,
+             "mp_window_shell_ylm2: (l1, m1, l2, m2) = (%d, %d, %d, %d) not supported", l1, m1, l2, m2);
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_window_shell_ylm2: workspace too small");
+  ShellGen g = make_shell(kx, ky, kz, k2_lo, k2_hi, 0, nullptr, scale);
+  g.ylm_on = 1;
+  g.ylm_ell = l1;
+  g.ylm_m = m1;
+  g.ylm2_on = 1;
+  g.ylm2_ell = l2;
+  g.ylm2_m = m2;
+  return irfft3d_shell((hipStream_t)stream, g, out, meshsize, scratch, sp);
+}

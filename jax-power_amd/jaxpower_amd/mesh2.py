@@ -678,8 +678,6 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
         ellsin, theory_los = ellsin
     if theory_los not in ('firstpoint', 'local'):
         raise NotImplementedError(f'window theory line of sight {theory_los!r} is not implemented on this path')
-    if theory_los == 'local' and not (infinite or smooth):
-        raise NotImplementedError("window theory line of sight 'local' needs flags=('infinite',) or ('smooth',) on this path")
     ellsin = [ellsin] if np.ndim(ellsin) == 0 and not isinstance(ellsin, tuple) else list(ellsin)
     pairs = []
     for ellin in ellsin:
@@ -726,7 +724,10 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]
     if theory_los == 'local':
         rmesh2 = rmesh if (periodic or fields[1] == fields[0]) else _2r(meshes[1]).value.contiguous().to(torch.float32)
-        wmat = _window_local_theory(rmesh, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, float(cell[0]), smooth)
+        if not (infinite or smooth):
+            wmat = _window_local_theory_mesh(rmesh, rmesh2, mattrs, xs, edgesin, ellsin, bin, rnorm)
+        else:
+            wmat = _window_local_theory(rmesh, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, float(cell[0]), smooth)
         return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
     if smooth:
         wmat = _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm)
@@ -829,6 +830,76 @@ def _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, 
 
 _LOCAL_THEORY_COEFF2 = {(0, 0): [(0, 1.), (4, -7. / 18.)], (0, 2): [(2, 1. / 2.), (4, -5. / 18.)], (2, 2): [(4, 35. / 18.)]}
 _LOCAL_THEORY_COEFF2[2, 0] = _LOCAL_THEORY_COEFF2[0, 2]  # the local-theory multipoles, mesh2.py:1449-1452
+
+
+def _window_local_theory_mesh(rmesh1, rmesh2, mattrs, xs, edgesin, ellsin, bin, rnorm):
+    """(n_ells n_bins, n_ellsin n_in) matrix of the window with the 'local' theory line of sight
+    and the theory on the mesh (mesh2.py:1415-1457): per (l1, l2) the terms
+    Q_lm,m1,m2 = (4 pi)^2/((2l1+1)(2l2+1)) c2r(conj(r2c(rho1 Y_lm Y_l1,m1)) r2c(rho2 Y_l2,m2)) are
+    kept on the device; each closed input shell gives xi_m1,m2 = c2r(shell Y_l1,m1(khat)
+    Y_l2,m2(-khat)), then A_lm = sum xi Q and one Y_lm(khat) binning pass per (l, m)."""
+    lib = _lib.load()
+    device = rmesh1.device
+    ells = bin.ells
+    n = _lib.i32x3(mattrs.meshsize)
+    stream = _lib.stream_handle(device)
+    ncell = rmesh1.numel()
+    nbins, nedges, nin = bin.nbins, len(bin._edges1d), len(edgesin)
+    t = bin._tables
+    ws_shell = _lib.workspace(lib.mp_window_shell_ylm_workspace_bytes(n), device)
+    ws_bin = _lib.workspace(lib.mp_bin_ylm_real_workspace_bytes(n, nedges), device)
+    lo = _k2_thresholds(edgesin[:, 0])
+    hi = _k2_thresholds(np.nextafter(edgesin[:, 1], np.inf))  # closed right edge (mesh2.py:1436)
+    tmp, acc = torch.empty_like(rmesh1), torch.empty_like(rmesh1)
+    out = torch.empty((nin, len(ells), 2 * max(ells) + 1, nbins, 2), dtype=torch.float64, device=device)
+    blocks = {}
+    for ell1, ell2 in ((2, 2), (2, 0), (0, 2), (0, 0)):
+        c12 = (4. * np.pi)**2 / ((2 * ell1 + 1) * (2 * ell2 + 1))
+        G2 = {}
+        for m2 in range(-ell2, ell2 + 1):
+            _mul_ylm(rmesh2, tmp, mattrs, xs, ell2, m2, False)
+            G2[m2] = _r2c(tmp, mattrs.meshsize)
+        base = torch.empty_like(rmesh1)
+        Q = {}
+        for ell in ells:
+            for m in range(-ell, ell + 1):
+                _mul_ylm(rmesh1, base, mattrs, xs, ell, m, False)
+                for m1 in range(-ell1, ell1 + 1):
+                    _mul_ylm(base, tmp, mattrs, xs, ell1, m1, False)
+                    F = _r2c(tmp, mattrs.meshsize)
+                    for m2 in range(-ell2, ell2 + 1):
+                        B = _empty_spectrum(mattrs.meshsize, device)
+                        _lib.call('mp_spectrum_product', stream, _lib.ptr(F), _spectrum_pitch(F), _lib.ptr(G2[m2]), _spectrum_pitch(G2[m2]),
+                                  n, _lib.ptr(B), _spectrum_pitch(B))
+                        Q[ell, m, m1, m2] = _c2r(B, mattrs.meshsize)
+                        del B
+                    del F
+        pairs = [(m1, m2) for m1 in range(-ell1, ell1 + 1) for m2 in range(-ell2, ell2 + 1)]
+        xi = torch.empty((len(pairs),) + tuple(rmesh1.shape), dtype=torch.float32, device=device)
+        for ie in range(nin):
+            for i, (m1, m2) in enumerate(pairs):
+                _lib.call('mp_window_shell_ylm2', stream, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), float(lo[ie]),
+                          float(hi[ie]), int(ell1), int(m1), int(ell2), int(m2), c12, _lib.ptr(xi[i]), _lib.ptr(ws_shell),
+                          ws_shell.numel())
+            for ill, ell in enumerate(ells):
+                for m in range(-ell, ell + 1):
+                    for i, (m1, m2) in enumerate(pairs):
+                        _lib.call('mp_mul_acc', stream, _lib.ptr(xi[i]), _lib.ptr(Q[ell, m, m1, m2]), _lib.ptr(acc), ncell, int(i > 0))
+                    _lib.call('mp_bin_ylm_real', stream, _lib.ptr(acc), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']),
+                              _lib.ptr(t['thr']), nedges, int(ell), int(m), int(ell == 0), _lib.ptr(out[ie, ill, m + ell]),
+                              _lib.ptr(ws_bin), ws_bin.numel())
+        sums = out[..., 0].cpu().numpy()
+        W = np.zeros((nin, len(ells), nbins))
+        for ill, ell in enumerate(ells):
+            W[:, ill] = sums[:, ill, :2 * ell + 1].sum(axis=1)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            blocks[ell1, ell2] = W * (4. * np.pi * rnorm)[None, :, None] / bin.nmodes
+        del Q, xi
+    wmat = np.zeros((len(ellsin), nin, len(ells), nbins))
+    for illin, ellin in enumerate(ellsin):
+        for key, lst in _LOCAL_THEORY_COEFF2.items():
+            wmat[illin] += sum(c * (ell == ellin) for ell, c in lst) * blocks[key]
+    return wmat.reshape(len(ellsin) * nin, -1).T
 
 
 def _window_local_theory(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, cell, smooth=False):

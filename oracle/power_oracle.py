@@ -1471,3 +1471,55 @@ def spectrum_window_local_theory_smooth(attrs, bin, edgesin, ellsin=(0, 2, 4), m
         for key, lst in LOCAL_THEORY_COEFF2.items():
             out[illin] += sum(c * (ell == ellin) for ell, c in lst) * tmp[key]
     return out.reshape(len(ellsin) * len(edgesin), -1).T
+
+
+def spectrum_window_local_theory(attrs, bin, edgesin, ellsin=(0, 2, 4), mesh=None, mesh2=None, norm=None):
+    """compute_mesh2_spectrum_window without flags, local line of sight, theory line of sight
+    'local' (mesh2.py:1415-1457): per (l1, l2) and closed input shell [e0, e1],
+    A_lm = sum_{m1, m2} c2r(shell Y_l1,m1(khat) Y_l2,m2(-khat)) (4 pi)^2/((2 l1 + 1)(2 l2 + 1))
+    c2r(conj(r2c(rho1 Y_lm Y_l1,m1)) r2c(rho2 Y_l2,m2)), then 4 pi bin(sum_m r2c(A_lm) Y_lm(khat),
+    remove_zero = (l == 0)) rnorm, combined into the input multipoles."""
+    n3 = float(np.prod(attrs.meshsize))
+    _norm = n3 / np.prod(attrs.cellsize)
+    ells = bin.ells
+    norm = np.broadcast_to(_norm if norm is None else np.asarray(norm, dtype='f8'), (len(ells),))
+    rnorm = _norm / norm / n3
+    edgesin = np.asarray(edgesin, dtype='f8')
+    if edgesin.ndim != 2:
+        edgesin = np.column_stack([edgesin[:-1], edgesin[1:]])
+    ones = np.ones(tuple(int(v) for v in attrs.meshsize))
+    rho1 = ones if mesh is None else mesh
+    rho2 = rho1 if mesh2 is None else mesh2
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    kn = knorm(attrs)
+    nb = len(bin.xavg)
+    tmp = {}
+    for ell1, ell2 in product((2, 0), (2, 0)):
+        W = np.zeros((len(edgesin), len(ells) * nb))
+        G2 = {m2: r2c(rho2 * real_ylm(ell2, m2, xx, xy, xz)) for m2 in range(-ell2, ell2 + 1)}
+        Q = {}
+        for ell in ells:
+            for m in range(-ell, ell + 1):
+                for m1 in range(-ell1, ell1 + 1):
+                    F = r2c(rho1 * real_ylm(ell, m, xx, xy, xz) * real_ylm(ell1, m1, xx, xy, xz))
+                    for m2 in range(-ell2, ell2 + 1):
+                        Q[ell, m, m1, m2] = (4. * np.pi)**2 / ((2 * ell1 + 1) * (2 * ell2 + 1)) * c2r(np.conj(F) * G2[m2], attrs.meshsize)
+        for ie, (e0, e1) in enumerate(edgesin):
+            shell = (kn >= e0) & (kn <= e1)
+            xi = {(m1, m2): c2r(shell * real_ylm(ell1, m1, kx, ky, kz) * real_ylm(ell2, m2, -kx, -ky, -kz), attrs.meshsize)
+                  for m1 in range(-ell1, ell1 + 1) for m2 in range(-ell2, ell2 + 1)}
+            for ill, ell in enumerate(ells):
+                acc = 0.
+                for m in range(-ell, ell + 1):
+                    A = sum(xi[m1, m2] * Q[ell, m, m1, m2] for (m1, m2) in xi)
+                    acc = acc + r2c(A) * real_ylm(ell, m, kx, ky, kz)
+                W[ie, ill * nb:(ill + 1) * nb] = np.real(4. * np.pi * bin(acc, remove_zero=ell == 0) * rnorm[ill])
+        tmp[ell1, ell2] = W
+    out = np.zeros((len(ellsin), len(edgesin), len(ells) * nb))
+    for illin, ellin in enumerate(ellsin):
+        for key, lst in LOCAL_THEORY_COEFF2.items():
+            out[illin] += sum(c * (ell == ellin) for ell, c in lst) * tmp[key]
+    return out.reshape(len(ellsin) * len(edgesin), -1).T

@@ -117,7 +117,7 @@ def test_window_unsupported_raise():
     attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
     bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.05}, ells=(0, 2))
     with pytest.raises(NotImplementedError):
-        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=((0, 2), 'local'))
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=[(0, 1), (2, 0)])
     with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('fftlog',))
 
@@ -413,3 +413,27 @@ def test_window_local_theory_smooth(periodic):
     wm = compute_mesh2_spectrum_window(*inputs, edgesin=edgesin, ellsin=((0, 2, 4), 'local'), bin=bin, los='firstpoint', flags=('smooth',))
     want = po.spectrum_window_local_theory_smooth(oattrs, obin, edgesin, ellsin=(0, 2, 4), mesh=mesh)
     assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin) - 1, label='local theory smooth')
+
+
+@pytest.mark.parametrize('periodic', [True, False])
+def test_window_local_theory_mesh(periodic):
+    """No flags with the 'local' theory line of sight (mesh2.py:1415-1457): shells weighted by
+    Y_l1,m1(khat) Y_l2,m2(-khat), the stored selection terms, Y_lm(khat) binning, against the oracle."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window local theory mesh', periodic)
+    center = (0., 0., 0.) if periodic else (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    kf = 2. * np.pi / 400.
+    edgesin = np.array([[kf, 3 * kf], [3 * kf, 5 * kf]])
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.03}, ells=(0, 2, 4))
+    mesh = None
+    inputs = [attrs]
+    if not periodic:
+        pos, w = _survey_selection(attrs, 100000, 43, scale=0.3)
+        sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+        inputs, mesh = [sel], host(sel.value).astype('f8')
+    wm = compute_mesh2_spectrum_window(*inputs, edgesin=edgesin, ellsin=((0, 2, 4), 'local'), bin=bin, los='firstpoint')
+    want = po.spectrum_window_local_theory(oattrs, obin, edgesin, ellsin=(0, 2, 4), mesh=mesh)
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin), label='local theory mesh')
