@@ -330,6 +330,12 @@ int mp_window_shell_ylm2(void* stream, const int32_t meshsize[3], const double* 
                          double k2_lo, double k2_hi, int l1, int m1, int l2, int m2, float scale, float* out, void* workspace,
                          size_t workspace_bytes);
 
+/* out = in |c|^power on a real mesh with c = (cx[i], cy[j], cz[k]) (positions or separations): the
+ * wide-angle factors |x|^-wa and |s|^wa of the firstpoint theory (jaxpower/mesh2.py:1244, 1269,
+ * 1642); in place allowed. */
+int mp_mul_radial_power(void* stream, const float* in, float* out, const int32_t meshsize[3], const double* cx,
+                        const double* cy, const double* cz, double power);
+
 /* y += alpha x on real meshes (device arrays of n floats): the Gaunt-weighted sums of the window
  * with the 'local' theory line of sight (jaxpower/mesh2.py:1398-1402). */
 int mp_axpy(void* stream, float alpha, const float* x, float* y, int64_t n);

@@ -445,6 +445,12 @@ def _product_spectrum(A1, A2=None, inverse_x=False):
     return out, False
 
 
+def _mul_radial_power(value, out, mattrs, coords, power):
+    """out = value |c|^power on real meshes; coords are three device float64 tables."""
+    _lib.call('mp_mul_radial_power', _lib.stream_handle(value.device), _lib.ptr(value), _lib.ptr(out), _lib.i32x3(mattrs.meshsize),
+              _lib.ptr(coords[0]), _lib.ptr(coords[1]), _lib.ptr(coords[2]), float(power))
+
+
 def _mul_ylm(value, out, mattrs, coords, ell, m, accumulate):
     """out (+)= value * Y_lm(coords) on real meshes; coords are three device float64 tables."""
     _lib.call('mp_mul_ylm', _lib.stream_handle(value.device), _lib.ptr(value), _lib.ptr(out), _lib.i32x3(mattrs.meshsize),
@@ -682,10 +688,11 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     pairs = []
     for ellin in ellsin:
         ellin, wa = ellin if isinstance(ellin, tuple) else (ellin, 0)
-        if wa != 0:
-            raise NotImplementedError('wide-angle orders > 0 are not implemented on this path')
-        pairs.append(int(ellin))
-    ellsin = pairs
+        if wa != 0 and theory_los != 'firstpoint':
+            raise NotImplementedError("wide-angle orders > 0 need the 'firstpoint' theory line of sight")
+        pairs.append((int(ellin), int(wa)))
+    was = [wa for _, wa in pairs]
+    ellsin = [ellin for ellin, _ in pairs]
     ells = bin.ells
     if max(ells) > 4 or max(ellsin) > 4:
         raise NotImplementedError('local line-of-sight window multipoles up to 4 on this path')
@@ -730,8 +737,8 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
             wmat = _window_local_theory(rmesh, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, float(cell[0]), smooth)
         return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
     if smooth:
-        wmat = _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm)
-        return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint')
+        wmat = _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, was)
+        return _window_matrix(wmat, bin, edgesin, ellsin, np.zeros(3), norm, los='firstpoint', was=was)
     n = _lib.i32x3(mattrs.meshsize)
     stream = _lib.stream_handle(device)
     nbins, nedges, nin = bin.nbins, len(bin._edges1d), len(edgesin)
@@ -745,18 +752,24 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
     # per (ellin, l, m): the binned sums of every input shell, summed over m on the host
     out = torch.empty((len(ellsin), len(ells), 2 * max(ells) + 1, nin, nbins, 2), dtype=torch.float64, device=device)
     if not infinite:
-        _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, bin, out)
+        _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, out, was)
     for iin, ellin in enumerate(ellsin if infinite else ()):
+        wa = was[iin]
         for ill, ell in enumerate(ells):
             for m in range(-ell, ell + 1):
                 _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
+                if wa:
+                    _mul_radial_power(base, base, mattrs, xs, -wa)
                 for m1 in range(-ellin, ellin + 1):
                     _mul_ylm(base, tmp, mattrs, xs, ellin, m1, False)
                     B = _r2c(tmp, mattrs.meshsize)
                     _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
                               _spectrum_pitch(B))
-                    _mul_ylm(_c2r(B, mattrs.meshsize), Qs, mattrs, ss, ellin, m1, m1 != -ellin)
-                    del B
+                    q = _c2r(B, mattrs.meshsize)
+                    if wa:
+                        _mul_radial_power(q, q, mattrs, ss, wa)
+                    _mul_ylm(q, Qs, mattrs, ss, ellin, m1, m1 != -ellin)
+                    del B, q
                 scale = 4. * np.pi / (2 * ellin + 1) * float(np.prod(mattrs.cellsize))
                 _lib.call('mp_window_bin_tophat_local', stream, _lib.ptr(Qs), n, float(cell[0]), _lib.ptr(t['kx']), _lib.ptr(t['ky']),
                           _lib.ptr(t['kz']), _lib.ptr(t['thr']), nedges, nin, k0, k1, int(ellin), int(ell), int(m), int(ell == 0),
@@ -777,11 +790,12 @@ def _window_local(meshes, fields, periodic, mattrs, edgesin, ellsin, bin, swap, 
                                      for ill, ell in enumerate(ells)], attrs=attrs)
     kin = edgesin.mean(axis=-1)
     theory = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=kin, k_edges=edgesin, nmodes=np.zeros(len(kin), dtype='i8'),
-                                                   num_raw=np.zeros(len(kin)), ell=ell) for ell in ellsin])
+                                                   num_raw=np.zeros(len(kin)), ell=ell, attrs=dict(wa_order=wa))
+                                 for ell, wa in zip(ellsin, was)])
     return WindowMatrix(observable=observable, theory=theory, value=wmat)
 
 
-def _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, bin, out):
+def _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, out, was=None):
     """Binned sums of the local window with the theory on the mesh (mesh2.py:1262-1305) into
     out[iin, ill, m + l, ie] (without the 4 pi / (2 lin + 1) of Q and the 4 pi rnorm / nmodes)."""
     device = rmesh.device
@@ -800,17 +814,22 @@ def _window_local_mesh_theory(rmesh, a0, a0_pitch, mattrs, xs, edgesin, ellsin, 
     tmp = torch.empty_like(rmesh)
     acc = torch.empty_like(rmesh)
     for iin, ellin in enumerate(ellsin):
+        wa = was[iin] if was else 0
         nm1 = 2 * ellin + 1
         Q = {}
         for ell in ells:
             for m in range(-ell, ell + 1):
                 _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
+                if wa:
+                    _mul_radial_power(base, base, mattrs, xs, -wa)
                 for m1 in range(-ellin, ellin + 1):
                     _mul_ylm(base, tmp, mattrs, xs, ellin, m1, False)
                     B = _r2c(tmp, mattrs.meshsize)
                     _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
                               _spectrum_pitch(B))
                     Q[ell, m, m1] = _c2r(B, mattrs.meshsize)
+                    if wa:
+                        _mul_radial_power(Q[ell, m, m1], Q[ell, m, m1], mattrs, ss, wa)
                     del B
         xi = torch.empty((nm1,) + tuple(rmesh.shape), dtype=torch.float32, device=device)
         for ie in range(len(edgesin)):
@@ -999,7 +1018,7 @@ def _window_local_theory(rmesh1, rmesh2, mattrs, xs, ss, edgesin, ellsin, bin, r
     return wmat.reshape(len(ellsin) * nin, -1).T
 
 
-def _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm):
+def _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, bin, rnorm, was=None):
     """(n_ells n_bins, n_ellsin n_in) matrix of the local smooth window (mesh2.py:1195-1233)."""
     device = rmesh.device
     n = _lib.i32x3(mattrs.meshsize)
@@ -1017,7 +1036,8 @@ def _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, b
     tmp = torch.empty_like(rmesh)
     acc = torch.empty_like(rmesh)
     blocks = []
-    for ellin in ellsin:
+    for iin, ellin in enumerate(ellsin):
+        wa = was[iin] if was else 0
         rows = np.zeros((len(edgesin), len(ells), bin.nbins))
         xi = [_host_array_fn('mp_tophat_shell_xi_array', ellin, savg, float(e0), float(e1)) for e0, e1 in edgesin]
         for ill, ell in enumerate(ells):
@@ -1025,11 +1045,15 @@ def _window_local_smooth(rmesh, a0, a0_pitch, mattrs, xs, ss, edgesin, ellsin, b
             for m1 in range(-ellin, ellin + 1):
                 for m in range(-ell, ell + 1):
                     _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
+                    if wa:
+                        _mul_radial_power(base, base, mattrs, xs, -wa)
                     _mul_ylm(base, tmp, mattrs, xs, ellin, m1, False)
                     B = _r2c(tmp, mattrs.meshsize)
                     _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
                               _spectrum_pitch(B))
                     Q = _c2r(B, mattrs.meshsize)
+                    if wa:
+                        _mul_radial_power(Q, Q, mattrs, ss, wa)
                     del B
                     _mul_ylm(Q, tmp, mattrs, ss, ell, m, False)
                     _mul_ylm(tmp, acc, mattrs, ss, ellin, m1, not first)
@@ -1066,8 +1090,6 @@ def _theory_tables(mattrs, theory, device):
     if not isinstance(poles, dict):
         raise NotImplementedError('theory must be given per multipole (dict, list or poles container) on this path')
     poles = {ell if isinstance(ell, tuple) else (ell, 0): pole for ell, pole in poles.items()}
-    if any(wa != 0 for _, wa in poles):
-        raise NotImplementedError('wide-angle orders > 0 are not implemented on this path')
     if all(callable(p) for p in poles.values()):
         kx, ky, kz = (torch.from_numpy(k).to(device) for k in _kvectors(mattrs))
         k2 = (kx[:, None, None]**2 + ky[None, :, None]**2) + kz[None, None, :]**2
@@ -1124,6 +1146,8 @@ def compute_mesh2_spectrum_mean(*meshes, theory, bin=None, los='z'):
     dvals = {key: torch.from_numpy(np.ascontiguousarray(v, dtype='f4')).to(device) for key, v in values.items()}
     stream = _lib.stream_handle(device)
     sums = np.zeros((len(ells), bin.nbins))
+    if vlos is not None and any(wa for _, wa in values):
+        raise NotImplementedError('wide-angle orders > 0 need a local line of sight')
     if vlos is not None:
         if periodic:
             Q = torch.full(shape, n3, dtype=torch.float32, device=device)
@@ -1164,21 +1188,27 @@ def compute_mesh2_spectrum_mean(*meshes, theory, bin=None, los='z'):
               for m, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]
         ws_xi = _lib.workspace(_lib.load().mp_window_shell_ylm_workspace_bytes(n), device)
         ws_bin = _lib.workspace(_lib.load().mp_bin_ylm_real_workspace_bytes(n, nedges), device)
+        ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]
         xi = {}
-        for (ell1, _), v in dvals.items():
+        for (ell1, wa), v in dvals.items():
             for m1 in range(-ell1, ell1 + 1):
-                xi[ell1, m1] = torch.empty(shape, dtype=torch.float32, device=device)
+                x = torch.empty(shape, dtype=torch.float32, device=device)
                 _lib.call('mp_theory_ylm_real', stream, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), len(keys),
                           _lib.ptr(dkeys), ntab, _lib.ptr(v), int(ell1), int(m1), rnorm * 4. * np.pi / (2 * ell1 + 1),
-                          _lib.ptr(xi[ell1, m1]), _lib.ptr(ws_xi), ws_xi.numel())
+                          _lib.ptr(x), _lib.ptr(ws_xi), ws_xi.numel())
+                if wa:  # |s|^wa (mesh2.py:1643)
+                    _mul_radial_power(x, x, mattrs, ss, wa)
+                xi[ell1, wa, m1] = x
         base, tmp, acc = torch.empty_like(rmesh), torch.empty_like(rmesh), torch.empty_like(rmesh)
         out = torch.empty((len(ells), 2 * max(ells) + 1, bin.nbins, 2), dtype=torch.float64, device=device)
         for ill, ell in enumerate(ells):
             for m in range(-ell, ell + 1):
                 _mul_ylm(rmesh, base, mattrs, xs, ell, m, False)
                 first = True
-                for (ell1, m1), x in xi.items():
+                for (ell1, wa, m1), x in xi.items():
                     _mul_ylm(base, tmp, mattrs, xs, ell1, m1, False)
+                    if wa:  # |x|^-wa (mesh2.py:1644)
+                        _mul_radial_power(tmp, tmp, mattrs, xs, -wa)
                     B = _r2c(tmp, mattrs.meshsize)
                     _lib.call('mp_spectrum_product', stream, _lib.ptr(B), _spectrum_pitch(B), _lib.ptr(a0), a0_pitch, n, _lib.ptr(B),
                               _spectrum_pitch(B))
@@ -1252,7 +1282,7 @@ def _window_smooth(mattrs, bin, edgesin, ellsin, vlos, Q, rnorm, device):
     return np.real(np.concatenate(blocks, axis=0).T)
 
 
-def _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm, los=None):
+def _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm, los=None, was=None):
     """WindowMatrix container of mesh2.py:1463-1482."""
     ells = bin.ells
     nbins = bin.nbins
@@ -1262,8 +1292,10 @@ def _window_matrix(wmat, bin, edgesin, ellsin, vlos, norm, los=None):
                                                        num_shotnoise=zeros, norm=np.full(nbins, norm[ill]), ell=ell, attrs=attrs)
                                      for ill, ell in enumerate(ells)], attrs=attrs)
     kin = edgesin.mean(axis=-1)
+    was = was or [0] * len(ellsin)
     theory = Mesh2SpectrumPoles([Mesh2SpectrumPole(k=kin, k_edges=edgesin, nmodes=np.zeros(len(kin), dtype='i8'),
-                                                   num_raw=np.zeros(len(kin)), ell=ell) for ell in ellsin])
+                                                   num_raw=np.zeros(len(kin)), ell=ell, attrs=dict(wa_order=wa))
+                                 for ell, wa in zip(ellsin, was)])
     return WindowMatrix(observable=observable, theory=theory, value=wmat)
 
 

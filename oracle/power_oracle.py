@@ -1036,25 +1036,34 @@ def spectrum_window_smooth(attrs, bin, edgesin, ellsin=0, meshes=None, los='z', 
     return np.real(np.concatenate(blocks, axis=0).T)
 
 
-def window_local_selection(attrs, rmesh, ell, m, ellin):
+def radial_power(coords, power):
+    """|c|^power on the mesh of the three 1-D coordinate arrays (0^p as numpy gives it)."""
+    cx, cy, cz = coords
+    r = np.sqrt(cx[:, None, None]**2 + cy[None, :, None]**2 + cz[None, None, :]**2)
+    with np.errstate(divide='ignore'):
+        return r**power
+
+
+def window_local_selection(attrs, rmesh, ell, m, ellin, wa=0):
     """Q_lm(s) of the local (firstpoint) window with a configuration-space theory
-    (mesh2.py:1241-1247, wide-angle order 0): V_cell sum_m1 Y_lin,m1(shat) 4 pi / (2 lin + 1)
-    c2r(conj(r2c(rho Y_lm(xhat) Y_lin,m1(xhat))) A0), A0 = r2c(rho); positions from the
-    origin, Y = 0 at the zero vector."""
+    (mesh2.py:1241-1247): V_cell sum_m1 Y_lin,m1(shat) 4 pi / (2 lin + 1)
+    c2r(conj(r2c(rho |x|^-wa Y_lm(xhat) Y_lin,m1(xhat))) A0) |s|^wa, A0 = r2c(rho); positions
+    from the origin, Y = 0 at the zero vector, wa the wide-angle order."""
     xs = attrs.rcoords()
     xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
     sx, sy, sz = separation_coords(attrs)
     sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
     A0 = r2c(rmesh)
-    base = rmesh * real_ylm(ell, m, xx, xy, xz)
+    base = rmesh * real_ylm(ell, m, xx, xy, xz) * (radial_power(xs, -wa) if wa else 1.)
+    spow = radial_power(separation_coords(attrs), wa) if wa else 1.
     Qs = 0.
     for m1 in range(-ellin, ellin + 1):
-        Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(base * real_ylm(ellin, m1, xx, xy, xz))) * A0, attrs.meshsize)
+        Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(base * real_ylm(ellin, m1, xx, xy, xz))) * A0, attrs.meshsize) * spow
         Qs = Qs + real_ylm(ellin, m1, sx, sy, sz) * Q * np.prod(attrs.cellsize)
     return Qs
 
 
-def window_local_selection_bruteforce(attrs, rmesh, ell, m, ellin):
+def window_local_selection_bruteforce(attrs, rmesh, ell, m, ellin, wa=0):
     """The same Q_lm(s) as an explicit double sum over cells x and lags s,
     V_cell sum_x rho(x) rho(x + s) Y_lm(xhat) P_lin(xhat . shat) (addition theorem; Y and P
     taken as 0 at the zero vector), independent of the Y_lin,m1 conventions; tiny meshes."""
@@ -1065,6 +1074,9 @@ def window_local_selection_bruteforce(attrs, rmesh, ell, m, ellin):
     xn = np.linalg.norm(X, axis=-1)
     xhat = X / np.where(xn == 0, 1., xn)[..., None]
     ylm = real_ylm(ell, m, X[..., 0], X[..., 1], X[..., 2])
+    if wa:
+        with np.errstate(divide='ignore'):
+            ylm = ylm * xn**(-wa)
     out = np.zeros(n)
     for idx in np.ndindex(*n):
         svec = np.array([ss[a][idx[a]] for a in range(3)])
@@ -1075,7 +1087,7 @@ def window_local_selection_bruteforce(attrs, rmesh, ell, m, ellin):
             mu = xhat @ (svec / sn)
         leg = np.where((xn == 0) | (sn == 0.), 0., legendre(ellin, mu))  # the Y are 0 at the zero vector
         shifted = np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2))  # rho(x + s)
-        out[idx] = np.sum(rmesh * shifted * ylm * leg)
+        out[idx] = np.sum(rmesh * shifted * ylm * leg) * sn**wa
     return out * np.prod(attrs.cellsize)
 
 
@@ -1102,11 +1114,12 @@ def spectrum_window_local_infinite(attrs, bin, edgesin, ellsin=0, mesh=None, nor
     nb = len(bin.xavg)
     blocks = []
     for ellin in ellsin:
+        ellin, wa = ellin if isinstance(ellin, tuple) else (ellin, 0)
         W = np.zeros((len(edgesin), len(ells) * nb))
         tophats = [tophat_shell_xi(ellin, e0, e1, sn) for e0, e1 in edgesin]
         for ill, ell in enumerate(ells):
             for m in range(-ell, ell + 1):
-                Qs = window_local_selection(attrs, rmesh, ell, m, ellin)
+                Qs = window_local_selection(attrs, rmesh, ell, m, ellin, wa=wa)
                 ykm = real_ylm(ell, m, kx, ky, kz)
                 for ie, tophat in enumerate(tophats):
                     spec = 4. * np.pi * bin(ykm * r2c(tophat * Qs), remove_zero=ell == 0) * rnorm[ill]
@@ -1115,7 +1128,7 @@ def spectrum_window_local_infinite(attrs, bin, edgesin, ellsin=0, mesh=None, nor
     return np.concatenate(blocks, axis=0).T
 
 
-def window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1):
+def window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1, wa=0):
     """A_lm(s) = sum_m1 xi_m1(s) Q_lm,m1(s) of the local (firstpoint) window with the theory on the
     mesh (mesh2.py:1267-1281, wide-angle order 0): xi_m1 = c2r([e0 <= |k| <= e1] Y_lin,m1(khat))
     and Q_lm,m1 = 4 pi / (2 lin + 1) c2r(conj(r2c(rho Y_lm(xhat) Y_lin,m1(xhat))) A0)."""
@@ -1126,11 +1139,12 @@ def window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1):
     kn = knorm(attrs)
     shell = (kn >= e0) & (kn <= e1)
     A0 = r2c(rmesh)
-    base = rmesh * real_ylm(ell, m, xx, xy, xz)
+    base = rmesh * real_ylm(ell, m, xx, xy, xz) * (radial_power(xs, -wa) if wa else 1.)
+    spow = radial_power(separation_coords(attrs), wa) if wa else 1.
     out = 0.
     for m1 in range(-ellin, ellin + 1):
         xi = c2r(shell * real_ylm(ellin, m1, kx, ky, kz), attrs.meshsize)
-        Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(base * real_ylm(ellin, m1, xx, xy, xz))) * A0, attrs.meshsize)
+        Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(base * real_ylm(ellin, m1, xx, xy, xz))) * A0, attrs.meshsize) * spow
         out = out + xi * Q
     return out
 
@@ -1189,18 +1203,19 @@ def spectrum_window_local(attrs, bin, edgesin, ellsin=0, mesh=None, norm=None):
     nb = len(bin.xavg)
     blocks = []
     for ellin in ellsin:
+        ellin, wa = ellin if isinstance(ellin, tuple) else (ellin, 0)
         W = np.zeros((len(edgesin), len(ells) * nb))
         for ie, (e0, e1) in enumerate(edgesin):
             for ill, ell in enumerate(ells):
                 acc = 0.
                 for m in range(-ell, ell + 1):
-                    acc = acc + r2c(window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1)) * real_ylm(ell, m, kx, ky, kz)
+                    acc = acc + r2c(window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1, wa=wa)) * real_ylm(ell, m, kx, ky, kz)
                 W[ie, ill * nb:(ill + 1) * nb] = np.real(4. * np.pi * bin(acc, remove_zero=ell == 0) * rnorm[ill])
         blocks.append(W)
     return np.concatenate(blocks, axis=0).T
 
 
-def window_local_smooth_terms(attrs, rmesh, ell, ellin):
+def window_local_smooth_terms(attrs, rmesh, ell, ellin, wa=0):
     """sum_{m, m1} 4 pi / (2 lin + 1) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0)(s) Y_lm(shat) Y_lin,m1(shat)
     of the local smooth window (mesh2.py:1205-1210, wide-angle order 0), before the separation
     binning, the 4 pi and V_cell."""
@@ -1209,11 +1224,13 @@ def window_local_smooth_terms(attrs, rmesh, ell, ellin):
     sx, sy, sz = separation_coords(attrs)
     sx, sy, sz = sx[:, None, None], sy[None, :, None], sz[None, None, :]
     A0 = r2c(rmesh)
+    xpow = radial_power(xs, -wa) if wa else 1.
+    spow = radial_power(separation_coords(attrs), wa) if wa else 1.
     out = 0.
     for m1 in range(-ellin, ellin + 1):
         for m in range(-ell, ell + 1):
-            Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(rmesh * real_ylm(ell, m, xx, xy, xz) * real_ylm(ellin, m1, xx, xy, xz))) * A0,
-                                                  attrs.meshsize)
+            Q = 4. * np.pi / (2 * ellin + 1) * c2r(np.conj(r2c(rmesh * xpow * real_ylm(ell, m, xx, xy, xz) * real_ylm(ellin, m1, xx, xy, xz))) * A0,
+                                                  attrs.meshsize) * spow
             out = out + Q * real_ylm(ell, m, sx, sy, sz) * real_ylm(ellin, m1, sx, sy, sz)
     return out
 
@@ -1263,9 +1280,10 @@ def spectrum_window_local_smooth(attrs, bin, edgesin, ellsin=0, mesh=None, norm=
     nb = len(bin.xavg)
     blocks = []
     for ellin in ellsin:
+        ellin, wa = ellin if isinstance(ellin, tuple) else (ellin, 0)
         W = np.zeros((len(edgesin), len(ells) * nb))
         for ill, ell in enumerate(ells):
-            Qs = 4. * np.pi * sbin(window_local_smooth_terms(attrs, rmesh, ell, ellin)) * np.prod(attrs.cellsize)
+            Qs = 4. * np.pi * sbin(window_local_smooth_terms(attrs, rmesh, ell, ellin, wa=wa)) * np.prod(attrs.cellsize)
             if ell != 0:
                 Qs[0] = 0.
             Qs = np.where(empty, 0., Qs)
@@ -1304,12 +1322,12 @@ def spectrum_mean(attrs, bin, kin, poles, edges=True, mesh=None, los='z'):
     kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
     kn = knorm(attrs)
     rmesh = np.ones(tuple(int(v) for v in attrs.meshsize)) if mesh is None else (c2r(mesh, attrs.meshsize) if np.iscomplexobj(mesh) else mesh)
-    tables = {ell: theory_on_mesh(attrs, kin, values, edges) for ell, values in poles.items()}
+    tables = {(key if isinstance(key, tuple) else (key, 0)): theory_on_mesh(attrs, kin, values, edges) for key, values in poles.items()}
     num = []
     if not (isinstance(los, str) and los in ('firstpoint', 'local', 'endpoint')):
         vlos = los_vector(los)
         mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / np.where(kn == 0., 1., kn)
-        A = sum(tables[ell] * legendre(ell, mu) for ell in tables) * rnorm
+        A = sum(tables[key] * legendre(key[0], mu) for key in tables) * rnorm
         if mesh is None:
             A = A * n3
         else:
@@ -1327,10 +1345,12 @@ def spectrum_mean(attrs, bin, kin, poles, edges=True, mesh=None, los='z'):
         acc = 0.
         for m in range(-ell, ell + 1):
             Q = 0.
-            for ell1, table in tables.items():
+            for (ell1, wa), table in tables.items():
+                xpow = radial_power(xs, -wa) if wa else 1.
+                spow = radial_power(separation_coords(attrs), wa) if wa else 1.
                 for m1 in range(-ell1, ell1 + 1):
-                    xi = c2r(table * rnorm * real_ylm(ell1, m1, kx, ky, kz), attrs.meshsize)
-                    Q = Q + 4. * np.pi / (2 * ell1 + 1) * xi * c2r(np.conj(r2c(rmesh * real_ylm(ell, m, xx, xy, xz) * real_ylm(ell1, m1, xx, xy, xz))) * A0,
+                    xi = c2r(table * rnorm * real_ylm(ell1, m1, kx, ky, kz), attrs.meshsize) * spow
+                    Q = Q + 4. * np.pi / (2 * ell1 + 1) * xi * c2r(np.conj(r2c(rmesh * xpow * real_ylm(ell, m, xx, xy, xz) * real_ylm(ell1, m1, xx, xy, xz))) * A0,
                                                                  attrs.meshsize)
             acc = acc + r2c(Q) * real_ylm(ell, m, kx, ky, kz)
         num.append(np.real(4. * np.pi * bin(acc)))

@@ -117,7 +117,8 @@ def test_window_unsupported_raise():
     attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
     bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.05}, ells=(0, 2))
     with pytest.raises(NotImplementedError):
-        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=[(0, 1), (2, 0)])
+        compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='firstpoint', ellsin=([(0, 1), (2, 0)], 'local'),
+                                      flags=('infinite',))
     with pytest.raises(NotImplementedError):
         compute_mesh2_spectrum_window(attrs, edgesin=np.arange(0., 0.2, 0.05), bin=bin, los='z', ellsin=0, flags=('fftlog',))
 
@@ -437,3 +438,50 @@ def test_window_local_theory_mesh(periodic):
     wm = compute_mesh2_spectrum_window(*inputs, edgesin=edgesin, ellsin=((0, 2, 4), 'local'), bin=bin, los='firstpoint')
     want = po.spectrum_window_local_theory(oattrs, obin, edgesin, ellsin=(0, 2, 4), mesh=mesh)
     assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin), label='local theory mesh')
+
+
+@pytest.mark.parametrize('flag', ['infinite', 'smooth', None])
+def test_window_local_wide_angle(flag):
+    """Wide-angle orders of the firstpoint theory (|x|^-wa and |s|^wa, mesh2.py:1244, 1269,
+    1207-1209) for each theory flag, against the oracle; selection kept off the observer."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_window
+    say('window local wide angle', flag)
+    center = (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.03}, ells=(0, 2))
+    kf = 2. * np.pi / 400.
+    edgesin = np.array([[kf, 3 * kf], [3 * kf, 5 * kf]])
+    ellsin = [(0, 0), (0, 1), (2, 1)]
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.03}, ells=(0, 2))
+    pos, w = _survey_selection(attrs, 100000, 47, scale=0.3)
+    sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+    mesh = host(sel.value).astype('f8')
+    flags = (flag,) if flag else ()
+    wm = compute_mesh2_spectrum_window(sel, edgesin=edgesin, ellsin=ellsin, bin=bin, los='firstpoint', flags=flags)
+    fn = {'infinite': po.spectrum_window_local_infinite, 'smooth': po.spectrum_window_local_smooth, None: po.spectrum_window_local}[flag]
+    want = fn(oattrs, obin, edgesin, ellsin=ellsin, mesh=mesh)
+    assert_window_close(wm.value(), want, len(obin.nmodes), len(edgesin), label=f'wide angle {flag}')
+    assert [p.attrs.get('wa_order') for p in wm.theory] == [0, 1, 1]
+
+
+def test_spectrum_mean_wide_angle():
+    """compute_mesh2_spectrum_mean with wide-angle theory terms (mesh2.py:1640-1645) against the oracle."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_mean
+    say('spectrum mean wide angle')
+    center = (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2))
+    rng = np.random.default_rng(5)
+    kedges = np.arange(0., 0.25, 0.015)
+    poles = {(0, 0): rng.uniform(1e3, 2e3, len(kedges) - 1), (1, 1): rng.uniform(-5e2, 5e2, len(kedges) - 1)}
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.02}, ells=(0, 2))
+    pos, w = _survey_selection(attrs, 100000, 53, scale=0.3)
+    sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+    got = compute_mesh2_spectrum_mean(sel, theory=(kedges, poles), bin=bin, los='firstpoint')
+    want = po.spectrum_mean(oattrs, obin, kedges, poles, edges=True, mesh=host(sel.value).astype('f8'), los='firstpoint')
+    floor = 1e-5 * np.nanmax(np.abs(want[0]))
+    for ill, ell in enumerate((0, 2)):
+        g = got.get(ell).values('num_raw')
+        assert np.all(np.abs(g - want[ill]) <= 1e-5 * np.abs(want[ill]) + floor), (ell, np.max(np.abs(g - want[ill])), floor)

@@ -749,3 +749,15 @@ def test_real_gaunt_vs_reference_table():
         tot = sum(po.real_gaunt(l1, a, l2, b, l3, c)**2 for a in range(-l1, l1 + 1) for b in range(-l2, l2 + 1) for c in range(-l3, l3 + 1))
         want = (2 * l1 + 1) * (2 * l2 + 1) * (2 * l3 + 1) / (4. * np.pi) * three_j_000(l1, l2, l3)**2
         assert abs(tot - want) < 1e-12, (l1, l2, l3, tot, want)
+
+
+@pytest.mark.parametrize('ell,m,ellin,wa', [(2, 1, 0, 1), (0, 0, 2, 1), (2, -2, 2, 2)])
+def test_window_local_selection_wide_angle_vs_bruteforce(ell, m, ellin, wa):
+    """The wide-angle factors |x|^-wa and |s|^wa of the local window (mesh2.py:1244) against the
+    brute-force double sum (positions kept off the origin, as a survey's are)."""
+    rng = np.random.default_rng(6)
+    a = po.Attrs(6, 60., 70.)
+    rho = rng.uniform(0.5, 1.5, (6, 6, 6))
+    got = po.window_local_selection(a, rho, ell, m, ellin, wa=wa)
+    want = po.window_local_selection_bruteforce(a, rho, ell, m, ellin, wa=wa)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
