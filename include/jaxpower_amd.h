@@ -336,6 +336,13 @@ int mp_window_shell_ylm2(void* stream, const int32_t meshsize[3], const double* 
 int mp_mul_radial_power(void* stream, const float* in, float* out, const int32_t meshsize[3], const double* cx,
                         const double* cy, const double* cz, double power);
 
+/* out = c2r(P(k) Y_l1,m1(khat) Y_l2,m2(-khat) scale) with the tabulated theory of
+ * mp_spectrum_mean_bin: the xi of compute_mesh2_spectrum_mean with the 'local' theory line of
+ * sight (jaxpower/mesh2.py:1647-1662); workspace mp_window_shell_ylm_workspace_bytes(meshsize). */
+int mp_theory_ylm2_real(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz, int nkeys,
+                        const double* tkeys, int ntab, const float* tvals, int l1, int m1, int l2, int m2, float scale, float* out,
+                        void* workspace, size_t workspace_bytes);
+
 /* y += alpha x on real meshes (device arrays of n floats): the Gaunt-weighted sums of the window
  * with the 'local' theory line of sight (jaxpower/mesh2.py:1398-1402). */
 int mp_axpy(void* stream, float alpha, const float* x, float* y, int64_t n);

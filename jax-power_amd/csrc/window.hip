@@ -619,3 +619,31 @@ MP_EXPORT int mp_mul_radial_power(void* stream, const float* in, float* out, con
             meshsize[0], meshsize[1], meshsize[2], cx, cy, cz, power);
   return MP_OK;
 }
+
+// out = c2r(P(k) Y_l1,m1(khat) Y_l2,m2(-khat) scale) with the tabulated theory P of
+// mp_spectrum_mean_bin: the xi of the mean spectrum with the 'local' theory line of sight
+// (jaxpower/mesh2.py:1659); workspace as mp_window_shell_ylm.  l1, l2 <= 4.
+MP_EXPORT int mp_theory_ylm2_real(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                                  int nkeys, const double* tkeys, int ntab, const float* tvals, int l1, int m1, int l2, int m2,
+                                  float scale, float* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && tkeys && tvals && out, "mp_theory_ylm2_real: null pointer");
+  MP_REQUIRE(nkeys >= 1 && ntab >= 1 && ntab <= nkeys, "mp_theory_ylm2_real: %d values for %d keys", ntab, nkeys);
+  MP_REQUIRE(l1 >= 0 && l1 <= 4 && m1 >= -l1 && m1 <= l1 && l2 >= 0 && l2 <= 4 && m2 >= -l2 && m2 <= l2,
+             "mp_theory_ylm2_real: (l1, m1, l2, m2) = (%d, %d, %d, %d) not supported", l1, m1, l2, m2);
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_theory_ylm2_real: workspace too small");
+  ShellGen g = make_shell(kx, ky, kz, 0., 0., 0, nullptr, scale);
+  g.ylm_on = 1;
+  g.ylm_ell = l1;
+  g.ylm_m = m1;
+  g.ylm2_on = 1;
+  g.ylm2_ell = l2;
+  g.ylm2_m = m2;
+  g.tkeys = tkeys;
+  g.tvals = tvals;
+  g.tnkeys = nkeys;
+  g.tn = ntab;
+  return irfft3d_shell((hipStream_t)stream, g, out, meshsize, scratch, sp);
+}

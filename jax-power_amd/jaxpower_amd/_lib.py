@@ -108,6 +108,8 @@ SIGNATURES = {
     'mp_window_shell_ylm2': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int,
                                      c_float, c_void_p, c_void_p, c_size_t]),
     'mp_mul_radial_power': (c_int, [c_void_p, c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_double]),
+    'mp_theory_ylm2_real': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
+                                    c_int, c_int, c_float, c_void_p, c_void_p, c_size_t]),
     'mp_reduce_workspace_bytes': (c_int64, []),
     'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),

@@ -1109,6 +1109,71 @@ def _theory_tables(mattrs, theory, device):
     return theory_los, values, _k2_thresholds(kin), n
 
 
+def _mean_local_theory(rmesh1, rmesh2, mattrs, xs, bin, values, dkeys, ntab, rnorm, device):
+    """num_raw of compute_mesh2_spectrum_mean with the 'local' theory line of sight
+    (mesh2.py:1647-1662): kernels K_l1l2 = sum c P_l (the coefficients of mesh2.py:1648-1651),
+    xi = c2r(K Y_l1,m1(khat) Y_l2,m2(-khat)) (generated in the inverse x pass),
+    Q_lm = sum (4 pi)^2/((2l1+1)(2l2+1)) xi c2r(conj(r2c(rho1 Y_lm Y_l1,m1)) r2c(rho2 Y_l2,m2)),
+    num_l = 4 pi bin(sum_m r2c(Q_lm) Y_lm(khat))."""
+    lib = _lib.load()
+    ells = bin.ells
+    if any(wa for _, wa in values):
+        raise NotImplementedError("the 'local' theory line of sight takes wide-angle order 0")
+    n = _lib.i32x3(mattrs.meshsize)
+    stream = _lib.stream_handle(device)
+    t = bin._tables
+    nedges = len(bin._edges1d)
+    ncell = rmesh1.numel()
+    ws_xi = _lib.workspace(lib.mp_window_shell_ylm_workspace_bytes(n), device)
+    ws_bin = _lib.workspace(lib.mp_bin_ylm_real_workspace_bytes(n, nedges), device)
+    kern = {}
+    for key, lst in _LOCAL_THEORY_COEFF2.items():
+        v = sum(c * values[l, 0] for l, c in lst if (l, 0) in values)
+        kern[key] = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(v, (ntab,)), dtype='f4')).to(device)
+    pairs = [(l1, m1, l2, m2) for l1, l2 in ((0, 0), (0, 2), (2, 0), (2, 2))
+             for m1 in range(-l1, l1 + 1) for m2 in range(-l2, l2 + 1)]
+    xi = {}
+    for l1, m1, l2, m2 in pairs:
+        x = torch.empty_like(rmesh1)
+        scale = rnorm * (4. * np.pi)**2 / ((2 * l1 + 1) * (2 * l2 + 1))
+        lib_args = (stream, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), len(dkeys), _lib.ptr(dkeys), ntab,
+                    _lib.ptr(kern[l1, l2]), l1, m1, l2, m2, scale, _lib.ptr(x), _lib.ptr(ws_xi), ws_xi.numel())
+        _lib.call('mp_theory_ylm2_real', *lib_args)
+        xi[l1, m1, l2, m2] = x
+    G2 = {}
+    tmp, base, acc = torch.empty_like(rmesh1), torch.empty_like(rmesh1), torch.empty_like(rmesh1)
+    for l2 in (0, 2):
+        for m2 in range(-l2, l2 + 1):
+            _mul_ylm(rmesh2, tmp, mattrs, xs, l2, m2, False)
+            G2[l2, m2] = _r2c(tmp, mattrs.meshsize)
+    out = torch.empty((len(ells), 2 * max(ells) + 1, bin.nbins, 2), dtype=torch.float64, device=device)
+    for ill, ell in enumerate(ells):
+        for m in range(-ell, ell + 1):
+            _mul_ylm(rmesh1, base, mattrs, xs, ell, m, False)
+            first = True
+            F = {}
+            for l1, m1, l2, m2 in pairs:
+                if (l1, m1) not in F:
+                    _mul_ylm(base, tmp, mattrs, xs, l1, m1, False)
+                    F[l1, m1] = _r2c(tmp, mattrs.meshsize)
+                B = _empty_spectrum(mattrs.meshsize, device)
+                _lib.call('mp_spectrum_product', stream, _lib.ptr(F[l1, m1]), _spectrum_pitch(F[l1, m1]), _lib.ptr(G2[l2, m2]),
+                          _spectrum_pitch(G2[l2, m2]), n, _lib.ptr(B), _spectrum_pitch(B))
+                Q = _c2r(B, mattrs.meshsize)
+                del B
+                _lib.call('mp_mul_acc', stream, _lib.ptr(xi[l1, m1, l2, m2]), _lib.ptr(Q), _lib.ptr(acc), ncell, int(not first))
+                first = False
+            del F
+            _lib.call('mp_bin_ylm_real', stream, _lib.ptr(acc), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']),
+                      nedges, int(ell), int(m), 0, _lib.ptr(out[ill, m + ell]), _lib.ptr(ws_bin), ws_bin.numel())
+    host_out = out[..., 0].cpu().numpy()
+    sums = np.zeros((len(ells), bin.nbins))
+    with np.errstate(invalid='ignore', divide='ignore'):
+        for ill, ell in enumerate(ells):
+            sums[ill] = 4. * np.pi * host_out[ill, :2 * ell + 1].sum(axis=0) / bin.nmodes
+    return sums
+
+
 def compute_mesh2_spectrum_mean(*meshes, theory, bin=None, los='z'):
     """Mean power-spectrum multipoles of a theory seen through a selection (mesh2.py:1485-1680),
     for a theory given per multipole (dict/list of arrays on k or k edges, a poles container, or
@@ -1169,7 +1234,7 @@ def compute_mesh2_spectrum_mean(*meshes, theory, bin=None, los='z'):
                 for ie, ell in enumerate(even):  # odd ells: P_l(mu) + P_l(-mu) = 0 (mesh2.py:1599-1602)
                     sums[ells.index(ell)] = (2 * ell + 1) * tot[ie] / bin.nmodes
     else:
-        if theory_los != 'firstpoint':
+        if theory_los not in ('firstpoint', 'local'):
             raise NotImplementedError(f'theory line of sight {theory_los!r} is not implemented on this path')
         if max(ells) > 4 or max(ell for ell, _ in values) > 4:
             raise NotImplementedError('local line-of-sight multipoles up to 4 on this path')
@@ -1186,6 +1251,13 @@ def compute_mesh2_spectrum_mean(*meshes, theory, bin=None, los='z'):
         offset = mattrs.boxcenter - mattrs.boxsize / 2.
         xs = [torch.from_numpy(np.arange(int(m)) * float(d) + float(o)).to(device)
               for m, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]
+        if theory_los == 'local':
+            rmesh2 = rmesh if (periodic or fields[1] == fields[0]) else _2r(meshes[1]).value.contiguous().to(torch.float32)
+            sums = _mean_local_theory(rmesh, rmesh2, mattrs, xs, bin, values, dkeys, ntab, rnorm, device)
+            poles = [Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=sums[ill], num_shotnoise=np.zeros(bin.nbins),
+                                       norm=np.full(bin.nbins, norm), volume=np.prod(mattrs.kfun) * bin.nmodes, ell=ell, attrs=attrs)
+                     for ill, ell in enumerate(ells)]
+            return Mesh2SpectrumPoles(poles, attrs=attrs)
         ws_xi = _lib.workspace(_lib.load().mp_window_shell_ylm_workspace_bytes(n), device)
         ws_bin = _lib.workspace(_lib.load().mp_bin_ylm_real_workspace_bytes(n, nedges), device)
         ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]

@@ -1308,7 +1308,7 @@ def theory_on_mesh(attrs, kin, values, edges):
     return np.where(mask, np.asarray(values, dtype='f8')[np.where(mask, idx, 0)], 0.)
 
 
-def spectrum_mean(attrs, bin, kin, poles, edges=True, mesh=None, los='z'):
+def spectrum_mean(attrs, bin, kin, poles, edges=True, mesh=None, los='z', theory_los='firstpoint'):
     """compute_mesh2_spectrum_mean (mesh2.py:1485-1680) for a theory tabulated per multipole
     (poles: dict ell -> values on kin), wide-angle order 0: global line of sight
     num_l = (2l+1) bin(r2c(Q c2r(P rnorm)) P_l(mu)) with P = sum_lin P_lin(|k|) L_lin(mu),
@@ -1341,6 +1341,22 @@ def spectrum_mean(attrs, bin, kin, poles, edges=True, mesh=None, los='z'):
     xs = attrs.rcoords()
     xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
     A0 = r2c(rmesh)
+    if theory_los == 'local':  # mesh2.py:1647-1662 (one selection mesh: rho2 = rho1)
+        for ell in bin.ells:
+            acc = 0.
+            for m in range(-ell, ell + 1):
+                Q = 0.
+                for ell1, ell2 in product((0, 2), (0, 2)):
+                    kernel = sum(c * tables[l, 0] for l, c in LOCAL_THEORY_COEFF2[ell1, ell2]) * rnorm
+                    for m1 in range(-ell1, ell1 + 1):
+                        F = r2c(rmesh * real_ylm(ell, m, xx, xy, xz) * real_ylm(ell1, m1, xx, xy, xz))
+                        for m2 in range(-ell2, ell2 + 1):
+                            xi = c2r(kernel * real_ylm(ell1, m1, kx, ky, kz) * real_ylm(ell2, m2, -kx, -ky, -kz), attrs.meshsize)
+                            Q = Q + (4. * np.pi)**2 / ((2 * ell1 + 1) * (2 * ell2 + 1)) * xi * c2r(np.conj(F) * r2c(rmesh * real_ylm(ell2, m2, xx, xy, xz)),
+                                                                                                    attrs.meshsize)
+                acc = acc + r2c(Q) * real_ylm(ell, m, kx, ky, kz)
+            num.append(np.real(4. * np.pi * bin(acc)))
+        return num
     for ell in bin.ells:
         acc = 0.
         for m in range(-ell, ell + 1):

@@ -485,3 +485,27 @@ def test_spectrum_mean_wide_angle():
     for ill, ell in enumerate((0, 2)):
         g = got.get(ell).values('num_raw')
         assert np.all(np.abs(g - want[ill]) <= 1e-5 * np.abs(want[ill]) + floor), (ell, np.max(np.abs(g - want[ill])), floor)
+
+
+def test_spectrum_mean_local_theory():
+    """compute_mesh2_spectrum_mean with the 'local' theory line of sight (mesh2.py:1647-1662)
+    against the oracle (another Y_lm sign convention)."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum_mean
+    say('spectrum mean local theory')
+    center = (900., 300., 500.)
+    attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=center)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    rng = np.random.default_rng(6)
+    kedges = np.arange(0., 0.25, 0.015)
+    poles = {0: rng.uniform(1e3, 2e3, len(kedges) - 1), 2: rng.uniform(-5e2, 5e2, len(kedges) - 1), 4: rng.uniform(-1e2, 1e2, len(kedges) - 1)}
+    oattrs = po.Attrs(16, 400., center)
+    obin = po.BinPoles(oattrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    pos, w = _survey_selection(attrs, 100000, 59, scale=0.3)
+    sel = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='cic', interlacing=1, compensate=False)
+    got = compute_mesh2_spectrum_mean(sel, theory=((kedges, poles), 'local'), bin=bin, los='firstpoint')
+    want = po.spectrum_mean(oattrs, obin, kedges, poles, edges=True, mesh=host(sel.value).astype('f8'), los='firstpoint',
+                            theory_los='local')
+    floor = 1e-5 * np.nanmax(np.abs(want[0]))
+    for ill, ell in enumerate((0, 2, 4)):
+        g = got.get(ell).values('num_raw')
+        assert np.all(np.abs(g - want[ill]) <= 1e-5 * np.abs(want[ill]) + floor), (ell, np.max(np.abs(g - want[ill])), floor)
