@@ -353,6 +353,13 @@ int mp_axpy(void* stream, float alpha, const float* x, float* y, int64_t n);
 int mp_tophat_shell_xi_array(int ell, double k0, double k1, const double* s, int64_t n, double* out);
 int mp_spherical_jn_array(int ell, const double* x, int64_t n, double* out);
 
+/* Weights of compute_fkp3_shotnoise, Scoccimarro basis, global line of sight
+ * (jaxpower/mesh3.py:786-852): out = Y_ell,m(khat) P_leg(k.los/|k|) as a half spectrum (imaginary
+ * part 0; ell < 0 drops the Y factor, leg = 0 the Legendre one), binned against the
+ * paint(w) conj(paint(w^2)) product with mp_bin_poles. */
+int mp_ylm_legendre_spectrum(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                             int ell, int m, int leg, const double los[3], void* out, int64_t out_pitch);
+
 /* ---- reductions (normalisation and shot noise, jaxpower/mesh.py:3092-3129,
  *      jaxpower/mesh2.py:516-621) ------------------------------------------------------ */
 int64_t mp_reduce_workspace_bytes(void);

@@ -143,7 +143,8 @@ struct ShellGen {
     if (ylm_on) {
       v *= ylm_of_vector(ylm_ell, ylm_m, (float)kxv, (float)kyv, (float)kzv);
       if (ylm2_on) v *= ylm_of_vector(ylm2_ell, ylm2_m, -(float)kxv, -(float)kyv, -(float)kzv);
-    } else if (ell_on) {
+    }
+    if (ell_on) {
       const double mu = k2 > 0. ? (kxv * los[0] + kyv * los[1] + kzv * los[2]) / sqrt(k2) : 0.;
       const float m = (float)mu;
       v *= legendre_eval(leg, 0, m, m * m);

@@ -647,3 +647,29 @@ MP_EXPORT int mp_theory_ylm2_real(void* stream, const int32_t meshsize[3], const
   g.tn = ntab;
   return irfft3d_shell((hipStream_t)stream, g, out, meshsize, scratch, sp);
 }
+
+// Weights of the bispectrum shot noise with a global line of sight (jaxpower/mesh3.py:820-839):
+// out = Y_ell,m(khat) P_leg(k.los/|k|) on the half spectrum (imaginary part 0); ell < 0 drops
+// the Y factor, leg = 0 the Legendre one.  Y = 0 and mu = 0 at k = 0.
+MP_EXPORT int mp_ylm_legendre_spectrum(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
+                                       int ell, int m, int leg, const double los[3], void* out, int64_t out_pitch) {
+  MP_REQUIRE(meshsize && kx && ky && kz && out, "mp_ylm_legendre_spectrum: null pointer");
+  MP_REQUIRE(ell < 0 || (ell <= 4 && m >= -ell && m <= ell), "mp_ylm_legendre_spectrum: (ell, m) = (%d, %d) not supported", ell, m);
+  MP_REQUIRE(leg >= 0 && leg <= 5 && (leg == 0 || los), "mp_ylm_legendre_spectrum: Legendre order %d", leg);
+  const int nc = meshsize[2] / 2 + 1;
+  MP_REQUIRE(out_pitch >= nc, "mp_ylm_legendre_spectrum: pitch too small");
+  ShellGen g = make_shell(kx, ky, kz, -1., INFINITY, leg, los, 1.f);
+  if (ell >= 0) {
+    g.ylm_on = 1;
+    g.ylm_ell = ell;
+    g.ylm_m = m;
+  }
+  const int64_t total = (int64_t)meshsize[0] * meshsize[1] * nc;
+  int dev = 0, ncu = 0;
+  MP_CHECK_HIP(hipGetDevice(&dev));
+  MP_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int blocks = (int)std::min<int64_t>(ceil_div(total, WS_NT), (int64_t)ncu * 16);
+  MP_LAUNCH("ylm_weights", window_shell_kernel, dim3((unsigned)blocks), dim3(WS_NT), 0, (hipStream_t)stream, g, meshsize[0], meshsize[1],
+            nc, static_cast<float2*>(out), out_pitch);
+  return MP_OK;
+}

@@ -110,6 +110,7 @@ SIGNATURES = {
     'mp_mul_radial_power': (c_int, [c_void_p, c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_double]),
     'mp_theory_ylm2_real': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                     c_int, c_int, c_float, c_void_p, c_void_p, c_size_t]),
+    'mp_ylm_legendre_spectrum': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, P_F64, c_void_p, c_int64]),
     'mp_reduce_workspace_bytes': (c_int64, []),
     'mp_sum_f64': (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_fill_f32': (c_int, [c_void_p, c_void_p, c_int64, c_float]),

@@ -1559,3 +1559,38 @@ def spectrum_window_local_theory(attrs, bin, edgesin, ellsin=(0, 2, 4), mesh=Non
         for key, lst in LOCAL_THEORY_COEFF2.items():
             out[illin] += sum(c * (ell == ellin) for ell, c in lst) * tmp[key]
     return out.reshape(len(ellsin) * len(edgesin), -1).T
+
+
+def bispectrum_shotnoise(attrs, bin, positions, weights=None, resampler='cic', interlacing=0, compensate=True, los='x'):
+    """compute_fkp3_shotnoise, Scoccimarro basis, one field, global line of sight
+    (mesh3.py:999-1062, _compute_scoccimarro_S 786-852, eq. 58 of arXiv:1506.02729):
+    A = paint(w) conj(paint(w^2)) with both k = 0 modes removed; l = 0:
+    sum_axis bin_axis(A)[b_axis] - 2 sum w^3; l > 0: (2l+1) [bin_3(A L_l)[b3] +
+    4 pi sum_m sum_{axis < 3} bin_axis(A L_l Y_lm)[b_axis] bin_3(Y_lm)[b3]], where bin_axis is the
+    hermitian-weighted shell mean of side axis.  Returns one array per ell (real for even ell)."""
+    weights = np.ones(len(positions)) if weights is None else np.asarray(weights, dtype='f8')
+    cw = paint(attrs, positions, weights, resampler=resampler, interlacing=interlacing, compensate=compensate, out='complex')
+    cw2 = paint(attrs, positions, weights**2, resampler=resampler, interlacing=interlacing, compensate=compensate, out='complex')
+    cw[0, 0, 0] = 0.
+    cw2[0, 0, 0] = 0.
+    sumw3 = np.sum(weights**3)
+    A = cw * np.conj(cw2)
+    vlos = los_vector(los)
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    kn = np.sqrt(kx**2 + ky**2 + kz**2)
+    mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / np.where(kn == 0., 1., kn)
+    out = []
+    for ell in bin.ells:
+        if ell == 0:
+            sn = sum(bin.axes[a](A)[bin.iedges[:, a]] for a in range(3)) - 2. * sumw3
+        else:
+            Al = A * legendre(ell, mu)
+            sn = bin.axes[2](Al)[bin.iedges[:, 2]]
+            for m in range(-ell, ell + 1):
+                y = real_ylm(ell, m, kx, ky, kz)
+                b3 = bin.axes[2](y * np.ones_like(Al))[bin.iedges[:, 2]]
+                sn = sn + 4. * np.pi * sum(bin.axes[a](Al * y)[bin.iedges[:, a]] for a in range(2)) * b3
+            sn = (2 * ell + 1) * sn
+        out.append(np.real(sn) if ell % 2 == 0 else np.imag(sn))
+    return out

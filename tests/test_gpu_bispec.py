@@ -146,3 +146,27 @@ def test_bispectrum_write_read_and_box_normalization(tmp_path):
         assert np.array_equal(back.get(ell).values('num_raw'), spec.get(ell).values('num_raw'))
         assert np.array_equal(back.get(ell).coords('k'), spec.get(ell).coords('k'))
         assert np.allclose(back.get(ell).value(), spec.get(ell).values('num_raw') / want)
+
+
+@pytest.mark.parametrize('resampler,interlacing,los', [('cic', 0, 'x'), ('tsc', 2, [0.48, 0.6, 0.64])])
+def test_bispectrum_shotnoise(resampler, interlacing, los):
+    """compute_fkp3_shotnoise, Scoccimarro basis (mesh3.py:999-1062, 786-852), l = 0, 2, against
+    the oracle: |d| <= 1e-5 |sn| + 1e-5 max|sn_0| (the Y_lm sign conventions of the two sides
+    differ; only the sum over m enters)."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_fkp3_shotnoise
+    say('bispectrum shot noise', resampler, interlacing, los)
+    rng = np.random.default_rng(17)
+    attrs = MeshAttrs(meshsize=32, boxsize=800., boxcenter=0.)
+    pos = rng.uniform(-400., 400., (20000, 3))
+    w = rng.uniform(0.5, 1.5, 20000)
+    p = ParticleField(to_dev(pos), to_dev(w.astype('f4')), attrs=attrs)
+    bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2), basis='scoccimarro')
+    got = compute_fkp3_shotnoise(p, bin=bin, los=los, resampler=resampler, interlacing=interlacing, compensate=True)
+    oattrs = po.Attrs(32, 800., 0.)
+    obin = po.BinBispectrumPoles(oattrs, edges={'step': 0.02}, ells=(0, 2))
+    want = po.bispectrum_shotnoise(oattrs, obin, pos.astype('f4').astype('f8'), w.astype('f4').astype('f8'), resampler=resampler,
+                                   interlacing=interlacing, compensate=True, los=los)
+    floor = 1e-5 * np.abs(want[0]).max()
+    for ill in range(2):
+        err = np.abs(got[ill] - want[ill])
+        assert np.all(err <= 1e-5 * np.abs(want[ill]) + floor), (ill, err.max(), floor)

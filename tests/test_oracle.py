@@ -761,3 +761,18 @@ def test_window_local_selection_wide_angle_vs_bruteforce(ell, m, ellin, wa):
     got = po.window_local_selection(a, rho, ell, m, ellin, wa=wa)
     want = po.window_local_selection_bruteforce(a, rho, ell, m, ellin, wa=wa)
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
+
+
+def test_bispectrum_shotnoise_monopole_is_power_spectrum_sum():
+    """With unit weights paint(w^2) = paint(w), so the l = 0 bispectrum shot noise
+    (mesh3.py:830-832) is the sum over the three sides of the (k = 0 removed) power-spectrum
+    monopole at the side's shell, minus 2 N: checked against the pinned spectrum oracle."""
+    a = po.Attrs(16, 400., 0.)
+    bin = po.BinBispectrumPoles(a, edges={'step': 0.04}, ells=(0, 2))
+    pos = np.random.default_rng(1).uniform(-200., 200., (3000, 3))
+    sn = po.bispectrum_shotnoise(a, bin, pos, resampler='tsc', interlacing=2, compensate=True, los='x')
+    mesh = po.paint(a, pos, None, resampler='tsc', interlacing=2, compensate=True, out='complex')
+    mesh[0, 0, 0] = 0.
+    pk = po.spectrum_poles(a, po.BinPoles(a, edges={'step': 0.04}, ells=(0,)), mesh, los='x')['num_raw'][0]
+    want = sum(pk[bin.iedges[:, i]] for i in range(3)) - 2. * len(pos)
+    np.testing.assert_allclose(sn[0], want, rtol=1e-12)
