@@ -194,7 +194,8 @@ int mp_combine_partial_power(void* stream, const void* partial, int64_t pitch, c
                              void* workspace, size_t workspace_bytes);
 
 /* ---- local line of sight ---------------------------------------------------------- */
-/* One term of the firstpoint estimator (jaxpower/mesh2.py:319-329):
+/* One term of the firstpoint estimator (jaxpower/mesh2.py:319-329) and the third side of the
+ * local line-of-sight bispectrum (jaxpower/mesh3.py:563-576):
  * out = sum_m r2c(rmesh * Y_lm(xhat)) * Y_lm(khat), ell in 1..4, x measured from the origin. */
 int64_t mp_ylm_workspace_bytes(const int32_t meshsize[3]);
 int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t meshsize[3], const double boxsize[3],

@@ -1,4 +1,4 @@
-"""Bispectrum multipoles in the Scoccimarro basis with a global line of sight: the binning
+"""Bispectrum multipoles in the Scoccimarro basis (global or local line of sight): the binning
 operator BinMesh3SpectrumPoles (jaxpower/mesh3.py:23-177, 180-313), compute_mesh3_spectrum
 (mesh3.py:509-629), compute_mesh3 (mesh3.py:475-506) and compute_box3_normalization
 (mesh3.py:730-735).
@@ -7,7 +7,8 @@ Each |k| shell b of a half spectrum A becomes the real mesh I_b = irfftn(A [bin 
 the GPU (``mp_bispec_shells``), and the closed-triangle sums sum_x I_b1 I_b2 I_b3 over the
 allowed (b1, b2, b3) are taken in one pass over the cells (``mp_bispec_triangles``), FP32 per
 cell and FP64 across cells.  The mode counts are the same triangle sums of the shell
-indicators (mesh3.py:229-236).  The Sugiyama basis, the local line of sight and the
+indicators (mesh3.py:229-236).  With the local line of sight the third side is
+sum_m r2c(rho_3 Y_lm(xhat)) Y_lm(khat) (mesh3.py:563-576).  The Sugiyama basis and the
 three-point correlation function are not implemented on this path and raise
 NotImplementedError.
 """
@@ -211,11 +212,13 @@ def compute_mesh3_spectrum(*meshes, bin=None, los='x'):
     the imaginary part of a real sum); norm = N^3 / Vcell^2."""
     if not isinstance(bin, BinMesh3SpectrumPoles):
         raise TypeError('bin must be a BinMesh3SpectrumPoles')
-    if isinstance(los, str) and los in ('local', 'firstpoint', 'endpoint'):
-        raise NotImplementedError('the local line of sight is not implemented for the bispectrum on this path')
-    vlos = _get_los_vector(los)
+    if isinstance(los, str) and los in ('firstpoint', 'endpoint'):
+        raise ValueError(f"the bispectrum line of sight is 'local' or a global one, got {los!r} (mesh3.py:433-440)")
     meshes, _ = format_meshes(*meshes, nmeshes=3)
     mattrs = meshes[0].attrs
+    if isinstance(los, str) and los == 'local':
+        return _mesh3_spectrum_local(meshes, bin, mattrs)
+    vlos = _get_los_vector(los)
     cs = [_2c(m) for m in meshes]
     spectra = [_as_spectrum(c.value) for c in cs]
     norm = float(np.prod(mattrs.meshsize, dtype='f8')) / float(np.prod(mattrs.cellsize))**2
@@ -228,6 +231,32 @@ def compute_mesh3_spectrum(*meshes, bin=None, los='x'):
         else:
             sums = bin._sums_dev(spectra, ells=(0, 0, ell), los=vlos)
             num = (2 * ell + 1) * sums * n6 / bin._nmodes_dev
+        poles.append(Mesh3SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes[ill], num_raw=Deferred(num), norm=norm, ell=ell,
+                                       attrs=attrs, basis=bin.basis))
+    return Mesh3SpectrumPoles(poles, attrs=attrs)
+
+
+def _mesh3_spectrum_local(meshes, bin, mattrs):
+    """Local line of sight (mesh3.py:563-576): the third side becomes
+    C_ell = sum_m r2c(rho_3 Y_ell,m(xhat)) Y_ell,m(khat), xhat the direction of the node position
+    from the origin, and num_raw_ell = 4 pi sum over closed triangles of A1 A2 C_ell divided by the
+    triangle counts.  For ell = 0, 4 pi Y_00^2 = 1 and the third side is r2c(rho_3) itself."""
+    from .mesh2 import _ylm_accumulate
+    spectra = [_as_spectrum(_2c(m).value) for m in meshes[:2]]
+    rho3 = meshes[2] if isinstance(meshes[2], RealMeshField) else meshes[2].c2r()
+    norm = float(np.prod(mattrs.meshsize, dtype='f8')) / float(np.prod(mattrs.cellsize))**2
+    n6 = float(np.prod(mattrs.meshsize, dtype='f8'))**2
+    attrs = dict(los='local')
+    poles = []
+    for ill, ell in enumerate(bin.ells):
+        if ell == 0:
+            C, scale = _2c(rho3).value, 1.
+        else:
+            C = mattrs.create('complex', device=bin._device).value
+            _ylm_accumulate(rho3, ell, C)
+            scale = 4. * np.pi
+        sums = bin._sums_dev(spectra + [_as_spectrum(C)], ells=(0, 0, 0))
+        num = scale * sums * n6 / bin._nmodes_dev
         poles.append(Mesh3SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes[ill], num_raw=Deferred(num), norm=norm, ell=ell,
                                        attrs=attrs, basis=bin.basis))
     return Mesh3SpectrumPoles(poles, attrs=attrs)

@@ -768,20 +768,24 @@ def bispectrum_poles(attrs, bin, mesh1, mesh2=None, mesh3=None, los='x'):
         if meshes[i] is None:
             meshes[i] = meshes[i - 1]
     A = [m if np.iscomplexobj(m) else r2c(m) for m in meshes]
+    n6 = float(np.prod(attrs.meshsize))**2
+    norm = np.prod(attrs.meshsize) / np.prod(attrs.cellsize)**2
+    if isinstance(los, str) and los in ('local', 'firstpoint'):  # mesh3.py:563-576
+        rho3 = c2r(A[2], attrs.meshsize) if np.iscomplexobj(meshes[2]) else meshes[2]
+        num = [np.real(4. * np.pi * bin.reduce(A[0], A[1], bispectrum_local_mesh(attrs, rho3, ell)) * n6 / bin.nmodes) for ell in bin.ells]
+        return dict(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num, norm=norm, ells=bin.ells, los='local')
     vlos = los_vector(los)
     kx, ky, kz = attrs.kcoords('wavenumber')
     kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
     kn = np.sqrt(kx**2 + ky**2 + kz**2)
     kn[0, 0, 0] = 1.
     mu = (kx * vlos[0] + ky * vlos[1] + kz * vlos[2]) / kn
-    n6 = float(np.prod(attrs.meshsize))**2
     num = []
     for ell in bin.ells:
         if ell % 2:
             num.append(np.zeros(len(bin.iedges)))
             continue
         num.append((2 * ell + 1) * bin.reduce(A[0], A[1], A[2] * legendre(ell, mu)) * n6 / bin.nmodes)
-    norm = np.prod(attrs.meshsize) / np.prod(attrs.cellsize)**2
     return dict(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num, norm=norm, ells=bin.ells, los=vlos)
 
 
@@ -1593,4 +1597,48 @@ def bispectrum_shotnoise(attrs, bin, positions, weights=None, resampler='cic', i
                 sn = sn + 4. * np.pi * sum(bin.axes[a](Al * y)[bin.iedges[:, a]] for a in range(2)) * b3
             sn = (2 * ell + 1) * sn
         out.append(np.real(sn) if ell % 2 == 0 else np.imag(sn))
+    return out
+
+
+def bispectrum_local_mesh(attrs, rho, ell):
+    """Third mesh of the local-line-of-sight Scoccimarro bispectrum (mesh3.py:563-572):
+    sum_m r2c(rho Y_lm(xhat)) Y_lm(khat), positions from the origin.  For l >= 1, Y = 0 at
+    zero vectors (the reference's Y_lm are homogeneous in xhat, utils.py:252-260, 299-327); Y_00 is
+    the constant 1/sqrt(4 pi) everywhere, so the l = 0 mesh is r2c(rho) / (4 pi), k = 0 included."""
+    if ell == 0:
+        return r2c(rho) / (4. * np.pi)
+    xs = attrs.rcoords()
+    xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
+    return sum(r2c(rho * real_ylm(ell, m, xx, xy, xz)) * real_ylm(ell, m, kx, ky, kz) for m in range(-ell, ell + 1))
+
+
+def bispectrum_local_mesh_bruteforce(attrs, rho, ell):
+    """The same mesh as a direct DFT with the addition theorem:
+    sum_n rho(n) exp(-2 pi i k.n / N) (2l+1)/(4 pi) P_l(xhat_n . khat), for l >= 1 0 where x or k
+    is the zero vector (P_0 = 1 everywhere); tiny meshes only."""
+    n = [int(v) for v in attrs.meshsize]
+    xs = attrs.rcoords()
+    X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1).reshape(-1, 3)
+    xn = np.linalg.norm(X, axis=-1)
+    xhat = X / np.where(xn == 0, 1., xn)[:, None]
+    idx = np.stack(np.meshgrid(*[np.arange(v) for v in n], indexing='ij'), axis=-1).reshape(-1, 3)
+    kk = attrs.kcoords('wavenumber')
+    freq = [np.fft.fftfreq(n[0]) * n[0], np.fft.fftfreq(n[1]) * n[1], np.fft.rfftfreq(n[2]) * n[2]]
+    out = np.zeros((n[0], n[1], n[2] // 2 + 1), dtype=complex)
+    flat = rho.ravel()
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for l in range(n[2] // 2 + 1):
+                kv = np.array([kk[0][i], kk[1][j], kk[2][l]])
+                knv = np.linalg.norm(kv)
+                if ell == 0:
+                    leg = np.ones_like(xn)
+                elif knv == 0.:
+                    continue
+                else:
+                    leg = np.where(xn == 0, 0., legendre(ell, xhat @ (kv / knv)))
+                phase = np.exp(-2j * np.pi * (idx @ np.array([freq[0][i] / n[0], freq[1][j] / n[1], freq[2][l] / n[2]])))
+                out[i, j, l] = (2 * ell + 1) / (4. * np.pi) * np.sum(flat * phase * leg)
     return out

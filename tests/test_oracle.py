@@ -776,3 +776,15 @@ def test_bispectrum_shotnoise_monopole_is_power_spectrum_sum():
     pk = po.spectrum_poles(a, po.BinPoles(a, edges={'step': 0.04}, ells=(0,)), mesh, los='x')['num_raw'][0]
     want = sum(pk[bin.iedges[:, i]] for i in range(3)) - 2. * len(pos)
     np.testing.assert_allclose(sn[0], want, rtol=1e-12)
+
+
+@pytest.mark.parametrize('ell', [0, 1, 2, 3])
+def test_bispectrum_local_mesh_vs_direct_dft(ell):
+    """The third mesh of the local-line-of-sight bispectrum, sum_m r2c(rho Y_lm(xhat)) Y_lm(khat)
+    (mesh3.py:563-572), against a direct DFT weighted by (2l+1)/(4 pi) P_l(xhat.khat)."""
+    rng = np.random.default_rng(4)
+    a = po.Attrs(6, 60., 20.)
+    rho = rng.uniform(0.5, 1.5, (6, 6, 6))
+    got = po.bispectrum_local_mesh(a, rho, ell)
+    want = po.bispectrum_local_mesh_bruteforce(a, rho, ell)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-10 * np.abs(want).max())
