@@ -18,7 +18,7 @@ import torch
 from . import _lib
 from .mesh import (MeshAttrs, RealMeshField, ComplexMeshField, ParticleField, FKPField, format_meshes, compute_box_normalization,
                    _as_spectrum, _default_device, _empty_spectrum, _spectrum_pitch, _sum_f64)
-from .mesh2 import BinMesh2SpectrumPoles, _get_los_vector
+from .mesh2 import BinMesh2SpectrumPoles, _get_los_vector, _ylm_accumulate
 from .types import Deferred, Mesh3SpectrumPole, Mesh3SpectrumPoles
 
 _BASES = ('sugiyama', 'sugiyama-diagonal', 'scoccimarro', 'scoccimarro-diagonal')
@@ -241,7 +241,6 @@ def _mesh3_spectrum_local(meshes, bin, mattrs):
     C_ell = sum_m r2c(rho_3 Y_ell,m(xhat)) Y_ell,m(khat), xhat the direction of the node position
     from the origin, and num_raw_ell = 4 pi sum over closed triangles of A1 A2 C_ell divided by the
     triangle counts.  For ell = 0, 4 pi Y_00^2 = 1 and the third side is r2c(rho_3) itself."""
-    from .mesh2 import _ylm_accumulate
     spectra = [_as_spectrum(_2c(m).value) for m in meshes[:2]]
     rho3 = meshes[2] if isinstance(meshes[2], RealMeshField) else meshes[2].c2r()
     norm = float(np.prod(mattrs.meshsize, dtype='f8')) / float(np.prod(mattrs.cellsize))**2
@@ -280,28 +279,32 @@ def compute_box3_normalization(*inputs, bin=None):
 
 
 def compute_fkp3_shotnoise(*fkps, bin=None, los='z', resampler='cic', interlacing=False, compensate=True, fields=None, **kwargs):
-    """Bispectrum shot noise, Scoccimarro basis, one field, global line of sight
+    """Bispectrum shot noise, Scoccimarro basis, one field, global or local line of sight
     (mesh3.py:999-1062, _compute_scoccimarro_S 786-852; eq. 58 of arXiv:1506.02729): with
     A = paint(w) conj(paint(w^2)) (both k = 0 modes removed) and bin_a the shell means of side a,
     ell = 0: sum_a bin_a(A)[b_a] - 2 sum w^3; ell > 0: (2 ell + 1) [bin_3(A P_ell)[b3]
     + 4 pi sum_m sum_{a < 3} bin_a(A P_ell Y_ell,m)[b_a] bin_3(Y_ell,m)[b3]].  Returns one host
-    array per ell (real part for even ell, imaginary part for odd ell)."""
+    array per ell (real part for even ell, imaginary part for odd ell).  los='local' replaces A P_ell by
+    H_ell(paint(w)) conj(paint(w^2)) in the first term and paint(w) conj(H_ell(paint(w^2))) in the
+    second, H_ell(c) = 4 pi / (2 ell + 1) sum_m r2c(c2r(c) Y_lm(xhat)) Y_lm(khat) (mesh3.py:823-834)."""
     if not isinstance(bin, BinMesh3SpectrumPoles):
         raise TypeError('bin must be a BinMesh3SpectrumPoles')
-    if isinstance(los, str) and los in ('local', 'firstpoint', 'endpoint'):
-        raise NotImplementedError('the local line of sight is not implemented for the bispectrum shot noise on this path')
+    if isinstance(los, str) and los in ('firstpoint', 'endpoint'):
+        raise ValueError(f"the bispectrum line of sight is 'local' or a global one, got {los!r} (mesh3.py:433-440)")
+    local = isinstance(los, str) and los == 'local'
     fkps, fields = format_meshes(*fkps, fields=fields, nmeshes=3)
     if fields[2] == fields[1] + 1 == fields[0] + 2:
         return tuple(np.zeros(bin.nbins) for _ in bin.ells)
     if not (fields[0] == fields[1] == fields[2]):
         raise NotImplementedError('the Scoccimarro shot noise takes one field (mesh3.py:1050-1051)')
     particles = fkps[0].particles if isinstance(fkps[0], FKPField) else fkps[0]
-    vlos = _get_los_vector(los)
+    vlos = _get_los_vector('z' if local else los)
     mattrs = particles.attrs
     kw = dict(kwargs, resampler=resampler, interlacing=interlacing, compensate=compensate, out='complex')
-    cw = particles.paint(**kw).at[0, 0, 0].set(0.).value
+    fw = particles.paint(**kw).at[0, 0, 0].set(0.)
     w = particles.weights
-    cw2 = particles.clone(weights=w * w).paint(**kw).at[0, 0, 0].set(0.).value
+    fw2 = particles.clone(weights=w * w).paint(**kw).at[0, 0, 0].set(0.)
+    cw, cw2 = fw.value, fw2.value
     sumw3 = float(_sum_f64(w * w, w).cpu().numpy()) if w.dtype == torch.float32 else float(torch.sum(w.double()**3).cpu())
     a, pa = _as_spectrum(cw)
     b, pb = _as_spectrum(cw2)
@@ -319,18 +322,36 @@ def compute_fkp3_shotnoise(*fkps, bin=None, los='z', resampler='cic', interlacin
             if ell == 0:
                 sn = sum(shells(ax, [A], [(-1, 0, 0, False)])[0][ie[:, ax]] for ax in range(3)) - 2. * sumw3
             else:
-                sn = shells(2, [A], [(-1, 0, ell, False)], vlos)[0][ie[:, 2]]
+                Al = A
+                if local:
+                    # mesh3.py:823-834: P_ell(mu) c -> (4 pi / (2 ell + 1)) sum_m r2c(c2r(c) Y_lm(xhat)) Y_lm(khat);
+                    # the factor is applied to sn at the end, both terms being linear in it
+                    H = mattrs.create('complex', device=device).value
+                    _ylm_accumulate(fw.c2r(), ell, H)
+                    h, ph = _as_spectrum(H)
+                    P = _empty_spectrum(mattrs.meshsize, device)
+                    _lib.call('mp_spectrum_product', _lib.stream_handle(device), _lib.ptr(b), pb, _lib.ptr(h), ph, n, _lib.ptr(P),
+                              _spectrum_pitch(P))
+                    sn = shells(2, [P], [(-1, 0, 0, False)])[0][ie[:, 2]]
+                    _ylm_accumulate(fw2.c2r(), ell, H)  # overwrites H
+                    Al = _empty_spectrum(mattrs.meshsize, device)
+                    _lib.call('mp_spectrum_product', _lib.stream_handle(device), _lib.ptr(h), ph, _lib.ptr(a), pa, n, _lib.ptr(Al),
+                              _spectrum_pitch(Al))
+                    del H, P
+                else:
+                    sn = shells(2, [A], [(-1, 0, ell, False)], vlos)[0][ie[:, 2]]
                 W = _empty_spectrum(mattrs.meshsize, device)
                 V = _empty_spectrum(mattrs.meshsize, device)
                 t = axes[0]._tables
                 for m in range(-ell, ell + 1):
                     _lib.call('mp_ylm_legendre_spectrum', _lib.stream_handle(device), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
-                              _lib.ptr(t['kz']), int(ell), int(m), int(ell), _lib.f64x3(vlos), _lib.ptr(W), _spectrum_pitch(W))
+                              _lib.ptr(t['kz']), int(ell), int(m), 0 if local else int(ell), _lib.f64x3(vlos), _lib.ptr(W),
+                              _spectrum_pitch(W))
                     _lib.call('mp_ylm_legendre_spectrum', _lib.stream_handle(device), n, _lib.ptr(t['kx']), _lib.ptr(t['ky']),
                               _lib.ptr(t['kz']), int(ell), int(m), 0, _lib.f64x3(vlos), _lib.ptr(V), _spectrum_pitch(V))
                     b3 = shells(2, [V], [(-1, 0, 0, False)])[0][ie[:, 2]]
                     # conj(W) A with W real: A P_ell Y_lm binned on the first two sides
-                    sn = sn + 4. * np.pi * sum(shells(ax, [W, A], [(0, 1, 0, False)])[0][ie[:, ax]] for ax in range(2)) * b3
-                sn = (2 * ell + 1) * sn
+                    sn = sn + 4. * np.pi * sum(shells(ax, [W, Al], [(0, 1, 0, False)])[0][ie[:, ax]] for ax in range(2)) * b3
+                sn = (4. * np.pi if local else 2 * ell + 1) * sn
             out.append(np.real(sn) if ell % 2 == 0 else np.imag(sn))
     return tuple(out)

@@ -1571,7 +1571,10 @@ def bispectrum_shotnoise(attrs, bin, positions, weights=None, resampler='cic', i
     A = paint(w) conj(paint(w^2)) with both k = 0 modes removed; l = 0:
     sum_axis bin_axis(A)[b_axis] - 2 sum w^3; l > 0: (2l+1) [bin_3(A L_l)[b3] +
     4 pi sum_m sum_{axis < 3} bin_axis(A L_l Y_lm)[b_axis] bin_3(Y_lm)[b3]], where bin_axis is the
-    hermitian-weighted shell mean of side axis.  Returns one array per ell (real for even ell)."""
+    hermitian-weighted shell mean of side axis.  Returns one array per ell (real for even ell).
+    los='local' replaces P_l(mu) c by H_l(c) (mesh3.py:823-834), in the first line of eq. 58 on
+    paint(w) and in the second on paint(w^2); H_l is built from bispectrum_local_mesh, pinned by
+    a direct DFT, and the rest is the global-line-of-sight code."""
     weights = np.ones(len(positions)) if weights is None else np.asarray(weights, dtype='f8')
     cw = paint(attrs, positions, weights, resampler=resampler, interlacing=interlacing, compensate=compensate, out='complex')
     cw2 = paint(attrs, positions, weights**2, resampler=resampler, interlacing=interlacing, compensate=compensate, out='complex')
@@ -1579,7 +1582,8 @@ def bispectrum_shotnoise(attrs, bin, positions, weights=None, resampler='cic', i
     cw2[0, 0, 0] = 0.
     sumw3 = np.sum(weights**3)
     A = cw * np.conj(cw2)
-    vlos = los_vector(los)
+    local = isinstance(los, str) and los == 'local'
+    vlos = los_vector('z' if local else los)
     kx, ky, kz = attrs.kcoords('wavenumber')
     kx, ky, kz = kx[:, None, None], ky[None, :, None], kz[None, None, :]
     kn = np.sqrt(kx**2 + ky**2 + kz**2)
@@ -1589,8 +1593,13 @@ def bispectrum_shotnoise(attrs, bin, positions, weights=None, resampler='cic', i
         if ell == 0:
             sn = sum(bin.axes[a](A)[bin.iedges[:, a]] for a in range(3)) - 2. * sumw3
         else:
-            Al = A * legendre(ell, mu)
-            sn = bin.axes[2](Al)[bin.iedges[:, 2]]
+            if local:  # mesh3.py:823-834: P_l(mu) c -> H_l(c) = 4 pi / (2l+1) sum_m r2c(c2r(c) Y_lm(xhat)) Y_lm(khat)
+                h = 4. * np.pi / (2 * ell + 1)
+                sn = bin.axes[2](h * bispectrum_local_mesh(attrs, c2r(cw, attrs.meshsize), ell) * np.conj(cw2))[bin.iedges[:, 2]]
+                Al = cw * np.conj(h * bispectrum_local_mesh(attrs, c2r(cw2, attrs.meshsize), ell))
+            else:
+                Al = A * legendre(ell, mu)
+                sn = bin.axes[2](Al)[bin.iedges[:, 2]]
             for m in range(-ell, ell + 1):
                 y = real_ylm(ell, m, kx, ky, kz)
                 b3 = bin.axes[2](y * np.ones_like(Al))[bin.iedges[:, 2]]

@@ -148,10 +148,10 @@ def test_bispectrum_write_read_and_box_normalization(tmp_path):
         assert np.allclose(back.get(ell).value(), spec.get(ell).values('num_raw') / want)
 
 
-@pytest.mark.parametrize('resampler,interlacing,los', [('cic', 0, 'x'), ('tsc', 2, [0.48, 0.6, 0.64])])
+@pytest.mark.parametrize('resampler,interlacing,los', [('cic', 0, 'x'), ('tsc', 2, [0.48, 0.6, 0.64]), ('tsc', 0, 'local')])
 def test_bispectrum_shotnoise(resampler, interlacing, los):
-    """compute_fkp3_shotnoise, Scoccimarro basis (mesh3.py:999-1062, 786-852), l = 0, 2, against
-    the oracle: |d| <= 1e-5 |sn| + 1e-5 max|sn_0| (the Y_lm sign conventions of the two sides
+    """compute_fkp3_shotnoise, Scoccimarro basis (mesh3.py:999-1062, 786-852), l = 0, 2, global
+    and local line of sight, against the oracle: |d| <= 1e-5 |sn| + 1e-5 max|sn_0| (the Y_lm sign conventions of the two sides
     differ; only the sum over m enters)."""
     from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_fkp3_shotnoise
     say('bispectrum shot noise', resampler, interlacing, los)

@@ -788,3 +788,24 @@ def test_bispectrum_local_mesh_vs_direct_dft(ell):
     got = po.bispectrum_local_mesh(a, rho, ell)
     want = po.bispectrum_local_mesh_bruteforce(a, rho, ell)
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-10 * np.abs(want).max())
+
+
+def test_bispectrum_shotnoise_local_monopole_and_ell2():
+    """Local line of sight of the bispectrum shot noise (mesh3.py:823-834): l = 0 does not
+    depend on the line of sight; for l = 2 the first line of eq. 58 with H_2 built by
+    bispectrum_local_mesh equals the same line with P_2(xhat.khat) applied by a direct DFT."""
+    rng = np.random.default_rng(8)
+    a = po.Attrs(8, 80., 40.)
+    pos = rng.uniform(0., 80., (300, 3))
+    w = rng.uniform(0.5, 1.5, 300)
+    b = po.BinBispectrumPoles(a, edges={'step': 0.2}, ells=(0, 2))
+    loc = po.bispectrum_shotnoise(a, b, pos, w, los='local')
+    glo = po.bispectrum_shotnoise(a, b, pos, w, los='z')
+    np.testing.assert_allclose(loc[0], glo[0], rtol=1e-12)
+    cw = po.paint(a, pos, w, resampler='cic', compensate=True, out='complex')
+    cw[0, 0, 0] = 0.
+    rho = po.c2r(cw, a.meshsize)
+    h = 4. * np.pi / 5. * po.bispectrum_local_mesh(a, rho, 2)
+    want = 4. * np.pi / 5. * po.bispectrum_local_mesh_bruteforce(a, rho, 2)
+    np.testing.assert_allclose(h, want, rtol=1e-9, atol=1e-10 * np.abs(want).max())
+    assert np.all(np.isfinite(loc[1])) and np.abs(loc[1] - glo[1]).max() > 0.
