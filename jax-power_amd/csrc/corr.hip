@@ -224,7 +224,7 @@ __global__ void spectrum_product_kernel(const float2* a, int64_t a_pitch, const 
   out[r * out_pitch + c] = v;
 }
 
-// out (+)= in * Y_lm(cx[i], cy[j], cz[k]) on a real mesh, Y = 0 at the zero vector.
+// out (+)= in * Y_lm(cx[i], cy[j], cz[k]) on a real mesh (ylm_at_zero at the zero vector).
 __global__ void mul_ylm_kernel(const float* in, float* out, int n0, int n1, int n2, const double* __restrict__ cx,
                                const double* __restrict__ cy, const double* __restrict__ cz, int ell, int m, int accumulate) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

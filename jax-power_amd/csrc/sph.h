@@ -2,7 +2,8 @@
 //
 // Real Y_lm follow the normalisation of jaxpower/utils.py:227-238:
 //   sqrt((2l+1)/4pi) sqrt((l-|m|)!/(l+|m|)!) sqrt(2) P_l^|m|(cos theta) {cos m phi | sin |m| phi},
-// written as homogeneous polynomials of the unit vector so that Y_lm(0) = 0 (utils.py:252-254).
+// written as polynomials of the unit vector; the value at the zero vector is the reference's
+// (ylm_at_zero below).
 // Only sum_m Y_lm(a) Y_lm(b) = (2l+1)/(4pi) P_l(a.b) enters the estimator, so the sign of
 // each m term is a convention that cancels.
 #pragma once
@@ -123,10 +124,20 @@ __host__ __device__ __forceinline__ float ylm_unit(int ell, int m, float x, floa
   }
 }
 
-// Y_lm of the direction of (x, y, z); zero at the origin.
+// Y_lm at the zero vector as jax-power evaluates it: utils.py:252-254 maps the zero vector to
+// xhat = 0 and evaluates the sympy polynomial of utils.py:299-327 there, i.e. its constant
+// term, which is Y_00 itself and, for l = 4, m = 0, 3/8 sqrt(9/(4pi)) (the 3 r^4 term of
+// 35 z^4 - 30 z^2 r^2 + 3 r^4 is left as the constant 3); every other l <= 4 gives 0.
+__host__ __device__ __forceinline__ float ylm_at_zero(int ell, int m) {
+  if (ell == 0) return 0.28209479177387814f;
+  if (ell == 4 && m == 0) return 0.31735664074561293f;
+  return 0.f;
+}
+
+// Y_lm of the direction of (x, y, z); ylm_at_zero at the origin.
 __host__ __device__ __forceinline__ float ylm_of_vector(int ell, int m, float x, float y, float z) {
   float r2 = x * x + y * y + z * z;
-  if (r2 == 0.f) return 0.f;
+  if (r2 == 0.f) return ylm_at_zero(ell, m);
   float inv = 1.f / sqrtf(r2);
   return ylm_unit(ell, m, x * inv, y * inv, z * inv);
 }

@@ -23,8 +23,14 @@ from functools import lru_cache
 from itertools import product
 import math
 
+import os
+
 import numpy as np
+from scipy import fft as scipy_fft
 from scipy import special
+
+# threads of the oracle's FFTs (the host cores this process may use)
+FFT_WORKERS = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else (os.cpu_count() or 1)
 
 
 # ----------------------------------------------------------------------------
@@ -159,19 +165,23 @@ def _stencil(order):
 
 def paint_grid(meshsize, xg, weights, order):
     """Scatter-add particles given in grid units, resamplers.py:73-103.
-    One np.bincount per stencil offset (same sum as the reference's scatter)."""
+    One np.bincount over every (particle, stencil offset) pair (the same sum as the
+    reference's scatter, in another order)."""
     shape = tuple(int(n) for n in meshsize)
-    mesh = np.zeros(int(np.prod(shape)), dtype='f8')
+    size = int(np.prod(shape))
     xg = np.asarray(xg, dtype='f8')
     w = np.broadcast_to(np.asarray(weights, dtype='f8'), xg.shape[:1])
     id0 = (np.round if order % 2 else np.floor)(xg).astype('i8')
+    flats, vals = [], []
     for ishift in _stencil(order):
         idx = id0 + ishift
         ker = resampler_kernel(order, np.abs(idx - xg)).prod(axis=-1)
         idx = idx % np.array(shape)
-        flat = np.ravel_multi_index(tuple(idx.T), shape)
-        mesh += np.bincount(flat, weights=w * ker, minlength=mesh.size)
-    return mesh.reshape(shape)
+        flats.append(np.ravel_multi_index(tuple(idx.T), shape))
+        vals.append(w * ker)
+    if not flats or not len(xg):
+        return np.zeros(shape)
+    return np.bincount(np.concatenate(flats), weights=np.concatenate(vals), minlength=size).reshape(shape)
 
 
 def paint_grid_bruteforce(meshsize, xg, weights, order):
@@ -216,13 +226,14 @@ def to_grid(attrs, positions):
 
 
 def r2c(value):
-    """MeshAttrs.r2c with fft_backend='jax': rfftn (mesh.py:1679-1697)."""
-    return np.fft.rfftn(value)
+    """MeshAttrs.r2c with fft_backend='jax': rfftn (mesh.py:1679-1697).  scipy's pocketfft
+    (numpy's own FFT algorithm) on FFT_WORKERS threads."""
+    return scipy_fft.rfftn(value, workers=FFT_WORKERS)
 
 
 def c2r(value, meshsize):
     """MeshAttrs.c2r: irfftn(s=meshsize) (mesh.py:1699-1719)."""
-    return np.fft.irfftn(value, s=tuple(int(n) for n in meshsize), axes=(0, 1, 2))
+    return scipy_fft.irfftn(value, s=tuple(int(n) for n in meshsize), axes=(0, 1, 2), workers=FFT_WORKERS)
 
 
 def paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compensate=False, out='real'):
@@ -361,12 +372,48 @@ def legendre(ell, x):
     return _LEGENDRE[ell](np.asarray(x, dtype='f8'))
 
 
+def ylm_at_zero(ell, m):
+    """Value of the reference's real Y_lm at the zero vector.  utils.py:252-254 maps the zero
+    vector to xhat = (0, 0, 0) and evaluates the sympy polynomial built at utils.py:299-327, in
+    which the r^2 substitution leaves a constant term for (l, m) = (0, 0) (Y_00 itself) and
+    (4, 0) (3/8 sqrt(9/4pi), since 35 z^4 - 30 z^2 r^2 + 3 r^4 stays 35 zhat^4 - 30 zhat^2 + 3);
+    every other l <= 4 polynomial is homogeneous and gives 0.  Pinned in tests/test_oracle.py by
+    redoing the sympy construction."""
+    if ell == 0:
+        return 1. / np.sqrt(4. * np.pi)
+    if ell == 4 and m == 0:
+        return 3. / 8. * np.sqrt(9. / (4. * np.pi))
+    return 0.
+
+
+def addition_at(ell, u, v):
+    """4 pi / (2l+1) sum_m Y_lm(u) Y_lm(v) for arrays of vectors u (..., 3) and v (..., 3), with
+    the reference's values at zero vectors (ylm_at_zero): P_l(uhat . vhat) when both are nonzero
+    (addition theorem); with u = 0, 4 pi/(2l+1) Y_l0(0) Y_l0(vhat), i.e. 1 for l = 0,
+    P_4(0) P_4(vhat_z) for l = 4 and 0 otherwise; both zero: 1, P_4(0)^2 or 0.  Independent of
+    the per-m sign convention of the Y_lm."""
+    u, v = np.broadcast_arrays(np.asarray(u, dtype='f8'), np.asarray(v, dtype='f8'))
+    un, vn = np.linalg.norm(u, axis=-1), np.linalg.norm(v, axis=-1)
+    uh = u / np.where(un == 0, 1., un)[..., None]
+    vh = v / np.where(vn == 0, 1., vn)[..., None]
+    both = legendre(ell, np.sum(uh * vh, axis=-1))
+    if ell == 0:
+        return np.ones_like(both)
+    if ell != 4:
+        return np.where((un == 0) | (vn == 0), 0., both)
+    p40 = legendre(4, 0.)
+    out = np.where(un == 0, p40 * legendre(4, vh[..., 2]), both)
+    out = np.where(vn == 0, p40 * legendre(4, uh[..., 2]), out)
+    return np.where((un == 0) & (vn == 0), p40 * p40, out)
+
+
 def real_ylm(ell, m, x, y, z):
     """Real spherical harmonic of (x, y, z)/|.| with the reference normalisation
-    sqrt((2l+1)/4pi) sqrt((l-|m|)!/(l+|m|)!) sqrt(2) (utils.py:227-238), zero at the origin
-    (utils.py:252-254).  Built from scipy's associated Legendre function; the per-m sign
-    convention may differ from the reference's sympy expression, which cancels in
-    sum_m Y_lm(a) Y_lm(b) (the only combination the estimator uses, mesh2.py:319-329)."""
+    sqrt((2l+1)/4pi) sqrt((l-|m|)!/(l+|m|)!) sqrt(2) (utils.py:227-238), and the reference's
+    value at the zero vector (ylm_at_zero; utils.py:252-254).  Built from scipy's associated
+    Legendre function; the per-m sign convention may differ from the reference's sympy
+    expression, which cancels in sum_m Y_lm(a) Y_lm(b) (the only combination the estimator
+    uses, mesh2.py:319-329)."""
     x, y, z = np.broadcast_arrays(*(np.asarray(v, dtype='f8') for v in (x, y, z)))
     r = np.sqrt(x**2 + y**2 + z**2)
     with np.errstate(invalid='ignore', divide='ignore'):
@@ -380,7 +427,7 @@ def real_ylm(ell, m, x, y, z):
         toret = toret * np.sin(abs(m) * phi)
     elif m > 0:
         toret = toret * np.cos(m * phi)
-    return np.where(r == 0, 0., toret)
+    return np.where(r == 0, ylm_at_zero(ell, m), toret)
 
 
 # ----------------------------------------------------------------------------
@@ -519,12 +566,11 @@ def dft3_half(value):
 def local_los_bruteforce(attrs, rmesh, ell):
     """Direct form of the firstpoint estimator before binning: for every half-spectrum mode,
     conj(sum_x rho(x) e^{-ikx} (2l+1) P_l(khat.xhat)) * A0(k) with A0 = rfftn(rho)
-    (addition theorem applied to mesh2.py:319-333, 4pi * sum_m Y_lm Y_lm = (2l+1) P_l)."""
+    (addition theorem applied to mesh2.py:319-333, 4pi * sum_m Y_lm Y_lm = (2l+1) P_l; at zero
+    vectors the reference's Y values, addition_at)."""
     nx, ny, nz = rmesh.shape
     xs = attrs.rcoords()
     X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1).reshape(-1, 3)
-    rx = np.linalg.norm(X, axis=-1)
-    xhat = np.where(rx[:, None] == 0, 0., X / np.where(rx == 0, 1., rx)[:, None])
     ix = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij'), axis=-1).reshape(-1, 3)
     kx, ky, kz = attrs.kcoords('wavenumber')
     A0 = r2c(rmesh)
@@ -534,11 +580,7 @@ def local_los_bruteforce(attrs, rmesh, ell):
         for b in range(ny):
             for c in range(nz // 2 + 1):
                 kv = np.array([kx[a], ky[b], kz[c]])
-                kn = np.linalg.norm(kv)
-                if kn == 0:
-                    continue
-                mu = xhat @ (kv / kn)
-                pl = legendre(ell, mu) * (rx != 0)
+                pl = addition_at(ell, X, kv)  # P_l(xhat.khat), reference values at zero vectors
                 ph = np.exp(-2j * np.pi * (a * ix[:, 0] / nx + b * ix[:, 1] / ny + c * ix[:, 2] / nz))
                 C = (2 * ell + 1) * np.sum(rho * pl * ph)
                 out[a, b, c] = np.conj(C) * A0[a, b, c]
@@ -680,21 +722,16 @@ def circular_correlation_bruteforce(mesh1, mesh2=None):
 
 def local_correlation_bruteforce(attrs, rmesh, ell):
     """sum over cells x and lags s of rho(x) rho(x + s) (2l+1)/(4pi) P_l(xhat . shat), per
-    lag s, with xhat = 0 at the origin (utils.py:252-254): the local line-of-sight term of
-    mesh2.py:445-460 written without Y_lm (addition theorem); small meshes only."""
+    lag s, with the reference's Y values at zero vectors (addition_at; utils.py:252-254): the
+    local line-of-sight term of mesh2.py:445-460 written without Y_lm (addition theorem);
+    small meshes only."""
     xs = attrs.rcoords()
     X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
-    xn = np.linalg.norm(X, axis=-1)
-    xhat = np.where(xn[..., None] > 0, X / np.where(xn > 0, xn, 1.)[..., None], 0.)
     ss = separation_coords(attrs)
     out = np.zeros(rmesh.shape)
     for idx in np.ndindex(*rmesh.shape):
         svec = np.array([ss[a][idx[a]] for a in range(3)])
-        sn = np.linalg.norm(svec)
-        if sn == 0.:
-            continue
-        mu = xhat @ (svec / sn)
-        w = np.where(xn > 0, (2 * ell + 1) / (4 * np.pi) * legendre(ell, mu), 0.)
+        w = (2 * ell + 1) / (4 * np.pi) * addition_at(ell, X, svec)
         out[idx] = np.sum(rmesh * w * np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2)))
     return out
 
@@ -1052,7 +1089,8 @@ def window_local_selection(attrs, rmesh, ell, m, ellin, wa=0):
     """Q_lm(s) of the local (firstpoint) window with a configuration-space theory
     (mesh2.py:1241-1247): V_cell sum_m1 Y_lin,m1(shat) 4 pi / (2 lin + 1)
     c2r(conj(r2c(rho |x|^-wa Y_lm(xhat) Y_lin,m1(xhat))) A0) |s|^wa, A0 = r2c(rho); positions
-    from the origin, Y = 0 at the zero vector, wa the wide-angle order."""
+    from the origin, Y at the zero vector as the reference (ylm_at_zero), wa the wide-angle
+    order."""
     xs = attrs.rcoords()
     xx, xy, xz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
     sx, sy, sz = separation_coords(attrs)
@@ -1069,14 +1107,14 @@ def window_local_selection(attrs, rmesh, ell, m, ellin, wa=0):
 
 def window_local_selection_bruteforce(attrs, rmesh, ell, m, ellin, wa=0):
     """The same Q_lm(s) as an explicit double sum over cells x and lags s,
-    V_cell sum_x rho(x) rho(x + s) Y_lm(xhat) P_lin(xhat . shat) (addition theorem; Y and P
-    taken as 0 at the zero vector), independent of the Y_lin,m1 conventions; tiny meshes."""
+    V_cell sum_x rho(x) rho(x + s) Y_lm(xhat) P_lin(xhat . shat) (addition theorem; the
+    reference's Y values at zero vectors, addition_at), independent of the Y_lin,m1
+    conventions; tiny meshes."""
     n = [int(v) for v in attrs.meshsize]
     xs = attrs.rcoords()
     ss = separation_coords(attrs)
     X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
     xn = np.linalg.norm(X, axis=-1)
-    xhat = X / np.where(xn == 0, 1., xn)[..., None]
     ylm = real_ylm(ell, m, X[..., 0], X[..., 1], X[..., 2])
     if wa:
         with np.errstate(divide='ignore'):
@@ -1085,11 +1123,7 @@ def window_local_selection_bruteforce(attrs, rmesh, ell, m, ellin, wa=0):
     for idx in np.ndindex(*n):
         svec = np.array([ss[a][idx[a]] for a in range(3)])
         sn = np.linalg.norm(svec)
-        if sn == 0.:
-            mu = np.zeros(n)
-        else:
-            mu = xhat @ (svec / sn)
-        leg = np.where((xn == 0) | (sn == 0.), 0., legendre(ellin, mu))  # the Y are 0 at the zero vector
+        leg = addition_at(ellin, X, svec)
         shifted = np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2))  # rho(x + s)
         out[idx] = np.sum(rmesh * shifted * ylm * leg) * sn**wa
     return out * np.prod(attrs.cellsize)
@@ -1156,13 +1190,11 @@ def window_local_terms(attrs, rmesh, ell, m, ellin, e0, e1, wa=0):
 def window_local_terms_bruteforce(attrs, rmesh, ell, m, ellin, e0, e1):
     """The same A_lm(s) from the addition theorem, without any Y_lin,m1:
     N^-3 sum_x sum_k rho(x) rho(x + s) Y_lm(xhat) [e0 <= |k| <= e1] P_lin(xhat . khat) cos(k . s)
-    over the full k grid (the shell and P_lin are even in k), the P and Y taken as 0 at zero
-    vectors; tiny meshes only."""
+    over the full k grid (the shell and P_lin are even in k), the reference's Y values at zero
+    vectors (addition_at); tiny meshes only."""
     n = [int(v) for v in attrs.meshsize]
     xs = attrs.rcoords()
     X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1).reshape(-1, 3)
-    xn = np.linalg.norm(X, axis=-1)
-    xhat = X / np.where(xn == 0, 1., xn)[:, None]
     ylm = real_ylm(ell, m, X[:, 0], X[:, 1], X[:, 2])
     kk = [np.fft.fftfreq(n[a]) * attrs.kfun[a] * n[a] for a in range(3)]
     K = np.stack(np.meshgrid(*kk, indexing='ij'), axis=-1).reshape(-1, 3)
@@ -1170,10 +1202,7 @@ def window_local_terms_bruteforce(attrs, rmesh, ell, m, ellin, e0, e1):
     # the half-spectrum shell uses |k| summed x, y, z as knorm does; on the full grid that is the same
     sel = (kn >= e0) & (kn <= e1)
     K, kn = K[sel], kn[sel]
-    khat = K / np.where(kn == 0, 1., kn)[:, None]
-    leg = legendre(ellin, xhat @ khat.T)  # (x, k)
-    leg[xn == 0, :] = 0.
-    leg[:, kn == 0] = 0.
+    leg = addition_at(ellin, X[:, None, :], K[None, :, :])  # (x, k)
     idx = [np.arange(v) for v in n]
     grid = np.stack(np.meshgrid(*idx, indexing='ij'), axis=-1).reshape(-1, 3)
     ss = [np.fft.fftfreq(n[a]) * attrs.cellsize[a] * n[a] for a in range(3)]
@@ -1241,21 +1270,16 @@ def window_local_smooth_terms(attrs, rmesh, ell, ellin, wa=0):
 
 def window_local_smooth_terms_bruteforce(attrs, rmesh, ell, ellin):
     """The same sum from the addition theorem: (2l+1)/(4 pi) sum_x rho(x) rho(x+s) P_l(xhat.shat)
-    P_lin(xhat.shat), the products taken as 0 where x or s is the zero vector; tiny meshes."""
+    P_lin(xhat.shat), with the reference's Y values at zero vectors (addition_at); tiny
+    meshes."""
     n = [int(v) for v in attrs.meshsize]
     xs = attrs.rcoords()
     ss = separation_coords(attrs)
     X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
-    xn = np.linalg.norm(X, axis=-1)
-    xhat = X / np.where(xn == 0, 1., xn)[..., None]
     out = np.zeros(n)
     for idx in np.ndindex(*n):
         svec = np.array([ss[a][idx[a]] for a in range(3)])
-        sn = np.linalg.norm(svec)
-        if sn == 0.:
-            continue
-        mu = xhat @ (svec / sn)
-        f = np.where(xn == 0, 0., legendre(ell, mu) * legendre(ellin, mu))
+        f = addition_at(ell, X, svec) * addition_at(ellin, X, svec)
         shifted = np.roll(rmesh, [-i for i in idx], axis=(0, 1, 2))
         out[idx] = (2 * ell + 1) / (4. * np.pi) * np.sum(rmesh * shifted * f)
     return out
@@ -1611,9 +1635,9 @@ def bispectrum_shotnoise(attrs, bin, positions, weights=None, resampler='cic', i
 
 def bispectrum_local_mesh(attrs, rho, ell):
     """Third mesh of the local-line-of-sight Scoccimarro bispectrum (mesh3.py:563-572):
-    sum_m r2c(rho Y_lm(xhat)) Y_lm(khat), positions from the origin.  For l >= 1, Y = 0 at
-    zero vectors (the reference's Y_lm are homogeneous in xhat, utils.py:252-260, 299-327); Y_00 is
-    the constant 1/sqrt(4 pi) everywhere, so the l = 0 mesh is r2c(rho) / (4 pi), k = 0 included."""
+    sum_m r2c(rho Y_lm(xhat)) Y_lm(khat), positions from the origin, Y at zero vectors as the
+    reference (ylm_at_zero): Y_00 is the constant 1/sqrt(4 pi) everywhere, so the l = 0 mesh is
+    r2c(rho) / (4 pi), k = 0 included."""
     if ell == 0:
         return r2c(rho) / (4. * np.pi)
     xs = attrs.rcoords()
@@ -1625,13 +1649,11 @@ def bispectrum_local_mesh(attrs, rho, ell):
 
 def bispectrum_local_mesh_bruteforce(attrs, rho, ell):
     """The same mesh as a direct DFT with the addition theorem:
-    sum_n rho(n) exp(-2 pi i k.n / N) (2l+1)/(4 pi) P_l(xhat_n . khat), for l >= 1 0 where x or k
-    is the zero vector (P_0 = 1 everywhere); tiny meshes only."""
+    sum_n rho(n) exp(-2 pi i k.n / N) (2l+1)/(4 pi) P_l(xhat_n . khat), with the reference's Y
+    values at zero vectors (addition_at); tiny meshes only."""
     n = [int(v) for v in attrs.meshsize]
     xs = attrs.rcoords()
     X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1).reshape(-1, 3)
-    xn = np.linalg.norm(X, axis=-1)
-    xhat = X / np.where(xn == 0, 1., xn)[:, None]
     idx = np.stack(np.meshgrid(*[np.arange(v) for v in n], indexing='ij'), axis=-1).reshape(-1, 3)
     kk = attrs.kcoords('wavenumber')
     freq = [np.fft.fftfreq(n[0]) * n[0], np.fft.fftfreq(n[1]) * n[1], np.fft.rfftfreq(n[2]) * n[2]]
@@ -1641,13 +1663,7 @@ def bispectrum_local_mesh_bruteforce(attrs, rho, ell):
         for j in range(n[1]):
             for l in range(n[2] // 2 + 1):
                 kv = np.array([kk[0][i], kk[1][j], kk[2][l]])
-                knv = np.linalg.norm(kv)
-                if ell == 0:
-                    leg = np.ones_like(xn)
-                elif knv == 0.:
-                    continue
-                else:
-                    leg = np.where(xn == 0, 0., legendre(ell, xhat @ (kv / knv)))
+                leg = addition_at(ell, X, kv)
                 phase = np.exp(-2j * np.pi * (idx @ np.array([freq[0][i] / n[0], freq[1][j] / n[1], freq[2][l] / n[2]])))
                 out[i, j, l] = (2 * ell + 1) / (4. * np.pi) * np.sum(flat * phase * leg)
     return out

@@ -172,14 +172,17 @@ def test_bispectrum_shotnoise(resampler, interlacing, los):
         assert np.all(err <= 1e-5 * np.abs(want[ill]) + floor), (ill, err.max(), floor)
 
 
-@pytest.mark.parametrize('interlacing,ells', [(0, (0, 1, 2)), (2, (0, 3))])
-def test_bispectrum_local_los(interlacing, ells):
-    """Local line of sight (mesh3.py:563-576) on a box away from the origin, against the oracle,
-    whose third side is pinned to a direct DFT with P_ell(xhat.khat) in test_oracle.py:
+@pytest.mark.parametrize('interlacing,ells,boxcenter', [(0, (0, 1, 2), (300., 200., 700.)), (2, (0, 3), (300., 200., 700.)),
+                                                       (2, (0, 4), 0.)])
+def test_bispectrum_local_los(interlacing, ells, boxcenter):
+    """Local line of sight (mesh3.py:563-576) against the oracle, whose third side is pinned to a
+    direct DFT with P_ell(xhat.khat) in test_oracle.py; boxcenter 0 puts the origin on a node,
+    and ell = 4 takes the reference's nonzero Y_40 at the zero vector (node and k = 0, which
+    enters the (0, 0, 0) triangle):
     |d| <= 1e-5 |want| + 2e-6 max|want_ell| + 1e-6 max|want_0| (float32 Y_lm sums over m)."""
     from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_mesh3_spectrum
-    say('bispec local', interlacing, ells)
-    attrs = MeshAttrs(meshsize=32, boxsize=400., boxcenter=(300., 200., 700.))
+    say('bispec local', interlacing, ells, boxcenter)
+    attrs = MeshAttrs(meshsize=32, boxsize=400., boxcenter=boxcenter)
     pos, w = _mock(attrs, 30000, 29)
     mesh = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='tsc', interlacing=interlacing, compensate=True,
                                                                    out='complex')

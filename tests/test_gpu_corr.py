@@ -115,11 +115,12 @@ def test_correlation_cross_and_real_input():
     assert_xi_close([pole.values('num_raw') for pole in xi], want['num_raw'], label='cross')
 
 
-@pytest.mark.parametrize('los', ['firstpoint', 'endpoint'])
-def test_correlation_local_los_matches_oracle(los):
+@pytest.mark.parametrize('los,boxcenter', [('firstpoint', (900., -300., 600.)), ('endpoint', (900., -300., 600.)), ('firstpoint', 0.)])
+def test_correlation_local_los_matches_oracle(los, boxcenter):
+    """Observer outside the box, or on a node (boxcenter 0: the reference's Y_40 constant term)."""
     from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2CorrelationPoles, compute_mesh2_correlation
-    say('corr local', los)
-    attrs = MeshAttrs(meshsize=32, boxsize=500., boxcenter=(900., -300., 600.))  # observer outside the box
+    say('corr local', los, boxcenter)
+    attrs = MeshAttrs(meshsize=32, boxsize=500., boxcenter=boxcenter)
     pos, w = _mock(attrs, 20000, 11)
     mesh = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='tsc', interlacing=2, compensate=True, out='real')
     ells = (0, 2, 4)

@@ -8,6 +8,7 @@ of a unitary-amplitude Gaussian field), 75-91 (shot noise), 459-467 (Ylm);
 tests/test_particle2.py:58-63 (Legendre).
 """
 import itertools
+import math
 
 import numpy as np
 import pytest
@@ -78,21 +79,113 @@ def test_paint_conserves_sum(resampler, interlacing, compensate):
     assert np.isclose(mesh.sum(), w.sum(), rtol=1e-12)
 
 
-def test_interlacing_phase_reproduces_shifted_paint():
-    # with interlacing, each shifted mesh times exp(+i s sum k) is the unshifted-centred field:
-    # for NGP-free check use a single particle on a node and s = 1/2 with CIC -> compare to direct
-    attrs = po.Attrs(meshsize=8, boxsize=8.)
-    pos = np.array([[0.5, 0.5, 0.5]]) - 4. + 0.  # grid units 4.5
-    w = np.array([1.])
-    acc = po.paint(attrs, pos + 4., w, resampler='cic', interlacing=2, out='complex')
-    # direct: mean of the two phased transforms
+def _direct_interlaced_ngp(meshsize, xg, w, ni):
+    """Interlaced NGP spectrum written from its meaning: the particle painted at x~ + s lands on
+    node n = round(x~ + s), which stands for the physical grid point n - s of the unshifted
+    mesh; the n_i meshes are averaged as Fourier sums over those points,
+    A(k) = 1/n_i sum_j sum_p w_p exp(-i k.(n_pj - s_j)), with k_a = 2 pi m_a / N_a and the
+    reference's index -> frequency map (numpy fftfreq on the first axes, rfftfreq on the last).
+    A direct sum over particles and half-spectrum modes; no FFT, no phase factor table."""
+    n = np.array(meshsize)
+    freqs = [np.fft.fftfreq(n[0]) * 2. * np.pi, np.fft.fftfreq(n[1]) * 2. * np.pi, np.fft.rfftfreq(n[2]) * 2. * np.pi]
+    K = np.stack(np.meshgrid(*freqs, indexing='ij'), axis=-1)  # (nx, ny, nz//2+1, 3)
+    out = np.zeros(K.shape[:3], dtype='c16')
+    for j in range(ni):
+        s = j / ni
+        nodes = np.round(xg + s) - s  # physical grid points, in grid units (half-even ties excluded below)
+        for x, wp in zip(nodes, w):
+            out += wp * np.exp(-1j * (K @ x)) / ni
+    return out
+
+
+@pytest.mark.parametrize('ni', [2, 3])
+@pytest.mark.parametrize('meshsize', [(8, 8, 8), (6, 4, 10)])
+def test_interlacing_phase_against_direct_sum(ni, meshsize):
+    """The interlacing combine (mesh.py:2738-2747: exp(+i s sum k_circ) / n_i, summed over the
+    shifts) against the direct Fourier sum over the physical node positions of every shifted
+    NGP paint, on all modes including the Nyquist planes.  A wrong sign of the phase, a shift of
+    -s instead of +s, a missing 1/n_i or a wrong Nyquist frequency fails it."""
+    rng = np.random.default_rng(ni + sum(meshsize))
+    n = np.array(meshsize)
+    attrs = po.Attrs(meshsize=n, boxsize=n * 2.5, boxcenter=(0.3, -0.2, 0.1))
+    pos = (rng.uniform(0., 1., (40, 3)) - 0.5) * attrs.boxsize + attrs.boxcenter
+    xg = po.to_grid(attrs, pos)
+    for j in range(ni):  # keep clear of exact halves, where NGP rounding would be a tie
+        assert np.all(np.abs(np.mod(xg + j / ni, 1.) - 0.5) > 1e-6)
+    w = rng.uniform(0.5, 1.5, len(pos))
+    got = po.paint(attrs, pos, w, resampler='ngp', interlacing=ni, out='complex')
+    want = _direct_interlaced_ngp(n, xg, w, ni)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    # a plausible mistake, planted in the oracle's phase: the opposite sign fails
     kx, ky, kz = attrs.kcoords('circular')
     ks = kx[:, None, None] + ky[None, :, None] + kz[None, None, :]
-    xg = po.to_grid(attrs, pos + 4.)
-    direct = sum(np.fft.rfftn(po.paint_grid(attrs.meshsize, xg + s, w, 2)) * np.exp(1j * s * ks) / 2 for s in (0., 0.5))
-    assert np.allclose(acc, direct)
-    # Nyquist sign convention: last axis uses +pi (rfftfreq), first axes -pi (fftfreq)
-    assert np.isclose(kz[-1], np.pi) and np.isclose(kx[4], -np.pi)
+    wrong = sum(po.r2c(po.paint_grid(n, xg + j / ni, w, 1)) * np.exp(-1j * (j / ni) * ks) / ni for j in range(ni))
+    assert np.abs(wrong - want).max() > 1e-3 * np.abs(want).max()
+
+
+def test_circular_nyquist_convention():
+    """Known answer of the circular frequencies (mesh.py:377-381: jnp.fft.fftfreq on the first
+    axes, rfftfreq on the hermitian last axis, times 2 pi): the Nyquist index N/2 is -pi along
+    x and y and +pi along z."""
+    attrs = po.Attrs(meshsize=(8, 6, 10), boxsize=100.)
+    kx, ky, kz = attrs.kcoords('circular')
+    assert kx[4] == -np.pi and ky[3] == -np.pi and kz[5] == np.pi
+    assert np.allclose(kx[:4], 2. * np.pi * np.arange(4) / 8.) and np.allclose(kx[5:], 2. * np.pi * np.arange(-3, 0) / 8.)
+    assert len(kz) == 6 and np.all(np.diff(kz) > 0.)
+
+
+def test_pole_value_single_particle_is_zero():
+    """value() = (num_raw - num_shotnoise) / norm (lsstypes Mesh2SpectrumPole): a single particle
+    of weight w painted with NGP on a node has |A(k)|^2 = w^2 in every mode and shot noise
+    sum w^2 = w^2 (mesh2.py:603), so the monopole value is exactly 0 in every bin; with the
+    shot noise added instead of subtracted it would be 2 w^2 / norm."""
+    attrs = po.Attrs(meshsize=16, boxsize=160., boxcenter=0.)
+    w = np.array([1.7])
+    pos = np.array([[20., -30., 50.]])  # a node: grid units (10, 5, 13)
+    mesh = po.paint(attrs, pos, w, resampler='ngp', interlacing=0, out='complex')
+    assert np.allclose(np.abs(mesh)**2, w[0]**2)
+    bin = po.BinPoles(attrs, edges={'step': 0.02}, ells=(0, 2, 4))
+    res = po.spectrum_poles(attrs, bin, mesh, los='z')
+    sn = po.fkp_shotnoise(w)
+    assert sn == w[0]**2
+    norm = po.box_normalization(attrs, w)
+    value = po.pole_value(res['num_raw'][0], norm, sn)
+    assert np.all(np.abs(value) < 1e-12 * w[0]**2 / norm)
+
+
+def test_pole_value_two_particles_closed_form():
+    """Two particles on nodes x1, x2: |A(k)|^2 = w1^2 + w2^2 + 2 w1 w2 cos(k.(x1 - x2)), so
+    value() in each bin is 2 w1 w2 <cos(k.d)>_bin / norm with the box normalization
+    (sum w)^2 / (N^3 V_cell) (mesh.py:3121-3129), written here as an explicit sum over the
+    bin's modes (hermitian weights 1 + (kz > 0))."""
+    attrs = po.Attrs(meshsize=16, boxsize=160., boxcenter=0.)
+    w = np.array([1.3, 0.6])
+    pos = np.array([[20., -30., 50.], [-40., 0., 10.]])
+    mesh = po.paint(attrs, pos, w, resampler='ngp', interlacing=0, out='complex')
+    bin = po.BinPoles(attrs, edges={'step': 0.03}, ells=(0,))
+    res = po.spectrum_poles(attrs, bin, mesh, los='z')
+    norm = po.box_normalization(attrs, w)
+    assert np.isclose(norm, w.sum()**2 / (16**3 * 10.**3))
+    value = po.pole_value(res['num_raw'][0], norm, po.fkp_shotnoise(w))
+    kx, ky, kz = attrs.kcoords('wavenumber')
+    d = pos[0] - pos[1]
+    want = np.zeros(bin.edges.shape[0])
+    count = np.zeros(bin.edges.shape[0])
+    for a in kx:
+        for b in ky:
+            for ic, c in enumerate(kz):
+                k = np.sqrt(a * a + b * b + c * c)
+                ib = np.searchsorted(bin.edges[:, 0], k, side='right') - 1
+                if ib < 0 or k >= bin.edges[ib, 1]:
+                    continue
+                wm = 1 + (c > 0)
+                want[ib] += wm * 2. * w[0] * w[1] * np.cos(a * d[0] + b * d[1] + c * d[2])
+                count[ib] += wm
+    with np.errstate(invalid='ignore'):
+        want = want / count / norm
+    ok = count > 0
+    assert np.array_equal(count[ok], bin.nmodes[ok])
+    assert np.allclose(value[ok], want[ok], rtol=1e-10, atol=1e-12 * np.abs(want).max())
 
 
 # ---------------------------------------------------------------- FFT
@@ -204,7 +297,59 @@ def test_ylm_normalisation_and_addition_theorem():
             if m != 0:
                 ref = np.sqrt(2.) * ref
             assert np.allclose(np.abs(po.real_ylm(ell, m, *a.T)), np.abs(ref), atol=1e-12)
-            assert po.real_ylm(ell, m, 0., 0., 0.) == 0.  # utils.py:252-254
+            assert po.real_ylm(ell, m, 0., 0., 0.) == po.ylm_at_zero(ell, m)
+
+
+def _sympy_real_ylm(ell, m):
+    """The real Y_lm polynomial in (xhat, yhat, zhat) built the way utils.py:299-327 builds it
+    (associated Legendre of cos(theta), cos/sin(m phi) expanded, phi and theta replaced by
+    Cartesian ratios, x^2 + y^2 + z^2 -> r^2, then x/r -> xhat ...), with the normalisation of
+    utils.py:227-238.  Written here from those steps; the reference itself is not imported."""
+    sp = pytest.importorskip('sympy')
+    amp = np.sqrt((2 * ell + 1) / (4 * np.pi))
+    if m != 0:
+        amp *= np.sqrt(math.factorial(ell - abs(m)) / math.factorial(ell + abs(m))) * np.sqrt(2.) * (-1)**m
+    x, y, z, r = sp.symbols('x y z r', real=True, positive=True)
+    xh, yh, zh = sp.symbols('xhat yhat zhat', real=True, positive=True)
+    phi, theta = sp.symbols('phi theta', real=True)
+    defs = [(sp.sin(phi), y / sp.sqrt(x**2 + y**2)), (sp.cos(phi), x / sp.sqrt(x**2 + y**2)),
+            (sp.cos(theta), z / sp.sqrt(x**2 + y**2 + z**2))]
+    expr = sp.assoc_legendre(ell, abs(m), sp.cos(theta))
+    expr *= sp.expand_trig(sp.sin(abs(m) * phi) if m < 0 else (sp.cos(m * phi) if m > 0 else 1))
+    expr = sp.together(expr.subs(defs)).subs(x**2 + y**2 + z**2, r**2)
+    expr = amp * expr.expand().subs([(x / r, xh), (y / r, yh), (z / r, zh)])
+    return sp.lambdify((xh, yh, zh), expr, modules=['numpy']), (xh, yh, zh), expr
+
+
+def test_ylm_zero_vector_matches_sympy_construction():
+    """utils.py:252-254 sends the zero vector to xhat = 0 and evaluates the sympy polynomial
+    there: its constant term.  The oracle's ylm_at_zero must equal that for every l <= 4 (nonzero
+    only for Y_00 and Y_40), and on unit vectors the polynomial must be the oracle's Y_lm up to
+    the per-m sign."""
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(50, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    for ell in range(5):
+        for m in range(-ell, ell + 1):
+            f, syms, expr = _sympy_real_ylm(ell, m)
+            at0 = float(expr.subs({s_: 0 for s_ in syms}))
+            assert np.isclose(po.ylm_at_zero(ell, m), at0, rtol=1e-12, atol=1e-15), (ell, m, at0)
+            vals = np.broadcast_to(np.asarray(f(*u.T), dtype='f8'), (len(u),))
+            assert np.allclose(np.abs(vals), np.abs(po.real_ylm(ell, m, *u.T)), atol=1e-12)
+    assert po.ylm_at_zero(4, 0) > 0.3 and po.ylm_at_zero(2, 0) == 0.
+
+
+def test_addition_at_zero_vectors():
+    """addition_at(l, u, v) = 4 pi/(2l+1) sum_m Y_lm(u) Y_lm(v) with the oracle's Y_lm, zero
+    vectors included."""
+    rng = np.random.default_rng(6)
+    u = rng.normal(size=(30, 3))
+    v = rng.normal(size=(30, 3))
+    u[:5] = 0.
+    v[3:8] = 0.
+    for ell in range(5):
+        s = sum(po.real_ylm(ell, m, *u.T) * po.real_ylm(ell, m, *v.T) for m in range(-ell, ell + 1)) * 4. * np.pi / (2 * ell + 1)
+        assert np.allclose(po.addition_at(ell, u, v), s, atol=1e-12), ell
 
 
 # ---------------------------------------------------------------- estimator
@@ -266,10 +411,12 @@ def test_cross_equals_auto_for_same_mesh():
 
 
 @pytest.mark.parametrize('ell', [2, 4])
-def test_local_los_vs_bruteforce(ell):
-    # firstpoint estimator (Ylm sums + FFTs) equals the direct double sum with P_l(khat.xhat)
+@pytest.mark.parametrize('boxcenter', [(100., -20., 45.), 0.])
+def test_local_los_vs_bruteforce(ell, boxcenter):
+    # firstpoint estimator (Ylm sums + FFTs) equals the direct double sum with P_l(khat.xhat);
+    # with boxcenter 0 the origin is a node, where the reference's Y_40 is not 0
     rng = np.random.default_rng(ell)
-    attrs = po.Attrs(meshsize=6, boxsize=60., boxcenter=(100., -20., 45.))
+    attrs = po.Attrs(meshsize=6, boxsize=60., boxcenter=boxcenter)
     mesh = rng.normal(size=(6, 6, 6))
     bin = po.BinPoles(attrs, edges={'step': 0.05}, ells=(0, ell))
     res = po.spectrum_poles(attrs, bin, mesh, los='firstpoint')
@@ -439,11 +586,12 @@ def test_correlation_global_los_bruteforce(los):
     assert np.allclose(got['norm'], np.prod(attrs.meshsize) / vcell)
 
 
-def test_correlation_local_los_bruteforce():
+@pytest.mark.parametrize('boxcenter', [(70., -20., 40.), 0.])
+def test_correlation_local_los_bruteforce(boxcenter):
     """Local (firstpoint) line of sight: the Y_lm route of mesh2.py:445-460 against the
-    explicit double sum over cells and lags with P_l(xhat . shat)."""
+    explicit double sum over cells and lags with P_l(xhat . shat) (origin a node or not)."""
     rng = np.random.default_rng(9)
-    attrs = po.Attrs((6, 6, 6), (60., 60., 60.), (70., -20., 40.))
+    attrs = po.Attrs((6, 6, 6), (60., 60., 60.), boxcenter)
     mesh = rng.normal(size=attrs.meshsize)
     ells = (0, 2, 4)
     bin = po.BinCorrelationPoles(attrs, edges={'step': 9.}, ells=ells)
@@ -778,16 +926,23 @@ def test_bispectrum_shotnoise_monopole_is_power_spectrum_sum():
     np.testing.assert_allclose(sn[0], want, rtol=1e-12)
 
 
-@pytest.mark.parametrize('ell', [0, 1, 2, 3])
+@pytest.mark.parametrize('ell', [0, 1, 2, 3, 4])
 def test_bispectrum_local_mesh_vs_direct_dft(ell):
     """The third mesh of the local-line-of-sight bispectrum, sum_m r2c(rho Y_lm(xhat)) Y_lm(khat)
-    (mesh3.py:563-572), against a direct DFT weighted by (2l+1)/(4 pi) P_l(xhat.khat)."""
+    (mesh3.py:563-572), against a direct DFT weighted by (2l+1)/(4 pi) P_l(xhat.khat), with the
+    reference's Y values at zero vectors (the origin is a node here, and k = 0 is included)."""
     rng = np.random.default_rng(4)
     a = po.Attrs(6, 60., 20.)
     rho = rng.uniform(0.5, 1.5, (6, 6, 6))
     got = po.bispectrum_local_mesh(a, rho, ell)
     want = po.bispectrum_local_mesh_bruteforce(a, rho, ell)
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-10 * np.abs(want).max())
+    if ell == 4:
+        # k = 0: sum_m Y_4m(0) sum_x rho Y_4m(xhat) = Y_40(0) sum_x rho Y_40(xhat), nonzero
+        xs = a.rcoords()
+        X = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
+        c0 = po.ylm_at_zero(4, 0) * np.sum(rho * po.real_ylm(4, 0, X[..., 0], X[..., 1], X[..., 2]))
+        assert abs(c0) > 1e-3 and np.isclose(got[0, 0, 0].real, c0, rtol=1e-12)
 
 
 def test_bispectrum_shotnoise_local_monopole_and_ell2():
