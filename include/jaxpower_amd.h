@@ -86,6 +86,18 @@ int mp_bin_poles_partial(void* stream, const void* partial, int64_t pitch, const
                          const double* kz, const double* k2_thresholds, int nedges, const double los[3], int nprod,
                          const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out, void* workspace,
                          size_t workspace_bytes);
+/* mp_bin_poles_partial without a workspace argument, for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The combine tables and per-block partial histograms live in a
+ * fixed device array of mp_noalloc_scratch_bytes() (16 MiB) per device, which the library holds
+ * until the process ends; sizes whose tables and one partial histogram exceed it return -2.
+ * Every call on one device shares that array: calls that overlap on one device race.  Same
+ * arguments, kernels and results as mp_bin_poles_partial. */
+int64_t mp_noalloc_scratch_bytes(void);
+int mp_bin_poles_partial_noalloc(void* stream, const void* partial, int64_t pitch, const int32_t meshsize[3], int order,
+                                 int interlacing, int compensate, int zero_dc, const double* kx, const double* ky,
+                                 const double* kz, const double* k2_thresholds, int nedges, const double los[3], int nprod,
+                                 const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out);
 /* ---- Backward (d loss / d positions, weights) ---------------------------------------- */
 /* jax.grad of num_raw through the binning of compute_mesh2_spectrum (jaxpower/mesh2.py:
  * 346-369) and the interlacing combine (jaxpower/mesh.py:2739-2751), transposed by hand:

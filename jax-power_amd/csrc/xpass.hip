@@ -674,7 +674,7 @@ size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int ne
 
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
-                double* out, void* ws, size_t ws_bytes, int linear, const int32_t* ylm_m) {
+                double* out, void* ws, size_t ws_bytes, int linear, const int32_t* ylm_m, bool fit_partials) {
   MP_REQUIRE(partial && out && ell && remove_zero, "bin_partial: null pointer");
   MP_REQUIRE(ni >= 1, "bin_partial: interlacing %d < 1", ni);
   MP_REQUIRE(nprod >= 1 && nprod <= XB_MAX_PROD, "bin_partial: %d products not in 1..%d", nprod, XB_MAX_PROD);
@@ -687,7 +687,16 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   CombineTables tab;
   rc = combine_tables(s, n, order, ni, compensate, wsp, tab);
   if (rc) return rc;
-  const size_t pd = (size_t)XB_MAX_BLOCKS * nprod * (g.nedges - 1) * 2;
+  const size_t per_block = (size_t)nprod * (g.nedges - 1) * 2;
+  size_t pd = (size_t)XB_MAX_BLOCKS * per_block;
+  if (fit_partials) {  // a fixed workspace: as many per-block histograms as it holds (fewer blocks)
+    const size_t off = align_up(wsp.used, 256);
+    const size_t room = ws_bytes > off ? (ws_bytes - off) / sizeof(double) : 0;
+    if (room < pd) pd = room / per_block * per_block;
+    if (pd < per_block)
+      return set_error(MP_ERR_WORKSPACE, "bin_partial: %zu bytes hold the tables but not one partial histogram of %zu bytes",
+                       ws_bytes, per_block * sizeof(double));
+  }
   double* pbuf = wsp.take<double>(pd);
   if (!pbuf) return set_error(MP_ERR_WORKSPACE, "bin_partial: workspace too small");
   XPassArgs a{};
