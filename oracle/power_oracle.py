@@ -163,25 +163,37 @@ def _stencil(order):
     return np.array(list(product(ishifts, repeat=3)), dtype='i8')
 
 
-def paint_grid(meshsize, xg, weights, order):
-    """Scatter-add particles given in grid units, resamplers.py:73-103.
-    One np.bincount over every (particle, stencil offset) pair (the same sum as the
-    reference's scatter, in another order)."""
+def paint_grid(meshsize, xg, weights, order, threads=1):
+    """Scatter-add particles given in grid units, resamplers.py:73-103: per axis the p stencil
+    indices (periodic) and 1-D kernel values, then one unbuffered np.add.at per 3-D stencil
+    offset of the product weight (the same sum as the reference's scatter, in another order).
+    threads > 1 splits the offsets over that many threads, each with its own mesh."""
     shape = tuple(int(n) for n in meshsize)
-    size = int(np.prod(shape))
     xg = np.asarray(xg, dtype='f8')
     w = np.broadcast_to(np.asarray(weights, dtype='f8'), xg.shape[:1])
-    id0 = (np.round if order % 2 else np.floor)(xg).astype('i8')
-    flats, vals = [], []
-    for ishift in _stencil(order):
-        idx = id0 + ishift
-        ker = resampler_kernel(order, np.abs(idx - xg)).prod(axis=-1)
-        idx = idx % np.array(shape)
-        flats.append(np.ravel_multi_index(tuple(idx.T), shape))
-        vals.append(w * ker)
-    if not flats or not len(xg):
+    size = int(np.prod(shape))
+    if not len(xg):
         return np.zeros(shape)
-    return np.bincount(np.concatenate(flats), weights=np.concatenate(vals), minlength=size).reshape(shape)
+    id0 = (np.round if order % 2 else np.floor)(xg).astype('i8')
+    ishifts = np.arange(order) - (order - 1) // 2
+    idx = [[(id0[:, a] + s) % shape[a] for s in ishifts] for a in range(3)]
+    ker = [[resampler_kernel(order, np.abs(id0[:, a] + s - xg[:, a])) for s in ishifts] for a in range(3)]
+    offsets = list(product(range(order), repeat=3))
+
+    def scatter(part):
+        mesh = np.zeros(size, dtype='f8')
+        for ox, oy, oz in part:
+            flat = (idx[0][ox] * shape[1] + idx[1][oy]) * shape[2] + idx[2][oz]
+            np.add.at(mesh, flat, w * ker[0][ox] * ker[1][oy] * ker[2][oz])
+        return mesh
+
+    threads = max(1, min(int(threads), len(offsets)))
+    if threads == 1:
+        return scatter(offsets).reshape(shape)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(threads) as ex:
+        meshes = list(ex.map(scatter, [offsets[i::threads] for i in range(threads)]))
+    return sum(meshes[1:], meshes[0]).reshape(shape)
 
 
 def paint_grid_bruteforce(meshsize, xg, weights, order):
@@ -236,24 +248,36 @@ def c2r(value, meshsize):
     return scipy_fft.irfftn(value, s=tuple(int(n) for n in meshsize), axes=(0, 1, 2), workers=FFT_WORKERS)
 
 
-def paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compensate=False, out='real'):
-    """_paint, mesh.py:2673-2754 (single device)."""
+def paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compensate=False, out='real', threads=1):
+    """_paint, mesh.py:2673-2754 (single device).  threads > 1 paints the interlaced meshes
+    concurrently and splits each scatter over threads / n_i threads (same sums)."""
     order = ORDERS[resampler]
     if weights is None:
         weights = np.ones(len(positions))
     xg = to_grid(attrs, positions)
     ni = max(interlacing, 1)
     if ni <= 1:
-        mesh = paint_grid(attrs.meshsize, xg, weights, order)
+        mesh = paint_grid(attrs.meshsize, xg, weights, order, threads=threads)
         if compensate:
             cmesh = r2c(mesh) * compensation(attrs, order)
             return c2r(cmesh, attrs.meshsize) if out == 'real' else cmesh
         return mesh if out == 'real' else r2c(mesh)
     kx, ky, kz = attrs.kcoords('circular')
     ksum = kx[:, None, None] + ky[None, :, None] + kz[None, None, :]
+
+    def shifted(shift):
+        return r2c(paint_grid(attrs.meshsize, xg + shift, weights, order, threads=max(1, threads // ni))) * np.exp(shift * 1j * ksum) / ni
+
+    shifts = np.arange(ni) * 1. / ni
+    if threads > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(ni) as ex:
+            terms = list(ex.map(shifted, shifts))
+    else:
+        terms = [shifted(shift) for shift in shifts]
     acc = 0.
-    for shift in np.arange(ni) * 1. / ni:
-        acc = acc + r2c(paint_grid(attrs.meshsize, xg + shift, weights, order)) * np.exp(shift * 1j * ksum) / ni
+    for term in terms:
+        acc = acc + term
     if compensate:
         acc = acc * compensation(attrs, order)
     return c2r(acc, attrs.meshsize) if out == 'real' else acc

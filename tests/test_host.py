@@ -249,3 +249,24 @@ def test_real_gaunt_host_invariants():
                     tot_lib += g * g
                     tot_or += po.real_gaunt(l1, a, l2, b, l3, c)**2
         assert abs(tot_lib - tot_or) < 1e-6 * max(tot_or, 1e-3), (l1, l2, l3, tot_lib, tot_or)
+
+
+def test_shell_cap_positions_distribution():
+    """generate_shell_cap_positions (the survey geometry of the FKP workload) against the law it
+    states, on the CPU: r^3 uniform in [rmin^3, rmax^3], cos(theta) uniform in [cos 60deg, 1],
+    phi uniform; shell_cap_volume against a Monte Carlo count in the bounding box."""
+    from scipy import stats
+    from jaxpower_amd.mock import generate_shell_cap_positions, shell_cap_volume
+    pos = generate_shell_cap_positions(200_000, seed=3, device='cpu').numpy().astype('f8')
+    r = np.linalg.norm(pos, axis=1)
+    assert np.all((r > 1000. * (1. - 1e-6)) & (r < 2000. * (1. + 1e-6)))
+    assert stats.kstest((r**3 - 1000.**3) / (2000.**3 - 1000.**3), 'uniform').pvalue > 1e-4
+    assert stats.kstest((pos[:, 2] / r - 0.5) / 0.5, 'uniform').pvalue > 1e-4
+    assert stats.kstest((np.arctan2(pos[:, 1], pos[:, 0]) + np.pi) / (2. * np.pi), 'uniform').pvalue > 1e-4
+    rng = np.random.default_rng(4)
+    u = rng.uniform(-2000., 2000., (2_000_000, 3))
+    u[:, 2] = np.abs(u[:, 2])  # the cap is in z > 0
+    ru = np.linalg.norm(u, axis=1)
+    inside = (ru >= 1000.) & (ru < 2000.) & (u[:, 2] >= 0.5 * ru)
+    frac, cube = inside.mean(), 4000.**2 * 2000.
+    assert abs(shell_cap_volume() - frac * cube) < 5. * np.sqrt(frac * (1. - frac) / len(u)) * cube
