@@ -70,4 +70,20 @@ struct RealBinArgs {
 // out[e] = sum over blocks of partial[block][e], in a fixed order.
 int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out);
 
+// Products of stored half spectra binned with Legendre weights (bin.hip): product p is
+// conj(mesh[pi[p]]) mesh[pj[p]] (pi = -1: the constant 1), weight P_ell[p](mu), the (0,0,0) mode
+// skipped where rz[p]; out[p][b][re, im].
+constexpr int BIN_MAX_PROD = 6;
+constexpr int BIN_MAX_MESH = 4;
+struct BinProducts {
+  int nmesh, nprod;
+  const float2* mesh[BIN_MAX_MESH];
+  int64_t pitch[BIN_MAX_MESH];
+  int pi[BIN_MAX_PROD], pj[BIN_MAX_PROD], ell[BIN_MAX_PROD], rz[BIN_MAX_PROD];
+  float los[3];
+  int need_mu;
+};
+size_t bin_workspace_bytes(int nprod, int nedges);
+int bin_poles(hipStream_t s, const BinGeom& g, const BinProducts& P, double* out, void* ws, size_t ws_bytes);
+
 }  // namespace mp

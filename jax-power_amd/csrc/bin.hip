@@ -21,17 +21,6 @@ namespace mp {
 
 constexpr int BIN_SEG = 16;
 constexpr int BIN_NT = 256;
-constexpr int BIN_MAX_PROD = 6;
-constexpr int BIN_MAX_MESH = 4;
-
-struct BinProducts {
-  int nmesh, nprod;
-  const float2* mesh[BIN_MAX_MESH];
-  int64_t pitch[BIN_MAX_MESH];
-  int pi[BIN_MAX_PROD], pj[BIN_MAX_PROD], ell[BIN_MAX_PROD], rz[BIN_MAX_PROD];
-  float los[3];
-  int need_mu;
-};
 
 // v[m] for a run-time m without indexing a private array (which would live in scratch memory).
 __device__ __forceinline__ float2 pick_mesh(const float2 (&v)[BIN_MAX_MESH], int m) {

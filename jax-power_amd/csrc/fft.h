@@ -10,6 +10,15 @@ constexpr int TW_N = 4096;  // twiddle table length (largest supported transform
 
 enum { PRO_NONE = 0, PRO_YLM = 1, PRO_TOPHAT = 2 };
 
+// Node coordinate o + i d formed as numpy forms arange(n) * d + o in float64 (the product
+// rounded, then the sum: no fused multiply-add), then rounded to float.  A node that should be
+// the origin can carry a residual such as -5.7e-14 this way (1000/122 * 61 - 500), which fixes
+// its direction as the reference's float64 coordinates (MeshAttrs.rcoords, mesh.py:1573-1603) do.
+__host__ __device__ __forceinline__ float node_coord(double o, double d, int i) {
+#pragma clang fp contract(off)
+  return (float)((double)i * d + o);
+}
+
 // Real-space prologue of the z pass: multiplies each real input by Y_lm(xhat) of the node
 // position (local line of sight, jaxpower/mesh2.py:319-323), by the configuration-space shell
 // of the window matrix, ttab[|n|^2] P_ell(n.los/|n|) with n the separation index vector
@@ -18,8 +27,8 @@ struct RealPrologue {
   int kind;  // PRO_NONE, PRO_YLM or PRO_TOPHAT
   int ell, m;
   int ny, nz;
-  float x0, y0, z0;  // position of node (0,0,0): boxcenter - boxsize/2
-  float dx, dy, dz;  // cellsize
+  double x0, y0, z0;  // position of node (0,0,0): boxcenter - boxsize/2
+  double dx, dy, dz;  // cellsize
   // PRO_TOPHAT
   int nx;
   const float* ttab;
@@ -37,8 +46,7 @@ struct RealPrologue {
       }
       return v;
     }
-    float px = x0 + dx * ix, py = y0 + dy * iy, pz = z0 + dz * z;
-    return ylm_of_vector(ell, m, px, py, pz);
+    return ylm_of_vector(ell, m, node_coord(x0, dx, ix), node_coord(y0, dy, iy), node_coord(z0, dz, z));
   }
 };
 
@@ -65,6 +73,17 @@ struct CombineTables {
   const float* cy;
   const float* cz;
 };
+
+// Any-length passes (fft_any.hip), for meshes whose sizes are not all powers of two.
+bool fft_any_shape_ok(const int32_t n[3]);
+int any_col_fft(hipStream_t s, const float2* in, float2* out, int L, int ncols, int64_t nouter, int64_t fstride, int64_t ostride,
+                int64_t out_fstride, int64_t out_ostride, bool inv, float scale, const char* name);
+int any_zpass_r2c(hipStream_t s, const float* in, float2* out, int L, int64_t nrows, int64_t pitch, const RealPrologue& pro);
+int any_zpass_c2r(hipStream_t s, const float2* in, int64_t pitch, float* out, int L, int64_t nrows, float scale);
+int ylm_acc_any(hipStream_t s, const float2* t, int64_t tpitch, float2* acc, int64_t apitch, const int32_t n[3], const KPrologue& kp,
+                int first);
+// true when every axis is a power of two in [2, 2048] (the specialised passes apply)
+bool pow2_shape(const int32_t n[3]);
 
 int ensure_twiddles();
 const float2* twiddle_device_ptr();

@@ -119,9 +119,11 @@ zpass_r2c_ylm_kernel(const float* __restrict__ in, float2* __restrict__ out, int
   constexpr int EPT = (C * M + NT - 1) / NT;
   extern __shared__ float2 lds[];
   __shared__ float2 stw[2 * M];
+  __shared__ float spz[2 * M];  // node z of every cell of a row
   const int tid = threadIdx.x;
   const int64_t row0 = (int64_t)blockIdx.x * C;
   load_twiddles<2 * M, NT>(stw, g_twiddle);
+  for (int z = tid; z < 2 * M; z += NT) spz[z] = node_coord(pro.z0, pro.dz, z);
   const float2* in2 = reinterpret_cast<const float2*>(in);
   float2 reg[EPT];
   float px[EPT], py[EPT];  // node x, y of each element's row, computed once for every m
@@ -133,9 +135,10 @@ zpass_r2c_ylm_kernel(const float* __restrict__ in, float2* __restrict__ out, int
     reg[e] = ok ? in2[row * M + idx % M] : make_float2(0.f, 0.f);
     const int r = ok ? (int)row : 0;
     const int ix = r / pro.ny, iy = r - ix * pro.ny;
-    px[e] = pro.x0 + pro.dx * ix;
-    py[e] = pro.y0 + pro.dy * iy;
+    px[e] = node_coord(pro.x0, pro.dx, ix);
+    py[e] = node_coord(pro.y0, pro.dy, iy);
   }
+  __syncthreads();
   constexpr int NK = M / 2 + 1;
   for (int m = -pro.ell; m <= pro.ell; ++m) {
 #pragma unroll
@@ -147,8 +150,8 @@ zpass_r2c_ylm_kernel(const float* __restrict__ in, float2* __restrict__ out, int
         // opaque per m: otherwise the compiler hoists every loop-invariant term of every
         // Y_lm case out of the m loop and runs out of registers
         asm volatile("" : "+v"(px[e]), "+v"(py[e]));
-        v.x *= ylm_of_vector(pro.ell, m, px[e], py[e], pro.z0 + pro.dz * (2 * l));
-        v.y *= ylm_of_vector(pro.ell, m, px[e], py[e], pro.z0 + pro.dz * (2 * l + 1));
+        v.x *= ylm_of_vector(pro.ell, m, px[e], py[e], spz[2 * l]);
+        v.y *= ylm_of_vector(pro.ell, m, px[e], py[e], spz[2 * l + 1]);
         lds[c * LP + l] = v;
       }
     }
@@ -665,9 +668,31 @@ static int check_shape(const int32_t n[3]) {
   return MP_OK;
 }
 
+bool pow2_shape(const int32_t n[3]) {
+  for (int a = 0; a < 3; ++a)
+    if (!is_pow2(n[a]) || n[a] < 2 || n[a] > 2048) return false;
+  return true;
+}
+
+// Any-length r2c along z then complex passes along y (and x): the specialised passes apply only
+// when every axis is a power of two.
+static int rfft3d_any(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro,
+                      bool with_x) {
+  if (!fft_any_shape_ok(n)) return set_error(MP_ERR_UNSUPPORTED, "meshsize (%d, %d, %d) not in 1..2048", n[0], n[1], n[2]);
+  const int ncols = n[2] / 2 + 1;
+  if (pitch < ncols) return set_error(MP_ERR_INVALID, "pitch %lld < Nz/2+1", (long long)pitch);
+  const int64_t nx = n[0], ny = n[1];
+  int rc = any_zpass_r2c(s, in, out, n[2], nx * ny, pitch, pro);
+  if (rc) return rc;
+  rc = any_col_fft(s, out, out, n[1], ncols, nx, pitch, ny * pitch, pitch, ny * pitch, false, 1.f, "fft_ypass_any");
+  if (rc || !with_x) return rc;
+  return any_col_fft(s, out, out, n[0], ncols, ny, ny * pitch, pitch, ny * pitch, pitch, false, 1.f, "fft_xpass_any");
+}
+
 // ------------------------------------------------------------------ internal API
 
 int rfft3d(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro) {
+  if (!pow2_shape(n)) return rfft3d_any(s, in, out, n, pitch, pro, true);
   int rc = check_shape(n);
   if (rc) return rc;
   rc = ensure_twiddles();
@@ -683,6 +708,7 @@ int rfft3d(hipStream_t s, const float* in, float2* out, const int32_t n[3], int6
 }
 
 int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], int64_t pitch, const RealPrologue& pro) {
+  if (!pow2_shape(n)) return rfft3d_any(s, in, out, n, pitch, pro, false);
   int rc = check_shape(n);
   if (rc) return rc;
   rc = ensure_twiddles();
@@ -695,6 +721,17 @@ int rfft3d_zy(hipStream_t s, const float* in, float2* out, const int32_t n[3], i
 }
 
 int ypass_meshes(hipStream_t s, float2* data, int nmesh, int64_t mesh_stride, const int32_t n[3], int64_t pitch) {
+  if (!pow2_shape(n)) {
+    if (!fft_any_shape_ok(n)) return set_error(MP_ERR_UNSUPPORTED, "meshsize (%d, %d, %d) not in 1..2048", n[0], n[1], n[2]);
+    const int ncols = n[2] / 2 + 1;
+    const int64_t ny = n[1];
+    for (int m = 0; m < nmesh; ++m) {
+      float2* d = data + m * mesh_stride;
+      int rc = any_col_fft(s, d, d, n[1], ncols, n[0], pitch, ny * pitch, pitch, ny * pitch, false, 1.f, "fft_ypass_any");
+      if (rc) return rc;
+    }
+    return MP_OK;
+  }
   int rc = check_shape(n);
   if (rc) return rc;
   rc = ensure_twiddles();
@@ -736,6 +773,7 @@ int rfft_zy_ylm(hipStream_t s, const float* in, float2* Z, int64_t mesh_stride, 
 }
 
 int ypass(hipStream_t s, float2* data, const int32_t n[3], int64_t pitch) {
+  if (!pow2_shape(n)) return ypass_meshes(s, data, 1, 0, n, pitch);
   int rc = check_shape(n);
   if (rc) return rc;
   rc = ensure_twiddles();
@@ -754,6 +792,17 @@ int ylm_acc_x(hipStream_t s, const float2* partial, float2* acc, const int32_t n
 
 int irfft3d(hipStream_t s, const float2* in, int64_t in_pitch, float* out, const int32_t n[3], float2* scratch,
             int64_t scratch_pitch) {
+  if (!pow2_shape(n)) {
+    if (!fft_any_shape_ok(n)) return set_error(MP_ERR_UNSUPPORTED, "meshsize (%d, %d, %d) not in 1..2048", n[0], n[1], n[2]);
+    const int ncols = n[2] / 2 + 1;
+    const int64_t nx = n[0], ny = n[1], sp = scratch_pitch;
+    int rc = any_col_fft(s, in, scratch, n[0], ncols, ny, ny * in_pitch, in_pitch, ny * sp, sp, true, 1.f, "fft_xpass_inv_any");
+    if (rc) return rc;
+    rc = any_col_fft(s, scratch, scratch, n[1], ncols, nx, sp, ny * sp, sp, ny * sp, true, 1.f, "fft_ypass_inv_any");
+    if (rc) return rc;
+    const float scale = (float)(1.0 / ((double)n[0] * (double)n[1] * (double)n[2]));
+    return any_zpass_c2r(s, scratch, sp, out, n[2], nx * ny, scale);
+  }
   int rc = check_shape(n);
   if (rc) return rc;
   rc = ensure_twiddles();

@@ -82,7 +82,7 @@ int paint_spectrum(hipStream_t s, const void* pos, int pos_f64, const float* w, 
                    int compensate, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes) {
   const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
   MP_REQUIRE(out_pitch >= n[2] / 2 + 1, "paint_spectrum: out_pitch too small");
-  if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "paint_spectrum: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
+  if (!fft_any_shape_ok(n)) return set_error(MP_ERR_UNSUPPORTED, "paint_spectrum: meshsize (%d, %d, %d) not in 1..2048", n[0], n[1], n[2]);
   const int64_t P = spectrum_pitch(n[2]);
   const int64_t nspec = (int64_t)n[0] * n[1] * P;
   Workspace wsp(ws, ws_bytes);
@@ -242,6 +242,7 @@ MP_EXPORT int mp_bin_poles_partial(void* stream, const void* partial, int64_t pi
 MP_EXPORT int64_t mp_ylm_workspace_bytes(const int32_t meshsize[3]) {
   return (int64_t)meshsize[0] * meshsize[1] * spectrum_pitch(meshsize[2]) * 8 + 1024;
 }
+// (meshes whose sizes are not all powers of two run the x pass in place on the same scratch)
 
 // C_l(k) = sum_m r2c(rho * Y_lm(xhat))(k) * Y_lm(khat), jaxpower/mesh2.py:319-329.
 MP_EXPORT int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t meshsize[3], const double boxsize[3],
@@ -262,12 +263,12 @@ MP_EXPORT int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t 
     pro.m = m;
     pro.ny = meshsize[1];
     pro.nz = meshsize[2];
-    pro.x0 = (float)(boxcenter[0] - boxsize[0] / 2.);
-    pro.y0 = (float)(boxcenter[1] - boxsize[1] / 2.);
-    pro.z0 = (float)(boxcenter[2] - boxsize[2] / 2.);
-    pro.dx = (float)(boxsize[0] / meshsize[0]);
-    pro.dy = (float)(boxsize[1] / meshsize[1]);
-    pro.dz = (float)(boxsize[2] / meshsize[2]);
+    pro.x0 = boxcenter[0] - boxsize[0] / 2.;
+    pro.y0 = boxcenter[1] - boxsize[1] / 2.;
+    pro.z0 = boxcenter[2] - boxsize[2] / 2.;
+    pro.dx = boxsize[0] / meshsize[0];
+    pro.dy = boxsize[1] / meshsize[1];
+    pro.dz = boxsize[2] / meshsize[2];
     int rc = rfft3d_zy(s, rmesh, tmp, meshsize, P, pro);
     if (rc) return rc;
     KPrologue kp{};
@@ -278,7 +279,13 @@ MP_EXPORT int mp_ylm_accumulate(void* stream, const float* rmesh, const int32_t 
     kp.kfx = (float)(2. * M_PI / boxsize[0]);
     kp.kfy = (float)(2. * M_PI / boxsize[1]);
     kp.kfz = (float)(2. * M_PI / boxsize[2]);
-    rc = ylm_acc_x(s, tmp, (float2*)out, meshsize, P, out_pitch, kp, m == -ell);
+    if (pow2_shape(meshsize)) {
+      rc = ylm_acc_x(s, tmp, (float2*)out, meshsize, P, out_pitch, kp, m == -ell);
+    } else {  // x pass in place on the scratch, then the Y_lm(khat) multiply-accumulate
+      const int64_t ny = meshsize[1];
+      rc = any_col_fft(s, tmp, tmp, meshsize[0], meshsize[2] / 2 + 1, ny, ny * P, P, ny * P, P, false, 1.f, "fft_xpass_any");
+      if (!rc) rc = ylm_acc_any(s, tmp, P, (float2*)out, out_pitch, meshsize, kp, m == -ell);
+    }
     if (rc) return rc;
   }
   return MP_OK;

@@ -19,6 +19,7 @@
 #include "fft_core.h"
 #include "sph.h"
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -626,6 +627,53 @@ bool xpass_supported(const int32_t n[3]) {
   return true;
 }
 
+// Meshes whose sizes are not all powers of two: the x FFT of each partial spectrum runs as an
+// any-length pass into a scratch mesh, and this elementwise kernel forms
+// acc (+)= exp(+i s_j (kx+ky+kz)) t / n_i, applying on the last shift the compensation, the
+// zeroed (0,0,0) mode and, for power, |A|^2 (jaxpower/mesh.py:2739-2751).
+__global__ void combine_any_kernel(const float2* __restrict__ t, int64_t tpitch, float2* __restrict__ acc, int64_t apitch, int n0,
+                                   int n1, int ncols, CombineTables tab, int j, float scale, int first, int last, int zero_dc,
+                                   int power) {
+  const int64_t total = (int64_t)n0 * n1 * ncols;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(i % ncols);
+    const int64_t row = i / ncols;
+    const int iy = (int)(row % n1), ix = (int)(row / n1);
+    float2 v = t[row * tpitch + iz];
+    if (tab.phx) {
+      const int64_t o = (int64_t)j * tab.stride;
+      v = cmul(v, cmul(tab.phx[o + ix], cmul(tab.phy[o + iy], tab.phz[o + iz])));
+    }
+    v = cscale(v, scale);
+    float2* d = acc + row * apitch + iz;
+    float2 a = first ? v : cadd(*d, v);
+    if (last) {
+      if (tab.cx) a = cscale(a, tab.cx[ix] * tab.cy[iy] * tab.cz[iz]);
+      if (zero_dc && i == 0) a = make_float2(0.f, 0.f);
+      if (power) a = make_float2(a.x * a.x + a.y * a.y, 0.f);
+    }
+    *d = a;
+  }
+}
+
+// the any-length combine of the n_i partial spectra into acc (pitch apitch); tmp holds one mesh
+static int combine_any(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int ni, const CombineTables& tab,
+                       int zero_dc, int power, float2* acc, int64_t apitch, float2* tmp, int64_t tpitch) {
+  if (!fft_any_shape_ok(n)) return set_error(MP_ERR_UNSUPPORTED, "combine: meshsize (%d, %d, %d) not in 1..2048", n[0], n[1], n[2]);
+  const int ncols = n[2] / 2 + 1;
+  const int64_t ny = n[1], mstride = (int64_t)n[0] * n[1] * pitch;
+  const int64_t total = (int64_t)n[0] * n[1] * ncols;
+  const int64_t blocks = std::min<int64_t>(std::max<int64_t>(ceil_div(total, 256), 1), 65536);
+  for (int j = 0; j < ni; ++j) {
+    int rc = any_col_fft(s, partial + j * mstride, tmp, n[0], ncols, ny, ny * pitch, pitch, ny * tpitch, tpitch, false, 1.f,
+                         "fft_xpass_any");
+    if (rc) return rc;
+    MP_LAUNCH("combine_any", combine_any_kernel, dim3((unsigned)blocks), dim3(256), 0, s, tmp, tpitch, acc, apitch, n[0], n[1],
+              ncols, tab, j, 1.f / (float)ni, j == 0, j == ni - 1, zero_dc, power);
+  }
+  return MP_OK;
+}
+
 static void fill_common(XPassArgs& a, const float2* partial, int64_t pitch, const int32_t n[3], int ni, const CombineTables& tab,
                         int zero_dc) {
   a.in = partial;
@@ -641,20 +689,31 @@ static void fill_common(XPassArgs& a, const float2* partial, int64_t pitch, cons
   a.tw = twiddle_device_ptr();
 }
 
-size_t combine_partial_workspace_bytes(const int32_t n[3], int ni) { return xpass_tables_bytes(n, ni) + 512; }
+static size_t mesh_bytes(const int32_t n[3]) { return align_up((size_t)n[0] * n[1] * spectrum_pitch(n[2]) * sizeof(float2), 256); }
+
+size_t combine_partial_workspace_bytes(const int32_t n[3], int ni) {
+  return xpass_tables_bytes(n, ni) + (xpass_supported(n) ? 0 : mesh_bytes(n)) + 512;
+}
 
 int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                     int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes, int power) {
   MP_REQUIRE(partial && out, "combine: null pointer");
   MP_REQUIRE(ni >= 1, "combine: interlacing %d < 1", ni);
-  if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "combine: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
   MP_REQUIRE(pitch >= n[2] / 2 + 1 && out_pitch >= n[2] / 2 + 1, "combine: pitch < Nz/2+1");
+  const bool any = !xpass_supported(n);
+  if (any && power == 2) return set_error(MP_ERR_UNSUPPORTED, "combine: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
   int rc = ensure_twiddles();
   if (rc) return rc;
   Workspace wsp(ws, ws_bytes);
   CombineTables tab;
   rc = combine_tables(s, n, order, ni, compensate, wsp, tab);
   if (rc) return rc;
+  if (any) {
+    const int64_t P = spectrum_pitch(n[2]);
+    float2* tmp = wsp.take<float2>((size_t)n[0] * n[1] * P);
+    if (!tmp) return set_error(MP_ERR_WORKSPACE, "combine: workspace of %zu bytes < %zu", ws_bytes, combine_partial_workspace_bytes(n, ni));
+    return combine_any(s, partial, pitch, n, ni, tab, zero_dc, power, out, out_pitch, tmp, P);
+  }
   XPassArgs a{};
   fill_common(a, partial, pitch, n, ni, tab, zero_dc);
   if (!a.tw) return set_error(MP_ERR_HIP, "combine: twiddle table address");
@@ -668,6 +727,8 @@ int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const i
 constexpr int XB_MAX_BLOCKS = 1024;  // partial histograms: the bin-mode grid is capped here
 
 size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges) {
+  if (!xpass_supported(n))  // combined mesh and one scratch mesh, then the plain binning
+    return xpass_tables_bytes(n, ni) + 2 * mesh_bytes(n) + bin_workspace_bytes(nprod, nedges) + 1024;
   const size_t blocks = XB_MAX_BLOCKS;
   return xpass_tables_bytes(n, ni) + align_up(blocks * nprod * (nedges - 1) * 2 * sizeof(double), 256) + 512;
 }
@@ -678,7 +739,9 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   MP_REQUIRE(partial && out && ell && remove_zero, "bin_partial: null pointer");
   MP_REQUIRE(ni >= 1, "bin_partial: interlacing %d < 1", ni);
   MP_REQUIRE(nprod >= 1 && nprod <= XB_MAX_PROD, "bin_partial: %d products not in 1..%d", nprod, XB_MAX_PROD);
-  if (!xpass_supported(n)) return set_error(MP_ERR_UNSUPPORTED, "bin_partial: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
+  const bool any = !xpass_supported(n);
+  if (any && (linear || ylm_m))
+    return set_error(MP_ERR_UNSUPPORTED, "bin_partial: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
   if (nprod * (g.nedges - 1) > BIN_LDS_DOUBLES) return set_error(MP_ERR_UNSUPPORTED, "bin_partial: too many bins for the LDS histogram");
   MP_REQUIRE(pitch >= n[2] / 2 + 1, "bin_partial: pitch < Nz/2+1");
   int rc = ensure_twiddles();
@@ -687,6 +750,34 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   CombineTables tab;
   rc = combine_tables(s, n, order, ni, compensate, wsp, tab);
   if (rc) return rc;
+  if (any) {  // combine into a stored mesh, then bin |A|^2 L_ell(mu) as mp_bin_poles does
+    const int64_t P = spectrum_pitch(n[2]);
+    float2* acc = wsp.take<float2>((size_t)n[0] * n[1] * P);
+    float2* tmp = wsp.take<float2>((size_t)n[0] * n[1] * P);
+    const size_t bws = bin_workspace_bytes(nprod, g.nedges);
+    void* bw = wsp.take<char>(bws);
+    if (!acc || !tmp || !bw)
+      return set_error(MP_ERR_WORKSPACE, "bin_partial: workspace of %zu bytes < %zu", ws_bytes,
+                       bin_partial_workspace_bytes(n, ni, nprod, g.nedges));
+    rc = combine_any(s, partial, pitch, n, ni, tab, zero_dc, 0, acc, P, tmp, P);
+    if (rc) return rc;
+    BinProducts prods{};
+    prods.nmesh = 1;
+    prods.nprod = nprod;
+    prods.mesh[0] = acc;
+    prods.pitch[0] = P;
+    prods.need_mu = 0;
+    for (int p = 0; p < nprod; ++p) {
+      MP_REQUIRE(ell[p] >= 0 && ell[p] <= 10, "bin_partial: ell %d not in 0..10", ell[p]);
+      prods.pi[p] = 0;
+      prods.pj[p] = 0;
+      prods.ell[p] = ell[p];
+      prods.rz[p] = remove_zero[p];
+      if (ell[p]) prods.need_mu = 1;
+    }
+    for (int q = 0; q < 3; ++q) prods.los[q] = los ? (float)los[q] : 0.f;
+    return bin_poles(s, g, prods, out, bw, bws);
+  }
   const size_t per_block = (size_t)nprod * (g.nedges - 1) * 2;
   size_t pd = (size_t)XB_MAX_BLOCKS * per_block;
   if (fit_partials) {  // a fixed workspace: as many per-block histograms as it holds (fewer blocks)
@@ -763,12 +854,12 @@ int ylm_bin(hipStream_t s, const float* rmesh, const float2* A0, int64_t a0_pitc
   pro.ell = ell;
   pro.ny = n[1];
   pro.nz = n[2];
-  pro.x0 = (float)(boxcenter[0] - boxsize[0] / 2.);
-  pro.y0 = (float)(boxcenter[1] - boxsize[1] / 2.);
-  pro.z0 = (float)(boxcenter[2] - boxsize[2] / 2.);
-  pro.dx = (float)(boxsize[0] / n[0]);
-  pro.dy = (float)(boxsize[1] / n[1]);
-  pro.dz = (float)(boxsize[2] / n[2]);
+  pro.x0 = boxcenter[0] - boxsize[0] / 2.;
+  pro.y0 = boxcenter[1] - boxsize[1] / 2.;
+  pro.z0 = boxcenter[2] - boxsize[2] / 2.;
+  pro.dx = boxsize[0] / n[0];
+  pro.dy = boxsize[1] / n[1];
+  pro.dz = boxsize[2] / n[2];
   int rc = rfft_zy_ylm(s, rmesh, Z, nspec, n, P, pro);
   if (rc) return rc;
   YlmBinArgs a{};
