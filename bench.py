@@ -48,10 +48,11 @@ def parse():
                    help='write the HBM traffic file from two rocprofv3 --pmc output folders and exit')
     p.add_argument('--summarize-trace', metavar='TRACE_DIR',
                    help='write the kernel statistics summary from a rocprofv3 --kernel-trace --stats folder and exit')
-    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024', 'batch256', 'corr512', 'window256'],
+    p.add_argument('--workload', default='box512', choices=['box512', 'fkp1024', 'batch256', 'corr512', 'window256', 'grad256'],
                    help='box512: the headline plain box; fkp1024: survey FKP field, 1024^3, local line of sight; '
                         'batch256: 64 lognormal mocks at 256^3 with uneven counts, batched (HIP graph) vs per-mock calls; '
-                        'corr512: correlation multipoles of the 512^3 mock; window256: window matrix of a survey selection at 256^3')
+                        'corr512: correlation multipoles of the 512^3 mock; window256: window matrix of a survey selection at 256^3; '
+                        'grad256: forward + hand-written backward of a multipole loss at 256^3 with 2e6 particles')
     p.add_argument('--paint-path', type=int, default=None,
                    help='0 automatic, 1 tiled paint + z pass, 2 fused paint + z-FFT, 3 atomics + z pass')
     return p.parse_args()
@@ -534,6 +535,107 @@ def run_batch256(args):
     print(json.dumps(result))
 
 
+def run_grad256(args):
+    """The differentiated estimator: 256^3, boxsize 1000, 2e6 lognormal particles (the 512^3
+    recipe), TSC with 3-fold interlacing, compensated, ells 0, 2, k step 0.005; loss =
+    sum over ell and k of (P_ell - T_ell)^2 / T_ell^2, P the normalized, shot-noise-subtracted
+    multipoles and T those of a second mock (seed + 1).  One step = forward + hand-written
+    backward (d loss / d positions and weights).  The gradient is checked against a central
+    difference of the float64 oracle along a random displacement of every particle (and a
+    random weight perturbation).  Prints one JSON line."""
+    import numpy as np
+    import torch
+    from jaxpower_amd import (MeshAttrs, BinMesh2SpectrumPoles, generate_lognormal_particles, mesh2_spectrum_poles,
+                              box_spectrum_values)
+    torch.cuda.set_device(0)
+    n, boxsize, size, ells = 256, 1000., 2_000_000, (0, 2)
+    attrs = MeshAttrs(meshsize=n, boxsize=boxsize, boxcenter=boxsize / 2.)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': STEP}, ells=ells)
+    part = generate_lognormal_particles(attrs, size=size, seed=args.seed)
+    tpart = generate_lognormal_particles(attrs, size=size, seed=args.seed + 1)
+    kw = dict(attrs=attrs, bin=bin, resampler='tsc', interlacing=NI, compensate=True, los='z')
+    pos0 = part.positions
+    w0 = torch.ones(pos0.shape[0], dtype=torch.float32, device=pos0.device)
+    with torch.no_grad():
+        tw = torch.ones(tpart.positions.shape[0], dtype=torch.float32, device=pos0.device)
+        target = box_spectrum_values(mesh2_spectrum_poles(tpart.positions, tw, **kw), tw, attrs, ells)
+    ok = target != 0.
+    torch.cuda.synchronize()
+    print(f'# grad setup done: {pos0.shape[0]} particles, {bin.nbins} bins', file=sys.stderr, flush=True)
+
+    def step():
+        pos = pos0.detach().clone().requires_grad_(True)
+        w = w0.detach().clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        values = box_spectrum_values(mesh2_spectrum_poles(pos, w, **kw), w, attrs, ells)
+        loss = torch.sum(((values - target)[ok] / target[ok])**2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return float(loss.detach()), pos.grad, w.grad, t1 - t0, t2 - t1
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.reset_peak_memory_stats()
+    fw, bw = [], []
+    for _ in range(args.steps):
+        loss, gpos, gw, tf, tb = step()
+        fw.append(tf)
+        bw.append(tb)
+    peak = torch.cuda.max_memory_allocated()
+    ms_f, ms_b = 1e3 * float(np.median(fw)), 1e3 * float(np.median(bw))
+    result = {'metric': 'loss-gradient evaluations/s (256^3, 2e6 particles, TSC x3)', 'value': round(1e3 / (ms_f + ms_b), 3),
+              'unit': 'evaluations/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_f + ms_b, 3),
+              'ms_forward': round(ms_f, 3), 'ms_backward': round(ms_b, 3), 'higher_is_better': True, 'scaling': 'weak',
+              'vs_baseline': None, 'dtype': 'f32 (f64 bin sums)',
+              'data': 'synthetic: seeded lognormal Poisson mocks (target from seed + 1)',
+              'config': {'workload': 'differentiated estimator', 'meshsize': n, 'boxsize': boxsize, 'particles': int(pos0.shape[0]),
+                         'resampler': 'tsc', 'interlacing': NI, 'compensate': True, 'ells': list(ells), 'kstep': STEP,
+                         'loss': 'sum_ell,k (P_ell - T_ell)^2 / T_ell^2'},
+              'memory': {'peak_bytes_256': int(peak),
+                         'plan_1024': 'the backward recomputes the painted spectrum (one interlaced paint + FFT) instead of '
+                                      'storing it; peak ~ n_i half spectra (partials, then Y) + n_i real meshes (D) + one '
+                                      'half spectrum: 3 x 4.3 + 3 x 4.3 + 4.3 GB ~ 30 GB at 1024^3'},
+              'loss': loss}
+    if not args.no_cpu_baseline:  # directional check against the float64 oracle (the same particles)
+        from oracle import power_oracle as po
+        po.FFT_WORKERS = len(os.sched_getaffinity(0))
+        oattrs = po.Attrs(n, boxsize, boxsize / 2.)
+        obin = po.BinPoles(oattrs, edges={'step': STEP}, ells=ells)
+        tgt = target.cpu().numpy()
+        okn = tgt != 0.
+        pos_h = pos0.cpu().numpy().astype('f8')
+        w_h = np.ones(len(pos_h))
+
+        def oloss(p, ww):
+            mesh = po.paint(oattrs, p, ww, resampler='tsc', interlacing=NI, compensate=True, out='complex',
+                            threads=po.FFT_WORKERS)
+            mesh[0, 0, 0] = 0.
+            num = np.array(po.spectrum_poles(oattrs, obin, mesh, los='z')['num_raw'])
+            norm, sn = po.box_normalization(oattrs, ww), po.fkp_shotnoise(ww)
+            vals = np.array([po.pole_value(num[i], norm, sn if ell == 0 else 0.) for i, ell in enumerate(ells)])
+            return float(np.sum(((vals - tgt)[okn] / tgt[okn])**2))
+
+        rng = np.random.default_rng(args.seed + 7)
+        v = rng.normal(size=pos_h.shape)
+        u = rng.normal(size=w_h.shape)
+        t0 = time.perf_counter()
+        h, hw = 1e-3, 1e-5
+        fdpos = (oloss(pos_h + h * v, w_h) - oloss(pos_h - h * v, w_h)) / (2 * h)
+        fdw = (oloss(pos_h, w_h + hw * u) - oloss(pos_h, w_h - hw * u)) / (2 * hw)
+        dt = time.perf_counter() - t0
+        dpos = float(np.sum(gpos.cpu().numpy().astype('f8') * v))
+        dw = float(np.sum(gw.cpu().numpy().astype('f8') * u))
+        result['gradient_check'] = {'against': 'central differences of the float64 oracle loss along a random displacement v '
+                                               '(h = 1e-3 Mpc/h) and a random weight perturbation u (h = 1e-5)',
+                                    'grad_dot_v': dpos, 'fd_v': fdpos, 'rel_err_v': abs(dpos - fdpos) / abs(fdpos),
+                                    'grad_dot_u': dw, 'fd_u': fdw, 'rel_err_u': abs(dw - fdw) / abs(fdw), 'oracle_s': round(dt, 1)}
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.summarize_trace:
@@ -559,6 +661,12 @@ def main():
             print('bench.py runs on one GPU only.', file=sys.stderr)
             sys.exit(2)
         run_window256(args)
+        return
+    if args.workload == 'grad256':
+        if args.gpus != 1:
+            print('bench.py runs on one GPU only.', file=sys.stderr)
+            sys.exit(2)
+        run_grad256(args)
         return
     if args.workload == 'fkp1024':
         if args.gpus != 1:

@@ -10,8 +10,8 @@ derivatives of the mass-assignment kernel (mp_read_grad).  The backward recomput
 spectrum instead of keeping it, so a step holds one complex mesh at a time.
 
 Derivatives of round/floor are zero, as under jax.grad.  Normalization and shot noise are
-functions of the weights alone; compose them with torch operations on the weights (see
-``spectrum_loss`` below) and autograd handles them.
+functions of the weights alone; ``box_spectrum_values`` composes them with torch operations on
+the weights, so autograd handles them.
 """
 import numpy as np
 import torch
@@ -20,7 +20,7 @@ from . import _lib, resamplers
 from .mesh import MeshAttrs, _paint_partial, _spectrum_pitch
 from .mesh2 import _format_los
 
-__all__ = ['mesh2_spectrum_poles']
+__all__ = ['mesh2_spectrum_poles', 'box_spectrum_values']
 
 
 def _ell_scales(bin, device):
@@ -109,3 +109,18 @@ def mesh2_spectrum_poles(positions, weights=None, attrs=None, bin=None, resample
         weights = torch.ones(positions.shape[0], dtype=torch.float32, device=positions.device)
     return _SpectrumPoles.apply(positions, weights, attrs, bin, order, ni, bool(compensate), tuple(float(v) for v in vlos),
                                 bool(remove_dc))
+
+
+def box_spectrum_values(num, weights, attrs, ells, shotnoise=True):
+    """The multipoles P_ell = (num_raw - SN [ell = 0]) / norm of a periodic box, as the box
+    harness of the reference forms them (scripts/abacus_hf.py:357-373): norm the box
+    normalization (sum w)^2 / (N^3 V_cell) (compute_box2_normalization, mesh.py:3121-3129) and
+    SN = sum w^2 (compute_fkp2_shotnoise, mesh2.py:577-621), here torch functions of the weights,
+    so that gradients flow through them.  num: the (len(ells), nbins) output of
+    mesh2_spectrum_poles."""
+    w = weights.to(torch.float64)
+    ncell = float(np.prod(attrs.meshsize))
+    vcell = float(np.prod(attrs.cellsize))
+    norm = w.sum()**2 / (ncell * vcell)
+    sn = (w * w).sum() if shotnoise else None
+    return torch.stack([((num[i] - sn) if (ell == 0 and shotnoise) else num[i]) / norm for i, ell in enumerate(ells)])

@@ -35,10 +35,29 @@ def _oracle_num(oattrs, obin, pos, w, resampler, interlacing, compensate, los):
     return np.asarray(po.spectrum_poles(oattrs, obin, mesh, los=los)['num_raw'])
 
 
-@pytest.mark.parametrize('resampler,interlacing,compensate,los', [('tsc', 2, True, 'z'), ('cic', 1, False, 'x'),
-                                                                  ('pcs', 3, True, (0.6, 0., 0.8))])
+def _oracle_values(oattrs, obin, pos, w, resampler, interlacing, compensate, los):
+    """P_ell = (num_raw - SN [ell = 0]) / norm with the box normalization and shot noise."""
+    num = _oracle_num(oattrs, obin, pos, w, resampler, interlacing, compensate, los)
+    norm = po.box_normalization(oattrs, w)
+    sn = po.fkp_shotnoise(w)
+    return np.array([po.pole_value(num[i], norm, sn if ell == 0 else 0.) for i, ell in enumerate(obin.ells)])
+
+
+def _chi2(values, target):
+    """sum over ell and k of (P_ell(k) - T_ell(k))^2 / T_ell(k)^2 over the bins that hold modes."""
+    ok = target != 0.
+    return np.sum(((values - target)[ok] / target[ok])**2)
+
+
+@pytest.mark.parametrize('resampler,interlacing,compensate,los', [('tsc', 3, True, 'z'), ('cic', 1, False, 'x'),
+                                                                  ('pcs', 2, True, (0.6, 0., 0.8))])
 def test_gradient_matches_finite_differences(resampler, interlacing, compensate, los):
-    from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, mesh2_spectrum_poles
+    """The loss of the differentiated-estimator workload, sum over ell = 0, 2 and k of
+    (P_ell - T_ell)^2 / T_ell^2 with P the normalized, shot-noise-subtracted multipoles and T
+    those of a second mock, differentiated by the hand-written backward (box normalization and
+    shot noise through autograd) against central finite differences of the float64 oracle, with
+    position steps of 1e-3 Mpc/h."""
+    from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, mesh2_spectrum_poles, box_spectrum_values
     say('grad', resampler, interlacing, compensate, los)
     attrs = MeshAttrs(meshsize=16, boxsize=400., boxcenter=0.)
     ells = (0, 2)
@@ -48,24 +67,26 @@ def test_gradient_matches_finite_differences(resampler, interlacing, compensate,
     obin = po.BinPoles(oattrs, edges={'step': step}, ells=ells)
     pos, w = _mock(attrs, 400, 1)
     tpos, tw = _mock(attrs, 400, 2)
-    target = _oracle_num(oattrs, obin, tpos, tw, resampler, interlacing, compensate, los)
-    denom = np.abs(target[0]) + 1e-30  # the monopole target scales every multipole
+    target = _oracle_values(oattrs, obin, tpos, tw, resampler, interlacing, compensate, los)
+    w64 = w.astype('f4').astype('f8')
 
     def loss_np(p, ww):
-        num = _oracle_num(oattrs, obin, p, ww, resampler, interlacing, compensate, los)
-        return float(np.sum(((num - target) / denom)**2))
+        return float(_chi2(_oracle_values(oattrs, obin, p, ww, resampler, interlacing, compensate, los), target))
 
     pos_t = to_dev(pos).requires_grad_(True)
     w_t = to_dev(w.astype('f4')).requires_grad_(True)
     num = mesh2_spectrum_poles(pos_t, w_t, attrs=attrs, bin=bin, resampler=resampler, interlacing=interlacing,
                                compensate=compensate, los=los)
-    assert_poles_close(host(num), _oracle_num(oattrs, obin, pos, w.astype('f4').astype('f8'), resampler, interlacing,
-                                              compensate, los), label='forward')
-    loss = torch.sum(((num - to_dev(target)) / to_dev(denom))**2)
+    assert_poles_close(host(num), _oracle_num(oattrs, obin, pos, w64, resampler, interlacing, compensate, los), label='forward')
+    values = box_spectrum_values(num, w_t, attrs, ells)
+    assert np.allclose(host(values), _oracle_values(oattrs, obin, pos, w64, resampler, interlacing, compensate, los), rtol=1e-5,
+                       atol=1e-5 * np.abs(target).max())
+    tt = to_dev(target)
+    ok = tt != 0.
+    loss = torch.sum(((values - tt)[ok] / tt[ok])**2)
+    assert np.isclose(float(loss), loss_np(pos, w64), rtol=1e-4)
     loss.backward()
     gpos, gw = host(pos_t.grad), host(w_t.grad)
-    # central differences on a few particles (clumped and field ones), h = 1e-3 Mpc/h
-    w64 = w.astype('f4').astype('f8')
     h, hw = 1e-3, 1e-4
     got, fd = [], []
     for i in (0, 1, 7, 250, 399):
@@ -82,7 +103,47 @@ def test_gradient_matches_finite_differences(resampler, interlacing, compensate,
         got.append(gw[i])
     got, fd = np.array(got), np.array(fd)
     err = np.abs(got - fd).max() / np.abs(fd).max()
+    print('max |g - g_fd| / max |g_fd|', err)
     assert err < 1e-4, (err, got, fd)
+
+
+def test_gradient_directional_derivative_64():
+    """At 64^3 with 5e4 lognormal-like particles, TSC x3: the gradient contracted with a random
+    displacement field v (all particles at once) and a random weight perturbation against the
+    central difference of the float64 oracle loss along them."""
+    from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, mesh2_spectrum_poles, box_spectrum_values
+    say('grad directional 64^3')
+    attrs = MeshAttrs(meshsize=64, boxsize=500., boxcenter=250.)
+    ells = (0, 2)
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': 0.02}, ells=ells)
+    oattrs = po.Attrs(64, 500., 250.)
+    obin = po.BinPoles(oattrs, edges={'step': 0.02}, ells=ells)
+    pos, w = _mock(attrs, 50000, 11)
+    tpos, tw = _mock(attrs, 50000, 12)
+    w64 = w.astype('f4').astype('f8')
+    target = _oracle_values(oattrs, obin, tpos, tw, 'tsc', 3, True, 'z')
+    rng = np.random.default_rng(13)
+    v = rng.normal(size=pos.shape)
+    u = rng.normal(size=w.shape)
+    pos_t = to_dev(pos).requires_grad_(True)
+    w_t = to_dev(w.astype('f4')).requires_grad_(True)
+    num = mesh2_spectrum_poles(pos_t, w_t, attrs=attrs, bin=bin, resampler='tsc', interlacing=3, compensate=True, los='z')
+    values = box_spectrum_values(num, w_t, attrs, ells)
+    tt = to_dev(target)
+    ok = tt != 0.
+    torch.sum(((values - tt)[ok] / tt[ok])**2).backward()
+    dpos = float(np.sum(host(pos_t.grad).astype('f8') * v))
+    dw = float(np.sum(host(w_t.grad).astype('f8') * u))
+
+    def loss_np(p, ww):
+        return float(_chi2(_oracle_values(oattrs, obin, p, ww, 'tsc', 3, True, 'z'), target))
+
+    h, hw = 1e-3, 1e-5
+    fdpos = (loss_np(pos + h * v, w64) - loss_np(pos - h * v, w64)) / (2 * h)
+    fdw = (loss_np(pos, w64 + hw * u) - loss_np(pos, w64 - hw * u)) / (2 * hw)
+    print('directional', dpos, fdpos, dw, fdw)
+    assert abs(dpos - fdpos) < 1e-4 * abs(fdpos)
+    assert abs(dw - fdw) < 1e-4 * abs(fdw)
 
 
 def test_gradient_linear_in_upstream_and_zero_for_odd_ells():
