@@ -654,6 +654,9 @@ class RealMeshField(BaseMeshField):
         if compensate:
             kernel = resampler.compensate if isinstance(compensate, bool) else compensate
             mesh = mesh.r2c().apply(kernel).c2r()
+        if not resamplers.is_builtin(resampler):  # the object's own gather (mesh.py:2065-2115)
+            pos = _as_device_tensor(positions, device=mesh.value.device)
+            return resampler.read(mesh.value, _grid_positions(mesh.attrs, pos))
         return _read(mesh, positions, resampler.order)
 
 
@@ -909,9 +912,60 @@ def _paint_partial(attrs, positions, weights, order, ni, compensate):
     return _PendingCombine(partial, pitch, order, ni, compensate)
 
 
+def _grid_positions(attrs, positions):
+    """(x + L/2 - c) / cellsize in the positions' dtype (mesh.py:2684)."""
+    dtype = positions.dtype if positions.dtype in (torch.float32, torch.float64) else torch.float32
+    shift = torch.as_tensor(attrs.boxsize / 2. - attrs.boxcenter, dtype=torch.float64, device=positions.device)
+    cell = torch.as_tensor(attrs.cellsize, dtype=torch.float64, device=positions.device)
+    return ((positions.to(torch.float64) + shift) / cell).to(dtype)
+
+
+def _paint_custom(attrs, positions, weights, resampler, interlacing, compensate, out):
+    """_paint (mesh.py:2673-2754) with a resampler object other than the four built-in ones
+    (the plugin point of resamplers.get_resampler, resamplers.py:134-144): its ``.paint(mesh,
+    positions, weights)`` scatters each shifted copy of the grid-unit positions (device tensors)
+    into a zero mesh; the r2c, the interlacing phases exp(+i s sum k_circ) / n_i, the sum, the
+    compensation (its ``.compensate`` kernel, or the given callable) and the c2r follow the
+    reference's order."""
+    device = positions.device
+    shape = tuple(int(n) for n in attrs.meshsize)
+    xg = _grid_positions(attrs, positions)
+    ni = max(int(interlacing), 1)
+    if isinstance(compensate, bool):
+        kernel_compensate = resampler.compensate if compensate else None
+    else:
+        kernel_compensate = compensate
+
+    def painted(shift):
+        value = resampler.paint(torch.zeros(shape, dtype=torch.float32, device=device), xg + shift if shift else xg, weights)
+        return RealMeshField(torch.as_tensor(value, dtype=torch.float32, device=device), attrs=attrs)
+
+    if ni <= 1:
+        mesh = painted(0.)
+        if kernel_compensate is not None:
+            mesh = mesh.r2c().apply(kernel_compensate)
+            return mesh.c2r() if out == 'real' else mesh
+        return mesh if out == 'real' else mesh.r2c()
+    acc = None
+    for j in range(ni):
+        s = j / ni
+
+        def phase(value, kvec, s=s):
+            return value * torch.exp(1j * s * sum(kvec)).to(value.dtype) / ni
+
+        term = painted(s).r2c().apply(phase, kind='circular', sparse=True)
+        acc = term if acc is None else acc + term
+    if kernel_compensate is not None:
+        acc = acc.apply(kernel_compensate)
+    return acc.c2r() if out == 'real' else acc
+
+
 def _paint(attrs, positions, weights=None, resampler='cic', interlacing=0, compensate=False, out='real'):
     """_paint, mesh.py:2673-2754, single device."""
     resampler = resamplers.get_resampler(resampler)
+    if not resamplers.is_builtin(resampler):
+        _lib.require_device(positions, 'positions')
+        return _paint_custom(attrs, positions, weights, resampler, interlacing, compensate, out)
     order = int(resampler.order)
     ni = max(int(interlacing), 1)
     device = positions.device

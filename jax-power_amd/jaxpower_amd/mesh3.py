@@ -92,7 +92,8 @@ class BinMesh3SpectrumPoles:
         if basis not in _BASES:
             raise ValueError(f'basis must be one of {_BASES}, got {basis!r}')
         if 'scoccimarro' not in basis:
-            raise NotImplementedError('only the scoccimarro basis is implemented on this path')
+            raise NotImplementedError(f"basis={basis!r} (the reference's default is 'sugiyama') is not implemented on this "
+                                      "path: pass basis='scoccimarro' or 'scoccimarro-diagonal'")
         device = device or _default_device()
         ndim = 3
         diagonal = 'diagonal' in basis

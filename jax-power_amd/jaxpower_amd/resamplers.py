@@ -109,6 +109,12 @@ for _i, _name in enumerate(['ngp', 'cic', 'tsc', 'pcs']):
                                             order=_order))
 
 
+def is_builtin(resampler):
+    """True for the four namespaces above, whose paint, read and compensation run in the fused
+    HIP kernels; any other object's own ``.paint`` / ``.read`` / ``.compensate`` are called."""
+    return any(resampler is globals()[name] for name in ('ngp', 'cic', 'tsc', 'pcs'))
+
+
 def get_resampler(resampler):
     """Name -> namespace; objects with ``.order`` pass through (resamplers.py:134-144)."""
     if isinstance(resampler, str):
