@@ -67,23 +67,6 @@ __host__ __device__ __forceinline__ float kern1(float s) {
   }
 }
 
-__device__ __forceinline__ int gcd_u32(unsigned a, unsigned b) {
-  while (b) {
-    unsigned t = a % b;
-    a = b;
-    b = t;
-  }
-  return (int)a;
-}
-
-// An odd stride near 0.618 * total that is coprime with total (1 for total <= 2).
-__device__ __forceinline__ int coprime_stride(int total) {
-  if (total <= 2) return 1;
-  unsigned s = ((unsigned)((double)total * 0.6180339887498949)) | 1u;
-  while (gcd_u32(s, (unsigned)total) != 1) s += 2;
-  return (int)(s % (unsigned)total);
-}
-
 __device__ __forceinline__ int pmod(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
@@ -497,7 +480,17 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   const float bound = sbound[0];
   const float scale = bound > 0.f ? 2126008811.f / bound : 1.f;
   int* irows = reinterpret_cast<int*>(rows);
-  for (int c = tid; c < total; c += PZ_NT) {
+  // Work items are taken from G regions of the chunk list in turn (lane l of a wave works in
+  // region l mod G), so the lanes of one wave hold chunks of different home rows: the particles
+  // of a dense clump, which share stencil cells, then go to different instructions instead of
+  // serialising one LDS atomic over many lanes at the same address (measured on the 512^3
+  // lognormal mock: 1.41 -> 1.19 ms; a coprime stride over single chunks, which also breaks the
+  // coalescing of the record loads, gave 1.22 ms).
+  constexpr int G = 16;
+  const int Rg = (total + G - 1) / G;
+  for (int ci = tid; ci < G * Rg; ci += PZ_NT) {
+    const int c = (ci % G) * Rg + ci / G;
+    if (c >= total) continue;
     int lo_i = 0, hi_i = nseg - 1;
     while (lo_i < hi_i) {
       int mid = (lo_i + hi_i + 1) >> 1;
@@ -663,7 +656,11 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   const float bound = sbound[0];
   const float scale = bound > 0.f ? 2126008811.f / bound : 1.f;
   const int rxmax = min(RX, rg.n[0] - x0), rymax = min(RY, rg.n[1] - y0);  // partial blocks at the edge
-  for (int c = tid; c < total; c += PR_NT) {
+  constexpr int G = 16;  // chunks taken from G regions in turn, as in the fused paint
+  const int Rg = (total + G - 1) / G;
+  for (int ci = tid; ci < G * Rg; ci += PR_NT) {
+    const int c = (ci % G) * Rg + ci / G;
+    if (c >= total) continue;
     int lo_i = 0, hi_i = nseg - 1;
     while (lo_i < hi_i) {
       const int mid = (lo_i + hi_i + 1) >> 1;
