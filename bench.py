@@ -213,7 +213,8 @@ def summarize_trace(folder, out=TRACE_FILE, workload='box512'):
         command = WINDOW_TRACE_COMMAND
     else:
         alg, symbols = algorithmic_bytes(), KERNEL_SYMBOLS
-        title = '# rocprofv3 kernel statistics of the 512^3 TSCx3 step (13 steps: 3 warm-up + 10 timed)'
+        title = ('# rocprofv3 kernel statistics of the 512^3 TSCx3 step (13 eager steps: 3 warm-up + 10 timed, then the same step '
+                 'captured as a HIP graph and replayed 13 times; the paint lines include the autotuner\'s timing launches)')
         command = TRACE_COMMAND
     lines = [title, '# made by: ' + command, f'#          python bench.py --workload {workload} --summarize-trace {folder}',
              f'# {"kernel":58s} {"calls":>6s} {"avg_us":>10s} {"total_ms":>10s} {"share":>6s} {"alg_MB":>9s} {"alg_GB/s":>9s}']
