@@ -149,36 +149,45 @@ int mp_irfft3d(void* stream, const void* in, int64_t in_pitch, float* out, const
 
 /* ---- binning ---------------------------------------------------------------------- */
 /* _get_bin_attrs with hermitian weights (jaxpower/mesh.py:2812-2863): nmodes[b] (exact
- * integers) and ksum[b] = sum w |k| for bins b = 0..nedges-2.  kx, ky, kz: device float64
- * wavenumbers per index; k2_thresholds[e] = min{t : fl(sqrt(t)) >= edge[e]}. */
+ * integers) and ksum[b] = sum w |k + shift| for bins b = 0..nedges-2.  kx, ky, kz: device
+ * float64 wavenumbers per index; k2_thresholds[e] = min{t : fl(sqrt(t)) >= edge[e]};
+ * mode_shift: host (sx, sy, sz) added to the wavenumbers before the norm that picks the bin,
+ * one shift of the mode-oversampling loop of _make_edges2 (jaxpower/mesh2.py:48-61), or NULL
+ * for none.  The caller sums the calls over the shifts. */
 int mp_bin_modes(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
-                 const double* k2_thresholds, int nedges, int64_t* nmodes, double* ksum);
+                 const double* k2_thresholds, int nedges, const double mode_shift[3], int64_t* nmodes, double* ksum);
 /* BinMesh2SpectrumPoles.__call__ over the products of compute_mesh2_spectrum
  * (jaxpower/mesh2.py:100-119, 304-333, 351-369): for product p,
  *   out[(p*nbins + b)*2 + {0,1}] = {Re, Im} sum_{modes in b} w conj(M_i) M_j P_ell(mu)
  * with mu = k.los/|k|, P_0 = 1, the (0,0,0) mode skipped when prod_remove_zero[p];
- * prod_i[p] = -1 stands for the constant 1 (plain binning of M_j). */
+ * prod_i[p] = -1 stands for the constant 1 (plain binning of M_j).  mode_shift as for
+ * mp_bin_modes: the bin of a mode is that of k + shift, mu and w those of k (one term of
+ * _bincount over the ibin list, jaxpower/mesh.py:2874-2910). */
 int64_t mp_bin_workspace_bytes(int nprod, int nedges);
 int mp_bin_poles(void* stream, int nmesh, const void* const* meshes, const int64_t* pitches,
                  const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
-                 const double* k2_thresholds, int nedges, const double los[3], int nprod, const int32_t* prod_i,
+                 const double* k2_thresholds, int nedges, const double mode_shift[3], const double los[3], int nprod,
+                 const int32_t* prod_i,
                  const int32_t* prod_j, const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out,
                  void* workspace, size_t workspace_bytes);
 
 /* ---- correlation function ---------------------------------------------------------- */
 /* _make_edges2(kind='real') -> _get_bin_attrs without weights (jaxpower/mesh2.py:25-62,
- * 120-158; jaxpower/mesh.py:2831-2863): nmodes[b] (exact cell counts) and ssum[b] = sum |s|
+ * 120-158; jaxpower/mesh.py:2831-2863): nmodes[b] (exact cell counts) and ssum[b] = sum |s + shift|
  * for bins b = 0..nedges-2 over the full real mesh.  sx, sy, sz: device float64 separations
- * fftfreq(N) * cellsize * N per index; s2_thresholds[e] = min{t : fl(sqrt(t)) >= edge[e]}. */
+ * fftfreq(N) * cellsize * N per index; s2_thresholds[e] = min{t : fl(sqrt(t)) >= edge[e]};
+ * mode_shift: host shift of the mode-oversampling loop (jaxpower/mesh2.py:48-61) or NULL. */
 int mp_bin_real_modes(void* stream, const int32_t meshsize[3], const double* sx, const double* sy, const double* sz,
-                      const double* s2_thresholds, int nedges, int64_t* nmodes, double* ssum);
+                      const double* s2_thresholds, int nedges, const double mode_shift[3], int64_t* nmodes, double* ssum);
 /* BinMesh2CorrelationPoles.__call__ with the Legendre weights of compute_mesh2_correlation
  * (jaxpower/mesh2.py:175-184, 490-498): out[e * nbins + b] = sum over the cells of bin b of
  * mesh * P_ell[e](mu_s), mu_s = s.los/|s| (0 at s = 0), the (0,0,0) cell skipped when
- * remove_zero[e]; FP64 sums, at most 6 ells. */
+ * remove_zero[e]; FP64 sums, at most 6 ells.  mode_shift as for mp_bin_real_modes (the bin
+ * of s + shift, mu of s). */
 int64_t mp_bin_real_workspace_bytes(const int32_t meshsize[3], int nell, int nedges);
 int mp_bin_real_poles(void* stream, const float* mesh, const int32_t meshsize[3], const double* sx, const double* sy,
-                      const double* sz, const double* s2_thresholds, int nedges, const double los[3], int nell,
+                      const double* sz, const double* s2_thresholds, int nedges, const double mode_shift[3],
+                      const double los[3], int nell,
                       const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace, size_t workspace_bytes);
 /* mp_irfft3d then mp_bin_real_poles in one go (jaxpower/mesh2.py:477-498: c2r of the product
  * spectrum, then the multipole binning), the last inverse pass fused with the binning so the

@@ -16,6 +16,9 @@ struct BinGeom {
   const double* ky;
   const double* kz;
   const double* thr;  // thr[e] = min{t : fl(sqrt(t)) >= edge e}
+  // Added to kx, ky, kz before the norm that picks the bin (mode oversampling); mu and the
+  // hermitian weights use the unshifted wavenumbers.  Zero except in mp_bin_modes/mp_bin_poles.
+  double shift[3];
 };
 
 // (kx^2 + ky^2) + kz^2 exactly as IEEE float64 (no fused multiply-add), the order in which the
@@ -60,6 +63,7 @@ struct RealBinArgs {
   const double* sy;
   const double* sz;
   const double* thr;  // thr[e] = min{t : fl(sqrt(t)) >= edge e}
+  double shift[3];    // added to sx, sy, sz for the bin only (mode oversampling), mu unshifted
   int nell;
   int ell[CB_MAX_ELL], rz[CB_MAX_ELL];
   LegendreTable leg;

@@ -10,9 +10,12 @@
 //     which is the same test and needs no device square root,
 //   * remove_zero zeroes the (0,0,0) mode for the products that ask for it (mesh2.py:117-118).
 //
+// Mode oversampling (mesh2.py:48-61): the bin of a mode is that of the shifted vector k + s for
+// one integer shift s per call; the caller sums the calls over the shifts.
+//
 // Work decomposition: each thread sweeps SEG consecutive kz of one (kx, ky) row; along a row
-// |k| increases, so the bin index only moves forward and the thread keeps FP64 running sums
-// in registers, flushing them into an LDS histogram when the bin changes.  Each block writes
+// the bin index moves by a step or two (forward for s = 0, either way otherwise) and the thread
+// keeps FP64 running sums in registers, flushing them into an LDS histogram when the bin changes.  Each block writes
 // its histogram once; a second kernel sums the block partials in a fixed order.
 #include "bin.h"
 #include "sph.h"
@@ -52,8 +55,8 @@ bin_poles_kernel(BinGeom g, BinProducts P, double* __restrict__ partial, double*
     const int ix = (int)(row / g.n1), iy = (int)(row % g.n1);
     const int z0 = seg * BIN_SEG, z1 = min(z0 + BIN_SEG, g.ncols);
     const double kx = g.kx[ix], ky = g.ky[iy];
-    const double a2 = k2_xy(kx, ky);
-    int b = first_bin(g.thr, g.nedges, k2_sum(a2, g.kz[z0]));
+    const double a2 = k2_xy(kx + g.shift[0], ky + g.shift[1]);
+    int b = first_bin(g.thr, g.nedges, k2_sum(a2, g.kz[z0] + g.shift[2]));
     double accr[NP], acci[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) accr[p] = acci[p] = 0.;
@@ -62,9 +65,7 @@ bin_poles_kernel(BinGeom g, BinProducts P, double* __restrict__ partial, double*
     const float fa2 = fkx * fkx + fky * fky;
     for (int iz = z0; iz < z1; ++iz) {
       const double kzv = g.kz[iz];
-      const double k2 = k2_sum(a2, kzv);
-      int nb = b;
-      while (nb < g.nedges && g.thr[nb] <= k2) ++nb;
+      const int nb = walk_bin(g.thr, g.nedges, b, k2_sum(a2, kzv + g.shift[2]));
       if (nb != b) {
         if (b >= 1 && b < g.nedges) {
 #pragma unroll
@@ -153,14 +154,13 @@ bin_modes_kernel(BinGeom g, unsigned long long* __restrict__ nmodes, double* __r
     const int seg = (int)(q % nseg);
     const int ix = (int)(row / g.n1), iy = (int)(row % g.n1);
     const int z0 = seg * BIN_SEG, z1 = min(z0 + BIN_SEG, g.ncols);
-    const double a2 = k2_xy(g.kx[ix], g.ky[iy]);
-    int b = first_bin(g.thr, g.nedges, k2_sum(a2, g.kz[z0]));
+    const double a2 = k2_xy(g.kx[ix] + g.shift[0], g.ky[iy] + g.shift[1]);
+    int b = first_bin(g.thr, g.nedges, k2_sum(a2, g.kz[z0] + g.shift[2]));
     unsigned long long cnt = 0;
     double ks = 0.;
     for (int iz = z0; iz < z1; ++iz) {
-      const double k2 = k2_sum(a2, g.kz[iz]);
-      int nb = b;
-      while (nb < g.nedges && g.thr[nb] <= k2) ++nb;
+      const double k2 = k2_sum(a2, g.kz[iz] + g.shift[2]);
+      const int nb = walk_bin(g.thr, g.nedges, b, k2);
       if (nb != b) {
         if (b >= 1 && b < g.nedges && cnt) {
           atomicAdd(&nmodes[b - 1], cnt);
@@ -231,7 +231,7 @@ int bin_poles(hipStream_t s, const BinGeom& g, const BinProducts& P, double* out
 using namespace mp;
 
 static int make_geom(BinGeom& g, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz, const double* thr,
-                     int nedges) {
+                     int nedges, const double* mode_shift) {
   MP_REQUIRE(meshsize && kx && ky && kz && thr, "bin: null pointer");
   MP_REQUIRE(nedges >= 2, "bin: need at least 2 edges, got %d", nedges);
   MP_REQUIRE(meshsize[0] >= 1 && meshsize[1] >= 1 && meshsize[2] >= 1, "bin: bad meshsize");
@@ -243,15 +243,17 @@ static int make_geom(BinGeom& g, const int32_t meshsize[3], const double* kx, co
   g.ky = ky;
   g.kz = kz;
   g.thr = thr;
+  for (int a = 0; a < 3; ++a) g.shift[a] = mode_shift ? mode_shift[a] : 0.;
   return MP_OK;
 }
 
 MP_EXPORT int64_t mp_bin_workspace_bytes(int nprod, int nedges) { return (int64_t)bin_workspace_bytes(nprod, nedges); }
 
 MP_EXPORT int mp_bin_modes(void* stream, const int32_t meshsize[3], const double* kx, const double* ky, const double* kz,
-                           const double* k2_thresholds, int nedges, int64_t* nmodes, double* ksum) {
-  BinGeom g;
-  int rc = make_geom(g, meshsize, kx, ky, kz, k2_thresholds, nedges);
+                           const double* k2_thresholds, int nedges, const double mode_shift[3], int64_t* nmodes,
+                           double* ksum) {
+  BinGeom g{};
+  int rc = make_geom(g, meshsize, kx, ky, kz, k2_thresholds, nedges, mode_shift);
   if (rc) return rc;
   MP_REQUIRE(nmodes && ksum, "mp_bin_modes: null output");
   hipStream_t s = (hipStream_t)stream;
@@ -264,10 +266,11 @@ MP_EXPORT int mp_bin_modes(void* stream, const int32_t meshsize[3], const double
 
 MP_EXPORT int mp_bin_poles(void* stream, int nmesh, const void* const* meshes, const int64_t* pitches, const int32_t meshsize[3],
                            const double* kx, const double* ky, const double* kz, const double* k2_thresholds, int nedges,
-                           const double los[3], int nprod, const int32_t* prod_i, const int32_t* prod_j, const int32_t* prod_ell,
-                           const int32_t* prod_remove_zero, double* out, void* workspace, size_t workspace_bytes) {
-  BinGeom g;
-  int rc = make_geom(g, meshsize, kx, ky, kz, k2_thresholds, nedges);
+                           const double mode_shift[3], const double los[3], int nprod, const int32_t* prod_i, const int32_t* prod_j,
+                           const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out, void* workspace,
+                           size_t workspace_bytes) {
+  BinGeom g{};
+  int rc = make_geom(g, meshsize, kx, ky, kz, k2_thresholds, nedges, mode_shift);
   if (rc) return rc;
   MP_REQUIRE(nmesh >= 1 && nmesh <= BIN_MAX_MESH, "mp_bin_poles: nmesh %d not in 1..%d", nmesh, BIN_MAX_MESH);
   MP_REQUIRE(nprod >= 1 && nprod <= BIN_MAX_PROD, "mp_bin_poles: nprod %d not in 1..%d", nprod, BIN_MAX_PROD);

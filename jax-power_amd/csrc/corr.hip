@@ -9,7 +9,9 @@
 //   * bin = digitize(|s|, edges, right=False) with |s| = fl(sqrt((sx^2 + sy^2) + sz^2)) in
 //     float64, tested on s^2 against the thresholds t_e = min{t : fl(sqrt(t)) >= edge e},
 //   * mu = s.los / |s| with |s| := 1 at s = 0 (mesh2.py:493), the (0,0,0) cell skipped for the
-//     ells that ask for remove_zero (mesh2.py:190-192).
+//     ells that ask for remove_zero (mesh2.py:190-192),
+//   * mode oversampling (mesh2.py:48-61): the bin is that of s + shift for one integer shift per
+//     call (mu from the unshifted s); the caller sums the calls over the shifts.
 //
 // Work decomposition: each thread sweeps CB_SEG consecutive z cells of one (x, y) row.  Along a
 // row |s| rises to the middle and falls after it, so the bin index moves by small steps in
@@ -43,12 +45,12 @@ bin_real_modes_kernel(RealBinArgs a, unsigned long long* __restrict__ nmodes, do
     const int seg = (int)(q % nseg);
     const int ix = (int)(row / a.n1), iy = (int)(row % a.n1);
     const int z0 = seg * CB_SEG, z1 = min(z0 + CB_SEG, a.n2);
-    const double a2 = k2_xy(a.sx[ix], a.sy[iy]);
-    int b = first_bin(a.thr, a.nedges, k2_sum(a2, a.sz[z0]));
+    const double a2 = k2_xy(a.sx[ix] + a.shift[0], a.sy[iy] + a.shift[1]);
+    int b = first_bin(a.thr, a.nedges, k2_sum(a2, a.sz[z0] + a.shift[2]));
     unsigned long long cnt = 0;
     double ss = 0.;
     for (int iz = z0; iz < z1; ++iz) {
-      const double s2 = k2_sum(a2, a.sz[iz]);
+      const double s2 = k2_sum(a2, a.sz[iz] + a.shift[2]);
       const int nb = walk_bin(a.thr, a.nedges, b, s2);
       if (nb != b) {
         if (b >= 1 && b < a.nedges && cnt) {
@@ -91,8 +93,8 @@ bin_real_kernel(RealBinArgs a, double* __restrict__ partial) {
     const int ix = (int)(row / a.n1), iy = (int)(row % a.n1);
     const int z0 = seg * CB_SEG, z1 = min(z0 + CB_SEG, a.n2);
     const double sxv = a.sx[ix], syv = a.sy[iy];
-    const double a2 = k2_xy(sxv, syv);
-    int b = first_bin(thr, a.nedges, k2_sum(a2, szt[z0]));
+    const double a2 = k2_xy(sxv + a.shift[0], syv + a.shift[1]);
+    int b = first_bin(thr, a.nedges, k2_sum(a2, szt[z0] + a.shift[2]));
     double acc[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) acc[e] = 0.;
@@ -102,7 +104,7 @@ bin_real_kernel(RealBinArgs a, double* __restrict__ partial) {
     const float* src = a.mesh + row * a.n2;
     for (int iz = z0; iz < z1; ++iz) {
       const double szv = szt[iz];
-      const int nb = walk_bin(thr, a.nedges, b, k2_sum(a2, szv));
+      const int nb = walk_bin(thr, a.nedges, b, k2_sum(a2, szv + a.shift[2]));
       if (nb != b) {
         if (b >= 1 && b < a.nedges) {
 #pragma unroll
@@ -156,8 +158,8 @@ bin_real_global_kernel(RealBinArgs a, double* __restrict__ out) {
     const int ix = (int)(row / a.n1), iy = (int)(row % a.n1);
     const int z0 = seg * CB_SEG, z1 = min(z0 + CB_SEG, a.n2);
     const double sxv = a.sx[ix], syv = a.sy[iy];
-    const double a2 = k2_xy(sxv, syv);
-    int b = first_bin(a.thr, a.nedges, k2_sum(a2, a.sz[z0]));
+    const double a2 = k2_xy(sxv + a.shift[0], syv + a.shift[1]);
+    int b = first_bin(a.thr, a.nedges, k2_sum(a2, a.sz[z0] + a.shift[2]));
     double acc[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) acc[e] = 0.;
@@ -167,7 +169,7 @@ bin_real_global_kernel(RealBinArgs a, double* __restrict__ out) {
     const float* src = a.mesh + row * a.n2;
     for (int iz = z0; iz < z1; ++iz) {
       const double szv = a.sz[iz];
-      const int nb = walk_bin(a.thr, a.nedges, b, k2_sum(a2, szv));
+      const int nb = walk_bin(a.thr, a.nedges, b, k2_sum(a2, szv + a.shift[2]));
       if (nb != b) {
         if (b >= 1 && b < a.nedges) {
 #pragma unroll
@@ -239,7 +241,7 @@ __global__ void mul_ylm_kernel(const float* in, float* out, int n0, int n1, int 
 }
 
 static int real_args(RealBinArgs& a, const float* mesh, const int32_t n[3], const double* sx, const double* sy, const double* sz,
-                     const double* thr, int nedges) {
+                     const double* thr, int nedges, const double* mode_shift = nullptr) {
   MP_REQUIRE(n && sx && sy && sz && thr, "correlation binning: null pointer");
   MP_REQUIRE(nedges >= 2, "correlation binning: need at least 2 edges, got %d", nedges);
   MP_REQUIRE(n[0] >= 1 && n[1] >= 1 && n[2] >= 1, "correlation binning: bad meshsize");
@@ -252,6 +254,7 @@ static int real_args(RealBinArgs& a, const float* mesh, const int32_t n[3], cons
   a.sy = sy;
   a.sz = sz;
   a.thr = thr;
+  for (int d = 0; d < 3; ++d) a.shift[d] = mode_shift ? mode_shift[d] : 0.;
   return MP_OK;
 }
 
@@ -295,9 +298,10 @@ static int launch_real_bin(hipStream_t s, const RealBinArgs& a, double* out, voi
 using namespace mp;
 
 MP_EXPORT int mp_bin_real_modes(void* stream, const int32_t meshsize[3], const double* sx, const double* sy, const double* sz,
-                                const double* s2_thresholds, int nedges, int64_t* nmodes, double* ssum) {
+                                const double* s2_thresholds, int nedges, const double mode_shift[3], int64_t* nmodes,
+                                double* ssum) {
   RealBinArgs a{};
-  int rc = real_args(a, nullptr, meshsize, sx, sy, sz, s2_thresholds, nedges);
+  int rc = real_args(a, nullptr, meshsize, sx, sy, sz, s2_thresholds, nedges, mode_shift);
   if (rc) return rc;
   MP_REQUIRE(nmodes && ssum, "mp_bin_real_modes: null output");
   hipStream_t s = (hipStream_t)stream;
@@ -320,10 +324,11 @@ MP_EXPORT int64_t mp_bin_real_workspace_bytes(const int32_t meshsize[3], int nel
 }
 
 MP_EXPORT int mp_bin_real_poles(void* stream, const float* mesh, const int32_t meshsize[3], const double* sx, const double* sy,
-                                const double* sz, const double* s2_thresholds, int nedges, const double los[3], int nell,
-                                const int32_t* ells, const int32_t* remove_zero, double* out, void* workspace, size_t workspace_bytes) {
+                                const double* sz, const double* s2_thresholds, int nedges, const double mode_shift[3],
+                                const double los[3], int nell, const int32_t* ells, const int32_t* remove_zero, double* out,
+                                void* workspace, size_t workspace_bytes) {
   RealBinArgs a{};
-  int rc = real_args(a, mesh, meshsize, sx, sy, sz, s2_thresholds, nedges);
+  int rc = real_args(a, mesh, meshsize, sx, sy, sz, s2_thresholds, nedges, mode_shift);
   if (rc) return rc;
   MP_REQUIRE(mesh && out, "mp_bin_real_poles: null pointer");
   rc = real_ells(a, los, nell, ells, remove_zero);

@@ -32,7 +32,7 @@ struct AdjArgs {
   float2* Y;        // n_i half spectra (same pitch), mesh j at Y + j * mesh_stride
   int64_t mesh_stride;
   int ni, nz;
-  BinGeom g;
+  BinGeom g{};
   CombineTables tab;
   float scale;      // N^3 / n_i
   int zero_dc;

@@ -74,7 +74,7 @@ MP_EXPORT int mp_bin_poles_partial_noalloc(void* stream, const void* partial, in
                      meshsize[0], meshsize[1], meshsize[2], nprod, nedges - 1, need, NOALLOC_SCRATCH_BYTES);
   void* ptr = scratch_for_current_device();
   if (!ptr) return MP_ERR_HIP;
-  BinGeom g;
+  BinGeom g{};
   g.n0 = meshsize[0];
   g.n1 = meshsize[1];
   g.ncols = meshsize[2] / 2 + 1;

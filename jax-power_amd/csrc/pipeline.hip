@@ -226,7 +226,7 @@ MP_EXPORT int mp_bin_poles_partial(void* stream, const void* partial, int64_t pi
   MP_REQUIRE(meshsize && kx && ky && kz && k2_thresholds, "mp_bin_poles_partial: null pointer");
   MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
   MP_REQUIRE(nedges >= 2, "mp_bin_poles_partial: need at least 2 edges, got %d", nedges);
-  BinGeom g;
+  BinGeom g{};
   g.n0 = meshsize[0];
   g.n1 = meshsize[1];
   g.ncols = meshsize[2] / 2 + 1;
@@ -304,7 +304,7 @@ MP_EXPORT int mp_ylm_bin(void* stream, const float* rmesh, const void* A0, int64
                          size_t workspace_bytes) {
   MP_REQUIRE(meshsize && boxsize && boxcenter && kx && ky && kz && k2_thresholds, "mp_ylm_bin: null pointer");
   MP_REQUIRE(nedges >= 2, "mp_ylm_bin: need at least 2 edges, got %d", nedges);
-  BinGeom g;
+  BinGeom g{};
   g.n0 = meshsize[0];
   g.n1 = meshsize[1];
   g.ncols = meshsize[2] / 2 + 1;

@@ -58,13 +58,13 @@ SIGNATURES = {
     'mp_rfft3d': (c_int, [c_void_p, c_void_p, c_void_p, P_I32, c_int64]),
     'mp_irfft3d_workspace_bytes': (c_int64, [P_I32]),
     'mp_irfft3d': (c_int, [c_void_p, c_void_p, c_int64, c_void_p, P_I32, c_void_p, c_size_t]),
-    'mp_bin_modes': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    'mp_bin_modes': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_void_p, c_void_p]),
     'mp_bin_workspace_bytes': (c_int64, [c_int, c_int]),
     'mp_bin_poles': (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), P_I64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p,
-                             c_int, P_F64, c_int, P_I32, P_I32, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
-    'mp_bin_real_modes': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+                             c_int, P_F64, P_F64, c_int, P_I32, P_I32, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
+    'mp_bin_real_modes': (c_int, [c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_void_p, c_void_p]),
     'mp_bin_real_workspace_bytes': (c_int64, [P_I32, c_int, c_int]),
-    'mp_bin_real_poles': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, c_int, P_I32,
+    'mp_bin_real_poles': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, P_F64, c_int, P_I32,
                                   P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_bispec_shells_workspace_bytes': (c_size_t, [P_I32]),
     'mp_bispec_shells': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, P_F64,

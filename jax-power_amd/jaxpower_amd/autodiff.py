@@ -18,7 +18,7 @@ import torch
 
 from . import _lib, resamplers
 from .mesh import MeshAttrs, _paint_partial, _spectrum_pitch
-from .mesh2 import _format_los
+from .mesh2 import _format_los, _single_shift
 
 __all__ = ['mesh2_spectrum_poles', 'box_spectrum_values']
 
@@ -99,6 +99,7 @@ def mesh2_spectrum_poles(positions, weights=None, attrs=None, bin=None, resample
     weights.  Global line of sight only (a string axis or a vector)."""
     if not isinstance(attrs, MeshAttrs):
         raise ValueError('attrs must be a MeshAttrs')
+    _single_shift(bin, 'mesh2_spectrum_poles')
     los, vlos, swap = _format_los(los)
     if vlos is None:
         raise NotImplementedError('the differentiable path takes a global line of sight')

@@ -71,14 +71,59 @@ def _find_unique_edges(mattrs, xmin=0., xmax=np.inf, device=None, vectors=None, 
     return np.concatenate([[tmp[0] - (x[1] - x[0])], tmp, [tmp[-1] + (x[-1] - x[-2])]])
 
 
+def _oversampling_shifts(mode_oversampling):
+    """Integer shifts of _make_edges2 (mesh2.py:48-49): every triple in [-m, m]^3, last axis
+    fastest.  They are added to the coordinates themselves (h/Mpc for wavenumbers, Mpc/h for
+    separations), not to mode indices."""
+    m = int(mode_oversampling)
+    if m < 0:
+        raise ValueError(f'mode_oversampling must be >= 0, got {mode_oversampling}')
+    r = range(-m, m + 1)
+    return [(float(a), float(b), float(c)) for a in r for b in r for c in r]
+
+
+def _shifted_modes(call, shifts, nbins, device):
+    """nmodes, xavg of _make_edges2 (mesh2.py:48-61) from call(shift, nmodes, xsum), which
+    fills the counts and the sum of shifted norms of one shift: counts summed over the shifts,
+    divided by their number when there are several (float then), xavg = xsum / nmodes."""
+    total = torch.zeros(nbins, dtype=torch.int64, device=device)
+    xsum = torch.zeros(nbins, dtype=torch.float64, device=device)
+    nmodes = torch.empty(nbins, dtype=torch.int64, device=device)
+    part = torch.empty(nbins, dtype=torch.float64, device=device)
+    for shift in shifts:
+        call(shift, nmodes, part)
+        total += nmodes
+        xsum += part
+    if len(shifts) > 1:
+        host = total.cpu().numpy() / len(shifts)
+    else:
+        host = total.cpu().numpy()
+    with np.errstate(invalid='ignore', divide='ignore'):
+        xavg = xsum.cpu().numpy() / host
+    return host, torch.from_numpy(np.asarray(host, dtype='f8')).to(device), xavg
+
+
+def _single_shift(bin, what):
+    """Paths that bin with their own fused kernels take bins without mode oversampling."""
+    if len(getattr(bin, '_shifts', ((0., 0., 0.),))) > 1:
+        raise NotImplementedError(f'{what}: bins with mode_oversampling > 0 are taken by compute_mesh2_spectrum, '
+                                  'compute_mesh2_correlation and the bin call, not here')
+
+
 class BinMesh2SpectrumPoles:
-    """Binning operator for mesh to power spectrum (mesh2.py:65-119)."""
+    """Binning operator for mesh to power spectrum (mesh2.py:65-119).
+
+    mode_oversampling = m bins every mode once per integer shift s in [-m, m]^3 by |k + s|
+    (mesh2.py:48-61): nmodes is the mean over the shifts of the counts (a float when m > 0),
+    xavg = (sum over the shifts of sum |k + s|) / nmodes, and a binned sum is the mean over
+    the shifts of the per-shift sums (mesh.py:2874-2910); mu and the hermitian weights stay
+    those of k."""
 
     def __init__(self, mattrs, edges=None, ells=0, mode_oversampling=0, device=None):
         if not isinstance(mattrs, MeshAttrs):
             mattrs = mattrs.attrs
-        if mode_oversampling:
-            raise NotImplementedError('mode_oversampling > 0 is not implemented on this path')
+        self.mode_oversampling = int(mode_oversampling)
+        self._shifts = _oversampling_shifts(mode_oversampling)
         device = device or _default_device()
         if edges is None:
             edges = {}
@@ -104,16 +149,14 @@ class BinMesh2SpectrumPoles:
         self._tables = dict(kx=torch.from_numpy(kx).to(device), ky=torch.from_numpy(ky).to(device),
                             kz=torch.from_numpy(kz).to(device),
                             thr=torch.from_numpy(_k2_thresholds(edges)).to(device))
-        nbins = len(edges) - 1
-        nmodes = torch.empty(nbins, dtype=torch.int64, device=device)
-        ksum = torch.empty(nbins, dtype=torch.float64, device=device)
         t = self._tables
-        _lib.call('mp_bin_modes', _lib.stream_handle(device), _lib.i32x3(mattrs.meshsize), _lib.ptr(t['kx']), _lib.ptr(t['ky']),
-                  _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(edges), _lib.ptr(nmodes), _lib.ptr(ksum))
-        self._nmodes_dev = nmodes.to(torch.float64)
-        self.nmodes = nmodes.cpu().numpy()
-        with np.errstate(invalid='ignore', divide='ignore'):
-            self.xavg = ksum.cpu().numpy() / self.nmodes
+
+        def count(shift, nmodes, ksum):
+            _lib.call('mp_bin_modes', _lib.stream_handle(device), _lib.i32x3(mattrs.meshsize), _lib.ptr(t['kx']),
+                      _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(edges), _lib.f64x3(shift), _lib.ptr(nmodes),
+                      _lib.ptr(ksum))
+
+        self.nmodes, self._nmodes_dev, self.xavg = _shifted_modes(count, self._shifts, len(edges) - 1, device)
         self.wmodes = None  # hermitian weights 1 + (kz > 0) are applied inside the kernel
 
     @property
@@ -140,10 +183,16 @@ class BinMesh2SpectrumPoles:
         pitch_arr = (_lib.ctypes.c_int64 * len(arrays))(*pitches)
         col = lambda k: (_lib.ctypes.c_int32 * nprod)(*[int(p[k]) for p in products])
         t = self._tables
-        _lib.call('mp_bin_poles', _lib.stream_handle(device), len(arrays), ptrs, pitch_arr, _lib.i32x3(self.mattrs.meshsize),
-                  _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(self._edges1d),
-                  _lib.f64x3(los), nprod, col(0), col(1), col(2), col(3), _lib.ptr(out), _lib.ptr(ws), ws.numel())
-        return torch.view_as_complex(out)
+        total = None
+        for shift in self._shifts:  # _bincount over the ibin list: per-shift sums, then their mean
+            _lib.call('mp_bin_poles', _lib.stream_handle(device), len(arrays), ptrs, pitch_arr, _lib.i32x3(self.mattrs.meshsize),
+                      _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(self._edges1d),
+                      _lib.f64x3(shift), _lib.f64x3(los), nprod, col(0), col(1), col(2), col(3), _lib.ptr(out), _lib.ptr(ws),
+                      ws.numel())
+            if len(self._shifts) == 1:
+                return torch.view_as_complex(out)
+            total = out.clone() if total is None else total + out
+        return torch.view_as_complex((total / len(self._shifts)).contiguous())
 
     def _pending_products(self, pending, ells, remove_zero, los):
         """_products_dev() of conj(A) A L_ell for the combined mesh A of a pending interlacing
@@ -152,7 +201,7 @@ class BinMesh2SpectrumPoles:
         device = pending.partial.device
         n = _lib.i32x3(self.mattrs.meshsize)
         nedges = len(self._edges1d)
-        if nprod > 6 or nprod * (nedges - 1) > 4096:
+        if nprod > 6 or nprod * (nedges - 1) > 4096 or len(self._shifts) > 1:
             return None
         out = torch.empty((nprod, self.nbins, 2), dtype=torch.float64, device=device)
         ws = _lib.workspace(_lib.load().mp_bin_partial_workspace_bytes(n, pending.interlacing, nprod, nedges), device)
@@ -275,6 +324,8 @@ def _ylm_bin_all(bin, rreal, A0, ells, with0=False, memory_fraction=0.5):
     (mp_ylm_bin), plus |A0|^2 under key 0 when with0; None when the fused kernel does not apply
     (shape, bins, or its 2 ell + 1 partial spectra would take more than memory_fraction of the
     free device memory), in which case the caller stores C_ell."""
+    if len(bin._shifts) > 1:
+        return None
     a = rreal.attrs
     device = rreal.value.device
     n = _lib.i32x3(a.meshsize)
@@ -324,8 +375,8 @@ class BinMesh2CorrelationPoles:
     def __init__(self, mattrs, edges=None, ells=0, mode_oversampling=0, basis=None, klimit=None, batch_size=None, device=None):
         if not isinstance(mattrs, MeshAttrs):
             mattrs = mattrs.attrs
-        if mode_oversampling:
-            raise NotImplementedError('mode_oversampling > 0 is not implemented on this path')
+        self.mode_oversampling = int(mode_oversampling)
+        self._shifts = _oversampling_shifts(mode_oversampling)
         if basis is not None:
             raise NotImplementedError(f"basis={basis!r} is not implemented on this path (only the default, basis=None)")
         device = device or _default_device()
@@ -355,16 +406,14 @@ class BinMesh2CorrelationPoles:
         sx, sy, sz = _svectors(mattrs)
         self._tables = dict(sx=torch.from_numpy(sx).to(device), sy=torch.from_numpy(sy).to(device),
                             sz=torch.from_numpy(sz).to(device), thr=torch.from_numpy(_k2_thresholds(edges)).to(device))
-        nbins = len(edges) - 1
-        nmodes = torch.empty(nbins, dtype=torch.int64, device=device)
-        ssum = torch.empty(nbins, dtype=torch.float64, device=device)
         t = self._tables
-        _lib.call('mp_bin_real_modes', _lib.stream_handle(device), _lib.i32x3(mattrs.meshsize), _lib.ptr(t['sx']),
-                  _lib.ptr(t['sy']), _lib.ptr(t['sz']), _lib.ptr(t['thr']), len(edges), _lib.ptr(nmodes), _lib.ptr(ssum))
-        self._nmodes_dev = nmodes.to(torch.float64)
-        self.nmodes = nmodes.cpu().numpy()
-        with np.errstate(invalid='ignore', divide='ignore'):
-            self.xavg = ssum.cpu().numpy() / self.nmodes
+
+        def count(shift, nmodes, ssum):
+            _lib.call('mp_bin_real_modes', _lib.stream_handle(device), _lib.i32x3(mattrs.meshsize), _lib.ptr(t['sx']),
+                      _lib.ptr(t['sy']), _lib.ptr(t['sz']), _lib.ptr(t['thr']), len(edges), _lib.f64x3(shift), _lib.ptr(nmodes),
+                      _lib.ptr(ssum))
+
+        self.nmodes, self._nmodes_dev, self.xavg = _shifted_modes(count, self._shifts, len(edges) - 1, device)
 
     @property
     def nbins(self):
@@ -384,15 +433,24 @@ class BinMesh2CorrelationPoles:
         ws = _lib.workspace(_lib.load().mp_bin_real_workspace_bytes(n, nell, nedges), device)
         col = lambda values: (_lib.ctypes.c_int32 * nell)(*[int(v) for v in values])
         t = self._tables
-        _lib.call('mp_bin_real_poles', _lib.stream_handle(device), _lib.ptr(value), n, _lib.ptr(t['sx']), _lib.ptr(t['sy']),
-                  _lib.ptr(t['sz']), _lib.ptr(t['thr']), nedges, _lib.f64x3(los), nell, col(ells), col(remove_zero),
-                  _lib.ptr(out), _lib.ptr(ws), ws.numel())
-        return out
+        total = None
+        for shift in self._shifts:  # per-shift sums, then their mean (mesh.py:2874-2910)
+            _lib.call('mp_bin_real_poles', _lib.stream_handle(device), _lib.ptr(value), n, _lib.ptr(t['sx']), _lib.ptr(t['sy']),
+                      _lib.ptr(t['sz']), _lib.ptr(t['thr']), nedges, _lib.f64x3(shift), _lib.f64x3(los), nell, col(ells),
+                      col(remove_zero), _lib.ptr(out), _lib.ptr(ws), ws.numel())
+            if len(self._shifts) == 1:
+                return out
+            total = out.clone() if total is None else total + out
+        return total / len(self._shifts)
 
     def _spectrum_sums_dev(self, spectrum, ells, remove_zero, los=(0., 0., 0.), x_done=False):
         """_sums_dev() of irfftn(spectrum) for a half spectrum, with the last pass of the
         inverse FFT fused with the binning (the real mesh is never stored); x_done: the
         spectrum already carries its inverse FFT along x."""
+        if len(self._shifts) > 1:  # the fused pass bins one shift: store the real mesh instead
+            if x_done:
+                raise ValueError('an oversampled bin takes the product spectrum without its inverse x pass')
+            return self._sums_dev(_c2r(spectrum, self.mattrs.meshsize), ells, remove_zero, los=los)
         value, pitch = _as_spectrum(spectrum)
         device = value.device
         nell = len(ells)
@@ -473,6 +531,7 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
     ells = bin.ells
     same = fields[1] == fields[0]
     nonzeroells = [ell for ell in ells if ell != 0]
+    fused = len(bin._shifts) == 1  # inverse FFT fused with the binning of one shift
 
     if vlos is None:  # local (varying) line of sight, mesh2.py:416-469
         if swap:
@@ -481,7 +540,7 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
         A0 = _2c(rmesh1 if same else meshes[1])
         num = {}
         if 0 in ells:
-            P, x_done = _product_spectrum(A0, inverse_x=True) if same else _product_spectrum(_2c(rmesh1), A0)
+            P, x_done = _product_spectrum(A0, inverse_x=fused) if same else _product_spectrum(_2c(rmesh1), A0)
             num[0] = bin._spectrum_sums_dev(P, [0], [False], x_done=x_done)[0]
             del P
         if nonzeroells:
@@ -510,7 +569,7 @@ def compute_mesh2_correlation(*meshes, bin=None, los='z'):
         num = [num[ell] / (bin._nmodes_dev * vcell) for ell in ells]
     else:  # fixed line of sight, mesh2.py:471-504
         A1 = _2c(meshes[0])
-        P, x_done = _product_spectrum(A1, inverse_x=True) if same else _product_spectrum(A1, _2c(meshes[1]))
+        P, x_done = _product_spectrum(A1, inverse_x=fused) if same else _product_spectrum(A1, _2c(meshes[1]))
         sums = bin._spectrum_sums_dev(P, ells, [ell != 0 for ell in ells], los=vlos, x_done=x_done)
         del P
         num = [sums[i] * ((2 * ell + 1) / (bin._nmodes_dev * vcell)) for i, ell in enumerate(ells)]
@@ -540,6 +599,7 @@ def compute_mesh2_spectrum_window(*meshes, edgesin, ellsin=None, bin=None, los='
     (mesh2.py:1045-1076).  The local line of sight is not implemented on this path."""
     if not isinstance(bin, BinMesh2SpectrumPoles):
         raise ValueError(f'bin must be BinMesh2SpectrumPoles, not {type(bin)}')
+    _single_shift(bin, 'compute_mesh2_spectrum_window')
     flags = tuple(flags or ())
     infinite, smooth = 'infinite' in flags, 'smooth' in flags
     if set(flags) - {'infinite', 'smooth'}:
@@ -1185,6 +1245,7 @@ def compute_mesh2_spectrum_mean(*meshes, theory, bin=None, los='z'):
     c2r(P_lin rnorm Y_lin,m1(khat)) c2r(conj(r2c(rho Y_lm Y_lin,m1)) A0)."""
     if not isinstance(bin, BinMesh2SpectrumPoles):
         raise ValueError(f'bin must be BinMesh2SpectrumPoles, not {type(bin)}')
+    _single_shift(bin, 'compute_mesh2_spectrum_mean')
     periodic = isinstance(meshes[0], MeshAttrs)
     if periodic:
         mattrs = meshes[0]

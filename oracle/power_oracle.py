@@ -343,20 +343,67 @@ def unique_edges(vec, x0, xmin=0., xmax=np.inf):
     return np.concatenate([[tmp[0] - (x[1] - x[0])], tmp, [tmp[-1] + (x[-1] - x[-2])]])
 
 
-class BinPoles:
-    """BinMesh2SpectrumPoles (mesh2.py:65-119), mode_oversampling=0."""
+def oversampling_shifts(mode_oversampling):
+    """The shifts of _make_edges2 (mesh2.py:48-49): every integer triple in [-m, m]^3, last
+    axis fastest (itertools.product order)."""
+    r = range(-int(mode_oversampling), int(mode_oversampling) + 1)
+    return [(a, b, c) for a in r for b in r for c in r]
 
-    def __init__(self, attrs, edges=None, ells=0):
+
+def shifted_norm(vec, shift):
+    """sqrt(sum((x + s)^2)) over the three axes in order (mesh2.py:51): ((x^2 + y^2) + z^2)."""
+    x, y, z = (v + s for v, s in zip(vec, shift))
+    k2 = x[:, None, None]**2 + y[None, :, None]**2
+    k2 = k2 + z[None, None, :]**2
+    return np.sqrt(k2)
+
+
+def oversampled_bins(vec, edges, weights, mode_oversampling):
+    """_make_edges2's loop (mesh2.py:48-61): one digitize per shift, nmodes and xsum summed over
+    the shifts; nmodes divided by the number of shifts when there are several (a float then)
+    and xavg = xsum / that nmodes.  With weights None every mode counts 1."""
+    shifts = oversampling_shifts(mode_oversampling)
+    length = len(edges) + 1
+    ibin, nmodes, xsum = [], 0, 0.
+    for shift in shifts:
+        coords = shifted_norm(vec, shift)
+        ib = np.digitize(coords, edges, right=False)  # mesh.py:2840
+        w = None if weights is None else np.broadcast_to(weights, coords.shape).ravel()
+        ibin.append(ib)
+        counts = np.bincount(ib.ravel(), weights=w, minlength=length)[1:-1]
+        nmodes = nmodes + counts.round().astype('i8')
+        xsum = xsum + np.bincount(ib.ravel(), weights=coords.ravel() * (1 if w is None else w), minlength=length)[1:-1]
+    if len(shifts) > 1:
+        nmodes = nmodes / len(shifts)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        xavg = xsum / nmodes  # mesh2.py:62 (NaN for empty bins)
+    return ibin, nmodes, xavg
+
+
+def bincount_shifts(ibin, value, nbins):
+    """_bincount over the list of per-shift bin indices (mesh.py:2874-2910): the per-shift
+    bincounts summed, then divided by their number."""
+    n = nbins + 2
+    s = 0.
+    for ib in ibin:
+        ib = ib.ravel()
+        if np.iscomplexobj(value):
+            s = s + np.bincount(ib, weights=value.real.ravel(), minlength=n) + 1j * np.bincount(ib, weights=value.imag.ravel(), minlength=n)
+        else:
+            s = s + np.bincount(ib, weights=value.ravel(), minlength=n)
+    return s[1:-1] / len(ibin)
+
+
+class BinPoles:
+    """BinMesh2SpectrumPoles (mesh2.py:65-119) with mode_oversampling (mesh2.py:48-61)."""
+
+    def __init__(self, attrs, edges=None, ells=0, mode_oversampling=0):
         self.attrs = attrs
         edges = make_edges(attrs, edges)
-        k = knorm(attrs)
         self.wmodes = hermitian_weights(attrs)
-        self.ibin = np.digitize(k, edges, right=False)  # mesh.py:2840
-        length = len(edges) + 1
-        self.nmodes = np.bincount(self.ibin.ravel(), weights=self.wmodes.ravel(), minlength=length)[1:-1].round().astype('i8')
-        xsum = np.bincount(self.ibin.ravel(), weights=(k * self.wmodes).ravel(), minlength=length)[1:-1]
-        with np.errstate(invalid='ignore', divide='ignore'):
-            self.xavg = xsum / self.nmodes  # mesh2.py:62 (NaN for empty bins)
+        self.ibin, self.nmodes, self.xavg = oversampled_bins(attrs.kcoords('wavenumber'), edges, self.wmodes, mode_oversampling)
+        if mode_oversampling == 0:
+            self.ibin = self.ibin[0]
         self.edges = np.column_stack([edges[:-1], edges[1:]])
         self.ells = tuple(sorted(ells if np.ndim(ells) else (ells,)))
 
@@ -366,14 +413,9 @@ class BinPoles:
         if remove_zero:
             value[0, 0, 0] = 0.
         value = value * self.wmodes
-        n = len(self.xavg) + 2
-        ib = self.ibin.ravel()
-        if np.iscomplexobj(value):
-            s = np.bincount(ib, weights=value.real.ravel(), minlength=n) + 1j * np.bincount(ib, weights=value.imag.ravel(), minlength=n)
-        else:
-            s = np.bincount(ib, weights=value.ravel(), minlength=n)
+        ibin = self.ibin if isinstance(self.ibin, list) else [self.ibin]
         with np.errstate(invalid='ignore', divide='ignore'):
-            return s[1:-1] / self.nmodes
+            return bincount_shifts(ibin, value, len(self.xavg)) / self.nmodes
 
 
 # ----------------------------------------------------------------------------
@@ -648,19 +690,16 @@ def make_sedges(attrs, edges=None):
 
 
 class BinCorrelationPoles:
-    """BinMesh2CorrelationPoles (mesh2.py:120-184), basis=None, mode_oversampling=0: every
-    cell of the real mesh weighted 1, bins by digitize(|s|, edges, right=False)."""
+    """BinMesh2CorrelationPoles (mesh2.py:120-184), basis=None, with mode_oversampling
+    (mesh2.py:48-61): every cell of the real mesh weighted 1, bins by digitize(|s + shift|,
+    edges, right=False)."""
 
-    def __init__(self, attrs, edges=None, ells=0):
+    def __init__(self, attrs, edges=None, ells=0, mode_oversampling=0):
         self.attrs = attrs
         edges = make_sedges(attrs, edges)
-        snrm = snorm(attrs)
-        self.ibin = np.digitize(snrm, edges, right=False)  # mesh.py:2840
-        length = len(edges) + 1
-        self.nmodes = np.bincount(self.ibin.ravel(), minlength=length)[1:-1].astype('i8')
-        xsum = np.bincount(self.ibin.ravel(), weights=snrm.ravel(), minlength=length)[1:-1]
-        with np.errstate(invalid='ignore', divide='ignore'):
-            self.xavg = xsum / self.nmodes  # mesh2.py:62
+        self.ibin, self.nmodes, self.xavg = oversampled_bins(separation_coords(attrs), edges, None, mode_oversampling)
+        if mode_oversampling == 0:
+            self.ibin = self.ibin[0]
         self.edges = np.column_stack([edges[:-1], edges[1:]])
         self.ells = tuple(sorted(ells if np.ndim(ells) else (ells,)))
 
@@ -669,10 +708,9 @@ class BinCorrelationPoles:
         value = np.array(value)
         if remove_zero:
             value[0, 0, 0] = 0.
-        n = len(self.xavg) + 2
-        s = np.bincount(self.ibin.ravel(), weights=value.ravel(), minlength=n)
+        ibin = self.ibin if isinstance(self.ibin, list) else [self.ibin]
         with np.errstate(invalid='ignore', divide='ignore'):
-            return s[1:-1] / self.nmodes
+            return bincount_shifts(ibin, value, len(self.xavg)) / self.nmodes
 
 
 def correlation_poles(attrs, bin, mesh1, mesh2=None, los='z'):

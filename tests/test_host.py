@@ -37,7 +37,7 @@ def test_host_queries_without_device():
     assert lib.mp_paint_workspace_bytes(_lib.i32x3([6, 6, 6]), 1000, 3, 3) == 256  # atomic path needs none
     assert lib.mp_bin_workspace_bytes(3, 161) >= 1024 * 3 * 160 * 16
     # argument errors are reported without touching a device
-    rc = lib.mp_bin_poles(None, 0, None, None, n, None, None, None, None, 1, None, 0, None, None, None, None, None, None, 0)
+    rc = lib.mp_bin_poles(None, 0, None, None, n, None, None, None, None, 1, None, None, 0, None, None, None, None, None, None, 0)
     assert rc == -1 and b'null' in lib.mp_last_error() or b'edges' in lib.mp_last_error()
 
 

@@ -262,6 +262,92 @@ def test_xavg_and_bincount_kats():
     assert np.allclose(bin(ones, remove_zero=True), [0., 1., 1.])  # nmodes keeps the DC mode
 
 
+def _bruteforce_oversampled(vec, edges, wz, m, value):
+    """Mode oversampling by explicit loops (mesh2.py:48-61, mesh.py:2874-2910): every integer
+    shift s in [-m, m]^3, every mode, bin = bisect of |v + s| in the edges, weight wz[iz];
+    returns nmodes (count / number of shifts when there are several), xavg = sum w |v + s| /
+    nmodes, and the binned value = sum w value / number of shifts / nmodes."""
+    import bisect
+    shifts = list(itertools.product(range(-m, m + 1), repeat=3))
+    nb = len(edges) - 1
+    count, xs, sums = [0] * nb, [0.] * nb, [0j] * nb
+    for sh in shifts:
+        for i, x in enumerate(vec[0]):
+            for j, y in enumerate(vec[1]):
+                for l, z in enumerate(vec[2]):
+                    a, b, c = x + sh[0], y + sh[1], z + sh[2]
+                    r = math.sqrt((a * a + b * b) + c * c)
+                    k = bisect.bisect_right(list(edges), r) - 1
+                    if 0 <= k < nb:
+                        w = int(wz[l])
+                        count[k] += w
+                        xs[k] += w * r
+                        sums[k] += w * complex(value[i, j, l])
+    S = len(shifts)
+    nmodes = np.array(count, dtype='i8') / (S if S > 1 else 1)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        return nmodes, np.array(xs) / nmodes, np.array(sums) / S / nmodes
+
+
+@pytest.mark.parametrize('m', [1, 2])
+def test_mode_oversampling_bruteforce(m):
+    """BinPoles(mode_oversampling=m) against explicit loops on a small box whose wavenumbers are
+    of order 1 h/Mpc, so that the integer shifts move modes between bins and out of range."""
+    attrs = po.Attrs((4, 6, 6), (12., 18., 15.), 0.)
+    edges = np.arange(0., 3.01, 0.2)
+    rng = np.random.default_rng(m)
+    shape = (4, 6, 4)
+    value = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    bin = po.BinPoles(attrs, edges=edges, mode_oversampling=m)
+    wz = po.hermitian_weights(attrs)[0, 0]
+    nmodes, xavg, binned = _bruteforce_oversampled(attrs.kcoords('wavenumber'), edges, wz, m, value)
+    assert len(bin.ibin) == (2 * m + 1)**3 and bin.nmodes.dtype == np.float64
+    assert np.array_equal(bin.nmodes, nmodes)
+    ok = nmodes > 0
+    assert ok.sum() >= 8 and np.allclose(bin.xavg[ok], xavg[ok], rtol=1e-13)
+    assert np.allclose(bin(value)[ok], binned[ok], rtol=1e-12)
+    # the reference divides the summed counts by the number of shifts before xavg = xsum / nmodes
+    # (mesh2.py:58-61), so xavg is that many times the mean shifted norm of the bin
+    S = (2 * m + 1)**3
+    assert np.all((bin.xavg[ok] / S >= bin.edges[ok, 0]) & (bin.xavg[ok] / S < bin.edges[ok, 1]))
+    # remove_zero zeroes the (0,0,0) mode in every shifted binning
+    v0 = value.copy()
+    v0[0, 0, 0] = 0.
+    assert np.allclose(bin(value, remove_zero=True)[ok], _bruteforce_oversampled(attrs.kcoords('wavenumber'), edges, wz, m, v0)[2][ok],
+                       rtol=1e-12)
+
+
+def test_mode_oversampling_wide_bin_and_no_shift():
+    """One bin holding every shifted mode: nmodes is the weighted mode count (a float), and the
+    binned value is the w-weighted mean whatever the shifts.  m = 0 keeps the integer counts and a
+    single index array."""
+    attrs = po.Attrs((4, 6, 6), (12., 18., 15.), 0.)
+    rng = np.random.default_rng(5)
+    value = rng.normal(size=(4, 6, 4))
+    w = po.hermitian_weights(attrs)
+    bin = po.BinPoles(attrs, edges=np.array([0., 1e3]), mode_oversampling=1)
+    assert bin.nmodes.dtype == np.float64 and bin.nmodes[0] == w.sum()
+    assert np.isclose(bin(value)[0], (w * value).sum() / w.sum(), rtol=1e-14)
+    plain = po.BinPoles(attrs, edges=np.arange(0., 3.01, 0.2), mode_oversampling=0)
+    assert plain.nmodes.dtype == np.int64 and isinstance(plain.ibin, np.ndarray)
+
+
+@pytest.mark.parametrize('m', [1])
+def test_correlation_mode_oversampling_bruteforce(m):
+    """BinCorrelationPoles(mode_oversampling=m): the same loops over the full real mesh, every
+    cell weighted 1, shifts in Mpc/h."""
+    attrs = po.Attrs((6, 4, 5), (9., 6., 7.5), 0.)
+    edges = np.arange(0., 6.01, 0.5)
+    rng = np.random.default_rng(7)
+    value = rng.normal(size=(6, 4, 5))
+    bin = po.BinCorrelationPoles(attrs, edges=edges, mode_oversampling=m)
+    nmodes, xavg, binned = _bruteforce_oversampled(po.separation_coords(attrs), edges, np.ones(5), m, value)
+    assert np.array_equal(bin.nmodes, nmodes)
+    ok = nmodes > 0
+    assert ok.sum() >= 8 and np.allclose(bin.xavg[ok], xavg[ok], rtol=1e-13)
+    assert np.allclose(bin(value)[ok], binned[ok].real, rtol=1e-12)
+
+
 def test_find_unique_edges():
     attrs = po.Attrs(meshsize=8, boxsize=2 * np.pi * 8)
     edges = po.find_unique_edges(attrs, xmax=0.3)
