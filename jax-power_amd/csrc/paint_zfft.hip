@@ -365,6 +365,43 @@ __device__ __forceinline__ int to_fixed(float v) {
   return r;
 }
 
+// Stencil weights reordered per lane, k'[q] = k[(q + rot) mod ORDER], with the cell offsets
+// reordered alike by the caller.  Lanes of one wave whose particles share stencil cells (a dense
+// clump) then add to different cells in the same LDS atomic instruction instead of serialising
+// on one address: lane l rotates z by l mod ORDER and y by (l / ORDER) mod ORDER (measured on
+// lognormal mocks, TSC: 256^3 with 2.25e6 particles 0.194 -> 0.126 ms, 512^3 with 1.6e7
+// particles 1.16 -> 1.09 ms).  The sums are unchanged: the adds are exact integers.
+template <int ORDER>
+__device__ __forceinline__ void rotate_stencil(float (&k)[ORDER], int rot) {
+  float r[ORDER];
+#pragma unroll
+  for (int q = 0; q < ORDER; ++q) {
+    float v = k[q];
+#pragma unroll
+    for (int t = 1; t < ORDER; ++t)
+      if (rot == t) v = k[(q + t) % ORDER];
+    r[q] = v;
+  }
+#pragma unroll
+  for (int q = 0; q < ORDER; ++q) k[q] = r[q];
+}
+
+template <int ORDER>
+struct LaneRotation {
+  int zo[ORDER], yo[ORDER];  // cell offset of stencil slot q along z and y
+  int zrot, yrot;
+  __device__ __forceinline__ explicit LaneRotation(int tid) {
+    const int lane = tid & 63;
+    zrot = lane % ORDER;
+    yrot = (lane / ORDER) % ORDER;
+#pragma unroll
+    for (int q = 0; q < ORDER; ++q) {
+      zo[q] = (q + zrot) % ORDER;
+      yo[q] = (q + yrot) % ORDER;
+    }
+  }
+};
+
 // One workgroup: RX x RY rows of shifted mesh j, painted in LDS, then z-FFT'd (r2c) in LDS.
 template <int ORDER, int M, int RX, int RY>
 __global__ void __launch_bounds__(PZ_NT)
@@ -488,6 +525,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   // coalescing of the record loads, gave 1.22 ms).
   constexpr int G = 16;
   const int Rg = (total + G - 1) / G;
+  const LaneRotation<ORDER> rot(tid);
   for (int ci = tid; ci < G * Rg; ci += PZ_NT) {
     const int c = (ci % G) * Rg + ci / G;
     if (c >= total) continue;
@@ -517,13 +555,16 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
         const int cx0 = stencil_1d<ORDER>(rec.x + s, hxr, kx);
         const int cy0 = stencil_1d<ORDER>(rec.y + s, hyr, ky);
         const int cz0 = stencil_1d<ORDER>(fz + s, zc, kz);
-        // cell (cx0 + ox, cy0 + oy, z) is word 2 ((cx0 + ox) RY + cy0 + oy) + zoff(z): one base
-        // per z cell (wrapped into the row), and the (ox, oy) part a constant offset.  The
-        // dynamic LDS follows the static tables, so a base below the first row stays >= 0.
+        rotate_stencil<ORDER>(ky, rot.yrot);
+        rotate_stencil<ORDER>(kz, rot.zrot);
+        // cell (cx0 + ox, cy0 + yo, z) is word 2 ((cx0 + ox) RY + cy0 + yo) + zoff(z): one base
+        // per z cell (wrapped into the row), the y part per lane and the x part a constant
+        // offset.  The dynamic LDS follows the static tables, so a base below the first row
+        // stays >= 0.
         int* zbase[ORDER];
 #pragma unroll
         for (int oz = 0; oz < ORDER; ++oz) {
-          const int z = (cz0 + oz) & (NZ - 1);
+          const int z = (cz0 + rot.zo[oz]) & (NZ - 1);
           int w = (z & ~1) * RP + (z & 1) + 2 * (cx0 * RY + cy0);
           asm volatile("" : "+v"(w));  // computed once: the atomics below use it with constant offsets
           zbase[oz] = irows + w;
@@ -535,10 +576,11 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
           const float wx = ws * kx[ox];
 #pragma unroll
           for (int oy = 0; oy < ORDER; ++oy) {
-            if ((unsigned)(cy0 + oy) >= (unsigned)RY) continue;
+            if ((unsigned)(cy0 + rot.yo[oy]) >= (unsigned)RY) continue;
             const float kxy = wx * ky[oy];
 #pragma unroll
-            for (int oz = 0; oz < ORDER; ++oz) atomicAdd(zbase[oz] + 2 * (ox * RY + oy), to_fixed(kxy * kz[oz]));
+            for (int oz = 0; oz < ORDER; ++oz)
+              atomicAdd(zbase[oz] + 2 * rot.yo[oy] + 2 * ox * RY, to_fixed(kxy * kz[oz]));
           }
         }
       }
@@ -658,6 +700,7 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   const int rxmax = min(RX, rg.n[0] - x0), rymax = min(RY, rg.n[1] - y0);  // partial blocks at the edge
   constexpr int G = 16;  // chunks taken from G regions in turn, as in the fused paint
   const int Rg = (total + G - 1) / G;
+  const LaneRotation<ORDER> rot(tid);
   for (int ci = tid; ci < G * Rg; ci += PR_NT) {
     const int c = (ci % G) * Rg + ci / G;
     if (c >= total) continue;
@@ -686,10 +729,12 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
       const int cx0 = stencil_1d<ORDER>(rec.x + s, hxr, kx);
       const int cy0 = stencil_1d<ORDER>(rec.y + s, hyr, ky);
       const int cz0 = stencil_1d<ORDER>(fz + s, zc, kz);
+      rotate_stencil<ORDER>(ky, rot.yrot);
+      rotate_stencil<ORDER>(kz, rot.zrot);
       int zoff[ORDER];
 #pragma unroll
       for (int oz = 0; oz < ORDER; ++oz) {
-        const int z = cz0 + oz;
+        const int z = cz0 + rot.zo[oz];
         zoff[oz] = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
       }
       const float ws = rec.w * scale;
@@ -700,7 +745,7 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
         const float wx = ws * kx[ox];
 #pragma unroll
         for (int oy = 0; oy < ORDER; ++oy) {
-          const int cy = cy0 + oy;
+          const int cy = cy0 + rot.yo[oy];
           if (!(cy >= 0 && cy < rymax)) continue;
           int* row = irows + (cx * RY + cy) * rs;
           const float kxy = wx * ky[oy];

@@ -1087,11 +1087,18 @@ def _iter_meshes(*inputs, resampler='cic', interlacing=0, compensate=False, **kw
             particles.append(inp)
     if particles and attrs:
         _, new = get_halo_for_new_mattrs(particles[0].attrs, **attrs)
-        particles = [p.clone(attrs=new) for p in particles]
+        clones = {}
+        for p in particles:
+            if id(p) not in clones:
+                clones[id(p)] = p.clone(attrs=new)
+        particles = [clones[id(p)] for p in particles]
     for mesh in meshes:
         yield mesh
+    painted = {}  # the same particles passed several times (auto normalizations) are painted once
     for p in particles:
-        yield p.paint(resampler=resampler, interlacing=interlacing, compensate=compensate)
+        if id(p) not in painted:
+            painted[id(p)] = p.paint(resampler=resampler, interlacing=interlacing, compensate=compensate)
+        yield painted[id(p)]
 
 
 def compute_normalization(*inputs, resampler='cic', start=1., **kwargs):

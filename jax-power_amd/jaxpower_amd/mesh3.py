@@ -17,8 +17,9 @@ import torch
 
 from . import _lib
 from .mesh import (MeshAttrs, RealMeshField, ComplexMeshField, ParticleField, FKPField, format_meshes, compute_box_normalization,
+                   compute_normalization,
                    _as_spectrum, _default_device, _empty_spectrum, _spectrum_pitch, _sum_f64)
-from .mesh2 import BinMesh2SpectrumPoles, _get_los_vector, _ylm_accumulate
+from .mesh2 import BinMesh2SpectrumPoles, _deferred, _get_los_vector, _ylm_accumulate
 from .types import Deferred, Mesh3SpectrumPole, Mesh3SpectrumPoles
 
 _BASES = ('sugiyama', 'sugiyama-diagonal', 'scoccimarro', 'scoccimarro-diagonal')
@@ -274,6 +275,26 @@ def compute_box3_normalization(*inputs, bin=None):
     normalization of the three (padded) inputs, once per ell when bin is given."""
     inputs, _ = format_meshes(*inputs, nmeshes=3)
     norm = compute_box_normalization(*inputs)
+    if bin is not None:
+        return [norm] * len(bin.ells)
+    return norm
+
+
+def compute_fkp3_normalization(*fkps, bin=None, cellsize=10., split=None, fields=None, **kwargs):
+    """FKP normalization of the bispectrum (mesh3.py:738-783), split=None: alpha, the product
+    over the three (padded) inputs of sum(data w) / sum(randoms w) (1 for a ParticleField),
+    times compute_normalization of the three randoms, CIC-painted on a mesh regridded to
+    ``cellsize`` (sum R1 R2 R3 / cellvolume^2).  Once per ell when bin is given."""
+    if split is not None:
+        raise NotImplementedError('split normalisation needs the reference random streams; use split=None')
+    kwargs.update(cellsize=cellsize)
+    fkps, fields = format_meshes(*fkps, fields=fields, nmeshes=3)
+    alpha = 1.
+    for fkp in fkps:
+        if isinstance(fkp, FKPField):
+            alpha = alpha * (fkp.data.sum() / fkp.randoms.sum())
+    randoms = [fkp.randoms if isinstance(fkp, FKPField) else fkp for fkp in fkps]
+    norm = _deferred(alpha * compute_normalization(*randoms, **kwargs))
     if bin is not None:
         return [norm] * len(bin.ells)
     return norm

@@ -596,6 +596,16 @@ def fkp_normalization(attrs, data_positions, data_weights, randoms_positions, ra
     return alpha * np.sum(d * r) * np.prod(new.cellsize)**(-1)
 
 
+def fkp3_normalization(attrs, data_positions, data_weights, randoms_positions, randoms_weights, cellsize=10.):
+    """compute_fkp3_normalization (mesh3.py:738-783), one field, split=None: alpha^3 *
+    sum(R^3) / cellvolume^2 with R CIC-painted on a regridded mesh (mesh.py:3092-3118), alpha =
+    sum(w_data) / sum(w_randoms) once per each of the three (identical) inputs."""
+    alpha = np.sum(data_weights) / np.sum(randoms_weights)
+    new = attrs if cellsize is None else regrid_attrs(attrs, cellsize)
+    r = paint(new, randoms_positions, randoms_weights, resampler='cic', interlacing=0, compensate=False)
+    return alpha**3 * np.sum(r * r * r) * np.prod(new.cellsize)**(-2)
+
+
 def fkp_shotnoise(data_weights, randoms_weights=None):
     """compute_fkp2_shotnoise (mesh2.py:577-621): sum of squared FKP weights
     [w_d, -alpha w_r] (mesh.py:2965-2971)."""

@@ -368,6 +368,24 @@ def test_fkp_normalization_matches_oracle():
         assert np.isclose(got, want, rtol=1e-5), (cellsize, got, want)
 
 
+def test_fkp3_normalization_matches_oracle():
+    from jaxpower_amd import MeshAttrs, ParticleField, FKPField, compute_fkp3_normalization, BinMesh3SpectrumPoles
+    say('fkp3 normalization')
+    rng = np.random.default_rng(13)
+    attrs = MeshAttrs(meshsize=64, boxsize=1005., boxcenter=(10., 20., 30.))
+    d = rng.uniform(-400., 400., (20000, 3)) + attrs.boxcenter
+    r = rng.uniform(-400., 400., (60000, 3)) + attrs.boxcenter
+    wd, wr = rng.uniform(0.5, 1., 20000).astype('f4'), rng.uniform(0.5, 1., 60000).astype('f4')
+    fkp = FKPField(ParticleField(to_dev(d), to_dev(wd), attrs=attrs), ParticleField(to_dev(r), to_dev(wr), attrs=attrs))
+    for cellsize in (10., None):
+        got = float(compute_fkp3_normalization(fkp, cellsize=cellsize))
+        want = po.fkp3_normalization(po.Attrs(64, 1005., (10., 20., 30.)), d, wd.astype('f8'), r, wr.astype('f8'), cellsize=cellsize)
+        assert np.isclose(got, want, rtol=1e-5), (cellsize, got, want)
+    bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.02}, ells=(0, 2), basis='scoccimarro')
+    norms = compute_fkp3_normalization(fkp, fkp, fkp, bin=bin)
+    assert len(norms) == 2 and np.isclose(float(norms[1]), float(compute_fkp3_normalization(fkp)), rtol=1e-12)
+
+
 def test_lognormal_reduced_tsc3():
     """The headline recipe at reduced size (128^3, L=500, ~4e5 particles), TSC x3, los z."""
     from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles, compute_mesh2_spectrum, generate_lognormal_particles

@@ -559,6 +559,9 @@ def test_normalization_closed_forms():
     R = np.zeros((10, 10, 10)); np.add.at(R, tuple(nodes[150:].T), wr)
     norm = po.fkp_normalization(attrs, pos_d, wd, pos_r, wr, cellsize=None)
     assert np.isclose(norm, wd.sum() / wr.sum() * np.sum(D * R) / 1000.)
+    # bispectrum: alpha^3 sum(R^3) / Vcell^2 (mesh3.py:738-783)
+    norm3 = po.fkp3_normalization(attrs, pos_d, wd, pos_r, wr, cellsize=None)
+    assert np.isclose(norm3, (wd.sum() / wr.sum())**3 * np.sum(R**3) / 1000.**2)
     # box normalisation (sum w)^2 / (N^3 Vcell)
     assert np.isclose(po.box_normalization(attrs, wd), wd.sum()**2 / (1000 * 1000.))
     # shot noise: sum of squared FKP weights [w_d, -alpha w_r] (tests/test_mesh2.py:75-91)
