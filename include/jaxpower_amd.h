@@ -243,6 +243,14 @@ size_t mp_bispec_shells_workspace_bytes(const int32_t meshsize[3]);
 int mp_bispec_shells(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double* kx, const double* ky,
                      const double* kz, const double* k2_thresholds, int nedges, int b0, int nshell, const double los[3], int ell,
                      float* out, void* workspace, size_t workspace_bytes);
+/* The Sugiyama basis' shell meshes (jaxpower/mesh3.py:595-621: bin(A Y*_l1m1(khat), ...) with
+ * complex harmonics, mesh3.py:599-606), by components: out + q * N0 N1 N2 = Q_q, a real mesh
+ * with ifftn(A [bin == b0 + q] Y_lm(khat)) = Q_q (even l) or i Q_q (odd l), Y_lm the real
+ * harmonic of jaxpower/utils.py:191-334 (l <= 4); A null: the shell indicators.  Workspace as
+ * mp_bispec_shells. */
+int mp_bispec_shells_ylm(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double* kx, const double* ky,
+                         const double* kz, const double* thr, int nedges, int b0, int nshell, int ell, int m, float* out,
+                         void* workspace, size_t workspace_bytes);
 /* The closed-triangle reduction of BinMesh3SpectrumPoles.bin_reduce_meshs with
  * reduce = prod(meshes).sum() (jaxpower/mesh3.py:238-275, 307-311): out[t] = sum over the ncells
  * cells of s1[b1] s2[b2] s3[b3] for (b1, b2, b3) = tri[3t .. 3t+2] (a device int32 array sorted by

@@ -59,6 +59,31 @@ shell_split_kernel(const float2* __restrict__ A, int64_t pitch, int n0, int n1, 
   }
 }
 
+// out = A(k) [bin(k) == b] Y_lm(khat) (-i if l is odd) on the half spectrum (A null: the shell
+// indicator).  Y_lm is the real harmonic (value at the zero vector as the reference's), so for
+// even l the product is hermitian and for odd l the factor -i makes it so: the inverse real FFT
+// then gives Q with ifftn(A [b] Y_lm) = Q (even l) or i Q (odd l) exactly.
+__global__ void __launch_bounds__(SH_NT)
+shell_split_ylm_kernel(const float2* __restrict__ A, int64_t pitch, int n0, int n1, int nc, const double* __restrict__ kx,
+                       const double* __restrict__ ky, const double* __restrict__ kz, const double* __restrict__ thr, int nedges,
+                       int b, int ell, int m, float2* __restrict__ out, int64_t out_pitch) {
+  const int64_t total = (int64_t)n0 * n1 * nc;
+  for (int64_t i = (int64_t)blockIdx.x * SH_NT + threadIdx.x; i < total; i += (int64_t)gridDim.x * SH_NT) {
+    const int iz = (int)(i % nc);
+    const int64_t row = i / nc;
+    const int iy = (int)(row % n1), ix = (int)(row / n1);
+    const double kxv = kx[ix], kyv = ky[iy], kzv = kz[iz];
+    const int nb = first_bin(thr, nedges, k2_sum(k2_xy(kxv, kyv), kzv)) - 1;
+    float2 v = make_float2(0.f, 0.f);
+    if (nb == b) {
+      const float2 a = A ? A[row * pitch + iz] : make_float2(1.f, 0.f);
+      const float y = ylm_of_vector(ell, m, (float)kxv, (float)kyv, (float)kzv);
+      v = (ell & 1) ? make_float2(a.y * y, -a.x * y) : make_float2(a.x * y, a.y * y);
+    }
+    out[row * out_pitch + iz] = v;
+  }
+}
+
 struct TriArgs {
   const float* stack[3];  // distinct shell stacks (up to three), shell q of stack s at stack[s] + q * shell_stride
   int nshell[3];
@@ -187,6 +212,39 @@ MP_EXPORT int mp_bispec_shells(void* stream, const void* A, int64_t pitch, const
     MP_LAUNCH("bispec_shell_split", shell_split_kernel, dim3((unsigned)blocks), dim3(SH_NT), 0, s, (const float2*)A, pitch, meshsize[0],
               meshsize[1], nc, kx, ky, kz, thr, nedges, b0 + q, lx, ly, lz, leg, ell > 0 ? 1 : 0, scratch, sp);
     rc = irfft3d(s, scratch, sp, out + q * cells, meshsize, scratch, sp);  // in place: each pass rewrites the tiles it read
+    if (rc) return rc;
+  }
+  return MP_OK;
+}
+
+// Real meshes of the shells b0 .. b0 + nshell - 1 weighted by the real harmonic Y_lm(khat):
+// out + q * N0 N1 N2 = Q_q with ifftn(A [bin == b0 + q] Y_lm(khat)) = Q_q for even l and i Q_q
+// for odd l (A null: the shell indicators).  These are the components of the Sugiyama basis'
+// shell meshes ifftn(A [b] Y*_lm(khat)) in the complex harmonics (mesh3.py:599-606).
+MP_EXPORT int mp_bispec_shells_ylm(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double* kx,
+                                   const double* ky, const double* kz, const double* thr, int nedges, int b0, int nshell, int ell,
+                                   int m, float* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && thr && out, "mp_bispec_shells_ylm: null pointer");
+  MP_REQUIRE(nedges >= 2 && b0 >= 0 && nshell >= 1 && b0 + nshell <= nedges - 1,
+             "mp_bispec_shells_ylm: shells %d..%d outside the %d bins", b0, b0 + nshell - 1, nedges - 1);
+  MP_REQUIRE(ell >= 0 && ell <= MAX_ELL && m >= -ell && m <= ell, "mp_bispec_shells_ylm: (ell, m) = (%d, %d) not supported", ell, m);
+  MP_REQUIRE(!A || pitch >= meshsize[2] / 2 + 1, "mp_bispec_shells_ylm: pitch too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_bispec_shells_ylm: workspace too small");
+  const int nc = meshsize[2] / 2 + 1;
+  const int64_t total = (int64_t)meshsize[0] * meshsize[1] * nc;
+  int ncu = 0;
+  int rc = multiprocessor_count(&ncu);
+  if (rc) return rc;
+  const int blocks = (int)std::min<int64_t>(ceil_div(total, SH_NT), (int64_t)ncu * 16);
+  const int64_t cells = (int64_t)meshsize[0] * meshsize[1] * meshsize[2];
+  for (int q = 0; q < nshell; ++q) {
+    MP_LAUNCH("bispec_shell_split", shell_split_ylm_kernel, dim3((unsigned)blocks), dim3(SH_NT), 0, s, (const float2*)A, pitch,
+              meshsize[0], meshsize[1], nc, kx, ky, kz, thr, nedges, b0 + q, ell, m, scratch, sp);
+    rc = irfft3d(s, scratch, sp, out + q * cells, meshsize, scratch, sp);
     if (rc) return rc;
   }
   return MP_OK;

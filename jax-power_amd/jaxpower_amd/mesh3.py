@@ -8,10 +8,18 @@ the GPU (``mp_bispec_shells``), and the closed-triangle sums sum_x I_b1 I_b2 I_b
 allowed (b1, b2, b3) are taken in one pass over the cells (``mp_bispec_triangles``), FP32 per
 cell and FP64 across cells.  The mode counts are the same triangle sums of the shell
 indicators (mesh3.py:229-236).  With the local line of sight the third side is
-sum_m r2c(rho_3 Y_lm(xhat)) Y_lm(khat) (mesh3.py:563-576).  The Sugiyama basis and the
-three-point correlation function are not implemented on this path and raise
-NotImplementedError.
+sum_m r2c(rho_3 Y_lm(xhat)) Y_lm(khat) (mesh3.py:563-576).
+
+The Sugiyama basis (the reference's default) bins pairs of shells (k1, k2): for each
+(l1, l2, L) and each (m1, m2, M) of nonzero Gaunt coefficient g (mesh3.py:443-471),
+g N^6 sum_x ifftn(A1 [b1] Y*_l1m1) ifftn(A2 [b2] Y*_l2m2) Y*_LM(los) rho_3 (mesh3.py:595-621).
+The complex harmonics are combinations of two real ones, and ifftn(A [b] Y_lm) of a real
+harmonic is a real mesh (even l) or i times one (odd l), so every term is a sum of real
+triple products taken by the same triangle pass.  The three-point correlation function is
+not implemented on this path and raises NotImplementedError.
 """
+import math
+
 import numpy as np
 import torch
 
@@ -68,6 +76,86 @@ def _mask_function(mask_edges, nyq):
     return mask
 
 
+def _wigner_3j(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol of integers by Racah's formula (the reference calls sympy's,
+    utils.py:634-637)."""
+    if m1 + m2 + m3 != 0 or not (abs(j1 - j2) <= j3 <= j1 + j2) or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.
+    f = math.factorial
+    tri = f(j1 + j2 - j3) * f(j1 - j2 + j3) * f(-j1 + j2 + j3) / f(j1 + j2 + j3 + 1)
+    pre = (-1.)**(j1 - j2 - m3) * math.sqrt(tri * f(j1 + m1) * f(j1 - m1) * f(j2 + m2) * f(j2 - m2) * f(j3 + m3) * f(j3 - m3))
+    total = 0.
+    for k in range(max(0, j2 - j3 - m1, j1 - j3 + m2), min(j1 + j2 - j3, j1 - m1, j2 + m2) + 1):
+        total += (-1.)**k / (f(k) * f(j3 - j2 + k + m1) * f(j3 - j1 + k - m2) * f(j1 + j2 - j3 - k) * f(j1 - k - m1) * f(j2 - k + m2))
+    return pre * total
+
+
+def _triposh_rows(ell1, ell2, ell3, los_z=False):
+    """(m1, m2, M, g) with g = (2l1+1)(2l2+1)(2L+1) W(l1 l2 L; 000) W(l1 l2 L; m1 m2 M) of modulus
+    >= 1e-7, M = 0 only for the line of sight 'z' (_iter_triposh, mesh3.py:443-471).  The
+    reference's pairing of opposite m triples (mesh3.py:462-468) compares a tuple with lists and
+    never matches, so every row enters once with its own coefficient."""
+    rows = []
+    for m1 in range(-ell1, ell1 + 1):
+        for m2 in range(-ell2, ell2 + 1):
+            for m3 in ([0] if los_z else range(-ell3, ell3 + 1)):
+                g = (2 * ell1 + 1) * (2 * ell2 + 1) * (2 * ell3 + 1) * _wigner_3j(ell1, ell2, ell3, 0, 0, 0) * \
+                    _wigner_3j(ell1, ell2, ell3, m1, m2, m3)
+                if abs(g) >= 1e-7:
+                    rows.append((m1, m2, m3, g))
+    return rows
+
+
+def _ylm_components(ell, m):
+    """get_Ylm(ell, m, reduced=True, real=False, conj=True) (utils.py:191-334) as
+    sum_mu c_mu Y_l,mu of the real harmonics (get_Ylm(real=True)): with a = sqrt((2l+1)/4pi),
+    m = 0: Y_l0 / a; m = p > 0: (-1)^p (Y_lp - i Y_l,-p) / (a sqrt 2); m = -p:
+    (Y_lp + i Y_l,-p) / (a sqrt 2).  The zero-vector values agree: only m = 0 harmonics are
+    nonzero there."""
+    a = math.sqrt((2 * ell + 1) / (4. * math.pi))
+    if m == 0:
+        return {0: 1. / a + 0j}
+    p = abs(m)
+    c = 1. / (a * math.sqrt(2.))
+    if m > 0:
+        return {p: (-1.)**p * c + 0j, -p: -1j * (-1.)**p * c}
+    return {p: c + 0j, -p: 1j * c}
+
+
+def _complex_ylm_conj_at(ell, m, v):
+    """get_Ylm(ell, m, reduced=True, real=False, conj=True) at a nonzero vector v:
+    sqrt((l-|m|)!/(l+|m|)!) [(-1)^m if m < 0] P_l^|m|(cos theta) exp(-i m phi), with the
+    Condon-Shortley phase in P_l^|m| as sympy's assoc_legendre (utils.py:223-238, 300-325)."""
+    from scipy.special import lpmv
+    v = np.asarray(v, dtype='f8')
+    x, y, z = v / np.linalg.norm(v)
+    amp = math.sqrt(math.factorial(ell - abs(m)) / math.factorial(ell + abs(m)))
+    if m < 0:
+        amp *= (-1.)**m
+    return complex(amp * lpmv(abs(m), ell, z) * np.exp(-1j * m * math.atan2(y, x)))
+
+
+def _full_grid_modes(axis, mattrs):
+    """Counts and sum |k| per shell over the full complex grid (the Sugiyama basis clones the
+    attributes to the complex dtype, mesh3.py:29-30, so no hermitian weights): the hermitian
+    counts of the half spectrum less one copy of the kz = +Nyquist plane, which the weights
+    1 + (kz > 0) count twice and the full grid holds once (at -Nyquist, same |k|)."""
+    counts = np.asarray(axis.nmodes, dtype='f8')
+    with np.errstate(invalid='ignore'):
+        xsum = np.where(counts > 0, np.nan_to_num(axis.xavg) * counts, 0.)
+    n = [int(x) for x in mattrs.meshsize]
+    if n[2] % 2 == 0:
+        from .mesh2 import _kvectors, _k2_thresholds
+        kx, ky, kz = _kvectors(mattrs)
+        k2 = (kx[:, None]**2 + ky[None, :]**2) + kz[-1]**2
+        b = np.searchsorted(_k2_thresholds(axis._edges1d), k2.ravel(), side='right') - 1
+        ok = (b >= 0) & (b < axis.nbins)
+        counts = counts - np.bincount(b[ok], minlength=axis.nbins)
+        xsum = xsum - np.bincount(b[ok], weights=np.sqrt(k2.ravel()[ok]), minlength=axis.nbins)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        return counts, xsum / counts
+
+
 def _product(arrays, diagonal):
     """mesh3.py:107-117: all (i, j, k) combinations in lexicographic order, or the diagonal."""
     if diagonal:
@@ -77,14 +165,15 @@ def _product(arrays, diagonal):
 
 
 class BinMesh3SpectrumPoles:
-    """Binning operator for the bispectrum (mesh3.py:180-313), basis 'scoccimarro' or
-    'scoccimarro-diagonal'.
+    """Binning operator for the bispectrum (mesh3.py:180-313), basis 'sugiyama' (the default),
+    'sugiyama-diagonal', 'scoccimarro' or 'scoccimarro-diagonal'.
 
-    Attributes mirror the reference: ``edges`` (ntri, 3, 2), ``xavg`` (ntri, 3), ``nmodes``
-    (one float array per ell: the closed-triangle counts), ``edges1d``, ``nmodes1d``,
-    ``xavg1d``, ``ells``, ``basis``.  ``batch_size`` and ``buffer_size`` (the reference's
-    memory/throughput knobs for jax.lax.map) are accepted and stored; the GPU path stages all
-    shells at once.
+    Attributes mirror the reference: ``edges`` (nbins, ndim, 2), ``xavg`` (nbins, ndim),
+    ``nmodes`` (one float array per ell: the closed-triangle counts in the Scoccimarro basis,
+    the products of the two shells' full-grid mode counts in the Sugiyama basis), ``edges1d``,
+    ``nmodes1d``, ``xavg1d``, ``ells``, ``basis``; ndim = 3 (Scoccimarro) or 2 (Sugiyama).
+    ``batch_size`` and ``buffer_size`` (the reference's memory/throughput knobs for
+    jax.lax.map) are accepted and stored; the GPU path stages the shells in groups of its own.
     """
 
     def __init__(self, mattrs, edges=None, ells=0, basis='sugiyama', batch_size=None, buffer_size=0, mask_edges=None, device=None):
@@ -92,11 +181,9 @@ class BinMesh3SpectrumPoles:
             mattrs = mattrs.attrs
         if basis not in _BASES:
             raise ValueError(f'basis must be one of {_BASES}, got {basis!r}')
-        if 'scoccimarro' not in basis:
-            raise NotImplementedError(f"basis={basis!r} (the reference's default is 'sugiyama') is not implemented on this "
-                                      "path: pass basis='scoccimarro' or 'scoccimarro-diagonal'")
         device = device or _default_device()
-        ndim = 3
+        sugiyama = 'sugiyama' in basis
+        ndim = 2 if sugiyama else 3
         diagonal = 'diagonal' in basis
         self.mattrs, self.basis, self._device = mattrs, basis, device
         self.batch_size = 1 if batch_size is None else int(batch_size)
@@ -116,8 +203,13 @@ class BinMesh3SpectrumPoles:
         axes = axes + [axes[-1]] * (ndim - len(axes))
         self._axes = axes
         self.edges1d = tuple(a.edges for a in axes)
-        self.nmodes1d = tuple(a.nmodes for a in axes)
-        self.xavg1d = tuple(a.xavg for a in axes)
+        if sugiyama:
+            full = [_full_grid_modes(a, mattrs) for a in axes]
+            self.nmodes1d = tuple(f[0] for f in full)
+            self.xavg1d = tuple(f[1] for f in full)
+        else:
+            self.nmodes1d = tuple(a.nmodes for a in axes)
+            self.xavg1d = tuple(a.xavg for a in axes)
         nyq = mattrs.kfun.min() * np.min(mattrs.meshsize) / 2.
         if mask_edges is None:
             mask_edges = _default_mask(basis)
@@ -131,6 +223,19 @@ class BinMesh3SpectrumPoles:
         self._iedges = _product([np.arange(len(x)) for x in self.xavg1d], diagonal)[keep].astype('i4')
         if not len(self._iedges):
             raise ValueError('no triangle passes the edge mask')
+        self.wmodes = None
+        if sugiyama:
+            # The complex-grid shells are reproduced from half spectra, which holds below the
+            # Nyquist planes (the default mask keeps every used shell there, mesh3.py:36-38).
+            knyq = float(np.min(np.pi / np.asarray(mattrs.cellsize, dtype='f8')))
+            top = max(float(self.edges1d[a][self._iedges[:, a], 1].max()) for a in range(ndim))
+            if top > knyq:
+                raise NotImplementedError(f'Sugiyama shells reaching {top:.4g} > the Nyquist wavenumber {knyq:.4g} are not '
+                                          'implemented on this path')
+            nmodes = np.prod(np.stack([self.nmodes1d[a][self._iedges[:, a]] for a in range(ndim)]), axis=0)
+            self._nmodes_dev = torch.from_numpy(nmodes).to(device)
+            self.nmodes = [nmodes] * len(self.ells)
+            return
         self._tri = torch.from_numpy(np.ascontiguousarray(self._iedges.ravel())).to(device)
         # closed-triangle counts: triangle sums of the shell indicators times N^6 (mesh3.py:229-236)
         stacks = self._shell_stacks([None] * ndim, [0] * ndim)
@@ -140,7 +245,6 @@ class BinMesh3SpectrumPoles:
         self._nmodes_dev = counts * norm
         nmodes = self._nmodes_dev.cpu().numpy()
         self.nmodes = [nmodes] * len(self.ells)
-        self.wmodes = None
 
     @property
     def nbins(self):
@@ -186,9 +290,68 @@ class BinMesh3SpectrumPoles:
     def _sums_dev(self, spectra, ells=(0, 0, 0), los=None):
         return self._triangles(self._shell_stacks(spectra, ells, los=los))
 
+    def _ylm_stack(self, side, spec, ell, mu, cache):
+        """Real meshes Q_b, one per shell b of the side, with ifftn(A [b] Y_l,mu(khat)) = Q_b
+        (even l) or i Q_b (odd l), for the half spectrum spec = (value, pitch) (None: the shell
+        indicators); ell = -1 gives the plain shells ifftn(A [b])."""
+        axis = self._axes[side]
+        key = (None if spec is None else spec[0].data_ptr(), id(axis), ell, mu)
+        if key in cache:
+            return cache[key]
+        n = _lib.i32x3(self.mattrs.meshsize)
+        shape = tuple(int(x) for x in self.mattrs.meshsize)
+        ws = _lib.workspace(_lib.load().mp_bispec_shells_workspace_bytes(n), self._device)
+        out = torch.empty((axis.nbins,) + shape, dtype=torch.float32, device=self._device)
+        t = axis._tables
+        args = (_lib.stream_handle(self._device), None if spec is None else _lib.ptr(spec[0]), 0 if spec is None else spec[1], n,
+                _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), _lib.ptr(t['thr']), len(axis._edges1d), 0, axis.nbins)
+        if ell < 0:
+            _lib.call('mp_bispec_shells', *args, _lib.f64x3((0., 0., 0.)), 0, _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        else:
+            _lib.call('mp_bispec_shells_ylm', *args, int(ell), int(mu), _lib.ptr(out), _lib.ptr(ws), ws.numel())
+        cache[key] = out
+        return out
+
+    def _pair_sums(self, Q0, Q1, H):
+        """sum_x Q0[b1](x) Q1[b2](x) H[j](x) for every pair (b1, b2) of the bin and every row j
+        of H: a (npairs, len(H)) float64 device tensor.  The shells are staged in groups that
+        fit one triangle pass."""
+        n3 = H.shape[0]
+        nb = (Q0.shape[0], Q1.shape[0])
+        budget = _MAX_SHELL_ROWS - n3
+        if budget < 2:
+            raise NotImplementedError(f'{n3} third-side meshes in one triangle pass')
+        if nb[0] + nb[1] <= budget:
+            groups = [(0, nb[0])], [(0, nb[1])]
+        else:
+            g = budget // 2
+            groups = [(lo, min(lo + g, nb[0])) for lo in range(0, nb[0], g)], [(lo, min(lo + g, nb[1])) for lo in range(0, nb[1], g)]
+        ie = self._iedges
+        out = torch.zeros((len(ie), n3), dtype=torch.float64, device=self._device)
+        ncells = int(np.prod(self.mattrs.meshsize))
+        for lo0, hi0 in groups[0]:
+            for lo1, hi1 in groups[1]:
+                sel = np.flatnonzero((ie[:, 0] >= lo0) & (ie[:, 0] < hi0) & (ie[:, 1] >= lo1) & (ie[:, 1] < hi1))
+                if not len(sel):
+                    continue
+                tri = np.stack([np.repeat(ie[sel, 0] - lo0, n3), np.repeat(ie[sel, 1] - lo1, n3), np.tile(np.arange(n3), len(sel))],
+                               axis=1).astype('i4')
+                tri_dev = torch.from_numpy(np.ascontiguousarray(tri.ravel())).to(self._device)
+                ntri = len(tri)
+                res = torch.empty(ntri, dtype=torch.float64, device=self._device)
+                ws = _lib.workspace(_lib.load().mp_bispec_triangles_workspace_bytes(ntri), self._device)
+                _lib.call('mp_bispec_triangles', _lib.stream_handle(self._device), _lib.ptr(Q0[lo0]), _lib.ptr(Q1[lo1]), _lib.ptr(H), ncells,
+                          hi0 - lo0, hi1 - lo1, n3, _lib.ptr(tri_dev), ntri, _lib.ptr(res), _lib.ptr(ws), ws.numel())
+                out[torch.from_numpy(sel).to(self._device)] = res.view(len(sel), n3)
+        return out
+
     def __call__(self, *meshes, remove_zero=False):
         """Triangle sums of three half spectra times N^6 (mesh3.py:277-313), as a float64
-        array; remove_zero sets each mesh's k = 0 mode to 0 first."""
+        array; remove_zero sets each mesh's k = 0 mode to 0 first.  In the Sugiyama basis the
+        third mesh is a configuration-space mesh, multiplied in as is (remove_zero subtracts
+        its mean) and the sums run over the pairs of shells of the first two."""
+        if 'sugiyama' in self.basis:
+            return self._call_sugiyama(meshes, remove_zero)
         if len(meshes) != 3:
             raise ValueError('the scoccimarro binning takes three meshes')
         spectra = []
@@ -203,21 +366,112 @@ class BinMesh3SpectrumPoles:
         return (self._sums_dev(spectra) * norm).cpu().numpy()
 
 
+    def _call_sugiyama(self, meshes, remove_zero):
+        if len(meshes) != 3:
+            raise ValueError('the sugiyama binning takes three meshes')
+        spectra = []
+        for mesh in meshes[:2]:
+            value = mesh.value if isinstance(mesh, ComplexMeshField) else mesh
+            _lib.require_device(value, 'mesh')
+            if remove_zero:
+                value = value.clone()
+                value[0, 0, 0] = 0.
+            spectra.append(_as_spectrum(value))
+        third = meshes[2].value if isinstance(meshes[2], RealMeshField) else meshes[2]
+        third = third.to(torch.float32).contiguous()
+        if remove_zero:
+            third = third - third.double().mean().float()
+        cache = {}
+        Q0 = self._ylm_stack(0, spectra[0], -1, 0, cache)
+        Q1 = self._ylm_stack(1, spectra[1], -1, 0, cache)
+        norm = float(np.prod(self.mattrs.meshsize, dtype='f8'))**2
+        return (self._pair_sums(Q0, Q1, third[None])[:, 0] * norm).cpu().numpy()
+
+
 def _2c(mesh):
     return mesh if isinstance(mesh, ComplexMeshField) else mesh.r2c()
 
 
+def _2r(mesh):
+    return mesh if isinstance(mesh, RealMeshField) else mesh.c2r()
+
+
+def _mesh3_spectrum_sugiyama(meshes, bin, mattrs, los):
+    """compute_mesh3_spectrum in the Sugiyama basis (mesh3.py:595-621), global or local line of
+    sight.  With Y*_lm = sum_mu c_mu Y_l,mu (real harmonics) and ifftn(A [b] Y_l,mu) = i^(l mod 2)
+    Q_b (mp_bispec_shells_ylm), each row (m1, m2, M, g) of each (l1, l2, L) contributes
+    g N^6 sum_mu1,mu2 c1 c2 i^(l1 mod 2 + l2 mod 2) sum_x Q1 Q2 H, H = Y*_LM(los) rho_3 (global)
+    or sum_mu3 c3 Y_L,mu3(xhat) rho_3 (local, positions from the origin); the real triple sums
+    come from the triangle pass, one per (mu1, mu2), and the complex coefficients are applied
+    on the host.  num_raw = real part (even l1 + l2) or imaginary part of the row sum / nmodes;
+    norm = N^3 / Vcell^2."""
+    local = isinstance(los, str) and los == 'local'
+    vlos = None if local else _get_los_vector(los)
+    device = bin._device
+    spectra = [_as_spectrum(_2c(m).value) for m in meshes[:2]]
+    rho = _2r(meshes[2]).value.to(torch.float32).contiguous()
+    norm = float(np.prod(mattrs.meshsize, dtype='f8')) / float(np.prod(mattrs.cellsize))**2
+    n6 = float(np.prod(mattrs.meshsize, dtype='f8'))**2
+    attrs = dict(los='local' if local else vlos.tolist())
+    cache, thirds = {}, {}
+    if local:
+        from .mesh2 import _mul_ylm
+        offset = np.asarray(mattrs.boxcenter) - np.asarray(mattrs.boxsize) / 2.
+        xs = [torch.from_numpy(np.arange(int(n)) * float(d) + float(o)).to(device)
+              for n, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]  # node positions (mesh.py:1594-1603)
+    poles = []
+    for ill, (ell1, ell2, ell3) in enumerate(bin.ells):
+        rows = _triposh_rows(ell1, ell2, ell3, los_z=isinstance(los, str) and los == 'z')
+        coef = {}  # (mu1, mu2) -> {row of H: complex coefficient}
+        phase = 1j**(ell1 % 2 + ell2 % 2)
+        for m1, m2, m3, g in rows:
+            h = _ylm_components(ell3, m3) if local else {0: _complex_ylm_conj_at(ell3, m3, vlos)}
+            for mu1, c1 in _ylm_components(ell1, m1).items():
+                for mu2, c2 in _ylm_components(ell2, m2).items():
+                    d = coef.setdefault((mu1, mu2), {})
+                    for mu3, c3 in h.items():
+                        j = mu3 + ell3 if local else 0
+                        d[j] = d.get(j, 0.) + g * c1 * c2 * c3 * phase
+        if local:
+            if ell3 not in thirds:
+                H = torch.empty((2 * ell3 + 1,) + tuple(rho.shape), dtype=torch.float32, device=device)
+                for mu3 in range(-ell3, ell3 + 1):
+                    _mul_ylm(rho, H[mu3 + ell3], mattrs, xs, ell3, mu3, False)
+                thirds[ell3] = H
+            H = thirds[ell3]
+        else:
+            H = rho[None]
+        total = np.zeros(bin.nbins, dtype=complex)
+        for (mu1, mu2), d in coef.items():
+            if all(abs(c) < 1e-14 for c in d.values()):
+                continue
+            Q0 = bin._ylm_stack(0, spectra[0], ell1, mu1, cache)
+            Q1 = bin._ylm_stack(1, spectra[1], ell2, mu2, cache)
+            T = bin._pair_sums(Q0, Q1, H).cpu().numpy()
+            for j, c in d.items():
+                total += c * T[:, j]
+        with np.errstate(invalid='ignore', divide='ignore'):
+            total = total * n6 / bin.nmodes[ill]
+        num = total.real if (ell1 + ell2) % 2 == 0 else total.imag
+        poles.append(Mesh3SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes[ill], num_raw=num, norm=norm,
+                                       ell=(ell1, ell2, ell3), attrs=attrs, basis=bin.basis))
+    return Mesh3SpectrumPoles(poles, attrs=attrs)
+
+
 def compute_mesh3_spectrum(*meshes, bin=None, los='x'):
-    """Bispectrum multipoles (Scoccimarro basis, global line of sight) of one to three meshes
-    (mesh3.py:509-629): num_raw_ell3 = (2 ell3 + 1) sum over closed triangles of
+    """Bispectrum multipoles of one to three meshes (mesh3.py:509-629).  Scoccimarro basis,
+    global line of sight: num_raw_ell3 = (2 ell3 + 1) sum over closed triangles of
     A1 A2 (A3 P_ell3(mu)), divided by the triangle counts; odd ell3 give 0 (the reference keeps
-    the imaginary part of a real sum); norm = N^3 / Vcell^2."""
+    the imaginary part of a real sum); local line of sight: see _mesh3_spectrum_local.  Sugiyama
+    basis: see _mesh3_spectrum_sugiyama.  norm = N^3 / Vcell^2."""
     if not isinstance(bin, BinMesh3SpectrumPoles):
         raise TypeError('bin must be a BinMesh3SpectrumPoles')
     if isinstance(los, str) and los in ('firstpoint', 'endpoint'):
         raise ValueError(f"the bispectrum line of sight is 'local' or a global one, got {los!r} (mesh3.py:433-440)")
     meshes, _ = format_meshes(*meshes, nmeshes=3)
     mattrs = meshes[0].attrs
+    if 'sugiyama' in bin.basis:
+        return _mesh3_spectrum_sugiyama(meshes, bin, mattrs, los)
     if isinstance(los, str) and los == 'local':
         return _mesh3_spectrum_local(meshes, bin, mattrs)
     vlos = _get_los_vector(los)

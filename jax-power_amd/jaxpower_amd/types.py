@@ -83,6 +83,17 @@ def _host(x):
     return np.asarray(x)
 
 
+def _format_ell(ell):
+    """An int multipole, or a tuple of ints for the Sugiyama bispectrum's (l1, l2, L)."""
+    if np.ndim(ell):
+        return tuple(int(v) for v in np.ravel(ell))
+    return int(ell)
+
+
+def _ell_key(ell):
+    return '_'.join(str(v) for v in ell) if isinstance(ell, tuple) else str(ell)
+
+
 class _MeshPole:
     """One multipole of a binned two-point measurement; ``_x`` names its coordinate ('k' for
     spectra, 's' for correlation functions)."""
@@ -98,7 +109,7 @@ class _MeshPole:
         # fields may be device tensors or Deferred values: they are read back on first access
         self._raw = {name: fields.get(name) for name in names}
         self._cache = None
-        self.ell = int(ell)
+        self.ell = _format_ell(ell)
         self.attrs = dict(attrs or {})
 
     @classmethod
@@ -231,8 +242,8 @@ class _MeshPoles:
     def get(self, ells=None, **kwargs):
         if ells is None:
             ells = kwargs.get('ell')
-        if isinstance(ells, (list, tuple)):
-            return type(self)([self.get(ell) for ell in ells])
+        if isinstance(ells, (list, tuple)) and not any(p.ell == ells for p in self._branches):
+            return type(self)([self.get(ell) for ell in ells])  # a list of multipoles (a Sugiyama (l1, l2, L) matches first)
         for p in self._branches:
             if p.ell == ells:
                 return p
@@ -274,7 +285,7 @@ class _MeshPoles:
         arrays = dict(extra or {})
         for p in self._branches:
             for name, v in p._data.items():
-                arrays[f'ell{p.ell}_{name}'] = v
+                arrays[f'ell{_ell_key(p.ell)}_{name}'] = v
         arrays['ells'] = np.array(self.ells)
         arrays['kind'] = np.array(type(self).__name__)
         los = self.attrs.get('los')
@@ -350,8 +361,9 @@ def read(fn):
         extra = {'basis': str(data['basis'])} if 'basis' in data.files else {}
         poles = []
         for ell in data['ells']:
-            kw = {name: data[f'ell{ell}_{name}'] for name in cls._pole._field_names()}
-            poles.append(cls._pole(**kw, ell=int(ell), attrs=attrs, **extra))
+            ell = _format_ell(ell)
+            kw = {name: data[f'ell{_ell_key(ell)}_{name}'] for name in cls._pole._field_names()}
+            poles.append(cls._pole(**kw, ell=ell, attrs=attrs, **extra))
     return cls(poles, attrs=attrs)
 
 

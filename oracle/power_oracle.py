@@ -1739,3 +1739,208 @@ def bispectrum_local_mesh_bruteforce(attrs, rho, ell):
                 phase = np.exp(-2j * np.pi * (idx @ np.array([freq[0][i] / n[0], freq[1][j] / n[1], freq[2][l] / n[2]])))
                 out[i, j, l] = (2 * ell + 1) / (4. * np.pi) * np.sum(flat * phase * leg)
     return out
+
+
+# ----------------------------------------------------------------------------
+# Bispectrum, Sugiyama basis (mesh3.py:23-177, 180-313, 443-471, 509-629)
+# ----------------------------------------------------------------------------
+
+def ref_real_ylm(ell, m, x, y, z):
+    """The reference's real Y_lm (get_Ylm(ell, m, real=True), utils.py:191-334) including its
+    per-m sign: the normalisation of utils.py:227-238 carries (-1)^m and sympy's associated
+    Legendre function the Condon-Shortley phase, so it is (-1)^m real_ylm for m != 0.  Pinned
+    against the sympy construction in tests/test_oracle.py."""
+    out = real_ylm(ell, m, x, y, z)
+    return out if m == 0 else (-1.)**m * out
+
+
+def complex_ylm_conj(ell, m, x, y, z):
+    """get_Ylm(ell, m, reduced=True, real=False, conj=True) (utils.py:191-334): the conjugate of
+    sqrt((l-|m|)!/(l+|m|)!) [(-1)^m for m < 0] P_l^|m|(cos theta) exp(i m phi), written through
+    the real harmonics: with a = sqrt((2l+1)/4pi), m = 0: Y_l0 / a; m = p > 0:
+    (-1)^p (Y_lp - i Y_l,-p) / (a sqrt 2); m = -p: (Y_lp + i Y_l,-p) / (a sqrt 2).  At the zero
+    vector this is the constant term of the sympy polynomial, as for the real harmonics.  Pinned
+    against the sympy construction in tests/test_oracle.py."""
+    a = np.sqrt((2 * ell + 1) / (4. * np.pi))
+    if m == 0:
+        return ref_real_ylm(ell, 0, x, y, z) / a + 0j
+    p = abs(m)
+    re, im = ref_real_ylm(ell, p, x, y, z), ref_real_ylm(ell, -p, x, y, z)
+    if m > 0:
+        return (-1.)**p * (re - 1j * im) / (a * np.sqrt(2.))
+    return (re + 1j * im) / (a * np.sqrt(2.))
+
+
+def wigner_3j(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol of integers by Racah's formula (the reference calls sympy's,
+    utils.py:634-637).  Pinned against sympy in tests/test_oracle.py."""
+    j1, j2, j3, m1, m2, m3 = (int(v) for v in (j1, j2, j3, m1, m2, m3))
+    if m1 + m2 + m3 != 0 or not (abs(j1 - j2) <= j3 <= j1 + j2) or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.
+    f = math.factorial
+    delta = f(j1 + j2 - j3) * f(j1 - j2 + j3) * f(-j1 + j2 + j3) / f(j1 + j2 + j3 + 1)
+    pre = (-1.)**(j1 - j2 - m3) * math.sqrt(delta * f(j1 + m1) * f(j1 - m1) * f(j2 + m2) * f(j2 - m2) * f(j3 + m3) * f(j3 - m3))
+    kmin = max(0, j2 - j3 - m1, j1 - j3 + m2)
+    kmax = min(j1 + j2 - j3, j1 - m1, j2 + m2)
+    s = 0.
+    for k in range(kmin, kmax + 1):
+        s += (-1.)**k / (f(k) * f(j3 - j2 + k + m1) * f(j3 - j1 + k - m2) * f(j1 + j2 - j3 - k) * f(j1 - k - m1) * f(j2 - k + m2))
+    return pre * s
+
+
+def triposh_rows(ell1, ell2, ell3, los_z=False):
+    """_iter_triposh (mesh3.py:443-471): every (m1, m2, M) with coefficient
+    (2l1+1)(2l2+1)(2L+1) W(l1 l2 L; 0 0 0) W(l1 l2 L; m1 m2 M) of modulus >= 1e-7, M = 0 only for
+    the line of sight 'z'.  The reference's pairing of (m1, m2, M) with (-m1, -m2, -M) compares a
+    tuple with lists (mesh3.py:462-468) and never matches, so every row is kept with sym = 0."""
+    rows = []
+    ms3 = [0] if los_z else range(-ell3, ell3 + 1)
+    for m1 in range(-ell1, ell1 + 1):
+        for m2 in range(-ell2, ell2 + 1):
+            for m3 in ms3:
+                g = (2 * ell1 + 1) * (2 * ell2 + 1) * (2 * ell3 + 1) * wigner_3j(ell1, ell2, ell3, 0, 0, 0) * \
+                    wigner_3j(ell1, ell2, ell3, m1, m2, m3)
+                if abs(g) < 1e-7:
+                    continue
+                rows.append((m1, m2, m3, g))
+    return rows
+
+
+def full_kvec(attrs):
+    """Wavenumbers of the full (non-hermitian) FFT grid, fftfreq on every axis (mesh.py:361-403
+    with hermitian=False, as the complex-dtype attributes of the Sugiyama basis give)."""
+    n = [int(x) for x in attrs.meshsize]
+    return [np.fft.fftfreq(n[a]) * attrs.kfun[a] * n[a] for a in range(3)]
+
+
+class BinBispectrumSugiyama:
+    """BinMesh3SpectrumPoles with basis 'sugiyama' or 'sugiyama-diagonal' (mesh3.py:23-177,
+    180-313): two sides, each binning the modes of the full complex grid (the Sugiyama basis
+    clones the attributes to the complex dtype, mesh3.py:29-30, so no hermitian weights) by
+    digitize(|k|, edges) - 1; pairs (b1, b2) in lexicographic order (or the diagonal) that pass
+    the mask, by default mid1 <= mid2 and upper1 + upper2 <= nyq (mesh3.py:36-38); nmodes is the
+    product of the two one-dimensional counts and xavg the pair of their mean |k|."""
+
+    def __init__(self, attrs, edges=None, ells=(0, 0, 0), mask_edges=None, diagonal=False):
+        ndim = 2
+        if not isinstance(edges, (list, tuple)):
+            edges = [edges] * ndim
+        self.attrs = attrs
+        kk = full_kvec(attrs)
+        k2 = kk[0][:, None, None]**2 + kk[1][None, :, None]**2
+        knorm = np.sqrt(k2 + kk[2][None, None, :]**2)
+        self.ibin1d, self.nmodes1d, self.xavg1d, self.edges1d = [], [], [], []
+        for e in edges[:ndim]:
+            e = make_edges(attrs, e)
+            ib = np.digitize(knorm, e, right=False) - 1
+            nb = len(e) - 1
+            ok = (ib >= 0) & (ib < nb)
+            cnt = np.bincount(ib[ok], minlength=nb).astype('f8')
+            xs = np.bincount(ib[ok], weights=knorm[ok], minlength=nb)
+            with np.errstate(invalid='ignore', divide='ignore'):
+                self.xavg1d.append(xs / cnt)
+            self.ibin1d.append(ib)
+            self.nmodes1d.append(cnt)
+            self.edges1d.append(np.column_stack([e[:-1], e[1:]]))
+        self.ells = [tuple(int(v) for v in ell) for ell in (ells if np.ndim(ells[0]) else [ells])]
+
+        def prod(arrays):
+            if diagonal:
+                return np.column_stack([np.asarray(arrays[0])] * ndim)
+            grid = np.meshgrid(*arrays, indexing='ij')
+            return np.column_stack([g.ravel() for g in grid])
+
+        lo = prod([e[:, 0] for e in self.edges1d])
+        hi = prod([e[:, 1] for e in self.edges1d])
+        edges2 = np.stack([lo, hi], axis=-1)
+        nyq = attrs.kfun.min() * np.min(attrs.meshsize) / 2.
+        if mask_edges is None:
+            mid = edges2.mean(axis=-1)
+            mask = (mid[:, 0] <= mid[:, 1]) & (edges2[:, :, 1].sum(axis=1) <= nyq)
+        else:
+            assert mask_edges == '', 'the oracle takes the default mask or none'
+            mask = np.ones(len(edges2), dtype=bool)
+        self.edges = edges2[mask]
+        self.xavg = prod(self.xavg1d)[mask]
+        self.iedges = prod([np.arange(len(x)) for x in self.xavg1d])[mask]
+        self.nmodes = np.prod(prod(self.nmodes1d)[mask], axis=-1)
+
+    def shells(self, axis, value):
+        """ifftn(value [bin == b]) for every shell b of side axis, value on the full grid
+        (mesh3.py:300-305 with the complex-dtype c2r)."""
+        return [np.fft.ifftn(value * (self.ibin1d[axis] == b)) for b in range(len(self.xavg1d[axis]))]
+
+    def reduce(self, F0, F1, third):
+        """N^6 sum_x ifftn(F0 [b1]) ifftn(F1 [b2]) third per pair (mesh3.py:277-313)."""
+        s0, s1 = self.shells(0, F0), self.shells(1, F1)
+        n6 = float(np.prod(self.attrs.meshsize))**2
+        return np.array([np.sum(s0[b1] * s1[b2] * third) for b1, b2 in self.iedges]) * n6
+
+
+def bispectrum_sugiyama(attrs, bin, mesh1, mesh2=None, mesh3=None, los='z'):
+    """compute_mesh3_spectrum in the Sugiyama basis (mesh3.py:595-621): for each (l1, l2, L)
+    and each row (m1, m2, M, g) of triposh_rows, g bin(A1 Y*_l1m1(khat), A2 Y*_l2m2(khat),
+    Y*_LM(los) rho3), with A the full FFTs of the first two (real) meshes, rho3 the third mesh in
+    configuration space and los the line-of-sight vector (global) or the node positions from the
+    origin ('local'); summed over the rows, divided by nmodes, real part for even l1 + l2 and
+    imaginary part otherwise; norm = N^3 / Vcell^2."""
+    meshes = [mesh1, mesh2, mesh3]
+    for i in (1, 2):
+        if meshes[i] is None:
+            meshes[i] = meshes[i - 1]
+    real = [c2r(m, attrs.meshsize) if np.iscomplexobj(m) else np.asarray(m, dtype='f8') for m in meshes]
+    A = [np.fft.fftn(r) for r in real[:2]]
+    kk = full_kvec(attrs)
+    kx, ky, kz = kk[0][:, None, None], kk[1][None, :, None], kk[2][None, None, :]
+    local = isinstance(los, str) and los == 'local'
+    if local:
+        xs = attrs.rcoords()
+        lx, ly, lz = xs[0][:, None, None], xs[1][None, :, None], xs[2][None, None, :]
+    else:
+        lx, ly, lz = los_vector(los)
+    norm = np.prod(attrs.meshsize) / np.prod(attrs.cellsize)**2
+    num = []
+    for ell1, ell2, ell3 in bin.ells:
+        total = np.zeros(len(bin.iedges), dtype=complex)
+        for m1, m2, m3, g in triposh_rows(ell1, ell2, ell3, los_z=isinstance(los, str) and los == 'z'):
+            third = complex_ylm_conj(ell3, m3, lx, ly, lz) * real[2]
+            total += g * bin.reduce(A[0] * complex_ylm_conj(ell1, m1, kx, ky, kz), A[1] * complex_ylm_conj(ell2, m2, kx, ky, kz), third)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            total = total / bin.nmodes
+        num.append(total.real if (ell1 + ell2) % 2 == 0 else total.imag)
+    return dict(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num, norm=norm, ells=bin.ells,
+                los='local' if local else los_vector(los))
+
+
+def bispectrum_sugiyama_bruteforce(attrs, bin, rmeshes, ell=(0, 0, 0), los='z'):
+    """The pair sums of bispectrum_sugiyama (before the division by nmodes) as direct sums over
+    pairs of full-grid modes: sum over rows of g sum_{k1 in b1, k2 in b2} A1(k1) Y*(k1) A2(k2)
+    Y*(k2) H(-(k1 + k2)), with H = fftn(Y*_LM(los) rho3), from N^6 sum_x ifftn(.) ifftn(.) h =
+    sum_{k1, k2} a(k1) b(k2) fftn(h)(-(k1 + k2)).  Tiny meshes only."""
+    n = np.array([int(x) for x in attrs.meshsize])
+    A = [np.fft.fftn(m) for m in rmeshes[:2]]
+    kk = full_kvec(attrs)
+    KX, KY, KZ = np.meshgrid(*kk, indexing='ij')
+    local = isinstance(los, str) and los == 'local'
+    if local:
+        xs = attrs.rcoords()
+        LX, LY, LZ = np.meshgrid(*xs, indexing='ij')
+    else:
+        LX, LY, LZ = los_vector(los)
+    idx = np.stack(np.meshgrid(*[np.arange(x) for x in n], indexing='ij'), axis=-1).reshape(-1, 3)
+    ell1, ell2, ell3 = ell
+    out = np.zeros(len(bin.iedges), dtype=complex)
+    key = {tuple(t): i for i, t in enumerate(bin.iedges)}
+    ib0, ib1 = bin.ibin1d[0].ravel(), bin.ibin1d[1].ravel()
+    for m1, m2, m3, g in triposh_rows(ell1, ell2, ell3, los_z=isinstance(los, str) and los == 'z'):
+        a = (A[0] * complex_ylm_conj(ell1, m1, KX, KY, KZ)).ravel()
+        b = (A[1] * complex_ylm_conj(ell2, m2, KX, KY, KZ)).ravel()
+        H = np.fft.fftn(complex_ylm_conj(ell3, m3, LX, LY, LZ) * rmeshes[2])
+        for i in np.flatnonzero(ib0 >= 0):
+            k3 = (-(idx[i][None, :] + idx)) % n
+            h = H[k3[:, 0], k3[:, 1], k3[:, 2]]
+            for j in np.flatnonzero(ib1 >= 0):
+                t = key.get((ib0[i], ib1[j]))
+                if t is not None:
+                    out[t] += g * a[i] * b[j] * h[j]
+    return out

@@ -1,0 +1,137 @@
+"""GPU parity of the bispectrum in the Sugiyama basis (BinMesh3SpectrumPoles(basis='sugiyama'),
+compute_mesh3_spectrum; jaxpower/mesh3.py:23-177, 180-313, 443-471, 595-621) against the float64
+oracle, itself pinned in test_oracle.py to direct sums over pairs of full-grid modes, to sympy's
+Wigner 3j symbols and to sympy's construction of the complex harmonics.
+
+Tolerances: edges, pairs and mode counts exact; shell-averaged |k| to 1e-12.  The multipoles
+come from float32 meshes, float32 inverse FFTs and float32 products per cell (FP64 across
+cells): each pair sum is compared as |d| <= 1e-5 |want| + ftol max|want| per multipole, ftol
+covering the float32 rounding of the cell products.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import power_oracle as po
+
+from test_gpu_bispec import _mock, assert_close
+from test_gpu_parity import host, say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_real_ylm_sign_convention():
+    """The device's real harmonics are the reference's, sign included (the Sugiyama basis
+    combines them into complex harmonics, where the sign of each m matters)."""
+    from jaxpower_amd import _lib
+    say('device real ylm')
+    n = (7, 6, 5)
+    coords = [np.linspace(-1.3, 1.1, n[0]), np.linspace(-0.7, 1.9, n[1]), np.array([-2., -0.5, 0., 0.8, 1.5])]
+    coords[0][3] = 0.
+    coords[1][2] = 0.
+    tabs = [to_dev(c) for c in coords]
+    ones = torch.ones(n, dtype=torch.float32, device='cuda')
+    out = torch.empty_like(ones)
+    X, Y, Z = np.meshgrid(*coords, indexing='ij')
+    for ell in range(5):
+        for m in range(-ell, ell + 1):
+            _lib.call('mp_mul_ylm', _lib.stream_handle(ones.device), _lib.ptr(ones), _lib.ptr(out), _lib.i32x3(n), _lib.ptr(tabs[0]),
+                      _lib.ptr(tabs[1]), _lib.ptr(tabs[2]), ell, m, 0)
+            assert np.allclose(host(out), po.ref_real_ylm(ell, m, X, Y, Z), atol=2e-6), (ell, m)
+
+
+@pytest.mark.parametrize('meshsize,boxsize,edges,basis', [
+    ((32, 32, 32), 320., {'step': 0.04}, 'sugiyama'),
+    ((24, 16, 15), (240., 160., 150.), {'step': 0.05, 'min': 0.01}, 'sugiyama'),
+    ((16, 16, 16), 160., {'step': 0.03}, 'sugiyama-diagonal'),
+])
+def test_sugiyama_bin_setup(meshsize, boxsize, edges, basis):
+    from jaxpower_amd import MeshAttrs, BinMesh3SpectrumPoles
+    say('sugiyama bin', meshsize, basis)
+    attrs = MeshAttrs(meshsize=meshsize, boxsize=boxsize, boxcenter=0.)
+    bin = BinMesh3SpectrumPoles(attrs, edges=edges, ells=[(0, 0, 0), (2, 0, 2)], basis=basis)
+    obin = po.BinBispectrumSugiyama(po.Attrs(meshsize, boxsize, 0.), edges=edges, ells=[(0, 0, 0), (2, 0, 2)],
+                                    diagonal='diagonal' in basis)
+    for a in range(2):
+        assert np.array_equal(bin.nmodes1d[a], obin.nmodes1d[a])
+        assert np.allclose(bin.xavg1d[a], obin.xavg1d[a], rtol=1e-12, atol=0., equal_nan=True)
+    assert np.array_equal(bin._iedges, obin.iedges)
+    assert np.array_equal(bin.edges, obin.edges)
+    assert np.allclose(bin.xavg, obin.xavg, rtol=1e-12, atol=0.)
+    assert all(np.array_equal(nm, obin.nmodes) for nm in bin.nmodes)
+    assert bin.ells == [(0, 0, 0), (2, 0, 2)]
+
+
+@pytest.mark.parametrize('los', ['z', 'x', (0.6, 0., 0.8), 'local'])
+def test_sugiyama_spectrum_matches_oracle(los):
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_mesh3_spectrum
+    say('sugiyama spectrum', los)
+    attrs = MeshAttrs(meshsize=(24, 24, 20), boxsize=(240., 240., 200.), boxcenter=(30., -20., 250.))
+    pos, w = _mock(attrs, 20000, 5)
+    mesh = ParticleField(to_dev(pos), to_dev(w), attrs=attrs).paint(resampler='tsc', interlacing=2, compensate=True, out='real')
+    ells = [(0, 0, 0), (2, 0, 2), (1, 1, 0), (1, 1, 2), (2, 2, 2)]
+    edges = {'step': 0.03, 'min': 0.005}
+    bin = BinMesh3SpectrumPoles(attrs, edges=edges, ells=ells, basis='sugiyama')
+    got = compute_mesh3_spectrum(mesh, bin=bin, los=los)
+    oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter)
+    obin = po.BinBispectrumSugiyama(oattrs, edges=edges, ells=ells)
+    want = po.bispectrum_sugiyama(oattrs, obin, host(mesh.value).astype('f8'), los=los)
+    for ill, ell in enumerate(ells):
+        pole = got.get(ell)
+        assert pole.ell == ell and pole.basis == 'sugiyama'
+        assert np.array_equal(pole.values('nmodes'), want['nmodes'])
+        assert_close(pole.values('num_raw'), want['num_raw'][ill], ftol=3e-6, label=f'{los} {ell}')
+    assert np.isclose(got.get((0, 0, 0)).values('norm')[0], want['norm'])
+
+
+def test_sugiyama_cross_and_diagonal(tmp_path):
+    from jaxpower_amd import MeshAttrs, RealMeshField, BinMesh3SpectrumPoles, compute_mesh3_spectrum, read_observable
+    say('sugiyama cross diagonal')
+    rng = np.random.default_rng(9)
+    shape = (16, 16, 16)
+    attrs = MeshAttrs(meshsize=shape, boxsize=160., boxcenter=0.)
+    ms = [rng.normal(size=shape).astype('f4') for _ in range(3)]
+    ells = [(0, 0, 0), (2, 0, 2), (1, 1, 1)]
+    for basis in ('sugiyama', 'sugiyama-diagonal'):
+        bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.04}, ells=ells, basis=basis)
+        got = compute_mesh3_spectrum(*[RealMeshField(to_dev(m), attrs=attrs) for m in ms], bin=bin, los='x')
+        oattrs = po.Attrs(shape, 160., 0.)
+        obin = po.BinBispectrumSugiyama(oattrs, edges={'step': 0.04}, ells=ells, diagonal='diagonal' in basis)
+        want = po.bispectrum_sugiyama(oattrs, obin, *[m.astype('f8') for m in ms], los='x')
+        for ill, ell in enumerate(ells):
+            assert_close(got.get(ell).values('num_raw'), want['num_raw'][ill], ftol=3e-6, label=f'{basis} {ell}')
+        fn = str(tmp_path / f'{basis}.npz')
+        got.write(fn)
+        back = read_observable(fn)
+        assert back.ells == tuple(ells) and back.get((2, 0, 2)).basis == basis
+        assert np.array_equal(back.get((1, 1, 1)).values('num_raw'), got.get((1, 1, 1)).values('num_raw'))
+
+
+def test_sugiyama_bin_call_remove_zero():
+    from jaxpower_amd import MeshAttrs, RealMeshField, BinMesh3SpectrumPoles
+    say('sugiyama bin call')
+    rng = np.random.default_rng(10)
+    shape = (16, 16, 16)
+    attrs = MeshAttrs(meshsize=shape, boxsize=160., boxcenter=0.)
+    ms = [(rng.normal(size=shape) + 0.3).astype('f4') for _ in range(3)]
+    bin = BinMesh3SpectrumPoles(attrs, edges={'step': 0.05}, ells=[(0, 0, 0)], basis='sugiyama', mask_edges='')
+    obin = po.BinBispectrumSugiyama(po.Attrs(shape, 160., 0.), edges={'step': 0.05}, ells=[(0, 0, 0)], mask_edges='')
+    A = [RealMeshField(to_dev(m), attrs=attrs).r2c() for m in ms[:2]]
+    for rz in (False, True):
+        got = bin(A[0], A[1], RealMeshField(to_dev(ms[2]), attrs=attrs), remove_zero=rz)
+        F = [np.fft.fftn(m.astype('f8')) for m in ms[:2]]
+        third = ms[2].astype('f8')
+        if rz:
+            for f in F:
+                f[0, 0, 0] = 0.
+            third = third - third.mean()
+        want = obin.reduce(F[0], F[1], third)
+        assert_close(got, want.real, ftol=2e-6, label=f'remove_zero={rz}')
+
+
+def test_sugiyama_shells_beyond_nyquist_refused():
+    from jaxpower_amd import MeshAttrs, BinMesh3SpectrumPoles
+    say('sugiyama nyquist refused')
+    attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
+    with pytest.raises(NotImplementedError, match='Nyquist'):
+        BinMesh3SpectrumPoles(attrs, edges=np.linspace(0., 0.4, 9), ells=[(0, 0, 0)], basis='sugiyama', mask_edges='')

@@ -1,6 +1,7 @@
 """CPU tests: the C-ABI library loads and exports every declared symbol, and the host-side
 logic of the package (geometry, bin thresholds, containers, argument checks) matches the
 oracle.  No GPU compute is called here."""
+import itertools
 import os
 import re
 
@@ -270,3 +271,29 @@ def test_shell_cap_positions_distribution():
     inside = (ru >= 1000.) & (ru < 2000.) & (u[:, 2] >= 0.5 * ru)
     frac, cube = inside.mean(), 4000.**2 * 2000.
     assert abs(shell_cap_volume() - frac * cube) < 5. * np.sqrt(frac * (1. - frac) / len(u)) * cube
+
+
+def test_sugiyama_host_coefficients():
+    """The package's Wigner 3j, triposh rows, complex-harmonic components and complex harmonic
+    at a line of sight against the oracle's (pinned to sympy in test_oracle.py)."""
+    from jaxpower_amd import mesh3
+    from oracle import power_oracle as po
+    for j1, j2, j3 in itertools.product(range(5), repeat=3):
+        for m1 in range(-j1, j1 + 1):
+            for m2 in range(-j2, j2 + 1):
+                assert np.isclose(mesh3._wigner_3j(j1, j2, j3, m1, m2, -m1 - m2), po.wigner_3j(j1, j2, j3, m1, m2, -m1 - m2),
+                                  atol=1e-14)
+    for ell in [(0, 0, 0), (2, 0, 2), (1, 1, 2), (2, 2, 2), (4, 2, 2)]:
+        for los_z in (False, True):
+            got, want = mesh3._triposh_rows(*ell, los_z=los_z), po.triposh_rows(*ell, los_z=los_z)
+            assert [r[:3] for r in got] == [r[:3] for r in want]
+            assert np.allclose([r[3] for r in got], [r[3] for r in want], rtol=1e-14)
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(20, 3))
+    for ell in range(5):
+        for m in range(-ell, ell + 1):
+            comp = mesh3._ylm_components(ell, m)
+            got = sum(c * po.ref_real_ylm(ell, mu, *u.T) for mu, c in comp.items())
+            assert np.allclose(got, po.complex_ylm_conj(ell, m, *u.T), atol=1e-13), (ell, m)
+            for v in u[:3]:
+                assert np.isclose(mesh3._complex_ylm_conj_at(ell, m, v), po.complex_ylm_conj(ell, m, *v), atol=1e-13)

@@ -1053,3 +1053,105 @@ def test_bispectrum_shotnoise_local_monopole_and_ell2():
     want = 4. * np.pi / 5. * po.bispectrum_local_mesh_bruteforce(a, rho, 2)
     np.testing.assert_allclose(h, want, rtol=1e-9, atol=1e-10 * np.abs(want).max())
     assert np.all(np.isfinite(loc[1])) and np.abs(loc[1] - glo[1]).max() > 0.
+
+
+# ----------------------------------------------------------------------------- Sugiyama basis
+
+def _sympy_ylm(ell, m, real, reduced, conj):
+    """get_Ylm's polynomial (utils.py:223-334) redone here step by step: normalisation of
+    utils.py:227-238, associated Legendre of cos(theta), cos/sin(m phi) (real) or
+    cos(m phi) + i sin(m phi), conjugated if asked (complex), expanded, phi and theta replaced by
+    Cartesian ratios, x^2 + y^2 + z^2 -> r^2, x/r -> xhat.  The reference is not imported."""
+    sp = pytest.importorskip('sympy')
+    amp = 1. if reduced else np.sqrt((2 * ell + 1) / (4 * np.pi))
+    if m != 0:
+        amp *= np.sqrt(math.factorial(ell - abs(m)) / math.factorial(ell + abs(m)))
+    if real and m != 0:
+        amp *= np.sqrt(2.) * (-1)**m
+    if not real and m < 0:
+        amp *= (-1)**m
+    x, y, z, r = sp.symbols('x y z r', real=True, positive=True)
+    xh, yh, zh = sp.symbols('xhat yhat zhat', real=True, positive=True)
+    phi, theta = sp.symbols('phi theta', real=True)
+    defs = [(sp.sin(phi), y / sp.sqrt(x**2 + y**2)), (sp.cos(phi), x / sp.sqrt(x**2 + y**2)),
+            (sp.cos(theta), z / sp.sqrt(x**2 + y**2 + z**2))]
+    expr = sp.assoc_legendre(ell, abs(m), sp.cos(theta))
+    if real:
+        eimphi = sp.sin(abs(m) * phi) if m < 0 else (sp.cos(m * phi) if m > 0 else 1)
+    else:
+        eimphi = sp.cos(m * phi) + sp.sin(m * phi) * sp.I
+        if conj:
+            eimphi = sp.conjugate(eimphi)
+    expr *= sp.expand_trig(eimphi)
+    expr = sp.together(expr.subs(defs)).subs(x**2 + y**2 + z**2, r**2)
+    expr = amp * expr.expand().subs([(x / r, xh), (y / r, yh), (z / r, zh)])
+    return sp.lambdify((xh, yh, zh), expr, modules=['numpy']), (xh, yh, zh), expr
+
+
+def test_reference_real_and_complex_ylm_match_sympy_construction():
+    """ref_real_ylm (sign included) and complex_ylm_conj (get_Ylm(reduced=True, real=False,
+    conj=True)) against the sympy construction, on unit vectors and at the zero vector."""
+    rng = np.random.default_rng(11)
+    u = rng.normal(size=(40, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    for ell in range(5):
+        for m in range(-ell, ell + 1):
+            f, syms, expr = _sympy_ylm(ell, m, real=True, reduced=False, conj=False)
+            want = np.broadcast_to(np.asarray(f(*u.T), dtype=complex), (len(u),)).real
+            assert np.allclose(po.ref_real_ylm(ell, m, *u.T), want, atol=1e-12), (ell, m)
+            assert np.isclose(po.ref_real_ylm(ell, m, 0., 0., 0.), float(expr.subs({s_: 0 for s_ in syms})), atol=1e-15)
+            f, syms, expr = _sympy_ylm(ell, m, real=False, reduced=True, conj=True)
+            want = np.broadcast_to(np.asarray(f(*u.T), dtype=complex), (len(u),))
+            assert np.allclose(po.complex_ylm_conj(ell, m, *u.T), want, atol=1e-12), (ell, m)
+            at0 = complex(expr.subs({s_: 0 for s_ in syms}))
+            assert np.isclose(po.complex_ylm_conj(ell, m, 0., 0., 0.), at0, atol=1e-15), (ell, m, at0)
+
+
+def test_wigner_3j_and_triposh_rows_against_sympy():
+    sp_wigner = pytest.importorskip('sympy.physics.wigner')
+    for j1, j2, j3 in itertools.product(range(4), repeat=3):
+        for m1 in range(-j1, j1 + 1):
+            for m2 in range(-j2, j2 + 1):
+                m3 = -m1 - m2
+                want = float(sp_wigner.wigner_3j(j1, j2, j3, m1, m2, m3)) if abs(m3) <= j3 else 0.
+                assert np.isclose(po.wigner_3j(j1, j2, j3, m1, m2, m3), want, atol=1e-14), (j1, j2, j3, m1, m2)
+    # rows: every (m1, m2, M) with a nonzero coefficient; M = 0 only for los 'z'
+    rows = po.triposh_rows(2, 2, 2, los_z=False)
+    assert all(m1 + m2 + m3 == 0 for m1, m2, m3, _ in rows) and len(rows) == 19
+    assert [r[:3] for r in po.triposh_rows(2, 0, 2, los_z=True)] == [(0, 0, 0)]
+    assert np.isclose(po.triposh_rows(0, 0, 0)[0][3], 1.)
+    # sum over all m of W(l1 l2 L; m1 m2 M)^2 = 1, so sum of g^2 over the rows = ((2l1+1)(2l2+1)(2L+1) W(000))^2
+    g = np.array([r[3] for r in rows])
+    assert np.isclose(np.sum(g**2), (125. * float(sp_wigner.wigner_3j(2, 2, 2, 0, 0, 0)))**2)
+
+
+@pytest.mark.parametrize('ell,los', [((0, 0, 0), 'z'), ((2, 0, 2), 'z'), ((2, 0, 2), 'x'), ((1, 1, 0), 'x'), ((1, 1, 2), 'local'),
+                                     ((2, 0, 2), 'local')])
+def test_sugiyama_bispectrum_vs_mode_pair_sums(ell, los):
+    """The oracle's FFT reduction against direct sums over pairs of full-grid modes."""
+    attrs = po.Attrs((6, 8, 6), (60., 80., 60.), (10., -20., 35.))
+    rng = np.random.default_rng(17)
+    meshes = [rng.normal(size=(6, 8, 6)) for _ in range(3)]
+    bin = po.BinBispectrumSugiyama(attrs, edges=np.array([0., 0.06, 0.12, 0.2]), ells=[ell], mask_edges='')
+    got = po.bispectrum_sugiyama(attrs, bin, *meshes, los=los)['num_raw'][0]
+    pair = po.bispectrum_sugiyama_bruteforce(attrs, bin, meshes, ell=ell, los=los) / bin.nmodes
+    want = pair.real if (ell[0] + ell[1]) % 2 == 0 else pair.imag
+    assert np.allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+    assert np.abs(want).max() > 0
+
+
+def test_sugiyama_counts_and_mask():
+    """Full-grid shell counts equal the hermitian-weighted counts below the Nyquist planes, and
+    the default mask keeps mid1 <= mid2 with the sum of upper edges within the Nyquist."""
+    attrs = po.Attrs(16, 160., 0.)
+    edges = np.arange(0.01, 0.3, 0.02)
+    bin = po.BinBispectrumSugiyama(attrs, edges=edges, ells=[(0, 0, 0)])
+    herm = po.BinPoles(attrs, edges=edges)
+    assert np.array_equal(bin.nmodes1d[0], herm.nmodes.astype('f8'))
+    assert np.allclose(bin.xavg1d[0], herm.xavg, rtol=1e-14, equal_nan=True)
+    nyq = attrs.kfun.min() * 16 / 2.
+    mid = bin.edges.mean(axis=-1)
+    assert np.all(mid[:, 0] <= mid[:, 1]) and np.all(bin.edges[:, :, 1].sum(axis=1) <= nyq)
+    assert np.array_equal(bin.nmodes, bin.nmodes1d[0][bin.iedges[:, 0]] * bin.nmodes1d[1][bin.iedges[:, 1]])
+    diag = po.BinBispectrumSugiyama(attrs, edges=edges, ells=[(0, 0, 0)], diagonal=True)
+    assert np.all(diag.iedges[:, 0] == diag.iedges[:, 1])
