@@ -251,6 +251,14 @@ int mp_bispec_shells(void* stream, const void* A, int64_t pitch, const int32_t m
 int mp_bispec_shells_ylm(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double* kx, const double* ky,
                          const double* kz, const double* thr, int nedges, int b0, int nshell, int ell, int m, float* out,
                          void* workspace, size_t workspace_bytes);
+/* The spherical-Bessel sums of the Sugiyama shot noise's last term (jaxpower/mesh3.py:875-903,
+ * get_spherical_jn of jaxpower/utils.py:418-450): out[p] = sum over the cells s of F(s)
+ * j_l1(|s| k1[b1]) j_l2(|s| k2[b2]) with (b1, b2) = pairs[2p .. 2p+1]; F a float32 real mesh,
+ * |s| from the float64 separation tables; l1, l2 <= 5; FP64 sums. */
+size_t mp_hankel_pairs_workspace_bytes(int npairs);
+int mp_hankel_pairs(void* stream, const float* F, const int32_t meshsize[3], const double* sx, const double* sy, const double* sz,
+                    int l1, const double* k1, int nk1, int l2, const double* k2, int nk2, const int32_t* pairs, int npairs,
+                    double* out, void* workspace, size_t workspace_bytes);
 /* The closed-triangle reduction of BinMesh3SpectrumPoles.bin_reduce_meshs with
  * reduce = prod(meshes).sum() (jaxpower/mesh3.py:238-275, 307-311): out[t] = sum over the ncells
  * cells of s1[b1] s2[b2] s3[b3] for (b1, b2, b3) = tri[3t .. 3t+2] (a device int32 array sorted by

@@ -12,6 +12,7 @@
 // The triangle pass is a persistent sweep over cell tiles: the block stages every shell of
 // its tile in LDS, and each wave owns a run of triangles (sorted by (b1, b2), so the product
 // I1_b1 I2_b2 of a lane's cells is reused along b3) with the lane's running sums in registers.
+#include "bessel.h"
 #include "bin.h"
 #include "common.h"
 #include "fft.h"
@@ -165,6 +166,76 @@ triangle_sum_kernel(TriArgs a) {
   if (lane < TR_PER_WAVE && t0 + lane < a.ntri) a.partial[(int64_t)blockIdx.x * a.ntri + t0 + lane] = own;
 }
 
+// Spherical-Bessel pair sums of a configuration-space mesh (the last step of the Sugiyama
+// shot noise, mesh3.py:875-903): out[p] = sum over the cells s of F(s) j_l1(|s| k1[b1])
+// j_l2(|s| k2[b2]) for the pairs (b1, b2) = pairs[2p], pairs[2p + 1].  A block walks tiles of
+// HK_TILE cells: it evaluates j_l1(|s| k) and j_l2(|s| k) once per cell and distinct k (float64,
+// stored as float in LDS), then each thread adds the tile's products for its HK_PER pairs
+// (float per tile, float64 across tiles); per-block partials are summed in a fixed order.
+constexpr int HK_NT = 256;
+constexpr int HK_TILE = 32;
+constexpr int HK_PER = 4;
+constexpr int HK_MAX_BLOCKS = 1024;
+
+__global__ void __launch_bounds__(HK_NT)
+hankel_pairs_kernel(const float* __restrict__ F, int n0, int n1, int n2, const double* __restrict__ sx, const double* __restrict__ sy,
+                    const double* __restrict__ sz, int l1, const double* __restrict__ k1, int nk1, int l2, const double* __restrict__ k2,
+                    int nk2, const int* __restrict__ pairs, int npairs, double* __restrict__ partial) {
+  extern __shared__ float hk[];
+  float* fv = hk;                       // [HK_TILE] field values
+  float* j1 = fv + HK_TILE;             // [nk1][HK_TILE]
+  float* j2 = j1 + (size_t)nk1 * HK_TILE;  // [nk2][HK_TILE]
+  const int tid = threadIdx.x;
+  const int64_t ncells = (int64_t)n0 * n1 * n2;
+  const int64_t ntiles = (ncells + HK_TILE - 1) / HK_TILE;
+  int b1[HK_PER], b2[HK_PER];
+  double acc[HK_PER];
+#pragma unroll
+  for (int q = 0; q < HK_PER; ++q) {
+    const int p = tid + q * HK_NT;
+    b1[q] = p < npairs ? pairs[2 * p] : -1;
+    b2[q] = p < npairs ? pairs[2 * p + 1] : -1;
+    acc[q] = 0.;
+  }
+  const int nk = nk1 + nk2;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t c0 = tile * HK_TILE;
+    for (int i = tid; i < HK_TILE; i += HK_NT) {
+      const int64_t c = c0 + i;
+      fv[i] = c < ncells ? F[c] : 0.f;
+    }
+    for (int i = tid; i < nk * HK_TILE; i += HK_NT) {
+      const int kq = i / HK_TILE, ci = i % HK_TILE;
+      const int64_t c = c0 + ci;
+      double v = 0.;
+      if (c < ncells) {
+        const int iz = (int)(c % n2);
+        const int64_t row = c / n2;
+        const int iy = (int)(row % n1), ix = (int)(row / n1);
+        const double r = sqrt(k2_sum(k2_xy(sx[ix], sy[iy]), sz[iz]));
+        v = kq < nk1 ? spherical_jn(l1, r * k1[kq]) : spherical_jn(l2, r * k2[kq - nk1]);
+      }
+      j1[(size_t)kq * HK_TILE + ci] = (float)v;  // j2 follows j1 contiguously
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < HK_PER; ++q) {
+      if (b1[q] < 0) continue;
+      const float* a = j1 + (size_t)b1[q] * HK_TILE;
+      const float* b = j2 + (size_t)b2[q] * HK_TILE;
+      float t = 0.f;
+      for (int ci = 0; ci < HK_TILE; ++ci) t = fmaf(fv[ci] * a[ci], b[ci], t);
+      acc[q] += (double)t;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < HK_PER; ++q) {
+    const int p = tid + q * HK_NT;
+    if (p < npairs) partial[(int64_t)blockIdx.x * npairs + p] = acc[q];
+  }
+}
+
 static int multiprocessor_count(int* ncu) {
   int dev = 0;
   MP_CHECK_HIP(hipGetDevice(&dev));
@@ -248,6 +319,36 @@ MP_EXPORT int mp_bispec_shells_ylm(void* stream, const void* A, int64_t pitch, c
     if (rc) return rc;
   }
   return MP_OK;
+}
+
+MP_EXPORT size_t mp_hankel_pairs_workspace_bytes(int npairs) {
+  return align_up((size_t)HK_MAX_BLOCKS * (npairs > 0 ? npairs : 1) * sizeof(double), 256) + 256;
+}
+
+// out[p] = sum over the N0 N1 N2 cells s of F(s) j_l1(|s| k1[b1]) j_l2(|s| k2[b2]), (b1, b2) =
+// pairs[2p .. 2p+1] (device int32), |s| from the device float64 separation tables sx, sy, sz,
+// k1 (nk1) and k2 (nk2) device float64; l1, l2 <= 5; FP64 sums in a fixed order.
+MP_EXPORT int mp_hankel_pairs(void* stream, const float* F, const int32_t meshsize[3], const double* sx, const double* sy,
+                              const double* sz, int l1, const double* k1, int nk1, int l2, const double* k2, int nk2,
+                              const int32_t* pairs, int npairs, double* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(F && meshsize && sx && sy && sz && k1 && k2 && pairs && out, "mp_hankel_pairs: null pointer");
+  MP_REQUIRE(l1 >= 0 && l1 <= 5 && l2 >= 0 && l2 <= 5, "mp_hankel_pairs: (l1, l2) = (%d, %d) not in 0..5", l1, l2);
+  MP_REQUIRE(nk1 >= 1 && nk2 >= 1 && nk1 + nk2 <= 1024, "mp_hankel_pairs: %d + %d wavenumbers", nk1, nk2);
+  MP_REQUIRE(npairs >= 1 && npairs <= HK_NT * HK_PER, "mp_hankel_pairs: %d pairs not in 1..%d", npairs, HK_NT * HK_PER);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ncells = (int64_t)meshsize[0] * meshsize[1] * meshsize[2];
+  int ncu = 0;
+  int rc = multiprocessor_count(&ncu);
+  if (rc) return rc;
+  const int blocks = (int)std::min<int64_t>(std::min<int64_t>(ceil_div(ncells, HK_TILE), (int64_t)ncu * 4), HK_MAX_BLOCKS);
+  Workspace ws(workspace, workspace_bytes);
+  double* partial = ws.take<double>((size_t)blocks * npairs);
+  if (!partial) return set_error(MP_ERR_WORKSPACE, "mp_hankel_pairs: workspace too small");
+  const size_t lds = (size_t)HK_TILE * (1 + nk1 + nk2) * sizeof(float);
+  if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)hankel_pairs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  MP_LAUNCH("hankel_pairs", hankel_pairs_kernel, dim3((unsigned)blocks), dim3(HK_NT), lds, s, F, meshsize[0], meshsize[1], meshsize[2], sx,
+            sy, sz, l1, k1, nk1, l2, k2, nk2, (const int*)pairs, npairs, partial);
+  return reduce_partials(s, partial, blocks, npairs, out);
 }
 
 MP_EXPORT size_t mp_bispec_triangles_workspace_bytes(int ntri) {

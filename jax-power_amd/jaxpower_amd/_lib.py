@@ -67,6 +67,9 @@ SIGNATURES = {
     'mp_bin_real_poles': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, P_F64, c_int, P_I32,
                                   P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_bispec_shells_workspace_bytes': (c_size_t, [P_I32]),
+    'mp_hankel_pairs_workspace_bytes': (c_size_t, [c_int]),
+    'mp_hankel_pairs': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int,
+                                c_void_p, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_bispec_shells_ylm': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                      c_int, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_bispec_shells': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, P_F64,

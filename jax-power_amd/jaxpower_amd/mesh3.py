@@ -571,6 +571,12 @@ def compute_fkp3_shotnoise(*fkps, bin=None, los='z', resampler='cic', interlacin
     fkps, fields = format_meshes(*fkps, fields=fields, nmeshes=3)
     if fields[2] == fields[1] + 1 == fields[0] + 2:
         return tuple(np.zeros(bin.nbins) for _ in bin.ells)
+    if 'sugiyama' in bin.basis:
+        particles = []
+        for fkp, f in zip(fkps, fields):
+            particles.append(particles[f] if f < len(particles) else (fkp.particles if isinstance(fkp, FKPField) else fkp))
+        return _sugiyama_shotnoise(particles, fields, bin, los, dict(kwargs, resampler=resampler, interlacing=interlacing,
+                                                                         compensate=compensate))
     if not (fields[0] == fields[1] == fields[2]):
         raise NotImplementedError('the Scoccimarro shot noise takes one field (mesh3.py:1050-1051)')
     particles = fkps[0].particles if isinstance(fkps[0], FKPField) else fkps[0]
@@ -631,3 +637,208 @@ def compute_fkp3_shotnoise(*fkps, bin=None, los='z', resampler='cic', interlacin
                 sn = (4. * np.pi if local else 2 * ell + 1) * sn
             out.append(np.real(sn) if ell % 2 == 0 else np.imag(sn))
     return tuple(out)
+
+
+def _shotnoise_window(mattrs, resampler, interlacing, compensate, device):
+    """compensate_shotnoise of mesh3.py:1081-1089 on the half spectrum: with interlacing (n_i >= 2)
+    1 if compensate else 1 / compensate(1, kcirc)^2, without it aliasing_shotnoise(1, kcirc)
+    times compensate(1, kcirc)^2 if compensate; a complex64 half spectrum (imaginary part 0)."""
+    from . import resamplers
+    r = resamplers.get_resampler(resampler)
+    kcirc = mattrs.kcoords(kind='circular', sparse=True, device=device)
+    one = torch.ones((), dtype=torch.float32, device=device)
+    if max(int(interlacing), 1) >= 2:
+        w = one if compensate else 1. / r.compensate(one, kcirc)**2
+    else:
+        w = r.aliasing_shotnoise(one, kcirc) * (r.compensate(one, kcirc)**2 if compensate else 1.)
+    out = _empty_spectrum(mattrs.meshsize, device)
+    out.copy_(torch.broadcast_to(w, out.shape).to(torch.complex64))
+    return out
+
+
+def _sugiyama_shotnoise(particles, fields, bin, los, paint_kw):
+    """compute_fkp3_shotnoise in the Sugiyama basis (mesh3.py:1091-1180, eqs. 45-46 of
+    arXiv:1803.02132), spectrum, global or local line of sight.  With rho_c the mean-subtracted
+    configuration-space paint, C the complex paint with its k = 0 mode removed, W the shot-noise
+    window of _shotnoise_window, Y the reduced complex harmonics and Y* their conjugates:
+      S111 (all three fields equal): s111_lm = sum_x paint(w^3)(x) Y*_lm(xhat); the (0, 0, 0)
+        multipole gets s111_00;
+      S122 (fields 2 and 3 equal; multipoles (l, 0, l)): (2l + 1) times the full-grid shell mean of
+        sum_m Y_lm(khat) [C_w1 conj(FFT(rho_c[w2 w3] Y_lm(los))) - s111_lm W], taken on side 1 (and
+        S121 likewise on side 2 for (0, l, l) when fields 1 and 3 are equal, mesh3.py:855-883);
+      S113 (fields 1 and 2 equal): i^(l1 + l2) sum over the rows (m1, m2, M, g) of
+        g sum_s ifftn(FFT(rho_c[w3] Y*_LM(los)) conj(C_w1w2) - s111_LM W)(s) Y*_l1m1(shat)
+        Y*_l2m2(shat) j_l1(|s| k1) j_l2(|s| k2) at the pairs' mean |k| (mesh3.py:886-915).
+    With real harmonics Y_l,mu: sum_m Y_lm(khat) Y*_lm(u) = 4 pi/(2l+1) sum_mu Y_l,mu(khat) Y_l,mu(u)
+    (= P_l(khat.u) for nonzero vectors), so the S122 sums are real-harmonic spectra binned with the
+    hermitian weights (real part for even l, i times the imaginary part for odd l, the full-grid
+    sum of an (anti)hermitian product); in S113 every ifftn is of a hermitian product and the
+    angular factor is a sum of products of two real harmonics of shat, all on the device.  Returns
+    one host array per multipole (real part for even l1 + l2, imaginary part otherwise)."""
+    from .mesh2 import _mul_ylm, _svectors
+    mattrs = particles[0].attrs
+    device = bin._device
+    ells = bin.ells
+    local = isinstance(los, str) and los == 'local'
+    vlos = None if local else _get_los_vector(los)
+    n = _lib.i32x3(mattrs.meshsize)
+    shape = tuple(int(x) for x in mattrs.meshsize)
+    nbins = bin.nbins
+    out = [np.zeros(nbins, dtype=complex) for _ in ells]
+    offset = np.asarray(mattrs.boxcenter) - np.asarray(mattrs.boxsize) / 2.
+    xs = [torch.from_numpy(np.arange(int(nn)) * float(d) + float(o)).to(device) for nn, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]
+    ss = [torch.from_numpy(v).to(device) for v in _svectors(mattrs)]
+    W = _shotnoise_window(mattrs, paint_kw['resampler'], paint_kw['interlacing'], paint_kw['compensate'], device)
+    stream = _lib.stream_handle(device)
+
+    def paint(p, weights=None, out_kind='real'):
+        q = p if weights is None else p.clone(weights=weights)
+        return q.paint(**paint_kw, out=out_kind)
+
+    def centered(mesh):
+        v = mesh.value.to(torch.float32)
+        return (v - v.double().mean().float()).contiguous()
+
+    def complex_zero_dc(mesh):
+        v = mesh.value.clone()
+        v[0, 0, 0] = 0.
+        return v
+
+    def prod_spectrum(a, b):
+        """conj(a) b on half spectra."""
+        pa, pb = _as_spectrum(a), _as_spectrum(b)
+        res = _empty_spectrum(mattrs.meshsize, device)
+        _lib.call('mp_spectrum_product', stream, _lib.ptr(pa[0]), pa[1], _lib.ptr(pb[0]), pb[1], n, _lib.ptr(res), _spectrum_pitch(res))
+        return res
+
+    def c2r(a):
+        from .mesh import _c2r
+        return _c2r(a, mattrs.meshsize)
+
+    # S111: s111_lm = sum_x paint(w^3) Y*_lm(xhat), through the real harmonics
+    uells = sorted({l for ell in ells for l in ell})
+    S = {l: np.zeros(2 * l + 1) for l in uells}  # S[l][mu + l] = sum_x rho3 Y_l,mu(xhat)
+    all_equal = fields[0] == fields[1] == fields[2]
+    if all_equal:
+        rho3 = paint(particles[0], particles[0].weights**3).value.to(torch.float32).contiguous()
+        tmp = torch.empty_like(rho3)
+        for l in uells:
+            for mu in range(-l, l + 1):
+                _mul_ylm(rho3, tmp, mattrs, xs, l, mu, False)
+                S[l][mu + l] = float(tmp.double().sum())
+        del rho3, tmp
+
+    def s111(l, m):
+        return sum(c * S[l][mu + l] for mu, c in _ylm_components(l, m).items())
+
+    if all_equal and (0, 0, 0) in ells:
+        out[ells.index((0, 0, 0))] += s111(0, 0)
+
+    def s122(ps, ells1, axis):
+        """(2l+1) full-grid shell means on side axis, per l in ells1 (mesh3.py:855-883)."""
+        side = bin._axes[axis]
+        rc = centered(paint(ps[1], ps[1].weights * ps[2].weights))
+        C = complex_zero_dc(paint(ps[0], out_kind='complex'))
+        res = {}
+        R = _as_spectrum(_2c(RealMeshField(rc, attrs=mattrs)).value)[0]
+        for l in ells1:
+            if local and l > 0:
+                G = mattrs.create('complex', device=device).value
+                _ylm_accumulate(RealMeshField(rc, attrs=mattrs), l, G)
+                t1 = side._products_dev([G, C], [(0, 1, 0, False)])[0] * (4. * np.pi / (2 * l + 1))
+                del G
+            else:  # global line of sight, or l = 0 where 4 pi Y_00(khat) Y_00(xhat) = 1
+                t1 = side._products_dev([R, C], [(0, 1, l, False)], los=vlos if l else (0., 0., 0.))[0]
+            t1 = t1.cpu().numpy()
+            full = t1.real + 0j if l % 2 == 0 else 1j * t1.imag
+            if all_equal and l % 2 == 0:
+                V = torch.zeros_like(W)
+                Y = _empty_spectrum(mattrs.meshsize, device)
+                t = side._tables
+                for mu in range(-l, l + 1):
+                    if S[l][mu + l] == 0.:
+                        continue
+                    _lib.call('mp_ylm_legendre_spectrum', stream, n, _lib.ptr(t['kx']), _lib.ptr(t['ky']), _lib.ptr(t['kz']), int(l), int(mu),
+                              0, _lib.f64x3((0., 0., 1.)), _lib.ptr(Y), _spectrum_pitch(Y))
+                    V += Y * float(S[l][mu + l])
+                V = V * W * (4. * np.pi / (2 * l + 1))
+                full = full - side._products_dev([V], [(-1, 0, 0, False)])[0].cpu().numpy().real
+            with np.errstate(invalid='ignore', divide='ignore'):
+                res[l] = (2 * l + 1) * full / bin.nmodes1d[axis]
+        return res
+
+    if fields[1] == fields[2]:
+        ells1 = [ell[0] for ell in ells if ell[2] == ell[0] and ell[1] == 0]
+        if ells1:
+            res = s122(particles, ells1, 0)
+            for ill, ell in enumerate(ells):
+                if ell[2] == ell[0] and ell[1] == 0:
+                    out[ill] += res[ell[0]][bin._iedges[:, 0]]
+    if fields[0] == fields[2]:
+        ells2 = [ell[1] for ell in ells if ell[2] == ell[1] and ell[0] == 0]
+        if ells2:
+            res = s122([particles[1], particles[0], particles[2]], ells2, 1)
+            for ill, ell in enumerate(ells):
+                if ell[2] == ell[1] and ell[0] == 0:
+                    out[ill] += res[ell[1]][bin._iedges[:, 1]]
+    if fields[0] == fields[1]:
+        rc = centered(paint(particles[2]))
+        Cw2 = complex_zero_dc(paint(particles[0], particles[0].weights * particles[1].weights, out_kind='complex'))
+        omega = c2r(W)
+        k1 = torch.from_numpy(np.ascontiguousarray(bin.xavg1d[0], dtype='f8')).to(device)
+        k2 = torch.from_numpy(np.ascontiguousarray(bin.xavg1d[1], dtype='f8')).to(device)
+        taus = {}
+
+        def tau(mu):  # c2r(conj(C_w1w2) FFT(rho_c Y_L,mu(xhat))) (local) or c2r(conj(C_w1w2) FFT(rho_c)) (global)
+            if mu not in taus:
+                if mu is None:
+                    taus[mu] = c2r(prod_spectrum(Cw2, _2c(RealMeshField(rc, attrs=mattrs)).value))
+                else:
+                    L, m = mu
+                    tmp = torch.empty_like(rc)
+                    _mul_ylm(rc, tmp, mattrs, xs, L, m, False)
+                    taus[mu] = c2r(prod_spectrum(Cw2, _2c(RealMeshField(tmp, attrs=mattrs)).value))
+            return taus[mu]
+
+        ws = _lib.workspace(_lib.load().mp_hankel_pairs_workspace_bytes(min(bin.nbins, 1024)), device)
+
+        def hankel(F, l1, l2):
+            res = np.empty(bin.nbins)
+            for lo in range(0, bin.nbins, 1024):
+                pr = np.ascontiguousarray(bin._iedges[lo:lo + 1024], dtype='i4')
+                pd = torch.from_numpy(pr.ravel()).to(device)
+                o = torch.empty(len(pr), dtype=torch.float64, device=device)
+                _lib.call('mp_hankel_pairs', stream, _lib.ptr(F), n, _lib.ptr(ss[0]), _lib.ptr(ss[1]), _lib.ptr(ss[2]), int(l1), _lib.ptr(k1),
+                          len(bin.xavg1d[0]), int(l2), _lib.ptr(k2), len(bin.xavg1d[1]), _lib.ptr(pd), len(pr), _lib.ptr(o), _lib.ptr(ws),
+                          ws.numel())
+                res[lo:lo + len(pr)] = o.cpu().numpy()
+            return res
+
+        for ill, (l1, l2, L) in enumerate(ells):
+            coef = {}  # (field key, mu1, mu2) -> complex coefficient of field(s) Y_l1,mu1(shat) Y_l2,mu2(shat)
+            for m1, m2, M, g in _triposh_rows(l1, l2, L, los_z=isinstance(los, str) and los == 'z'):
+                base = {('tau', (L, mu)): c for mu, c in _ylm_components(L, M).items()} if local else \
+                    {('tau', None): _complex_ylm_conj_at(L, M, vlos)}
+                if all_equal:
+                    base[('omega', None)] = base.get(('omega', None), 0.) - s111(L, M)
+                for mu1, c1 in _ylm_components(l1, m1).items():
+                    for mu2, c2 in _ylm_components(l2, m2).items():
+                        for key, cb in base.items():
+                            k = (key, mu1, mu2)
+                            coef[k] = coef.get(k, 0.) + g * cb * c1 * c2
+            Ure = torch.zeros(shape, dtype=torch.float32, device=device)
+            Uim = torch.zeros(shape, dtype=torch.float32, device=device)
+            tmp = torch.empty(shape, dtype=torch.float32, device=device)
+            for ((kind, arg), mu1, mu2), c in coef.items():
+                if abs(c) < 1e-14:
+                    continue
+                field = omega if kind == 'omega' else tau(arg)
+                _mul_ylm(field, tmp, mattrs, ss, l1, mu1, False)
+                _mul_ylm(tmp, tmp, mattrs, ss, l2, mu2, False)
+                if c.real != 0.:
+                    Ure.add_(tmp, alpha=float(c.real))
+                if c.imag != 0.:
+                    Uim.add_(tmp, alpha=float(c.imag))
+            total = hankel(Ure, l1, l2) + 1j * hankel(Uim, l1, l2)
+            out[ill] += (1j)**(l1 + l2) * total
+    return tuple(o.real if (ell[0] + ell[1]) % 2 == 0 else o.imag for ell, o in zip(ells, out))

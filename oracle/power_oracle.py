@@ -1944,3 +1944,113 @@ def bispectrum_sugiyama_bruteforce(attrs, bin, rmeshes, ell=(0, 0, 0), los='z'):
                 if t is not None:
                     out[t] += g * a[i] * b[j] * h[j]
     return out
+
+
+def aliasing_shotnoise_window(order, kcirc):
+    """Aliased shot-noise factor prod_i K_order(k_i) on circular wavenumbers
+    (_aliasing_shotnoise_kernel, resamplers.py:29-55)."""
+    def one(a):
+        s2 = np.sin(a / 2.)**2
+        return {1: np.ones_like(a), 2: 1. - 2. / 3. * s2, 3: 1. - s2 + 2. / 15. * s2**2,
+                4: 1. - 4. / 3. * s2 + 2. / 5. * s2 * s2 - 4. / 315. * s2**3}[order]
+    out = 1.
+    for a in kcirc:
+        out = out * one(a)
+    return out
+
+
+def sugiyama_shotnoise(attrs, bin, positions, weights=None, los='z', resampler='cic', interlacing=0, compensate=True, parts=None):
+    """compute_fkp3_shotnoise in the Sugiyama basis for one field (mesh3.py:1091-1180 with
+    _compute_sugiyama_spectrum_S122 855-883 and _S113 886-915, eqs. 45-46 of arXiv:1803.02132),
+    on the full complex grid as the reference's complex-dtype attributes give:
+      S111: s111_lm = sum_x paint(w^3) Y*_lm(xhat) (conjugated reduced harmonics, xhat from the
+        origin), added to the (0, 0, 0) multipole;
+      S122 (and S121 on the second side): for (l, 0, l) [(0, l, l)], (2l + 1) times the shell
+        mean over the full grid of sum_m Y_lm(khat) [C(k) conj(FFT(rho_c Y_lm(los))(k)) -
+        s111_lm W(k)], with C = FFT(paint(w)) (k = 0 removed), rho_c = paint(w^2) - mean;
+      S113: i^(l1 + l2) sum over the rows (m1, m2, M, g) of g sum_s
+        ifftn(FFT(rho_c Y*_LM(los)) conj(C_2) - s111_LM W)(s) Y*_l1m1(shat) Y*_l2m2(shat)
+        j_l1(|s| k1) j_l2(|s| k2), rho_c = paint(w) - mean, C_2 = FFT(paint(w^2)) (k = 0
+        removed), (k1, k2) the pair's mean |k|;
+    W = 1 (compensated) or the squared window (not compensated) with interlacing, else the
+    aliased shot-noise factor times the squared compensation when compensated (mesh3.py:1081-1089).
+    Real part for even l1 + l2, imaginary part otherwise.  A dict given as parts receives the
+    pieces: 's111' {(l, m): value}, 's122' {(axis, l): per-shell array}, 's113' {ell: per-pair
+    array}."""
+    order = ORDERS[resampler]
+    w = np.ones(len(positions)) if weights is None else np.asarray(weights, dtype='f8')
+    n = [int(x) for x in attrs.meshsize]
+
+    def painted(ww):
+        return paint(attrs, positions, ww, resampler=resampler, interlacing=interlacing, compensate=compensate, out='real')
+
+    kk = full_kvec(attrs)
+    KX, KY, KZ = np.meshgrid(*kk, indexing='ij')
+    kcirc = np.meshgrid(*[np.fft.fftfreq(x) * 2. * np.pi for x in n], indexing='ij')
+    comp = 1.
+    for a in kcirc:
+        comp = comp * np.sinc(a / (2. * np.pi))**(-order)
+    if max(interlacing, 1) >= 2:
+        W = np.ones(n) if compensate else 1. / comp**2
+    else:
+        W = aliasing_shotnoise_window(order, kcirc) * (comp**2 if compensate else 1.)
+    xs = attrs.rcoords()
+    X = np.meshgrid(*xs, indexing='ij')
+    ssv = np.meshgrid(*separation_coords(attrs), indexing='ij')
+    snorm_ = np.sqrt((ssv[0]**2 + ssv[1]**2) + ssv[2]**2)
+    local = isinstance(los, str) and los == 'local'
+    L = X if local else list(los_vector(los))
+    ells = [tuple(e) for e in bin.ells]
+    out = [np.zeros(len(bin.iedges), dtype=complex) for _ in ells]
+    uells = sorted({l for e in ells for l in e})
+    rho3 = painted(w**3)
+    s111 = {(l, m): np.sum(rho3 * complex_ylm_conj(l, m, *X)) for l in uells for m in range(-l, l + 1)}
+    if (0, 0, 0) in ells:
+        out[ells.index((0, 0, 0))] += s111[0, 0]
+
+    def shell_mean(value, axis):
+        ib = bin.ibin1d[axis]
+        nb = len(bin.nmodes1d[axis])
+        ok = (ib >= 0) & (ib < nb)
+        s = np.bincount(ib[ok], weights=value.real[ok], minlength=nb) + 1j * np.bincount(ib[ok], weights=value.imag[ok], minlength=nb)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            return s / bin.nmodes1d[axis]
+
+    C = np.fft.fftn(painted(w))
+    C[0, 0, 0] = 0.
+    rc = painted(w * w)
+    rc = rc - rc.mean()
+    for axis in (0, 1):
+        sel = [e for e in ells if (e[2] == e[0] and e[1] == 0) if axis == 0] or [] if axis == 0 else \
+              [e for e in ells if e[2] == e[1] and e[0] == 0]
+        for e in sel:
+            l = e[0] if axis == 0 else e[1]
+            carry = 0.
+            for m in range(-l, l + 1):
+                ylm_k = np.conj(complex_ylm_conj(l, m, KX, KY, KZ))  # Y_lm (not conjugated)
+                ylm_l = np.conj(complex_ylm_conj(l, m, *L))
+                carry = carry + ylm_k * (C * np.conj(np.fft.fftn(rc * ylm_l)) - s111[l, m] * W)
+            piece = (2 * l + 1) * shell_mean(carry, axis)
+            if parts is not None:
+                parts.setdefault('s122', {})[axis, l] = piece
+            out[ells.index(e)] += piece[bin.iedges[:, axis]]
+    rc = painted(w)
+    rc = rc - rc.mean()
+    C2 = np.fft.fftn(painted(w * w))
+    C2[0, 0, 0] = 0.
+    k1 = bin.xavg1d[0][bin.iedges[:, 0]]
+    k2 = bin.xavg1d[1][bin.iedges[:, 1]]
+    for ill, (l1, l2, l3) in enumerate(ells):
+        U = 0.
+        for m1, m2, m3, g in triposh_rows(l1, l2, l3, los_z=isinstance(los, str) and los == 'z'):
+            t = np.fft.ifftn(np.fft.fftn(rc * complex_ylm_conj(l3, m3, *L)) * np.conj(C2) - s111[l3, m3] * W)
+            U = U + g * t * complex_ylm_conj(l1, m1, *ssv) * complex_ylm_conj(l2, m2, *ssv)
+        if np.ndim(U) == 0:
+            continue
+        total = np.array([np.sum(U * special.spherical_jn(l1, snorm_ * a) * special.spherical_jn(l2, snorm_ * b)) for a, b in zip(k1, k2)])
+        out[ill] += (1j)**(l1 + l2) * total
+        if parts is not None:
+            parts.setdefault('s113', {})[l1, l2, l3] = (1j)**(l1 + l2) * total
+    if parts is not None:
+        parts['s111'] = s111
+    return [o.real if (e[0] + e[1]) % 2 == 0 else o.imag for e, o in zip(ells, out)]

@@ -135,3 +135,29 @@ def test_sugiyama_shells_beyond_nyquist_refused():
     attrs = MeshAttrs(meshsize=16, boxsize=160., boxcenter=0.)
     with pytest.raises(NotImplementedError, match='Nyquist'):
         BinMesh3SpectrumPoles(attrs, edges=np.linspace(0., 0.4, 9), ells=[(0, 0, 0)], basis='sugiyama', mask_edges='')
+
+
+@pytest.mark.parametrize('los,resampler,interlacing,compensate,ells', [
+    ('z', 'cic', 2, True, [(0, 0, 0), (2, 0, 2), (0, 2, 2), (1, 1, 0)]),
+    ('x', 'tsc', 0, True, [(0, 0, 0), (2, 2, 2), (1, 0, 1)]),
+    ('local', 'tsc', 2, False, [(0, 0, 0), (2, 0, 2), (1, 1, 2)]),
+    ('local', 'ngp', 0, False, [(0, 0, 0), (0, 1, 1), (2, 2, 0)]),
+])
+def test_sugiyama_shotnoise_matches_oracle(los, resampler, interlacing, compensate, ells):
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3SpectrumPoles, compute_fkp3_shotnoise
+    say('sugiyama shot noise', los, resampler, interlacing, compensate)
+    attrs = MeshAttrs(meshsize=(16, 16, 12), boxsize=(160., 160., 120.), boxcenter=(20., -10., 90.))
+    pos, w = _mock(attrs, 6000, 21)
+    p = ParticleField(to_dev(pos), to_dev(w), attrs=attrs)
+    edges = {'step': 0.05}
+    bin = BinMesh3SpectrumPoles(attrs, edges=edges, ells=ells, basis='sugiyama')
+    got = compute_fkp3_shotnoise(p, bin=bin, los=los, resampler=resampler, interlacing=interlacing, compensate=compensate)
+    oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter)
+    obin = po.BinBispectrumSugiyama(oattrs, edges=edges, ells=ells)
+    want = po.sugiyama_shotnoise(oattrs, obin, pos, w.astype('f8'), los=los, resampler=resampler, interlacing=interlacing,
+                                 compensate=compensate)
+    for ill, ell in enumerate(ells):
+        g, wv = np.asarray(got[ill]), np.asarray(want[ill])
+        scale = np.abs(want[0]).max()
+        print(f'[err] {los} {ell}: max |d| / max|SN_000| = {np.abs(g - wv).max() / scale:.2e}', flush=True)
+        assert np.all(np.abs(g - wv) <= 1e-5 * np.abs(wv) + 1e-5 * scale), (ell, np.abs(g - wv).max() / scale)

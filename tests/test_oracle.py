@@ -1155,3 +1155,45 @@ def test_sugiyama_counts_and_mask():
     assert np.array_equal(bin.nmodes, bin.nmodes1d[0][bin.iedges[:, 0]] * bin.nmodes1d[1][bin.iedges[:, 1]])
     diag = po.BinBispectrumSugiyama(attrs, edges=edges, ells=[(0, 0, 0)], diagonal=True)
     assert np.all(diag.iedges[:, 0] == diag.iedges[:, 1])
+
+
+def test_sugiyama_shotnoise_s111_known_answer():
+    """S111 with NGP, no interlacing, no compensation: the painted w^3 mesh is exactly the per-cell
+    sum, so s111_lm = sum over particles of w^3 Y*_lm(xhat of the particle's node)."""
+    attrs = po.Attrs(8, 80., (15., -10., 60.))
+    rng = np.random.default_rng(4)
+    nodes = rng.integers(0, 8, (500, 3))
+    pos = attrs.boxcenter - attrs.boxsize / 2. + nodes * attrs.cellsize + 0.3 * attrs.cellsize
+    w = rng.uniform(0.5, 1.5, 500)
+    bin = po.BinBispectrumSugiyama(attrs, edges={'step': 0.08}, ells=[(0, 0, 0), (2, 0, 2)])
+    parts = {}
+    po.sugiyama_shotnoise(attrs, bin, pos, w, los='z', resampler='ngp', interlacing=0, compensate=False, parts=parts)
+    xn = attrs.boxcenter - attrs.boxsize / 2. + nodes * attrs.cellsize
+    for (l, m), v in parts['s111'].items():
+        assert np.isclose(v, np.sum(w**3 * po.complex_ylm_conj(l, m, *xn.T)), rtol=1e-12, atol=1e-9), (l, m)
+    assert np.isclose(parts['s111'][0, 0], np.sum(w**3))
+
+
+@pytest.mark.parametrize('ell', [0, 1, 2, 3])
+def test_sugiyama_s122_kernel_addition_theorem(ell):
+    """sum_m Y_lm(khat) conj(FFT(rho Y_lm(xhat)))(k) = sum_x rho(x) exp(+i k.x) P_l(khat.xhat), the
+    addition theorem of the reduced harmonics that the S122 term rests on, by direct sums on a
+    grid without the origin node (k = 0 excluded)."""
+    attrs = po.Attrs((6, 4, 5), (60., 40., 50.), (13., -7., 40.))
+    rng = np.random.default_rng(ell)
+    rho = rng.normal(size=(6, 4, 5))
+    X = np.meshgrid(*attrs.rcoords(), indexing='ij')
+    kk = po.full_kvec(attrs)
+    K = np.meshgrid(*kk, indexing='ij')
+    got = 0.
+    for m in range(-ell, ell + 1):
+        got = got + np.conj(po.complex_ylm_conj(ell, m, *K)) * np.conj(np.fft.fftn(rho * np.conj(po.complex_ylm_conj(ell, m, *X))))
+    xv = np.stack([x.ravel() for x in X], axis=1)
+    xh = xv / np.linalg.norm(xv, axis=1)[:, None]
+    kv = np.stack([k.ravel() for k in K], axis=1)
+    idx = np.stack(np.meshgrid(*[np.arange(n) for n in (6, 4, 5)], indexing='ij'), axis=-1).reshape(-1, 3)
+    for i in range(1, len(kv), 7):
+        kh = kv[i] / np.linalg.norm(kv[i])
+        phase = np.exp(2j * np.pi * (idx @ (idx[i] / np.array([6, 4, 5]))))
+        want = np.sum(rho.ravel() * phase * special.eval_legendre(ell, xh @ kh))
+        assert np.isclose(got.ravel()[i], want, rtol=1e-10, atol=1e-10 * np.abs(rho).sum())
