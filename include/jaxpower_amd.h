@@ -251,6 +251,16 @@ int mp_bispec_shells(void* stream, const void* A, int64_t pitch, const int32_t m
 int mp_bispec_shells_ylm(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double* kx, const double* ky,
                          const double* kz, const double* thr, int nedges, int b0, int nshell, int ell, int m, float* out,
                          void* workspace, size_t workspace_bytes);
+/* The Bessel-filtered meshes of BinMesh3CorrelationPoles.__call__, Sugiyama basis
+ * (jaxpower/mesh3.py:383-395), weighted by a real harmonic for compute_mesh3_correlation
+ * (mesh3.py:690-716): for each separation s[q] (host array, q < ns),
+ * ifftn(A(k) j_ellj(|k| s[q]) [klo <= |k| < khi] Y_lm(k)) = out_h[q] + i out_a[q] on the full
+ * complex grid (k the fftfreq wavenumber vector of each index, -Nyquist at N/2), A a half
+ * spectrum of a real mesh; kfun the fundamental wavenumbers.  l <= 4 (l = -1: no harmonic,
+ * the filters of the bin call itself), ellj <= 5; workspace as mp_bispec_shells. */
+int mp_bessel_shells_ylm(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double kfun[3], const double* s,
+                         int ns, int ell, int m, int ellj, double klo, double khi, float* out_h, float* out_a, void* workspace,
+                         size_t workspace_bytes);
 /* The spherical-Bessel sums of the Sugiyama shot noise's last term (jaxpower/mesh3.py:875-903,
  * get_spherical_jn of jaxpower/utils.py:418-450): out[p] = sum over the cells s of F(s)
  * j_l1(|s| k1[b1]) j_l2(|s| k2[b2]) with (b1, b2) = pairs[2p .. 2p+1]; F a float32 real mesh,

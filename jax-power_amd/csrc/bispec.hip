@@ -85,6 +85,40 @@ shell_split_ylm_kernel(const float2* __restrict__ A, int64_t pitch, int n0, int 
   }
 }
 
+// The Bessel-filtered meshes of BinMesh3CorrelationPoles in the Sugiyama basis
+// (mesh3.py:383-395): B(k) = A(k) j_l(|k| s) [k0 <= |k| < k1] Y_lm(k_ref) on the full complex
+// grid, where k_ref is the reference's wavenumber vector of the mode's index (fftfreq on every
+// axis, so -Nyquist at index N/2).  With k' the partner index -k mod N, the hermitian part
+// B_h = A j (Y(k_ref) + Y(k'_ref)) / 2 and the antihermitian part B_a = A j (Y(k_ref) - Y(k'_ref)) / 2
+// give ifftn(B) = irfftn(B_h) + i irfftn(-i B_a) exactly, the Nyquist planes included (there
+// k'_ref is not -k_ref).  part 0 writes B_h, part 1 writes -i B_a, on the half spectrum.
+__global__ void __launch_bounds__(SH_NT)
+bessel_split_kernel(const float2* __restrict__ A, int64_t pitch, int n0, int n1, int n2, double kf0, double kf1, double kf2, int ell,
+                    int m, int ellj, double s, double klo, double khi, int part, float2* __restrict__ out, int64_t out_pitch) {
+  const int nc = n2 / 2 + 1;
+  const int64_t total = (int64_t)n0 * n1 * nc;
+  for (int64_t i = (int64_t)blockIdx.x * SH_NT + threadIdx.x; i < total; i += (int64_t)gridDim.x * SH_NT) {
+    const int iz = (int)(i % nc);
+    const int64_t row = i / nc;
+    const int iy = (int)(row % n1), ix = (int)(row / n1);
+    // signed index of fftfreq: i for i < (n + 1) / 2, i - n above (so -n/2 at n/2 for even n)
+    auto sgn = [](int q, int n) { return q < (n + 1) / 2 ? q : q - n; };
+    const double kx = sgn(ix, n0) * kf0, ky = sgn(iy, n1) * kf1, kz = sgn(iz, n2) * kf2;
+    const int jx = ix ? n0 - ix : 0, jy = iy ? n1 - iy : 0, jz = iz ? n2 - iz : 0;
+    const double px = sgn(jx, n0) * kf0, py = sgn(jy, n1) * kf1, pz = sgn(jz, n2) * kf2;
+    const double kn = sqrt(k2_sum(k2_xy(kx, ky), kz));
+    float2 v = make_float2(0.f, 0.f);
+    if (kn >= klo && kn < khi) {
+      const float2 a = A[row * pitch + iz];
+      const float yk = ell < 0 ? 1.f : ylm_of_vector(ell, m, (float)kx, (float)ky, (float)kz);
+      const float yp = ell < 0 ? 1.f : ylm_of_vector(ell, m, (float)px, (float)py, (float)pz);
+      const float w = (float)spherical_jn(ellj, kn * s) * (part == 0 ? 0.5f * (yk + yp) : 0.5f * (yk - yp));
+      v = part == 0 ? make_float2(a.x * w, a.y * w) : make_float2(a.y * w, -a.x * w);
+    }
+    out[row * out_pitch + iz] = v;
+  }
+}
+
 struct TriArgs {
   const float* stack[3];  // distinct shell stacks (up to three), shell q of stack s at stack[s] + q * shell_stride
   int nshell[3];
@@ -317,6 +351,42 @@ MP_EXPORT int mp_bispec_shells_ylm(void* stream, const void* A, int64_t pitch, c
               meshsize[0], meshsize[1], nc, kx, ky, kz, thr, nedges, b0 + q, ell, m, scratch, sp);
     rc = irfft3d(s, scratch, sp, out + q * cells, meshsize, scratch, sp);
     if (rc) return rc;
+  }
+  return MP_OK;
+}
+
+// Real meshes of the Bessel filters of the Sugiyama three-point correlation function, one per
+// separation s[q], q < ns: ifftn(A(k) j_ellj(|k| s[q]) [klo <= |k| < khi] Y_lm(k_ref)) =
+// out_h[q] + i out_a[q] on the full complex grid (mesh3.py:383-395), A a half spectrum.  s: host
+// array; kfun: the fundamental wavenumbers; l <= 4 (l = -1: no harmonic), ellj <= 5.
+MP_EXPORT int mp_bessel_shells_ylm(void* stream, const void* A, int64_t pitch, const int32_t meshsize[3], const double kfun[3],
+                                   const double* s, int ns, int ell, int m, int ellj, double klo, double khi, float* out_h,
+                                   float* out_a, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(A && meshsize && kfun && s && out_h && out_a, "mp_bessel_shells_ylm: null pointer");
+  MP_REQUIRE(ns >= 1, "mp_bessel_shells_ylm: no separation");
+  MP_REQUIRE(ell == -1 || (ell >= 0 && ell <= MAX_ELL && m >= -ell && m <= ell), "mp_bessel_shells_ylm: (ell, m) = (%d, %d) not supported",
+             ell, m);
+  MP_REQUIRE(ellj >= 0 && ellj <= 5, "mp_bessel_shells_ylm: Bessel order %d not in 0..5", ellj);
+  MP_REQUIRE(pitch >= meshsize[2] / 2 + 1, "mp_bessel_shells_ylm: pitch too small");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t sp = spectrum_pitch(meshsize[2]);
+  Workspace ws(workspace, workspace_bytes);
+  float2* scratch = ws.take<float2>((size_t)meshsize[0] * meshsize[1] * sp);
+  if (!scratch) return set_error(MP_ERR_WORKSPACE, "mp_bessel_shells_ylm: workspace too small");
+  const int nc = meshsize[2] / 2 + 1;
+  const int64_t total = (int64_t)meshsize[0] * meshsize[1] * nc;
+  int ncu = 0;
+  int rc = multiprocessor_count(&ncu);
+  if (rc) return rc;
+  const int blocks = (int)std::min<int64_t>(ceil_div(total, SH_NT), (int64_t)ncu * 16);
+  const int64_t cells = (int64_t)meshsize[0] * meshsize[1] * meshsize[2];
+  for (int q = 0; q < ns; ++q) {
+    for (int part = 0; part < 2; ++part) {
+      MP_LAUNCH("bessel_shell_split", bessel_split_kernel, dim3((unsigned)blocks), dim3(SH_NT), 0, st, (const float2*)A, pitch, meshsize[0],
+                meshsize[1], meshsize[2], kfun[0], kfun[1], kfun[2], ell, m, ellj, s[q], klo, khi, part, scratch, sp);
+      rc = irfft3d(st, scratch, sp, (part ? out_a : out_h) + q * cells, meshsize, scratch, sp);
+      if (rc) return rc;
+    }
   }
   return MP_OK;
 }

@@ -13,11 +13,11 @@ from .mesh2 import (BinMesh2SpectrumPoles, BinMesh2CorrelationPoles, compute_mes
                     compute_mesh2_correlation, compute_mesh2_spectrum_window, compute_mesh2_spectrum_mean,
                     compute_fkp2_normalization,
                     compute_box2_normalization, compute_fkp2_shotnoise)
-from .mesh3 import (BinMesh3SpectrumPoles, compute_mesh3, compute_mesh3_spectrum, compute_box3_normalization, compute_fkp3_normalization,
-                    compute_fkp3_shotnoise)
+from .mesh3 import (BinMesh3SpectrumPoles, BinMesh3CorrelationPoles, compute_mesh3, compute_mesh3_spectrum, compute_mesh3_correlation,
+                    compute_box3_normalization, compute_fkp3_normalization, compute_fkp3_shotnoise)
 from .mock import generate_uniform_particles, generate_gaussian_mesh, generate_lognormal_particles
 from .types import (Mesh2SpectrumPole, Mesh2SpectrumPoles, Mesh2CorrelationPole, Mesh2CorrelationPoles, Mesh3SpectrumPole,
-                    Mesh3SpectrumPoles, WindowMatrix, read as read_observable, write as write_observable)
+                    Mesh3SpectrumPoles, Mesh3CorrelationPole, Mesh3CorrelationPoles, WindowMatrix, read as read_observable, write as write_observable)
 from .autodiff import mesh2_spectrum_poles, box_spectrum_values
 from .batch import Mesh2SpectrumBatch
 from . import resamplers
@@ -30,4 +30,5 @@ __all__ = ['MeshAttrs', 'BaseMeshField', 'RealMeshField', 'ComplexMeshField', 'P
            'generate_uniform_particles', 'generate_gaussian_mesh', 'generate_lognormal_particles', 'Mesh2SpectrumPole',
            'Mesh2SpectrumPoles', 'mesh2_spectrum_poles', 'box_spectrum_values', 'Mesh2SpectrumBatch', 'resamplers', 'BinMesh3SpectrumPoles',
            'compute_mesh3', 'compute_mesh3_spectrum', 'compute_box3_normalization', 'compute_fkp3_normalization', 'Mesh3SpectrumPole', 'Mesh3SpectrumPoles',
-           'compute_mesh2_spectrum_window', 'compute_mesh2_spectrum_mean', 'WindowMatrix', 'compute_fkp3_shotnoise']
+           'compute_mesh2_spectrum_window', 'compute_mesh2_spectrum_mean', 'WindowMatrix', 'compute_fkp3_shotnoise',
+           'BinMesh3CorrelationPoles', 'compute_mesh3_correlation', 'Mesh3CorrelationPole', 'Mesh3CorrelationPoles']

@@ -67,6 +67,8 @@ SIGNATURES = {
     'mp_bin_real_poles': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P_F64, P_F64, c_int, P_I32,
                                   P_I32, c_void_p, c_void_p, c_size_t]),
     'mp_bispec_shells_workspace_bytes': (c_size_t, [P_I32]),
+    'mp_bessel_shells_ylm': (c_int, [c_void_p, c_void_p, c_int64, P_I32, P_F64, P_F64, c_int, c_int, c_int, c_int, c_double, c_double,
+                                     c_void_p, c_void_p, c_void_p, c_size_t]),
     'mp_hankel_pairs_workspace_bytes': (c_size_t, [c_int]),
     'mp_hankel_pairs': (c_int, [c_void_p, c_void_p, P_I32, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int,
                                 c_void_p, c_int, c_void_p, c_void_p, c_size_t]),

@@ -15,8 +15,9 @@ The Sugiyama basis (the reference's default) bins pairs of shells (k1, k2): for 
 g N^6 sum_x ifftn(A1 [b1] Y*_l1m1) ifftn(A2 [b2] Y*_l2m2) Y*_LM(los) rho_3 (mesh3.py:595-621).
 The complex harmonics are combinations of two real ones, and ifftn(A [b] Y_lm) of a real
 harmonic is a real mesh (even l) or i times one (odd l), so every term is a sum of real
-triple products taken by the same triangle pass.  The three-point correlation function is
-not implemented on this path and raises NotImplementedError.
+triple products taken by the same triangle pass.  The three-point correlation function
+(BinMesh3CorrelationPoles, compute_mesh3_correlation, mesh3.py:316-409, 632-727) replaces the
+shell indicators by spherical Bessel filters j_l(|k| s_b) on the full complex grid.
 """
 import math
 
@@ -517,11 +518,195 @@ def _mesh3_spectrum_local(meshes, bin, mattrs):
     return Mesh3SpectrumPoles(poles, attrs=attrs)
 
 
+class BinMesh3CorrelationPoles:
+    """Binning operator for the three-point correlation function (mesh3.py:316-409): separation
+    bins of the full real grid per side (every cell weighted 1, as BinMesh2CorrelationPoles),
+    pairs (Sugiyama) or triangles (Scoccimarro) that pass the mask, ``nmodes`` the products of
+    the per-side cell counts, ``xavg`` the per-side mean |s|.  The call filters a mesh's half
+    spectrum by j_l(|k| s_b) for each bin b (s_b its mean |s|, mesh3.py:383-395) instead of a
+    shell indicator, optionally restricted to klimit[0] <= |k| < klimit[1] (True: up to the
+    smallest Nyquist wavenumber).  Only the Sugiyama basis has an estimator, as in the reference
+    (compute_mesh3_correlation raises for Scoccimarro, mesh3.py:680-682)."""
+
+    def __init__(self, mattrs, edges=None, ells=0, basis='sugiyama', batch_size=None, buffer_size=0, mask_edges=None, klimit=None,
+                 device=None):
+        from .mesh2 import BinMesh2CorrelationPoles
+        if not isinstance(mattrs, MeshAttrs):
+            mattrs = mattrs.attrs
+        if basis not in _BASES:
+            raise ValueError(f'basis must be one of {_BASES}, got {basis!r}')
+        device = device or _default_device()
+        ndim = 2 if 'sugiyama' in basis else 3
+        diagonal = 'diagonal' in basis
+        self.mattrs, self.basis, self._device = mattrs, basis, device
+        self.batch_size = 1 if batch_size is None else int(batch_size)
+        self.buffer_size = int(buffer_size)
+        self.ells = _format_ells(ells, basis=basis)
+        if edges is None:
+            edges = {}
+        if not isinstance(edges, (tuple, list)):
+            edges = [edges] * ndim
+        axes = []
+        for e in list(edges)[:ndim]:
+            axis = BinMesh2CorrelationPoles(mattrs, edges=e, device=device)
+            same = [a for a in axes if np.array_equal(a._edges1d, axis._edges1d)]
+            axes.append(same[0] if same else axis)
+        axes = axes + [axes[-1]] * (ndim - len(axes))
+        self._axes = axes
+        self.edges1d = tuple(a.edges for a in axes)
+        self.nmodes1d = tuple(a.nmodes for a in axes)
+        self.xavg1d = tuple(a.xavg for a in axes)
+        nyq = float(np.min(mattrs.cellsize)) * np.min(mattrs.meshsize) / 2.
+        mask = _mask_function(_default_mask(basis) if mask_edges is None else mask_edges, nyq)
+        lo = _product([e[:, 0] for e in self.edges1d], diagonal)
+        hi = _product([e[:, 1] for e in self.edges1d], diagonal)
+        edges3 = np.stack([lo, hi], axis=-1)
+        keep = np.asarray(mask(*[edges3[:, i, :] for i in range(ndim)]), dtype=bool)
+        self.edges = edges3[keep]
+        self.xavg = _product(self.xavg1d, diagonal)[keep]
+        self._iedges = _product([np.arange(len(x)) for x in self.xavg1d], diagonal)[keep].astype('i4')
+        if not len(self._iedges):
+            raise ValueError('no pair passes the edge mask')
+        nmodes = np.prod(np.stack([np.asarray(self.nmodes1d[a])[self._iedges[:, a]] for a in range(ndim)]), axis=0)
+        self.nmodes = [nmodes] * len(self.ells)
+        if isinstance(klimit, bool):
+            klimit = (0., float(np.min(mattrs.knyq))) if klimit else None
+        self.klimit = klimit
+        self.wmodes = None
+
+    @property
+    def nbins(self):
+        return len(self.xavg)
+
+    _pair_sums = BinMesh3SpectrumPoles._pair_sums
+
+    def _bessel_stack(self, side, spec, ellj, ell, mu, cache):
+        """(R_h, R_a): real meshes per separation bin b of the side with
+        ifftn(A j_ellj(|k| s_b) [klimit] Y_l,mu(khat)) = R_h[b] + i R_a[b] (ell = -1: no harmonic)."""
+        key = (spec[0].data_ptr(), side, ellj, ell, mu)
+        if key in cache:
+            return cache[key]
+        n = _lib.i32x3(self.mattrs.meshsize)
+        shape = tuple(int(x) for x in self.mattrs.meshsize)
+        s = np.ascontiguousarray(self.xavg1d[side], dtype='f8')
+        ws = _lib.workspace(_lib.load().mp_bispec_shells_workspace_bytes(n), self._device)
+        rh = torch.empty((len(s),) + shape, dtype=torch.float32, device=self._device)
+        ra = torch.empty_like(rh)
+        klo, khi = (-1., np.inf) if self.klimit is None else (float(self.klimit[0]), float(self.klimit[1]))
+        _lib.call('mp_bessel_shells_ylm', _lib.stream_handle(self._device), _lib.ptr(spec[0]), spec[1], n, _lib.f64x3(self.mattrs.kfun),
+                  s.ctypes.data_as(_lib.P_F64), len(s), int(ell), int(mu), int(ellj), klo, khi, _lib.ptr(rh), _lib.ptr(ra), _lib.ptr(ws),
+                  ws.numel())
+        cache[key] = (rh, ra)
+        return cache[key]
+
+    def _complex_pair_sums(self, F0, F1, H):
+        """sum_x F0[b1] F1[b2] H[j] for complex F = (R_h, R_a): (npairs, len(H)) complex128."""
+        hh = self._pair_sums(F0[0], F1[0], H)
+        aa = self._pair_sums(F0[1], F1[1], H)
+        ha = self._pair_sums(F0[0], F1[1], H)
+        ah = self._pair_sums(F0[1], F1[0], H)
+        return torch.complex(hh - aa, ha + ah)
+
+    def __call__(self, *meshes, ell=(0, 0, 0), remove_zero=False):
+        """i^(l1 + l2) / Vcell^2 sum_x c2r(A1 j_l1(|k| s_b1)) c2r(A2 j_l2(|k| s_b2)) m3 for every pair
+        (mesh3.py:362-395), complex; remove_zero sets the k = 0 mode of the first two meshes to 0
+        and subtracts the mean of the third."""
+        if 'sugiyama' not in self.basis:
+            raise NotImplementedError('the three-point correlation function is built in the Sugiyama basis only')
+        if len(meshes) != 3:
+            raise ValueError('the binning takes three meshes')
+        spectra = []
+        for mesh in meshes[:2]:
+            value = mesh.value if isinstance(mesh, ComplexMeshField) else mesh
+            _lib.require_device(value, 'mesh')
+            if remove_zero:
+                value = value.clone()
+                value[0, 0, 0] = 0.
+            spectra.append(_as_spectrum(value))
+        third = meshes[2].value if isinstance(meshes[2], RealMeshField) else meshes[2]
+        third = third.to(torch.float32).contiguous()
+        if remove_zero:
+            third = third - third.double().mean().float()
+        cache = {}
+        ell = tuple(int(v) for v in ell)[:2]
+        F0 = self._bessel_stack(0, spectra[0], ell[0], -1, 0, cache)
+        F1 = self._bessel_stack(1, spectra[1], ell[1], -1, 0, cache)
+        norm = (1j)**sum(ell) / float(np.prod(self.mattrs.cellsize))**2
+        return (self._complex_pair_sums(F0, F1, third[None])[:, 0] * norm).cpu().numpy()
+
+
+def compute_mesh3_correlation(*meshes, bin=None, los='x'):
+    """Three-point correlation multipoles in the Sugiyama basis (mesh3.py:632-727): for each
+    (l1, l2, L) and each row (m1, m2, M, g) of nonzero Gaunt coefficient,
+    g i^(l1 + l2) / Vcell^2 sum_x c2r(A1 Y*_l1m1 j_l1(|k| s1)) c2r(A2 Y*_l2m2 j_l2(|k| s2)) Y*_LM(los) rho_3,
+    real part; norm = N^3 / Vcell^2.  The complex harmonics are combinations of real ones and each
+    filtered mesh is R_h + i R_a on the full complex grid (mp_bessel_shells_ylm), so the sums are
+    real triple sums from the triangle pass with complex coefficients applied on the host."""
+    from .types import Mesh3CorrelationPole, Mesh3CorrelationPoles
+    if not isinstance(bin, BinMesh3CorrelationPoles):
+        raise TypeError('bin must be a BinMesh3CorrelationPoles')
+    if 'scoccimarro' in bin.basis:
+        raise NotImplementedError('the three-point correlation function in the Scoccimarro basis (mesh3.py:680-682)')
+    if isinstance(los, str) and los in ('firstpoint', 'endpoint'):
+        raise ValueError(f"the line of sight is 'local' or a global one, got {los!r} (mesh3.py:433-440)")
+    meshes, _ = format_meshes(*meshes, nmeshes=3)
+    mattrs = meshes[0].attrs
+    local = isinstance(los, str) and los == 'local'
+    vlos = None if local else _get_los_vector(los)
+    device = bin._device
+    spectra = [_as_spectrum(_2c(m).value) for m in meshes[:2]]
+    rho = _2r(meshes[2]).value.to(torch.float32).contiguous()
+    norm = float(np.prod(mattrs.meshsize, dtype='f8')) / float(np.prod(mattrs.cellsize))**2
+    vcell2 = float(np.prod(mattrs.cellsize))**2
+    attrs = dict(los='local' if local else vlos.tolist())
+    cache, thirds = {}, {}
+    if local:
+        from .mesh2 import _mul_ylm
+        offset = np.asarray(mattrs.boxcenter) - np.asarray(mattrs.boxsize) / 2.
+        xs = [torch.from_numpy(np.arange(int(n)) * float(d) + float(o)).to(device)
+              for n, d, o in zip(mattrs.meshsize, mattrs.cellsize, offset)]
+    poles = []
+    for ill, (ell1, ell2, ell3) in enumerate(bin.ells):
+        coef = {}
+        for m1, m2, m3, g in _triposh_rows(ell1, ell2, ell3, los_z=isinstance(los, str) and los == 'z'):
+            h = _ylm_components(ell3, m3) if local else {0: _complex_ylm_conj_at(ell3, m3, vlos)}
+            for mu1, c1 in _ylm_components(ell1, m1).items():
+                for mu2, c2 in _ylm_components(ell2, m2).items():
+                    d = coef.setdefault((mu1, mu2), {})
+                    for mu3, c3 in h.items():
+                        j = mu3 + ell3 if local else 0
+                        d[j] = d.get(j, 0.) + g * c1 * c2 * c3
+        if local:
+            if ell3 not in thirds:
+                H = torch.empty((2 * ell3 + 1,) + tuple(rho.shape), dtype=torch.float32, device=device)
+                for mu3 in range(-ell3, ell3 + 1):
+                    _mul_ylm(rho, H[mu3 + ell3], mattrs, xs, ell3, mu3, False)
+                thirds[ell3] = H
+            H = thirds[ell3]
+        else:
+            H = rho[None]
+        total = np.zeros(bin.nbins, dtype=complex)
+        for (mu1, mu2), d in coef.items():
+            if all(abs(c) < 1e-14 for c in d.values()):
+                continue
+            F0 = bin._bessel_stack(0, spectra[0], ell1, ell1, mu1, cache)
+            F1 = bin._bessel_stack(1, spectra[1], ell2, ell2, mu2, cache)
+            T = bin._complex_pair_sums(F0, F1, H).cpu().numpy()
+            for j, c in d.items():
+                total += c * T[:, j]
+        num = ((1j)**(ell1 + ell2) * total / vcell2).real
+        poles.append(Mesh3CorrelationPole(s=bin.xavg, s_edges=bin.edges, nmodes=bin.nmodes[ill], num_raw=num, norm=norm,
+                                          ell=(ell1, ell2, ell3), attrs=attrs, basis=bin.basis))
+    return Mesh3CorrelationPoles(poles, attrs=attrs)
+
+
 def compute_mesh3(*meshes, bin=None, los='z'):
-    """Dispatch on the binning operator (mesh3.py:475-506); only the bispectrum is built."""
+    """Dispatch on the binning operator (mesh3.py:475-506)."""
     if isinstance(bin, BinMesh3SpectrumPoles):
         return compute_mesh3_spectrum(*meshes, bin=bin, los=los)
-    raise NotImplementedError('the three-point correlation function is not implemented on this path')
+    if isinstance(bin, BinMesh3CorrelationPoles):
+        return compute_mesh3_correlation(*meshes, bin=bin, los=los)
+    raise TypeError(f'bin must be a BinMesh3SpectrumPoles or BinMesh3CorrelationPoles, not {type(bin)}')
 
 
 def compute_box3_normalization(*inputs, bin=None):

@@ -227,6 +227,34 @@ class Mesh3SpectrumPole(_MeshPole):
         return type(self)(**data, ell=self.ell, attrs=self.attrs, basis=self.basis)
 
 
+class Mesh3CorrelationPole(Mesh3SpectrumPole):
+    """Three-point correlation multipole (mesh3.py:723-725): s (npairs, 2) bin-averaged |s| of
+    the two separations, s_edges (npairs, 2, 2), nmodes (products of the two cell counts),
+    num_raw, num_shotnoise, norm, volume, ell (l1, l2, L) and the binning ``basis``."""
+
+    _x = 's'
+
+    def __init__(self, ell=0, attrs=None, basis='sugiyama', **fields):
+        super().__init__(ell=ell, attrs=attrs, basis=basis, **fields)
+
+    def select(self, **kwargs):
+        """Keep pairs whose separations all satisfy smin <= s < smax (s=(smin, smax)), or a
+        slice of pairs."""
+        sel = kwargs.pop('s', None)
+        if kwargs:
+            raise TypeError(f'select: unexpected arguments {sorted(kwargs)}')
+        if sel is None:
+            return self
+        if isinstance(sel, slice):
+            mask = np.zeros(len(self._data['s']), dtype=bool)
+            mask[sel] = True
+        else:
+            ss = self._data['s']
+            mask = np.all((ss >= sel[0]) & (ss < sel[1]), axis=-1)
+        data = {name: v[mask] for name, v in self._data.items()}
+        return type(self)(**data, ell=self.ell, attrs=self.attrs, basis=self.basis)
+
+
 class _MeshPoles:
 
     _pole = Mesh2SpectrumPole
@@ -312,6 +340,10 @@ class Mesh3SpectrumPoles(_MeshPoles):
         super().write(fn, extra={'basis': np.array(self._branches[0].basis if self._branches else 'scoccimarro')})
 
 
+class Mesh3CorrelationPoles(Mesh3SpectrumPoles):
+    _pole = Mesh3CorrelationPole
+
+
 class WindowMatrix:
     """Window matrix W mapping a theory (input multipoles on input k bins) to the binned
     observable: observable = W . theory (mesh2.py:1453-1482).  ``observable`` holds the output
@@ -352,11 +384,12 @@ class WindowMatrix:
 
 
 def read(fn):
-    """Read a Mesh2SpectrumPoles, Mesh2CorrelationPoles or Mesh3SpectrumPoles written by ``write``."""
+    """Read a Mesh2SpectrumPoles, Mesh2CorrelationPoles, Mesh3SpectrumPoles or Mesh3CorrelationPoles
+    written by ``write``."""
     with np.load(fn, allow_pickle=False) as data:
         kind = str(data['kind']) if 'kind' in data.files else 'Mesh2SpectrumPoles'
         cls = {'Mesh2SpectrumPoles': Mesh2SpectrumPoles, 'Mesh2CorrelationPoles': Mesh2CorrelationPoles,
-               'Mesh3SpectrumPoles': Mesh3SpectrumPoles}[kind]
+               'Mesh3SpectrumPoles': Mesh3SpectrumPoles, 'Mesh3CorrelationPoles': Mesh3CorrelationPoles}[kind]
         attrs = {'los': data['los'].tolist()} if 'los' in data.files else {}
         extra = {'basis': str(data['basis'])} if 'basis' in data.files else {}
         poles = []

@@ -2054,3 +2054,120 @@ def sugiyama_shotnoise(attrs, bin, positions, weights=None, los='z', resampler='
     if parts is not None:
         parts['s111'] = s111
     return [o.real if (e[0] + e[1]) % 2 == 0 else o.imag for e, o in zip(ells, out)]
+
+
+# ----------------------------------------------------------------------------
+# Three-point correlation function, Sugiyama basis (mesh3.py:316-409, 632-727)
+# ----------------------------------------------------------------------------
+
+class BinCorrelation3Sugiyama:
+    """BinMesh3CorrelationPoles, basis 'sugiyama' or 'sugiyama-diagonal' (mesh3.py:316-409 with
+    _make_edges3(kind='real'), mesh3.py:23-177): per side, separation bins of the full real grid
+    (every cell weighted 1, as BinCorrelationPoles); pairs in lexicographic order (or the
+    diagonal) that pass the mask (mid1 <= mid2 and upper1 + upper2 <= cellsize_min N_min / 2 by
+    default); nmodes the product of the cell counts; xavg the mean |s| of each side's bin.  The
+    call's filter of bin b is j_l(|k| xavg_b) (times [klimit[0] <= |k| < klimit[1]])."""
+
+    def __init__(self, attrs, edges=None, ells=(0, 0, 0), mask_edges=None, diagonal=False, klimit=None):
+        ndim = 2
+        if not isinstance(edges, (list, tuple)):
+            edges = [edges] * ndim
+        self.attrs = attrs
+        axes = [BinCorrelationPoles(attrs, edges=e) for e in edges[:ndim]]
+        self.edges1d = [ax.edges for ax in axes]
+        self.nmodes1d = [ax.nmodes for ax in axes]
+        self.xavg1d = [ax.xavg for ax in axes]
+        self.ells = [tuple(int(v) for v in ell) for ell in (ells if np.ndim(ells[0]) else [ells])]
+
+        def prod(arrays):
+            if diagonal:
+                return np.column_stack([np.asarray(arrays[0])] * ndim)
+            grid = np.meshgrid(*arrays, indexing='ij')
+            return np.column_stack([g.ravel() for g in grid])
+
+        lo = prod([e[:, 0] for e in self.edges1d])
+        hi = prod([e[:, 1] for e in self.edges1d])
+        edges2 = np.stack([lo, hi], axis=-1)
+        nyq = attrs.cellsize.min() * np.min(attrs.meshsize) / 2.
+        if mask_edges is None:
+            mid = edges2.mean(axis=-1)
+            mask = (mid[:, 0] <= mid[:, 1]) & (edges2[:, :, 1].sum(axis=1) <= nyq)
+        else:
+            assert mask_edges == '', 'the oracle takes the default mask or none'
+            mask = np.ones(len(edges2), dtype=bool)
+        self.edges = edges2[mask]
+        self.xavg = prod(self.xavg1d)[mask]
+        self.iedges = prod([np.arange(len(x)) for x in self.xavg1d])[mask]
+        self.nmodes = np.prod(prod(self.nmodes1d)[mask], axis=-1)
+        self.klimit = (0., float(np.min(attrs.knyq))) if klimit is True else klimit
+
+    def filters(self, axis, value, ell):
+        """ifftn(value j_ell(|k| xavg_b) [klimit]) for every bin b of side axis, value on the full
+        complex grid (mesh3.py:383-388 with the complex-dtype c2r)."""
+        kk = full_kvec(self.attrs)
+        kn = np.sqrt((kk[0][:, None, None]**2 + kk[1][None, :, None]**2) + kk[2][None, None, :]**2)
+        lim = 1. if self.klimit is None else ((kn >= self.klimit[0]) & (kn < self.klimit[1]))
+        return [np.fft.ifftn(value * special.spherical_jn(ell, kn * s) * lim) for s in self.xavg1d[axis]]
+
+    def reduce(self, F0, F1, third, ell=(0, 0)):
+        """i^(l1 + l2) / Vcell^2 sum_x filters(0, F0, l1)[b1] filters(1, F1, l2)[b2] third per pair
+        (mesh3.py:362-395)."""
+        f0, f1 = self.filters(0, F0, ell[0]), self.filters(1, F1, ell[1])
+        norm = (1j)**(ell[0] + ell[1]) / np.prod(self.attrs.cellsize)**2
+        return np.array([np.sum(f0[b1] * f1[b2] * third) for b1, b2 in self.iedges]) * norm
+
+
+def correlation3_sugiyama(attrs, bin, mesh1, mesh2=None, mesh3=None, los='z'):
+    """compute_mesh3_correlation, Sugiyama basis (mesh3.py:632-727): for each (l1, l2, L) and each
+    row (m1, m2, M, g) of triposh_rows, g bin.reduce(A1 Y*_l1m1(khat), A2 Y*_l2m2(khat),
+    Y*_LM(los) rho3, ell=(l1, l2)), summed over the rows, real part; A the full FFTs of the first
+    two (real) meshes, rho3 the third in configuration space; norm = N^3 / Vcell^2."""
+    meshes = [mesh1, mesh2, mesh3]
+    for i in (1, 2):
+        if meshes[i] is None:
+            meshes[i] = meshes[i - 1]
+    real = [c2r(m, attrs.meshsize) if np.iscomplexobj(m) else np.asarray(m, dtype='f8') for m in meshes]
+    A = [np.fft.fftn(r) for r in real[:2]]
+    K = np.meshgrid(*full_kvec(attrs), indexing='ij')
+    local = isinstance(los, str) and los == 'local'
+    L = np.meshgrid(*attrs.rcoords(), indexing='ij') if local else list(los_vector(los))
+    num = []
+    for ell1, ell2, ell3 in bin.ells:
+        total = np.zeros(len(bin.iedges), dtype=complex)
+        for m1, m2, m3, g in triposh_rows(ell1, ell2, ell3, los_z=isinstance(los, str) and los == 'z'):
+            total += g * bin.reduce(A[0] * complex_ylm_conj(ell1, m1, *K), A[1] * complex_ylm_conj(ell2, m2, *K),
+                                    complex_ylm_conj(ell3, m3, *L) * real[2], ell=(ell1, ell2))
+        num.append(total.real)
+    return dict(s=bin.xavg, s_edges=bin.edges, nmodes=bin.nmodes, num_raw=num,
+                norm=np.prod(attrs.meshsize) / np.prod(attrs.cellsize)**2, ells=bin.ells)
+
+
+def correlation3_sugiyama_bruteforce(attrs, bin, rmeshes, ell=(0, 0, 0), los='z'):
+    """The row sums of correlation3_sugiyama before the real part, as direct sums over pairs of
+    full-grid modes: sum over rows of g i^(l1 + l2) / (Vcell^2 N^6) sum_{k1, k2} a(k1) b(k2)
+    H(-(k1 + k2)), with a = A1 Y*(k1) j_l1(|k1| s1), b = A2 Y*(k2) j_l2(|k2| s2) and
+    H = fftn(Y*_LM(los) rho3), from sum_x ifftn(a) ifftn(b) h = N^-6 sum a b fftn(h)(-(k1 + k2))
+    (N^3 cells).  Tiny meshes only."""
+    n = np.array([int(x) for x in attrs.meshsize])
+    ncell = float(np.prod(n))
+    A = [np.fft.fftn(m) for m in rmeshes[:2]]
+    kk = full_kvec(attrs)
+    KX, KY, KZ = np.meshgrid(*kk, indexing='ij')
+    kn = np.sqrt((KX**2 + KY**2) + KZ**2).ravel()
+    local = isinstance(los, str) and los == 'local'
+    L = np.meshgrid(*attrs.rcoords(), indexing='ij') if local else list(los_vector(los))
+    idx = np.stack(np.meshgrid(*[np.arange(x) for x in n], indexing='ij'), axis=-1).reshape(-1, 3)
+    ell1, ell2, ell3 = ell
+    out = np.zeros(len(bin.iedges), dtype=complex)
+    lim = np.ones_like(kn) if bin.klimit is None else ((kn >= bin.klimit[0]) & (kn < bin.klimit[1])).astype(float)
+    for m1, m2, m3, g in triposh_rows(ell1, ell2, ell3, los_z=isinstance(los, str) and los == 'z'):
+        a0 = (A[0] * complex_ylm_conj(ell1, m1, KX, KY, KZ)).ravel() * lim
+        b0 = (A[1] * complex_ylm_conj(ell2, m2, KX, KY, KZ)).ravel() * lim
+        H = np.fft.fftn(complex_ylm_conj(ell3, m3, *L) * rmeshes[2])
+        k3 = (-(idx[:, None, :] + idx[None, :, :])) % n
+        Hk = H[k3[..., 0], k3[..., 1], k3[..., 2]]
+        for t, (b1, b2) in enumerate(bin.iedges):
+            a = a0 * special.spherical_jn(ell1, kn * bin.xavg1d[0][b1])
+            b = b0 * special.spherical_jn(ell2, kn * bin.xavg1d[1][b2])
+            out[t] += g * np.sum(a[:, None] * b[None, :] * Hk)
+    return out * (1j)**(ell1 + ell2) / np.prod(attrs.cellsize)**2 / ncell**2

@@ -1197,3 +1197,18 @@ def test_sugiyama_s122_kernel_addition_theorem(ell):
         phase = np.exp(2j * np.pi * (idx @ (idx[i] / np.array([6, 4, 5]))))
         want = np.sum(rho.ravel() * phase * special.eval_legendre(ell, xh @ kh))
         assert np.isclose(got.ravel()[i], want, rtol=1e-10, atol=1e-10 * np.abs(rho).sum())
+
+
+@pytest.mark.parametrize('ell,los,klimit', [((0, 0, 0), 'z', None), ((2, 0, 2), 'x', None), ((1, 1, 0), 'local', None),
+                                            ((2, 2, 2), 'local', True)])
+def test_correlation3_sugiyama_vs_mode_pair_sums(ell, los, klimit):
+    """The three-point correlation function's FFT reduction against direct sums over pairs of
+    full-grid modes (Nyquist planes included)."""
+    attrs = po.Attrs((6, 4, 6), (60., 40., 60.), (10., -20., 35.))
+    rng = np.random.default_rng(31)
+    meshes = [rng.normal(size=(6, 4, 6)) for _ in range(3)]
+    bin = po.BinCorrelation3Sugiyama(attrs, edges=np.array([0., 12., 25., 40.]), ells=[ell], mask_edges='', klimit=klimit)
+    got = po.correlation3_sugiyama(attrs, bin, *meshes, los=los)['num_raw'][0]
+    want = po.correlation3_sugiyama_bruteforce(attrs, bin, meshes, ell=ell, los=los).real
+    assert np.allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+    assert np.abs(want).max() > 0
