@@ -123,6 +123,21 @@ def _ylm_components(ell, m):
     return {p: c + 0j, -p: 1j * c}
 
 
+def _real_ylm_at(ell, m, v):
+    """get_Ylm(ell, m, real=True) (utils.py:191-334) at a nonzero vector v: sqrt((2l+1)/4pi)
+    [sqrt((l-|m|)!/(l+|m|)!) sqrt(2) (-1)^m for m != 0] P_l^|m|(cos theta) {cos m phi | sin |m| phi},
+    P_l^|m| with the Condon-Shortley phase."""
+    from scipy.special import lpmv
+    v = np.asarray(v, dtype='f8')
+    x, y, z = v / np.linalg.norm(v)
+    amp = math.sqrt((2 * ell + 1) / (4. * math.pi))
+    if m != 0:
+        amp *= math.sqrt(math.factorial(ell - abs(m)) / math.factorial(ell + abs(m))) * math.sqrt(2.) * (-1.)**m
+    phi = math.atan2(y, x)
+    trig = math.sin(abs(m) * phi) if m < 0 else (math.cos(m * phi) if m > 0 else 1.)
+    return float(amp * lpmv(abs(m), ell, z) * trig)
+
+
 def _complex_ylm_conj_at(ell, m, v):
     """get_Ylm(ell, m, reduced=True, real=False, conj=True) at a nonzero vector v:
     sqrt((l-|m|)!/(l+|m|)!) [(-1)^m if m < 0] P_l^|m|(cos theta) exp(-i m phi), with the
@@ -748,10 +763,12 @@ def compute_fkp3_shotnoise(*fkps, bin=None, los='z', resampler='cic', interlacin
     array per ell (real part for even ell, imaginary part for odd ell).  los='local' replaces A P_ell by
     H_ell(paint(w)) conj(paint(w^2)) in the first term and paint(w) conj(H_ell(paint(w^2))) in the
     second, H_ell(c) = 4 pi / (2 ell + 1) sum_m r2c(c2r(c) Y_lm(xhat)) Y_lm(khat) (mesh3.py:823-834)."""
-    if not isinstance(bin, BinMesh3SpectrumPoles):
-        raise TypeError('bin must be a BinMesh3SpectrumPoles')
+    if not isinstance(bin, (BinMesh3SpectrumPoles, BinMesh3CorrelationPoles)):
+        raise TypeError('bin must be a BinMesh3SpectrumPoles or BinMesh3CorrelationPoles')
     if isinstance(los, str) and los in ('firstpoint', 'endpoint'):
         raise ValueError(f"the bispectrum line of sight is 'local' or a global one, got {los!r} (mesh3.py:433-440)")
+    if isinstance(bin, BinMesh3CorrelationPoles) and 'scoccimarro' in bin.basis:
+        raise NotImplementedError('the Scoccimarro shot noise of the three-point correlation function (mesh3.py:1064-1066)')
     local = isinstance(los, str) and los == 'local'
     fkps, fields = format_meshes(*fkps, fields=fields, nmeshes=3)
     if fields[2] == fields[1] + 1 == fields[0] + 2:
@@ -860,10 +877,12 @@ def _sugiyama_shotnoise(particles, fields, bin, los, paint_kw):
     sum of an (anti)hermitian product); in S113 every ifftn is of a hermitian product and the
     angular factor is a sum of products of two real harmonics of shat, all on the device.  Returns
     one host array per multipole (real part for even l1 + l2, imaginary part otherwise)."""
-    from .mesh2 import _mul_ylm, _svectors
+    from .mesh2 import _mul_ylm, _svectors, BinMesh2CorrelationPoles
     mattrs = particles[0].attrs
     device = bin._device
     ells = bin.ells
+    correlation = isinstance(bin, BinMesh3CorrelationPoles)
+    vc2 = float(np.prod(mattrs.cellsize))**2
     local = isinstance(los, str) and los == 'local'
     vlos = None if local else _get_los_vector(los)
     n = _lib.i32x3(mattrs.meshsize)
@@ -917,7 +936,51 @@ def _sugiyama_shotnoise(particles, fields, bin, los, paint_kw):
         return sum(c * S[l][mu + l] for mu, c in _ylm_components(l, m).items())
 
     if all_equal and (0, 0, 0) in ells:
-        out[ells.index((0, 0, 0))] += s111(0, 0)
+        if correlation:  # both separation bins must contain s = 0 (mesh3.py:1075-1077)
+            mask = np.all((bin.edges[..., 0] <= 0.) & (bin.edges[..., 1] >= 0.), axis=1)
+            out[ells.index((0, 0, 0))] += s111(0, 0) * mask / vc2
+        else:
+            out[ells.index((0, 0, 0))] += s111(0, 0)
+
+    def s122_correlation(ps, ells1, axis):
+        """(2l+1) separation-bin means on side axis of sum_m Y_lm(shat) c2r(C conj(FFT(rho_c
+        Y_lm(los))) - s111_lm W) (mesh3.py:918-944), per l in ells1: by the addition theorem
+        4 pi/(2l+1) sum_mu Y_l,mu(shat) [c2r(C conj(FFT(rho_c Y_l,mu(xhat))))  (local) or
+        Y_l,mu(los) c2r(C conj(FFT(rho_c))) (global)] - 4 pi/(2l+1) sum_mu S_l,mu Y_l,mu(shat) c2r(W)."""
+        side = bin._axes[axis]
+        rc = centered(paint(ps[1], ps[1].weights * ps[2].weights))
+        C = complex_zero_dc(paint(ps[0], out_kind='complex'))
+        R = _as_spectrum(_2c(RealMeshField(rc, attrs=mattrs)).value)[0]
+        omega = c2r(W) if all_equal else None
+        res = {}
+        for l in ells1:
+            carry = torch.zeros(shape, dtype=torch.float32, device=device)
+            tmp = torch.empty(shape, dtype=torch.float32, device=device)
+            if local and l > 0:
+                for mu in range(-l, l + 1):
+                    _mul_ylm(rc, tmp, mattrs, xs, l, mu, False)
+                    t = c2r(prod_spectrum(_2c(RealMeshField(tmp, attrs=mattrs)).value, C))
+                    _mul_ylm(t, tmp, mattrs, ss, l, mu, False)
+                    carry.add_(tmp)
+            else:
+                t = c2r(prod_spectrum(R, C))
+                for mu in range(-l, l + 1):
+                    y = _real_ylm_at(l, mu, vlos) if l else 1. / math.sqrt(4. * math.pi)
+                    if y == 0.:
+                        continue
+                    _mul_ylm(t, tmp, mattrs, ss, l, mu, False)
+                    carry.add_(tmp, alpha=y)
+            if all_equal:
+                for mu in range(-l, l + 1):
+                    if S[l][mu + l] == 0.:
+                        continue
+                    _mul_ylm(omega, tmp, mattrs, ss, l, mu, False)
+                    carry.add_(tmp, alpha=-float(S[l][mu + l]))
+            carry.mul_(4. * math.pi / (2 * l + 1))
+            sums = side._sums_dev(carry, [0], [False])[0].cpu().numpy()
+            with np.errstate(invalid='ignore', divide='ignore'):
+                res[l] = (2 * l + 1) * sums / np.asarray(bin.nmodes1d[axis], dtype='f8')
+        return res
 
     def s122(ps, ells1, axis):
         """(2l+1) full-grid shell means on side axis, per l in ells1 (mesh3.py:855-883)."""
@@ -952,20 +1015,25 @@ def _sugiyama_shotnoise(particles, fields, bin, los, paint_kw):
                 res[l] = (2 * l + 1) * full / bin.nmodes1d[axis]
         return res
 
+    def s122_mask(axis):  # the other side's bin must contain s = 0 (mesh3.py:1098-1099)
+        if not correlation:
+            return 1.
+        return ((bin.edges[:, 1 - axis, 0] <= 0.) & (bin.edges[:, 1 - axis, 1] >= 0.)) / vc2
+
     if fields[1] == fields[2]:
         ells1 = [ell[0] for ell in ells if ell[2] == ell[0] and ell[1] == 0]
         if ells1:
-            res = s122(particles, ells1, 0)
+            res = (s122_correlation if correlation else s122)(particles, ells1, 0)
             for ill, ell in enumerate(ells):
                 if ell[2] == ell[0] and ell[1] == 0:
-                    out[ill] += res[ell[0]][bin._iedges[:, 0]]
+                    out[ill] += res[ell[0]][bin._iedges[:, 0]] * s122_mask(0)
     if fields[0] == fields[2]:
         ells2 = [ell[1] for ell in ells if ell[2] == ell[1] and ell[0] == 0]
         if ells2:
-            res = s122([particles[1], particles[0], particles[2]], ells2, 1)
+            res = (s122_correlation if correlation else s122)([particles[1], particles[0], particles[2]], ells2, 1)
             for ill, ell in enumerate(ells):
                 if ell[2] == ell[1] and ell[0] == 0:
-                    out[ill] += res[ell[1]][bin._iedges[:, 1]]
+                    out[ill] += res[ell[1]][bin._iedges[:, 1]] * s122_mask(1)
     if fields[0] == fields[1]:
         rc = centered(paint(particles[2]))
         Cw2 = complex_zero_dc(paint(particles[0], particles[0].weights * particles[1].weights, out_kind='complex'))
@@ -986,6 +1054,24 @@ def _sugiyama_shotnoise(particles, fields, bin, los, paint_kw):
             return taus[mu]
 
         ws = _lib.workspace(_lib.load().mp_hankel_pairs_workspace_bytes(min(bin.nbins, 1024)), device)
+        if correlation:
+            # sums over the intersection of the two separation bins of each pair, by fine bins on
+            # the unique edges and a rebinning matrix, divided by both cell counts (mesh3.py:1118-1128, 955-957)
+            lo = np.max(bin.edges[..., 0], axis=1)
+            hi = np.min(bin.edges[..., 1], axis=1)
+            empty = hi <= lo
+            lo, hi = np.where(empty, 0., lo), np.where(empty, 0., hi)
+            fine = np.unique(np.concatenate([lo, hi]))
+            rebin = ((fine[:-1][None, :] >= lo[:, None]) & (fine[1:][None, :] <= hi[:, None])).astype('f8')
+            inter_bin = BinMesh2CorrelationPoles(mattrs, edges=fine, device=device) if len(fine) >= 2 else None
+            counts = (np.asarray(bin.nmodes1d[0], dtype='f8')[bin._iedges[:, 0]] * np.asarray(bin.nmodes1d[1], dtype='f8')[bin._iedges[:, 1]])
+
+        def inter_sums(F):
+            if inter_bin is None:
+                return np.zeros(bin.nbins)
+            sums = inter_bin._sums_dev(F, [0], [False])[0].cpu().numpy()
+            with np.errstate(invalid='ignore', divide='ignore'):
+                return (rebin @ sums) / counts
 
         def hankel(F, l1, l2):
             res = np.empty(bin.nbins)
@@ -1024,6 +1110,12 @@ def _sugiyama_shotnoise(particles, fields, bin, los, paint_kw):
                     Ure.add_(tmp, alpha=float(c.real))
                 if c.imag != 0.:
                     Uim.add_(tmp, alpha=float(c.imag))
-            total = hankel(Ure, l1, l2) + 1j * hankel(Uim, l1, l2)
-            out[ill] += (1j)**(l1 + l2) * total
+            if correlation:
+                total = inter_sums(Ure) + 1j * inter_sums(Uim)
+                out[ill] += (-1.)**(l1 + l2) * total / vc2
+            else:
+                total = hankel(Ure, l1, l2) + 1j * hankel(Uim, l1, l2)
+                out[ill] += (1j)**(l1 + l2) * total
+    if correlation:
+        return tuple(o.real for o in out)
     return tuple(o.real if (ell[0] + ell[1]) % 2 == 0 else o.imag for ell, o in zip(ells, out))

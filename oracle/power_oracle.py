@@ -1974,7 +1974,12 @@ def sugiyama_shotnoise(attrs, bin, positions, weights=None, los='z', resampler='
         removed), (k1, k2) the pair's mean |k|;
     W = 1 (compensated) or the squared window (not compensated) with interlacing, else the
     aliased shot-noise factor times the squared compensation when compensated (mesh3.py:1081-1089).
-    Real part for even l1 + l2, imaginary part otherwise.  A dict given as parts receives the
+    Real part for even l1 + l2, imaginary part otherwise.  With a BinCorrelation3Sugiyama (the
+    three-point correlation function, mesh3.py:918-976 and the is_correlation branches of
+    1064-1180): S111 counts only pairs whose two bins contain s = 0, S122 bins
+    sum_m Y_lm(shat) ifftn(.) in separation and counts where the other bin contains s = 0, S113
+    sums U(s) over the intersection of the two bins divided by both cell counts with sign
+    (-1)^(l1 + l2), all divided by Vcell^2; real part.  A dict given as parts receives the
     pieces: 's111' {(l, m): value}, 's122' {(axis, l): per-shell array}, 's113' {ell: per-pair
     array}."""
     order = ORDERS[resampler]
@@ -2003,10 +2008,13 @@ def sugiyama_shotnoise(attrs, bin, positions, weights=None, los='z', resampler='
     ells = [tuple(e) for e in bin.ells]
     out = [np.zeros(len(bin.iedges), dtype=complex) for _ in ells]
     uells = sorted({l for e in ells for l in e})
+    correlation = isinstance(bin, BinCorrelation3Sugiyama)  # mesh3.py:1064-1180 with is_correlation
+    vc2 = np.prod(attrs.cellsize)**2
     rho3 = painted(w**3)
     s111 = {(l, m): np.sum(rho3 * complex_ylm_conj(l, m, *X)) for l in uells for m in range(-l, l + 1)}
     if (0, 0, 0) in ells:
-        out[ells.index((0, 0, 0))] += s111[0, 0]
+        mask0 = np.all((bin.edges[..., 0] <= 0.) & (bin.edges[..., 1] >= 0.), axis=1) / vc2 if correlation else 1.
+        out[ells.index((0, 0, 0))] += s111[0, 0] * mask0
 
     def shell_mean(value, axis):
         ib = bin.ibin1d[axis]
@@ -2027,13 +2035,17 @@ def sugiyama_shotnoise(attrs, bin, positions, weights=None, los='z', resampler='
             l = e[0] if axis == 0 else e[1]
             carry = 0.
             for m in range(-l, l + 1):
-                ylm_k = np.conj(complex_ylm_conj(l, m, KX, KY, KZ))  # Y_lm (not conjugated)
                 ylm_l = np.conj(complex_ylm_conj(l, m, *L))
-                carry = carry + ylm_k * (C * np.conj(np.fft.fftn(rc * ylm_l)) - s111[l, m] * W)
+                prod_ = C * np.conj(np.fft.fftn(rc * ylm_l)) - s111[l, m] * W
+                if correlation:  # Y_lm(shat) c2r(.), binned in separation (mesh3.py:918-944)
+                    carry = carry + np.conj(complex_ylm_conj(l, m, *ssv)) * np.fft.ifftn(prod_)
+                else:
+                    carry = carry + np.conj(complex_ylm_conj(l, m, KX, KY, KZ)) * prod_
             piece = (2 * l + 1) * shell_mean(carry, axis)
             if parts is not None:
                 parts.setdefault('s122', {})[axis, l] = piece
-            out[ells.index(e)] += piece[bin.iedges[:, axis]]
+            mask = ((bin.edges[:, 1 - axis, 0] <= 0.) & (bin.edges[:, 1 - axis, 1] >= 0.)) / vc2 if correlation else 1.
+            out[ells.index(e)] += piece[bin.iedges[:, axis]] * mask
     rc = painted(w)
     rc = rc - rc.mean()
     C2 = np.fft.fftn(painted(w * w))
@@ -2047,12 +2059,21 @@ def sugiyama_shotnoise(attrs, bin, positions, weights=None, los='z', resampler='
             U = U + g * t * complex_ylm_conj(l1, m1, *ssv) * complex_ylm_conj(l2, m2, *ssv)
         if np.ndim(U) == 0:
             continue
-        total = np.array([np.sum(U * special.spherical_jn(l1, snorm_ * a) * special.spherical_jn(l2, snorm_ * b)) for a, b in zip(k1, k2)])
-        out[ill] += (1j)**(l1 + l2) * total
+        if correlation:  # sums over the intersection of the pair's two separation bins (mesh3.py:947-976, 1118-1128)
+            lo = np.max(bin.edges[..., 0], axis=1)
+            hi = np.min(bin.edges[..., 1], axis=1)
+            total = np.array([np.sum(U[(snorm_ >= a) & (snorm_ < b)]) if b > a else 0. for a, b in zip(lo, hi)])
+            total = (-1.)**(l1 + l2) * total / (bin.nmodes1d[0][bin.iedges[:, 0]] * bin.nmodes1d[1][bin.iedges[:, 1]]) / vc2
+        else:
+            total = (1j)**(l1 + l2) * np.array([np.sum(U * special.spherical_jn(l1, snorm_ * a) * special.spherical_jn(l2, snorm_ * b))
+                                                for a, b in zip(k1, k2)])
+        out[ill] += total
         if parts is not None:
-            parts.setdefault('s113', {})[l1, l2, l3] = (1j)**(l1 + l2) * total
+            parts.setdefault('s113', {})[l1, l2, l3] = total
     if parts is not None:
         parts['s111'] = s111
+    if correlation:
+        return [o.real for o in out]
     return [o.real if (e[0] + e[1]) % 2 == 0 else o.imag for e, o in zip(ells, out)]
 
 
@@ -2074,6 +2095,7 @@ class BinCorrelation3Sugiyama:
             edges = [edges] * ndim
         self.attrs = attrs
         axes = [BinCorrelationPoles(attrs, edges=e) for e in edges[:ndim]]
+        self.ibin1d = [ax.ibin - 1 for ax in axes]  # separation bin of every cell (digitize - 1)
         self.edges1d = [ax.edges for ax in axes]
         self.nmodes1d = [ax.nmodes for ax in axes]
         self.xavg1d = [ax.xavg for ax in axes]

@@ -89,3 +89,27 @@ def test_corr3_bin_call_remove_zero(tmp_path):
     back = read_observable(fn)
     assert type(back).__name__ == 'Mesh3CorrelationPoles' and back.ells == ((0, 0, 0),)
     assert np.array_equal(back.get((0, 0, 0)).values('num_raw'), xi.get((0, 0, 0)).values('num_raw'))
+
+
+@pytest.mark.parametrize('los,resampler,interlacing,compensate,ells', [
+    ('z', 'cic', 2, True, [(0, 0, 0), (2, 0, 2), (0, 2, 2), (1, 1, 0)]),
+    ('local', 'tsc', 0, True, [(0, 0, 0), (2, 0, 2), (1, 1, 2)]),
+    ('x', 'ngp', 0, False, [(0, 0, 0), (0, 1, 1), (2, 2, 2)]),
+])
+def test_corr3_shotnoise_matches_oracle(los, resampler, interlacing, compensate, ells):
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh3CorrelationPoles, compute_fkp3_shotnoise
+    say('corr3 shot noise', los, resampler, interlacing, compensate)
+    attrs = MeshAttrs(meshsize=(16, 16, 12), boxsize=(160., 160., 120.), boxcenter=(20., -10., 90.))
+    pos, w = _mock(attrs, 6000, 23)
+    p = ParticleField(to_dev(pos), to_dev(w), attrs=attrs)
+    edges = {'step': 10.}
+    bin = BinMesh3CorrelationPoles(attrs, edges=edges, ells=ells)
+    got = compute_fkp3_shotnoise(p, bin=bin, los=los, resampler=resampler, interlacing=interlacing, compensate=compensate)
+    oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter)
+    obin = po.BinCorrelation3Sugiyama(oattrs, edges=edges, ells=ells)
+    want = po.sugiyama_shotnoise(oattrs, obin, pos, w.astype('f8'), los=los, resampler=resampler, interlacing=interlacing,
+                                 compensate=compensate)
+    scale = np.abs(want[0]).max()
+    for ill, ell in enumerate(ells):
+        g, wv = np.asarray(got[ill]), np.asarray(want[ill])
+        assert np.all(np.abs(g - wv) <= 1e-5 * np.abs(wv) + 1e-5 * scale), (ell, np.abs(g - wv).max() / scale)

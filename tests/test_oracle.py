@@ -1212,3 +1212,26 @@ def test_correlation3_sugiyama_vs_mode_pair_sums(ell, los, klimit):
     want = po.correlation3_sugiyama_bruteforce(attrs, bin, meshes, ell=ell, los=los).real
     assert np.allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
     assert np.abs(want).max() > 0
+
+
+def test_sugiyama_correlation_shotnoise_full_grid_sums():
+    """Three-point correlation shot noise with one separation bin holding every cell: summing
+    ifftn(P) over all cells gives P(k = 0), so for (0, 0, 0) the S122 sum is
+    C(0) conj(rho_c~(0)) - s111 W(0) = -s111 (k = 0 of the paint removed, W(0) = 1) and the S113
+    sum is -s111 likewise; the S111 term is s111 / Vcell^2."""
+    attrs = po.Attrs((8, 6, 8), (80., 60., 80.), (5., 3., 40.))
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(-1., 1., (800, 3)) * attrs.boxsize / 2. + attrs.boxcenter
+    w = rng.uniform(0.5, 1.5, 800)
+    bin = po.BinCorrelation3Sugiyama(attrs, edges=np.array([0., 1e3]), ells=[(0, 0, 0)], mask_edges='')
+    parts = {}
+    total = po.sugiyama_shotnoise(attrs, bin, pos, w, los='z', resampler='tsc', interlacing=2, compensate=True, parts=parts)[0]
+    s111 = parts['s111'][0, 0]
+    ncell = np.prod(attrs.meshsize)
+    vc2 = np.prod(attrs.cellsize)**2
+    assert bin.nmodes1d[0][0] == ncell
+    assert np.isclose(parts['s122'][0, 0][0] * ncell, -s111, rtol=1e-9)
+    assert np.isclose(parts['s122'][1, 0][0] * ncell, -s111, rtol=1e-9)
+    assert np.isclose(parts['s113'][0, 0, 0][0], -s111 / ncell**2 / vc2, rtol=1e-9)
+    want = s111 / vc2 + 2 * (-s111 / ncell) / vc2 + (-s111 / ncell**2 / vc2)
+    assert np.isclose(total[0], want, rtol=1e-9)
