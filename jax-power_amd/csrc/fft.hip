@@ -472,20 +472,37 @@ colpass_kernel(ColPassArgs a) {
   };
   int tile = blockIdx.x;
   if (tile < a.ntiles) load(tile);
-  while (tile < a.ntiles) {
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];
-    __syncthreads();
+  while (tile < a.ntiles) {  // (the first sync below also publishes the twiddle table)
     const int next = tile + gridDim.x;
-    if (next < a.ntiles) load(next);
-    fft_lds<L, C, C, 1, true, NT, INV, L>(lds, stw);
     float2* dst = base(tile);
+    if constexpr (RegFFT<L, C, NT>::ok) {
+      // first and last stages on registers; the middle stages in LDS
+      fft_first_from_regs<L, C, NT, INV>(reg, lds);
+      __syncthreads();
+      if (next < a.ntiles) load(next);
+      fft_stages_until<L, C, C, 1, true, NT, INV, RegFFT<L, C, NT>::R1, RegFFT<L, C, NT>::NSLAST, L>(lds, stw);
+      float2 out[EPT];
+      fft_last_to_regs<L, C, NT, INV, L>(lds, out, stw);
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const int idx = tid + e * NT, cc = idx % C, l = idx / C;
-      if (!valid(tile, cc)) continue;
-      if constexpr (C >= 16) store_stream(dst + l * a.fstride + cc, lds[idx]);
-      else dst[l * a.fstride + cc] = lds[idx];
+      for (int e = 0; e < EPT; ++e) {
+        const int idx = tid + e * NT, cc = idx % C, l = idx / C;
+        if (!valid(tile, cc)) continue;
+        if constexpr (C >= 16) store_stream(dst + l * a.fstride + cc, out[e]);
+        else dst[l * a.fstride + cc] = out[e];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];
+      __syncthreads();
+      if (next < a.ntiles) load(next);
+      fft_lds<L, C, C, 1, true, NT, INV, L>(lds, stw);
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int idx = tid + e * NT, cc = idx % C, l = idx / C;
+        if (!valid(tile, cc)) continue;
+        if constexpr (C >= 16) store_stream(dst + l * a.fstride + cc, lds[idx]);
+        else dst[l * a.fstride + cc] = lds[idx];
+      }
     }
     __syncthreads();
     tile = next;

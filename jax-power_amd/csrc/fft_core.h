@@ -161,6 +161,70 @@ __device__ __forceinline__ void fft_stages(float2* buf, const float2* __restrict
   }
 }
 
+template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int NS, int NSEND, int TWN>
+__device__ __forceinline__ void fft_stages_until(float2* buf, const float2* __restrict__ tw) {
+  if constexpr (NS < NSEND) {
+    constexpr int R = NS == 1 ? first_radix(L) : 8;
+    fft_stage<L, C, LS, CS, CFAST, NT, INV, R, NS, TWN>(buf, tw);
+    fft_stages_until<L, C, LS, CS, CFAST, NT, INV, NS * R, NSEND, TWN>(buf, tw);
+  }
+}
+
+// The column passes hold their tile in registers between the global loads/stores and LDS:
+// element e of thread t is (l = t / C + e * NT / C, c = t % C).  Those are whole butterflies of
+// the first Stockham stage (stride L / R1 in l) and whole outputs of the last one (positions
+// jj + r L / 8), so these two stages can run on registers, saving an LDS round trip each.
+template <int L, int C, int NT>
+struct RegFFT {
+  static constexpr int EPT = C * L / NT, S = NT / C, R1 = first_radix(L);
+  static constexpr int NSLAST = L / 8;  // the last stage (radix 8) when there are at least two
+  static constexpr bool ok = L > R1 && NT % C == 0 && EPT % R1 == 0 && (L / R1) % S == 0 && EPT % 8 == 0 && (L / 8) % S == 0;
+};
+
+// First stage (no twiddles) from the thread's registers into buf (element (l, c) at l * C + c);
+// the caller syncs before the next stage reads buf.
+template <int L, int C, int NT, bool INV>
+__device__ __forceinline__ void fft_first_from_regs(const float2* reg, float2* buf) {
+  using F = RegFFT<L, C, NT>;
+  constexpr int R = F::R1, NB = F::EPT / R;
+  const int c = threadIdx.x % C, j = threadIdx.x / C;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = reg[q + r * NB];  // element j + q S + r L / R
+    dft<R, INV>(v);
+    const int out = (j + q * F::S) * R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[(out + r) * C + c] = v[r];
+  }
+}
+
+// Last stage (radix 8, NS = L / 8) from buf (synced by the caller) into the thread's registers
+// in the load layout; buf is only read.
+template <int L, int C, int NT, bool INV, int TWN>
+__device__ __forceinline__ void fft_last_to_regs(const float2* buf, float2* reg, const float2* __restrict__ tw) {
+  using F = RegFFT<L, C, NT>;
+  constexpr int R = 8, NS = F::NSLAST, NB = F::EPT / R, TWSTEP = TWN / (NS * R);
+  const int c = threadIdx.x % C, j = threadIdx.x / C;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int jj = j + q * F::S;  // < NS, so k = jj and the outputs are jj + r NS
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = buf[(jj + r * NS) * C + c];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = tw[r * jj * TWSTEP];
+      if (INV) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+    dft<R, INV>(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) reg[q + r * NB] = v[r];
+  }
+}
+
 // Full in-place FFT of C columns in LDS (callers sync before; this syncs after), with a
 // TWN-entry twiddle table (normally an LDS copy made by load_twiddles).
 template <int L, int C, int LS, int CS, bool CFAST, int NT, bool INV, int TWN = TW_N>

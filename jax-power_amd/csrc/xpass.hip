@@ -167,8 +167,12 @@ xpass_kernel(XPassArgs a) {
   int tile = xcd_first_tile(), j = 0;
   if (tile < a.ntiles) load(tile, 0);
   while (tile < a.ntiles) {
+    if constexpr (RegFFT<L, C, NT>::ok) {
+      fft_first_from_regs<L, C, NT, false>(reg, lds);  // the first stage on registers
+    } else {
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];  // element (l, cc) at l * C + cc
+      for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];  // element (l, cc) at l * C + cc
+    }
     __syncthreads();
     int ntile = tile, nj = j + 1;
     if (nj == a.ni) {
@@ -176,7 +180,11 @@ xpass_kernel(XPassArgs a) {
       ntile += gridDim.x;
     }
     if (ntile < a.ntiles) load(ntile, nj);
-    fft_lds<L, C, C, 1, true, NT, false, L>(lds, stw);
+    if constexpr (RegFFT<L, C, NT>::ok) {
+      fft_stages_until<L, C, C, 1, true, NT, false, RegFFT<L, C, NT>::R1, L, L>(lds, stw);
+    } else {
+      fft_lds<L, C, C, 1, true, NT, false, L>(lds, stw);
+    }
     const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
     const int kz = min(c0 + c, a.ncols - 1);
     if (phases) {
@@ -356,8 +364,12 @@ xpass_ylm_bin_kernel(YlmBinArgs a) {
   int tile = xcd_first_tile(), m = 0;
   if (tile < a.ntiles) load(tile, 0);
   while (tile < a.ntiles) {
+    if constexpr (RegFFT<L, C, NT>::ok) {
+      fft_first_from_regs<L, C, NT, false>(reg, lds);
+    } else {
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];
+      for (int e = 0; e < EPT; ++e) lds[tid + e * NT] = reg[e];
+    }
     __syncthreads();
     int ntile = tile, nm = m + 1;
     if (nm == a.nm) {
@@ -365,7 +377,11 @@ xpass_ylm_bin_kernel(YlmBinArgs a) {
       ntile += gridDim.x;
     }
     if (ntile < a.ntiles) load(ntile, nm);
-    fft_lds<L, C, C, 1, true, NT, false, L>(lds, stw);
+    if constexpr (RegFFT<L, C, NT>::ok) {
+      fft_stages_until<L, C, C, 1, true, NT, false, RegFFT<L, C, NT>::R1, L, L>(lds, stw);
+    } else {
+      fft_lds<L, C, C, 1, true, NT, false, L>(lds, stw);
+    }
     const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
     const int kz = min(c0 + c, a.ncols - 1);
     KPrologue kp = a.kp;
