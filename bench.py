@@ -357,14 +357,21 @@ def batch_rates(nmock=64, steps=3, warmup=1, seed=42, resamplers=('cic', 'tsc'),
         rates['plain_calls'] = {'mocks_per_s': round(nmock * steps / el, 2), 'ms_per_mock': round(el * 1e3 / (nmock * steps), 4)}
         if log:
             print(f'# {resampler} plain_calls: {rates["plain_calls"]}', file=sys.stderr, flush=True)
-        for mode, graph, streams in (('per_mock', False, 1), ('batched_graph', True, 1), ('batched_graph_4streams', True, 4)):
-            batch = Mesh2SpectrumBatch(fields, bin, resampler=resampler, interlacing=2, compensate=True, graph=graph, streams=streams)
+        def consume(specs):
+            # every multipole read back to the host, as the plain calls have them
+            return [[spec.get(ell).value() for ell in ELLS] for spec in specs]
+
+        modes = (('per_mock', False, 1, False), ('batched_graph', True, 1, False), ('batched_graph_4streams', True, 4, False),
+                 ('batched_merged', False, 1, True), ('batched_merged_graph', True, 1, True))
+        for mode, graph, streams, merged in modes:
+            batch = Mesh2SpectrumBatch(fields, bin, resampler=resampler, interlacing=2, compensate=True, graph=graph, streams=streams,
+                                       merged=merged)
             for _ in range(max(warmup, 1)):
-                batch()
+                consume(batch())
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(steps):
-                batch()
+                consume(batch())
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             rates[mode] = {'mocks_per_s': round(nmock * steps / el, 2), 'ms_per_mock': round(el * 1e3 / (nmock * steps), 4)}

@@ -86,6 +86,31 @@ int mp_bin_poles_partial(void* stream, const void* partial, int64_t pitch, const
                          const double* kz, const double* k2_thresholds, int nedges, const double los[3], int nprod,
                          const int32_t* prod_ell, const int32_t* prod_remove_zero, double* out, void* workspace,
                          size_t workspace_bytes);
+/* Batches of mocks on one mesh geometry (the per-mock loop of the reference's measurement
+ * scripts, scripts/abacus_hf.py:357-373, which calls _paint and compute_mesh2_spectrum once per
+ * mock) as one launch per phase: nmock catalogues stacked in one particle array, catalogue m
+ * holding particles moff[m] .. moff[m + 1] (moff: device int64, nmock + 1 entries, moff[0] = 0,
+ * np = moff[nmock] < 2^31).  No counterpart in the reference's API.
+ * mp_paint_partial_batch: mp_paint_partial for every catalogue (fused paint only; -2 where it
+ * does not apply), the nmock * interlacing partial spectra catalogue-major at
+ * partial + (m * interlacing + j) * N0 * N1 * pitch.
+ * mp_bin_poles_partial_batch: mp_bin_poles_partial for each catalogue of such a partial array,
+ * out[m][p][b][re, im].
+ * mp_catalogue_sums: out[m][0] = sum w, out[m][1] = sum w^2 over catalogue m in FP64 (weights
+ * null: unit weights), for the box normalization and shot noise of each mock. */
+int64_t mp_paint_partial_batch_workspace_bytes(const int32_t meshsize[3], int64_t np, int nmock, int order, int interlacing);
+int mp_paint_partial_batch(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+                           const int64_t* moff, int nmock, const int32_t meshsize[3], const double boxsize[3],
+                           const double boxcenter[3], int order, int interlacing, void* partial, int64_t pitch, void* workspace,
+                           size_t workspace_bytes);
+int64_t mp_bin_partial_batch_workspace_bytes(const int32_t meshsize[3], int interlacing, int nprod, int nedges, int nmock);
+int mp_bin_poles_partial_batch(void* stream, const void* partial, int64_t pitch, int nmock, const int32_t meshsize[3], int order,
+                               int interlacing, int compensate, int zero_dc, const double* kx, const double* ky, const double* kz,
+                               const double* k2_thresholds, int nedges, const double los[3], int nprod, const int32_t* prod_ell,
+                               const int32_t* prod_remove_zero, double* out, void* workspace, size_t workspace_bytes);
+int64_t mp_catalogue_sums_workspace_bytes(int nmock);
+int mp_catalogue_sums(void* stream, const float* weights, const int64_t* moff, int nmock, double* out, void* workspace,
+                      size_t workspace_bytes);
 /* mp_bin_poles_partial without a workspace argument, for callers that cannot allocate device
  * memory (code under stream capture, serving loops, harnesses that forbid hipMalloc).  No
  * counterpart in the reference.  The combine tables and per-block partial histograms live in a

@@ -72,7 +72,7 @@ struct RealBinArgs {
 };
 
 // out[e] = sum over blocks of partial[block][e], in a fixed order.
-int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out);
+int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out, int nmock = 1);
 
 // Products of stored half spectra binned with Legendre weights (bin.hip): product p is
 // conj(mesh[pi[p]]) mesh[pj[p]] (pi = -1: the constant 1), weight P_ell[p](mu), the (0,0,0) mode

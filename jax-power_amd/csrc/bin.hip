@@ -123,6 +123,8 @@ __global__ void __launch_bounds__(RED_NT)
 reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int size, double* __restrict__ out) {
   __shared__ double sh[RED_NT];
   const int e = blockIdx.x;
+  partial += (int64_t)blockIdx.y * nblocks * size;  // catalogue blockIdx.y of a batch
+  out += (int64_t)blockIdx.y * size;
   double s = 0.;
   for (int b = threadIdx.x; b < nblocks; b += RED_NT) s += partial[(int64_t)b * size + e];
   sh[threadIdx.x] = s;
@@ -134,8 +136,9 @@ reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int size
   if (threadIdx.x == 0) out[e] = sh[0];
 }
 
-int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out) {
-  MP_LAUNCH("bin_reduce", reduce_partials_kernel, dim3((unsigned)size), dim3(RED_NT), 0, s, partial, nblocks, size, out);
+int reduce_partials(hipStream_t s, const double* partial, int nblocks, int size, double* out, int nmock) {
+  MP_LAUNCH("bin_reduce", reduce_partials_kernel, dim3((unsigned)size, (unsigned)nmock), dim3(RED_NT), 0, s, partial, nblocks, size,
+            out);
   return MP_OK;
 }
 

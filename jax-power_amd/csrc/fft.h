@@ -106,11 +106,13 @@ size_t combine_partial_workspace_bytes(const int32_t n[3], int ni);
 int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                     int zero_dc, float2* out, int64_t out_pitch, void* ws, size_t ws_bytes, int power = 0);
 size_t xpass_tables_bytes(const int32_t n[3], int ni);  // the phase and compensation tables of one combine
-size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges);
+// With nmock > 1, partial holds the ni partial spectra of nmock catalogues one after the other
+// and out their nmock sums, catalogue-major (one launch, one grid row per catalogue).
+size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges, int nmock = 1);
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
                 double* out, void* ws, size_t ws_bytes, int linear = 0, const int32_t* ylm_m = nullptr,
-                bool fit_partials = false);
+                bool fit_partials = false, int nmock = 1);
 // Local line of sight for one ell, fused (xpass.hip): z/y passes of rho Y_lm(xhat) for all m,
 // then the x pass forms C = sum_m FFT(.) Y_lm(khat) per tile and bins conj(C) A0 w (and |A0|^2 w
 // when with0) without writing C; out[p][b][re, im] as mp_bin_poles writes it.

@@ -46,7 +46,40 @@ struct RowGeom {
   int lo, hi;          // stencil reach relative to the home cell
   int ni;
   int zfrac;           // fractional bits of z in the records
+  int nmock;           // catalogues stacked along x in the row sort (1 for one catalogue)
 };
+
+// Catalogues stacked in one particle array (a batch of mocks): catalogue m holds particles
+// [moff[m], moff[m + 1]).  A launch cuts every catalogue into chunks of its own size CH that
+// never straddle two catalogues; choff[m] is the number of chunks before catalogue m.  One
+// catalogue has moff == nullptr and its chunks are those of [0, np).
+struct Catalogues {
+  const int64_t* moff;  // device, nmock + 1 entries
+  const int* choff;     // device, nmock + 1 entries, for the launch's chunk size
+  int nmock;
+  int64_t np;
+};
+
+// Catalogue and particle range of chunk `chunk`; false past the last chunk.
+__device__ __forceinline__ bool chunk_range(const Catalogues& c, int chunk, int CH, int& m, int64_t& i0, int64_t& i1) {
+  if (!c.moff) {
+    m = 0;
+    i0 = (int64_t)chunk * CH;
+    i1 = i0 + CH < c.np ? i0 + CH : c.np;
+    return i0 < i1;
+  }
+  if (chunk >= c.choff[c.nmock]) return false;
+  int lo = 0, hi = c.nmock - 1;  // the last catalogue with choff <= chunk (it has chunks)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (c.choff[mid] <= chunk) lo = mid;
+    else hi = mid - 1;
+  }
+  m = lo;
+  i0 = c.moff[m] + (int64_t)(chunk - c.choff[m]) * CH;
+  i1 = i0 + CH < c.moff[m + 1] ? i0 + CH : c.moff[m + 1];
+  return i0 < i1;
+}
 
 // Home z cell and offset in [0, 1] of a record's z word (the offset rounds to float).
 __device__ __forceinline__ void z_of_record(unsigned zb, int zfrac, int& zc, float& fz) {
@@ -116,14 +149,16 @@ __device__ __forceinline__ float wave_max_f32(float v) {
 
 template <typename T>
 __global__ void __launch_bounds__(RS_NT)
-rowsort_hist_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int* __restrict__ bucket_total,
+rowsort_hist_kernel(const T* __restrict__ pos, const float* __restrict__ w, Catalogues cat, GridMap g, int* __restrict__ bucket_total,
                     unsigned* __restrict__ wmax_bits) {
   __shared__ int hist[RS_MAX_KEYS];
   __shared__ float wmax[RS_NT / 64];
+  int mk;
+  int64_t i0, i1;
+  if (!chunk_range(cat, blockIdx.x, RS_CHUNK, mk, i0, i1)) return;
+  bucket_total += (int64_t)mk * g.n[0];  // the x buckets of catalogue mk
   for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
   __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * RS_CHUNK;
-  const int64_t i1 = i0 + RS_CHUNK < np ? i0 + RS_CHUNK : np;
   float m = w ? 0.f : 1.f;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += RS_NT) {
     T xg[3];
@@ -139,7 +174,7 @@ rowsort_hist_kernel(const T* __restrict__ pos, const float* __restrict__ w, int6
   if (threadIdx.x == 0) {
     float bm = 0.f;
     for (int q = 0; q < RS_NT / 64; ++q) bm = fmaxf(bm, wmax[q]);
-    atomicMax(wmax_bits, __float_as_uint(bm));  // non-negative floats order like their bits
+    atomicMax(wmax_bits + mk, __float_as_uint(bm));  // non-negative floats order like their bits
   }
 }
 
@@ -186,7 +221,7 @@ constexpr int RS_SCHUNK = 6144;
 
 template <typename T>
 __global__ void __launch_bounds__(RS_NT)
-rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, int64_t np, GridMap g, int zfrac,
+rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, Catalogues cat, GridMap g, int zfrac,
                        int* __restrict__ bucket_cursor, float4* __restrict__ tmp_rec, short* __restrict__ tmp_key) {
   __shared__ int hist[RS_MAX_KEYS];
   __shared__ int lstart[RS_MAX_KEYS];
@@ -196,10 +231,13 @@ rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, i
   short* skey = reinterpret_cast<short*>(srec + RS_SCHUNK);
   short* sbkt = skey + RS_SCHUNK;
   const int n0 = g.n[0];
+  int mk;
+  int64_t i0, i1;
+  if (!chunk_range(cat, blockIdx.x, RS_SCHUNK, mk, i0, i1)) return;
+  bucket_cursor += (int64_t)mk * n0;  // the x buckets of catalogue mk
   for (int b = threadIdx.x; b < n0; b += RS_NT) hist[b] = 0;
   __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * RS_SCHUNK;
-  const int cnt = (int)(i0 + RS_SCHUNK < np ? RS_SCHUNK : np - i0);
+  const int cnt = (int)(i1 - i0);
   constexpr int PER = RS_SCHUNK / RS_NT;
   int rank[PER], hxq[PER], hyq[PER];
   float4 recq[PER];
@@ -243,13 +281,46 @@ rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, i
   }
 }
 
-// One workgroup per x bucket: counting sort by y into ysorted (the order the paint reads);
-// writes the row offsets and a bound on the sum of |w| of every row.  The bucket is moved in
+// Chunk counts before each catalogue for two chunk sizes (one workgroup; any number of
+// catalogues, RS_NT * 2 per round).
+__global__ void __launch_bounds__(RS_NT)
+catalogue_chunks_kernel(const int64_t* __restrict__ moff, int nmock, int ch_a, int* __restrict__ choff_a, int ch_b,
+                        int* __restrict__ choff_b) {
+  __shared__ int ca[2 * RS_NT], cb[2 * RS_NT];
+  __shared__ int wsum[RS_NT / 64 + 1];
+  int carry_a = 0, carry_b = 0;
+  for (int m0 = 0; m0 < nmock; m0 += 2 * RS_NT) {
+    const int n = min(2 * RS_NT, nmock - m0);
+    for (int q = threadIdx.x; q < n; q += RS_NT) {
+      const int64_t cnt = moff[m0 + q + 1] - moff[m0 + q];
+      ca[q] = (int)((cnt + ch_a - 1) / ch_a);
+      cb[q] = (int)((cnt + ch_b - 1) / ch_b);
+    }
+    __syncthreads();
+    const int ta = block_excl_scan(ca, n, wsum);
+    const int tb = block_excl_scan(cb, n, wsum);
+    for (int q = threadIdx.x; q < n; q += RS_NT) {
+      choff_a[m0 + q] = carry_a + ca[q];
+      choff_b[m0 + q] = carry_b + cb[q];
+    }
+    carry_a += ta;
+    carry_b += tb;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    choff_a[nmock] = carry_a;
+    choff_b[nmock] = carry_b;
+  }
+}
+
+// One workgroup per x bucket (bucket m * nx + x for x cell x of catalogue m): counting sort by
+// y into ysorted (the order the paint reads); writes the row offsets and a bound on the sum of
+// |w| of every row.  The bucket is moved in
 // sub-chunks of RS_SCHUNK records: each is ordered by y in LDS and every y run is written
 // contiguously after the records of that row already placed.
 __global__ void __launch_bounds__(RS_NT)
 rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restrict__ tmp_rec, const short* __restrict__ tmp_key, int ny,
-                    int nz, int64_t nkeys, const unsigned* __restrict__ wmax_bits, float4* __restrict__ ysorted,
+                    int nx, int64_t nkeys, const unsigned* __restrict__ wmax_bits, float4* __restrict__ ysorted,
                     int* __restrict__ offsets, float* __restrict__ row_wabs) {
   __shared__ int cnt[RS_MAX_KEYS];     // whole bucket: counts, then the next free slot of each row
   __shared__ int lcnt[RS_MAX_KEYS];    // sub-chunk counts
@@ -277,7 +348,7 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restri
   // depend on the order of the row's records (which the atomics leave unspecified), so it is
   // an integer sum: q_i = ceil(|w_i| / m * 2^24) with m the largest |w| of the catalogue, and
   // the bound is m * sum q_i / 2^24 >= sum |w|.
-  const float m = __uint_as_float(*wmax_bits);
+  const float m = __uint_as_float(wmax_bits[hx / nx]);  // of this bucket's catalogue
   const float qs = m > 0.f ? 16777216.f / m : 0.f;
   constexpr int PER = RS_SCHUNK / RS_NT;
   for (int sub = start; sub < end; sub += RS_SCHUNK) {
@@ -437,9 +508,13 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   const int j = k % rg.ni;
   const int rest = k / rg.ni;
   const int by = rest % rg.nby;
-  const int bx = (rest / rg.nby) * 8 + xcd;
+  const int nq = (rg.nbx + 7) >> 3;
+  const int q = rest / rg.nby;
+  const int mk = q / nq;  // catalogue: its rows follow those of the catalogues before it
+  const int bx = (q % nq) * 8 + xcd;
   if (bx >= rg.nbx) return;
   const int x0 = bx * RX, y0 = by * RY;
+  const int64_t key0 = (int64_t)mk * rg.n[0] * rg.n[1];
   const float s = (float)j / (float)rg.ni;
 
   for (int i = tid; i < M * RP; i += PZ_NT) lds2[i] = make_float2(0.f, 0.f);
@@ -457,7 +532,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   for (int i = tid; i < nseg; i += PZ_NT) {
     int ix = i / sy, iy = i % sy;
     int hx = pmod(x0 - hi + ix, rg.n[0]), hy = pmod(y0 - hi + iy, rg.n[1]);
-    int64_t key = (int64_t)hx * rg.n[1] + hy;
+    int64_t key = key0 + (int64_t)hx * rg.n[1] + hy;
     int st = offsets[key];
     int c = offsets[key + 1] - st;
     sstart[i] = st;
@@ -596,7 +671,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   // z-axis r2c of the NROW rows held in LDS
   fft_lds<M, NROW, RP, 1, true, PZ_NT, false, NZ>(lds2, stw);
   constexpr int NK = M / 2 + 1;
-  float2* dst = out + j * mesh_stride;
+  float2* dst = out + ((int64_t)mk * rg.ni + j) * mesh_stride;
   for (int idx = tid; idx < NROW * NK; idx += PZ_NT) {
     int r = idx / NK, kk = idx % NK;
     int rx = r / RY, ry = r % RY;
@@ -819,12 +894,15 @@ bool paint_zfft_supported(const int32_t n[3], int64_t np, int order, int ni) {
   return false;
 }
 
-size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np) {
-  int64_t nkeys = (int64_t)n[0] * n[1];
+size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np, int nmock) {
+  const int64_t nb = (int64_t)nmock * n[0];  // x buckets
+  const int64_t nkeys = nb * n[1];
   size_t npp = (size_t)(np > 0 ? np : 1);
   size_t bytes = align_up((size_t)(nkeys + 1) * 4, 256);          // row offsets
-  bytes += 2 * align_up((size_t)(n[0] + 2) * 4, 256);             // bucket start / cursor (+ max |w|)
-  bytes += scan_workspace_bytes(n[0] + 1);
+  bytes += 2 * align_up((size_t)(nb + 2) * 4, 256);               // bucket start / cursor
+  bytes += align_up((size_t)nmock * 4, 256);                      // max |w| per catalogue
+  bytes += 2 * align_up((size_t)(nmock + 1) * 4, 256);            // chunk offsets of the catalogues
+  bytes += scan_workspace_bytes(nb + 1);
   bytes += align_up(npp * 16, 256) * 2 + align_up(npp * 2, 256);  // tmp records, sorted records, tmp keys
   bytes += align_up((size_t)nkeys * 4, 256);                      // per-row sum of |w|
   return bytes + 256;
@@ -837,9 +915,10 @@ static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, co
   static_assert(lds <= PZ_LDS_BUDGET, "LDS budget");
   auto kern = paint_zfft_kernel<ORDER, M, RX, RY>;
   if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  // ids are laid out as ((q * nby + by) * ni + j) * 8 + xcd with bx = q * 8 + xcd
+  // ids are laid out as (((m * nq + q) * nby + by) * ni + j) * 8 + xcd with bx = q * 8 + xcd
+  // for catalogue m
   int64_t nq = ceil_div(rg.nbx, 8);
-  unsigned grid = (unsigned)(nq * rg.nby * rg.ni * 8);
+  unsigned grid = (unsigned)(rg.nmock * nq * rg.nby * rg.ni * 8);
   const float2* tw = twiddle_device_ptr();
   if (!tw) return set_error(MP_ERR_HIP, "paint_zfft: no twiddle table");
   MP_LAUNCH("paint_zfft", kern, dim3(grid), dim3(PZ_NT), lds, s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride, tw);
@@ -899,10 +978,10 @@ static int launch_pz_any(int order, int M, const PzShape& sh, hipStream_t s, con
 // and for small catalogues, which take the default shape.
 
 struct PzTuneKey {
-  int dev, n0, n1, n2, order, ni;
+  int dev, n0, n1, n2, order, ni, batch;
   bool operator<(const PzTuneKey& o) const {
-    const int a[6] = {dev, n0, n1, n2, order, ni}, b[6] = {o.dev, o.n0, o.n1, o.n2, o.order, o.ni};
-    for (int q = 0; q < 6; ++q)
+    const int a[7] = {dev, n0, n1, n2, order, ni, batch}, b[7] = {o.dev, o.n0, o.n1, o.n2, o.order, o.ni, o.batch};
+    for (int q = 0; q < 7; ++q)
       if (a[q] != b[q]) return a[q] < b[q];
     return false;
   }
@@ -937,7 +1016,7 @@ static int pz_choose(hipStream_t s, int order, int M, const int32_t n[3], int ni
   }
   int dev = 0;
   MP_CHECK_HIP(hipGetDevice(&dev));
-  PzTuneKey key{dev, n[0], n[1], n[2], order, ni};
+  PzTuneKey key{dev, n[0], n[1], n[2], order, ni, rg.nmock > 1};
   {
     std::lock_guard<std::mutex> lock(g_pz_tune_mu);
     auto it = g_pz_tuned.find(key);
@@ -1002,8 +1081,8 @@ int paint_tuning_report(char* buf, int64_t len) {
     const PzTuneEntry& e = kv.second;
     int count = 0;
     const PzShape* c = pz_candidates(k.n2 / 2, count);
-    int off = snprintf(line, sizeof(line), "paint_zfft n=%dx%dx%d order=%d interlacing=%d rows=%dx%d ms:", k.n0, k.n1, k.n2,
-                       k.order, k.ni, e.shape.rx, e.shape.ry);
+    int off = snprintf(line, sizeof(line), "paint_zfft n=%dx%dx%d order=%d interlacing=%d%s rows=%dx%d ms:", k.n0, k.n1, k.n2,
+                       k.order, k.ni, k.batch ? " batch" : "", e.shape.rx, e.shape.ry);
     for (int q = 0; q < count && off < (int)sizeof(line) - 32; ++q)
       off += snprintf(line + off, sizeof(line) - off, " %dx%d=%.4f", c[q].rx, c[q].ry, e.ms[q]);
     text += line;
@@ -1025,44 +1104,57 @@ struct RowSorted {
   const float* row_wabs;
 };
 
+// moff: nullptr for one catalogue of np particles, else the device offsets of nmock catalogues
+// stacked in pos/w (np in total), whose rows then follow each other: the row of home cell
+// (x, y) of catalogue m has key (m n0 + x) n1 + y.
 template <typename T>
 static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, const GridMap& g, void* ws, size_t ws_bytes,
-                    RowSorted& out) {
+                    RowSorted& out, const int64_t* moff = nullptr, int nmock = 1) {
   const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
-  const int64_t nkeys = (int64_t)n[0] * n[1];
   MP_REQUIRE(n[0] <= RS_MAX_KEYS && n[1] <= RS_MAX_KEYS, "row sort: meshsize beyond %d", RS_MAX_KEYS);
+  MP_REQUIRE(nmock >= 1 && (nmock == 1 || moff), "row sort: %d catalogues without offsets", nmock);
+  MP_REQUIRE(np < (int64_t)INT32_MAX, "row sort: %lld particles, the row offsets are 32-bit", (long long)np);
+  const int64_t nb = (int64_t)nmock * n[0];
+  const int64_t nkeys = nb * n[1];
   Workspace wsp(ws, ws_bytes);
   int* offsets = wsp.take<int>(nkeys + 1);
-  int* bstart = wsp.take<int>(n[0] + 1);
-  int* bcursor = wsp.take<int>(n[0] + 2);
-  int* scan_tmp = (int*)wsp.take<char>(scan_workspace_bytes(n[0] + 1));
+  int* bstart = wsp.take<int>(nb + 1);
+  int* bcursor = wsp.take<int>(nb + 2);
+  unsigned* wmax = wsp.take<unsigned>(nmock);
+  int* choff_a = wsp.take<int>(nmock + 1);
+  int* choff_b = wsp.take<int>(nmock + 1);
+  int* scan_tmp = (int*)wsp.take<char>(scan_workspace_bytes(nb + 1));
   float4* tmp_rec = wsp.take<float4>(np > 0 ? np : 1);
   float4* sorted = wsp.take<float4>(np > 0 ? np : 1);
   short* tmp_key = wsp.take<short>(np > 0 ? np : 1);
   float* row_wabs = wsp.take<float>(nkeys);
-  if (!offsets || !bstart || !bcursor || !scan_tmp || !tmp_rec || !sorted || !tmp_key || !row_wabs)
-    return set_error(MP_ERR_WORKSPACE, "row sort: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(n, np));
-  unsigned* wmax = reinterpret_cast<unsigned*>(bcursor + n[0] + 1);  // past the n0 + 1 cursors the scan writes
-  int rc = fill_i32_dev(s, bstart, n[0] + 1, 0);
+  if (!offsets || !bstart || !bcursor || !wmax || !choff_a || !choff_b || !scan_tmp || !tmp_rec || !sorted || !tmp_key || !row_wabs)
+    return set_error(MP_ERR_WORKSPACE, "row sort: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(n, np, nmock));
+  int rc = fill_i32_dev(s, bstart, nb + 1, 0);
   if (rc) return rc;
-  rc = fill_i32_dev(s, reinterpret_cast<int*>(wmax), 1, 0);
+  rc = fill_i32_dev(s, reinterpret_cast<int*>(wmax), nmock, 0);
   if (rc) return rc;
-  const unsigned nchunk = (unsigned)ceil_div(np, RS_CHUNK);
+  Catalogues ca{moff, choff_a, nmock, np}, cb{moff, choff_b, nmock, np};
+  if (moff) MP_LAUNCH("paint_catalogue_chunks", catalogue_chunks_kernel, dim3(1), dim3(RS_NT), 0, s, moff, nmock, RS_CHUNK, choff_a,
+                      RS_SCHUNK, choff_b);
+  // at most nmock partial chunks beyond np / CH: the extra blocks find no chunk and return
+  const int extra = moff ? nmock : 0;
   if (np > 0)
-    MP_LAUNCH("paint_rowsort_hist", rowsort_hist_kernel<T>, dim3(nchunk), dim3(RS_NT), 0, s, pos, w, np, g, bstart, wmax);
-  rc = exclusive_scan_i32(s, bstart, n[0] + 1, scan_tmp, bcursor);
+    MP_LAUNCH("paint_rowsort_hist", rowsort_hist_kernel<T>, dim3((unsigned)(ceil_div(np, RS_CHUNK) + extra)), dim3(RS_NT), 0, s,
+              pos, w, ca, g, bstart, wmax);
+  rc = exclusive_scan_i32(s, bstart, nb + 1, scan_tmp, bcursor);
   if (rc) return rc;
   if (np > 0) {
     constexpr size_t slds = (size_t)RS_SCHUNK * (sizeof(float4) + 2 * sizeof(short));
     auto skern = rowsort_scatter_kernel<T>;
     MP_CHECK_HIP(hipFuncSetAttribute((const void*)skern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));
-    MP_LAUNCH("paint_rowsort_scatter", skern, dim3((unsigned)ceil_div(np, RS_SCHUNK)), dim3(RS_NT), slds, s, pos, w, np, g,
-              zfrac_of(n[2]), bcursor, tmp_rec, tmp_key);
+    MP_LAUNCH("paint_rowsort_scatter", skern, dim3((unsigned)(ceil_div(np, RS_SCHUNK) + extra)), dim3(RS_NT), slds, s, pos, w, cb,
+              g, zfrac_of(n[2]), bcursor, tmp_rec, tmp_key);
   }
   constexpr size_t flds = (size_t)RS_SCHUNK * (sizeof(float4) + sizeof(short));
   MP_CHECK_HIP(hipFuncSetAttribute((const void*)rowsort_fine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
-  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)n[0]), dim3(RS_NT), flds, s, (const int*)bstart, tmp_rec,
-            (const short*)tmp_key, n[1], n[2], nkeys, (const unsigned*)wmax, sorted, offsets, row_wabs);
+  MP_LAUNCH("paint_rowsort_fine", rowsort_fine_kernel, dim3((unsigned)nb), dim3(RS_NT), flds, s, (const int*)bstart, tmp_rec,
+            (const short*)tmp_key, n[1], n[0], nkeys, (const unsigned*)wmax, sorted, offsets, row_wabs);
   out.sorted = sorted;
   out.offsets = offsets;
   out.row_wabs = row_wabs;
@@ -1071,16 +1163,18 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
 
 template <typename T>
 static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t np, const GridMap& g, int order, int ni,
-                            float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes) {
+                            float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes, const int64_t* moff,
+                            int nmock) {
   const int32_t n[3] = {g.n[0], g.n[1], g.n[2]};
   RowSorted rs;
-  int rc = row_sort<T>(s, pos, w, np, g, ws, ws_bytes, rs);
+  int rc = row_sort<T>(s, pos, w, np, g, ws, ws_bytes, rs, moff, nmock);
   if (rc) return rc;
   RowGeom rg;
   for (int a = 0; a < 3; ++a) rg.n[a] = n[a];
   reach_of(order, ni, rg.lo, rg.hi);
   rg.ni = ni;
   rg.zfrac = zfrac_of(n[2]);
+  rg.nmock = nmock;
   const int M = n[2] / 2;
   PzShape shape{0, 0};
   rc = pz_choose(s, order, M, n, ni, np, rs.sorted, rs.offsets, rs.row_wabs, rg, out, pitch, mesh_stride, shape);
@@ -1125,6 +1219,7 @@ static int paint_rows_typed(hipStream_t s, const T* pos, const float* w, int64_t
   reach_of(order, ni, rg.lo, rg.hi);
   rg.ni = ni;
   rg.zfrac = zfrac_of(g.n[2]);
+  rg.nmock = 1;
   const bool big = rows_big(g.n[2]);
 #define MP_ROWS(O)                                                                                   \
   case O:                                                                                            \
@@ -1146,11 +1241,12 @@ int paint_rows(hipStream_t s, const void* pos, int pos_f64, const float* w, int6
 
 // ni z-transformed half-spectrum rows (pitch `pitch`, mesh j at out + j * mesh_stride).
 int paint_zfft(hipStream_t s, const void* pos, int pos_f64, const float* w, int64_t np, const GridMap& g, int order, int ni,
-               float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes) {
+               float2* out, int64_t pitch, int64_t mesh_stride, void* ws, size_t ws_bytes, const int64_t* moff, int nmock) {
   int rc = ensure_twiddles();
   if (rc) return rc;
-  if (pos_f64) return paint_zfft_typed<double>(s, (const double*)pos, w, np, g, order, ni, out, pitch, mesh_stride, ws, ws_bytes);
-  return paint_zfft_typed<float>(s, (const float*)pos, w, np, g, order, ni, out, pitch, mesh_stride, ws, ws_bytes);
+  if (pos_f64)
+    return paint_zfft_typed<double>(s, (const double*)pos, w, np, g, order, ni, out, pitch, mesh_stride, ws, ws_bytes, moff, nmock);
+  return paint_zfft_typed<float>(s, (const float*)pos, w, np, g, order, ni, out, pitch, mesh_stride, ws, ws_bytes, moff, nmock);
 }
 
 }  // namespace mp

@@ -136,8 +136,11 @@ dot_partial_kernel(const float* __restrict__ a, const float* __restrict__ b, int
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+// out[r] = sum of the nblocks partial sums of row r = blockIdx.x, in a fixed order.
 __global__ void __launch_bounds__(RED_NT) final_sum_kernel(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
   __shared__ double sh[RED_NT];
+  partial += (int64_t)blockIdx.x * nblocks;
+  out += blockIdx.x;
   double s = 0.;
   for (int i = threadIdx.x; i < nblocks; i += RED_NT) s += partial[i];
   sh[threadIdx.x] = s;
@@ -147,6 +150,42 @@ __global__ void __launch_bounds__(RED_NT) final_sum_kernel(const double* __restr
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = sh[0];
+}
+
+// Sum w and sum w^2 (FP64) of each catalogue of a batch, catalogue m = blockIdx.y holding
+// w[moff[m] .. moff[m + 1]) (w null: unit weights); partial[(2 m + q) CAT_BLOCKS + b] for
+// block b, q = 0 (w) and 1 (w^2), reduced by final_sum_kernel.
+constexpr int CAT_BLOCKS = 64;
+
+__global__ void __launch_bounds__(RED_NT)
+catalogue_sums_kernel(const float* __restrict__ w, const int64_t* __restrict__ moff, double* __restrict__ partial) {
+  __shared__ double sh[2][RED_NT];
+  const int m = blockIdx.y;
+  const int64_t i0 = moff[m], i1 = moff[m + 1];
+  double s1 = 0., s2 = 0.;
+  if (w) {
+    for (int64_t i = i0 + (int64_t)blockIdx.x * RED_NT + threadIdx.x; i < i1; i += (int64_t)gridDim.x * RED_NT) {
+      const double x = (double)w[i];
+      s1 += x;
+      s2 += x * x;
+    }
+  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s1 = s2 = (double)(i1 - i0);
+  }
+  sh[0][threadIdx.x] = s1;
+  sh[1][threadIdx.x] = s2;
+  __syncthreads();
+  for (int d = RED_NT / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + d];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + d];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[(2 * (int64_t)m) * CAT_BLOCKS + blockIdx.x] = sh[0][0];
+    partial[(2 * (int64_t)m + 1) * CAT_BLOCKS + blockIdx.x] = sh[1][0];
+  }
 }
 
 }  // namespace mp
@@ -348,3 +387,83 @@ MP_EXPORT int mp_set_paint_tuning(int enable) { return paint_tuning_option(enabl
 MP_EXPORT int mp_paint_tuning_report(char* buf, int64_t buflen) { return paint_tuning_report(buf, buflen); }
 
 MP_EXPORT int mp_fill_f32(void* stream, float* p, int64_t n, float value) { return fill_f32((hipStream_t)stream, p, n, value); }
+
+// ------------------------------------------------------------------ batches of catalogues
+//
+// A batch of mocks on one mesh geometry: nmock catalogues stacked in one particle array,
+// catalogue m holding particles moff[m] .. moff[m + 1] (moff on the device, int64, nmock + 1
+// entries, moff[0] = 0).  Each phase of the per-mock spectrum (row sort, fused paint + z pass,
+// y pass, x pass + binning, weight sums) runs as one launch over every catalogue.
+
+MP_EXPORT int64_t mp_paint_partial_batch_workspace_bytes(const int32_t meshsize[3], int64_t np, int nmock, int order,
+                                                         int interlacing) {
+  if (!meshsize || nmock < 1) return 0;
+  return (int64_t)paint_zfft_workspace_bytes(meshsize, np, nmock) + 512;
+}
+
+MP_EXPORT int mp_paint_partial_batch(void* stream, const void* positions, int positions_f64, const float* weights, int64_t np,
+                                     const int64_t* moff, int nmock, const int32_t meshsize[3], const double boxsize[3],
+                                     const double boxcenter[3], int order, int interlacing, void* partial, int64_t pitch,
+                                     void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && boxsize && boxcenter && partial && moff, "mp_paint_partial_batch: null pointer");
+  MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
+  MP_REQUIRE(nmock >= 1, "mp_paint_partial_batch: %d catalogues", nmock);
+  MP_REQUIRE(np >= 0 && (np == 0 || positions), "mp_paint_partial_batch: %lld particles", (long long)np);
+  MP_REQUIRE(pitch >= meshsize[2] / 2 + 1, "mp_paint_partial_batch: pitch too small");
+  const int ni = interlacing < 1 ? 1 : interlacing;
+  if (!paint_zfft_supported(meshsize, np, order, ni))
+    return set_error(MP_ERR_UNSUPPORTED, "mp_paint_partial_batch: no fused paint for meshsize (%d, %d, %d), order %d, %lld particles",
+                     meshsize[0], meshsize[1], meshsize[2], order, (long long)np);
+  GridMap g = make_map(meshsize, boxsize, boxcenter);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nspec = (int64_t)meshsize[0] * meshsize[1] * pitch;
+  int rc = paint_zfft(s, positions, positions_f64, weights, np, g, order, ni, (float2*)partial, pitch, nspec, workspace,
+                      workspace_bytes, moff, nmock);
+  if (rc) return rc;
+  return ypass_meshes(s, (float2*)partial, nmock * ni, nspec, meshsize, pitch);
+}
+
+MP_EXPORT int64_t mp_bin_partial_batch_workspace_bytes(const int32_t meshsize[3], int interlacing, int nprod, int nedges,
+                                                       int nmock) {
+  if (!meshsize || nmock < 1) return 0;
+  return (int64_t)bin_partial_workspace_bytes(meshsize, interlacing < 1 ? 1 : interlacing, nprod, nedges, nmock);
+}
+
+MP_EXPORT int mp_bin_poles_partial_batch(void* stream, const void* partial, int64_t pitch, int nmock, const int32_t meshsize[3],
+                                         int order, int interlacing, int compensate, int zero_dc, const double* kx,
+                                         const double* ky, const double* kz, const double* k2_thresholds, int nedges,
+                                         const double los[3], int nprod, const int32_t* prod_ell, const int32_t* prod_remove_zero,
+                                         double* out, void* workspace, size_t workspace_bytes) {
+  MP_REQUIRE(meshsize && kx && ky && kz && k2_thresholds, "mp_bin_poles_partial_batch: null pointer");
+  MP_REQUIRE(order >= 1 && order <= 4, "resampler order %d not in 1..4", order);
+  MP_REQUIRE(nedges >= 2, "mp_bin_poles_partial_batch: need at least 2 edges, got %d", nedges);
+  BinGeom g{};
+  g.n0 = meshsize[0];
+  g.n1 = meshsize[1];
+  g.ncols = meshsize[2] / 2 + 1;
+  g.nedges = nedges;
+  g.kx = kx;
+  g.ky = ky;
+  g.kz = kz;
+  g.thr = k2_thresholds;
+  return bin_partial((hipStream_t)stream, (const float2*)partial, pitch, meshsize, order, interlacing < 1 ? 1 : interlacing,
+                     compensate, zero_dc, g, los, nprod, prod_ell, prod_remove_zero, out, workspace, workspace_bytes, 0, nullptr,
+                     false, nmock);
+}
+
+MP_EXPORT int64_t mp_catalogue_sums_workspace_bytes(int nmock) {
+  return nmock < 1 ? 0 : (int64_t)align_up((size_t)nmock * 2 * CAT_BLOCKS * sizeof(double), 256) + 256;
+}
+
+MP_EXPORT int mp_catalogue_sums(void* stream, const float* weights, const int64_t* moff, int nmock, double* out, void* workspace,
+                                size_t workspace_bytes) {
+  MP_REQUIRE(moff && out, "mp_catalogue_sums: null pointer");
+  MP_REQUIRE(nmock >= 1 && nmock <= 65535, "mp_catalogue_sums: %d catalogues not in 1..65535", nmock);
+  hipStream_t s = (hipStream_t)stream;
+  Workspace wsp(workspace, workspace_bytes);
+  double* partial = wsp.take<double>((size_t)nmock * 2 * CAT_BLOCKS);
+  if (!partial) return set_error(MP_ERR_WORKSPACE, "mp_catalogue_sums: workspace too small");
+  MP_LAUNCH("catalogue_sums", catalogue_sums_kernel, dim3(CAT_BLOCKS, (unsigned)nmock), dim3(RED_NT), 0, s, weights, moff, partial);
+  MP_LAUNCH("sum_final", final_sum_kernel, dim3((unsigned)(2 * nmock)), dim3(RED_NT), 0, s, (const double*)partial, CAT_BLOCKS, out);
+  return MP_OK;
+}

@@ -52,7 +52,8 @@ struct XPassArgs {
   LegendreTable leg;
   float los[3];
   int need_mu;
-  double* partial;  // [gridDim.x][nprod][nbins][2]
+  double* partial;      // [gridDim.y][gridDim.x][nprod][nbins][2]
+  int64_t mock_stride;  // bin mode: between the partial spectra of the catalogues (blockIdx.y) of a batch
 };
 static_assert(XB_MAX_PROD <= LEG_MAX, "Legendre table too short");
 
@@ -152,9 +153,10 @@ xpass_kernel(XPassArgs a) {
     for (int i = tid; i < NP * nbins; i += NT) hist[i] = 0.;
   }
   float2 reg[EPT];
+  const float2* in = a.in + blockIdx.y * a.mock_stride;  // this block's catalogue
   auto load = [&](int tile, int j) {
     const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
-    const float2* src = a.in + j * a.mesh_stride + (int64_t)y * a.ostride + c0;
+    const float2* src = in + j * a.mesh_stride + (int64_t)y * a.ostride + c0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int idx = tid + e * NT, cc = idx % C, l = idx / C;
@@ -304,7 +306,7 @@ xpass_kernel(XPassArgs a) {
   }
   if constexpr (BIN) {
     __syncthreads();
-    double* dst = a.partial + (int64_t)blockIdx.x * NP * nbins * 2;
+    double* dst = a.partial + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NP * nbins * 2;
     for (int i = tid; i < NP * nbins; i += NT) {
       dst[2 * i] = hist[i];
       dst[2 * i + 1] = 0.;
@@ -549,7 +551,7 @@ int combine_tables(hipStream_t s, const int32_t n[3], int order, int ni, int com
 }
 
 template <int L, bool BIN, int NP, bool TAB, bool YLM = false>
-static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles) {
+static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles, int nmock) {
   using Cfg = XPCfg<L>;
   auto kern = xpass_kernel<L, BIN, NP, TAB, YLM>;
   a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
@@ -561,25 +563,26 @@ static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double*
   MP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, Cfg::NT, lds));
   if (per < 1) per = 1;
   int blocks = multiprocessors() * per;
+  if (nmock > 1) blocks = (int)ceil_div(ceil_div(blocks, nmock), 8) * 8;  // per catalogue, a multiple of the 8 XCDs
   if (blocks > a.ntiles) blocks = a.ntiles;
   if (BIN) {
     const size_t per_block = (size_t)NP * (a.g.nedges - 1) * 2;
-    const int cap = (int)(partial_doubles / per_block);
+    const int cap = (int)(partial_doubles / per_block / nmock);
     if (cap < 1) return set_error(MP_ERR_WORKSPACE, "x pass: no room for the partial histograms");
     if (blocks > cap) blocks = cap;
     a.partial = partial_ws;
   }
   *nblocks_out = blocks;
-  MP_LAUNCH(BIN ? "fft_xpass_bin" : "fft_xpass_combine", kern, dim3((unsigned)blocks), dim3(Cfg::NT), lds, s, a);
+  MP_LAUNCH(BIN ? "fft_xpass_bin" : "fft_xpass_combine", kern, dim3((unsigned)blocks, (unsigned)nmock), dim3(Cfg::NT), lds, s, a);
   return MP_OK;
 }
 
 // The kx/kz tables go to LDS when they fit beside the tile.
 template <int L, bool BIN, int NP, bool YLM = false>
-static int launch_xpass(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles) {
+static int launch_xpass(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles, int nmock = 1) {
   const size_t with_tab = XPCfg<L>::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, true);
-  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true, YLM>(s, a, nblocks_out, partial_ws, partial_doubles);
-  return launch_xpass_t<L, BIN, NP, false, YLM>(s, a, nblocks_out, partial_ws, partial_doubles);
+  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true, YLM>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
+  return launch_xpass_t<L, BIN, NP, false, YLM>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
 }
 
 template <int L>
@@ -589,18 +592,18 @@ static int xpass_write_L(hipStream_t s, XPassArgs& a) {
 }
 
 template <int L>
-static int xpass_bin_L(hipStream_t s, XPassArgs& a, int nprod, double* partial, size_t partial_doubles, int* nblocks) {
+static int xpass_bin_L(hipStream_t s, XPassArgs& a, int nprod, double* partial, size_t partial_doubles, int* nblocks, int nmock) {
   if (a.ylm) {
     if (nprod != 1) return set_error(MP_ERR_INVALID, "x pass: Y_lm binning takes one product, got %d", nprod);
-    return launch_xpass<L, true, 1, true>(s, a, nblocks, partial, partial_doubles);
+    return launch_xpass<L, true, 1, true>(s, a, nblocks, partial, partial_doubles, nmock);
   }
   switch (nprod) {
-    case 1: return launch_xpass<L, true, 1>(s, a, nblocks, partial, partial_doubles);
-    case 2: return launch_xpass<L, true, 2>(s, a, nblocks, partial, partial_doubles);
-    case 3: return launch_xpass<L, true, 3>(s, a, nblocks, partial, partial_doubles);
-    case 4: return launch_xpass<L, true, 4>(s, a, nblocks, partial, partial_doubles);
-    case 5: return launch_xpass<L, true, 5>(s, a, nblocks, partial, partial_doubles);
-    case 6: return launch_xpass<L, true, 6>(s, a, nblocks, partial, partial_doubles);
+    case 1: return launch_xpass<L, true, 1>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 2: return launch_xpass<L, true, 2>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 3: return launch_xpass<L, true, 3>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 4: return launch_xpass<L, true, 4>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 5: return launch_xpass<L, true, 5>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 6: return launch_xpass<L, true, 6>(s, a, nblocks, partial, partial_doubles, nmock);
     default: return set_error(MP_ERR_INVALID, "x pass: %d products not in 1..%d", nprod, XB_MAX_PROD);
   }
 }
@@ -742,21 +745,23 @@ int combine_partial(hipStream_t s, const float2* partial, int64_t pitch, const i
 
 constexpr int XB_MAX_BLOCKS = 1024;  // partial histograms: the bin-mode grid is capped here
 
-size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges) {
+size_t bin_partial_workspace_bytes(const int32_t n[3], int ni, int nprod, int nedges, int nmock) {
   if (!xpass_supported(n))  // combined mesh and one scratch mesh, then the plain binning
     return xpass_tables_bytes(n, ni) + 2 * mesh_bytes(n) + bin_workspace_bytes(nprod, nedges) + 1024;
-  const size_t blocks = XB_MAX_BLOCKS;
+  // a batch: per catalogue at most ceil(XB_MAX_BLOCKS / nmock) blocks rounded up to 8
+  const size_t blocks = nmock > 1 ? (size_t)nmock * (ceil_div(ceil_div(XB_MAX_BLOCKS, nmock), 8) * 8) : XB_MAX_BLOCKS;
   return xpass_tables_bytes(n, ni) + align_up(blocks * nprod * (nedges - 1) * 2 * sizeof(double), 256) + 512;
 }
 
 int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32_t n[3], int order, int ni, int compensate,
                 int zero_dc, const BinGeom& g, const double los[3], int nprod, const int32_t* ell, const int32_t* remove_zero,
-                double* out, void* ws, size_t ws_bytes, int linear, const int32_t* ylm_m, bool fit_partials) {
+                double* out, void* ws, size_t ws_bytes, int linear, const int32_t* ylm_m, bool fit_partials, int nmock) {
   MP_REQUIRE(partial && out && ell && remove_zero, "bin_partial: null pointer");
   MP_REQUIRE(ni >= 1, "bin_partial: interlacing %d < 1", ni);
+  MP_REQUIRE(nmock >= 1 && nmock <= 65535, "bin_partial: %d catalogues not in 1..65535", nmock);
   MP_REQUIRE(nprod >= 1 && nprod <= XB_MAX_PROD, "bin_partial: %d products not in 1..%d", nprod, XB_MAX_PROD);
   const bool any = !xpass_supported(n);
-  if (any && (linear || ylm_m))
+  if (any && (linear || ylm_m || nmock > 1))
     return set_error(MP_ERR_UNSUPPORTED, "bin_partial: meshsize (%d, %d, %d) not supported", n[0], n[1], n[2]);
   if (nprod * (g.nedges - 1) > BIN_LDS_DOUBLES) return set_error(MP_ERR_UNSUPPORTED, "bin_partial: too many bins for the LDS histogram");
   MP_REQUIRE(pitch >= n[2] / 2 + 1, "bin_partial: pitch < Nz/2+1");
@@ -795,7 +800,7 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
     return bin_poles(s, g, prods, out, bw, bws);
   }
   const size_t per_block = (size_t)nprod * (g.nedges - 1) * 2;
-  size_t pd = (size_t)XB_MAX_BLOCKS * per_block;
+  size_t pd = (nmock > 1 ? (size_t)nmock * (ceil_div(ceil_div(XB_MAX_BLOCKS, nmock), 8) * 8) : (size_t)XB_MAX_BLOCKS) * per_block;
   if (fit_partials) {  // a fixed workspace: as many per-block histograms as it holds (fewer blocks)
     const size_t off = align_up(wsp.used, 256);
     const size_t room = ws_bytes > off ? (ws_bytes - off) / sizeof(double) : 0;
@@ -809,6 +814,7 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   XPassArgs a{};
   fill_common(a, partial, pitch, n, ni, tab, zero_dc);
   if (!a.tw) return set_error(MP_ERR_HIP, "bin_partial: twiddle table address");
+  a.mock_stride = ni * a.mesh_stride;
   a.g = g;
   a.need_mu = 0;
   a.linear = linear;
@@ -828,21 +834,21 @@ int bin_partial(hipStream_t s, const float2* partial, int64_t pitch, const int32
   for (int q = 0; q < 3; ++q) a.los[q] = los ? (float)los[q] : 0.f;
   int nblocks = 0;
   switch (n[0]) {
-    case 2: rc = xpass_bin_L<2>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 4: rc = xpass_bin_L<4>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 8: rc = xpass_bin_L<8>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 16: rc = xpass_bin_L<16>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 32: rc = xpass_bin_L<32>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 64: rc = xpass_bin_L<64>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 128: rc = xpass_bin_L<128>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 256: rc = xpass_bin_L<256>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 512: rc = xpass_bin_L<512>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 1024: rc = xpass_bin_L<1024>(s, a, nprod, pbuf, pd, &nblocks); break;
-    case 2048: rc = xpass_bin_L<2048>(s, a, nprod, pbuf, pd, &nblocks); break;
+    case 2: rc = xpass_bin_L<2>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 4: rc = xpass_bin_L<4>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 8: rc = xpass_bin_L<8>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 16: rc = xpass_bin_L<16>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 32: rc = xpass_bin_L<32>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 64: rc = xpass_bin_L<64>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 128: rc = xpass_bin_L<128>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 256: rc = xpass_bin_L<256>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 512: rc = xpass_bin_L<512>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 1024: rc = xpass_bin_L<1024>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
+    case 2048: rc = xpass_bin_L<2048>(s, a, nprod, pbuf, pd, &nblocks, nmock); break;
     default: return set_error(MP_ERR_UNSUPPORTED, "bin_partial: x length %d", n[0]);
   }
   if (rc) return rc;
-  return reduce_partials(s, pbuf, nblocks, nprod * (g.nedges - 1) * 2, out);
+  return reduce_partials(s, pbuf, nblocks, nprod * (g.nedges - 1) * 2, out, nmock);
 }
 
 size_t ylm_bin_workspace_bytes(const int32_t n[3], int ell, int nedges) {

@@ -45,6 +45,15 @@ SIGNATURES = {
     'mp_bin_partial_workspace_bytes': (c_int64, [P_I32, c_int, c_int, c_int]),
     'mp_bin_poles_partial': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_int, P_F64, c_int, P_I32, P_I32, c_void_p, c_void_p, c_size_t]),
+    'mp_paint_partial_batch_workspace_bytes': (c_int64, [P_I32, c_int64, c_int, c_int, c_int]),
+    'mp_paint_partial_batch': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int, P_I32, P_F64, P_F64, c_int,
+                                       c_int, c_void_p, c_int64, c_void_p, c_size_t]),
+    'mp_bin_partial_batch_workspace_bytes': (c_int64, [P_I32, c_int, c_int, c_int, c_int]),
+    'mp_bin_poles_partial_batch': (c_int, [c_void_p, c_void_p, c_int64, c_int, P_I32, c_int, c_int, c_int, c_int, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_int, P_F64, c_int, P_I32, P_I32, c_void_p, c_void_p,
+                                           c_size_t]),
+    'mp_catalogue_sums_workspace_bytes': (c_int64, [c_int]),
+    'mp_catalogue_sums': (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t]),
     'mp_noalloc_scratch_bytes': (c_int64, []),
     'mp_bin_poles_partial_noalloc': (c_int, [c_void_p, c_void_p, c_int64, P_I32, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                              c_void_p, c_void_p, c_int, P_F64, c_int, P_I32, P_I32, c_void_p]),
