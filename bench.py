@@ -834,7 +834,8 @@ def main():
     # timed replays; the per-kernel table above comes from the eager timed steps (HIP events
     # cannot be recorded from the host inside a replayed graph).
     from jaxpower_amd import Mesh2SpectrumBatch
-    graph = Mesh2SpectrumBatch([particles], bin, resampler='tsc', interlacing=NI, compensate=True, los='z', graph=True)
+    graph = Mesh2SpectrumBatch([particles], bin, resampler='tsc', interlacing=NI, compensate=True, los='z', graph=True,
+                               merged=False)
     for _ in range(max(args.warmup, 1)):
         gspec = graph()[0]
     torch.cuda.synchronize()

@@ -153,9 +153,14 @@ __global__ void __launch_bounds__(RED_NT) final_sum_kernel(const double* __restr
 }
 
 // Sum w and sum w^2 (FP64) of each catalogue of a batch, catalogue m = blockIdx.y holding
-// w[moff[m] .. moff[m + 1]) (w null: unit weights); partial[(2 m + q) CAT_BLOCKS + b] for
-// block b, q = 0 (w) and 1 (w^2), reduced by final_sum_kernel.
-constexpr int CAT_BLOCKS = 64;
+// w[moff[m] .. moff[m + 1]) (w null: unit weights); partial[(2 m + q) gridDim.x + b] for
+// block b, q = 0 (w) and 1 (w^2), reduced by final_sum_kernel.  About 4096 blocks in all
+// (at least 16 per catalogue): the loads are 4-byte and latency-bound, so the grid must fill
+// the GPU for one catalogue as for many.
+static int cat_blocks(int nmock) {
+  const int b = (4096 + nmock - 1) / nmock;
+  return b < 16 ? 16 : (b > 1024 ? 1024 : b);
+}
 
 __global__ void __launch_bounds__(RED_NT)
 catalogue_sums_kernel(const float* __restrict__ w, const int64_t* __restrict__ moff, double* __restrict__ partial) {
@@ -183,8 +188,8 @@ catalogue_sums_kernel(const float* __restrict__ w, const int64_t* __restrict__ m
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partial[(2 * (int64_t)m) * CAT_BLOCKS + blockIdx.x] = sh[0][0];
-    partial[(2 * (int64_t)m + 1) * CAT_BLOCKS + blockIdx.x] = sh[1][0];
+    partial[(2 * (int64_t)m) * gridDim.x + blockIdx.x] = sh[0][0];
+    partial[(2 * (int64_t)m + 1) * gridDim.x + blockIdx.x] = sh[1][0];
   }
 }
 
@@ -452,7 +457,7 @@ MP_EXPORT int mp_bin_poles_partial_batch(void* stream, const void* partial, int6
 }
 
 MP_EXPORT int64_t mp_catalogue_sums_workspace_bytes(int nmock) {
-  return nmock < 1 ? 0 : (int64_t)align_up((size_t)nmock * 2 * CAT_BLOCKS * sizeof(double), 256) + 256;
+  return nmock < 1 ? 0 : (int64_t)align_up((size_t)nmock * 2 * cat_blocks(nmock) * sizeof(double), 256) + 256;
 }
 
 MP_EXPORT int mp_catalogue_sums(void* stream, const float* weights, const int64_t* moff, int nmock, double* out, void* workspace,
@@ -461,9 +466,10 @@ MP_EXPORT int mp_catalogue_sums(void* stream, const float* weights, const int64_
   MP_REQUIRE(nmock >= 1 && nmock <= 65535, "mp_catalogue_sums: %d catalogues not in 1..65535", nmock);
   hipStream_t s = (hipStream_t)stream;
   Workspace wsp(workspace, workspace_bytes);
-  double* partial = wsp.take<double>((size_t)nmock * 2 * CAT_BLOCKS);
+  const int nb = cat_blocks(nmock);
+  double* partial = wsp.take<double>((size_t)nmock * 2 * nb);
   if (!partial) return set_error(MP_ERR_WORKSPACE, "mp_catalogue_sums: workspace too small");
-  MP_LAUNCH("catalogue_sums", catalogue_sums_kernel, dim3(CAT_BLOCKS, (unsigned)nmock), dim3(RED_NT), 0, s, weights, moff, partial);
-  MP_LAUNCH("sum_final", final_sum_kernel, dim3((unsigned)(2 * nmock)), dim3(RED_NT), 0, s, (const double*)partial, CAT_BLOCKS, out);
+  MP_LAUNCH("catalogue_sums", catalogue_sums_kernel, dim3((unsigned)nb, (unsigned)nmock), dim3(RED_NT), 0, s, weights, moff, partial);
+  MP_LAUNCH("sum_final", final_sum_kernel, dim3((unsigned)(2 * nmock)), dim3(RED_NT), 0, s, (const double*)partial, nb, out);
   return MP_OK;
 }
