@@ -30,6 +30,9 @@ static int zfrac_of(int nz) {
   return 32 - b;
 }
 constexpr int PZ_NT = 1024;
+#ifndef PZ_YROT
+#define PZ_YROT 1  // lanes rotate their y stencil slots as well as z (see LaneRotation)
+#endif
 // Segment table (home rows of a block's neighbourhood): the reach of a stencil is at most
 // p + 1 <= 5 rows per axis, so (RX + 5) (RY + 5) rounded to a wave.
 __host__ __device__ constexpr int pz_seg_capacity(int rx, int ry) { return ((rx + 5) * (ry + 5) + 63) / 64 * 64; }
@@ -429,6 +432,11 @@ __device__ __forceinline__ int stencil_1d(float u, int base, float* k) {
   }
 }
 
+// LDS-space pointers: held in one 32-bit register, so an add to them stays one instruction and
+// a constant offset folds into the LDS instruction.
+typedef __attribute__((address_space(3))) int lds_i32;
+typedef __attribute__((address_space(3))) unsigned char lds_u8;
+
 // Fixed-point addend: round to nearest (halves up), in one instruction.
 __device__ __forceinline__ int to_fixed(float v) {
   int r;
@@ -473,16 +481,57 @@ struct LaneRotation {
   }
 };
 
+// First Stockham stage (radix first_radix(M), no twiddles) of the z transform of NROW rows:
+// reads the painted fixed-point cells (row r, cell z at ints[r * RS + z]; complex element e is
+// the pair z = 2e, 2e + 1), scales them to float and writes the stage output in the column
+// layout (element l of row r at buf[l * RP + r]).  Every read precedes the barrier and every
+// write follows it, as the two layouts share the LDS; the caller syncs before the call.
+template <int M, int NROW, int RP, int RS, int NT>
+__device__ __forceinline__ void pz_first_stage(const int* ints, float2* buf, float scale) {
+  constexpr int R = first_radix(M);
+  constexpr int NB = NROW * (M / R);
+  constexpr int BPT = (NB + NT - 1) / NT;
+  float2 v[BPT][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      const int c = b % NROW, jj = b / NROW;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int2 p = *reinterpret_cast<const int2*>(ints + c * RS + 2 * (jj + r * (M / R)));
+        v[q][r] = make_float2((float)p.x * scale, (float)p.y * scale);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      const int c = b % NROW, jj = b / NROW;
+      dft<R, false>(v[q]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(jj * R + r) * RP + c] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
 // One workgroup: RX x RY rows of shifted mesh j, painted in LDS, then z-FFT'd (r2c) in LDS.
 template <int ORDER, int M, int RX, int RY>
 __global__ void __launch_bounds__(PZ_NT)
 paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ offsets, const float* __restrict__ row_wabs, RowGeom rg,
                   float2* __restrict__ out, int64_t pitch, int64_t mesh_stride, const float2* __restrict__ tw) {
   constexpr int NZ = 2 * M;
-  // LDS layout: complex element l (reals z = 2l, 2l+1) of row r at lds2[l * RP + r], rows
-  // fastest with one pad column, so the column FFT is bank-conflict free and a stencil add at
-  // (r, z) lands in bank (z + 2r) mod 64
+  // Two LDS layouts share the dynamic LDS.  While painting, row r holds its NZ int32 cells at
+  // words r RS + z (RS = NZ + 2: a stencil cell is one add from its row, and the first FFT stage
+  // reads the 8-byte pairs of consecutive rows from different banks).  The z transform then works
+  // on complex element l (reals z = 2l, 2l+1) of row r at lds2[l * RP + r], rows fastest with
+  // one pad column, so the column FFT is bank-conflict free.
   constexpr int RP = RX * RY + 1;
+  constexpr int RS = NZ + 2;
   constexpr int NROW = RX * RY;
   constexpr int OMIN = -((ORDER - 1) / 2);
   constexpr int CHUNK = 4;  // particles per work item, loaded together before any is painted
@@ -517,7 +566,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   const int64_t key0 = (int64_t)mk * rg.n[0] * rg.n[1];
   const float s = (float)j / (float)rg.ni;
 
-  for (int i = tid; i < M * RP; i += PZ_NT) lds2[i] = make_float2(0.f, 0.f);
+  for (int i = tid; i < NROW * RS / 2; i += PZ_NT) lds2[i] = make_float2(0.f, 0.f);
 
   // neighbourhood rows: homes in [x0 - hi, x0 + RX - 1 - lo]; each row's particles are
   // contiguous.  Work items are chunks of up to CHUNK consecutive particles of one row.
@@ -591,7 +640,8 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   // |sum| <= 0.99 * 2^31 (each K <= 1), and each addend is rounded to 1/scale.
   const float bound = sbound[0];
   const float scale = bound > 0.f ? 2126008811.f / bound : 1.f;
-  int* irows = reinterpret_cast<int*>(rows);
+  const int* irows = reinterpret_cast<const int*>(rows);
+  lds_u8* const lrows = (lds_u8*)(rows);  // the rows as an LDS-space byte pointer
   // Work items are taken from G regions of the chunk list in turn (lane l of a wave works in
   // region l mod G), so the lanes of one wave hold chunks of different home rows: the particles
   // of a dense clump, which share stencil cells, then go to different instructions instead of
@@ -612,71 +662,91 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     }
     const int seg = lo_i;
     const int p0 = sstart[seg] + (c - spre[seg]) * CHUNK;
-    const int cnt = min(CHUNK, sstart[seg] + scount[seg] - p0);
+    const int plast = sstart[seg] + scount[seg] - 1;  // >= p0: a chunk holds at least one record
+    const int cnt = min(CHUNK, plast + 1 - p0);
     const int home = shome[seg];
     const int hxr = home >> 16, hyr = (int)(short)(home & 0xffff);  // home row relative to the block
     float4 recs[CHUNK];
 #pragma unroll
-    for (int q = 0; q < CHUNK; ++q) recs[q] = q < cnt ? sorted[p0 + q] : make_float4(0.f, 0.f, 0.f, 0.f);
-    {
+    for (int q = 0; q < CHUNK; ++q) recs[q] = sorted[min(p0 + q, plast)];  // no branch per load
 #pragma unroll
-      for (int q = 0; q < CHUNK; ++q) {
-        if (q >= cnt) break;
-        const float4 rec = recs[q];
-        int zc;
-        float fz;
-        z_of_record(__float_as_uint(rec.z), rg.zfrac, zc, fz);
-        float kx[ORDER], ky[ORDER], kz[ORDER];
-        const int cx0 = stencil_1d<ORDER>(rec.x + s, hxr, kx);
-        const int cy0 = stencil_1d<ORDER>(rec.y + s, hyr, ky);
-        const int cz0 = stencil_1d<ORDER>(fz + s, zc, kz);
-        rotate_stencil<ORDER>(ky, rot.yrot);
-        rotate_stencil<ORDER>(kz, rot.zrot);
-        // cell (cx0 + ox, cy0 + yo, z) is word 2 ((cx0 + ox) RY + cy0 + yo) + zoff(z): one base
-        // per z cell (wrapped into the row), the y part per lane and the x part a constant
-        // offset.  The dynamic LDS follows the static tables, so a base below the first row
-        // stays >= 0.
-        int* zbase[ORDER];
+    for (int q = 0; q < CHUNK; ++q) {
+      if (q >= cnt) break;
+      const float4 rec = recs[q];
+      int zc;
+      float fz;
+      z_of_record(__float_as_uint(rec.z), rg.zfrac, zc, fz);
+      float kx[ORDER], ky[ORDER], kz[ORDER];
+      const int cx0 = stencil_1d<ORDER>(rec.x + s, hxr, kx);
+      const int cy0 = stencil_1d<ORDER>(rec.y + s, hyr, ky);
+      const int cz0 = stencil_1d<ORDER>(fz + s, zc, kz);
+      if constexpr (PZ_YROT) rotate_stencil<ORDER>(ky, rot.yrot);
+      rotate_stencil<ORDER>(kz, rot.zrot);
+      // Cell (cx0 + ox, cy0 + yo, z) is word ((cx0 + ox) RY + cy0 + yo) RS + z: one base per
+      // (z slot, y slot), computed before the x loop, and the x part an immediate offset of the
+      // LDS atomic.  The dynamic LDS follows the static tables, so a base below the first row
+      // stays >= 0.
+      const int rowb = 4 * RS * (cx0 * RY + cy0);
+      int zb[ORDER];  // byte offsets into the rows
+#pragma unroll
+      for (int oz = 0; oz < ORDER; ++oz) zb[oz] = rowb + 4 * ((cz0 + rot.zo[oz]) & (NZ - 1));
+      bool inx[ORDER];
+#pragma unroll
+      for (int ox = 0; ox < ORDER; ++ox) inx[ox] = (unsigned)(cx0 + ox) < (unsigned)RX;
+      const float ws = rec.w * scale;
+      float wx[ORDER];
+#pragma unroll
+      for (int ox = 0; ox < ORDER; ++ox) wx[ox] = ws * kx[ox];
+#pragma unroll
+      for (int oy = 0; oy < ORDER; ++oy) {
+        const int yo = PZ_YROT ? rot.yo[oy] : oy;
+        if ((unsigned)(cy0 + yo) >= (unsigned)RY) continue;
+        lds_i32* base[ORDER];
 #pragma unroll
         for (int oz = 0; oz < ORDER; ++oz) {
-          const int z = (cz0 + rot.zo[oz]) & (NZ - 1);
-          int w = (z & ~1) * RP + (z & 1) + 2 * (cx0 * RY + cy0);
-          asm volatile("" : "+v"(w));  // computed once: the atomics below use it with constant offsets
-          zbase[oz] = irows + w;
+          base[oz] = reinterpret_cast<lds_i32*>(lrows + zb[oz] + (PZ_YROT ? 4 * RS * yo : 0));
+          asm volatile("" : "+v"(base[oz]));  // one add per (z, y) slot; the x slots use immediates
         }
-        const float ws = rec.w * scale;
 #pragma unroll
         for (int ox = 0; ox < ORDER; ++ox) {
-          if ((unsigned)(cx0 + ox) >= (unsigned)RX) continue;
-          const float wx = ws * kx[ox];
+          if (!inx[ox]) continue;
+          const float kxy = wx[ox] * ky[oy];
 #pragma unroll
-          for (int oy = 0; oy < ORDER; ++oy) {
-            if ((unsigned)(cy0 + rot.yo[oy]) >= (unsigned)RY) continue;
-            const float kxy = wx * ky[oy];
-#pragma unroll
-            for (int oz = 0; oz < ORDER; ++oz)
-              atomicAdd(zbase[oz] + 2 * rot.yo[oy] + 2 * ox * RY, to_fixed(kxy * kz[oz]));
-          }
+          for (int oz = 0; oz < ORDER; ++oz)
+            __hip_atomic_fetch_add(base[oz] + RS * RY * ox + (PZ_YROT ? 0 : RS * oy), to_fixed(kxy * kz[oz]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
     }
   }
   __syncthreads();
-  {
-    const float inv = 1.f / scale;
-    for (int i = tid; i < M * RP * 2; i += PZ_NT) rows[i] = (float)irows[i] * inv;
-  }
-  __syncthreads();
 
-  // z-axis r2c of the NROW rows held in LDS
-  fft_lds<M, NROW, RP, 1, true, PZ_NT, false, NZ>(lds2, stw);
-  constexpr int NK = M / 2 + 1;
+  // z-axis r2c of the NROW rows: the first Stockham stage converts the fixed-point cells (times
+  // 1/(2 scale): the post-processing below then needs no factor 1/2) and moves them into the
+  // column layout; the other stages run in place.
+  pz_first_stage<M, NROW, RP, RS, PZ_NT>(irows, lds2, 0.5f / scale);
+  fft_stages<M, NROW, RP, 1, true, PZ_NT, false, first_radix(M), NZ>(lds2, stw);
   float2* dst = out + ((int64_t)mk * rg.ni + j) * mesh_stride;
-  for (int idx = tid; idx < NROW * NK; idx += PZ_NT) {
-    int r = idx / NK, kk = idx % NK;
-    int rx = r / RY, ry = r % RY;
-    float2* o = dst + ((int64_t)(x0 + rx) * rg.n[1] + (y0 + ry)) * pitch;
-    rfft_post<NZ>(lds2 + r, o, M, kk, stw, RP);
+  // Outputs k and M - k of the N = 2M point real transform for k in [1, M/2] (k = M/2 writes
+  // one), from Z = FFT_M(x[2m] + i x[2m+1]) / 2; k = 0 (outputs 0 and M) apart.  Offsets within
+  // the mesh fit 32 bits (< 2^31 complex elements).
+  constexpr int HM = M / 2;
+  for (int idx = tid; idx < NROW * HM; idx += PZ_NT) {
+    const int r = idx / HM, k = idx % HM + 1;
+    float2* o = dst + (unsigned)((x0 + r / RY) * rg.n[1] + y0 + r % RY) * (unsigned)pitch;
+    const float2 zk = lds2[k * RP + r];
+    const float2 zmk = cconj(lds2[(M - k) * RP + r]);
+    const float2 E = cadd(zk, zmk), D = csub(zk, zmk);
+    const float2 t = cmul(stw[k], make_float2(D.y, -D.x));  // exp(-2 pi i k / N) (-i D)
+    o[k] = cadd(E, t);
+    if (k != HM) o[M - k] = cconj(csub(E, t));
+  }
+  if (tid < NROW) {
+    const int r = tid;
+    float2* o = dst + (unsigned)((x0 + r / RY) * rg.n[1] + y0 + r % RY) * (unsigned)pitch;
+    const float2 z0 = lds2[r];
+    o[0] = make_float2(2.f * (z0.x + z0.y), 0.f);
+    o[M] = make_float2(2.f * (z0.x - z0.y), 0.f);
   }
 }
 
@@ -911,7 +981,10 @@ size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np, int nmock) {
 template <int ORDER, int M, int RX, int RY>
 static int launch_pz(hipStream_t s, const float4* sorted, const int* offsets, const float* row_wabs, const RowGeom& rg, float2* out,
                      int64_t pitch, int64_t mesh_stride) {
-  constexpr size_t lds = (size_t)M * (RX * RY + 1) * sizeof(float2);
+  // the larger of the two layouts: complex columns M (RX RY + 1), painted rows RX RY (2M + 2) int32
+  constexpr size_t lds_fft = (size_t)M * (RX * RY + 1) * sizeof(float2);
+  constexpr size_t lds_rows = (size_t)RX * RY * (2 * M + 2) * sizeof(int);
+  constexpr size_t lds = lds_fft > lds_rows ? lds_fft : lds_rows;
   static_assert(lds <= PZ_LDS_BUDGET, "LDS budget");
   auto kern = paint_zfft_kernel<ORDER, M, RX, RY>;
   if (lds > 65536) MP_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
