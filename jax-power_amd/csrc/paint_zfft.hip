@@ -546,6 +546,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __shared__ float sbound[1];
   __shared__ float tbound[NROW];
   __shared__ float2 stw[NZ];
+  __shared__ int sregion[17];  // first segment of each chunk region, and the last segment
   const int tid = threadIdx.x;
   load_twiddles<NZ, PZ_NT>(stw, tw);
 
@@ -626,15 +627,32 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     tbound[tid] = sum;
   }
   __syncthreads();
+  // Work items are taken from G regions of the chunk list in turn (see the loop below); the
+  // first segment of each region bounds the segment search of its chunks.
+  constexpr int G = 16;
+  const int total = spre[nseg];  // number of chunks
+  const int Rg = (total + G - 1) / G;
   if (tid < 64) {
     float m = 0.f;
     for (int i = tid; i < NROW; i += 64) m = fmaxf(m, tbound[i]);
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
     if (tid == 0) sbound[0] = m;
+  } else if (tid < 64 + G + 1) {
+    const int g = tid - 64, c = g * Rg;
+    int lo_i = 0, hi_i = nseg - 1;
+    if (g == G || c >= total) {
+      lo_i = hi_i;
+    } else {
+      while (lo_i < hi_i) {
+        const int mid = (lo_i + hi_i + 1) >> 1;
+        if (spre[mid] <= c) lo_i = mid;
+        else hi_i = mid - 1;
+      }
+    }
+    sregion[g] = lo_i;
   }
   __syncthreads();
-  const int total = spre[nseg];  // number of chunks
   // Fixed-point accumulation: integer LDS atomics are exact, order independent (the painted
   // rows are bitwise reproducible) and much faster than float LDS atomics on this target.
   // |sum| <= 0.99 * 2^31 (each K <= 1), and each addend is rounded to 1/scale.
@@ -648,13 +666,11 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   // serialising one LDS atomic over many lanes at the same address (measured on the 512^3
   // lognormal mock: 1.41 -> 1.19 ms; a coprime stride over single chunks, which also breaks the
   // coalescing of the record loads, gave 1.22 ms).
-  constexpr int G = 16;
-  const int Rg = (total + G - 1) / G;
   const LaneRotation<ORDER> rot(tid);
   for (int ci = tid; ci < G * Rg; ci += PZ_NT) {
-    const int c = (ci % G) * Rg + ci / G;
+    const int g = ci % G, c = g * Rg + ci / G;
     if (c >= total) continue;
-    int lo_i = 0, hi_i = nseg - 1;
+    int lo_i = sregion[g], hi_i = sregion[g + 1];  // the segment of chunk c lies in this range
     while (lo_i < hi_i) {
       int mid = (lo_i + hi_i + 1) >> 1;
       if (spre[mid] <= c) lo_i = mid;
