@@ -148,9 +148,24 @@ xpass_kernel(XPassArgs a) {
   }
   double* thr = reinterpret_cast<double*>(take(BIN ? (size_t)nedges * 8 : 0));
   double* hist = reinterpret_cast<double*>(take(BIN ? (size_t)NP * nbins * 8 : 0));
+  __shared__ int swsteps;
   if constexpr (BIN) {
     for (int i = tid; i < nedges; i += NT) thr[i] = a.g.thr[i];
     for (int i = tid; i < NP * nbins; i += NT) hist[i] = 0.;
+    // Thresholds one kx step can cross: |k| changes by at most the kx spacing between
+    // neighbours of a run, so ceil-like (spacing / narrowest bin) + 1, with a 1 % margin; 0 (the
+    // general walk) for narrower bins or empty ones.
+    if (tid < 64) {
+      double wmin = 1e300;
+      for (int i = tid; i + 1 < nedges; i += 64) wmin = fmin(wmin, sqrt(a.g.thr[i + 1]) - sqrt(a.g.thr[i]));
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) wmin = fmin(wmin, __shfl_xor(wmin, d, 64));
+      if (tid == 0) {
+        const double dk = L > 1 ? fabs(a.g.kx[1] - a.g.kx[0]) : 0.;
+        const double r = wmin > 0. ? 1.01 * dk / wmin : 1e300;
+        swsteps = r < 4. ? (int)r + 1 : 0;
+      }
+    }
   }
   float2 reg[EPT];
   const float2* in = a.in + blockIdx.y * a.mock_stride;  // this block's catalogue
@@ -245,21 +260,37 @@ xpass_kernel(XPassArgs a) {
           double sum[NP];
 #pragma unroll
           for (int p = 0; p < NP; ++p) sum[p] = 0.;
+          // Bin position b (thresholds <= k2) along the thread's run of kx, on which k2 is
+          // monotone: found once by bisection, then moved by at most `steps` thresholds per kx
+          // step in the run's direction (a fixed number of reads, no data-dependent loop); runs
+          // through bins narrower than the kx spacing take the general walk.
+          const int steps = swsteps;
+          const int dir = l0 < L / 2 ? 1 : -1;  // k2 increases along the run below L/2
           int b = -1;
 #pragma unroll
           for (int e = 0; e < EPT; ++e) {
             const int l = l0 + e;
             const double kxv = kxt[l];
             const double k2 = k2_sum(k2_xy(kxv, kyv), kzv);
-            const int nb = e == 0 ? first_bin(thr, nedges, k2) : walk_bin(thr, nedges, b, k2);
+            int nb;
+            if (e == 0) {
+              nb = first_bin(thr, nedges, k2);
+            } else if (steps > 0) {
+              nb = b;
+              for (int t = 0; t < steps; ++t) {
+                if (dir > 0) nb += (nb < nedges && thr[min(nb, nedges - 1)] <= k2) ? 1 : 0;
+                else nb -= (nb > 0 && thr[max(nb - 1, 0)] > k2) ? 1 : 0;
+              }
+            } else {
+              nb = walk_bin(thr, nedges, b, k2);
+            }
             if (nb != b) {
               if (b >= 1 && b < nedges) {
 #pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                  if (sum[p] != 0.) atomicAdd(&hist[p * nbins + b - 1], sum[p]);
-                  sum[p] = 0.;
-                }
+                for (int p = 0; p < NP; ++p) atomicAdd(&hist[p * nbins + b - 1], sum[p]);
               }
+#pragma unroll
+              for (int p = 0; p < NP; ++p) sum[p] = 0.;
               b = nb;
             }
             if (b < 1 || b >= nedges) continue;
