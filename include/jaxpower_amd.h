@@ -156,6 +156,21 @@ int mp_set_paint_path(int path);
  * report writes one line per tuned case ("... rows=RXxRY ms: ...") and returns their count. */
 int mp_set_paint_tuning(int enable);
 int mp_paint_tuning_report(char* buf, int64_t buflen);
+/* The autotuner's table (no counterpart in the reference, which leaves launch configuration to
+ * XLA).  Knobs: 0 the fused paint's row-block shape, 1 the tile width of the binning x pass,
+ * 2 the tile width of the y pass, 3 the particles per row-sort scatter workgroup.  On the first
+ * launch of a case every candidate is timed on the actual data and the fastest kept (not under
+ * graph capture or per-kernel timing).  mp_tune_export writes the table as a JSON object
+ * {"<case key>": {"knob": name, "choice": q, "ms": [...], "measured": bool}} and returns the
+ * length of the full text; mp_tune_set imports one entry; mp_tune_force(knob, q) forces
+ * candidate q of a knob (-1 frees it) and returns the previous forced candidate;
+ * mp_tune_enable(0) turns measurement off (untuned cases take candidate 0), a negative value
+ * only queries; mp_tune_clear empties the table. */
+int mp_tune_export(char* buf, int64_t buflen);
+int mp_tune_set(const char* key, int knob, int choice);
+int mp_tune_force(int knob, int choice);
+int mp_tune_enable(int enable);
+int mp_tune_clear(void);
 
 /* resamplers.read via RealMeshField.read (jaxpower/resamplers.py:106-131,
  * jaxpower/mesh.py:2065-2115): out[i] = sum_stencil mesh * K. */

@@ -19,7 +19,9 @@
 #include "fft.h"
 #include "fft_core.h"
 #include "sph.h"
+#include "tune.h"
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -440,14 +442,14 @@ xpass_inv_shell_kernel(ShellGen g, float2* out, int ncols, int64_t out_fstride, 
 struct ColPassArgs {
   float2* data;
   int64_t mesh_stride, fstride, ostride;  // between meshes, FFT rows, outer slices
-  int ncols, nouter, nchunks, ntiles;
+  int ncols, nouter, nmesh, nchunks, ntiles;
   const float2* tw;
 };
 
-template <int L, bool INV>
-__global__ void __launch_bounds__(XCfg<L>::NT)
+template <int L, bool INV, int CC = XCfg<L>::C>
+__global__ void __launch_bounds__((XCfgC<L, CC>::NT))
 colpass_kernel(ColPassArgs a) {
-  using Cfg = XCfg<L>;
+  using Cfg = XCfgC<L, CC>;
   constexpr int C = Cfg::C, NT = Cfg::NT, EPT = Cfg::EPT;
   extern __shared__ float2 lds[];
   __shared__ float2 stw[L];
@@ -651,11 +653,12 @@ static int col_fft(int L, bool inv, const char* name, const float2* in, float2* 
   MP_POW2_SWITCH(L, launch_col_fwd, name, in, out, ncols, nouter, ifs, ios, ofs, oos, scale, s);
 }
 
-template <int L>
+template <int L, int CC = XCfg<L>::C>
 static int launch_colpass(hipStream_t s, ColPassArgs& a) {
-  using Cfg = XCfg<L>;
-  auto kern = colpass_kernel<L, false>;
+  using Cfg = XCfgC<L, CC>;
+  auto kern = colpass_kernel<L, false, CC>;
   a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
+  a.ntiles = a.nchunks * a.nouter * a.nmesh;
   int per = 0;
   MP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, Cfg::NT, Cfg::LDS));
   if (per < 1) per = 1;
@@ -666,6 +669,61 @@ static int launch_colpass(hipStream_t s, ColPassArgs& a) {
   if (blocks > a.ntiles) blocks = a.ntiles;
   MP_LAUNCH("fft_ypass", kern, dim3((unsigned)blocks), dim3(Cfg::NT), Cfg::LDS, s, a);
   return MP_OK;
+}
+
+// Tile widths the autotuner tries for the y pass (candidate 0 is the default): half of it while
+// at least 4 columns remain, and twice it up to 64 KiB tiles (0: does not apply; lengths below 64
+// keep the default).
+template <int L>
+struct YPTune {
+  static constexpr int C0 = XCfg<L>::C;
+  static constexpr int C1 = L >= 64 && C0 >= 8 ? C0 / 2 : 0;
+  static constexpr int C2 = L >= 64 && 2 * C0 * L * 8 <= 65536 ? 2 * C0 : 0;
+};
+constexpr size_t YP_TUNE_BYTES = (size_t)128 << 20;
+
+// The y pass with its tile width chosen by the autotuner per (L, columns, x rows, meshes).  The
+// pass works in place, so candidates are timed on a scratch copy of the first x rows of the
+// data (at most 128 MiB, allocated for the measurement only).
+template <int L>
+static int launch_colpass_tuned(hipStream_t s, ColPassArgs& a) {
+  using T = YPTune<L>;
+  auto run = [&](int q, ColPassArgs& b) -> int {
+    if constexpr (T::C1 > 0) {
+      if (q == 1) return launch_colpass<L, T::C1>(s, b);
+    }
+    if constexpr (T::C2 > 0) {
+      if (q == 2) return launch_colpass<L, T::C2>(s, b);
+    }
+    return launch_colpass<L, T::C0>(s, b);
+  };
+  const bool ok[3] = {true, T::C1 > 0, T::C2 > 0};
+  const int forced = tune_forced(TK_YPASS_COLS);
+  if (forced >= 0) return run(forced < 3 && ok[forced] ? forced : 0, a);
+  char key[160];
+  snprintf(key, sizeof(key), "ypass_cols L=%d ncols=%d nouter=%d meshes=%d", L, a.ncols, a.nouter, a.nmesh);
+  const int q0 = tune_get(key);
+  if (q0 >= 0) return run(q0 < 3 && ok[q0] ? q0 : 0, a);
+  if (!(ok[1] || ok[2]) || !tune_measure_ok(s)) return run(0, a);
+  const size_t row_bytes = (size_t)a.ostride * sizeof(float2);  // one x row of one mesh
+  const int64_t rows = std::max<int64_t>(1, std::min<int64_t>(a.nouter, (int64_t)(YP_TUNE_BYTES / row_bytes)));
+  float2* scratch = nullptr;
+  MP_CHECK_HIP(hipMalloc(&scratch, (size_t)rows * row_bytes));
+  ColPassArgs b = a;
+  b.data = scratch;
+  b.nouter = (int)rows;
+  b.nmesh = 1;
+  int rc = MP_OK;
+  if (hipMemcpyAsync(scratch, a.data, (size_t)rows * row_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    rc = set_error(MP_ERR_HIP, "y pass tuning: copy to the scratch rows");
+  float ms[3];
+  const int best = rc ? -1 : tune_time(s, 3, ok, [&](int q) { return run(q, b); }, ms);
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = set_error(MP_ERR_HIP, "y pass tuning: synchronize");
+  (void)hipFree(scratch);
+  if (rc) return rc;
+  if (best < 0) return MP_ERR_HIP;
+  tune_put(key, TK_YPASS_COLS, best, ms, 3);
+  return run(best, a);
 }
 
 static int xpass_ylm_acc(int L, const float2* in, float2* acc, int ncols, int ny, int64_t ifs, int64_t ios, int64_t afs,
@@ -762,11 +820,11 @@ int ypass_meshes(hipStream_t s, float2* data, int nmesh, int64_t mesh_stride, co
   a.nouter = n[0];
   a.tw = twiddle_device_ptr();
   if (!a.tw) return set_error(MP_ERR_HIP, "ypass: twiddle table address");
+  a.nmesh = nmesh;
   switch (n[1]) {
-#define MP_YCASE(L)                                                  \
-  case L:                                                            \
-    a.ntiles = (int)ceil_div(a.ncols, XCfg<L>::C) * n[0] * nmesh;    \
-    return launch_colpass<L>(s, a);
+#define MP_YCASE(L) \
+  case L:           \
+    return launch_colpass_tuned<L>(s, a);
     MP_YCASE(2) MP_YCASE(4) MP_YCASE(8) MP_YCASE(16) MP_YCASE(32) MP_YCASE(64) MP_YCASE(128) MP_YCASE(256) MP_YCASE(512)
     MP_YCASE(1024) MP_YCASE(2048)
 #undef MP_YCASE

@@ -39,7 +39,5 @@ int paint_path_option();
 
 // Autotuner of the fused paint's row-block shape (paint_zfft.hip): enable >= 0 sets it,
 // returns the previous setting; the report lists each tuned case with the timings.
-int paint_tuning_option(int enable);
-int paint_tuning_report(char* buf, int64_t len);
 
 }  // namespace mp

@@ -15,6 +15,7 @@
 #include "fft.h"
 #include "fft_core.h"
 #include "paint.h"
+#include "tune.h"
 
 #include <cstring>
 #include <map>
@@ -222,7 +223,7 @@ __device__ int block_excl_scan(int* data, int n, int* wsum) {
 // scattered 16-byte records.
 constexpr int RS_SCHUNK = 6144;
 
-template <typename T>
+template <typename T, int SCH = RS_SCHUNK>
 __global__ void __launch_bounds__(RS_NT)
 rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, Catalogues cat, GridMap g, int zfrac,
                        int* __restrict__ bucket_cursor, float4* __restrict__ tmp_rec, short* __restrict__ tmp_key) {
@@ -230,18 +231,18 @@ rowsort_scatter_kernel(const T* __restrict__ pos, const float* __restrict__ w, C
   __shared__ int lstart[RS_MAX_KEYS];
   __shared__ int gbase[RS_MAX_KEYS];
   __shared__ int wsum[RS_NT / 64 + 1];
-  extern __shared__ float4 srec[];  // RS_SCHUNK records, then their y keys and x buckets
-  short* skey = reinterpret_cast<short*>(srec + RS_SCHUNK);
-  short* sbkt = skey + RS_SCHUNK;
+  extern __shared__ float4 srec[];  // SCH records, then their y keys and x buckets
+  short* skey = reinterpret_cast<short*>(srec + SCH);
+  short* sbkt = skey + SCH;
   const int n0 = g.n[0];
   int mk;
   int64_t i0, i1;
-  if (!chunk_range(cat, blockIdx.x, RS_SCHUNK, mk, i0, i1)) return;
+  if (!chunk_range(cat, blockIdx.x, SCH, mk, i0, i1)) return;
   bucket_cursor += (int64_t)mk * n0;  // the x buckets of catalogue mk
   for (int b = threadIdx.x; b < n0; b += RS_NT) hist[b] = 0;
   __syncthreads();
   const int cnt = (int)(i1 - i0);
-  constexpr int PER = RS_SCHUNK / RS_NT;
+  constexpr int PER = SCH / RS_NT;
   int rank[PER], hxq[PER], hyq[PER];
   float4 recq[PER];
 #pragma unroll
@@ -1061,28 +1062,10 @@ static int launch_pz_any(int order, int M, const PzShape& sh, hipStream_t s, con
 }
 
 // ------------------------------------------------------------------ autotuner
-// On the first paint of a (device, meshsize, order, interlacing) case, every supported row-block
-// shape is timed on the actual sorted particles (two launches each, HIP events) and the
-// fastest is kept for the process.  Skipped while the stream is being captured into a graph,
-// and for small catalogues, which take the default shape.
+// On the first paint of a (meshsize, order, interlacing, batch) case, every row-block shape
+// that fits is timed on the actual sorted particles (tune.h) and the fastest is kept.  Small
+// catalogues take the first shape that fits.
 
-struct PzTuneKey {
-  int dev, n0, n1, n2, order, ni, batch;
-  bool operator<(const PzTuneKey& o) const {
-    const int a[7] = {dev, n0, n1, n2, order, ni, batch}, b[7] = {o.dev, o.n0, o.n1, o.n2, o.order, o.ni, o.batch};
-    for (int q = 0; q < 7; ++q)
-      if (a[q] != b[q]) return a[q] < b[q];
-    return false;
-  }
-};
-struct PzTuneEntry {
-  PzShape shape;
-  float ms[4];
-  int count;
-};
-static std::mutex g_pz_tune_mu;
-static std::map<PzTuneKey, PzTuneEntry> g_pz_tuned;
-static int g_pz_tune = 1;
 constexpr int64_t PZ_TUNE_MIN_NP = 1 << 18;
 
 static int pz_choose(hipStream_t s, int order, int M, const int32_t n[3], int ni, int64_t np, const float4* sorted,
@@ -1090,99 +1073,52 @@ static int pz_choose(hipStream_t s, int order, int M, const int32_t n[3], int ni
                      PzShape& chosen) {
   int count = 0;
   const PzShape* c = pz_candidates(M, count);
+  bool ok[TUNE_MAX_CAND] = {};
   int first = -1, nok = 0;
-  for (int q = 0; q < count; ++q)
-    if (pz_shape_ok(n, c[q], order, ni)) {
+  for (int q = 0; q < count && q < TUNE_MAX_CAND; ++q)
+    if ((ok[q] = pz_shape_ok(n, c[q], order, ni))) {
       if (first < 0) first = q;
       ++nok;
     }
   if (first < 0) return set_error(MP_ERR_UNSUPPORTED, "paint_zfft: no row-block shape fits (%d, %d, %d)", n[0], n[1], n[2]);
   chosen = c[first];
-  if (g_pz_tune >= 100) {  // a forced candidate (tests of every shape)
-    const int q = g_pz_tune - 100;
-    if (q < count && pz_shape_ok(n, c[q], order, ni)) chosen = c[q];
+  const int forced = tune_forced(TK_PAINT_ROWS);
+  if (forced >= 0) {  // tests of every shape
+    if (forced < count && ok[forced]) chosen = c[forced];
     return MP_OK;
   }
-  int dev = 0;
-  MP_CHECK_HIP(hipGetDevice(&dev));
-  PzTuneKey key{dev, n[0], n[1], n[2], order, ni, rg.nmock > 1};
-  {
-    std::lock_guard<std::mutex> lock(g_pz_tune_mu);
-    auto it = g_pz_tuned.find(key);
-    if (it != g_pz_tuned.end()) {
-      chosen = it->second.shape;
-      return MP_OK;
-    }
+  char key[160];
+  snprintf(key, sizeof(key), "paint_rows n=%dx%dx%d order=%d interlacing=%d%s", n[0], n[1], n[2], order, ni,
+           rg.nmock > 1 ? " batch" : "");
+  const int q0 = tune_get(key);
+  if (q0 >= 0) {
+    if (q0 < count && ok[q0]) chosen = c[q0];
+    return MP_OK;
   }
-  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  MP_CHECK_HIP(hipStreamIsCapturing(s, &cap));
-  if (!g_pz_tune || nok < 2 || np < PZ_TUNE_MIN_NP || cap != hipStreamCaptureStatusNone || timing_enabled()) return MP_OK;
-  PzTuneEntry entry{};
-  entry.count = count;
-  float best = 1e30f;
-  hipEvent_t e0, e1;
-  MP_CHECK_HIP(hipEventCreate(&e0));
-  MP_CHECK_HIP(hipEventCreate(&e1));
-  int rc = MP_OK;
-  for (int q = 0; q < count && rc == MP_OK; ++q) {
-    entry.ms[q] = -1.f;
-    if (!pz_shape_ok(n, c[q], order, ni)) continue;
-    float tmin = 1e30f;
-    for (int rep = 0; rep < 2 && rc == MP_OK; ++rep) {
-      if (hipEventRecord(e0, s) != hipSuccess) { rc = set_error(MP_ERR_HIP, "paint_zfft tuning: event record"); break; }
-      rc = launch_pz_any(order, M, c[q], s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
-      if (rc) break;
-      float ms = 0.f;
-      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = set_error(MP_ERR_HIP, "paint_zfft tuning: event timing");
-        break;
-      }
-      tmin = ms < tmin ? ms : tmin;
-    }
-    entry.ms[q] = tmin;
-    if (tmin < best) {
-      best = tmin;
-      chosen = c[q];
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc) return rc;
-  entry.shape = chosen;
-  std::lock_guard<std::mutex> lock(g_pz_tune_mu);
-  g_pz_tuned[key] = entry;
+  if (nok < 2 || np < PZ_TUNE_MIN_NP || !tune_measure_ok(s)) return MP_OK;
+  float ms[TUNE_MAX_CAND];
+  const int best = tune_time(s, count, ok, [&](int q) {
+    return launch_pz_any(order, M, c[q], s, sorted, offsets, row_wabs, rg, out, pitch, mesh_stride);
+  }, ms);
+  if (best < 0) return MP_ERR_HIP;
+  chosen = c[best];
+  tune_put(key, TK_PAINT_ROWS, best, ms, count);
   return MP_OK;
 }
 
-int paint_tuning_option(int enable) {
-  int old = g_pz_tune;
-  if (enable >= 0) g_pz_tune = enable >= 100 ? enable : (enable ? 1 : 0);
-  return old;
-}
+// Scatter workgroup sizes the autotuner tries besides RS_SCHUNK: smaller LDS staging lets two
+// workgroups share a CU.
+constexpr int RS_SCHUNK_ALT[2] = {3072, 4096};
 
-int paint_tuning_report(char* buf, int64_t len) {
-  std::lock_guard<std::mutex> lock(g_pz_tune_mu);
-  std::string text;
-  char line[512];
-  for (const auto& kv : g_pz_tuned) {
-    const PzTuneKey& k = kv.first;
-    const PzTuneEntry& e = kv.second;
-    int count = 0;
-    const PzShape* c = pz_candidates(k.n2 / 2, count);
-    int off = snprintf(line, sizeof(line), "paint_zfft n=%dx%dx%d order=%d interlacing=%d%s rows=%dx%d ms:", k.n0, k.n1, k.n2,
-                       k.order, k.ni, k.batch ? " batch" : "", e.shape.rx, e.shape.ry);
-    for (int q = 0; q < count && off < (int)sizeof(line) - 32; ++q)
-      off += snprintf(line + off, sizeof(line) - off, " %dx%d=%.4f", c[q].rx, c[q].ry, e.ms[q]);
-    text += line;
-    text += "\n";
-  }
-  if (buf && len > 0) {
-    size_t ncopy = text.size() < (size_t)(len - 1) ? text.size() : (size_t)(len - 1);
-    memcpy(buf, text.data(), ncopy);
-    buf[ncopy] = 0;
-  }
-  return (int)g_pz_tuned.size();
+template <typename T, int SCH>
+static int launch_scatter(hipStream_t s, const T* pos, const float* w, int64_t np, const Catalogues& cb, int extra, const GridMap& g,
+                          int* bcursor, float4* tmp_rec, short* tmp_key) {
+  constexpr size_t slds = (size_t)SCH * (sizeof(float4) + 2 * sizeof(short));
+  auto skern = rowsort_scatter_kernel<T, SCH>;
+  MP_CHECK_HIP(hipFuncSetAttribute((const void*)skern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));
+  MP_LAUNCH("paint_rowsort_scatter", skern, dim3((unsigned)(ceil_div(np, SCH) + extra)), dim3(RS_NT), slds, s, pos, w, cb, g,
+            zfrac_of(g.n[2]), bcursor, tmp_rec, tmp_key);
+  return MP_OK;
 }
 
 // The row sort shared by the fused paint and the real-space row paint: records grouped by home
@@ -1234,11 +1170,40 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
   rc = exclusive_scan_i32(s, bstart, nb + 1, scan_tmp, bcursor);
   if (rc) return rc;
   if (np > 0) {
-    constexpr size_t slds = (size_t)RS_SCHUNK * (sizeof(float4) + 2 * sizeof(short));
-    auto skern = rowsort_scatter_kernel<T>;
-    MP_CHECK_HIP(hipFuncSetAttribute((const void*)skern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));
-    MP_LAUNCH("paint_rowsort_scatter", skern, dim3((unsigned)(ceil_div(np, RS_SCHUNK) + extra)), dim3(RS_NT), slds, s, pos, w, cb,
-              g, zfrac_of(n[2]), bcursor, tmp_rec, tmp_key);
+    auto run = [&](int q) -> int {
+      switch (q) {
+        case 1: return launch_scatter<T, RS_SCHUNK_ALT[0]>(s, pos, w, np, cb, extra, g, bcursor, tmp_rec, tmp_key);
+        case 2: return launch_scatter<T, RS_SCHUNK_ALT[1]>(s, pos, w, np, cb, extra, g, bcursor, tmp_rec, tmp_key);
+        default: return launch_scatter<T, RS_SCHUNK>(s, pos, w, np, cb, extra, g, bcursor, tmp_rec, tmp_key);
+      }
+    };
+    // Particles per scatter workgroup: tuned per mesh for single catalogues (a batch's chunk
+    // offsets are laid out for the default).  Each timing run restarts the bucket cursors.
+    int q = 0;
+    if (!moff) {
+      const int forced = tune_forced(TK_SORT_CHUNK);
+      char key[128];
+      snprintf(key, sizeof(key), "sort_chunk nx=%d ny=%d positions=%s", n[0], n[1], sizeof(T) == 8 ? "f64" : "f32");
+      if (forced >= 0) {
+        q = forced < 3 ? forced : 0;
+      } else if ((q = tune_get(key)) < 0) {
+        q = 0;
+        if (np >= PZ_TUNE_MIN_NP && tune_measure_ok(s)) {
+          const bool ok[3] = {true, true, true};
+          float ms[3];
+          q = tune_time(s, 3, ok, [&](int c) -> int {
+            if (hipMemcpyAsync(bcursor, bstart, (size_t)(nb + 1) * sizeof(int), hipMemcpyDeviceToDevice, s) != hipSuccess)
+              return set_error(MP_ERR_HIP, "row sort tuning: cursor reset");
+            return run(c);
+          }, ms);
+          if (q < 0) return MP_ERR_HIP;
+          tune_put(key, TK_SORT_CHUNK, q, ms, 3);
+          MP_CHECK_HIP(hipMemcpyAsync(bcursor, bstart, (size_t)(nb + 1) * sizeof(int), hipMemcpyDeviceToDevice, s));
+        }
+      }
+    }
+    rc = run(q);
+    if (rc) return rc;
   }
   constexpr size_t flds = (size_t)RS_SCHUNK * (sizeof(float4) + sizeof(short));
   MP_CHECK_HIP(hipFuncSetAttribute((const void*)rowsort_fine_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));

@@ -387,9 +387,7 @@ MP_EXPORT int mp_set_paint_path(int path) {
   return old;
 }
 
-MP_EXPORT int mp_set_paint_tuning(int enable) { return paint_tuning_option(enable); }
 
-MP_EXPORT int mp_paint_tuning_report(char* buf, int64_t buflen) { return paint_tuning_report(buf, buflen); }
 
 MP_EXPORT int mp_fill_f32(void* stream, float* p, int64_t n, float value) { return fill_f32((hipStream_t)stream, p, n, value); }
 

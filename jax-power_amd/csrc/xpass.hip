@@ -18,6 +18,7 @@
 #include "fft.h"
 #include "fft_core.h"
 #include "sph.h"
+#include "tune.h"
 
 #include <algorithm>
 #include <mutex>
@@ -89,10 +90,10 @@ __device__ __forceinline__ int xcd_first_tile() {
   return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
 }
 
-template <int L, bool BIN, int NP, bool TAB, bool YLM = false>
-__global__ void __launch_bounds__(XPCfg<L>::NT)
+template <int L, bool BIN, int NP, bool TAB, bool YLM = false, int CC = XPCfg<L>::C>
+__global__ void __launch_bounds__((XCfgC<L, CC>::NT))
 xpass_kernel(XPassArgs a) {
-  using Cfg = XPCfg<L>;
+  using Cfg = XCfgC<L, CC>;
   constexpr int C = Cfg::C, NT = Cfg::NT;
   constexpr int EPT = Cfg::EPT;
   static_assert(EPT >= 1 && EPT * NT == C * L && (L / 2) % EPT == 0, "tile must split into monotone runs");
@@ -581,10 +582,10 @@ int combine_tables(hipStream_t s, const int32_t n[3], int order, int ni, int com
   return MP_OK;
 }
 
-template <int L, bool BIN, int NP, bool TAB, bool YLM = false>
+template <int L, bool BIN, int NP, bool TAB, bool YLM = false, int CC = XPCfg<L>::C>
 static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles, int nmock) {
-  using Cfg = XPCfg<L>;
-  auto kern = xpass_kernel<L, BIN, NP, TAB, YLM>;
+  using Cfg = XCfgC<L, CC>;
+  auto kern = xpass_kernel<L, BIN, NP, TAB, YLM, CC>;
   a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
   a.ntiles = a.nchunks * a.ny;
   const size_t lds = Cfg::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, TAB);
@@ -609,11 +610,52 @@ static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double*
 }
 
 // The kx/kz tables go to LDS when they fit beside the tile.
-template <int L, bool BIN, int NP, bool YLM = false>
+template <int L, bool BIN, int NP, bool YLM = false, int CC = XPCfg<L>::C>
 static int launch_xpass(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles, int nmock = 1) {
-  const size_t with_tab = XPCfg<L>::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, true);
-  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true, YLM>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
-  return launch_xpass_t<L, BIN, NP, false, YLM>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
+  const size_t with_tab = XCfgC<L, CC>::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, true);
+  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true, YLM, CC>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
+  return launch_xpass_t<L, BIN, NP, false, YLM, CC>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
+}
+
+// Tile widths the autotuner tries for the binning x pass (candidate 0 is the default): half the
+// default while at least 4 columns remain, and twice it up to 64 KiB tiles; 0 marks a candidate
+// that does not apply.  Lengths below 64 keep the default.
+template <int L>
+struct XPTune {
+  static constexpr int C0 = XPCfg<L>::C;
+  static constexpr int C1 = L >= 64 && C0 >= 8 ? C0 / 2 : 0;
+  static constexpr int C2 = L >= 64 && 2 * C0 * L * 8 <= 65536 ? 2 * C0 : 0;
+};
+
+// The binning x pass with a global line of sight (P_ell weights) and at most 3 products, its
+// tile width chosen by the autotuner per (L, Ny, columns, n_i, products, catalogues).  Timing
+// launches only rewrite the partial histograms; the last launch is the chosen one.
+template <int L, int NP>
+static int launch_xpass_bin_tuned(hipStream_t s, XPassArgs& a, int* nblocks, double* partial, size_t partial_doubles, int nmock) {
+  using T = XPTune<L>;
+  auto run = [&](int q) -> int {
+    if constexpr (T::C1 > 0) {
+      if (q == 1) return launch_xpass<L, true, NP, false, T::C1>(s, a, nblocks, partial, partial_doubles, nmock);
+    }
+    if constexpr (T::C2 > 0) {
+      if (q == 2) return launch_xpass<L, true, NP, false, T::C2>(s, a, nblocks, partial, partial_doubles, nmock);
+    }
+    return launch_xpass<L, true, NP, false, T::C0>(s, a, nblocks, partial, partial_doubles, nmock);
+  };
+  const bool ok[3] = {true, T::C1 > 0, T::C2 > 0};
+  const int forced = tune_forced(TK_XPASS_COLS);
+  if (forced >= 0) return run(forced < 3 && ok[forced] ? forced : 0);
+  char key[160];
+  snprintf(key, sizeof(key), "xpass_cols L=%d ny=%d ncols=%d interlacing=%d products=%d catalogues=%d", L, a.ny, a.ncols, a.ni, NP,
+           nmock);
+  const int q0 = tune_get(key);
+  if (q0 >= 0) return run(q0 < 3 && ok[q0] ? q0 : 0);
+  if (!(ok[1] || ok[2]) || !tune_measure_ok(s)) return run(0);
+  float ms[3];
+  const int best = tune_time(s, 3, ok, run, ms);
+  if (best < 0) return MP_ERR_HIP;
+  tune_put(key, TK_XPASS_COLS, best, ms, 3);
+  return run(best);
 }
 
 template <int L>
@@ -629,9 +671,9 @@ static int xpass_bin_L(hipStream_t s, XPassArgs& a, int nprod, double* partial, 
     return launch_xpass<L, true, 1, true>(s, a, nblocks, partial, partial_doubles, nmock);
   }
   switch (nprod) {
-    case 1: return launch_xpass<L, true, 1>(s, a, nblocks, partial, partial_doubles, nmock);
-    case 2: return launch_xpass<L, true, 2>(s, a, nblocks, partial, partial_doubles, nmock);
-    case 3: return launch_xpass<L, true, 3>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 1: return launch_xpass_bin_tuned<L, 1>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 2: return launch_xpass_bin_tuned<L, 2>(s, a, nblocks, partial, partial_doubles, nmock);
+    case 3: return launch_xpass_bin_tuned<L, 3>(s, a, nblocks, partial, partial_doubles, nmock);
     case 4: return launch_xpass<L, true, 4>(s, a, nblocks, partial, partial_doubles, nmock);
     case 5: return launch_xpass<L, true, 5>(s, a, nblocks, partial, partial_doubles, nmock);
     case 6: return launch_xpass<L, true, 6>(s, a, nblocks, partial, partial_doubles, nmock);

@@ -21,6 +21,9 @@ from .types import (Mesh2SpectrumPole, Mesh2SpectrumPoles, Mesh2CorrelationPole,
 from .autodiff import mesh2_spectrum_poles, box_spectrum_values
 from .batch import Mesh2SpectrumBatch
 from . import resamplers
+from . import tuning
+
+tuning._autoload()  # only when JAXPOWER_AMD_TUNING_CACHE names a cache file
 
 __all__ = ['MeshAttrs', 'BaseMeshField', 'RealMeshField', 'ComplexMeshField', 'ParticleField', 'FKPField', 'r2c', 'c2r',
            'apply', 'read', 'paint', 'fftfreq', 'get_mesh_attrs', 'next_fft_size', 'compute_normalization',
@@ -28,7 +31,7 @@ __all__ = ['MeshAttrs', 'BaseMeshField', 'RealMeshField', 'ComplexMeshField', 'P
            'compute_mesh2_spectrum', 'compute_mesh2_correlation', 'Mesh2CorrelationPole', 'Mesh2CorrelationPoles',
            'compute_fkp2_normalization', 'compute_box2_normalization', 'compute_fkp2_shotnoise',
            'generate_uniform_particles', 'generate_gaussian_mesh', 'generate_lognormal_particles', 'Mesh2SpectrumPole',
-           'Mesh2SpectrumPoles', 'mesh2_spectrum_poles', 'box_spectrum_values', 'Mesh2SpectrumBatch', 'resamplers', 'BinMesh3SpectrumPoles',
+           'Mesh2SpectrumPoles', 'mesh2_spectrum_poles', 'box_spectrum_values', 'Mesh2SpectrumBatch', 'resamplers', 'tuning', 'BinMesh3SpectrumPoles',
            'compute_mesh3', 'compute_mesh3_spectrum', 'compute_box3_normalization', 'compute_fkp3_normalization', 'Mesh3SpectrumPole', 'Mesh3SpectrumPoles',
            'compute_mesh2_spectrum_window', 'compute_mesh2_spectrum_mean', 'WindowMatrix', 'compute_fkp3_shotnoise',
            'BinMesh3CorrelationPoles', 'compute_mesh3_correlation', 'Mesh3CorrelationPole', 'Mesh3CorrelationPoles']

@@ -136,6 +136,11 @@ SIGNATURES = {
     'mp_set_paint_path': (c_int, [c_int]),
     'mp_set_paint_tuning': (c_int, [c_int]),
     'mp_paint_tuning_report': (c_int, [c_char_p, c_int64]),
+    'mp_tune_export': (c_int, [c_char_p, c_int64]),
+    'mp_tune_set': (c_int, [c_char_p, c_int, c_int]),
+    'mp_tune_force': (c_int, [c_int, c_int]),
+    'mp_tune_enable': (c_int, [c_int]),
+    'mp_tune_clear': (c_int, []),
 }
 
 

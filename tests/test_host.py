@@ -297,3 +297,34 @@ def test_sugiyama_host_coefficients():
             assert np.allclose(got, po.complex_ylm_conj(ell, m, *u.T), atol=1e-13), (ell, m)
             for v in u[:3]:
                 assert np.isclose(mesh3._complex_ylm_conj_at(ell, m, v), po.complex_ylm_conj(ell, m, *v), atol=1e-13)
+
+
+def test_tuning_table_json_cache(tmp_path):
+    """The autotuner's table imports a JSON cache entry by entry, exports it again, skips a
+    cache measured on another device type and rejects malformed entries (no device needed)."""
+    import json
+    from jaxpower_amd import tuning
+    tuning.clear()
+    try:
+        doc = {'format': 1, 'device': 'gfx950', 'entries': {
+            'xpass_cols L=512 ny=512 ncols=257 interlacing=3 products=3 catalogues=1': {'knob': 'xpass_cols', 'choice': 2},
+            'sort_chunk nx=512 ny=512 positions=f32': {'knob': 'sort_chunk', 'choice': 1}}}
+        path = tmp_path / 'cache.json'
+        path.write_text(json.dumps(doc))
+        assert tuning.load(str(path), device='gfx942') == 0 and tuning.table() == {}
+        assert tuning.load(str(path), device='gfx950') == 2
+        table = tuning.table()
+        assert {k: (e['knob'], e['choice'], e['measured']) for k, e in table.items()} == {
+            k: (e['knob'], e['choice'], False) for k, e in doc['entries'].items()}
+        out = tmp_path / 'out.json'
+        assert tuning.save(str(out)) == 2
+        assert json.loads(out.read_text())['entries'] == table
+        bad = dict(doc, entries={'x': {'knob': 'nope', 'choice': 0}})
+        path.write_text(json.dumps(bad))
+        with pytest.raises(ValueError):
+            tuning.load(str(path), device='gfx950')
+        with pytest.raises(ValueError):
+            path.write_text(json.dumps(dict(doc, format=99)))
+            tuning.load(str(path), device='gfx950')
+    finally:
+        tuning.clear()
