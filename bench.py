@@ -153,8 +153,12 @@ def summarize_pmc(fetch_dir, write_dir, copy_bytes, out=TRAFFIC_FILE):
 
 
 SQ_FILE = os.path.join(ROOT, 'profiles', 'sq_counters_512_tsc3.txt')
-SQ_COUNTERS = ('SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_VALU '
-               'SQ_BUSY_CU_CYCLES SQ_WAVE_CYCLES')
+SQ_COUNTERS = ('SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS SQ_INSTS_SALU '
+               'SQ_BUSY_CU_CYCLES SQ_WAVE_CYCLES GRBM_GUI_ACTIVE')
+# MI355X: GRBM_GUI_ACTIVE is summed over the 8 XCDs; 256 CUs.  A wave64 VALU instruction
+# occupies a 16-lane SIMD for 4 clocks, so a CU issues at most 1 VALU wave-instruction per clock
+# (measured: 0.98 with a pure-FMA kernel).
+SQ_XCDS, SQ_CUS = 8, 256
 SQ_COMMAND = ('rocprofv3 --pmc ' + SQ_COUNTERS + ' -d gpurun_out/pmc_sq -o run --output-format csv -- '
               'python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-batch')
 
@@ -170,8 +174,10 @@ def summarize_sq(folder, out=SQ_FILE):
             per[kname][counter][disp] += val
     lines = ['# rocprofv3 SQ counters of the 512^3 TSCx3 step kernels (means per dispatch, summed over the chip)',
              '# made by: ' + SQ_COMMAND, f'#          python bench.py --summarize-sq {folder}',
-             f'# {"kernel":44s} {"waves":>8s} {"VALU/wave":>10s} {"VALU/CUcyc":>10s} {"waves/CU":>9s} {"LDSconf/LDSact":>14s} '
-             f'{"LDS/wave":>9s}']
+             '# VALU/clk and SALU/clk: wave-instructions per CU per clock (GRBM_GUI_ACTIVE / 8 XCDs clocks, 256 CUs; the',
+             '# VALU issue limit is 1.0); LDSconf/clk: LDS bank-conflict cycles per CU per clock',
+             f'# {"kernel":52s} {"waves":>8s} {"VALU/wave":>10s} {"VALU/clk":>9s} {"SALU/clk":>9s} {"LDS/wave":>9s} '
+             f'{"LDSconf/clk":>11s} {"clocks":>9s}']
     for kname, cnt in sorted(per.items()):
         if 'mp::' not in kname:
             continue
@@ -180,12 +186,14 @@ def summarize_sq(folder, out=SQ_FILE):
         busy = m.get('SQ_BUSY_CU_CYCLES', 0.)
         if waves <= 0 or busy <= 0:
             continue
-        name = kname.replace('void ', '').split('(')[0][:44]
-        lds_act = m.get('SQ_ACTIVE_INST_LDS', 0.)
-        lines.append(f'  {name:44s} {waves:8.0f} {m.get("SQ_INSTS_VALU", 0.) / waves:10.1f} '
-                     f'{m.get("SQ_INSTS_VALU", 0.) / busy:10.3f} {m.get("SQ_WAVE_CYCLES", 0.) / busy:9.2f} '
-                     f'{(m.get("SQ_LDS_BANK_CONFLICT", 0.) / lds_act if lds_act else 0.):14.3f} '
-                     f'{m.get("SQ_INSTS_LDS", 0.) / waves:9.1f}')
+        name = kname.replace('void ', '').split('(')[0][:52]
+        clk = m.get('GRBM_GUI_ACTIVE', 0.) / SQ_XCDS
+        if clk <= 0:
+            continue
+        per_clk = lambda c: m.get(c, 0.) / SQ_CUS / clk  # noqa: E731
+        lines.append(f'  {name:52s} {waves:8.0f} {m.get("SQ_INSTS_VALU", 0.) / waves:10.1f} {per_clk("SQ_INSTS_VALU"):9.3f} '
+                     f'{per_clk("SQ_INSTS_SALU"):9.3f} {m.get("SQ_INSTS_LDS", 0.) / waves:9.1f} '
+                     f'{per_clk("SQ_LDS_BANK_CONFLICT"):11.3f} {clk:9.0f}')
     with open(out, 'w') as fh:
         fh.write('\n'.join(lines) + '\n')
     print('\n'.join(lines))
