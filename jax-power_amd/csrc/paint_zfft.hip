@@ -51,6 +51,8 @@ struct RowGeom {
   int ni;
   int zfrac;           // fractional bits of z in the records
   int nmock;           // catalogues stacked along x in the row sort (1 for one catalogue)
+  const unsigned* wmax_bits;  // per catalogue: bits of the largest |w| (a non-negative float)
+  const int* negw;            // per catalogue: nonzero when some weight is negative
 };
 
 // Catalogues stacked in one particle array (a batch of mocks): catalogue m holds particles
@@ -154,7 +156,7 @@ __device__ __forceinline__ float wave_max_f32(float v) {
 template <typename T>
 __global__ void __launch_bounds__(RS_NT)
 rowsort_hist_kernel(const T* __restrict__ pos, const float* __restrict__ w, Catalogues cat, GridMap g, int* __restrict__ bucket_total,
-                    unsigned* __restrict__ wmax_bits) {
+                    unsigned* __restrict__ wmax_bits, int* __restrict__ negw) {
   __shared__ int hist[RS_MAX_KEYS];
   __shared__ float wmax[RS_NT / 64];
   int mk;
@@ -164,12 +166,17 @@ rowsort_hist_kernel(const T* __restrict__ pos, const float* __restrict__ w, Cata
   for (int b = threadIdx.x; b < g.n[0]; b += RS_NT) hist[b] = 0;
   __syncthreads();
   float m = w ? 0.f : 1.f;
+  bool neg = false;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += RS_NT) {
     T xg[3];
     grid_coords(pos, i, g, xg);
     atomicAdd(&hist[pmod((int)floor(xg[0]), g.n[0])], 1);
-    if (w) m = fmaxf(m, fabsf(w[i]));
+    if (w) {
+      m = fmaxf(m, fabsf(w[i]));
+      neg = neg || w[i] < 0.f;
+    }
   }
+  if (__any(neg) && (threadIdx.x & 63) == 0) atomicOr(negw + mk, 1);
   m = wave_max_f32(m);
   if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -482,6 +489,63 @@ struct LaneRotation {
   }
 };
 
+// A tighter bound on |cell| for catalogues with weights of both signs.  Their painted cells
+// nearly cancel, so the row bound (sum of |w| over the home rows that reach a row, over the
+// whole z extent) leaves most of the fixed-point range unused and the rounding of each addend
+// becomes visible in the spectrum.  Here |w| is rounded up to 1/4096 of the catalogue's largest
+// |w| and histogrammed by (home x row, home z cell) over every particle that reaches the block
+// (all home y rows of the reach merged); summed over the homes that reach each target (x row,
+// z cell), its largest value bounds every cell of the block.  Integer sums: the bound does not
+// depend on the order of the particles.  `hist` is the zeroed row LDS with room for
+// (RX + hi - lo) * nz ints; it is zeroed again on return.  All threads of the block call this.
+template <int NT>
+__device__ float tight_row_bound(int* hist, int hist_cap, int RX, int nz, int lo, int hi, int zfrac, float wmax,
+                                 const float4* __restrict__ sorted, const int* sstart, const int* scount, const int* shome,
+                                 const int* spre, int nseg, int total, int chunk, unsigned long long* sred) {
+  const int span = hi - lo, sx = RX + span;
+  const int tid = threadIdx.x;
+  if (sx * nz > hist_cap || !(wmax > 0.f)) return 3.4e38f;  // no room or no weight: keep the row bound
+  const float qs = 4096.f / wmax;
+  for (int c = tid; c < total; c += NT) {
+    int lo_i = 0, hi_i = nseg - 1;
+    while (lo_i < hi_i) {
+      const int mid = (lo_i + hi_i + 1) >> 1;
+      if (spre[mid] <= c) lo_i = mid;
+      else hi_i = mid - 1;
+    }
+    const int p0 = sstart[lo_i] + (c - spre[lo_i]) * chunk;
+    const int p1 = min(p0 + chunk, sstart[lo_i] + scount[lo_i]);
+    const int ix = (shome[lo_i] >> 16) + hi;  // home x row relative to the reach, in [0, sx)
+    for (int p = p0; p < p1; ++p) {
+      const float4 rec = sorted[p];
+      const int zc = (int)(__float_as_uint(rec.z) >> zfrac);
+      atomicAdd(&hist[ix * nz + zc], (int)ceilf(fabsf(rec.w) * qs));
+    }
+  }
+  if (tid == 0) *sred = 0ull;
+  __syncthreads();
+  // target x row t is reached from ix in [t, t + span]; target z from home z in [z - hi, z - lo]
+  unsigned long long m = 0ull;
+  for (int i = tid; i < RX * nz; i += NT) {
+    const int t = i / nz, z = i % nz;
+    unsigned long long b = 0ull;
+    for (int ix = t; ix <= t + span; ++ix)
+      for (int d = lo; d <= hi; ++d) b += (unsigned)hist[ix * nz + ((z - d) % nz + nz) % nz];
+    m = b > m ? b : m;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(m, d, 64);
+    m = o > m ? o : m;
+  }
+  if ((tid & 63) == 0) atomicMax(sred, m);
+  __syncthreads();
+  const float bound = (float)((double)*sred * (double)wmax * (1.0 / 4096.0) * (1.0 + 1e-6));
+  for (int i = tid; i < sx * nz; i += NT) hist[i] = 0;
+  __syncthreads();
+  return bound;
+}
+
 // First Stockham stage (radix first_radix(M), no twiddles) of the z transform of NROW rows:
 // reads the painted fixed-point cells (row r, cell z at ints[r * RS + z]; complex element e is
 // the pair z = 2e, 2e + 1), scales them to float and writes the stage output in the column
@@ -548,6 +612,7 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __shared__ float tbound[NROW];
   __shared__ float2 stw[NZ];
   __shared__ int sregion[17];  // first segment of each chunk region, and the last segment
+  __shared__ unsigned long long sredu;
   const int tid = threadIdx.x;
   load_twiddles<NZ, PZ_NT>(stw, tw);
 
@@ -654,10 +719,14 @@ paint_zfft_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
     sregion[g] = lo_i;
   }
   __syncthreads();
+  float bound = sbound[0];
+  if (rg.negw[mk])
+    bound = fminf(bound, tight_row_bound<PZ_NT>(reinterpret_cast<int*>(rows), NROW * RS, RX, NZ, lo, hi, rg.zfrac,
+                                                 __uint_as_float(rg.wmax_bits[mk]), sorted, sstart, scount, shome, spre, nseg,
+                                                 total, CHUNK, &sredu));
   // Fixed-point accumulation: integer LDS atomics are exact, order independent (the painted
   // rows are bitwise reproducible) and much faster than float LDS atomics on this target.
   // |sum| <= 0.99 * 2^31 (each K <= 1), and each addend is rounded to 1/scale.
-  const float bound = sbound[0];
   const float scale = bound > 0.f ? 2126008811.f / bound : 1.f;
   const int* irows = reinterpret_cast<const int*>(rows);
   lds_u8* const lrows = (lds_u8*)(rows);  // the rows as an LDS-space byte pointer
@@ -789,6 +858,7 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   __shared__ float swabs[SEG];
   __shared__ float tbound[NROW];
   __shared__ float sbound[1];
+  __shared__ unsigned long long sredu;
   const int tid = threadIdx.x;
   const int n2 = rg.n[2], rs = n2 + 1;
   const int id = blockIdx.x;
@@ -857,7 +927,10 @@ paint_rows_kernel(const float4* __restrict__ sorted, const int* __restrict__ off
   }
   __syncthreads();
   const int total = spre[nseg];
-  const float bound = sbound[0];
+  float bound = sbound[0];
+  if (rg.negw[0])
+    bound = fminf(bound, tight_row_bound<PR_NT>(irows, NROW * rs, RX, n2, lo, hi, rg.zfrac, __uint_as_float(rg.wmax_bits[0]),
+                                                sorted, sstart, scount, shome, spre, nseg, total, CHUNK, &sredu));
   const float scale = bound > 0.f ? 2126008811.f / bound : 1.f;
   const int rxmax = min(RX, rg.n[0] - x0), rymax = min(RY, rg.n[1] - y0);  // partial blocks at the edge
   constexpr int G = 16;  // chunks taken from G regions in turn, as in the fused paint
@@ -987,7 +1060,7 @@ size_t paint_zfft_workspace_bytes(const int32_t n[3], int64_t np, int nmock) {
   size_t npp = (size_t)(np > 0 ? np : 1);
   size_t bytes = align_up((size_t)(nkeys + 1) * 4, 256);          // row offsets
   bytes += 2 * align_up((size_t)(nb + 2) * 4, 256);               // bucket start / cursor
-  bytes += align_up((size_t)nmock * 4, 256);                      // max |w| per catalogue
+  bytes += 2 * align_up((size_t)nmock * 4, 256);                  // max |w| and a negative-weight flag per catalogue
   bytes += 2 * align_up((size_t)(nmock + 1) * 4, 256);            // chunk offsets of the catalogues
   bytes += scan_workspace_bytes(nb + 1);
   bytes += align_up(npp * 16, 256) * 2 + align_up(npp * 2, 256);  // tmp records, sorted records, tmp keys
@@ -1127,6 +1200,8 @@ struct RowSorted {
   const float4* sorted;
   const int* offsets;
   const float* row_wabs;
+  const unsigned* wmax_bits;
+  const int* negw;
 };
 
 // moff: nullptr for one catalogue of np particles, else the device offsets of nmock catalogues
@@ -1146,6 +1221,7 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
   int* bstart = wsp.take<int>(nb + 1);
   int* bcursor = wsp.take<int>(nb + 2);
   unsigned* wmax = wsp.take<unsigned>(nmock);
+  int* negw = wsp.take<int>(nmock);
   int* choff_a = wsp.take<int>(nmock + 1);
   int* choff_b = wsp.take<int>(nmock + 1);
   int* scan_tmp = (int*)wsp.take<char>(scan_workspace_bytes(nb + 1));
@@ -1153,11 +1229,13 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
   float4* sorted = wsp.take<float4>(np > 0 ? np : 1);
   short* tmp_key = wsp.take<short>(np > 0 ? np : 1);
   float* row_wabs = wsp.take<float>(nkeys);
-  if (!offsets || !bstart || !bcursor || !wmax || !choff_a || !choff_b || !scan_tmp || !tmp_rec || !sorted || !tmp_key || !row_wabs)
+  if (!offsets || !bstart || !bcursor || !wmax || !negw || !choff_a || !choff_b || !scan_tmp || !tmp_rec || !sorted || !tmp_key || !row_wabs)
     return set_error(MP_ERR_WORKSPACE, "row sort: workspace of %zu bytes < %zu", ws_bytes, paint_zfft_workspace_bytes(n, np, nmock));
   int rc = fill_i32_dev(s, bstart, nb + 1, 0);
   if (rc) return rc;
   rc = fill_i32_dev(s, reinterpret_cast<int*>(wmax), nmock, 0);
+  if (rc) return rc;
+  rc = fill_i32_dev(s, negw, nmock, 0);
   if (rc) return rc;
   Catalogues ca{moff, choff_a, nmock, np}, cb{moff, choff_b, nmock, np};
   if (moff) MP_LAUNCH("paint_catalogue_chunks", catalogue_chunks_kernel, dim3(1), dim3(RS_NT), 0, s, moff, nmock, RS_CHUNK, choff_a,
@@ -1166,7 +1244,7 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
   const int extra = moff ? nmock : 0;
   if (np > 0)
     MP_LAUNCH("paint_rowsort_hist", rowsort_hist_kernel<T>, dim3((unsigned)(ceil_div(np, RS_CHUNK) + extra)), dim3(RS_NT), 0, s,
-              pos, w, ca, g, bstart, wmax);
+              pos, w, ca, g, bstart, wmax, negw);
   rc = exclusive_scan_i32(s, bstart, nb + 1, scan_tmp, bcursor);
   if (rc) return rc;
   if (np > 0) {
@@ -1212,6 +1290,8 @@ static int row_sort(hipStream_t s, const T* pos, const float* w, int64_t np, con
   out.sorted = sorted;
   out.offsets = offsets;
   out.row_wabs = row_wabs;
+  out.wmax_bits = wmax;
+  out.negw = negw;
   return MP_OK;
 }
 
@@ -1229,6 +1309,8 @@ static int paint_zfft_typed(hipStream_t s, const T* pos, const float* w, int64_t
   rg.ni = ni;
   rg.zfrac = zfrac_of(n[2]);
   rg.nmock = nmock;
+  rg.wmax_bits = rs.wmax_bits;
+  rg.negw = rs.negw;
   const int M = n[2] / 2;
   PzShape shape{0, 0};
   rc = pz_choose(s, order, M, n, ni, np, rs.sorted, rs.offsets, rs.row_wabs, rg, out, pitch, mesh_stride, shape);
@@ -1274,6 +1356,8 @@ static int paint_rows_typed(hipStream_t s, const T* pos, const float* w, int64_t
   rg.ni = ni;
   rg.zfrac = zfrac_of(g.n[2]);
   rg.nmock = 1;
+  rg.wmax_bits = rs.wmax_bits;
+  rg.negw = rs.negw;
   const bool big = rows_big(g.n[2]);
 #define MP_ROWS(O)                                                                                   \
   case O:                                                                                            \
