@@ -20,6 +20,8 @@
 #include "sph.h"
 #include "tune.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -90,7 +92,7 @@ __device__ __forceinline__ int xcd_first_tile() {
   return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
 }
 
-template <int L, bool BIN, int NP, bool TAB, bool YLM = false, int CC = XPCfg<L>::C>
+template <int L, bool BIN, int NP, bool TAB, bool YLM = false, int CC = XPCfg<L>::C, bool LEAN = false>
 __global__ void __launch_bounds__((XCfgC<L, CC>::NT))
 xpass_kernel(XPassArgs a) {
   using Cfg = XCfgC<L, CC>;
@@ -128,10 +130,9 @@ xpass_kernel(XPassArgs a) {
       for (int i = tid; i < a.ni * L; i += NT) sphx[i] = a.tab.phx[(i / L) * a.tab.stride + i % L];
       for (int i = tid; i < a.ni * a.ncols; i += NT) sphz[i] = a.tab.phz[(i / a.ncols) * a.tab.stride + i % a.ncols];
     }
-    if (comp) {
-      for (int i = tid; i < L; i += NT) scx[i] = a.tab.cx[i];
-      for (int i = tid; i < a.ncols; i += NT) scz[i] = a.tab.cz[i];
-    }
+    // ones without compensation, so that kernels reading the tables here need no test of comp
+    for (int i = tid; i < L; i += NT) scx[i] = comp ? a.tab.cx[i] : 1.f;
+    for (int i = tid; i < a.ncols; i += NT) scz[i] = comp ? a.tab.cz[i] : 1.f;
     phx = sphx;
     phz = sphz;
     cx = scx;
@@ -265,8 +266,72 @@ xpass_kernel(XPassArgs a) {
           // monotone: found once by bisection, then moved by at most `steps` thresholds per kx
           // step in the run's direction (a fixed number of reads, no data-dependent loop); runs
           // through bins narrower than the kx spacing take the general walk.
-          const int steps = swsteps;
+          const int steps = __builtin_amdgcn_readfirstlane(swsteps);  // uniform
           const int dir = l0 < L / 2 ? 1 : -1;  // k2 increases along the run below L/2
+          if constexpr (LEAN && TAB && !YLM) {
+            // |A|^2 w L_ell(mu) with even ell <= 4 only (chosen on the host): the general loop
+            // below without its other cases.  With FS > 0 the bin moves by exactly FS conditional
+            // threshold reads per kx step (steps beyond the needed ones do not move it), so a
+            // run is one basic block around the flush branch; FS = 0 takes the general walk.
+            const bool orow = (l0 == 0) & (y == 0) & (kz == 0);  // the origin is a run's first mode
+            auto run = [&](auto fixed) {
+              constexpr int FS = decltype(fixed)::value;
+              int b = first_bin(thr, nedges, k2_sum(k2_xy(kxt[l0], kyv), kzv));
+#pragma unroll
+              for (int e = 0; e < EPT; ++e) {
+                const int l = l0 + e;
+                const double kxv = kxt[l];
+                if (e > 0) {
+                  const double k2 = k2_sum(k2_xy(kxv, kyv), kzv);
+                  int nb = b;
+                  if constexpr (FS > 0) {
+#pragma unroll
+                    for (int t = 0; t < FS; ++t) {
+                      if (dir > 0) nb += (nb < nedges && thr[min(nb, nedges - 1)] <= k2) ? 1 : 0;
+                      else nb -= (nb > 0 && thr[max(nb - 1, 0)] > k2) ? 1 : 0;
+                    }
+                  } else if (steps > 0) {
+                    for (int t = 0; t < steps; ++t) {
+                      if (dir > 0) nb += (nb < nedges && thr[min(nb, nedges - 1)] <= k2) ? 1 : 0;
+                      else nb -= (nb > 0 && thr[max(nb - 1, 0)] > k2) ? 1 : 0;
+                    }
+                  } else {
+                    nb = walk_bin(thr, nedges, b, k2);
+                  }
+                  if (nb != b) {
+                    if (b >= 1 && b < nedges) {
+#pragma unroll
+                      for (int p = 0; p < NP; ++p) atomicAdd(&hist[p * nbins + b - 1], sum[p]);
+                    }
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) sum[p] = 0.;
+                    b = nb;
+                  }
+                }
+                const bool valid = b >= 1 && b < nedges;
+                const bool origin = e == 0 && orow;
+                float2 A = cscale(acc[e], s * cx[l]);  // cx: an LDS table, ones without compensation
+                if (origin && a.zero_dc) A = make_float2(0.f, 0.f);
+                // mu is finite (los is), and with ell = 0 only c1 = c2 = 0: no need_mu test
+                const float fkx = (float)kxv;
+                const float kn2 = fkx * fkx + f2yz;
+                const float mu = kn2 > 0.f ? (fkx * a.los[0] + lyz) * __builtin_amdgcn_rsqf(kn2) : 0.f;
+                const float m2 = mu * mu;
+                const float wp2 = valid ? w * (A.x * A.x + A.y * A.y) : 0.f;  // w = 1 or 2: exact
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                  const float q = fmaf(fmaf(a.leg.c2[p], m2, a.leg.c1[p]), m2, a.leg.c0[p]);
+                  sum[p] += (double)(origin && a.rz[p] ? 0.f : wp2 * q);
+                }
+              }
+              if (b >= 1 && b < nedges) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p) atomicAdd(&hist[p * nbins + b - 1], sum[p]);
+              }
+            };
+            if (steps == 1 || steps == 2) run(std::integral_constant<int, 2>{});
+            else run(std::integral_constant<int, 0>{});
+          } else {
           int b = -1;
 #pragma unroll
           for (int e = 0; e < EPT; ++e) {
@@ -326,6 +391,7 @@ xpass_kernel(XPassArgs a) {
 #pragma unroll
             for (int p = 0; p < NP; ++p)
               if (sum[p] != 0.) atomicAdd(&hist[p * nbins + b - 1], sum[p]);
+          }
           }
         }
       }
@@ -582,10 +648,10 @@ int combine_tables(hipStream_t s, const int32_t n[3], int order, int ni, int com
   return MP_OK;
 }
 
-template <int L, bool BIN, int NP, bool TAB, bool YLM = false, int CC = XPCfg<L>::C>
+template <int L, bool BIN, int NP, bool TAB, bool YLM = false, int CC = XPCfg<L>::C, bool LEAN = false>
 static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles, int nmock) {
   using Cfg = XCfgC<L, CC>;
-  auto kern = xpass_kernel<L, BIN, NP, TAB, YLM, CC>;
+  auto kern = xpass_kernel<L, BIN, NP, TAB, YLM, CC, LEAN>;
   a.nchunks = (int)ceil_div(a.ncols, Cfg::C);
   a.ntiles = a.nchunks * a.ny;
   const size_t lds = Cfg::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, TAB);
@@ -610,10 +676,11 @@ static int launch_xpass_t(hipStream_t s, XPassArgs& a, int* nblocks_out, double*
 }
 
 // The kx/kz tables go to LDS when they fit beside the tile.
-template <int L, bool BIN, int NP, bool YLM = false, int CC = XPCfg<L>::C>
+template <int L, bool BIN, int NP, bool YLM = false, int CC = XPCfg<L>::C, bool LEAN = false>
 static int launch_xpass(hipStream_t s, XPassArgs& a, int* nblocks_out, double* partial_ws, size_t partial_doubles, int nmock = 1) {
   const size_t with_tab = XCfgC<L, CC>::LDS + xpass_extra_lds(L, a.ncols, a.ni, a.g.nedges, NP, BIN, true);
-  if (with_tab <= 160 * 1024) return launch_xpass_t<L, BIN, NP, true, YLM, CC>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
+  if (with_tab <= 160 * 1024)
+    return launch_xpass_t<L, BIN, NP, true, YLM, CC, LEAN>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
   return launch_xpass_t<L, BIN, NP, false, YLM, CC>(s, a, nblocks_out, partial_ws, partial_doubles, nmock);
 }
 
@@ -633,14 +700,22 @@ struct XPTune {
 template <int L, int NP>
 static int launch_xpass_bin_tuned(hipStream_t s, XPassArgs& a, int* nblocks, double* partial, size_t partial_doubles, int nmock) {
   using T = XPTune<L>;
+  // the lean binning applies to |A|^2 with even multipoles <= 4 (a Legendre polynomial of mu^2)
+  bool lean = !a.linear;
+  for (int p = 0; p < NP; ++p) lean = lean && a.ell[p] % 2 == 0 && a.ell[p] <= 4;
   auto run = [&](int q) -> int {
     if constexpr (T::C1 > 0) {
-      if (q == 1) return launch_xpass<L, true, NP, false, T::C1>(s, a, nblocks, partial, partial_doubles, nmock);
+      if (q == 1)
+        return lean ? launch_xpass<L, true, NP, false, T::C1, true>(s, a, nblocks, partial, partial_doubles, nmock)
+                    : launch_xpass<L, true, NP, false, T::C1>(s, a, nblocks, partial, partial_doubles, nmock);
     }
     if constexpr (T::C2 > 0) {
-      if (q == 2) return launch_xpass<L, true, NP, false, T::C2>(s, a, nblocks, partial, partial_doubles, nmock);
+      if (q == 2)
+        return lean ? launch_xpass<L, true, NP, false, T::C2, true>(s, a, nblocks, partial, partial_doubles, nmock)
+                    : launch_xpass<L, true, NP, false, T::C2>(s, a, nblocks, partial, partial_doubles, nmock);
     }
-    return launch_xpass<L, true, NP, false, T::C0>(s, a, nblocks, partial, partial_doubles, nmock);
+    return lean ? launch_xpass<L, true, NP, false, T::C0, true>(s, a, nblocks, partial, partial_doubles, nmock)
+                : launch_xpass<L, true, NP, false, T::C0>(s, a, nblocks, partial, partial_doubles, nmock);
   };
   const bool ok[3] = {true, T::C1 > 0, T::C2 > 0};
   const int forced = tune_forced(TK_XPASS_COLS);

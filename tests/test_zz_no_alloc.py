@@ -1,7 +1,9 @@
 """The entry point that allocates nothing (mp_bin_poles_partial_noalloc), through the C ABI:
 its spectra must equal those of mp_bin_poles_partial (the same kernels with a caller-provided
-workspace) bit for bit, on two calls in a row, the second on a smaller mesh with fewer
-products, so that the shared scratch array is reused with what the first call left in it.
+workspace) to 1e-12 relative (the FP64 bin sums of an x-pass block are added into one LDS
+histogram in an order that the wave scheduling decides, so the last bits may differ), on two
+calls in a row, the second on a smaller mesh with fewer products, so that the shared scratch
+array is reused with what the first call left in it.
 """
 import ctypes
 
@@ -45,16 +47,21 @@ def _case(lib, n, boxsize, ni, order, ells, rz, los, npart, seed):
     return host(got), host(ref), args
 
 
+def _same(got, ref, rtol=1e-12):
+    assert np.all(np.isfinite(ref)) and np.all(np.isfinite(got))
+    assert np.allclose(got, ref, rtol=rtol, atol=rtol * np.max(np.abs(ref))), np.max(np.abs(got - ref))
+
+
 def test_noalloc_matches_workspace_entry_point_twice():
     from jaxpower_amd import _lib
     lib = _lib.load()
     assert lib.mp_noalloc_scratch_bytes() == 16 << 20
     say('noalloc first call (128, 64, 64) tsc x3')
     got, ref, _ = _case(lib, (128, 64, 64), (1280., 640., 640.), 3, 3, [0, 2, 4], [0, 1, 1], (0., 0., 1.), 40000, 1)
-    assert np.all(np.isfinite(ref)) and np.array_equal(got, ref)
+    _same(got, ref)
     say('noalloc second call (32, 32, 32) cic x2')
     got, ref, args = _case(lib, (32, 32, 32), (320., 320., 320.), 2, 2, [0, 2], [0, 1], (0.6, 0., 0.8), 5000, 2)
-    assert np.all(np.isfinite(ref)) and np.array_equal(got, ref)
+    _same(got, ref)
     # errors are reported before any launch: too many bins for the LDS histogram, bad order
     out = torch.zeros(16, dtype=torch.float64, device=DEV)
     bad = list(args)
