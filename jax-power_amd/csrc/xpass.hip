@@ -69,7 +69,7 @@ __host__ __device__ inline size_t xpass_extra_lds(int L, int ncols, int ni, int 
     bytes += r8((size_t)ni * L * 8) + r8((size_t)ni * ncols * 8) + r8((size_t)L * 4) + r8((size_t)ncols * 4);
     if (bin) bytes += (size_t)L * 8 + (size_t)ncols * 8;
   }
-  if (bin) bytes += (size_t)nedges * 8 + (size_t)nprod * (nedges - 1) * 8;
+  if (bin) bytes += (size_t)(nedges + 4) * 8 + (size_t)nprod * (nedges - 1) * 8;  // thresholds + 2 sentinels a side
   return bytes;
 }
 
@@ -148,11 +148,13 @@ xpass_kernel(XPassArgs a) {
       kzt = skz;
     }
   }
-  double* thr = reinterpret_cast<double*>(take(BIN ? (size_t)nedges * 8 : 0));
+  // thresholds with two sentinels on each side: thr[-2], thr[-1] = -inf, thr[nedges], thr[nedges + 1] = +inf
+  double* thr = reinterpret_cast<double*>(take(BIN ? (size_t)(nedges + 4) * 8 : 0)) + 2;
   double* hist = reinterpret_cast<double*>(take(BIN ? (size_t)NP * nbins * 8 : 0));
   __shared__ int swsteps;
   if constexpr (BIN) {
     for (int i = tid; i < nedges; i += NT) thr[i] = a.g.thr[i];
+    if (tid < 4) thr[tid < 2 ? tid - 2 : nedges + tid - 2] = tid < 2 ? -INFINITY : INFINITY;
     for (int i = tid; i < NP * nbins; i += NT) hist[i] = 0.;
     // Thresholds one kx step can cross: |k| changes by at most the kx spacing between
     // neighbours of a run, so ceil-like (spacing / narrowest bin) + 1, with a 1 % margin; 0 (the
@@ -274,6 +276,7 @@ xpass_kernel(XPassArgs a) {
             // threshold reads per kx step (steps beyond the needed ones do not move it), so a
             // run is one basic block around the flush branch; FS = 0 takes the general walk.
             const bool orow = (l0 == 0) & (y == 0) & (kz == 0);  // the origin is a run's first mode
+            const int po = dir > 0 ? 0 : -2, pstep = dir > 0 ? 2 : 0;
             auto run = [&](auto fixed) {
               constexpr int FS = decltype(fixed)::value;
               int b = first_bin(thr, nedges, k2_sum(k2_xy(kxt[l0], kyv), kzv));
@@ -285,11 +288,10 @@ xpass_kernel(XPassArgs a) {
                   const double k2 = k2_sum(k2_xy(kxv, kyv), kzv);
                   int nb = b;
                   if constexpr (FS > 0) {
-#pragma unroll
-                    for (int t = 0; t < FS; ++t) {
-                      if (dir > 0) nb += (nb < nedges && thr[min(nb, nedges - 1)] <= k2) ? 1 : 0;
-                      else nb -= (nb > 0 && thr[max(nb - 1, 0)] > k2) ? 1 : 0;
-                    }
+                    // up to two thresholds crossed: the pair beyond b in the run's direction,
+                    // one ds_read2 and no branch (the sentinels keep b in [0, nedges])
+                    const double t0 = thr[nb + po], t1 = thr[nb + po + 1];
+                    nb += pstep - ((t0 > k2 ? 1 : 0) + (t1 > k2 ? 1 : 0));
                   } else if (steps > 0) {
                     for (int t = 0; t < steps; ++t) {
                       if (dir > 0) nb += (nb < nedges && thr[min(nb, nedges - 1)] <= k2) ? 1 : 0;
