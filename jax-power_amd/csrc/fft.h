@@ -46,7 +46,7 @@ struct RealPrologue {
       }
       return v;
     }
-    return ylm_of_vector(ell, m, node_coord(x0, dx, ix), node_coord(y0, dy, iy), node_coord(z0, dz, z));
+    return ylm_of_vector_rsq(ell, m, node_coord(x0, dx, ix), node_coord(y0, dy, iy), node_coord(z0, dz, z));
   }
 };
 
@@ -58,7 +58,7 @@ struct KPrologue {
   __device__ __forceinline__ float eval(int ix, int iy, int iz) const {
     int mx = ix < (nx + 1) / 2 ? ix : ix - nx;
     int my = iy < (ny + 1) / 2 ? iy : iy - ny;
-    return ylm_of_vector(ell, m, kfx * mx, kfy * my, kfz * iz);
+    return ylm_of_vector_rsq(ell, m, kfx * mx, kfy * my, kfz * iz);
   }
 };
 

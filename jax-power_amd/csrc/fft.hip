@@ -152,8 +152,8 @@ zpass_r2c_ylm_kernel(const float* __restrict__ in, float2* __restrict__ out, int
         // opaque per m: otherwise the compiler hoists every loop-invariant term of every
         // Y_lm case out of the m loop and runs out of registers
         asm volatile("" : "+v"(px[e]), "+v"(py[e]));
-        v.x *= ylm_of_vector(pro.ell, m, px[e], py[e], spz[2 * l]);
-        v.y *= ylm_of_vector(pro.ell, m, px[e], py[e], spz[2 * l + 1]);
+        v.x *= ylm_of_vector_rsq(pro.ell, m, px[e], py[e], spz[2 * l]);
+        v.y *= ylm_of_vector_rsq(pro.ell, m, px[e], py[e], spz[2 * l + 1]);
         lds[c * LP + l] = v;
       }
     }

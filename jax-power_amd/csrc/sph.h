@@ -142,4 +142,14 @@ __host__ __device__ __forceinline__ float ylm_of_vector(int ell, int m, float x,
   return ylm_unit(ell, m, x * inv, y * inv, z * inv);
 }
 
+// The same with 1/|v| from rsqrtf (within an ulp or two of 1/sqrtf): for the passes that
+// evaluate Y_lm at every mode or cell for each m, where the correctly rounded square root
+// and division were most of the per-m arithmetic.
+__device__ __forceinline__ float ylm_of_vector_rsq(int ell, int m, float x, float y, float z) {
+  const float r2 = x * x + y * y + z * z;
+  if (r2 == 0.f) return ylm_at_zero(ell, m);
+  const float inv = rsqrtf(r2);
+  return ylm_unit(ell, m, x * inv, y * inv, z * inv);
+}
+
 }  // namespace mp

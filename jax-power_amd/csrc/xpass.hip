@@ -370,7 +370,7 @@ xpass_kernel(XPassArgs a) {
 #pragma unroll
               for (int p = 0; p < NP; ++p) {
                 if (origin && a.rz[p]) continue;
-                const float f = w * ylm_of_vector(a.ell[p], a.ylm_m[p], (float)kxv, fky, fkz);
+                const float f = w * ylm_of_vector_rsq(a.ell[p], a.ylm_m[p], (float)kxv, fky, fkz);
                 sum[p] += (double)(p2 * f);
               }
             } else {
