@@ -70,6 +70,8 @@ Deferred.__hash__ = None
 
 
 def _host(x):
+    if isinstance(x, np.ndarray):
+        return x
     if isinstance(x, Deferred):
         return x.numpy()
     try:
@@ -122,7 +124,15 @@ class _MeshPole:
             r, x = self._raw, self._x
             xv = _host(r[x]).astype('f8')
             nbins = len(xv)
-            bcast = lambda v, dtype='f8': np.broadcast_to(_host(v).astype(dtype), (nbins,)).copy()
+
+            def bcast(v, dtype='f8'):  # a copy of shape (nbins,)
+                a = _host(v)
+                if a.shape == (nbins,):
+                    return a.astype(dtype)
+                if a.ndim == 0:
+                    return np.full(nbins, a, dtype=dtype)
+                return np.broadcast_to(a.astype(dtype), (nbins,)).copy()
+
             nmodes = _host(r['nmodes'])
             self._cache = {x: xv, x + '_edges': _host(r[x + '_edges']).astype('f8'),
                            'nmodes': nmodes.astype('i8') if np.issubdtype(nmodes.dtype, np.integer) else bcast(nmodes),
