@@ -116,6 +116,14 @@ __device__ __forceinline__ void grid_coords(const T* __restrict__ pos, int64_t i
 #pragma unroll
   for (int a = 0; a < 3; ++a) xg[a] = (pos[3 * i + a] + (T)g.shift[a]) / (T)g.cell[a];
 }
+// The same for the last kernel to read the positions (the row-sort scatter): non-temporal
+// loads, which leave the L2 to the records being written (512^3, 1.6e7 particles: scatter and
+// fine sort 0.171 + 0.171 -> 0.165 + 0.160 ms).
+template <typename T>
+__device__ __forceinline__ void grid_coords_last(const T* __restrict__ pos, int64_t i, const GridMap& g, T xg[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) xg[a] = (__builtin_nontemporal_load(pos + 3 * i + a) + (T)g.shift[a]) / (T)g.cell[a];
+}
 
 // ------------------------------------------------------------------ two-level row sort
 //
@@ -131,7 +139,7 @@ template <typename T>
 __device__ __forceinline__ void row_record(const T* __restrict__ pos, const float* __restrict__ w, int64_t i, const GridMap& g,
                                            int zfrac, int& hx, int& hy, float4& rec) {
   T xg[3];
-  grid_coords(pos, i, g, xg);
+  grid_coords_last(pos, i, g, xg);
   T fx = floor(xg[0]), fy = floor(xg[1]);
   hx = pmod((int)fx, g.n[0]);
   hy = pmod((int)fy, g.n[1]);
@@ -144,7 +152,7 @@ __device__ __forceinline__ void row_record(const T* __restrict__ pos, const floa
   double zs = rint((double)z * zone);
   double zmax = (double)g.n[2] * zone - 1.;
   unsigned zbits = (unsigned)fmin(fmax(zs, 0.), zmax);
-  rec = make_float4(ox, oy, __uint_as_float(zbits), w ? w[i] : 1.f);
+  rec = make_float4(ox, oy, __uint_as_float(zbits), w ? __builtin_nontemporal_load(w + i) : 1.f);
 }
 
 __device__ __forceinline__ float wave_max_f32(float v) {
@@ -197,6 +205,14 @@ __device__ __forceinline__ int wave_scan_incl(int v) {
     if (lane >= d) v += t;
   }
   return v;
+}
+
+// Non-temporal load of a record that is read once (the fine sort's input: 512^3 fine sort
+// 0.227 -> 0.171 ms).
+typedef float rec_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_rec_stream(const float4* p) {
+  const rec_f4 v = __builtin_nontemporal_load(reinterpret_cast<const rec_f4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
 }
 
 // Exclusive scan of data[0..n) in LDS, n <= 2 * RS_NT; returns the total.
@@ -373,7 +389,7 @@ rowsort_fine_kernel(const int* __restrict__ bucket_start, const float4* __restri
       const int t = threadIdx.x + q * RS_NT;
       if (t < n) {
         key[q] = tmp_key[sub + t];
-        rec[q] = tmp_rec[sub + t];
+        rec[q] = load_rec_stream(tmp_rec + sub + t);
         rank[q] = atomicAdd(&lcnt[key[q]], 1);
         atomicAdd(&qsum[key[q]], (unsigned long long)ceilf(fabsf(rec[q].w) * qs));
       }
