@@ -214,7 +214,8 @@ def test_read_matches_oracle():
 
 @pytest.mark.parametrize('meshsize,boxsize,edges', [
     (64, 1000., {'step': 0.01}), (32, 2000., {'step': 0.005}), ((16, 32, 8), (100., 200., 50.), {'step': 0.02}),
-    (16, 1000., {}), (32, 500., np.array([0., 0.01, 0.05, 0.1, 0.2, 0.3]))])
+    (16, 1000., {}), (32, 500., np.array([0., 0.01, 0.05, 0.1, 0.2, 0.3])),
+    (64, 64., {'step': 0.0005})])  # 6283 bins: more than the LDS histogram holds (global path)
 def test_bin_modes_exact(meshsize, boxsize, edges):
     from jaxpower_amd import MeshAttrs, BinMesh2SpectrumPoles
     say('bin modes', meshsize, boxsize)
