@@ -472,7 +472,10 @@ colpass_kernel(ColPassArgs a) {
       else reg[e] = src[l * a.fstride + cc];
     }
   };
-  int tile = blockIdx.x;
+  // Tiles narrower than a 128-byte line share lines with their neighbours: those take
+  // XCD-contiguous runs (1024^3 y pass 11.64 -> 11.41 ms); 16-column tiles keep the
+  // round-robin order (XCD-contiguous runs made the 512^3 y pass 26 % slower).
+  int tile = C < 16 ? xcd_first_tile() : (int)blockIdx.x;
   if (tile < a.ntiles) load(tile);
   while (tile < a.ntiles) {  // (the first sync below also publishes the twiddle table)
     const int next = tile + gridDim.x;

@@ -25,6 +25,15 @@ __device__ __forceinline__ void store_stream(float2* p, float2 v) {
   __builtin_nontemporal_store(t, reinterpret_cast<stream_f2*>(p));
 }
 
+// First tile of a persistent block.  Blocks are dispatched to the 8 XCDs round-robin; here
+// the blocks of one XCD take a contiguous run of tiles, so the two column chunks that share a
+// 128-byte line of the input meet in that XCD's L2 instead of being fetched by two XCDs.
+// Advancing by gridDim.x keeps the runs contiguous.
+__device__ __forceinline__ int xcd_first_tile() {
+  const int g = gridDim.x, b = blockIdx.x;
+  return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -83,15 +83,6 @@ using XPCfg = XCfgC<L, L < 512 ? (L >= 4 ? 16 : 64 / L) : 4096 / L>;
 // keeps its registers: capped it spills and slows down.
 constexpr int XP_WAVES_PER_SIMD = 4;
 
-// First tile of a persistent block.  Blocks are dispatched to the 8 XCDs round-robin; here
-// the blocks of one XCD take a contiguous run of tiles, so the two column chunks that share a
-// 128-byte line of the input meet in that XCD's L2 instead of being fetched by two XCDs.
-// Advancing by gridDim.x keeps the runs contiguous.
-__device__ __forceinline__ int xcd_first_tile() {
-  const int g = gridDim.x, b = blockIdx.x;
-  return (g & 7) ? b : (b & 7) * (g >> 3) + (b >> 3);
-}
-
 template <int L, bool BIN, int NP, bool TAB, bool YLM = false, int CC = XPCfg<L>::C, bool LEAN = false>
 __global__ void __launch_bounds__((XCfgC<L, CC>::NT))
 xpass_kernel(XPassArgs a) {
