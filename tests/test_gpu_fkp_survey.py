@@ -2,7 +2,8 @@
 finishes in seconds: sparse data against 20 times as many randoms in a shell cap, so the
 painted field mixes positive and negative weights that nearly cancel cell by cell.  The fused
 paint accumulates each row block in int32 fixed point with a scale set by a bound on |cell|
-(for mixed-sign catalogues, a z-resolved histogram of |w| over the rows that reach the block);
+(for mixed-sign catalogues, a z-resolved histogram of |w| over the rows that reach the block;
+above one particle per mesh cell, data and randoms are painted apart and added);
 this checks that its quantization stays below the 1e-5 bar on every multipole of the spectrum
 (reference: jaxpower/mesh.py:2973-2999, 2673-2754, jaxpower/mesh2.py:293-344)."""
 import numpy as np
@@ -18,14 +19,11 @@ def say(*args):
     print('[case]', *args, flush=True)
 
 
-DENSE = ('about 7 data and 130 randoms per survey cell: the fixed point keeps ~17 significant bits per addend there, '
-         '1.5e-5 on P_2 at the lowest k (float32 LDS atomics would give 3e-7 but run 7x slower on this target)')
-
-
 # (meshsize, data): the first two at the per-cell density of the 1024^3 survey configuration
-# (0.2 data and 4 randoms per survey cell), then denser fields
-@pytest.mark.parametrize('meshsize, nd', [(128, 10_000), (256, 75_000), (256, 400_000),
-                                          pytest.param(128, 200_000, marks=pytest.mark.xfail(reason=DENSE, strict=False))])
+# (0.2 data and 4 randoms per survey cell), then denser fields; the last (about 7 data and 130
+# randoms per survey cell, 2 particles per mesh cell) is above the density where FKPField.paint
+# paints data and randoms apart (one fixed-point paint of the concatenation gave 1.5e-5 there)
+@pytest.mark.parametrize('meshsize, nd', [(128, 10_000), (256, 75_000), (256, 400_000), (128, 200_000)])
 def test_fkp_survey_firstpoint_matches_oracle(meshsize, nd):
     from jaxpower_amd import ParticleField, FKPField, BinMesh2SpectrumPoles, compute_mesh2_spectrum, get_mesh_attrs
     from jaxpower_amd.mock import generate_shell_cap_positions, shell_cap_volume
