@@ -472,10 +472,10 @@ colpass_kernel(ColPassArgs a) {
       else reg[e] = src[l * a.fstride + cc];
     }
   };
-  // Tiles narrower than a 128-byte line share lines with their neighbours: those take
-  // XCD-contiguous runs (1024^3 y pass 11.64 -> 11.41 ms); 16-column tiles keep the
-  // round-robin order (XCD-contiguous runs made the 512^3 y pass 26 % slower).
-  int tile = C < 16 ? xcd_first_tile() : (int)blockIdx.x;
+  // Tiles narrower than a 128-byte line share lines with their neighbours: at L >= 1024 those
+  // take XCD-contiguous runs (1024^3 y pass 11.64 -> 11.41 ms); at L = 512 the runs made the
+  // pass 0.61 -> 0.79 ms with 16 or 8 columns, so it keeps the round-robin order.
+  int tile = (C < 16 && L >= 1024) ? xcd_first_tile() : (int)blockIdx.x;
   if (tile < a.ntiles) load(tile);
   while (tile < a.ntiles) {  // (the first sync below also publishes the twiddle table)
     const int next = tile + gridDim.x;
@@ -683,11 +683,13 @@ struct YPTune {
   static constexpr int C1 = L >= 64 && C0 >= 8 ? C0 / 2 : 0;
   static constexpr int C2 = L >= 64 && 2 * C0 * L * 8 <= 65536 ? 2 * C0 : 0;
 };
-constexpr size_t YP_TUNE_BYTES = (size_t)128 << 20;
+constexpr size_t YP_TUNE_BYTES = (size_t)512 << 20;  // larger than the last-level cache
 
 // The y pass with its tile width chosen by the autotuner per (L, columns, x rows, meshes).  The
 // pass works in place, so candidates are timed on a scratch copy of the first x rows of the
-// data (at most 128 MiB, allocated for the measurement only).
+// data (at most 512 MiB, allocated for the measurement only: with a copy that fit the 256 MB
+// last-level cache the candidates were ranked by their cache-resident speed, and an 8-column
+// tile that took 0.79 ms on the whole 512^3 data against 0.61 for 16 columns came first).
 template <int L>
 static int launch_colpass_tuned(hipStream_t s, ColPassArgs& a) {
   using T = YPTune<L>;
