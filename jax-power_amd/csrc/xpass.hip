@@ -199,7 +199,7 @@ xpass_kernel(XPassArgs a) {
     }
     const int c0 = (tile % a.nchunks) * C, y = tile / a.nchunks;
     const int kz = min(c0 + c, a.ncols - 1);
-    if (phases) {
+    if (phases && j > 0) {  // shift 0: every phase is exactly 1
       const float2 pyz = cmul(a.tab.phy[j * a.tab.stride + y], phz[j * phz_stride + kz]);
       const float2* px = phx + j * phx_stride + l0;
 #pragma unroll
