@@ -262,6 +262,27 @@ def read_traffic(path=TRAFFIC_FILE):
     return out
 
 
+def read_issue_limit(name, path=SQ_FILE):
+    """VALU wave-instructions per CU clock of kernel `name` in the committed SQ-counter summary
+    (the fastest of its autotuner candidates, i.e. the one the step runs), or None.  A CU issues
+    at most one VALU wave-instruction per clock, so a value near 1 marks a kernel bound by
+    instruction issue rather than by HBM."""
+    symbol = KERNEL_SYMBOLS.get(name)
+    if symbol is None or not os.path.exists(path):
+        return None
+    best = None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if line.startswith('#') or len(parts) < 8 or symbol not in line:
+                continue
+            clocks, valu = float(parts[-1]), float(parts[-5])
+            if best is None or clocks < best[0]:
+                best = (clocks, valu)
+    return None if best is None else {'valu_per_cu_clock': best[1], 'issue_limit': 1.0,
+                                      'source': 'profiles/sq_counters_512_tsc3.txt'}
+
+
 def end_to_end_bytes(n=MESHSIZE, np_=NPART, ni=NI):
     """Algorithmic bytes of one mock: paint (16 B/particle + n_i 4N^3), n_i r2c (4N^3 + 8H),
     binning (n_i 8H), as counted in the project's baseline accounting."""
@@ -879,7 +900,8 @@ def main():
                    'paint_path': _lib.load().mp_set_paint_path(-1)},
         'roofline': {'kernel': dominant, 'bound': 'hbm', 'achieved': d['achieved_GBs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(d['achieved_GBs'] / HBM_PEAK_GBS, 4), 'traffic': traffic.get(dominant),
-                     'algorithmic_bytes_per_launch': alg[dominant], 'avg_ms': d['avg_ms']},
+                     'algorithmic_bytes_per_launch': alg[dominant], 'avg_ms': d['avg_ms'],
+                     'instruction_issue': read_issue_limit(dominant)},
         'end_to_end': {'algorithmic_GB_per_mock': round(end_to_end_bytes() / 1e9, 3), 'GBs': round(e2e_gbs, 1),
                        'frac_of_peak': round(e2e_gbs / HBM_PEAK_GBS, 4)},
         'kernels': table,
