@@ -874,7 +874,11 @@ def main():
     torch.cuda.synchronize()
     gelapsed = time.perf_counter() - t0
     print(f'# {args.steps} graph replays in {gelapsed:.3f} s', file=sys.stderr, flush=True)
-    same = all(np.array_equal(gspec.get(ell).values('num_raw'), spec.get(ell).values('num_raw')) for ell in ELLS)
+    # the FP64 bin sums of an x-pass block are added in LDS in an order the wave scheduling
+    # decides, so a replay agrees with the eager step to rounding, not necessarily bit for bit
+    def close(a, b):
+        return np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.nanmax(np.abs(b)), equal_nan=True)
+    same = all(close(gspec.get(ell).values('num_raw'), spec.get(ell).values('num_raw')) for ell in ELLS)
     eager_ms = elapsed * 1e3 / args.steps
     ms_per_step = gelapsed * 1e3 / args.steps
     e2e_gbs = end_to_end_bytes() / (ms_per_step * 1e-3) / 1e9
@@ -906,7 +910,7 @@ def main():
                        'frac_of_peak': round(e2e_gbs / HBM_PEAK_GBS, 4)},
         'kernels': table,
         'check': {'p0_finite': bool(np.all(np.isfinite(p0))), 'p0_first_bins': [float(x) for x in p0[1:4]],
-                  'graph_equals_eager': bool(same)},
+                  'graph_equals_eager_1e-12': bool(same)},
     }
     if not args.no_batch:
         # the batched form of the metric: 64 uneven lognormal mocks at 256^3, batched vs per-mock
