@@ -43,7 +43,9 @@ class _SpectrumPoles(torch.autograd.Function):
         if sums is None:
             sums = bin._products_dev([pending.materialize(attrs)], [(0, 0, ell, r) for ell, r in zip(ells, rz)], los=vlos)
         del pending
-        num = torch.stack([(sums[i] * s).real if ell % 2 == 0 else (sums[i] * s).imag
+        # odd ell of this auto spectrum: the imaginary part of a real sum, 0 in every bin
+        # (empty ones included), as the reference's .imag of |A|^2 binned (mesh2.py:350-351, 368)
+        num = torch.stack([(sums[i] * s).real if ell % 2 == 0 else torch.zeros_like((sums[i] * s).real)
                            for i, (ell, s) in enumerate(zip(ells, _ell_scales(bin, pos.device)))])
         ctx.save_for_backward(pos, w)
         ctx.params = (attrs, bin, order, ni, compensate, vlos, zero_dc, positions.dtype)

@@ -161,7 +161,11 @@ class Mesh2SpectrumBatch:
         # ell, imaginary for odd (made here: no host-to-device copy may run under graph capture)
         ells = list(bin.ells)
         scale = torch.tensor([2. * ell + 1. if ell else 1. for ell in ells], dtype=torch.float64, device=pos.device)
-        self._scale = (scale[:, None] / bin._nmodes_dev[None, :])[None, :, :, None]
+        scale = scale[:, None] / bin._nmodes_dev[None, :]
+        # odd ell of an auto spectrum: the imaginary part of a real sum, 0 in every bin (empty
+        # ones included, as the reference's .imag of |A|^2 binned, mesh2.py:350-351, 368)
+        scale[[i for i, ell in enumerate(ells) if ell % 2]] = 0.
+        self._scale = scale[None, :, :, None]
         odd = torch.tensor([ell % 2 for ell in ells], dtype=torch.long, device=pos.device)
         self._part = odd[None, :, None, None].expand(nmock, nprod, bin.nbins, 1).contiguous()
 

@@ -312,7 +312,11 @@ def compute_mesh2_spectrum(*meshes, bin=None, los='z'):
             sums = bin._products_dev(bmeshes, products, los=vlos)
         num = [sums[i] * ((2 * ell + 1 if ell else 1) / bin._nmodes_dev) for i, ell in enumerate(ells)]
     # left on the device: the spectrum reads them back on first access
-    num = [(n.real if ell % 2 == 0 else n.imag).contiguous() for ell, n in zip(ells, num)]
+    # an auto spectrum along a fixed line of sight bins |A|^2, a real array, whose .imag the
+    # reference takes for odd ell: 0 in every bin, empty ones included (mesh2.py:350-351, 368)
+    auto_real = vlos is not None and same
+    num = [(n.real if ell % 2 == 0 else (torch.zeros_like(n.real) if auto_real else n.imag)).contiguous()
+           for ell, n in zip(ells, num)]
     poles = [Mesh2SpectrumPole(k=bin.xavg, k_edges=bin.edges, nmodes=bin.nmodes, num_raw=num[i], num_shotnoise=0.,
                                norm=norm, volume=np.prod(mattrs.kfun) * bin.nmodes, ell=ell, attrs=attrs)
              for i, ell in enumerate(ells)]

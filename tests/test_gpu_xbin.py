@@ -187,3 +187,29 @@ def test_local_los_fused_matches_stored(los, cross, boxcenter, monkeypatch):
     want = po.spectrum_poles(oattrs, po.BinPoles(oattrs, edges={'step': 0.005}, ells=ells), m1.astype('f8'),
                              m2_.astype('f8') if cross else None, los=los)
     assert_poles_close(got, want['num_raw'], rtol=1e-5, label='fused vs oracle')
+
+
+@pytest.mark.parametrize('factor', [0.7, 0.3, 0.12])
+@pytest.mark.parametrize('ells', [(0, 2, 4), (0, 1, 2)])
+def test_fused_bin_bins_narrower_than_kx_spacing(factor, ells):
+    """Bins of factor * kf: the fused x pass moves the bin by two conditional thresholds per kx
+    step (0.7 kf), by a fixed number of them (0.3 kf: four), or walks (0.12 kf); even multipoles
+    take the lean binning, odd ones the general loop.  Exact mode counts and the oracle's
+    multipoles."""
+    from jaxpower_amd import MeshAttrs, ParticleField, BinMesh2SpectrumPoles, compute_mesh2_spectrum
+    say('fused bin narrow bins', factor, ells)
+    attrs = MeshAttrs(meshsize=64, boxsize=1000., boxcenter=0.)
+    pos, w = _particles(attrs, 20000, 7)
+    p = ParticleField(to_dev(pos), to_dev(w), attrs=attrs)
+    step = factor * float(np.min(attrs.kfun))
+    bin = BinMesh2SpectrumPoles(attrs, edges={'step': step}, ells=ells)
+    lazy = p.paint(resampler='tsc', interlacing=3, compensate=True, out='complex').at[0, 0, 0].set(0.)
+    assert lazy._pending_combine() is not None
+    fused = compute_mesh2_spectrum(lazy, bin=bin, los='z')
+    got = np.array([fused.get(ell).values('num_raw') for ell in ells])
+    oattrs = po.Attrs(attrs.meshsize, attrs.boxsize, attrs.boxcenter)
+    omesh = po.paint(oattrs, pos, w.astype('f8'), resampler='tsc', interlacing=3, compensate=True, out='complex')
+    omesh[0, 0, 0] = 0.
+    want = po.spectrum_poles(oattrs, po.BinPoles(oattrs, edges={'step': step}, ells=ells), omesh, los='z')
+    assert np.array_equal(fused.get(0).values('nmodes'), want['nmodes'])
+    assert_poles_close(got, want['num_raw'], label=f'narrow bins {factor} kf')
