@@ -24,7 +24,7 @@ from . import _lib
 
 KNOBS = {'paint_rows': 0, 'xpass_cols': 1, 'ypass_cols': 2, 'sort_chunk': 3}
 ENV_CACHE = 'JAXPOWER_AMD_TUNING_CACHE'
-_FORMAT = 1
+_FORMAT = 2  # 2: y-pass candidates timed on a copy larger than the last-level cache (format-1 choices dropped)
 
 
 def table():

@@ -306,7 +306,7 @@ def test_tuning_table_json_cache(tmp_path):
     from jaxpower_amd import tuning
     tuning.clear()
     try:
-        doc = {'format': 1, 'device': 'gfx950', 'entries': {
+        doc = {'format': tuning._FORMAT, 'device': 'gfx950', 'entries': {
             'xpass_cols L=512 ny=512 ncols=257 interlacing=3 products=3 catalogues=1': {'knob': 'xpass_cols', 'choice': 2},
             'sort_chunk nx=512 ny=512 positions=f32': {'knob': 'sort_chunk', 'choice': 1}}}
         path = tmp_path / 'cache.json'
